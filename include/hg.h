@@ -1,0 +1,166 @@
+/*
+ * hg.h -- C-ABI of the MI355X (gfx950) explicit-step engine for
+ * libHybridGrains scenes: MPM (mpmgrains2d ExplicitIntegrator), circle DEM
+ * (rigidbody2d PenaltyImpactFrictionMap) and the hybridgrains2dnew coupling.
+ *
+ * Conventions
+ *  - Every call returns hg_status: HG_OK (0) or a negative HG_ERR_* code;
+ *    hg_last_error(ctx) returns the message of the last failure on a context.
+ *  - Handles are opaque and own their device memory; state stays resident in
+ *    HBM between calls.  Each handle launches on one HIP stream of its own.
+ *  - Calls on one handle are not thread-safe; calls on different handles are.
+ *  - Host arrays use the reference's Eigen layouts: 2-vectors interleaved
+ *    (x0,y0,x1,y1,...), 2x2 matrices column-major (m00,m10,m01,m11), DEM
+ *    coordinates (x,y,theta) per body.
+ *  - Point and body arrays passed to *_download come back in creation order,
+ *    whatever order the engine keeps them in on the device.
+ */
+#ifndef HG_H
+#define HG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t hg_status;
+#define HG_OK 0
+#define HG_ERR_INVALID (-1)     /* bad argument or descriptor */
+#define HG_ERR_HIP (-2)         /* HIP runtime failure */
+#define HG_ERR_NOMEM (-3)       /* device allocation failed */
+#define HG_ERR_STATE (-4)       /* simulation left its valid domain (point off the grid, NaN) */
+#define HG_ERR_UNSUPPORTED (-5) /* configuration outside what the engine implements */
+
+enum { HG_FP64 = 0, HG_FP32 = 1 };
+enum { HG_BASIS_UGIMP_LINEAR = 0, HG_BASIS_LINEAR = 1, HG_BASIS_CUBIC = 3 };
+enum { HG_PLANE_SLIDING = 0, HG_PLANE_STICKING = 1 };
+
+typedef struct hg_ctx hg_ctx;
+typedef struct hg_mpm hg_mpm;
+
+/* ---- context -------------------------------------------------------------- */
+/* No reference counterpart: the reference is CPU-only.  Binds one GPU. */
+hg_status hg_ctx_create(int32_t device, hg_ctx **out);
+hg_status hg_ctx_destroy(hg_ctx *ctx);
+const char *hg_last_error(const hg_ctx *ctx);
+/* Library version string, e.g. "hg 0.1 gfx950". */
+const char *hg_version(void);
+
+/* ---- MPM: mpmgrains2d SimulationState + ExplicitIntegrator ----------------- */
+/*
+ * Replaces the state a caller builds with
+ * InitialSimulationState::generateSimulationState
+ * (mpmgrains2d/InitialSimulationState.cpp:147-228) or
+ * SimulationState::deserialize (mpmgrains2d/SimulationState.cpp:131-143):
+ * grid geometry as PhysicsGrid::setDimensions leaves it (min, cell_width,
+ * cell_count; PhysicsGrid.cpp:14-55), the material of SimulationState.h:46-57,
+ * the static planes of StaticPlane.h:6-22 and the MaterialPoints fields the
+ * step reads (MaterialPoints.h:290-326).
+ */
+typedef struct hg_mpm_desc {
+  int32_t precision;         /* HG_FP64: plain FP64 path; HG_FP32: FP32 bulk, FP64 positions and return map */
+  int32_t basis;             /* HG_BASIS_* (BasisFunctions.h:13) */
+  double grid_min[2];
+  double cell_width;
+  uint32_t cell_count[2];
+  double shear_modulus;      /* mu */
+  double bulk_modulus;       /* kappa */
+  double dp_alpha;           /* Drucker_Prager_alpha */
+  double pic_alpha;          /* PIC/FLIP blend 'alpha' */
+  double gravity[2];         /* near_earth_gravity */
+  uint32_t nplanes;
+  const double *plane_x;       /* 2*nplanes */
+  const double *plane_n;       /* 2*nplanes (normalised by the engine, as MPMStaticPlane's ctor does) */
+  const double *plane_lower;   /* nplanes; -INFINITY when absent */
+  const int32_t *plane_behavior; /* nplanes; HG_PLANE_* */
+  uint32_t npoints;
+  const double *q;           /* 2N */
+  const double *v;           /* 2N */
+  const double *m;           /* N */
+  const double *volume;      /* N */
+  const double *hl;          /* N, uGIMP half width */
+  const double *be_bar;      /* 4N */
+  const double *J;           /* N */
+  const uint8_t *kinematically_scripted; /* N or NULL */
+} hg_mpm_desc;
+
+hg_status hg_mpm_create(hg_ctx *ctx, const hg_mpm_desc *desc, hg_mpm **out);
+hg_status hg_mpm_destroy(hg_mpm *h);
+
+/* ExplicitIntegrator::flow (mpmgrains2d/ExplicitIntegrator.cpp:63-67), nsteps
+ * times.  Asynchronous on the handle's stream unless an error check is due:
+ * the engine checks its device error flag (point off the grid, non-finite
+ * state: the isConsistent criteria of SimulationState.cpp:52-111) on every
+ * hg_mpm_sync / download and returns HG_ERR_STATE. */
+hg_status hg_mpm_step(hg_mpm *h, double dt, uint32_t nsteps);
+
+/* ExplicitIntegrator::flow_zero_phase / flow_first_phase / flow_second_phase
+ * (ExplicitIntegrator.cpp:5-61) for callers that interleave other work, as
+ * the hybrid integrator does.  phase 0 rasterises mass and momentum; phase 1
+ * adds the internal and gravity forces, updates the node momentum and applies
+ * the grid plane conditions; phase 2 forms node velocity and acceleration and
+ * runs the particle update.  Between phases 1 and 2 the node momenta may be
+ * edited through hg_mpm_grid_device. */
+hg_status hg_mpm_phase(hg_mpm *h, int32_t phase, double dt);
+
+/* Waits for the handle's stream and reports HG_ERR_STATE if any step so far
+ * left the valid domain. */
+hg_status hg_mpm_sync(hg_mpm *h);
+
+/* Copies the points back to the caller, in creation order (any pointer may be
+ * NULL).  Mirrors the MaterialPoints fields (MaterialPoints.h:290-326). */
+hg_status hg_mpm_download_points(hg_mpm *h, double *q, double *v, double *be_bar,
+                                 double *J);
+
+/* Copies the grid fields of the last step, in the reference node order
+ * (flat index y*(cell_count[0]+1)+x, PhysicsGrid.cpp:76-80).  Fields not kept
+ * by the last step's mode are returned as zeros; hg_mpm_set_keep_grid(h, 1)
+ * keeps all of them (mass, momentum, momentum_new, force, velocity,
+ * acceleration), as PhysicsGrid holds after flow(). */
+hg_status hg_mpm_set_keep_grid(hg_mpm *h, int32_t keep_all);
+hg_status hg_mpm_download_grid(hg_mpm *h, double *mass, double *mom,
+                               double *mom_new, double *force, double *vel,
+                               double *acc);
+
+/* Number of points / grid nodes (reference indexing) of scene 0. */
+hg_status hg_mpm_counts(const hg_mpm *h, uint32_t *npoints, uint32_t *nnodes);
+
+/* ---- batches of independent scenes (no reference counterpart: the reference
+ * steps one SimulationState per process) ------------------------------------
+ * One handle may hold several scenes of uneven size; every step kernel covers
+ * all of them in one launch through per-scene offset tables.  All scenes of a
+ * batch share precision and basis.  Each scene evolves bit-identically to the
+ * same scene stepped alone.  hg_mpm_create is hg_mpm_create_batch with one
+ * scene; the un-suffixed download calls address scene 0. */
+hg_status hg_mpm_create_batch(hg_ctx *ctx, const hg_mpm_desc *descs, uint32_t nscenes,
+                              hg_mpm **out);
+hg_status hg_mpm_download_points_scene(hg_mpm *h, uint32_t scene, double *q, double *v,
+                                       double *be_bar, double *J);
+hg_status hg_mpm_download_grid_scene(hg_mpm *h, uint32_t scene, double *mass, double *mom,
+                                     double *mom_new, double *force, double *vel,
+                                     double *acc);
+/* Per-scene device error bits after the steps queued so far (waits for the
+ * stream): 1 = a point's stencil left the grid, 2 = non-finite point state. */
+hg_status hg_mpm_scene_error(hg_mpm *h, uint32_t scene, int32_t *flags);
+
+/* P2G workgroup shape: threads per workgroup = aw*ah, each owning one stencil
+ * base cell of a tile of (aw-S+1) x (ah-S+1) owned nodes (S = stencil width).
+ * Supported: 16x16, 32x16, 32x8.  Chosen by the autotuner. */
+hg_status hg_mpm_set_tile(hg_mpm *h, int32_t aw, int32_t ah);
+
+/* ---- timing (no reference counterpart) --------------------------------------
+ * With profiling on, every kernel launch is bracketed by HIP events on the
+ * handle's stream; hg_mpm_kernel_times returns, per kernel name, the summed
+ * device milliseconds and the launch count since profiling was switched on. */
+hg_status hg_mpm_set_profiling(hg_mpm *h, int32_t on);
+hg_status hg_mpm_kernel_times(hg_mpm *h, int32_t max_kernels, char *names /* max_kernels*32 */,
+                              double *ms, int64_t *launches, int32_t *nkernels);
+/* Captures one step into a hipGraph and replays it nsteps times. */
+hg_status hg_mpm_step_graph(hg_mpm *h, double dt, uint32_t nsteps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,119 @@
+// hg_ctx.hip -- context, errors and kernel timing for libhg.
+#include "hg_internal.h"
+
+#include <cstring>
+
+hg_status hg_fail(hg_ctx *ctx, hg_status code, const std::string &msg) {
+  if (ctx) ctx->last_error = msg;
+  return code;
+}
+
+extern "C" hg_status hg_ctx_create(int32_t device, hg_ctx **out) {
+  if (!out) return HG_ERR_INVALID;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return HG_ERR_HIP;
+  if (device < 0 || device >= n) return HG_ERR_INVALID;
+  if (hipSetDevice(device) != hipSuccess) return HG_ERR_HIP;
+  hg_ctx *c = new hg_ctx();
+  c->device = device;
+  *out = c;
+  return HG_OK;
+}
+
+extern "C" hg_status hg_ctx_destroy(hg_ctx *ctx) {
+  delete ctx;
+  return HG_OK;
+}
+
+extern "C" const char *hg_last_error(const hg_ctx *ctx) {
+  return ctx ? ctx->last_error.c_str() : "null context";
+}
+
+extern "C" const char *hg_version(void) { return "hg 0.1 gfx950"; }
+
+// ---- KernelProfiler -----------------------------------------------------------
+
+int KernelProfiler::kernel_id(const char *name) {
+  for (size_t i = 0; i < names.size(); ++i)
+    if (names[i] == name) return int(i);
+  names.emplace_back(name);
+  total_ms.push_back(0.0);
+  launches.push_back(0);
+  return int(names.size() - 1);
+}
+
+hipEvent_t KernelProfiler::take() {
+  if (!pool.empty()) {
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+int KernelProfiler::begin(const char *name, hipStream_t s) {
+  if (!on) return -1;
+  Rec r;
+  r.kernel = kernel_id(name);
+  r.a = take();
+  r.b = take();
+  if (!r.a || !r.b) return -1;
+  hipEventRecord(r.a, s);
+  pending.push_back(r);
+  return int(pending.size() - 1);
+}
+
+void KernelProfiler::end(int rec, hipStream_t s) {
+  if (rec < 0) return;
+  hipEventRecord(pending[size_t(rec)].b, s);
+}
+
+hipError_t KernelProfiler::collect() {
+  for (Rec &r : pending) {
+    hipError_t e = hipEventSynchronize(r.b);
+    if (e != hipSuccess) return e;
+    float ms = 0.0f;
+    e = hipEventElapsedTime(&ms, r.a, r.b);
+    if (e != hipSuccess) return e;
+    total_ms[size_t(r.kernel)] += double(ms);
+    launches[size_t(r.kernel)] += 1;
+    pool.push_back(r.a);
+    pool.push_back(r.b);
+  }
+  pending.clear();
+  return hipSuccess;
+}
+
+void KernelProfiler::reset() {
+  for (Rec &r : pending) {
+    pool.push_back(r.a);
+    pool.push_back(r.b);
+  }
+  pending.clear();
+  for (double &t : total_ms) t = 0.0;
+  for (int64_t &l : launches) l = 0;
+}
+
+KernelProfiler::~KernelProfiler() {
+  reset();
+  for (hipEvent_t e : pool) hipEventDestroy(e);
+}
+
+hg_status hg_profiler_report(KernelProfiler &prof, hg_ctx *ctx, int32_t max_kernels,
+                             char *names, double *ms, int64_t *launches,
+                             int32_t *nkernels) {
+  HG_HIP(ctx, prof.collect());
+  const int32_t n = int32_t(prof.names.size());
+  if (nkernels) *nkernels = n;
+  for (int32_t i = 0; i < n && i < max_kernels; ++i) {
+    if (names) {
+      std::memset(names + 32 * i, 0, 32);
+      std::strncpy(names + 32 * i, prof.names[size_t(i)].c_str(), 31);
+    }
+    if (ms) ms[i] = prof.total_ms[size_t(i)];
+    if (launches) launches[i] = prof.launches[size_t(i)];
+  }
+  return HG_OK;
+}

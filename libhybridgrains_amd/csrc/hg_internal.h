@@ -1,0 +1,86 @@
+// hg_internal.h -- shared host/device internals of libhg (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/hg.h"
+
+struct hg_ctx {
+  int device = 0;
+  std::string last_error;
+};
+
+hg_status hg_fail(hg_ctx *ctx, hg_status code, const std::string &msg);
+
+#define HG_HIP(ctx, expr)                                                        \
+  do {                                                                           \
+    hipError_t e_ = (expr);                                                      \
+    if (e_ != hipSuccess)                                                        \
+      return hg_fail((ctx), HG_ERR_HIP,                                          \
+                     std::string(#expr) + ": " + hipGetErrorString(e_));         \
+  } while (0)
+
+// Per-kernel device timing with HIP events on one stream.  Off by default;
+// when on, each launch is bracketed by two events and the totals are read on
+// demand (which synchronises the stream).
+struct KernelProfiler {
+  bool on = false;
+  struct Rec {
+    int kernel;
+    hipEvent_t a, b;
+  };
+  std::vector<std::string> names;
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> pool;
+  std::vector<double> total_ms;
+  std::vector<int64_t> launches;
+
+  int kernel_id(const char *name);
+  hipEvent_t take();
+  // Returns the record index or -1 when profiling is off.
+  int begin(const char *name, hipStream_t s);
+  void end(int rec, hipStream_t s);
+  hipError_t collect();
+  void reset();
+  ~KernelProfiler();
+};
+
+struct ProfScope {
+  KernelProfiler &p;
+  hipStream_t s;
+  int rec;
+  ProfScope(KernelProfiler &prof, const char *name, hipStream_t st)
+      : p(prof), s(st), rec(prof.begin(name, st)) {}
+  ~ProfScope() { p.end(rec, s); }
+};
+
+hg_status hg_profiler_report(KernelProfiler &prof, hg_ctx *ctx, int32_t max_kernels,
+                             char *names, double *ms, int64_t *launches,
+                             int32_t *nkernels);
+
+// ---- device helpers ---------------------------------------------------------
+
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; this
+// maps workgroup b of a launch padded to a multiple of 8 so that each XCD gets
+// one contiguous range of work items (tiles, particle chunks), which keeps
+// neighbouring items in the same L2.
+__device__ __forceinline__ uint32_t hg_xcd_remap(uint32_t b, uint32_t nblocks_padded) {
+  const uint32_t per = nblocks_padded >> 3;
+  return (b & 7u) * per + (b >> 3);
+}
+static inline uint32_t hg_pad8(uint32_t n) { return (n + 7u) & ~7u; }
+
+// Wavefront (64-lane) inclusive prefix sum of 32-bit values.
+__device__ __forceinline__ uint32_t hg_wave_incl_scan(uint32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t t = __shfl_up(v, off, 64);
+    if (lane >= off) v += t;
+  }
+  return v;
+}

@@ -1,0 +1,1398 @@
+// hg_mpm.hip -- MPM explicit step on gfx950 and its C-ABI.
+//
+// Replaces mpmgrains2d/ExplicitIntegrator.cpp:5-67 and the PhysicsGrid /
+// MaterialPoints / ConstitutiveModel routines it calls.  See
+// hg_mpm_kernels.h for the HBM layout.
+#include "hg_mpm_kernels.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+
+using namespace hgmpm;
+
+namespace {
+
+constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 4;
+constexpr int kScanChunk = kScanThreads * kScanItems;
+constexpr int kG2PThreads = 256;
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__device__ __forceinline__ T ld(const void *p, uint32_t i) {
+  return static_cast<const T *>(p)[i];
+}
+template <typename T>
+__device__ __forceinline__ void st(void *p, uint32_t i, T v) {
+  static_cast<T *>(p)[i] = v;
+}
+
+__device__ __forceinline__ uint32_t scene_of_point(const uint32_t *p_off, uint32_t nscenes,
+                                                   uint32_t i) {
+  uint32_t lo = 0, hi = nscenes;  // p_off[lo] <= i < p_off[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (p_off[mid] <= i) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Computes the stencil base key of a point and whether its whole stencil is
+// on the grid (SimulationState::isConsistent's bound check, SimulationState.cpp:54-62).
+template <int BASIS>
+__device__ __forceinline__ bool point_key(const SceneDev &sc, double x, double y, double hl,
+                                          int &bx, int &by, int &cx, int &cy) {
+  double lx, hx, ly, hy;
+  stencil_1d<BASIS>(x, hl, sc.min0, sc.h, lx, hx);
+  stencil_1d<BASIS>(y, hl, sc.min1, sc.h, ly, hy);
+  const bool ok = (lx >= 0.0) && (ly >= 0.0) && (hx <= double(sc.nnx - 1)) &&
+                  (hy <= double(sc.nny - 1));
+  constexpr int S = BasisTraits<BASIS>::S;
+  if (ok) {
+    bx = int(lx);
+    by = int(ly);
+    cx = int(hx) - bx + 1;
+    cy = int(hy) - by + 1;
+  } else {
+    // keep memory safe; the scene is flagged and its results are void
+    bx = (lx >= 0.0) ? int(fmin(lx, double(sc.nnx - S))) : 0;
+    by = (ly >= 0.0) ? int(fmin(ly, double(sc.nny - S))) : 0;
+    bx = max(0, min(bx, sc.nnx - S));
+    by = max(0, min(by, sc.nny - S));
+    cx = S;
+    cy = S;
+  }
+  return ok;
+}
+
+// ---------------------------------------------------------------------------
+// P2G + grid update.  One workgroup per tile of TW x TH owned nodes; its
+// AW x AH = (TW+S-1) x (TH+S-1) threads each own one stencil-base cell of the
+// tile's apron.  A thread sums its cell's points (in id order, so the result
+// does not depend on the sort's slot order) into S*S node accumulators in
+// registers, then the S*S stencil offsets are added into LDS one offset per
+// phase: within a phase every cell targets a different node, so no atomics
+// are needed and each node's sum has a fixed order.  Contributions to nodes
+// outside the tile are dropped (the neighbouring tile computes them), so each
+// node is written once, with plain stores, by its owner.
+//
+// Follows PhysicsGrid::rasterizePointMasses (PhysicsGrid.cpp:168-207),
+// rasterizePointMomentum (:274-311), MaterialPoints::computeHyperelasticCauchyStress
+// (MaterialPoints.cpp:183-202) with ConstitutiveModel::computeTau
+// (ConstitutiveModel.cpp:83-97), PhysicsGrid::computeForces (:413-473),
+// updateMomentum (:565-586), resolvePlaneCollisions (:611-673) and
+// lumpedMassVelocityAndAccelerationUpdate (:588-609).
+// ---------------------------------------------------------------------------
+struct P2GArgs {
+  const SceneDev *scenes;
+  const uint32_t *tile_scene;
+  PtsRaw pts;
+  const uint32_t *perm;
+  const uint32_t *cell_start;
+  void *gva;
+  double *gm, *gp, *gpn, *gf;
+  uint8_t *tile_active;
+  uint32_t store;
+  uint32_t ntiles, nblocks_padded;
+};
+
+template <typename T, int BASIS, int AW, int AH>
+__global__ __launch_bounds__(AW *AH) void mpm_p2g_kernel(P2GArgs args) {
+  constexpr int S = BasisTraits<BASIS>::S;
+  constexpr int TW = AW - (S - 1);
+  constexpr int TH = AH - (S - 1);
+  constexpr int NT = AW * AH;
+  __shared__ double s_m[TW * TH], s_p0[TW * TH], s_p1[TW * TH], s_f0[TW * TH],
+      s_f1[TW * TH];
+  __shared__ int s_any;
+
+  const uint32_t tile = hg_xcd_remap(blockIdx.x, args.nblocks_padded);
+  if (tile >= args.ntiles) return;
+  const uint32_t sid = args.tile_scene[tile];
+  const SceneDev &sc = args.scenes[sid];
+  const uint32_t ltile = tile - sc.tile_off;
+  const int x0 = int(ltile % uint32_t(sc.tiles_x)) * TW;
+  const int y0 = int(ltile / uint32_t(sc.tiles_x)) * TH;
+  const int t = threadIdx.x;
+  const uint32_t *cs = args.cell_start + sc.cell_off;
+
+  // Apron cell of this thread and its point range.
+  const int bx = x0 - (S - 1) + (t % AW);
+  const int by = y0 - (S - 1) + (t / AW);
+  uint32_t s0 = 0, s1 = 0;
+  if (bx >= 0 && by >= 0 && bx < sc.nnx && by < sc.nny) {
+    const uint32_t key = uint32_t(by) * uint32_t(sc.snx) + uint32_t(bx);
+    s0 = cs[key];
+    s1 = cs[key + 1];
+  }
+  if (t == 0) s_any = 0;
+  for (int k = t; k < TW * TH; k += NT) {
+    s_m[k] = 0.0;
+    s_p0[k] = 0.0;
+    s_p1[k] = 0.0;
+    s_f0[k] = 0.0;
+    s_f1[k] = 0.0;
+  }
+  __syncthreads();
+  if (s1 > s0) s_any = 1;
+  __syncthreads();
+  const bool any = s_any != 0;
+  if (t == 0) args.tile_active[tile] = any ? 1 : 0;
+  if (!any) return;
+
+  double acc[S][S][5];
+#pragma unroll
+  for (int j = 0; j < S; ++j)
+#pragma unroll
+    for (int i = 0; i < S; ++i)
+#pragma unroll
+      for (int f = 0; f < 5; ++f) acc[j][i][f] = 0.0;
+
+  const double mu = sc.mu, kappa = sc.kappa;
+  const uint32_t n = s1 - s0;
+  uint32_t last_id = 0;
+  for (uint32_t k = 0; k < n; ++k) {
+    // next point of the cell in id order
+    uint32_t best = 0xffffffffu, bp = 0;
+    for (uint32_t s = s0; s < s1; ++s) {
+      const uint32_t pi = args.perm[s];
+      const uint32_t idv = args.pts.id[pi];
+      if ((k == 0 || idv > last_id) && idv <= best) {
+        best = idv;
+        bp = pi;
+      }
+    }
+    last_id = best;
+    const uint32_t p = bp;
+
+    const double x = args.pts.x[p], y = args.pts.y[p];
+    const T m = ld<T>(args.pts.m, p);
+    const T v0 = ld<T>(args.pts.vx, p), v1 = ld<T>(args.pts.vy, p);
+    const T V = ld<T>(args.pts.V, p);
+    const double J = double(ld<T>(args.pts.J, p));
+    const double b00 = double(ld<T>(args.pts.b00, p));
+    const double b01 = double(ld<T>(args.pts.b01, p));
+    const double b11 = double(ld<T>(args.pts.b11, p));
+    const T hl = args.pts.hl ? ld<T>(args.pts.hl, p) : T(sc.hl_uniform);
+
+    // Cauchy stress sigma = tau / J, tau = 0 when J > 1 (no tension), FP64.
+    double t00 = 0.0, t01 = 0.0, t11 = 0.0;
+    if (J <= 1.0) {
+      const double e00 = b00 * J, e01 = b01 * J, e11 = b11 * J;
+      const double det = e00 * e11 - e01 * e01;
+      const double Jt = sqrt(det);
+      const double nrm = sqrt(det);
+      const double bb00 = e00 / nrm, bb01 = e01 / nrm, bb11 = e11 / nrm;
+      const double half_tr = (bb00 + bb11) * 1.0 / 2.0;
+      const double c = kappa * 0.5 * (Jt + 1.0) * (Jt - 1.0);
+      t00 = c + mu * (bb00 - half_tr);
+      t01 = 0.0 + mu * (bb01 - 0.0);
+      t11 = c + mu * (bb11 - half_tr);
+    }
+    const T s00 = T(t00 / J), s01 = T(t01 / J), s11 = T(t11 / J);
+    const T cf = T(-double(V) * J);
+
+    // the stencil base of p is (bx, by) by construction of the sort
+    double lo_d, hi_d;
+    stencil_1d<BASIS>(x, double(hl), sc.min0, sc.h, lo_d, hi_d);
+    const double cxd = hi_d - lo_d + 1.0;
+    const int cntx = (cxd >= 1.0 && cxd <= double(S)) ? int(cxd) : S;
+    stencil_1d<BASIS>(y, double(hl), sc.min1, sc.h, lo_d, hi_d);
+    const double cyd = hi_d - lo_d + 1.0;
+    const int cnty = (cyd >= 1.0 && cyd <= double(S)) ? int(cyd) : S;
+    T axs[S], dxs[S], ays[S], dys[S];
+    basis_1d<T, BASIS>(x, hl, sc.min0, sc.h, bx, cntx, axs, dxs);
+    basis_1d<T, BASIS>(y, hl, sc.min1, sc.h, by, cnty, ays, dys);
+    const T div = (BASIS == HG_BASIS_UGIMP_LINEAR) ? T(4) * hl * hl : T(sc.h);
+    const T mv0 = m * v0, mv1 = m * v1;
+#pragma unroll
+    for (int j = 0; j < S; ++j)
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        T w, gx, gy;
+        weight_2d<T, BASIS>(axs[i], ays[j], dxs[i], dys[j], div, w, gx, gy);
+        acc[j][i][0] += double(w * m);
+        acc[j][i][1] += double(w * mv0);
+        acc[j][i][2] += double(w * mv1);
+        if (w != T(0)) {
+          const T sw0 = s00 * gx + s01 * gy;
+          const T sw1 = s01 * gx + s11 * gy;
+          acc[j][i][3] += double(cf * sw0);
+          acc[j][i][4] += double(cf * sw1);
+        }
+      }
+  }
+
+  // S*S conflict-free accumulation phases into the owned-node LDS tile.
+  const bool has = n > 0;
+#pragma unroll
+  for (int j = 0; j < S; ++j)
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      const int lx = bx + i - x0, ly = by + j - y0;
+      if (has && lx >= 0 && lx < TW && ly >= 0 && ly < TH) {
+        const int q = ly * TW + lx;
+        s_m[q] += acc[j][i][0];
+        s_p0[q] += acc[j][i][1];
+        s_p1[q] += acc[j][i][2];
+        s_f0[q] += acc[j][i][3];
+        s_f1[q] += acc[j][i][4];
+      }
+      __syncthreads();
+    }
+
+  // Grid update of the owned nodes.
+  using T4 = typename Vec4T<T>::type;
+  const double dt = sc.dt;
+  for (int q = t; q < TW * TH; q += NT) {
+    const int nx = x0 + (q % TW), ny = y0 + (q / TW);
+    if (nx >= sc.nnx || ny >= sc.nny) continue;
+    const uint32_t node = sc.node_off + uint32_t(ny) * uint32_t(sc.snx) + uint32_t(nx);
+    const double m = s_m[q];
+    const double p0 = s_p0[q], p1 = s_p1[q];
+    double f0 = s_f0[q], f1 = s_f1[q];
+    f0 += m * sc.g0;
+    f1 += m * sc.g1;
+    double pn0 = 0.0, pn1 = 0.0;
+    if (m > 0.0) {
+      pn0 = p0 + dt * f0;
+      pn1 = p1 + dt * f1;
+    }
+    const double px0 = sc.min0 + sc.h * double(nx);
+    const double px1 = sc.min1 + sc.h * double(ny);
+    for (int k = 0; k < sc.nplanes; ++k) {
+      if (px1 < sc.plane_lower[k]) continue;
+      const double n0 = sc.plane_n[2 * k], n1 = sc.plane_n[2 * k + 1];
+      const double d = n0 * (px0 - sc.plane_x[2 * k]) + n1 * (px1 - sc.plane_x[2 * k + 1]);
+      if (d > 0.0) continue;
+      if (sc.plane_beh[k] == HG_PLANE_SLIDING) {
+        if (m <= 0.0) continue;
+        double u0 = pn0 / m, u1 = pn1 / m;
+        const double vn = n0 * u0 + n1 * u1;
+        if (vn > 0.0) continue;
+        u0 -= vn * n0;
+        u1 -= vn * n1;
+        pn0 = m * u0;
+        pn1 = m * u1;
+      } else {
+        pn0 = 0.0;
+        pn1 = 0.0;
+      }
+    }
+    if (args.store & kStoreVelAcc) {
+      T4 va;
+      if (m > 0.0) {
+        const double dm = dt * m;
+        va.x = T(pn0 / m);
+        va.y = T(pn1 / m);
+        va.z = T((pn0 - p0) / dm);
+        va.w = T((pn1 - p1) / dm);
+      } else {
+        va.x = va.y = va.z = va.w = T(0);
+      }
+      static_cast<T4 *>(args.gva)[node] = va;
+    }
+    if (args.store & kStoreMass) args.gm[node] = m;
+    if (args.store & kStoreMom) {
+      args.gp[2 * node] = p0;
+      args.gp[2 * node + 1] = p1;
+    }
+    if (args.store & kStoreMomNew) {
+      args.gpn[2 * node] = pn0;
+      args.gpn[2 * node + 1] = pn1;
+    }
+    if (args.store & kStoreForce) {
+      args.gf[2 * node] = f0;
+      args.gf[2 * node + 1] = f1;
+    }
+  }
+}
+
+// Node velocity and acceleration from stored mass and momenta, for the split
+// (phase-by-phase) step: PhysicsGrid::lumpedMassVelocityAndAccelerationUpdate
+// (PhysicsGrid.cpp:588-609).
+struct VelAccArgs {
+  const SceneDev *scenes;
+  const uint32_t *tile_scene;
+  const uint8_t *tile_active;
+  const double *gm, *gp, *gpn;
+  void *gva;
+  uint32_t ntiles;
+  int32_t tw, th;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void mpm_velacc_kernel(VelAccArgs a) {
+  const uint32_t tile = blockIdx.x;
+  if (tile >= a.ntiles || !a.tile_active[tile]) return;
+  const SceneDev &sc = a.scenes[a.tile_scene[tile]];
+  const uint32_t lt = tile - sc.tile_off;
+  const int x0 = int(lt % uint32_t(sc.tiles_x)) * a.tw, y0 = int(lt / uint32_t(sc.tiles_x)) * a.th;
+  using T4 = typename Vec4T<T>::type;
+  for (int q = threadIdx.x; q < a.tw * a.th; q += 256) {
+    const int nx = x0 + q % a.tw, ny = y0 + q / a.tw;
+    if (nx >= sc.nnx || ny >= sc.nny) continue;
+    const uint32_t node = sc.node_off + uint32_t(ny) * uint32_t(sc.snx) + uint32_t(nx);
+    const double m = a.gm[node];
+    T4 va;
+    if (m > 0.0) {
+      const double dm = sc.dt * m;
+      va.x = T(a.gpn[2 * node] / m);
+      va.y = T(a.gpn[2 * node + 1] / m);
+      va.z = T((a.gpn[2 * node] - a.gp[2 * node]) / dm);
+      va.w = T((a.gpn[2 * node + 1] - a.gp[2 * node + 1]) / dm);
+    } else {
+      va.x = va.y = va.z = va.w = T(0);
+    }
+    static_cast<T4 *>(a.gva)[node] = va;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// G2P: one thread per point, in cell order.  MaterialPoints::computeVelGrad
+// (MaterialPoints.cpp:296-322), elasticPrediction (:324-348) with
+// ConstitutiveModel::elasticPrediction (ConstitutiveModel.cpp:19-22),
+// plasticCorrection (:350-369) with the Drucker-Prager return
+// (ConstitutiveModel.cpp:26-81), updateVelocities (:204-245),
+// updatePositions (:247-288) and resolvePlaneCollisions (:371-423).  The
+// return map runs in FP64 in every precision.  The point is written to slot i
+// of the other buffer, and its new cell key is counted for the next sort.
+// ---------------------------------------------------------------------------
+struct G2PArgs {
+  const SceneDev *scenes;
+  const uint32_t *p_off;
+  uint32_t nscenes;
+  uint32_t npoints;
+  uint32_t nblocks_padded;
+  PtsRaw src, dst;
+  const uint32_t *perm;
+  const void *gva;
+  uint32_t *key, *slot, *counts;
+  int32_t *err;
+};
+
+template <typename T, int BASIS>
+__global__ __launch_bounds__(kG2PThreads) void mpm_g2p_kernel(G2PArgs a) {
+  constexpr int S = BasisTraits<BASIS>::S;
+  using T4 = typename Vec4T<T>::type;
+  const uint32_t blk = hg_xcd_remap(blockIdx.x, a.nblocks_padded);
+  const uint32_t i = blk * kG2PThreads + threadIdx.x;
+  if (i >= a.npoints) return;
+  const uint32_t sid = scene_of_point(a.p_off, a.nscenes, i);
+  const SceneDev &sc = a.scenes[sid];
+  const uint32_t p = a.perm[i];
+
+  double x = a.src.x[p], y = a.src.y[p];
+  T v0 = ld<T>(a.src.vx, p), v1 = ld<T>(a.src.vy, p);
+  const double J0 = double(ld<T>(a.src.J, p));
+  const double b00 = double(ld<T>(a.src.b00, p));
+  const double b01 = double(ld<T>(a.src.b01, p));
+  const double b11 = double(ld<T>(a.src.b11, p));
+  const T hl = a.src.hl ? ld<T>(a.src.hl, p) : T(sc.hl_uniform);
+  const bool scripted = a.src.flags ? (a.src.flags[p] != 0) : false;
+  const double dt = sc.dt;
+
+  int bx, by, cx, cy;
+  point_key<BASIS>(sc, x, y, double(hl), bx, by, cx, cy);
+  T axs[S], dxs[S], ays[S], dys[S];
+  basis_1d<T, BASIS>(x, hl, sc.min0, sc.h, bx, cx, axs, dxs);
+  basis_1d<T, BASIS>(y, hl, sc.min1, sc.h, by, cy, ays, dys);
+  const T div = (BASIS == HG_BASIS_UGIMP_LINEAR) ? T(4) * hl * hl : T(sc.h);
+
+  const T4 *g = static_cast<const T4 *>(a.gva) + sc.node_off;
+  T L00 = 0, L10 = 0, L01 = 0, L11 = 0;
+  T vp0 = 0, vp1 = 0, ac0 = 0, ac1 = 0;
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    const uint32_t row = uint32_t(by + j) * uint32_t(sc.snx) + uint32_t(bx);
+#pragma unroll
+    for (int ii = 0; ii < S; ++ii) {
+      const T4 nv = g[row + ii];
+      T w, gx, gy;
+      weight_2d<T, BASIS>(axs[ii], ays[j], dxs[ii], dys[j], div, w, gx, gy);
+      L00 += nv.x * gx;
+      L10 += nv.y * gx;
+      L01 += nv.x * gy;
+      L11 += nv.y * gy;
+      vp0 += w * nv.x;
+      vp1 += w * nv.y;
+      ac0 += w * nv.z;
+      ac1 += w * nv.w;
+    }
+  }
+
+  // elastic predictor b* = b_n + dt (L b_n + b_n L^T), b_n = J b_bar
+  const double l00 = double(L00), l10 = double(L10), l01 = double(L01), l11 = double(L11);
+  const double n00 = b00 * J0, n01 = b01 * J0, n11 = b11 * J0;
+  const double Lb00 = l00 * n00 + l01 * n01;
+  const double Lb10 = l10 * n00 + l11 * n01;
+  const double Lb11 = l10 * n01 + l11 * n11;
+  const double bL00 = n00 * l00 + n01 * l01;
+  const double bL10 = n01 * l00 + n11 * l01;
+  const double bL11 = n01 * l10 + n11 * l11;
+  const double e00 = n00 + dt * (Lb00 + bL00);
+  const double e01 = n01 + dt * (Lb10 + bL10);
+  const double e11 = n11 + dt * (Lb11 + bL11);
+  double detE = e00 * e11 - e01 * e01;
+  double J1 = sqrt(detE);
+  double nrm = sqrt(detE);
+  double c00 = e00 / nrm, c01 = e01 / nrm, c11 = e11 / nrm;
+
+  // Drucker-Prager return on b* = J b_bar
+  {
+    const double s00 = c00 * J1, s01 = c01 * J1, s11 = c11 * J1;
+    const double detS = s00 * s11 - s01 * s01;
+    const double Jr = sqrt(detS);
+    double ys = 0.5 * sc.dp_alpha * sc.kappa * (1.0 + Jr) * (1.0 - Jr);
+    if (ys < 0.0) ys = 0.0;
+    const double half_tr = 0.5 * (s00 + s11);
+    const double d00 = s00 - half_tr, d01 = s01 - 0.0, d11 = s11 - half_tr;
+    const double q00 = d00 / Jr, q01 = d01 / Jr, q11 = d11 / Jr;
+    const double fn = sqrt(q00 * q00 + q01 * q01 + q01 * q01 + q11 * q11);
+    double r00 = s00, r01 = s01, r11 = s11;
+    if (sc.mu * fn > ys) {
+      const double lambda2 = ys / (sc.mu * fn);
+      const double detD = d00 * d11 - d01 * d01;
+      const double l1sq = detS - lambda2 * lambda2 * detD;
+      const double lambda1 = sqrt(l1sq);
+      r00 = lambda1 + lambda2 * d00;
+      r01 = 0.0 + lambda2 * d01;
+      r11 = lambda1 + lambda2 * d11;
+    }
+    const double detR = r00 * r11 - r01 * r01;
+    J1 = sqrt(detR);
+    const double rn = sqrt(detR);
+    c00 = r00 / rn;
+    c01 = r01 / rn;
+    c11 = r11 / rn;
+  }
+
+  // PIC/FLIP velocity and advection
+  const T alpha = T(sc.pic_alpha);
+  const T vf0 = v0 + ac0 * T(dt), vf1 = v1 + ac1 * T(dt);
+  if (!scripted) {
+    v0 = (T(1) - alpha) * vp0 + alpha * vf0;
+    v1 = (T(1) - alpha) * vp1 + alpha * vf1;
+    x += dt * double(vp0);
+    y += dt * double(vp1);
+  } else {
+    x += dt * double(v0);
+    y += dt * double(v1);
+  }
+
+  // point plane conditions
+  for (int k = 0; k < sc.nplanes; ++k) {
+    if (y < sc.plane_lower[k]) continue;
+    const double pn0 = sc.plane_n[2 * k], pn1 = sc.plane_n[2 * k + 1];
+    const double d = pn0 * (x - sc.plane_x[2 * k]) + pn1 * (y - sc.plane_x[2 * k + 1]);
+    if (d > 0.0) continue;
+    x += -d * pn0;
+    y += -d * pn1;
+    if (sc.plane_beh[k] == HG_PLANE_SLIDING) {
+      const double vn = pn0 * double(v0) + pn1 * double(v1);
+      if (vn > 0.0) continue;
+      v0 = T(double(v0) - vn * pn0);
+      v1 = T(double(v1) - vn * pn1);
+    } else {
+      v0 = T(0);
+      v1 = T(0);
+    }
+  }
+
+  // write the point to slot i of the other buffer and count its new cell
+  a.dst.x[i] = x;
+  a.dst.y[i] = y;
+  st<T>(a.dst.vx, i, v0);
+  st<T>(a.dst.vy, i, v1);
+  st<T>(a.dst.b00, i, T(c00));
+  st<T>(a.dst.b01, i, T(c01));
+  st<T>(a.dst.b11, i, T(c11));
+  st<T>(a.dst.J, i, T(J1));
+  st<T>(a.dst.m, i, ld<T>(a.src.m, p));
+  st<T>(a.dst.V, i, ld<T>(a.src.V, p));
+  a.dst.id[i] = a.src.id[p];
+  if (a.src.hl) st<T>(a.dst.hl, i, hl);
+  if (a.src.flags) a.dst.flags[i] = a.src.flags[p];
+
+  int nbx, nby, ncx, ncy;
+  const bool ok = point_key<BASIS>(sc, x, y, double(hl), nbx, nby, ncx, ncy);
+  const bool finite = isfinite(x) && isfinite(y) && isfinite(double(v0)) &&
+                      isfinite(double(v1)) && isfinite(J1);
+  if (!ok || !finite) atomicOr(&a.err[sid], (ok ? 0 : 1) | (finite ? 0 : 2));
+  const uint32_t k = sc.cell_off + uint32_t(nby) * uint32_t(sc.snx) + uint32_t(nbx);
+  a.key[i] = k;
+  a.slot[i] = atomicAdd(&a.counts[k], 1u);
+}
+
+// Initial binning of points in creation order.
+struct BinArgs {
+  const SceneDev *scenes;
+  const uint32_t *p_off;
+  uint32_t nscenes, npoints;
+  PtsRaw pts;
+  uint32_t *key, *slot, *counts;
+  int32_t *err;
+};
+
+template <typename T, int BASIS>
+__global__ __launch_bounds__(256) void mpm_bin_kernel(BinArgs a) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.npoints) return;
+  const uint32_t sid = scene_of_point(a.p_off, a.nscenes, i);
+  const SceneDev &sc = a.scenes[sid];
+  const T hl = a.pts.hl ? ld<T>(a.pts.hl, i) : T(sc.hl_uniform);
+  int bx, by, cx, cy;
+  const bool ok = point_key<BASIS>(sc, a.pts.x[i], a.pts.y[i], double(hl), bx, by, cx, cy);
+  if (!ok) atomicOr(&a.err[sid], 1);
+  const uint32_t k = sc.cell_off + uint32_t(by) * uint32_t(sc.snx) + uint32_t(bx);
+  a.key[i] = k;
+  a.slot[i] = atomicAdd(&a.counts[k], 1u);
+}
+
+// ---------------------------------------------------------------------------
+// Counting sort: exclusive scan of the per-cell counts (three passes), then a
+// scatter of point indices.  scan_apply also zeroes the counts for the next
+// step's G2P.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t block_excl_scan_1024(uint32_t v, uint32_t &total) {
+  __shared__ uint32_t s_wave[16];
+  __shared__ uint32_t s_total;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t inc = hg_wave_incl_scan(v);
+  if (lane == 63) s_wave[wave] = inc;
+  __syncthreads();
+  if (wave == 0) {
+    const uint32_t wv = lane < 16 ? s_wave[lane] : 0u;
+    const uint32_t winc = hg_wave_incl_scan(wv);
+    if (lane < 16) s_wave[lane] = winc - wv;  // exclusive wave offsets
+    if (lane == 15) s_total = winc;
+  }
+  __syncthreads();
+  total = s_total;
+  return s_wave[wave] + inc - v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t *counts,
+                                                                   uint32_t n,
+                                                                   uint32_t *block_sums) {
+  const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * kScanItems;
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k)
+    if (base + k < n) s += counts[base + k];
+  uint32_t total = 0;
+  (void)block_excl_scan_1024(s, total);
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_top_kernel(uint32_t *block_sums,
+                                                                uint32_t nb) {
+  uint32_t vals[kScanItems];
+  uint32_t s = 0;
+  const uint32_t base = threadIdx.x * kScanItems;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    vals[k] = (base + k < nb) ? block_sums[base + k] : 0u;
+    s += vals[k];
+  }
+  uint32_t total = 0;
+  uint32_t run = block_excl_scan_1024(s, total);
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < nb) block_sums[base + k] = run;
+    run += vals[k];
+  }
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(uint32_t *counts, uint32_t n,
+                                                                  const uint32_t *block_off,
+                                                                  uint32_t *cell_start) {
+  const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * kScanItems;
+  uint32_t vals[kScanItems];
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    vals[k] = (base + k < n) ? counts[base + k] : 0u;
+    s += vals[k];
+  }
+  uint32_t total = 0;
+  uint32_t run = block_excl_scan_1024(s, total) + block_off[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < n) {
+      cell_start[base + k] = run;
+      counts[base + k] = 0u;
+      if (base + k == n - 1) cell_start[n] = run + vals[k];
+    }
+    run += vals[k];
+  }
+}
+
+__global__ __launch_bounds__(256) void scatter_kernel(const uint32_t *key, const uint32_t *slot,
+                                                      const uint32_t *cell_start, uint32_t n,
+                                                      uint32_t *perm) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  perm[cell_start[key[i]] + slot[i]] = i;
+}
+
+}  // namespace
+
+// ===========================================================================
+// host side
+// ===========================================================================
+
+struct SceneHost {
+  SceneDev d;
+};
+
+struct hg_mpm {
+  hg_ctx *ctx = nullptr;
+  hipStream_t stream = nullptr;
+  int precision = HG_FP64;
+  int basis = HG_BASIS_UGIMP_LINEAR;
+  int aw = 16, ah = 16;  // P2G apron shape (threads per workgroup = aw*ah)
+  std::vector<SceneDev> scenes;
+  uint32_t nscenes = 0;
+  uint32_t npoints = 0;
+  uint32_t ncell = 0, nnode = 0, ntiles = 0;
+  bool has_hl = false, has_flags = false;
+  PtsRaw buf[2] = {};
+  int cur = 0;
+  uint32_t *perm = nullptr, *key = nullptr, *slot = nullptr, *counts = nullptr,
+           *cell_start = nullptr, *block_sums = nullptr;
+  uint32_t nscan = 0;
+  SceneDev *d_scenes = nullptr;
+  uint32_t *d_p_off = nullptr;
+  uint32_t *d_tile_scene = nullptr;
+  uint8_t *tile_active = nullptr;
+  void *gva = nullptr;
+  double *gm = nullptr, *gp = nullptr, *gpn = nullptr, *gf = nullptr;
+  int32_t *err = nullptr;
+  bool keep_all = false;
+  uint32_t last_store = 0;
+  int phase_next = 0;  // 0: flow or phase 0 allowed; 1: phase 1; 2: phase 2
+  double dt_dev = -1.0;  // dt currently in the device scene table
+  KernelProfiler prof;
+  hipGraphExec_t graph = nullptr;
+  std::vector<void *> allocs;
+
+  size_t tsize() const { return precision == HG_FP64 ? 8 : 4; }
+  int tw() const;
+  int th() const;
+};
+
+static int stencil_width(int basis) {
+  return basis == HG_BASIS_LINEAR ? 2 : (basis == HG_BASIS_CUBIC ? 4 : 3);
+}
+int hg_mpm::tw() const { return aw - (stencil_width(basis) - 1); }
+int hg_mpm::th() const { return ah - (stencil_width(basis) - 1); }
+
+template <typename P>
+static hg_status dalloc(hg_mpm *h, P **out, size_t bytes) {
+  void *p = nullptr;
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess)
+    return hg_fail(h->ctx, HG_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  h->allocs.push_back(p);
+  *out = static_cast<P *>(p);
+  return HG_OK;
+}
+
+#define HG_TRY(expr)                 \
+  do {                               \
+    hg_status s_ = (expr);           \
+    if (s_ != HG_OK) return s_;      \
+  } while (0)
+
+// ---- kernel dispatch ----------------------------------------------------------
+
+template <typename T, int BASIS, int AW, int AH>
+static void launch_p2g_t(hg_mpm *h, uint32_t store) {
+  P2GArgs a;
+  a.scenes = h->d_scenes;
+  a.tile_scene = h->d_tile_scene;
+  a.pts = h->buf[h->cur];
+  a.perm = h->perm;
+  a.cell_start = h->cell_start;
+  a.gva = h->gva;
+  a.gm = h->gm;
+  a.gp = h->gp;
+  a.gpn = h->gpn;
+  a.gf = h->gf;
+  a.tile_active = h->tile_active;
+  a.store = store;
+  a.ntiles = h->ntiles;
+  a.nblocks_padded = hg_pad8(h->ntiles);
+  ProfScope ps(h->prof, "mpm_p2g", h->stream);
+  hipLaunchKernelGGL((mpm_p2g_kernel<T, BASIS, AW, AH>), dim3(a.nblocks_padded), dim3(AW * AH),
+                     0, h->stream, a);
+}
+
+template <typename T, int BASIS>
+static void launch_p2g_b(hg_mpm *h, uint32_t store) {
+  if (h->aw == 32 && h->ah == 16) launch_p2g_t<T, BASIS, 32, 16>(h, store);
+  else if (h->aw == 32 && h->ah == 8) launch_p2g_t<T, BASIS, 32, 8>(h, store);
+  else launch_p2g_t<T, BASIS, 16, 16>(h, store);
+}
+
+template <typename T>
+static void launch_p2g_p(hg_mpm *h, uint32_t store) {
+  switch (h->basis) {
+    case HG_BASIS_LINEAR: launch_p2g_b<T, HG_BASIS_LINEAR>(h, store); break;
+    case HG_BASIS_CUBIC: launch_p2g_b<T, HG_BASIS_CUBIC>(h, store); break;
+    default: launch_p2g_b<T, HG_BASIS_UGIMP_LINEAR>(h, store); break;
+  }
+}
+
+static void launch_p2g(hg_mpm *h, uint32_t store) {
+  h->last_store = store;
+  if (h->precision == HG_FP64) launch_p2g_p<double>(h, store);
+  else launch_p2g_p<float>(h, store);
+}
+
+template <typename T, int BASIS>
+static void launch_g2p_t(hg_mpm *h) {
+  G2PArgs a;
+  a.scenes = h->d_scenes;
+  a.p_off = h->d_p_off;
+  a.nscenes = h->nscenes;
+  a.npoints = h->npoints;
+  const uint32_t nb = (h->npoints + kG2PThreads - 1) / kG2PThreads;
+  a.nblocks_padded = hg_pad8(nb);
+  a.src = h->buf[h->cur];
+  a.dst = h->buf[1 - h->cur];
+  a.perm = h->perm;
+  a.gva = h->gva;
+  a.key = h->key;
+  a.slot = h->slot;
+  a.counts = h->counts;
+  a.err = h->err;
+  ProfScope ps(h->prof, "mpm_g2p", h->stream);
+  hipLaunchKernelGGL((mpm_g2p_kernel<T, BASIS>), dim3(a.nblocks_padded), dim3(kG2PThreads), 0,
+                     h->stream, a);
+}
+
+template <typename T>
+static void launch_g2p_p(hg_mpm *h) {
+  switch (h->basis) {
+    case HG_BASIS_LINEAR: launch_g2p_t<T, HG_BASIS_LINEAR>(h); break;
+    case HG_BASIS_CUBIC: launch_g2p_t<T, HG_BASIS_CUBIC>(h); break;
+    default: launch_g2p_t<T, HG_BASIS_UGIMP_LINEAR>(h); break;
+  }
+}
+
+template <typename T, int BASIS>
+static void launch_bin_t(hg_mpm *h) {
+  BinArgs a;
+  a.scenes = h->d_scenes;
+  a.p_off = h->d_p_off;
+  a.nscenes = h->nscenes;
+  a.npoints = h->npoints;
+  a.pts = h->buf[h->cur];
+  a.key = h->key;
+  a.slot = h->slot;
+  a.counts = h->counts;
+  a.err = h->err;
+  const uint32_t nb = (h->npoints + 255) / 256;
+  if (nb) hipLaunchKernelGGL((mpm_bin_kernel<T, BASIS>), dim3(nb), dim3(256), 0, h->stream, a);
+}
+
+template <typename T>
+static void launch_bin_p(hg_mpm *h) {
+  switch (h->basis) {
+    case HG_BASIS_LINEAR: launch_bin_t<T, HG_BASIS_LINEAR>(h); break;
+    case HG_BASIS_CUBIC: launch_bin_t<T, HG_BASIS_CUBIC>(h); break;
+    default: launch_bin_t<T, HG_BASIS_UGIMP_LINEAR>(h); break;
+  }
+}
+
+static void launch_sort(hg_mpm *h) {
+  {
+    ProfScope ps(h->prof, "scan_reduce", h->stream);
+    hipLaunchKernelGGL(scan_reduce_kernel, dim3(h->nscan), dim3(kScanThreads), 0, h->stream,
+                       h->counts, h->ncell, h->block_sums);
+  }
+  {
+    ProfScope ps(h->prof, "scan_top", h->stream);
+    hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(kScanThreads), 0, h->stream,
+                       h->block_sums, h->nscan);
+  }
+  {
+    ProfScope ps(h->prof, "scan_apply", h->stream);
+    hipLaunchKernelGGL(scan_apply_kernel, dim3(h->nscan), dim3(kScanThreads), 0, h->stream,
+                       h->counts, h->ncell, h->block_sums, h->cell_start);
+  }
+  if (h->npoints) {
+    ProfScope ps(h->prof, "scatter", h->stream);
+    hipLaunchKernelGGL(scatter_kernel, dim3((h->npoints + 255) / 256), dim3(256), 0, h->stream,
+                       h->key, h->slot, h->cell_start, h->npoints, h->perm);
+  }
+}
+
+static void launch_g2p_and_sort(hg_mpm *h) {
+  if (h->npoints) {
+    if (h->precision == HG_FP64) launch_g2p_p<double>(h);
+    else launch_g2p_p<float>(h);
+  }
+  launch_sort(h);
+  h->cur = 1 - h->cur;
+}
+
+static void launch_velacc(hg_mpm *h) {
+  VelAccArgs a;
+  a.scenes = h->d_scenes;
+  a.tile_scene = h->d_tile_scene;
+  a.tile_active = h->tile_active;
+  a.gm = h->gm;
+  a.gp = h->gp;
+  a.gpn = h->gpn;
+  a.gva = h->gva;
+  a.ntiles = h->ntiles;
+  a.tw = h->tw();
+  a.th = h->th();
+  ProfScope ps(h->prof, "mpm_velacc", h->stream);
+  if (h->precision == HG_FP64)
+    hipLaunchKernelGGL(mpm_velacc_kernel<double>, dim3(h->ntiles), dim3(256), 0, h->stream, a);
+  else
+    hipLaunchKernelGGL(mpm_velacc_kernel<float>, dim3(h->ntiles), dim3(256), 0, h->stream, a);
+}
+
+static hg_status upload_dt(hg_mpm *h, double dt) {
+  if (dt == h->dt_dev) return HG_OK;
+  // the table is read by queued kernels: wait for them before rewriting it
+  HG_HIP(h->ctx, hipStreamSynchronize(h->stream));
+  for (SceneDev &s : h->scenes) s.dt = dt;
+  HG_HIP(h->ctx, hipMemcpy(h->d_scenes, h->scenes.data(), h->scenes.size() * sizeof(SceneDev),
+                           hipMemcpyHostToDevice));
+  h->dt_dev = dt;
+  if (h->graph) {
+    hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  return HG_OK;
+}
+
+static uint32_t fused_store(const hg_mpm *h) {
+  return h->keep_all ? (kStoreVelAcc | kStoreMass | kStoreMom | kStoreMomNew | kStoreForce)
+                     : kStoreVelAcc;
+}
+
+static void launch_step(hg_mpm *h) {
+  launch_p2g(h, fused_store(h));
+  launch_g2p_and_sort(h);
+}
+
+// ---- creation -----------------------------------------------------------------
+
+static hg_status validate_desc(hg_ctx *ctx, const hg_mpm_desc &d) {
+  if (d.precision != HG_FP64 && d.precision != HG_FP32)
+    return hg_fail(ctx, HG_ERR_INVALID, "precision must be HG_FP64 or HG_FP32");
+  if (d.basis != HG_BASIS_UGIMP_LINEAR && d.basis != HG_BASIS_LINEAR && d.basis != HG_BASIS_CUBIC)
+    return hg_fail(ctx, HG_ERR_INVALID, "unknown basis");
+  if (!(d.cell_width > 0.0)) return hg_fail(ctx, HG_ERR_INVALID, "cell_width must be > 0");
+  if (d.cell_count[0] < 4 || d.cell_count[1] < 4)
+    return hg_fail(ctx, HG_ERR_INVALID, "grid needs at least 4 cells per axis");
+  if (uint64_t(d.cell_count[0] + 2) * uint64_t(d.cell_count[1] + 2) > (1ull << 31))
+    return hg_fail(ctx, HG_ERR_UNSUPPORTED, "grid too large");
+  if (d.nplanes > uint32_t(kMaxPlanes))
+    return hg_fail(ctx, HG_ERR_UNSUPPORTED, "at most 8 static planes per scene");
+  if (d.nplanes && (!d.plane_x || !d.plane_n || !d.plane_behavior))
+    return hg_fail(ctx, HG_ERR_INVALID, "plane arrays missing");
+  if (d.npoints && (!d.q || !d.v || !d.m || !d.volume || !d.hl || !d.be_bar || !d.J))
+    return hg_fail(ctx, HG_ERR_INVALID, "point arrays missing");
+  for (uint32_t i = 0; i < d.npoints; ++i) {
+    if (!(d.m[i] > 0.0) || !(d.volume[i] > 0.0) || !(d.hl[i] > 0.0))
+      return hg_fail(ctx, HG_ERR_INVALID, "point mass, volume and hl must be > 0");
+    if (d.basis == HG_BASIS_UGIMP_LINEAR && !(2.0 * d.hl[i] < d.cell_width))
+      return hg_fail(ctx, HG_ERR_INVALID, "uGIMP needs hl < cell_width/2");
+    if (!(d.J[i] != 0.0)) return hg_fail(ctx, HG_ERR_INVALID, "J must be non-zero");
+  }
+  return HG_OK;
+}
+
+template <typename T>
+static void fill_points(const hg_mpm_desc &d, uint32_t id0, std::vector<double> &x,
+                        std::vector<double> &y, std::vector<T> (&f)[9],
+                        std::vector<uint32_t> &ids, std::vector<uint8_t> &flags) {
+  for (uint32_t i = 0; i < d.npoints; ++i) {
+    x.push_back(d.q[2 * i]);
+    y.push_back(d.q[2 * i + 1]);
+    f[0].push_back(T(d.v[2 * i]));
+    f[1].push_back(T(d.v[2 * i + 1]));
+    f[2].push_back(T(d.be_bar[4 * i]));
+    f[3].push_back(T(0.5 * (d.be_bar[4 * i + 1] + d.be_bar[4 * i + 2])));
+    f[4].push_back(T(d.be_bar[4 * i + 3]));
+    f[5].push_back(T(d.J[i]));
+    f[6].push_back(T(d.m[i]));
+    f[7].push_back(T(d.volume[i]));
+    f[8].push_back(T(d.hl[i]));
+    ids.push_back(id0 + i);
+    flags.push_back(d.kinematically_scripted ? d.kinematically_scripted[i] : 0);
+  }
+}
+
+template <typename T>
+static hg_status upload_points(hg_mpm *h, const hg_mpm_desc *descs) {
+  std::vector<double> x, y;
+  std::vector<T> f[9];
+  std::vector<uint32_t> ids;
+  std::vector<uint8_t> flags;
+  for (uint32_t s = 0; s < h->nscenes; ++s) fill_points<T>(descs[s], h->scenes[s].p_off, x, y, f, ids, flags);
+  const uint32_t N = h->npoints;
+  for (int b = 0; b < 2; ++b) {
+    PtsRaw &p = h->buf[b];
+    HG_TRY(dalloc(h, &p.x, N * 8));
+    HG_TRY(dalloc(h, &p.y, N * 8));
+    void **fs[9] = {&p.vx, &p.vy, &p.b00, &p.b01, &p.b11, &p.J, &p.m, &p.V, &p.hl};
+    for (int k = 0; k < 9; ++k) {
+      if (k == 8 && !h->has_hl) {
+        *fs[k] = nullptr;
+        continue;
+      }
+      HG_TRY(dalloc(h, fs[k], N * sizeof(T)));
+    }
+    HG_TRY(dalloc(h, &p.id, N * 4));
+    if (h->has_flags) HG_TRY(dalloc(h, &p.flags, N));
+    else p.flags = nullptr;
+  }
+  PtsRaw &p = h->buf[0];
+  if (N) {
+    HG_HIP(h->ctx, hipMemcpy(p.x, x.data(), N * 8, hipMemcpyHostToDevice));
+    HG_HIP(h->ctx, hipMemcpy(p.y, y.data(), N * 8, hipMemcpyHostToDevice));
+    void *fs[9] = {p.vx, p.vy, p.b00, p.b01, p.b11, p.J, p.m, p.V, p.hl};
+    for (int k = 0; k < 9; ++k)
+      if (fs[k]) HG_HIP(h->ctx, hipMemcpy(fs[k], f[k].data(), N * sizeof(T), hipMemcpyHostToDevice));
+    HG_HIP(h->ctx, hipMemcpy(p.id, ids.data(), N * 4, hipMemcpyHostToDevice));
+    if (p.flags) HG_HIP(h->ctx, hipMemcpy(p.flags, flags.data(), N, hipMemcpyHostToDevice));
+  }
+  return HG_OK;
+}
+
+static hg_status mpm_create_impl(hg_ctx *ctx, const hg_mpm_desc *descs, uint32_t nscenes,
+                                 hg_mpm **out) {
+  if (!ctx || !descs || !out || nscenes == 0) return hg_fail(ctx, HG_ERR_INVALID, "null argument");
+  for (uint32_t s = 0; s < nscenes; ++s) {
+    HG_TRY(validate_desc(ctx, descs[s]));
+    if (descs[s].precision != descs[0].precision || descs[s].basis != descs[0].basis)
+      return hg_fail(ctx, HG_ERR_INVALID, "all scenes of a batch share precision and basis");
+  }
+  HG_HIP(ctx, hipSetDevice(ctx->device));
+  std::unique_ptr<hg_mpm> h(new hg_mpm());
+  h->ctx = ctx;
+  h->precision = descs[0].precision;
+  h->basis = descs[0].basis;
+  h->nscenes = nscenes;
+  HG_HIP(ctx, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+
+  // scene tables
+  const int S = stencil_width(h->basis);
+  const int TW = h->tw(), TH = h->th();
+  uint64_t p_off = 0, cell_off = 0, node_off = 0, tile_off = 0;
+  std::vector<uint32_t> h_p_off;
+  std::vector<uint32_t> tile_scene;
+  for (uint32_t s = 0; s < nscenes; ++s) {
+    const hg_mpm_desc &d = descs[s];
+    SceneDev sd;
+    std::memset(&sd, 0, sizeof(sd));
+    sd.min0 = d.grid_min[0];
+    sd.min1 = d.grid_min[1];
+    sd.h = d.cell_width;
+    sd.mu = d.shear_modulus;
+    sd.kappa = d.bulk_modulus;
+    sd.dp_alpha = d.dp_alpha;
+    sd.pic_alpha = d.pic_alpha;
+    sd.g0 = d.gravity[0];
+    sd.g1 = d.gravity[1];
+    sd.dt = 0.0;
+    sd.hl_uniform = d.npoints ? d.hl[0] : 0.25 * d.cell_width;
+    for (uint32_t i = 0; i < d.npoints; ++i) {
+      if (d.hl[i] != d.hl[0]) h->has_hl = true;
+      if (d.kinematically_scripted && d.kinematically_scripted[i]) h->has_flags = true;
+    }
+    sd.nnx = int32_t(d.cell_count[0] + 1);
+    sd.nny = int32_t(d.cell_count[1] + 1);
+    sd.snx = sd.nnx + 1;
+    sd.sny = sd.nny + 1;
+    sd.tiles_x = (sd.nnx + TW - 1) / TW;
+    sd.tiles_y = (sd.nny + TH - 1) / TH;
+    sd.p_off = uint32_t(p_off);
+    sd.p_cnt = d.npoints;
+    sd.cell_off = uint32_t(cell_off);
+    sd.node_off = uint32_t(node_off);
+    sd.tile_off = uint32_t(tile_off);
+    sd.nplanes = int32_t(d.nplanes);
+    for (uint32_t k = 0; k < d.nplanes; ++k) {
+      const double n0 = d.plane_n[2 * k], n1 = d.plane_n[2 * k + 1];
+      const double nn = std::sqrt(n0 * n0 + n1 * n1);
+      if (!(nn > 0.0)) return hg_fail(ctx, HG_ERR_INVALID, "plane normal is zero");
+      sd.plane_x[2 * k] = d.plane_x[2 * k];
+      sd.plane_x[2 * k + 1] = d.plane_x[2 * k + 1];
+      sd.plane_n[2 * k] = n0 / nn;
+      sd.plane_n[2 * k + 1] = n1 / nn;
+      sd.plane_lower[k] = d.plane_lower ? d.plane_lower[k] : -INFINITY;
+      sd.plane_beh[k] = d.plane_behavior[k];
+    }
+    h_p_off.push_back(uint32_t(p_off));
+    for (int t = 0; t < sd.tiles_x * sd.tiles_y; ++t) tile_scene.push_back(s);
+    p_off += d.npoints;
+    cell_off += uint64_t(sd.snx) * uint64_t(sd.sny);
+    node_off += uint64_t(sd.snx) * uint64_t(sd.sny);
+    tile_off += uint64_t(sd.tiles_x) * uint64_t(sd.tiles_y);
+    h->scenes.push_back(sd);
+  }
+  (void)S;
+  if (p_off >= (1ull << 32) || cell_off >= (1ull << 31) || node_off >= (1ull << 31))
+    return hg_fail(ctx, HG_ERR_UNSUPPORTED, "batch too large for 32-bit indexing");
+  h_p_off.push_back(uint32_t(p_off));
+  h->npoints = uint32_t(p_off);
+  h->ncell = uint32_t(cell_off);
+  h->nnode = uint32_t(node_off);
+  h->ntiles = uint32_t(tile_off);
+  h->nscan = (h->ncell + kScanChunk - 1) / kScanChunk;
+  if (h->nscan > uint32_t(kScanChunk))
+    return hg_fail(ctx, HG_ERR_UNSUPPORTED, "too many cells for the two-level scan");
+
+  hg_mpm *hp = h.get();
+  HG_TRY(dalloc(hp, &h->d_scenes, nscenes * sizeof(SceneDev)));
+  HG_TRY(dalloc(hp, &h->d_p_off, (nscenes + 1) * 4));
+  HG_TRY(dalloc(hp, &h->d_tile_scene, h->ntiles * 4));
+  HG_TRY(dalloc(hp, &h->tile_active, h->ntiles));
+  HG_TRY(dalloc(hp, &h->perm, size_t(h->npoints) * 4));
+  HG_TRY(dalloc(hp, &h->key, size_t(h->npoints) * 4));
+  HG_TRY(dalloc(hp, &h->slot, size_t(h->npoints) * 4));
+  HG_TRY(dalloc(hp, &h->counts, size_t(h->ncell) * 4));
+  HG_TRY(dalloc(hp, &h->cell_start, size_t(h->ncell + 1) * 4));
+  HG_TRY(dalloc(hp, &h->block_sums, size_t(h->nscan) * 4));
+  HG_TRY(dalloc(hp, &h->gva, size_t(h->nnode) * 4 * hp->tsize()));
+  HG_TRY(dalloc(hp, &h->gm, size_t(h->nnode) * 8));
+  HG_TRY(dalloc(hp, &h->gp, size_t(h->nnode) * 16));
+  HG_TRY(dalloc(hp, &h->gpn, size_t(h->nnode) * 16));
+  HG_TRY(dalloc(hp, &h->gf, size_t(h->nnode) * 16));
+  HG_TRY(dalloc(hp, &h->err, nscenes * 4));
+  HG_HIP(ctx, hipMemcpy(h->d_scenes, h->scenes.data(), nscenes * sizeof(SceneDev), hipMemcpyHostToDevice));
+  HG_HIP(ctx, hipMemcpy(h->d_p_off, h_p_off.data(), (nscenes + 1) * 4, hipMemcpyHostToDevice));
+  HG_HIP(ctx, hipMemcpy(h->d_tile_scene, tile_scene.data(), h->ntiles * 4, hipMemcpyHostToDevice));
+  // Zero everything a kernel may read before writing: masked stencil nodes
+  // are read with weight zero and must hold finite values.
+  HG_HIP(ctx, hipMemset(h->tile_active, 0, h->ntiles));
+  HG_HIP(ctx, hipMemset(h->counts, 0, size_t(h->ncell) * 4));
+  HG_HIP(ctx, hipMemset(h->gva, 0, size_t(h->nnode) * 4 * hp->tsize()));
+  HG_HIP(ctx, hipMemset(h->gm, 0, size_t(h->nnode) * 8));
+  HG_HIP(ctx, hipMemset(h->gp, 0, size_t(h->nnode) * 16));
+  HG_HIP(ctx, hipMemset(h->gpn, 0, size_t(h->nnode) * 16));
+  HG_HIP(ctx, hipMemset(h->gf, 0, size_t(h->nnode) * 16));
+  HG_HIP(ctx, hipMemset(h->err, 0, nscenes * 4));
+  if (h->precision == HG_FP64) HG_TRY(upload_points<double>(hp, descs));
+  else HG_TRY(upload_points<float>(hp, descs));
+  // the memsets above run on the null stream; the engine's stream does not
+  // wait for it, so finish them before the first kernel
+  HG_HIP(ctx, hipDeviceSynchronize());
+
+  // initial sort
+  if (h->precision == HG_FP64) launch_bin_p<double>(hp);
+  else launch_bin_p<float>(hp);
+  launch_sort(hp);
+  HG_HIP(ctx, hipGetLastError());
+  HG_HIP(ctx, hipStreamSynchronize(h->stream));
+  std::vector<int32_t> e(nscenes);
+  HG_HIP(ctx, hipMemcpy(e.data(), h->err, nscenes * 4, hipMemcpyDeviceToHost));
+  for (uint32_t s = 0; s < nscenes; ++s)
+    if (e[s]) return hg_fail(ctx, HG_ERR_INVALID, "scene " + std::to_string(s) + ": a point's stencil is off the grid");
+  *out = h.release();
+  return HG_OK;
+}
+
+static void mpm_free(hg_mpm *h) {
+  if (!h) return;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->graph) hipGraphExecDestroy(h->graph);
+  for (void *p : h->allocs) hipFree(p);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+}
+
+static hg_status check_err(hg_mpm *h) {
+  HG_HIP(h->ctx, hipStreamSynchronize(h->stream));
+  std::vector<int32_t> e(h->nscenes);
+  HG_HIP(h->ctx, hipMemcpy(e.data(), h->err, h->nscenes * 4, hipMemcpyDeviceToHost));
+  for (uint32_t s = 0; s < h->nscenes; ++s) {
+    if (e[s] & 1) return hg_fail(h->ctx, HG_ERR_STATE, "scene " + std::to_string(s) + ": a material point left the grid");
+    if (e[s] & 2) return hg_fail(h->ctx, HG_ERR_STATE, "scene " + std::to_string(s) + ": non-finite material point state");
+  }
+  return HG_OK;
+}
+
+// ---- C-ABI --------------------------------------------------------------------
+
+extern "C" hg_status hg_mpm_create(hg_ctx *ctx, const hg_mpm_desc *desc, hg_mpm **out) {
+  return mpm_create_impl(ctx, desc, 1, out);
+}
+
+extern "C" hg_status hg_mpm_create_batch(hg_ctx *ctx, const hg_mpm_desc *descs, uint32_t nscenes,
+                                         hg_mpm **out) {
+  return mpm_create_impl(ctx, descs, nscenes, out);
+}
+
+extern "C" hg_status hg_mpm_destroy(hg_mpm *h) {
+  mpm_free(h);
+  return HG_OK;
+}
+
+extern "C" hg_status hg_mpm_step(hg_mpm *h, double dt, uint32_t nsteps) {
+  if (!h) return HG_ERR_INVALID;
+  if (!(dt > 0.0)) return hg_fail(h->ctx, HG_ERR_INVALID, "dt must be > 0");
+  if (h->phase_next != 0) return hg_fail(h->ctx, HG_ERR_INVALID, "a phased step is in progress");
+  HG_TRY(upload_dt(h, dt));
+  for (uint32_t k = 0; k < nsteps; ++k) launch_step(h);
+  HG_HIP(h->ctx, hipGetLastError());
+  return HG_OK;
+}
+
+extern "C" hg_status hg_mpm_phase(hg_mpm *h, int32_t phase, double dt) {
+  if (!h) return HG_ERR_INVALID;
+  if (phase != h->phase_next)
+    return hg_fail(h->ctx, HG_ERR_INVALID, "phases must run in the order 0, 1, 2");
+  if (!(dt > 0.0) && phase != 0) return hg_fail(h->ctx, HG_ERR_INVALID, "dt must be > 0");
+  if (phase == 0) {
+    // Mass and momentum; the forces and the grid update need only the point
+    // state this phase leaves unchanged, so they are computed in the same pass
+    // and kept until phase 1 hands them over.
+    h->phase_next = 1;
+    HG_TRY(upload_dt(h, dt > 0.0 ? dt : (h->dt_dev > 0.0 ? h->dt_dev : 1.0)));
+    launch_p2g(h, kStoreMass | kStoreMom | kStoreMomNew | kStoreForce);
+  } else if (phase == 1) {
+    if (dt != h->dt_dev) {
+      // dt enters the momentum update: redo the grid pass with this dt
+      HG_TRY(upload_dt(h, dt));
+      launch_p2g(h, kStoreMass | kStoreMom | kStoreMomNew | kStoreForce);
+    }
+    h->phase_next = 2;
+  } else {
+    if (dt != h->dt_dev) return hg_fail(h->ctx, HG_ERR_INVALID, "phase 2 dt differs from phase 1 dt");
+    launch_velacc(h);
+    h->last_store |= kStoreVelAcc;
+    launch_g2p_and_sort(h);
+    h->phase_next = 0;
+  }
+  HG_HIP(h->ctx, hipGetLastError());
+  return HG_OK;
+}
+
+extern "C" hg_status hg_mpm_sync(hg_mpm *h) {
+  if (!h) return HG_ERR_INVALID;
+  return check_err(h);
+}
+
+extern "C" hg_status hg_mpm_scene_error(hg_mpm *h, uint32_t scene, int32_t *flags) {
+  if (!h || scene >= h->nscenes || !flags) return HG_ERR_INVALID;
+  HG_HIP(h->ctx, hipStreamSynchronize(h->stream));
+  HG_HIP(h->ctx, hipMemcpy(flags, h->err + scene, 4, hipMemcpyDeviceToHost));
+  return HG_OK;
+}
+
+template <typename T>
+static hg_status download_points_t(hg_mpm *h, uint32_t scene, double *q, double *v,
+                                   double *be_bar, double *J) {
+  const SceneDev &sc = h->scenes[scene];
+  const uint32_t n = sc.p_cnt, off = sc.p_off;
+  if (n == 0) return HG_OK;
+  const PtsRaw &p = h->buf[h->cur];
+  std::vector<double> x(n), y(n);
+  std::vector<T> f[6];
+  std::vector<uint32_t> id(n);
+  HG_HIP(h->ctx, hipMemcpy(x.data(), p.x + off, n * 8, hipMemcpyDeviceToHost));
+  HG_HIP(h->ctx, hipMemcpy(y.data(), p.y + off, n * 8, hipMemcpyDeviceToHost));
+  HG_HIP(h->ctx, hipMemcpy(id.data(), p.id + off, n * 4, hipMemcpyDeviceToHost));
+  const void *fs[6] = {p.vx, p.vy, p.b00, p.b01, p.b11, p.J};
+  for (int k = 0; k < 6; ++k) {
+    f[k].resize(n);
+    HG_HIP(h->ctx, hipMemcpy(f[k].data(), static_cast<const T *>(fs[k]) + off, n * sizeof(T),
+                             hipMemcpyDeviceToHost));
+  }
+  for (uint32_t s = 0; s < n; ++s) {
+    const uint32_t i = id[s] - off;
+    if (i >= n) return hg_fail(h->ctx, HG_ERR_STATE, "corrupt point ids");
+    if (q) { q[2 * i] = x[s]; q[2 * i + 1] = y[s]; }
+    if (v) { v[2 * i] = double(f[0][s]); v[2 * i + 1] = double(f[1][s]); }
+    if (be_bar) {
+      be_bar[4 * i] = double(f[2][s]);
+      be_bar[4 * i + 1] = double(f[3][s]);
+      be_bar[4 * i + 2] = double(f[3][s]);
+      be_bar[4 * i + 3] = double(f[4][s]);
+    }
+    if (J) J[i] = double(f[5][s]);
+  }
+  return HG_OK;
+}
+
+extern "C" hg_status hg_mpm_download_points_scene(hg_mpm *h, uint32_t scene, double *q, double *v,
+                                                  double *be_bar, double *J) {
+  if (!h || scene >= h->nscenes) return HG_ERR_INVALID;
+  if (h->phase_next != 0) return hg_fail(h->ctx, HG_ERR_INVALID, "a phased step is in progress");
+  HG_TRY(check_err(h));
+  if (h->precision == HG_FP64) return download_points_t<double>(h, scene, q, v, be_bar, J);
+  return download_points_t<float>(h, scene, q, v, be_bar, J);
+}
+
+extern "C" hg_status hg_mpm_download_points(hg_mpm *h, double *q, double *v, double *be_bar,
+                                            double *J) {
+  return hg_mpm_download_points_scene(h, 0, q, v, be_bar, J);
+}
+
+extern "C" hg_status hg_mpm_set_keep_grid(hg_mpm *h, int32_t keep_all) {
+  if (!h) return HG_ERR_INVALID;
+  h->keep_all = keep_all != 0;
+  if (h->graph) {
+    hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  return HG_OK;
+}
+
+template <typename T>
+static hg_status download_grid_t(hg_mpm *h, uint32_t scene, double *mass, double *mom,
+                                 double *mom_new, double *force, double *vel, double *acc) {
+  const SceneDev &sc = h->scenes[scene];
+  const size_t ns = size_t(sc.snx) * size_t(sc.sny);
+  const int TW = h->tw(), TH = h->th();
+  std::vector<uint8_t> act(size_t(sc.tiles_x) * size_t(sc.tiles_y));
+  HG_HIP(h->ctx, hipMemcpy(act.data(), h->tile_active + sc.tile_off, act.size(), hipMemcpyDeviceToHost));
+  std::vector<double> gm(ns), gp(2 * ns), gpn(2 * ns), gf(2 * ns);
+  std::vector<T> gva(4 * ns);
+  const uint32_t st = h->last_store;
+  if (st & kStoreMass) HG_HIP(h->ctx, hipMemcpy(gm.data(), h->gm + sc.node_off, ns * 8, hipMemcpyDeviceToHost));
+  if (st & kStoreMom) HG_HIP(h->ctx, hipMemcpy(gp.data(), h->gp + 2 * size_t(sc.node_off), ns * 16, hipMemcpyDeviceToHost));
+  if (st & kStoreMomNew) HG_HIP(h->ctx, hipMemcpy(gpn.data(), h->gpn + 2 * size_t(sc.node_off), ns * 16, hipMemcpyDeviceToHost));
+  if (st & kStoreForce) HG_HIP(h->ctx, hipMemcpy(gf.data(), h->gf + 2 * size_t(sc.node_off), ns * 16, hipMemcpyDeviceToHost));
+  if (st & kStoreVelAcc)
+    HG_HIP(h->ctx, hipMemcpy(gva.data(), static_cast<const T *>(h->gva) + 4 * size_t(sc.node_off),
+                             ns * 4 * sizeof(T), hipMemcpyDeviceToHost));
+  for (int y = 0; y < sc.nny; ++y)
+    for (int x = 0; x < sc.nnx; ++x) {
+      const size_t r = size_t(y) * size_t(sc.nnx) + size_t(x);
+      const size_t s = size_t(y) * size_t(sc.snx) + size_t(x);
+      const bool on = act[size_t(y / TH) * size_t(sc.tiles_x) + size_t(x / TW)] != 0;
+      if (mass) mass[r] = (on && (st & kStoreMass)) ? gm[s] : 0.0;
+      for (int c = 0; c < 2; ++c) {
+        if (mom) mom[2 * r + c] = (on && (st & kStoreMom)) ? gp[2 * s + c] : 0.0;
+        if (mom_new) mom_new[2 * r + c] = (on && (st & kStoreMomNew)) ? gpn[2 * s + c] : 0.0;
+        if (force) force[2 * r + c] = (on && (st & kStoreForce)) ? gf[2 * s + c] : 0.0;
+        if (vel) vel[2 * r + c] = (on && (st & kStoreVelAcc)) ? double(gva[4 * s + c]) : 0.0;
+        if (acc) acc[2 * r + c] = (on && (st & kStoreVelAcc)) ? double(gva[4 * s + 2 + c]) : 0.0;
+      }
+    }
+  return HG_OK;
+}
+
+extern "C" hg_status hg_mpm_download_grid_scene(hg_mpm *h, uint32_t scene, double *mass, double *mom,
+                                                double *mom_new, double *force, double *vel,
+                                                double *acc) {
+  if (!h || scene >= h->nscenes) return HG_ERR_INVALID;
+  HG_HIP(h->ctx, hipStreamSynchronize(h->stream));
+  if (h->precision == HG_FP64) return download_grid_t<double>(h, scene, mass, mom, mom_new, force, vel, acc);
+  return download_grid_t<float>(h, scene, mass, mom, mom_new, force, vel, acc);
+}
+
+extern "C" hg_status hg_mpm_download_grid(hg_mpm *h, double *mass, double *mom, double *mom_new,
+                                          double *force, double *vel, double *acc) {
+  return hg_mpm_download_grid_scene(h, 0, mass, mom, mom_new, force, vel, acc);
+}
+
+extern "C" hg_status hg_mpm_counts(const hg_mpm *h, uint32_t *npoints, uint32_t *nnodes) {
+  if (!h) return HG_ERR_INVALID;
+  if (npoints) *npoints = h->scenes[0].p_cnt;
+  if (nnodes) *nnodes = uint32_t(h->scenes[0].nnx) * uint32_t(h->scenes[0].nny);
+  return HG_OK;
+}
+
+extern "C" hg_status hg_mpm_set_profiling(hg_mpm *h, int32_t on) {
+  if (!h) return HG_ERR_INVALID;
+  HG_HIP(h->ctx, hipStreamSynchronize(h->stream));
+  h->prof.reset();
+  h->prof.on = on != 0;
+  return HG_OK;
+}
+
+extern "C" hg_status hg_mpm_kernel_times(hg_mpm *h, int32_t max_kernels, char *names, double *ms,
+                                         int64_t *launches, int32_t *nkernels) {
+  if (!h) return HG_ERR_INVALID;
+  return hg_profiler_report(h->prof, h->ctx, max_kernels, names, ms, launches, nkernels);
+}
+
+extern "C" hg_status hg_mpm_set_tile(hg_mpm *h, int32_t aw, int32_t ah) {
+  if (!h) return HG_ERR_INVALID;
+  if (!((aw == 16 && ah == 16) || (aw == 32 && ah == 16) || (aw == 32 && ah == 8)))
+    return hg_fail(h->ctx, HG_ERR_UNSUPPORTED, "tile shape must be 16x16, 32x16 or 32x8");
+  if (h->phase_next != 0) return hg_fail(h->ctx, HG_ERR_INVALID, "a phased step is in progress");
+  if (aw == h->aw && ah == h->ah) return HG_OK;
+  // Tiles index the grid; rebuild the tile tables for the new shape.
+  HG_HIP(h->ctx, hipStreamSynchronize(h->stream));
+  h->aw = aw;
+  h->ah = ah;
+  const int TW = h->tw(), TH = h->th();
+  uint64_t tile_off = 0;
+  std::vector<uint32_t> tile_scene;
+  for (uint32_t s = 0; s < h->nscenes; ++s) {
+    SceneDev &sd = h->scenes[s];
+    sd.tiles_x = (sd.nnx + TW - 1) / TW;
+    sd.tiles_y = (sd.nny + TH - 1) / TH;
+    sd.tile_off = uint32_t(tile_off);
+    for (int t = 0; t < sd.tiles_x * sd.tiles_y; ++t) tile_scene.push_back(s);
+    tile_off += uint64_t(sd.tiles_x) * uint64_t(sd.tiles_y);
+  }
+  h->ntiles = uint32_t(tile_off);
+  void *nts = nullptr, *nta = nullptr;
+  HG_HIP(h->ctx, hipMalloc(&nts, h->ntiles * 4));
+  HG_HIP(h->ctx, hipMalloc(&nta, h->ntiles));
+  h->allocs.push_back(nts);
+  h->allocs.push_back(nta);
+  h->d_tile_scene = static_cast<uint32_t *>(nts);
+  h->tile_active = static_cast<uint8_t *>(nta);
+  HG_HIP(h->ctx, hipMemcpy(h->d_tile_scene, tile_scene.data(), h->ntiles * 4, hipMemcpyHostToDevice));
+  HG_HIP(h->ctx, hipMemset(h->tile_active, 0, h->ntiles));
+  HG_HIP(h->ctx, hipMemcpy(h->d_scenes, h->scenes.data(), h->scenes.size() * sizeof(SceneDev),
+                           hipMemcpyHostToDevice));
+  if (h->graph) {
+    hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  return HG_OK;
+}
+
+extern "C" hg_status hg_mpm_step_graph(hg_mpm *h, double dt, uint32_t nsteps) {
+  if (!h) return HG_ERR_INVALID;
+  if (!(dt > 0.0)) return hg_fail(h->ctx, HG_ERR_INVALID, "dt must be > 0");
+  if (h->phase_next != 0) return hg_fail(h->ctx, HG_ERR_INVALID, "a phased step is in progress");
+  if (h->prof.on) return hg_mpm_step(h, dt, nsteps);  // events are not captured
+  HG_TRY(upload_dt(h, dt));
+  // The point buffers alternate every step, so one graph holds a pair of
+  // steps that starts and ends on buffer 0.
+  if (h->cur != 0 && nsteps) {
+    launch_step(h);
+    --nsteps;
+  }
+  const uint32_t pairs = nsteps / 2;
+  if (pairs) {
+    if (!h->graph) {
+      hipGraph_t g = nullptr;
+      HG_HIP(h->ctx, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      launch_step(h);
+      launch_step(h);
+      HG_HIP(h->ctx, hipStreamEndCapture(h->stream, &g));
+      HG_HIP(h->ctx, hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0));
+      HG_HIP(h->ctx, hipGraphDestroy(g));
+    }
+    for (uint32_t k = 0; k < pairs; ++k) HG_HIP(h->ctx, hipGraphLaunch(h->graph, h->stream));
+  }
+  if (nsteps & 1u) launch_step(h);
+  HG_HIP(h->ctx, hipGetLastError());
+  return HG_OK;
+}

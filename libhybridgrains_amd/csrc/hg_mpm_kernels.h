@@ -1,0 +1,164 @@
+// hg_mpm_kernels.h -- device side of the MPM explicit step (gfx950).
+//
+// Data layout in HBM (one batch = one or more independent scenes):
+//  * points: SoA arrays concatenated over scenes (scene s owns
+//    [p_off, p_off + p_cnt)); x,y are FP64 in every precision, the rest is T
+//    (double for HG_FP64, float for HG_FP32); b_bar is stored as its three
+//    independent entries (b00, b01, b11) because the step keeps it exactly
+//    symmetric.  Two buffers: each step's G2P reads buffer A through the cell
+//    permutation and writes buffer B in cell order, so the points are
+//    physically re-sorted every step at no extra traffic.
+//  * cells: one key per grid node that can be a stencil base; key = base node
+//    (bx, by) as by*snx + bx, where snx = nnx + 1 pads one node column and row
+//    so the masked third uGIMP node of a two-node stencil is always in bounds.
+//  * grid: node storage with the same padded pitch, per scene at node_off.
+//    The fused step writes only (v, a) per node as one T4; split/keep modes
+//    also write FP64 mass, momentum, new momentum and force.
+//
+// Kernels of one step: p2g (tile-owned gather-scatter, also the whole grid
+// update), g2p (gather + return map + advection + re-key), then the counting
+// sort: scan_reduce, scan_top, scan_apply, scatter.
+#pragma once
+
+#include "hg_internal.h"
+
+namespace hgmpm {
+
+constexpr int kMaxPlanes = 8;
+
+enum : uint32_t {
+  kStoreVelAcc = 1u,
+  kStoreMass = 2u,
+  kStoreMom = 4u,
+  kStoreMomNew = 8u,
+  kStoreForce = 16u,
+};
+
+struct SceneDev {
+  double min0, min1, h;
+  double mu, kappa, dp_alpha, pic_alpha, g0, g1;
+  double dt;
+  double hl_uniform;
+  int32_t nnx, nny;        // reference node counts (cell_count + 1)
+  int32_t snx, sny;        // padded storage pitch
+  int32_t tiles_x, tiles_y;
+  uint32_t p_off, p_cnt;   // points
+  uint32_t cell_off;       // keys
+  uint32_t node_off;       // grid storage
+  uint32_t tile_off;       // tiles
+  int32_t nplanes;
+  double plane_x[2 * kMaxPlanes];
+  double plane_n[2 * kMaxPlanes];
+  double plane_lower[kMaxPlanes];
+  int32_t plane_beh[kMaxPlanes];
+};
+
+struct PtsRaw {
+  double *x, *y;
+  void *vx, *vy, *b00, *b01, *b11, *J, *m, *V, *hl;
+  uint32_t *id;
+  uint8_t *flags;
+};
+
+template <int BASIS> struct BasisTraits;
+template <> struct BasisTraits<HG_BASIS_UGIMP_LINEAR> { static constexpr int S = 3; };
+template <> struct BasisTraits<HG_BASIS_LINEAR> { static constexpr int S = 2; };
+template <> struct BasisTraits<HG_BASIS_CUBIC> { static constexpr int S = 4; };
+
+template <typename T> struct Vec4T;
+template <> struct Vec4T<double> { using type = double4; };
+template <> struct Vec4T<float> { using type = float4; };
+
+// Stencil of one axis: first node lo and last node hi (inclusive), as double
+// so an off-grid or NaN coordinate can be detected before any integer cast.
+// uGIMPLinearBasisFunctions.cpp:68-90, LinearBasisFunctions.cpp:48-60,
+// CubicBasisFunctions.cpp:61-75.
+template <int BASIS>
+__device__ __forceinline__ void stencil_1d(double x, double hl, double gmin, double h,
+                                           double &lo, double &hi) {
+  if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) {
+    lo = floor(((x - hl) - gmin) / h);
+    hi = ceil(((x + hl) - gmin) / h);
+  } else if constexpr (BASIS == HG_BASIS_LINEAR) {
+    lo = floor((x - gmin) / h);
+    hi = lo + 1.0;
+  } else {
+    lo = floor((x - gmin) / h) - 1.0;
+    hi = lo + 3.0;
+  }
+}
+
+// Raw 1-D basis factors for nodes lo..lo+S-1 along one axis; nodes past the
+// stencil's hi are masked to exactly zero, as the reference never visits them.
+// uGIMP: LinearIntegral / LinearIntegralGrad (uGIMPLinearBasisFunctions.cpp:6-47);
+// Linear: linearN / lineardN (LinearBasisFunctions.cpp:6-23);
+// Cubic: cubicN / cubicdN (CubicBasisFunctions.cpp:6-46).
+template <typename T, int BASIS>
+__device__ __forceinline__ void basis_1d(double x, T hl, double gmin, double h, int lo,
+                                         int cnt, T (&a)[BasisTraits<BASIS>::S],
+                                         T (&d)[BasisTraits<BASIS>::S]) {
+  constexpr int S = BasisTraits<BASIS>::S;
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) {
+      const double xi = h * double(lo + k) + gmin;
+      const T r = T(x - xi);
+      const T w = T(h);
+      const T diff = fabs(r);
+      const T sgn = r >= T(0) ? T(1) : T(-1);
+      T av, dv;
+      if (diff >= w + hl) {
+        av = T(0);
+        dv = T(0);
+      } else if (diff >= w - hl) {
+        av = (w + hl - diff) * (w + hl - diff) / (T(2) * w);
+        dv = -sgn * (w + hl - diff) / w;
+      } else if (diff >= hl) {
+        av = T(2) * hl * (T(1) - diff / w);
+        dv = -sgn * T(2) * hl / w;
+      } else {
+        av = T(2) * hl - (hl * hl + diff * diff) / w;
+        dv = T(2) * (-r) / w;
+      }
+      a[k] = (k < cnt) ? av : T(0);
+      d[k] = (k < cnt) ? dv : T(0);
+    } else if constexpr (BASIS == HG_BASIS_LINEAR) {
+      const T t = T((x - h * double(lo + k) - gmin) / h);
+      const T at = fabs(t);
+      a[k] = at < T(1) ? T(1) - at : T(0);
+      d[k] = (at <= T(1)) ? ((t >= T(0)) ? T(-1) : T(1)) : T(0);
+    } else {
+      const T t = T((x - h * double(lo + k) - gmin) / h);
+      const T at = fabs(t);
+      const T sg = (t >= T(0)) ? T(1) : T(-1);
+      if (at < T(1)) {
+        a[k] = (T(0.5) * at - T(1)) * at * at + T(2) / T(3);
+        d[k] = sg * (T(1.5) * at - T(2)) * at;
+      } else if (at < T(2)) {
+        a[k] = ((-at / T(6) + T(1)) * at - T(2)) * at + T(4) / T(3);
+        d[k] = sg * ((T(-0.5) * at + T(2)) * at - T(2));
+      } else {
+        a[k] = T(0);
+        d[k] = T(0);
+      }
+    }
+  }
+}
+
+// Node weight and gradient from the 1-D factors, in the reference's operation
+// order (uGIMP :108-123/:141-156; Linear :79-94/:113-135; Cubic :96-110/:129-146).
+template <typename T, int BASIS>
+__device__ __forceinline__ void weight_2d(T ax, T ay, T dx, T dy, T vp_or_h, T &w, T &gx,
+                                          T &gy) {
+  if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) {
+    w = ax * ay / vp_or_h;
+    gx = ay * dx / vp_or_h;
+    gy = ax * dy / vp_or_h;
+  } else {
+    w = ax * ay;
+    gx = ay * dx / vp_or_h;
+    gy = ax * dy / vp_or_h;
+  }
+}
+
+}  // namespace hgmpm

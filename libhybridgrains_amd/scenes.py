@@ -1,0 +1,132 @@
+"""Seeded synthetic scene generators for the benchmark and test workloads.
+
+Parameter mappings (the reference takes shear/bulk moduli and a
+Drucker-Prager alpha, not E, nu and a friction angle):
+  mu = E / (2 (1 + nu)),  kappa = E / (3 (1 - 2 nu))
+  alpha_DP = sqrt(2/3) * 2 sin(phi) / (3 - sin(phi))
+The reference's yield test is ||dev tau|| <= alpha kappa (1 - J^2) / 2
+(mpmgrains2d/ConstitutiveModel.cpp:46-55), i.e. alpha times the Kirchhoff
+pressure; the mapping above is the Drucker-Prager cone matched to the
+Mohr-Coulomb angle phi in plane strain.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from ._lib import HG_BASIS_UGIMP_LINEAR, HG_PLANE_SLIDING
+from .mpm import MpmScene
+
+
+def lame_from_young(E: float, nu: float) -> tuple[float, float]:
+    return E / (2.0 * (1.0 + nu)), E / (3.0 * (1.0 - 2.0 * nu))
+
+
+def dp_alpha_from_friction_angle(phi_deg: float) -> float:
+    s = math.sin(math.radians(phi_deg))
+    return math.sqrt(2.0 / 3.0) * 2.0 * s / (3.0 - s)
+
+
+def column_collapse(
+    *,
+    domain=(1.0, 0.5),
+    h: float = 0.005,
+    column=(0.2, 0.2),
+    ppc_per_dim: int = 2,
+    jitter: float = 0.25,
+    rho: float = 2500.0,
+    E: float = 1.0e6,
+    nu: float = 0.3,
+    phi_deg: float = 30.0,
+    pic_alpha: float = 0.999,
+    gravity=(0.0, -9.81),
+    wall_cells: int = 2,
+    seed: int = 0,
+    basis: int = HG_BASIS_UGIMP_LINEAR,
+) -> MpmScene:
+    """Granular column against the left wall, on the floor.
+
+    The grid spans [0, domain]; the floor and left wall are sliding static
+    planes placed ``wall_cells`` cells inside it so the stencil never leaves
+    the grid.  Points sit on the reference's rectangle lattice (offsets
+    (k + 0.5) h / n inside each cell, InitialSimulationState.cpp:94-99) plus a
+    seeded uniform jitter of +-``jitter`` * h; hl = h / (2 n) and the volume
+    is (2 hl)^2, as rasterizeRectangle sets them (:130-142).
+    """
+    rng = np.random.default_rng(seed)
+    ncx = int(round(domain[0] / h))
+    ncy = int(round(domain[1] / h))
+    x0 = wall_cells * h
+    y0 = wall_cells * h
+    n = ppc_per_dim
+    cw = int(round(column[0] / h))
+    ch = int(round(column[1] / h))
+    ii = np.arange(cw * n)
+    jj = np.arange(ch * n)
+    X, Y = np.meshgrid(x0 + (ii + 0.5) * h / n, y0 + (jj + 0.5) * h / n)
+    q = np.stack([X.ravel(), Y.ravel()], axis=1)
+    if jitter:
+        q = q + rng.uniform(-jitter * h, jitter * h, size=q.shape)
+        q[:, 0] = np.maximum(q[:, 0], x0)
+        q[:, 1] = np.maximum(q[:, 1], y0)
+    N = q.shape[0]
+    hl = 0.5 * h / n
+    vol = hl * hl * 4.0
+    mu, kappa = lame_from_young(E, nu)
+    s = MpmScene(
+        grid_min=np.array([0.0, 0.0]),
+        cell_width=h,
+        cell_count=np.array([ncx, ncy]),
+        shear_modulus=mu,
+        bulk_modulus=kappa,
+        dp_alpha=dp_alpha_from_friction_angle(phi_deg),
+        pic_alpha=pic_alpha,
+        gravity=np.asarray(gravity, float),
+        q=q,
+        v=np.zeros((N, 2)),
+        m=np.full(N, rho * vol),
+        volume=np.full(N, vol),
+        hl=np.full(N, hl),
+        be_bar=np.tile(np.array([1.0, 0.0, 0.0, 1.0]), (N, 1)),
+        J=np.ones(N),
+        basis=basis,
+    )
+    s.add_plane([x0, 0.0], [1.0, 0.0], HG_PLANE_SLIDING)
+    s.add_plane([0.0, y0], [0.0, 1.0], HG_PLANE_SLIDING)
+    return s
+
+
+def mpm_config0(seed: int = 0) -> MpmScene:
+    """Plain MPM column collapse: 1.0 x 0.5 m, h = 5 mm, 0.2 x 0.2 m column,
+    2 x 2 points per cell, 6,400 points, E = 1e6, nu = 0.3, phi = 30 deg."""
+    return column_collapse(seed=seed)
+
+
+def mpm_1m(seed: int = 0) -> MpmScene:
+    """MPM part of the 1M-point hybrid scene: 2000 x 1000 cells at h = 1 mm,
+    0.5 x 0.5 m column at 4 points per cell (1,000,000 points)."""
+    return column_collapse(domain=(2.0, 1.0), h=0.001, column=(0.5, 0.5), seed=seed)
+
+
+def mpm_batch_scenes(nscenes: int = 64, seed: int = 0, n_min: float = 5e3, n_max: float = 5e5,
+                     h: float = 0.001) -> list[MpmScene]:
+    """Independent column collapses of uneven size: point count log-uniform in
+    [n_min, n_max], aspect (height / width) in [0.5, 4], phi in [20, 40] deg,
+    E log-uniform in [1e5, 1e7] Pa, nu = 0.3, 4 points per cell."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(nscenes):
+        N = math.exp(rng.uniform(math.log(n_min), math.log(n_max)))
+        aspect = rng.uniform(0.5, 4.0)
+        phi = rng.uniform(20.0, 40.0)
+        E = math.exp(rng.uniform(math.log(1e5), math.log(1e7)))
+        cells = N / 4.0
+        wc = max(4, int(round(math.sqrt(cells / aspect))))
+        hc = max(4, int(round(wc * aspect)))
+        # room for the collapse: run-out of a few column heights
+        ncx = wc + 3 * hc + 8
+        ncy = hc + 8
+        out.append(column_collapse(domain=(ncx * h, ncy * h), h=h, column=(wc * h, hc * h),
+                                   E=E, phi_deg=phi, seed=seed * 1000 + k))
+    return out
