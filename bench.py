@@ -55,6 +55,16 @@ def _time_mpm(scene, precision, steps, warmup, dt):
     return wall, kt
 
 
+def _copy_gbs(_lib, nbytes=1 << 30, iters=20):
+    """Measured streaming-copy bandwidth (read + write) of this GPU."""
+    import ctypes as C
+
+    ctx = _lib.default_context()
+    out = C.c_double()
+    ctx.check(_lib.lib().hg_bench_copy(ctx.handle, nbytes, iters, C.byref(out)))
+    return out.value
+
+
 def _cpu_baseline(scene, dt, budget_s=20.0):
     """The oracle (FP64 CPU restatement, the reference's OpenMP structure with
     per-thread full-grid copies) on the host cores: a bounded number of steps
@@ -62,6 +72,7 @@ def _cpu_baseline(scene, dt, budget_s=20.0):
     import oracle
 
     nthreads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    nthreads = max(1, min(nthreads, 16))  # the host share of one GPU
     o = oracle.OracleMpm(scene)
     o.flow_omp(dt, 1, nthreads)  # warm-up (first-touch of the per-thread grids)
     steps = 0
@@ -88,6 +99,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--mode", default="full", choices=["full", "mpm32", "mpm64", "copy"],
+                    help="full: the JSON line; mpm32/mpm64: only that precision's steps "
+                         "(for profilers); copy: the streaming-copy reference kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus > 1 is not supported "
@@ -95,8 +109,25 @@ def main() -> int:
         return 2
 
     from libhybridgrains_amd import HG_FP32, HG_FP64, scenes
+    from libhybridgrains_amd import _lib
+
+    if args.mode == "copy":
+        print(json.dumps({"copy_gbs": _copy_gbs(_lib)}), flush=True)
+        return 0
 
     dt = 5.0e-6
+    if args.mode in ("mpm32", "mpm64"):
+        from libhybridgrains_amd import GpuMpm
+
+        gpu = GpuMpm(scenes.mpm_1m(), HG_FP32 if args.mode == "mpm32" else HG_FP64)
+        gpu.flow(dt, args.warmup, graph=True)
+        gpu.sync()
+        t0 = time.perf_counter()
+        gpu.flow(dt, args.steps, graph=True)
+        gpu.sync()
+        print(json.dumps({"mode": args.mode, "ms_per_step": 1e3 * (time.perf_counter() - t0) / args.steps}),
+              flush=True)
+        return 0
     scene = scenes.mpm_1m()
     N = scene.npoints
     steps = args.steps
@@ -128,6 +159,7 @@ def main() -> int:
         cpu = _cpu_baseline(scene, dt, args.cpu_budget)
 
     f32, f64 = out_prec["fp32"], out_prec["fp64"]
+    copy_gbs = _copy_gbs(_lib)
     line = {
         "metric": METRIC,
         "value": f32["value"],
@@ -156,6 +188,7 @@ def main() -> int:
             "roofline": f64["roofline"],
         },
         "fp32_over_fp64": f32["value"] / f64["value"],
+        "hbm_copy_gbs_measured": copy_gbs,
         "kernels_ms_fp32": f32["kernels_ms"],
         "kernels_ms_fp64": f64["kernels_ms"],
     }

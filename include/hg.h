@@ -157,6 +157,10 @@ hg_status hg_mpm_set_tile(hg_mpm *h, int32_t aw, int32_t ah);
 hg_status hg_mpm_set_profiling(hg_mpm *h, int32_t on);
 hg_status hg_mpm_kernel_times(hg_mpm *h, int32_t max_kernels, char *names /* max_kernels*32 */,
                               double *ms, int64_t *launches, int32_t *nkernels);
+/* Streams nbytes from one device buffer to another iters times and returns
+ * the achieved read+write bandwidth; the reference point for HBM counters and
+ * the measured copy roofline of the box. */
+hg_status hg_bench_copy(hg_ctx *ctx, uint64_t nbytes, int32_t iters, double *gbps);
 /* Captures one step into a hipGraph and replays it nsteps times. */
 hg_status hg_mpm_step_graph(hg_mpm *h, double dt, uint32_t nsteps);
 

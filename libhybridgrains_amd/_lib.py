@@ -104,6 +104,7 @@ def lib() -> C.CDLL:
             C.c_int32,
         ),
         "hg_mpm_set_tile": ([vp, C.c_int32, C.c_int32], C.c_int32),
+        "hg_bench_copy": ([vp, C.c_uint64, C.c_int32, _dp], C.c_int32),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)

@@ -117,3 +117,49 @@ hg_status hg_profiler_report(KernelProfiler &prof, hg_ctx *ctx, int32_t max_kern
   }
   return HG_OK;
 }
+
+// ---- streaming copy (bandwidth reference) -------------------------------------
+
+__global__ __launch_bounds__(256) void hg_copy_kernel(const float4 *__restrict__ src,
+                                                      float4 *__restrict__ dst, uint64_t n) {
+  const uint64_t stride = uint64_t(gridDim.x) * 256u;
+  for (uint64_t i = uint64_t(blockIdx.x) * 256u + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+extern "C" hg_status hg_bench_copy(hg_ctx *ctx, uint64_t nbytes, int32_t iters, double *gbps) {
+  if (!ctx || nbytes < 16 || iters <= 0 || !gbps) return hg_fail(ctx, HG_ERR_INVALID, "bad copy arguments");
+  HG_HIP(ctx, hipSetDevice(ctx->device));
+  const uint64_t n = nbytes / 16;
+  void *a = nullptr, *b = nullptr;
+  HG_HIP(ctx, hipMalloc(&a, n * 16));
+  if (hipMalloc(&b, n * 16) != hipSuccess) {
+    hipFree(a);
+    return hg_fail(ctx, HG_ERR_NOMEM, "copy buffer");
+  }
+  hipMemset(a, 0, n * 16);
+  hipStream_t s = nullptr;
+  hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  hipDeviceSynchronize();
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  const uint32_t blocks = 256u * 8u;
+  hipLaunchKernelGGL(hg_copy_kernel, dim3(blocks), dim3(256), 0, s, static_cast<const float4 *>(a),
+                     static_cast<float4 *>(b), n);
+  hipEventRecord(e0, s);
+  for (int i = 0; i < iters; ++i)
+    hipLaunchKernelGGL(hg_copy_kernel, dim3(blocks), dim3(256), 0, s,
+                       static_cast<const float4 *>(a), static_cast<float4 *>(b), n);
+  hipEventRecord(e1, s);
+  hipError_t e = hipEventSynchronize(e1);
+  float ms = 0.0f;
+  hipEventElapsedTime(&ms, e0, e1);
+  *gbps = double(2 * n * 16) * iters / (double(ms) * 1e-3) / 1e9;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipStreamDestroy(s);
+  hipFree(a);
+  hipFree(b);
+  if (e != hipSuccess) return hg_fail(ctx, HG_ERR_HIP, hipGetErrorString(e));
+  return HG_OK;
+}

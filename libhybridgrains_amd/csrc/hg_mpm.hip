@@ -72,15 +72,24 @@ __device__ __forceinline__ bool point_key(const SceneDev &sc, double x, double y
 }
 
 // ---------------------------------------------------------------------------
-// P2G + grid update.  One workgroup per tile of TW x TH owned nodes; its
+// P2G + grid update.  One workgroup per tile of TW x TH owned nodes.  Its
 // AW x AH = (TW+S-1) x (TH+S-1) threads each own one stencil-base cell of the
-// tile's apron.  A thread sums its cell's points (in id order, so the result
-// does not depend on the sort's slot order) into S*S node accumulators in
-// registers, then the S*S stencil offsets are added into LDS one offset per
-// phase: within a phase every cell targets a different node, so no atomics
-// are needed and each node's sum has a fixed order.  Contributions to nodes
-// outside the tile are dropped (the neighbouring tile computes them), so each
-// node is written once, with plain stores, by its owner.
+// tile's apron; the apron's points lie in AH contiguous runs of the cell-sorted
+// order (one per cell row).
+//  1. stage (point-parallel): the apron points are loaded through the
+//     permutation, their Cauchy stress is evaluated (FP64), and a compact
+//     record (position, mass, momentum, -V J, sigma, id) goes to LDS.  Rows
+//     are staged in chunks of at most CAP points.
+//  2. accumulate (cell-parallel, from LDS): each thread sums its cell's points
+//     in id order -- so the sums do not depend on the sort's slot order --
+//     into S*S node accumulators in registers.
+//  3. S*S conflict-free phases add the accumulators into the owned-node LDS
+//     tile: in one phase every cell targets a different node, so no atomics
+//     are needed and each node's sum has a fixed order.  Contributions to
+//     nodes outside the tile are dropped (the neighbouring tile computes
+//     them), so each node is written once, with plain stores, by its owner.
+//  4. grid update of the owned nodes (forces, momentum, plane conditions,
+//     velocity and acceleration), written as one (v, a) vector per node.
 //
 // Follows PhysicsGrid::rasterizePointMasses (PhysicsGrid.cpp:168-207),
 // rasterizePointMomentum (:274-311), MaterialPoints::computeHyperelasticCauchyStress
@@ -102,14 +111,154 @@ struct P2GArgs {
   uint32_t ntiles, nblocks_padded;
 };
 
-template <typename T, int BASIS, int AW, int AH>
-__global__ __launch_bounds__(AW *AH) void mpm_p2g_kernel(P2GArgs args) {
+// What P2G needs of one point.
+template <typename T>
+struct P2GRec {
+  double x, y;
+  T m, mv0, mv1, c, s00, s01, s11, hl;
+  uint32_t id;
+  uint32_t cnt;  // stencil node counts: x in bits 0-3, y in bits 4-7
+};
+
+template <int BASIS>
+__device__ __forceinline__ uint32_t stencil_counts(double x, double y, double hl, const SceneDev &sc) {
+  constexpr int S = BasisTraits<BASIS>::S;
+  double lo_d, hi_d;
+  stencil_1d<BASIS>(x, hl, sc.min0, sc.h, lo_d, hi_d);
+  const double cxd = hi_d - lo_d + 1.0;
+  const int cntx = (cxd >= 1.0 && cxd <= double(S)) ? int(cxd) : S;
+  stencil_1d<BASIS>(y, hl, sc.min1, sc.h, lo_d, hi_d);
+  const double cyd = hi_d - lo_d + 1.0;
+  const int cnty = (cyd >= 1.0 && cyd <= double(S)) ? int(cyd) : S;
+  return uint32_t(cntx) | (uint32_t(cnty) << 4);
+}
+
+template <typename T, int BASIS>
+__device__ __forceinline__ P2GRec<T> p2g_record(const PtsRaw &pts, uint32_t p, const SceneDev &sc) {
+  P2GRec<T> r;
+  r.x = pts.x[p];
+  r.y = pts.y[p];
+  r.m = ld<T>(pts.m, p);
+  const T v0 = ld<T>(pts.vx, p), v1 = ld<T>(pts.vy, p);
+  const T V = ld<T>(pts.V, p);
+  const double J = double(ld<T>(pts.J, p));
+  const double b00 = double(ld<T>(pts.b00, p));
+  const double b01 = double(ld<T>(pts.b01, p));
+  const double b11 = double(ld<T>(pts.b11, p));
+  r.hl = pts.hl ? ld<T>(pts.hl, p) : T(sc.hl_uniform);
+  r.id = pts.id[p];
+  // Cauchy stress sigma = tau / J; tau = 0 when J > 1 (no tension).  FP64.
+  double t00 = 0.0, t01 = 0.0, t11 = 0.0;
+  if (J <= 1.0) {
+    const double e00 = b00 * J, e01 = b01 * J, e11 = b11 * J;
+    const double det = e00 * e11 - e01 * e01;
+    const double Jt = sqrt(det);
+    const double inrm = 1.0 / Jt;
+    const double bb00 = e00 * inrm, bb01 = e01 * inrm, bb11 = e11 * inrm;
+    const double half_tr = (bb00 + bb11) * 0.5;
+    const double c = sc.kappa * 0.5 * (Jt + 1.0) * (Jt - 1.0);
+    t00 = c + sc.mu * (bb00 - half_tr);
+    t01 = sc.mu * bb01;
+    t11 = c + sc.mu * (bb11 - half_tr);
+  }
+  const double iJ = 1.0 / J;
+  r.s00 = T(t00 * iJ);
+  r.s01 = T(t01 * iJ);
+  r.s11 = T(t11 * iJ);
+  r.c = T(-double(V) * J);
+  r.mv0 = r.m * v0;
+  r.mv1 = r.m * v1;
+  r.cnt = stencil_counts<BASIS>(r.x, r.y, double(r.hl), sc);
+  return r;
+}
+
+// Per-cell node accumulators.  FP64 path: FP64 sums.  FP32 path: FP32 sums
+// with the momentum carried as a compensated two-float (TwoSum) pair.
+template <typename T, int S> struct P2GAcc;
+
+template <int S> struct P2GAcc<double, S> {
+  double m[S][S], p0[S][S], p1[S][S], f0[S][S], f1[S][S];
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int j = 0; j < S; ++j)
+#pragma unroll
+      for (int i = 0; i < S; ++i) m[j][i] = p0[j][i] = p1[j][i] = f0[j][i] = f1[j][i] = 0.0;
+  }
+  __device__ __forceinline__ void add_mom(int j, int i, double a, double b) {
+    p0[j][i] += a;
+    p1[j][i] += b;
+  }
+  __device__ __forceinline__ double mom0(int j, int i) const { return p0[j][i]; }
+  __device__ __forceinline__ double mom1(int j, int i) const { return p1[j][i]; }
+};
+
+__device__ __forceinline__ void two_sum(float &s, float &e, float b) {
+  const float a = s;
+  const float t = a + b;
+  const float bb = t - a;
+  e += (a - (t - bb)) + (b - bb);
+  s = t;
+}
+
+template <int S> struct P2GAcc<float, S> {
+  float m[S][S], p0[S][S], p0l[S][S], p1[S][S], p1l[S][S], f0[S][S], f1[S][S];
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int j = 0; j < S; ++j)
+#pragma unroll
+      for (int i = 0; i < S; ++i)
+        m[j][i] = p0[j][i] = p0l[j][i] = p1[j][i] = p1l[j][i] = f0[j][i] = f1[j][i] = 0.0f;
+  }
+  __device__ __forceinline__ void add_mom(int j, int i, float a, float b) {
+    two_sum(p0[j][i], p0l[j][i], a);
+    two_sum(p1[j][i], p1l[j][i], b);
+  }
+  __device__ __forceinline__ double mom0(int j, int i) const { return double(p0[j][i]) + double(p0l[j][i]); }
+  __device__ __forceinline__ double mom1(int j, int i) const { return double(p1[j][i]) + double(p1l[j][i]); }
+};
+
+// Adds one point's contributions to the S x S nodes of stencil base (bx, by).
+template <typename T, int BASIS>
+__device__ __forceinline__ void p2g_accumulate(const P2GRec<T> &r, const SceneDev &sc, int bx,
+                                               int by, P2GAcc<T, BasisTraits<BASIS>::S> &acc) {
+  constexpr int S = BasisTraits<BASIS>::S;
+  const int cntx = int(r.cnt & 15u), cnty = int(r.cnt >> 4);
+  T axs[S], dxs[S], ays[S], dys[S];
+  basis_2d<T, BASIS>(r.x, r.y, r.hl, sc.min0, sc.min1, sc.h, T(sc.inv_h), bx, by, cntx, cnty, axs,
+                     dxs, ays, dys);
+#pragma unroll
+  for (int j = 0; j < S; ++j)
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      const T w = axs[i] * ays[j];
+      const T gx = dxs[i] * ays[j];
+      const T gy = axs[i] * dys[j];
+      acc.m[j][i] += w * r.m;
+      acc.add_mom(j, i, w * r.mv0, w * r.mv1);
+      if (w != T(0)) {
+        const T sw0 = r.s00 * gx + r.s01 * gy;
+        const T sw1 = r.s01 * gx + r.s11 * gy;
+        acc.f0[j][i] += r.c * sw0;
+        acc.f1[j][i] += r.c * sw1;
+      }
+    }
+}
+
+template <typename T, int BASIS, int AW, int AH, int CAP>
+__global__ __launch_bounds__(AW *AH) __attribute__((amdgpu_waves_per_eu(3)))
+void mpm_p2g_kernel(P2GArgs args) {
   constexpr int S = BasisTraits<BASIS>::S;
   constexpr int TW = AW - (S - 1);
   constexpr int TH = AH - (S - 1);
   constexpr int NT = AW * AH;
-  __shared__ double s_m[TW * TH], s_p0[TW * TH], s_p1[TW * TH], s_f0[TW * TH],
-      s_f1[TW * TH];
+  __shared__ double s_x[CAP], s_y[CAP];
+  __shared__ T s_m[CAP], s_mv0[CAP], s_mv1[CAP], s_c[CAP], s_s00[CAP], s_s01[CAP], s_s11[CAP],
+      s_hl[CAP];
+  __shared__ uint32_t s_id[CAP];
+  __shared__ uint8_t s_cnt[CAP];
+  __shared__ double s_nm[TW * TH], s_np0[TW * TH], s_np1[TW * TH], s_nf0[TW * TH],
+      s_nf1[TW * TH];
+  __shared__ uint32_t s_rs[AH], s_rc[AH];
   __shared__ int s_any;
 
   const uint32_t tile = hg_xcd_remap(blockIdx.x, args.nblocks_padded);
@@ -122,141 +271,152 @@ __global__ __launch_bounds__(AW *AH) void mpm_p2g_kernel(P2GArgs args) {
   const int t = threadIdx.x;
   const uint32_t *cs = args.cell_start + sc.cell_off;
 
-  // Apron cell of this thread and its point range.
+  // This thread's apron cell and its run of the sorted order.
+  const int myrow = t / AW;
   const int bx = x0 - (S - 1) + (t % AW);
-  const int by = y0 - (S - 1) + (t / AW);
+  const int by = y0 - (S - 1) + myrow;
   uint32_t s0 = 0, s1 = 0;
   if (bx >= 0 && by >= 0 && bx < sc.nnx && by < sc.nny) {
     const uint32_t key = uint32_t(by) * uint32_t(sc.snx) + uint32_t(bx);
     s0 = cs[key];
     s1 = cs[key + 1];
   }
+  // Row runs: cells of one apron row are consecutive keys.
+  if (t < AH) {
+    const int ry = y0 - (S - 1) + t;
+    const int xl = max(x0 - (S - 1), 0), xh = min(x0 - (S - 1) + AW - 1, sc.nnx - 1);
+    uint32_t rs = 0, rc = 0;
+    if (ry >= 0 && ry < sc.nny && xl <= xh) {
+      const uint32_t k0 = uint32_t(ry) * uint32_t(sc.snx) + uint32_t(xl);
+      const uint32_t k1 = uint32_t(ry) * uint32_t(sc.snx) + uint32_t(xh) + 1u;
+      rs = cs[k0];
+      rc = cs[k1] - rs;
+    }
+    s_rs[t] = rs;
+    s_rc[t] = rc;
+  }
   if (t == 0) s_any = 0;
   for (int k = t; k < TW * TH; k += NT) {
-    s_m[k] = 0.0;
-    s_p0[k] = 0.0;
-    s_p1[k] = 0.0;
-    s_f0[k] = 0.0;
-    s_f1[k] = 0.0;
+    s_nm[k] = 0.0;
+    s_np0[k] = 0.0;
+    s_np1[k] = 0.0;
+    s_nf0[k] = 0.0;
+    s_nf1[k] = 0.0;
   }
   __syncthreads();
-  if (s1 > s0) s_any = 1;
+  if (t < AH && s_rc[t] > 0) s_any = 1;
   __syncthreads();
   const bool any = s_any != 0;
   if (t == 0) args.tile_active[tile] = any ? 1 : 0;
   if (!any) return;
 
-  double acc[S][S][5];
-#pragma unroll
-  for (int j = 0; j < S; ++j)
-#pragma unroll
-    for (int i = 0; i < S; ++i)
-#pragma unroll
-      for (int f = 0; f < 5; ++f) acc[j][i][f] = 0.0;
-
-  const double mu = sc.mu, kappa = sc.kappa;
   const uint32_t n = s1 - s0;
-  uint32_t last_id = 0;
-  for (uint32_t k = 0; k < n; ++k) {
-    // next point of the cell in id order
-    uint32_t best = 0xffffffffu, bp = 0;
-    for (uint32_t s = s0; s < s1; ++s) {
-      const uint32_t pi = args.perm[s];
-      const uint32_t idv = args.pts.id[pi];
-      if ((k == 0 || idv > last_id) && idv <= best) {
-        best = idv;
-        bp = pi;
+  int r0 = 0;
+  while (r0 < AH) {
+    // rows [r0, r1) are staged together (block-uniform decision); a single
+    // row holding more than CAP points is read straight from HBM instead
+    uint32_t tot = 0;
+    int r1 = r0;
+    while (r1 < AH && tot + s_rc[r1] <= uint32_t(CAP)) {
+      tot += s_rc[r1];
+      ++r1;
+    }
+    const bool staged = r1 > r0;
+    if (!staged) r1 = r0 + 1;
+    // 1. stage the chunk's points
+    if (staged) {
+      for (uint32_t k = uint32_t(t); k < tot; k += NT) {
+        int r = r0;
+        uint32_t pre = 0;
+        while (k >= pre + s_rc[r]) {
+          pre += s_rc[r];
+          ++r;
+        }
+        const uint32_t p = args.perm[s_rs[r] + (k - pre)];
+        const P2GRec<T> rec = p2g_record<T, BASIS>(args.pts, p, sc);
+        s_x[k] = rec.x;
+        s_y[k] = rec.y;
+        s_m[k] = rec.m;
+        s_mv0[k] = rec.mv0;
+        s_mv1[k] = rec.mv1;
+        s_c[k] = rec.c;
+        s_s00[k] = rec.s00;
+        s_s01[k] = rec.s01;
+        s_s11[k] = rec.s11;
+        s_hl[k] = rec.hl;
+        s_id[k] = rec.id;
+        s_cnt[k] = uint8_t(rec.cnt);
+      }
+      __syncthreads();
+    }
+    // 2. accumulate this thread's cell, in id order
+    const bool mine = myrow >= r0 && myrow < r1 && n > 0;
+    P2GAcc<T, S> acc;
+    acc.zero();
+    if (mine) {
+      uint32_t base = s0 - s_rs[myrow];
+      for (int r = r0; r < myrow; ++r) base += s_rc[r];
+      uint32_t last_id = 0;
+      for (uint32_t k = 0; k < n; ++k) {
+        uint32_t best = 0xffffffffu, bq = 0;
+        for (uint32_t q = 0; q < n; ++q) {
+          const uint32_t idv = staged ? s_id[base + q] : args.pts.id[args.perm[s0 + q]];
+          if ((k == 0 || idv > last_id) && idv <= best) {
+            best = idv;
+            bq = q;
+          }
+        }
+        last_id = best;
+        P2GRec<T> rec;
+        if (staged) {
+          const uint32_t q = base + bq;
+          rec.x = s_x[q];
+          rec.y = s_y[q];
+          rec.m = s_m[q];
+          rec.mv0 = s_mv0[q];
+          rec.mv1 = s_mv1[q];
+          rec.c = s_c[q];
+          rec.s00 = s_s00[q];
+          rec.s01 = s_s01[q];
+          rec.s11 = s_s11[q];
+          rec.hl = s_hl[q];
+          rec.cnt = s_cnt[q];
+        } else {
+          rec = p2g_record<T, BASIS>(args.pts, args.perm[s0 + bq], sc);
+        }
+        p2g_accumulate<T, BASIS>(rec, sc, bx, by, acc);
       }
     }
-    last_id = best;
-    const uint32_t p = bp;
-
-    const double x = args.pts.x[p], y = args.pts.y[p];
-    const T m = ld<T>(args.pts.m, p);
-    const T v0 = ld<T>(args.pts.vx, p), v1 = ld<T>(args.pts.vy, p);
-    const T V = ld<T>(args.pts.V, p);
-    const double J = double(ld<T>(args.pts.J, p));
-    const double b00 = double(ld<T>(args.pts.b00, p));
-    const double b01 = double(ld<T>(args.pts.b01, p));
-    const double b11 = double(ld<T>(args.pts.b11, p));
-    const T hl = args.pts.hl ? ld<T>(args.pts.hl, p) : T(sc.hl_uniform);
-
-    // Cauchy stress sigma = tau / J, tau = 0 when J > 1 (no tension), FP64.
-    double t00 = 0.0, t01 = 0.0, t11 = 0.0;
-    if (J <= 1.0) {
-      const double e00 = b00 * J, e01 = b01 * J, e11 = b11 * J;
-      const double det = e00 * e11 - e01 * e01;
-      const double Jt = sqrt(det);
-      const double nrm = sqrt(det);
-      const double bb00 = e00 / nrm, bb01 = e01 / nrm, bb11 = e11 / nrm;
-      const double half_tr = (bb00 + bb11) * 1.0 / 2.0;
-      const double c = kappa * 0.5 * (Jt + 1.0) * (Jt - 1.0);
-      t00 = c + mu * (bb00 - half_tr);
-      t01 = 0.0 + mu * (bb01 - 0.0);
-      t11 = c + mu * (bb11 - half_tr);
-    }
-    const T s00 = T(t00 / J), s01 = T(t01 / J), s11 = T(t11 / J);
-    const T cf = T(-double(V) * J);
-
-    // the stencil base of p is (bx, by) by construction of the sort
-    double lo_d, hi_d;
-    stencil_1d<BASIS>(x, double(hl), sc.min0, sc.h, lo_d, hi_d);
-    const double cxd = hi_d - lo_d + 1.0;
-    const int cntx = (cxd >= 1.0 && cxd <= double(S)) ? int(cxd) : S;
-    stencil_1d<BASIS>(y, double(hl), sc.min1, sc.h, lo_d, hi_d);
-    const double cyd = hi_d - lo_d + 1.0;
-    const int cnty = (cyd >= 1.0 && cyd <= double(S)) ? int(cyd) : S;
-    T axs[S], dxs[S], ays[S], dys[S];
-    basis_1d<T, BASIS>(x, hl, sc.min0, sc.h, bx, cntx, axs, dxs);
-    basis_1d<T, BASIS>(y, hl, sc.min1, sc.h, by, cnty, ays, dys);
-    const T div = (BASIS == HG_BASIS_UGIMP_LINEAR) ? T(4) * hl * hl : T(sc.h);
-    const T mv0 = m * v0, mv1 = m * v1;
+    // 3. S*S conflict-free phases add the chunk's cell sums into the
+    //    owned-node LDS tile (the last barrier also frees the staging arrays)
 #pragma unroll
     for (int j = 0; j < S; ++j)
 #pragma unroll
       for (int i = 0; i < S; ++i) {
-        T w, gx, gy;
-        weight_2d<T, BASIS>(axs[i], ays[j], dxs[i], dys[j], div, w, gx, gy);
-        acc[j][i][0] += double(w * m);
-        acc[j][i][1] += double(w * mv0);
-        acc[j][i][2] += double(w * mv1);
-        if (w != T(0)) {
-          const T sw0 = s00 * gx + s01 * gy;
-          const T sw1 = s01 * gx + s11 * gy;
-          acc[j][i][3] += double(cf * sw0);
-          acc[j][i][4] += double(cf * sw1);
+        const int lx = bx + i - x0, ly = by + j - y0;
+        if (mine && lx >= 0 && lx < TW && ly >= 0 && ly < TH) {
+          const int q = ly * TW + lx;
+          s_nm[q] += double(acc.m[j][i]);
+          s_np0[q] += acc.mom0(j, i);
+          s_np1[q] += acc.mom1(j, i);
+          s_nf0[q] += double(acc.f0[j][i]);
+          s_nf1[q] += double(acc.f1[j][i]);
         }
+        __syncthreads();
       }
+    r0 = r1;
   }
 
-  // S*S conflict-free accumulation phases into the owned-node LDS tile.
-  const bool has = n > 0;
-#pragma unroll
-  for (int j = 0; j < S; ++j)
-#pragma unroll
-    for (int i = 0; i < S; ++i) {
-      const int lx = bx + i - x0, ly = by + j - y0;
-      if (has && lx >= 0 && lx < TW && ly >= 0 && ly < TH) {
-        const int q = ly * TW + lx;
-        s_m[q] += acc[j][i][0];
-        s_p0[q] += acc[j][i][1];
-        s_p1[q] += acc[j][i][2];
-        s_f0[q] += acc[j][i][3];
-        s_f1[q] += acc[j][i][4];
-      }
-      __syncthreads();
-    }
-
-  // Grid update of the owned nodes.
+  // 4. grid update of the owned nodes.
   using T4 = typename Vec4T<T>::type;
   const double dt = sc.dt;
   for (int q = t; q < TW * TH; q += NT) {
     const int nx = x0 + (q % TW), ny = y0 + (q / TW);
     if (nx >= sc.nnx || ny >= sc.nny) continue;
     const uint32_t node = sc.node_off + uint32_t(ny) * uint32_t(sc.snx) + uint32_t(nx);
-    const double m = s_m[q];
-    const double p0 = s_p0[q], p1 = s_p1[q];
-    double f0 = s_f0[q], f1 = s_f1[q];
+    const double m = s_nm[q];
+    const double p0 = s_np0[q], p1 = s_np1[q];
+    double f0 = s_nf0[q], f1 = s_nf1[q];
     f0 += m * sc.g0;
     f1 += m * sc.g1;
     double pn0 = 0.0, pn1 = 0.0;
@@ -401,9 +561,8 @@ __global__ __launch_bounds__(kG2PThreads) void mpm_g2p_kernel(G2PArgs a) {
   int bx, by, cx, cy;
   point_key<BASIS>(sc, x, y, double(hl), bx, by, cx, cy);
   T axs[S], dxs[S], ays[S], dys[S];
-  basis_1d<T, BASIS>(x, hl, sc.min0, sc.h, bx, cx, axs, dxs);
-  basis_1d<T, BASIS>(y, hl, sc.min1, sc.h, by, cy, ays, dys);
-  const T div = (BASIS == HG_BASIS_UGIMP_LINEAR) ? T(4) * hl * hl : T(sc.h);
+  basis_2d<T, BASIS>(x, y, hl, sc.min0, sc.min1, sc.h, T(sc.inv_h), bx, by, cx, cy, axs, dxs, ays,
+                     dys);
 
   const T4 *g = static_cast<const T4 *>(a.gva) + sc.node_off;
   T L00 = 0, L10 = 0, L01 = 0, L11 = 0;
@@ -414,8 +573,9 @@ __global__ __launch_bounds__(kG2PThreads) void mpm_g2p_kernel(G2PArgs a) {
 #pragma unroll
     for (int ii = 0; ii < S; ++ii) {
       const T4 nv = g[row + ii];
-      T w, gx, gy;
-      weight_2d<T, BASIS>(axs[ii], ays[j], dxs[ii], dys[j], div, w, gx, gy);
+      const T w = axs[ii] * ays[j];
+      const T gx = dxs[ii] * ays[j];
+      const T gy = axs[ii] * dys[j];
       L00 += nv.x * gx;
       L10 += nv.y * gx;
       L01 += nv.x * gy;
@@ -714,6 +874,10 @@ static hg_status dalloc(hg_mpm *h, P **out, size_t bytes) {
 
 // ---- kernel dispatch ----------------------------------------------------------
 
+template <typename T> struct P2GCap;
+template <> struct P2GCap<float> { static constexpr int value = 640; };
+template <> struct P2GCap<double> { static constexpr int value = 512; };
+
 template <typename T, int BASIS, int AW, int AH>
 static void launch_p2g_t(hg_mpm *h, uint32_t store) {
   P2GArgs a;
@@ -732,7 +896,7 @@ static void launch_p2g_t(hg_mpm *h, uint32_t store) {
   a.ntiles = h->ntiles;
   a.nblocks_padded = hg_pad8(h->ntiles);
   ProfScope ps(h->prof, "mpm_p2g", h->stream);
-  hipLaunchKernelGGL((mpm_p2g_kernel<T, BASIS, AW, AH>), dim3(a.nblocks_padded), dim3(AW * AH),
+  hipLaunchKernelGGL((mpm_p2g_kernel<T, BASIS, AW, AH, P2GCap<T>::value>), dim3(a.nblocks_padded), dim3(AW * AH),
                      0, h->stream, a);
 }
 
@@ -1005,6 +1169,7 @@ static hg_status mpm_create_impl(hg_ctx *ctx, const hg_mpm_desc *descs, uint32_t
     sd.min0 = d.grid_min[0];
     sd.min1 = d.grid_min[1];
     sd.h = d.cell_width;
+    sd.inv_h = 1.0 / d.cell_width;
     sd.mu = d.shear_modulus;
     sd.kappa = d.bulk_modulus;
     sd.dp_alpha = d.dp_alpha;

@@ -39,6 +39,7 @@ struct SceneDev {
   double mu, kappa, dp_alpha, pic_alpha, g0, g1;
   double dt;
   double hl_uniform;
+  double inv_h;            // 1 / h
   int32_t nnx, nny;        // reference node counts (cell_count + 1)
   int32_t snx, sny;        // padded storage pitch
   int32_t tiles_x, tiles_y;
@@ -88,13 +89,15 @@ __device__ __forceinline__ void stencil_1d(double x, double hl, double gmin, dou
   }
 }
 
-// Raw 1-D basis factors for nodes lo..lo+S-1 along one axis; nodes past the
+// 1-D basis factors for nodes lo..lo+S-1 along one axis; nodes past the
 // stencil's hi are masked to exactly zero, as the reference never visits them.
 // uGIMP: LinearIntegral / LinearIntegralGrad (uGIMPLinearBasisFunctions.cpp:6-47);
 // Linear: linearN / lineardN (LinearBasisFunctions.cpp:6-23);
 // Cubic: cubicN / cubicdN (CubicBasisFunctions.cpp:6-46).
+// Divisions by the cell width are taken as products with its reciprocal
+// (inv_w), which moves results by at most an ulp or two.
 template <typename T, int BASIS>
-__device__ __forceinline__ void basis_1d(double x, T hl, double gmin, double h, int lo,
+__device__ __forceinline__ void basis_1d(double x, T hl, double gmin, double h, T inv_w, int lo,
                                          int cnt, T (&a)[BasisTraits<BASIS>::S],
                                          T (&d)[BasisTraits<BASIS>::S]) {
   constexpr int S = BasisTraits<BASIS>::S;
@@ -111,31 +114,32 @@ __device__ __forceinline__ void basis_1d(double x, T hl, double gmin, double h, 
         av = T(0);
         dv = T(0);
       } else if (diff >= w - hl) {
-        av = (w + hl - diff) * (w + hl - diff) / (T(2) * w);
-        dv = -sgn * (w + hl - diff) / w;
+        const T u = w + hl - diff;
+        av = u * u * (T(0.5) * inv_w);
+        dv = -sgn * u * inv_w;
       } else if (diff >= hl) {
-        av = T(2) * hl * (T(1) - diff / w);
-        dv = -sgn * T(2) * hl / w;
+        av = T(2) * hl * (T(1) - diff * inv_w);
+        dv = -sgn * T(2) * hl * inv_w;
       } else {
-        av = T(2) * hl - (hl * hl + diff * diff) / w;
-        dv = T(2) * (-r) / w;
+        av = T(2) * hl - (hl * hl + diff * diff) * inv_w;
+        dv = T(2) * (-r) * inv_w;
       }
       a[k] = (k < cnt) ? av : T(0);
       d[k] = (k < cnt) ? dv : T(0);
     } else if constexpr (BASIS == HG_BASIS_LINEAR) {
-      const T t = T((x - h * double(lo + k) - gmin) / h);
+      const T t = T((x - h * double(lo + k) - gmin) * double(inv_w));
       const T at = fabs(t);
       a[k] = at < T(1) ? T(1) - at : T(0);
       d[k] = (at <= T(1)) ? ((t >= T(0)) ? T(-1) : T(1)) : T(0);
     } else {
-      const T t = T((x - h * double(lo + k) - gmin) / h);
+      const T t = T((x - h * double(lo + k) - gmin) * double(inv_w));
       const T at = fabs(t);
       const T sg = (t >= T(0)) ? T(1) : T(-1);
       if (at < T(1)) {
-        a[k] = (T(0.5) * at - T(1)) * at * at + T(2) / T(3);
+        a[k] = (T(0.5) * at - T(1)) * at * at + T(2.0 / 3.0);
         d[k] = sg * (T(1.5) * at - T(2)) * at;
       } else if (at < T(2)) {
-        a[k] = ((-at / T(6) + T(1)) * at - T(2)) * at + T(4) / T(3);
+        a[k] = ((-at * T(1.0 / 6.0) + T(1)) * at - T(2)) * at + T(4.0 / 3.0);
         d[k] = sg * ((T(-0.5) * at + T(2)) * at - T(2));
       } else {
         a[k] = T(0);
@@ -145,19 +149,34 @@ __device__ __forceinline__ void basis_1d(double x, T hl, double gmin, double h, 
   }
 }
 
-// Node weight and gradient from the 1-D factors, in the reference's operation
-// order (uGIMP :108-123/:141-156; Linear :79-94/:113-135; Cubic :96-110/:129-146).
+// Both axes' factors, pre-scaled so that a node's weight is ax[i]*ay[j] and
+// its gradient (dx[i]*ay[j], ax[i]*dy[j]): uGIMP divides the product of the
+// overlap integrals by the point volume 4 hl^2 (uGIMP :108-123/:141-156);
+// the standard bases divide the gradient by the cell width (Linear
+// :113-135, Cubic :129-146).
 template <typename T, int BASIS>
-__device__ __forceinline__ void weight_2d(T ax, T ay, T dx, T dy, T vp_or_h, T &w, T &gx,
-                                          T &gy) {
+__device__ __forceinline__ void basis_2d(double x, double y, T hl, double min0, double min1,
+                                         double h, T inv_h, int bx, int by, int cntx, int cnty,
+                                         T (&ax)[BasisTraits<BASIS>::S],
+                                         T (&dx)[BasisTraits<BASIS>::S],
+                                         T (&ay)[BasisTraits<BASIS>::S],
+                                         T (&dy)[BasisTraits<BASIS>::S]) {
+  constexpr int S = BasisTraits<BASIS>::S;
+  basis_1d<T, BASIS>(x, hl, min0, h, inv_h, bx, cntx, ax, dx);
+  basis_1d<T, BASIS>(y, hl, min1, h, inv_h, by, cnty, ay, dy);
   if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) {
-    w = ax * ay / vp_or_h;
-    gx = ay * dx / vp_or_h;
-    gy = ax * dy / vp_or_h;
+    const T inv_vp = T(1) / (T(4) * hl * hl);
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      ax[k] *= inv_vp;
+      dx[k] *= inv_vp;
+    }
   } else {
-    w = ax * ay;
-    gx = ay * dx / vp_or_h;
-    gy = ax * dy / vp_or_h;
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      dx[k] *= inv_h;
+      dy[k] *= inv_h;
+    }
   }
 }
 

@@ -130,3 +130,68 @@ def mpm_batch_scenes(nscenes: int = 64, seed: int = 0, n_min: float = 5e3, n_max
         out.append(column_collapse(domain=(ncx * h, ncy * h), h=h, column=(wc * h, hc * h),
                                    E=E, phi_deg=phi, seed=seed * 1000 + k))
     return out
+
+
+# ---- DEM ---------------------------------------------------------------------
+
+def damping_from_restitution(e: float, kn: float, m_eff: float) -> float:
+    """gamma_n for the reference's normal law F = -kn d - gamma_n v_n / 2
+    (PenaltyImpactFrictionMap.cpp:650-651): with zeta = -ln e / sqrt(pi^2 +
+    ln^2 e), the linear spring-dashpot m* x'' = -kn x - (gamma_n/2) x' has
+    restitution e when gamma_n = 4 zeta sqrt(kn m*)."""
+    le = math.log(e)
+    zeta = -le / math.sqrt(math.pi ** 2 + le * le)
+    return 4.0 * zeta * math.sqrt(kn * m_eff)
+
+
+def dem_column(
+    *,
+    n_bodies: int = 20000,
+    width: float = 0.15,
+    spacing: float = 2.2e-3,
+    r_mean: float = 1.0e-3,
+    r_spread: float = 0.1e-3,
+    jitter: float = 0.05e-3,
+    rho: float = 2500.0,
+    kn: float = 1.0e5,
+    restitution: float = 0.3,
+    mu: float = 0.3,
+    box_width: float = 1.2,
+    gravity=(0.0, -9.81),
+    seed: int = 0,
+):
+    """Circle column collapse: radii uniform in r_mean +- r_spread on a
+    hexagonal lattice of the given spacing with seeded jitter, filled row by
+    row over ``width`` until ``n_bodies`` circles are placed (a 0.15 m wide
+    column of 20k circles is about 0.56 m tall).  Floor and two walls; the
+    right wall stands ``box_width`` from the left one so the column can
+    collapse.  kt = kn and gamma_t = gamma_n; gamma_n from the restitution of
+    a pair of mean circles."""
+    from .dem import DemScene, circle_mass
+
+    rng = np.random.default_rng(seed)
+    per_row = int(width / spacing)
+    dy = spacing * math.sqrt(3.0) / 2.0
+    rows = (n_bodies + per_row - 1) // per_row
+    pts = []
+    for j in range(rows):
+        off = 0.5 * spacing if (j % 2) else 0.0
+        for i in range(per_row):
+            if len(pts) == n_bodies:
+                break
+            pts.append((spacing * (i + 0.5) + off, r_mean + r_spread + 1e-4 + j * dy))
+    q = np.zeros((n_bodies, 3))
+    q[:, :2] = np.asarray(pts) + rng.uniform(-jitter, jitter, size=(n_bodies, 2))
+    q[:, 1] = np.maximum(q[:, 1], r_mean + r_spread)
+    q[:, 0] = np.maximum(q[:, 0], r_mean + r_spread)
+    radius = rng.uniform(r_mean - r_spread, r_mean + r_spread, n_bodies)
+    mass = circle_mass(rho, radius)
+    m_pair = 0.5 * rho * math.pi * r_mean * r_mean
+    gn = damping_from_restitution(restitution, kn, m_pair)
+    s = DemScene(q=q, v=np.zeros((n_bodies, 3)), radius=radius, mass=mass,
+                 fixed=np.zeros(n_bodies, np.uint8), gravity=np.asarray(gravity, float),
+                 kn=kn, kt=kn, gamman=gn, gammat=gn, mu=mu)
+    s.add_plane([0.0, 0.0], [0.0, 1.0])
+    s.add_plane([0.0, 0.0], [1.0, 0.0])
+    s.add_plane([box_width, 0.0], [-1.0, 0.0])
+    return s

@@ -40,6 +40,20 @@ class OracleMpmC(C.Structure):
     ]
 
 
+class OracleDemC(C.Structure):
+    _fields_ = [
+        ("nbodies", C.c_uint32),
+        ("q", _dp), ("v", _dp), ("mass", _dp), ("minv", _dp), ("radius", _dp),
+        ("fixed", _u8p), ("weights", _dp),
+        ("gravity", C.c_double * 2),
+        ("nplanes", C.c_uint32),
+        ("plane_x", _dp), ("plane_n", _dp),
+        ("kn", C.c_double), ("kt", C.c_double), ("gamman", C.c_double),
+        ("gammat", C.c_double), ("mu", C.c_double),
+        ("hist", C.c_void_p),
+    ]
+
+
 _lib = None
 
 
@@ -95,6 +109,19 @@ def lib() -> C.CDLL:
     L.oracle_plastic_correction.restype = None
     L.oracle_compute_tau.argtypes = [_dp, C.c_double, C.c_double, C.c_int32, _dp]
     L.oracle_compute_tau.restype = None
+    PD = C.POINTER(OracleDemC)
+    L.oracle_dem_init.argtypes = [PD]
+    L.oracle_dem_init.restype = C.c_int
+    L.oracle_dem_free.argtypes = [PD]
+    L.oracle_dem_free.restype = None
+    L.oracle_dem_flow.argtypes = [PD, C.c_double, C.c_uint32]
+    L.oracle_dem_flow.restype = C.c_int
+    L.oracle_dem_contacts.argtypes = [PD, _ip, _ip, _ip, _dp, C.c_uint32]
+    L.oracle_dem_contacts.restype = C.c_uint32
+    L.oracle_dem_pairs_bruteforce.argtypes = [PD, _ip, _ip, C.c_uint32]
+    L.oracle_dem_pairs_bruteforce.restype = C.c_uint32
+    L.oracle_dem_last_force.argtypes = [PD, _dp]
+    L.oracle_dem_last_force.restype = None
     _lib = L
     return L
 
@@ -188,3 +215,73 @@ class OracleMpm:
 
     def is_consistent(self) -> int:
         return int(self.L.oracle_mpm_is_consistent(self.ref))
+
+
+class OracleDem:
+    """A RigidBody2DState of circles stepped by the C restatement of the
+    penalty map.  ``scene`` has the ``DemScene`` fields."""
+
+    def __init__(self, scene, weights=None):
+        self.L = lib()
+        s = scene
+        B = int(s.q.shape[0])
+        self.B = B
+        f8 = lambda a: np.ascontiguousarray(np.array(a, dtype=np.float64))  # noqa: E731
+        self.q = f8(s.q).reshape(-1).copy()
+        self.v = f8(s.v).reshape(-1).copy()
+        self.mass = f8(s.mass).reshape(-1).copy()
+        self.minv = 1.0 / self.mass
+        self.radius = f8(s.radius).copy()
+        self.fixed = np.ascontiguousarray(s.fixed, np.uint8).copy()
+        self.weights = f8(weights).copy() if weights is not None else None
+        self.plane_x = f8(s.plane_x).reshape(-1).copy()
+        self.plane_n = f8(s.plane_n).reshape(-1).copy()
+        c = OracleDemC()
+        c.nbodies = B
+        for k in ("q", "v", "mass", "minv", "radius", "plane_x", "plane_n"):
+            setattr(c, k, _p(getattr(self, k)))
+        c.fixed = _p(self.fixed, C.c_uint8)
+        c.weights = _p(self.weights) if self.weights is not None else None
+        c.gravity[:] = [float(s.gravity[0]), float(s.gravity[1])]
+        c.nplanes = int(s.plane_x.shape[0])
+        c.kn, c.kt, c.gamman, c.gammat, c.mu = (float(s.kn), float(s.kt), float(s.gamman),
+                                                float(s.gammat), float(s.mu))
+        c.hist = None
+        self.c = c
+        if self.L.oracle_dem_init(C.byref(c)) != 0:
+            raise MemoryError("oracle_dem_init")
+
+    def __del__(self):
+        try:
+            self.L.oracle_dem_free(C.byref(self.c))
+        except Exception:
+            pass
+
+    def flow(self, dt: float, nsteps: int = 1) -> None:
+        if self.L.oracle_dem_flow(C.byref(self.c), float(dt), int(nsteps)) != 0:
+            raise MemoryError("oracle_dem_flow")
+
+    def contacts(self) -> dict:
+        n = self.L.oracle_dem_contacts(C.byref(self.c), None, None, None, None, 0)
+        t = np.zeros(n, np.int32)
+        i = np.zeros(n, np.int32)
+        j = np.zeros(n, np.int32)
+        ds = np.zeros((n, 2))
+        self.L.oracle_dem_contacts(C.byref(self.c), _p(t, C.c_int32), _p(i, C.c_int32),
+                                   _p(j, C.c_int32), _p(ds), n)
+        return {"type": t, "i": i, "j": j, "delta_s": ds}
+
+    def pairs_bruteforce(self) -> np.ndarray:
+        n = self.L.oracle_dem_pairs_bruteforce(C.byref(self.c), None, None, 0)
+        i = np.zeros(n, np.int32)
+        j = np.zeros(n, np.int32)
+        self.L.oracle_dem_pairs_bruteforce(C.byref(self.c), _p(i, C.c_int32), _p(j, C.c_int32), n)
+        return np.stack([i, j], 1)
+
+    def last_force(self) -> np.ndarray:
+        F = np.zeros(3 * self.B)
+        self.L.oracle_dem_last_force(C.byref(self.c), _p(F))
+        return F.reshape(-1, 3)
+
+    def state(self) -> dict:
+        return {"q": self.q.reshape(-1, 3).copy(), "v": self.v.reshape(-1, 3).copy()}

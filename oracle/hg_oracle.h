@@ -120,6 +120,49 @@ int oracle_mpm_is_consistent(const oracle_mpm *s);
  * summation order only. */
 void oracle_mpm_flow_n_omp(oracle_mpm *s, double dt, uint32_t nsteps, int nthreads);
 
+/* ---- DEM: rigidbody2d circles with PenaltyImpactFrictionMap -------------- */
+
+enum { ORACLE_CONTACT_PLANE = 0, ORACLE_CONTACT_CIRCLE = 1, ORACLE_CONTACT_KINEMATIC = 2 };
+
+/* RigidBody2DState (rigidbody2d/RigidBody2DState.h:13-190) restricted to
+ * circles, static planes (RigidBody2DStaticPlane.h:12-45), gravity
+ * (NearEarthGravityForce.cpp:34-51) and the penalty map's parameters
+ * (PenaltyImpactFrictionMap.h:362-436).  The contact history (the map's
+ * CollisionCache) is owned by the oracle: oracle_dem_init / oracle_dem_free. */
+typedef struct oracle_dem {
+  uint32_t nbodies;
+  double *q;        /* 3B: x, y, theta */
+  double *v;        /* 3B */
+  double *mass;     /* 3B: diagonal of M (m, m, I) */
+  double *minv;     /* 3B: diagonal of M^-1 */
+  double *radius;   /* B */
+  uint8_t *fixed;   /* B: kinematically scripted */
+  double *weights;  /* B mass weights for flowWithWeights, NULL for flow */
+  double gravity[2];
+  uint32_t nplanes;
+  double *plane_x;  /* 2P */
+  double *plane_n;  /* 2P */
+  double kn, kt, gamman, gammat, mu;
+  void *hist;       /* contact history (opaque) */
+} oracle_dem;
+
+int oracle_dem_init(oracle_dem *s);
+void oracle_dem_free(oracle_dem *s);
+/* PenaltyImpactFrictionMap::flow (PenaltyImpactFrictionMap.cpp:1083-1115) or
+ * flowWithWeights (:1117-1149) when s->weights is set, nsteps times. */
+int oracle_dem_flow(oracle_dem *s, double dt, uint32_t nsteps);
+/* The current contact cache, in the reference's iteration order (type, then
+ * owner body, then insertion order).  Returns the number of contacts (may
+ * exceed max; only max are written). */
+uint32_t oracle_dem_contacts(const oracle_dem *s, int32_t *type, int32_t *i, int32_t *j,
+                             double *delta_s, uint32_t max);
+/* Circle pairs (i<j) the narrow phase accepts, by brute force over all pairs
+ * (RigidBody2DSim::dispatchNarrowPhaseCollision with q0 == q1,
+ * CollisionDetectionUtilities.cpp:8-13, :81-90).  Returns the count. */
+uint32_t oracle_dem_pairs_bruteforce(const oracle_dem *s, int32_t *i, int32_t *j, uint32_t max);
+/* Total force per body of the last step (gravity + contacts, fixed zeroed). */
+void oracle_dem_last_force(const oracle_dem *s, double *F);
+
 #ifdef __cplusplus
 }
 #endif
