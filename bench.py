@@ -55,6 +55,31 @@ def _time_mpm(scene, precision, steps, warmup, dt):
     return wall, kt
 
 
+def _time_dem(args, dt=1e-5):
+    """Circle DEM column (20k bodies, penalty contacts with friction): body-steps/s."""
+    from libhybridgrains_amd import HG_FP32, HG_FP64, GpuDem, scenes
+
+    s = scenes.dem_column()
+    out = {"bodies": s.nbodies}
+    for name, prec in (("fp64", HG_FP64), ("fp32", HG_FP32)):
+        g = GpuDem(s, prec)
+        g.flow(dt, max(args.warmup, 50), graph=True)  # settle into contact
+        g.sync()
+        t0 = time.perf_counter()
+        g.flow(dt, args.steps, graph=True)
+        g.sync()
+        wall = time.perf_counter() - t0
+        g.set_profiling(True)
+        g.flow(dt, args.steps)
+        g.sync()
+        kt = g.kernel_times()
+        g.close()
+        out[name] = {"body_steps_per_s": s.nbodies * args.steps / wall,
+                     "ms_per_step": 1e3 * wall / args.steps,
+                     "kernels_ms": {k: v[0] / v[1] for k, v in kt.items()}}
+    return out
+
+
 def _copy_gbs(_lib, nbytes=1 << 30, iters=20):
     """Measured streaming-copy bandwidth (read + write) of this GPU."""
     import ctypes as C
@@ -160,6 +185,7 @@ def main() -> int:
 
     f32, f64 = out_prec["fp32"], out_prec["fp64"]
     copy_gbs = _copy_gbs(_lib)
+    dem = _time_dem(args)
     line = {
         "metric": METRIC,
         "value": f32["value"],
@@ -189,6 +215,7 @@ def main() -> int:
         },
         "fp32_over_fp64": f32["value"] / f64["value"],
         "hbm_copy_gbs_measured": copy_gbs,
+        "dem_column_20k": dem,
         "kernels_ms_fp32": f32["kernels_ms"],
         "kernels_ms_fp64": f64["kernels_ms"],
     }

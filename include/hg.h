@@ -39,6 +39,7 @@ enum { HG_PLANE_SLIDING = 0, HG_PLANE_STICKING = 1 };
 
 typedef struct hg_ctx hg_ctx;
 typedef struct hg_mpm hg_mpm;
+typedef struct hg_dem hg_dem;
 
 /* ---- context -------------------------------------------------------------- */
 /* No reference counterpart: the reference is CPU-only.  Binds one GPU. */
@@ -145,9 +146,8 @@ hg_status hg_mpm_download_grid_scene(hg_mpm *h, uint32_t scene, double *mass, do
  * stream): 1 = a point's stencil left the grid, 2 = non-finite point state. */
 hg_status hg_mpm_scene_error(hg_mpm *h, uint32_t scene, int32_t *flags);
 
-/* P2G workgroup shape: threads per workgroup = aw*ah, each owning one stencil
- * base cell of a tile of (aw-S+1) x (ah-S+1) owned nodes (S = stencil width).
- * Supported: 16x16, 32x16, 32x8.  Chosen by the autotuner. */
+/* P2G tile: aw x ah grid nodes per workgroup, one thread per node.
+ * Supported: 16x16, 16x8, 32x8.  Chosen by the autotuner. */
 hg_status hg_mpm_set_tile(hg_mpm *h, int32_t aw, int32_t ah);
 
 /* ---- timing (no reference counterpart) --------------------------------------
@@ -163,6 +163,68 @@ hg_status hg_mpm_kernel_times(hg_mpm *h, int32_t max_kernels, char *names /* max
 hg_status hg_bench_copy(hg_ctx *ctx, uint64_t nbytes, int32_t iters, double *gbps);
 /* Captures one step into a hipGraph and replays it nsteps times. */
 hg_status hg_mpm_step_graph(hg_mpm *h, double dt, uint32_t nsteps);
+
+/* ---- DEM: rigidbody2d circles with PenaltyImpactFrictionMap ----------------- */
+/*
+ * Replaces the circle state RigidBody2DSceneParser builds
+ * (rigidbody2dutils/RigidBody2DSceneParser.cpp:715-902, :976-1125): bodies
+ * (RigidBody2DState.h:13-190 -- q, v, the diagonal of M, fixed flags), circle
+ * radii (CircleGeometry.h), near-earth gravity, static planes
+ * (RigidBody2DStaticPlane.h:12-45) and the penalty map's kn, kt, gamman,
+ * gammat, mu (PenaltyImpactFrictionMap.h:362-367).  The contact history the
+ * map keeps between steps (its CollisionCache) lives on the device.
+ */
+typedef struct hg_dem_desc {
+  int32_t precision;       /* HG_FP64, or HG_FP32 (FP64 positions, radii and pair tests) */
+  uint32_t nbodies;
+  const double *q;         /* 3B: x, y, theta */
+  const double *v;         /* 3B */
+  const double *mass;      /* 3B: diagonal of M (m, m, I) */
+  const double *radius;    /* B */
+  const uint8_t *fixed;    /* B or NULL: kinematically scripted bodies */
+  double gravity[2];
+  uint32_t nplanes;        /* <= 8 */
+  const double *plane_x;   /* 2P */
+  const double *plane_n;   /* 2P, unit */
+  double kn, kt, gamman, gammat, mu;
+  double cell_width;       /* broad-phase cell; at least 2 * max radius (0: that) */
+  double domain_min[2];    /* broad-phase box; bodies outside clamp to its rim */
+  double domain_max[2];    /* (max <= min: derived from the bodies and planes) */
+} hg_dem_desc;
+
+hg_status hg_dem_create(hg_ctx *ctx, const hg_dem_desc *desc, hg_dem **out);
+/* Several independent scenes stepped by the same launches (no reference
+ * counterpart); each evolves bit-identically to the scene alone. */
+hg_status hg_dem_create_batch(hg_ctx *ctx, const hg_dem_desc *descs, uint32_t nscenes,
+                              hg_dem **out);
+hg_status hg_dem_destroy(hg_dem *h);
+/* RigidBody2DSim::flow with the penalty map (RigidBody2DSim.cpp:1013-1039 ->
+ * PenaltyImpactFrictionMap::flow, PenaltyImpactFrictionMap.cpp:1083-1115),
+ * nsteps times; with weights set, flowWithWeight / flowWithWeights
+ * (RigidBody2DSim.cpp:1041-1069, PenaltyImpactFrictionMap.cpp:1117-1149). */
+hg_status hg_dem_step(hg_dem *h, double dt, uint32_t nsteps);
+hg_status hg_dem_step_graph(hg_dem *h, double dt, uint32_t nsteps);
+/* Mass weights of one scene's bodies (RigidBody2DState::massWeights), used as
+ * contact weights: plane w_i, pair (w_i + w_j)/2, kinematic w_i
+ * (PenaltyImpactFrictionMap.cpp:296-599).  NULL returns to unweighted flow. */
+hg_status hg_dem_set_weights(hg_dem *h, uint32_t scene, const double *weights);
+/* Waits; HG_ERR_STATE on non-finite bodies, HG_ERR_UNSUPPORTED when a body had
+ * more than 8 circle contacts (the step's results are then void). */
+hg_status hg_dem_sync(hg_dem *h);
+hg_status hg_dem_download_bodies(hg_dem *h, uint32_t scene, double *q, double *v);
+/* The contact cache after the last step in the reference's iteration order
+ * (planes, circle pairs, kinematic pairs; each by owner body, then partner):
+ * type 0 plane (j = plane index), 1 circle pair (i < j), 2 body i against
+ * fixed body j; delta_s is the tangential spring.  *count is the total; at
+ * most max entries are written. */
+hg_status hg_dem_download_contacts(hg_dem *h, uint32_t scene, uint32_t max, int32_t *type,
+                                   int32_t *i, int32_t *j, double *delta_s, uint32_t *count);
+/* Keep the total per-body force (3B) of each step for download. */
+hg_status hg_dem_keep_force(hg_dem *h, int32_t keep);
+hg_status hg_dem_download_force(hg_dem *h, uint32_t scene, double *F);
+hg_status hg_dem_set_profiling(hg_dem *h, int32_t on);
+hg_status hg_dem_kernel_times(hg_dem *h, int32_t max_kernels, char *names, double *ms,
+                              int64_t *launches, int32_t *nkernels);
 
 #ifdef __cplusplus
 }

@@ -17,5 +17,6 @@ from ._lib import (  # noqa: F401
     HgError,
 )
 from .mpm import GpuMpm, MpmScene  # noqa: F401
+from .dem import DemScene, GpuDem  # noqa: F401
 
 __version__ = "0.1.0"

@@ -68,6 +68,30 @@ class MpmDesc(C.Structure):
     ]
 
 
+class DemDesc(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int32),
+        ("nbodies", C.c_uint32),
+        ("q", _dp),
+        ("v", _dp),
+        ("mass", _dp),
+        ("radius", _dp),
+        ("fixed", _u8p),
+        ("gravity", C.c_double * 2),
+        ("nplanes", C.c_uint32),
+        ("plane_x", _dp),
+        ("plane_n", _dp),
+        ("kn", C.c_double),
+        ("kt", C.c_double),
+        ("gamman", C.c_double),
+        ("gammat", C.c_double),
+        ("mu", C.c_double),
+        ("cell_width", C.c_double),
+        ("domain_min", C.c_double * 2),
+        ("domain_max", C.c_double * 2),
+    ]
+
+
 def lib() -> C.CDLL:
     """Loads libhg.so; raises if it has not been built."""
     global _lib
@@ -105,6 +129,21 @@ def lib() -> C.CDLL:
         ),
         "hg_mpm_set_tile": ([vp, C.c_int32, C.c_int32], C.c_int32),
         "hg_bench_copy": ([vp, C.c_uint64, C.c_int32, _dp], C.c_int32),
+        "hg_dem_create": ([vp, C.POINTER(DemDesc), C.POINTER(vp)], C.c_int32),
+        "hg_dem_create_batch": ([vp, C.POINTER(DemDesc), C.c_uint32, C.POINTER(vp)], C.c_int32),
+        "hg_dem_destroy": ([vp], C.c_int32),
+        "hg_dem_step": ([vp, C.c_double, C.c_uint32], C.c_int32),
+        "hg_dem_step_graph": ([vp, C.c_double, C.c_uint32], C.c_int32),
+        "hg_dem_set_weights": ([vp, C.c_uint32, _dp], C.c_int32),
+        "hg_dem_sync": ([vp], C.c_int32),
+        "hg_dem_download_bodies": ([vp, C.c_uint32, _dp, _dp], C.c_int32),
+        "hg_dem_download_contacts": (
+            [vp, C.c_uint32, C.c_uint32, _ip, _ip, _ip, _dp, C.POINTER(C.c_uint32)], C.c_int32),
+        "hg_dem_keep_force": ([vp, C.c_int32], C.c_int32),
+        "hg_dem_download_force": ([vp, C.c_uint32, _dp], C.c_int32),
+        "hg_dem_set_profiling": ([vp, C.c_int32], C.c_int32),
+        "hg_dem_kernel_times": (
+            [vp, C.c_int32, C.c_char_p, _dp, C.POINTER(C.c_int64), _ip], C.c_int32),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)

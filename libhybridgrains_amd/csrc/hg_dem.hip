@@ -1,0 +1,1010 @@
+// hg_dem.hip -- circle DEM step with the penalty contact map, on gfx950.
+//
+// Replaces rigidbody2d PenaltyImpactFrictionMap::flow / flowWithWeights
+// (rigidbody2d/PenaltyImpactFrictionMap.cpp:1083-1149) together with the
+// active set RigidBody2DSim::computeActiveSet builds for circles and static
+// planes (RigidBody2DSim.cpp:270-413, :777-842, :886-912, :1140-1178).
+//
+// Layout in HBM (bodies keep their creation order; scenes are concatenated):
+//  * bodies: x, y, theta (FP64), vx, vy, omega (T), radius (FP64: the pair
+//    set is decided in FP64 in every precision), 1/m, 1/I, m (T), fixed, and
+//    optional contact weights (T).
+//  * broad phase: a uniform cell grid of width >= the largest diameter per
+//    scene, points counting-sorted by cell every step (hg_sort).
+//  * contacts: each contact is owned by one body -- the smaller index of a
+//    circle pair, or the simulated body of a pair with a fixed body, as the
+//    reference keys its CollisionTypeCache.  A body keeps up to kMaxOwned
+//    owned slots (partner key, tangential spring delta_s, force, torques),
+//    sorted by partner key, double-buffered so that the next step's
+//    contacts join their spring history by partner; per-plane slots likewise.
+//
+// Kernels of one step: dem_contacts (thread per body in cell order: narrow
+// phase over the 3x3 cells, history join, penalty force with the Coulomb
+// projection of the spring, plane contacts), dem_integrate (thread per body:
+// gathers its forces in the reference's summation order -- gravity, planes,
+// pairs owned by smaller partners, own pairs, kinematic pairs -- then
+// symplectic Euler, then the new cell key), and the counting sort.
+#include "hg_internal.h"
+#include "hg_sort.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+namespace hgdem {
+
+constexpr int kMaxPlanes = 8;
+constexpr int kMaxOwned = 8;
+constexpr int kMaxIn = 8;
+constexpr uint32_t kKinematic = 0x80000000u;
+constexpr int kThreads = 256;
+
+struct SceneDev {
+  double g0, g1;
+  double kn, kt, gamman, gammat, mu;
+  double dt;
+  double cmin0, cmin1, cw;  // broad-phase grid origin and cell width
+  int32_t ncx, ncy;
+  uint32_t b_off, b_cnt, cell_off;
+  int32_t nplanes;
+  int32_t weighted;
+  double plane_x[2 * kMaxPlanes];
+  double plane_n[2 * kMaxPlanes];
+};
+
+struct BodiesRaw {
+  double *x, *y, *th;
+  void *vx, *vy, *om;
+  double *r;
+  void *im, *iI, *m;
+  uint8_t *fixed;
+  void *w;
+};
+
+struct SlotsRaw {
+  uint32_t *okey;  // [B][kMaxOwned]
+  void *ods;       // [B][kMaxOwned][2]
+  uint8_t *ocnt;   // [B]
+  uint8_t *pact;   // [B][kMaxPlanes]
+  void *pds;       // [B][kMaxPlanes][2]
+};
+
+struct ForceRaw {
+  void *oF;     // [B][kMaxOwned][2]
+  void *otau0;  // [B][kMaxOwned]
+  void *otau1;  // [B][kMaxOwned]
+  void *pF;     // [B][kMaxPlanes][2]
+  void *ptau;   // [B][kMaxPlanes]
+  uint32_t *ikey;  // [B][kMaxIn] owners of contacts with this body
+  uint8_t *icnt;
+};
+
+template <typename T>
+__device__ __forceinline__ T ld(const void *p, uint32_t i) {
+  return static_cast<const T *>(p)[i];
+}
+template <typename T>
+__device__ __forceinline__ void st(void *p, uint32_t i, T v) {
+  static_cast<T *>(p)[i] = v;
+}
+
+__device__ __forceinline__ uint32_t scene_of(const uint32_t *b_off, uint32_t ns, uint32_t i) {
+  uint32_t lo = 0, hi = ns;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (b_off[mid] <= i) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ void cell_of(const SceneDev &sc, double x, double y, int &cx, int &cy) {
+  const double fx = floor((x - sc.cmin0) / sc.cw), fy = floor((y - sc.cmin1) / sc.cw);
+  // clamping keeps neighbouring bodies in neighbouring cells, so the 3x3
+  // search stays complete for bodies that leave the grid's box
+  cx = (fx >= 0.0) ? int(fmin(fx, double(sc.ncx - 1))) : 0;
+  cy = (fy >= 0.0) ? int(fmin(fy, double(sc.ncy - 1))) : 0;
+}
+
+// One contact's penalty force with the Coulomb projection of the tangential
+// spring: PenaltyImpactFrictionMap::updateAnchorsAndAccumulateCollisionForces
+// (PenaltyImpactFrictionMap.cpp:616-700; identical for every contact type),
+// with secondRootOfQuadratic (:601-614).  FP64.
+__device__ __forceinline__ void penalty_force(const SceneDev &sc, double w, double depth,
+                                              double n0, double n1, double vr0, double vr1,
+                                              double &ds0, double &ds1, double &f0, double &f1) {
+  const double vdn = n0 * vr0 + n1 * vr1;
+  const double vt0 = vr0 - vdn * n0, vt1 = vr1 - vdn * n1;
+  const double vn0 = vr0 - vt0, vn1 = vr1 - vt1;
+  const double cn = -sc.kn * w * depth;
+  const double dn = 0.5 * sc.gamman * w;
+  const double nf0 = cn * n0 - dn * vn0, nf1 = cn * n1 - dn * vn1;
+  const double ck = -sc.kt * w;
+  const double dt_ = 0.5 * sc.gammat * w;
+  double ff0 = ck * ds0 - dt_ * vt0, ff1 = ck * ds1 - dt_ * vt1;
+  const double mu_fn = sc.mu * sqrt(nf0 * nf0 + nf1 * nf1);
+  const double ft = sqrt(ff0 * ff0 + ff1 * ff1);
+  if (0.5 * sc.gammat * w * sqrt(vt0 * vt0 + vt1 * vt1) > mu_fn) {
+    ds0 = 0.0;
+    ds1 = 0.0;
+    const double g = -0.5 * sc.gammat * w;
+    ff0 = g * vt0;
+    ff1 = g * vt1;
+    const double z = ff0 * ff0 + ff1 * ff1;
+    if (z > 0.0) {
+      const double nz = sqrt(z);
+      ff0 = ff0 / nz;
+      ff1 = ff1 / nz;
+    }
+    ff0 *= mu_fn;
+    ff1 *= mu_fn;
+  } else if (ft > mu_fn) {
+    const double a = sc.kt * w * sc.kt * w * (ds0 * ds0 + ds1 * ds1);
+    const double b = sc.kt * w * sc.gammat * w * (ds0 * vt0 + ds1 * vt1);
+    const double c = 0.25 * sc.gammat * w * sc.gammat * w * (vt0 * vt0 + vt1 * vt1) -
+                     sc.mu * sc.mu * (nf0 * nf0 + nf1 * nf1);
+    const double dscr = b * b - 4 * a * c;
+    const double dsq = sqrt(dscr);
+    const double root1 = (b > 0.0) ? (2.0 * c) / (-b - dsq) : (-b + dsq) / (2.0 * a);
+    const double nd0 = root1 * ds0, nd1 = root1 * ds1;
+    ff0 = ck * nd0 - dt_ * vt0;
+    ff1 = ck * nd1 - dt_ * vt1;
+    ds0 = nd0;
+    ds1 = nd1;
+  }
+  f0 = nf0 + ff0;
+  f1 = nf1 + ff1;
+}
+
+// delta_s update of a new or continuing contact (PenaltyImpactFrictionMap.cpp:61-80)
+__device__ __forceinline__ void spring(bool had, double old0, double old1, double dt, double vr0,
+                                       double vr1, double n0, double n1, double &ds0,
+                                       double &ds1) {
+  if (!had) {
+    ds0 = dt * vr0;
+    ds1 = dt * vr1;
+  } else {
+    ds0 = old0;
+    ds1 = old1;
+    ds0 += dt * vr0;
+    ds1 += dt * vr1;
+  }
+  const double d = n0 * ds0 + n1 * ds1;
+  ds0 -= d * n0;
+  ds1 -= d * n1;
+}
+
+struct ContactArgs {
+  const SceneDev *scenes;
+  const uint32_t *b_off;
+  uint32_t nscenes, nbodies, nblocks_padded;
+  BodiesRaw b;
+  SlotsRaw old_s, new_s;
+  ForceRaw f;
+  const uint32_t *perm, *cell_start;
+  int32_t *err;
+};
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void dem_contacts_kernel(ContactArgs a) {
+  __shared__ uint32_t s_own[kThreads][kMaxOwned + 1];
+  __shared__ uint32_t s_in[kThreads][kMaxIn + 1];
+  const uint32_t blk = hg_xcd_remap(blockIdx.x, a.nblocks_padded);
+  const uint32_t s = blk * kThreads + threadIdx.x;
+  if (s >= a.nbodies) return;
+  const uint32_t i = a.perm[s];
+  const uint32_t sid = scene_of(a.b_off, a.nscenes, i);
+  const SceneDev &sc = a.scenes[sid];
+  const double xi = a.b.x[i], yi = a.b.y[i], ri = a.b.r[i];
+  const bool fi = a.b.fixed[i] != 0;
+  uint32_t *own = s_own[threadIdx.x];
+  uint32_t *inl = s_in[threadIdx.x];
+  int nown = 0, nin = 0;
+  bool overflow = false;
+
+  int cx, cy;
+  cell_of(sc, xi, yi, cx, cy);
+  const uint32_t *cs = a.cell_start + sc.cell_off;
+  for (int yy = max(cy - 1, 0); yy <= min(cy + 1, sc.ncy - 1); ++yy) {
+    const uint32_t k0 = uint32_t(yy) * uint32_t(sc.ncx) + uint32_t(max(cx - 1, 0));
+    const uint32_t k1 = uint32_t(yy) * uint32_t(sc.ncx) + uint32_t(min(cx + 1, sc.ncx - 1)) + 1u;
+    for (uint32_t t = cs[k0]; t < cs[k1]; ++t) {
+      const uint32_t j = a.perm[t];
+      if (j == i) continue;
+      const bool fj = a.b.fixed[j] != 0;
+      if (fi && fj) continue;
+      const double dx = xi - a.b.x[j], dy = yi - a.b.y[j];
+      const double rs = ri + a.b.r[j];
+      const double c0 = (dx * dx + dy * dy) - rs * rs;
+      if (!(c0 <= 0.0)) continue;
+      if (!fi && (fj || i < j)) {
+        const uint32_t key = (fj ? kKinematic : 0u) | (j - sc.b_off);
+        if (nown < kMaxOwned) {
+          // insertion into the sorted list
+          int q = nown++;
+          while (q > 0 && own[q - 1] > key) {
+            own[q] = own[q - 1];
+            --q;
+          }
+          own[q] = key;
+        } else {
+          overflow = true;
+        }
+      } else if (!fi && !fj) {
+        const uint32_t key = j - sc.b_off;
+        if (nin < kMaxIn) {
+          int q = nin++;
+          while (q > 0 && inl[q - 1] > key) {
+            inl[q] = inl[q - 1];
+            --q;
+          }
+          inl[q] = key;
+        } else {
+          overflow = true;
+        }
+      }
+    }
+  }
+  if (overflow) atomicOr(&a.err[sid], 4);
+
+  const double dt = sc.dt;
+  const double vxi = double(ld<T>(a.b.vx, i)), vyi = double(ld<T>(a.b.vy, i));
+  const double omi = double(ld<T>(a.b.om, i));
+  const double wi = sc.weighted ? double(ld<T>(a.b.w, i)) : 1.0;
+  const uint32_t obase = i * kMaxOwned;
+  const int old_n = a.old_s.ocnt[i];
+  for (int k = 0; k < nown; ++k) {
+    const uint32_t key = own[k];
+    const bool kin = (key & kKinematic) != 0;
+    const uint32_t j = (key & ~kKinematic) + sc.b_off;
+    const double xj = a.b.x[j], yj = a.b.y[j], rj = a.b.r[j];
+    // n = normalized(q_i - q_j) (RigidBody2DSim.cpp:320)
+    const double d0 = xi - xj, d1 = yi - yj;
+    const double dz = d0 * d0 + d1 * d1;
+    double n0 = d0, n1 = d1;
+    if (dz > 0.0) {
+      const double nz = sqrt(dz);
+      n0 = d0 / nz;
+      n1 = d1 / nz;
+    }
+    const double gap = sqrt(d0 * d0 + d1 * d1) - ri - rj;
+    const double depth = gap < 0.0 ? gap : 0.0;
+    const double vxj = double(ld<T>(a.b.vx, j)), vyj = double(ld<T>(a.b.vy, j));
+    const double omj = double(ld<T>(a.b.om, j));
+    double vr0, vr1, r00, r01, r10 = 0.0, r11 = 0.0, w;
+    if (!kin) {
+      // CircleCircleConstraint (CircleCircleConstraint.cpp:171-193, :224-233)
+      const double f = ri / (ri + rj);
+      const double p0 = xi + f * (xj - xi), p1 = yi + f * (yj - yi);
+      r00 = p0 - xi;
+      r01 = p1 - yi;
+      r10 = p0 - xj;
+      r11 = p1 - yj;
+      vr0 = vxi + omi * (-r01) - vxj - omj * (-r11);
+      vr1 = vyi + omi * r00 - vyj - omj * r10;
+      w = sc.weighted ? (wi + double(ld<T>(a.b.w, j))) * 0.5 : 1.0;
+    } else {
+      // KinematicCircleCircleConstraint (KinematicObjectCircleConstraint.cpp:121-199)
+      r00 = -ri * n0;
+      r01 = -ri * n1;
+      const double cp0 = xi - ri * n0, cp1 = yi - ri * n1;
+      const double arm0 = cp0 - xj, arm1 = cp1 - yj;
+      const double kv0 = vxj + omj * (-arm1), kv1 = vyj + omj * arm0;
+      vr0 = vxi + omi * (-r01) - kv0;
+      vr1 = vyi + omi * r00 - kv1;
+      w = wi;
+    }
+    // spring history of this pair
+    bool had = false;
+    double o0 = 0.0, o1 = 0.0;
+    for (int q = 0; q < old_n; ++q)
+      if (a.old_s.okey[obase + q] == key) {
+        had = true;
+        o0 = double(ld<T>(a.old_s.ods, 2 * (obase + q)));
+        o1 = double(ld<T>(a.old_s.ods, 2 * (obase + q) + 1));
+        break;
+      }
+    double ds0, ds1, F0, F1;
+    spring(had, o0, o1, dt, vr0, vr1, n0, n1, ds0, ds1);
+    penalty_force(sc, w, depth, n0, n1, vr0, vr1, ds0, ds1, F0, F1);
+    const uint32_t sl = obase + uint32_t(k);
+    a.new_s.okey[sl] = key;
+    st<T>(a.new_s.ods, 2 * sl, T(ds0));
+    st<T>(a.new_s.ods, 2 * sl + 1, T(ds1));
+    st<T>(a.f.oF, 2 * sl, T(F0));
+    st<T>(a.f.oF, 2 * sl + 1, T(F1));
+    st<T>(a.f.otau0, sl, T(r00 * F1 - r01 * F0));
+    st<T>(a.f.otau1, sl, T(r10 * F1 - r11 * F0));
+  }
+  a.new_s.ocnt[i] = uint8_t(nown);
+  for (int k = 0; k < nin; ++k) a.f.ikey[i * kMaxIn + uint32_t(k)] = inl[k];
+  a.f.icnt[i] = uint8_t(nin);
+
+  // static planes (RigidBody2DSim.cpp:777-842; StaticPlaneCircleConstraint.cpp)
+  for (int k = 0; k < sc.nplanes; ++k) {
+    const uint32_t ps = i * kMaxPlanes + uint32_t(k);
+    const double pn0 = sc.plane_n[2 * k], pn1 = sc.plane_n[2 * k + 1];
+    const double rel0 = xi - sc.plane_x[2 * k], rel1 = yi - sc.plane_x[2 * k + 1];
+    const double dist = pn0 * rel0 + pn1 * rel1;
+    if (fi || !(dist <= ri)) {
+      a.new_s.pact[ps] = 0;
+      continue;
+    }
+    const double gap = dist - ri;
+    const double depth = gap < 0.0 ? gap : 0.0;
+    const double r00 = -ri * pn0, r01 = -ri * pn1;
+    const double vr0 = vxi + omi * (-r01) - 0.0;
+    const double vr1 = vyi + omi * r00 - 0.0;
+    const bool had = a.old_s.pact[ps] != 0;
+    const double o0 = had ? double(ld<T>(a.old_s.pds, 2 * ps)) : 0.0;
+    const double o1 = had ? double(ld<T>(a.old_s.pds, 2 * ps + 1)) : 0.0;
+    double ds0, ds1, F0, F1;
+    spring(had, o0, o1, dt, vr0, vr1, pn0, pn1, ds0, ds1);
+    penalty_force(sc, wi, depth, pn0, pn1, vr0, vr1, ds0, ds1, F0, F1);
+    a.new_s.pact[ps] = 1;
+    st<T>(a.new_s.pds, 2 * ps, T(ds0));
+    st<T>(a.new_s.pds, 2 * ps + 1, T(ds1));
+    st<T>(a.f.pF, 2 * ps, T(F0));
+    st<T>(a.f.pF, 2 * ps + 1, T(F1));
+    st<T>(a.f.ptau, ps, T(r00 * F1 - r01 * F0));
+  }
+}
+
+struct IntegrateArgs {
+  const SceneDev *scenes;
+  const uint32_t *b_off;
+  uint32_t nscenes, nbodies;
+  BodiesRaw b;
+  SlotsRaw s;  // the slots this step wrote
+  ForceRaw f;
+  uint32_t *key, *slot, *counts;
+  int32_t *err;
+  double *F_out;  // optional: total force per body (3B)
+};
+
+// Gathers a body's forces in the reference's summation order, then
+// v1 = v0 + dt M^-1 F, q1 = q0 + dt v1 (PenaltyImpactFrictionMap.cpp:1107-1114).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void dem_integrate_kernel(IntegrateArgs a) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= a.nbodies) return;
+  const uint32_t sid = scene_of(a.b_off, a.nscenes, i);
+  const SceneDev &sc = a.scenes[sid];
+  const double m = double(ld<T>(a.b.m, i));
+  double F0 = 0.0, F1 = 0.0, F2 = 0.0;
+  F0 += m * sc.g0;
+  F1 += m * sc.g1;
+  for (int k = 0; k < sc.nplanes; ++k) {
+    const uint32_t ps = i * kMaxPlanes + uint32_t(k);
+    if (!a.s.pact[ps]) continue;
+    F0 += double(ld<T>(a.f.pF, 2 * ps));
+    F1 += double(ld<T>(a.f.pF, 2 * ps + 1));
+    F2 += double(ld<T>(a.f.ptau, ps));
+  }
+  // pairs owned by a smaller partner, in partner order
+  const uint32_t lid = i - sc.b_off;
+  const int nin = a.f.icnt[i];
+  for (int k = 0; k < nin; ++k) {
+    const uint32_t j = a.f.ikey[i * kMaxIn + uint32_t(k)] + sc.b_off;
+    const int nj = a.s.ocnt[j];
+    for (int q = 0; q < nj; ++q) {
+      const uint32_t sl = j * kMaxOwned + uint32_t(q);
+      if (a.s.okey[sl] == lid) {
+        F0 -= double(ld<T>(a.f.oF, 2 * sl));
+        F1 -= double(ld<T>(a.f.oF, 2 * sl + 1));
+        F2 -= double(ld<T>(a.f.otau1, sl));
+        break;
+      }
+    }
+  }
+  // own pairs, then kinematic pairs (the key order)
+  const int nown = a.s.ocnt[i];
+  for (int q = 0; q < nown; ++q) {
+    const uint32_t sl = i * kMaxOwned + uint32_t(q);
+    F0 += double(ld<T>(a.f.oF, 2 * sl));
+    F1 += double(ld<T>(a.f.oF, 2 * sl + 1));
+    F2 += double(ld<T>(a.f.otau0, sl));
+  }
+  if (a.b.fixed[i]) F0 = F1 = F2 = 0.0;
+  if (a.F_out) {
+    a.F_out[3 * i] = F0;
+    a.F_out[3 * i + 1] = F1;
+    a.F_out[3 * i + 2] = F2;
+  }
+  const double dt = sc.dt;
+  const double im = double(ld<T>(a.b.im, i)), iI = double(ld<T>(a.b.iI, i));
+  const double v0 = double(ld<T>(a.b.vx, i)) + (dt * im) * F0;
+  const double v1 = double(ld<T>(a.b.vy, i)) + (dt * im) * F1;
+  const double v2 = double(ld<T>(a.b.om, i)) + (dt * iI) * F2;
+  const T tv0 = T(v0), tv1 = T(v1), tv2 = T(v2);
+  st<T>(a.b.vx, i, tv0);
+  st<T>(a.b.vy, i, tv1);
+  st<T>(a.b.om, i, tv2);
+  const double x = a.b.x[i] + dt * double(tv0);
+  const double y = a.b.y[i] + dt * double(tv1);
+  a.b.x[i] = x;
+  a.b.y[i] = y;
+  a.b.th[i] = a.b.th[i] + dt * double(tv2);
+  if (!(isfinite(x) && isfinite(y) && isfinite(v2))) atomicOr(&a.err[sid], 2);
+  int cx, cy;
+  cell_of(sc, x, y, cx, cy);
+  const uint32_t k = sc.cell_off + uint32_t(cy) * uint32_t(sc.ncx) + uint32_t(cx);
+  a.key[i] = k;
+  a.slot[i] = atomicAdd(&a.counts[k], 1u);
+}
+
+struct BinArgs {
+  const SceneDev *scenes;
+  const uint32_t *b_off;
+  uint32_t nscenes, nbodies;
+  const double *x, *y;
+  uint32_t *key, *slot, *counts;
+};
+
+__global__ __launch_bounds__(kThreads) void dem_bin_kernel(BinArgs a) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= a.nbodies) return;
+  const SceneDev &sc = a.scenes[scene_of(a.b_off, a.nscenes, i)];
+  int cx, cy;
+  cell_of(sc, a.x[i], a.y[i], cx, cy);
+  const uint32_t k = sc.cell_off + uint32_t(cy) * uint32_t(sc.ncx) + uint32_t(cx);
+  a.key[i] = k;
+  a.slot[i] = atomicAdd(&a.counts[k], 1u);
+}
+
+}  // namespace hgdem
+
+using namespace hgdem;
+
+// ===========================================================================
+// host side
+// ===========================================================================
+
+struct hg_dem {
+  hg_ctx *ctx = nullptr;
+  hipStream_t stream = nullptr;
+  int precision = HG_FP64;
+  std::vector<SceneDev> scenes;
+  uint32_t nscenes = 0, nbodies = 0, ncell = 0, nscan = 0;
+  SceneDev *d_scenes = nullptr;
+  uint32_t *d_b_off = nullptr;
+  BodiesRaw b = {};
+  SlotsRaw slots[2] = {};
+  int cur = 0;  // slots[cur] holds the last step's contacts
+  ForceRaw f = {};
+  uint32_t *perm = nullptr, *key = nullptr, *slot = nullptr, *counts = nullptr,
+           *cell_start = nullptr, *block_sums = nullptr;
+  int32_t *err = nullptr;
+  double *F_out = nullptr;
+  bool keep_force = false;
+  double dt_dev = -1.0;
+  KernelProfiler prof;
+  hipGraphExec_t graph = nullptr;
+  std::vector<void *> allocs;
+  size_t tsize() const { return precision == HG_FP64 ? 8 : 4; }
+};
+
+template <typename P>
+static hg_status dalloc(hg_dem *h, P **out, size_t bytes) {
+  void *p = nullptr;
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess)
+    return hg_fail(h->ctx, HG_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  h->allocs.push_back(p);
+  *out = static_cast<P *>(p);
+  return HG_OK;
+}
+
+#define HG_TRY(expr)              \
+  do {                            \
+    hg_status s_ = (expr);        \
+    if (s_ != HG_OK) return s_;   \
+  } while (0)
+
+template <typename T>
+static void launch_contacts_t(hg_dem *h) {
+  ContactArgs a;
+  a.scenes = h->d_scenes;
+  a.b_off = h->d_b_off;
+  a.nscenes = h->nscenes;
+  a.nbodies = h->nbodies;
+  a.nblocks_padded = hg_pad8((h->nbodies + kThreads - 1) / kThreads);
+  a.b = h->b;
+  a.old_s = h->slots[h->cur];
+  a.new_s = h->slots[1 - h->cur];
+  a.f = h->f;
+  a.perm = h->perm;
+  a.cell_start = h->cell_start;
+  a.err = h->err;
+  ProfScope ps(h->prof, "dem_contacts", h->stream);
+  hipLaunchKernelGGL(dem_contacts_kernel<T>, dim3(a.nblocks_padded), dim3(kThreads), 0, h->stream, a);
+}
+
+template <typename T>
+static void launch_integrate_t(hg_dem *h) {
+  IntegrateArgs a;
+  a.scenes = h->d_scenes;
+  a.b_off = h->d_b_off;
+  a.nscenes = h->nscenes;
+  a.nbodies = h->nbodies;
+  a.b = h->b;
+  a.s = h->slots[1 - h->cur];
+  a.f = h->f;
+  a.key = h->key;
+  a.slot = h->slot;
+  a.counts = h->counts;
+  a.err = h->err;
+  a.F_out = h->keep_force ? h->F_out : nullptr;
+  ProfScope ps(h->prof, "dem_integrate", h->stream);
+  hipLaunchKernelGGL(dem_integrate_kernel<T>, dim3((h->nbodies + kThreads - 1) / kThreads),
+                     dim3(kThreads), 0, h->stream, a);
+}
+
+static void launch_step(hg_dem *h) {
+  if (h->nbodies) {
+    if (h->precision == HG_FP64) {
+      launch_contacts_t<double>(h);
+      launch_integrate_t<double>(h);
+    } else {
+      launch_contacts_t<float>(h);
+      launch_integrate_t<float>(h);
+    }
+  }
+  h->cur = 1 - h->cur;
+  hg_counting_sort(h->prof, h->stream, h->counts, h->ncell, h->block_sums, h->nscan,
+                   h->cell_start, h->key, h->slot, h->nbodies, h->perm);
+}
+
+static hg_status dem_upload_dt(hg_dem *h, double dt) {
+  if (dt == h->dt_dev) return HG_OK;
+  HG_HIP(h->ctx, hipStreamSynchronize(h->stream));
+  for (SceneDev &s : h->scenes) s.dt = dt;
+  HG_HIP(h->ctx, hipMemcpy(h->d_scenes, h->scenes.data(), h->scenes.size() * sizeof(SceneDev),
+                           hipMemcpyHostToDevice));
+  h->dt_dev = dt;
+  if (h->graph) {
+    hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  return HG_OK;
+}
+
+template <typename T>
+static hg_status upload_bodies(hg_dem *h, const hg_dem_desc *descs) {
+  const uint32_t B = h->nbodies;
+  std::vector<double> x, y, th, r;
+  std::vector<T> vx, vy, om, im, iI, m;
+  std::vector<uint8_t> fixed;
+  for (uint32_t s = 0; s < h->nscenes; ++s) {
+    const hg_dem_desc &d = descs[s];
+    for (uint32_t k = 0; k < d.nbodies; ++k) {
+      x.push_back(d.q[3 * k]);
+      y.push_back(d.q[3 * k + 1]);
+      th.push_back(d.q[3 * k + 2]);
+      vx.push_back(T(d.v[3 * k]));
+      vy.push_back(T(d.v[3 * k + 1]));
+      om.push_back(T(d.v[3 * k + 2]));
+      r.push_back(d.radius[k]);
+      m.push_back(T(d.mass[3 * k]));
+      im.push_back(T(1.0 / d.mass[3 * k]));
+      iI.push_back(T(1.0 / d.mass[3 * k + 2]));
+      fixed.push_back(d.fixed ? d.fixed[k] : 0);
+    }
+  }
+  BodiesRaw &b = h->b;
+  HG_TRY(dalloc(h, &b.x, B * 8));
+  HG_TRY(dalloc(h, &b.y, B * 8));
+  HG_TRY(dalloc(h, &b.th, B * 8));
+  HG_TRY(dalloc(h, &b.r, B * 8));
+  HG_TRY(dalloc(h, &b.vx, B * sizeof(T)));
+  HG_TRY(dalloc(h, &b.vy, B * sizeof(T)));
+  HG_TRY(dalloc(h, &b.om, B * sizeof(T)));
+  HG_TRY(dalloc(h, &b.im, B * sizeof(T)));
+  HG_TRY(dalloc(h, &b.iI, B * sizeof(T)));
+  HG_TRY(dalloc(h, &b.m, B * sizeof(T)));
+  HG_TRY(dalloc(h, &b.w, B * sizeof(T)));
+  HG_TRY(dalloc(h, &b.fixed, B));
+  if (B) {
+    HG_HIP(h->ctx, hipMemcpy(b.x, x.data(), B * 8, hipMemcpyHostToDevice));
+    HG_HIP(h->ctx, hipMemcpy(b.y, y.data(), B * 8, hipMemcpyHostToDevice));
+    HG_HIP(h->ctx, hipMemcpy(b.th, th.data(), B * 8, hipMemcpyHostToDevice));
+    HG_HIP(h->ctx, hipMemcpy(b.r, r.data(), B * 8, hipMemcpyHostToDevice));
+    HG_HIP(h->ctx, hipMemcpy(b.vx, vx.data(), B * sizeof(T), hipMemcpyHostToDevice));
+    HG_HIP(h->ctx, hipMemcpy(b.vy, vy.data(), B * sizeof(T), hipMemcpyHostToDevice));
+    HG_HIP(h->ctx, hipMemcpy(b.om, om.data(), B * sizeof(T), hipMemcpyHostToDevice));
+    HG_HIP(h->ctx, hipMemcpy(b.im, im.data(), B * sizeof(T), hipMemcpyHostToDevice));
+    HG_HIP(h->ctx, hipMemcpy(b.iI, iI.data(), B * sizeof(T), hipMemcpyHostToDevice));
+    HG_HIP(h->ctx, hipMemcpy(b.m, m.data(), B * sizeof(T), hipMemcpyHostToDevice));
+    HG_HIP(h->ctx, hipMemcpy(b.fixed, fixed.data(), B, hipMemcpyHostToDevice));
+    std::vector<T> ones(B, T(1));
+    HG_HIP(h->ctx, hipMemcpy(b.w, ones.data(), B * sizeof(T), hipMemcpyHostToDevice));
+  }
+  for (int k = 0; k < 2; ++k) {
+    SlotsRaw &s = h->slots[k];
+    HG_TRY(dalloc(h, &s.okey, size_t(B) * kMaxOwned * 4));
+    HG_TRY(dalloc(h, &s.ods, size_t(B) * kMaxOwned * 2 * sizeof(T)));
+    HG_TRY(dalloc(h, &s.ocnt, B));
+    HG_TRY(dalloc(h, &s.pact, size_t(B) * kMaxPlanes));
+    HG_TRY(dalloc(h, &s.pds, size_t(B) * kMaxPlanes * 2 * sizeof(T)));
+    HG_HIP(h->ctx, hipMemset(s.ocnt, 0, B ? B : 1));
+    HG_HIP(h->ctx, hipMemset(s.pact, 0, size_t(B ? B : 1) * kMaxPlanes));
+  }
+  ForceRaw &f = h->f;
+  HG_TRY(dalloc(h, &f.oF, size_t(B) * kMaxOwned * 2 * sizeof(T)));
+  HG_TRY(dalloc(h, &f.otau0, size_t(B) * kMaxOwned * sizeof(T)));
+  HG_TRY(dalloc(h, &f.otau1, size_t(B) * kMaxOwned * sizeof(T)));
+  HG_TRY(dalloc(h, &f.pF, size_t(B) * kMaxPlanes * 2 * sizeof(T)));
+  HG_TRY(dalloc(h, &f.ptau, size_t(B) * kMaxPlanes * sizeof(T)));
+  HG_TRY(dalloc(h, &f.ikey, size_t(B) * kMaxIn * 4));
+  HG_TRY(dalloc(h, &f.icnt, B));
+  return HG_OK;
+}
+
+static hg_status dem_create_impl(hg_ctx *ctx, const hg_dem_desc *descs, uint32_t nscenes,
+                                 hg_dem **out) {
+  if (!ctx || !descs || !out || nscenes == 0) return hg_fail(ctx, HG_ERR_INVALID, "null argument");
+  for (uint32_t s = 0; s < nscenes; ++s) {
+    const hg_dem_desc &d = descs[s];
+    if (d.precision != HG_FP64 && d.precision != HG_FP32)
+      return hg_fail(ctx, HG_ERR_INVALID, "precision must be HG_FP64 or HG_FP32");
+    if (d.precision != descs[0].precision)
+      return hg_fail(ctx, HG_ERR_INVALID, "all scenes of a batch share the precision");
+    if (d.nplanes > uint32_t(kMaxPlanes))
+      return hg_fail(ctx, HG_ERR_UNSUPPORTED, "at most 8 static planes per scene");
+    if (d.nbodies && (!d.q || !d.v || !d.mass || !d.radius))
+      return hg_fail(ctx, HG_ERR_INVALID, "body arrays missing");
+    if (d.nplanes && (!d.plane_x || !d.plane_n))
+      return hg_fail(ctx, HG_ERR_INVALID, "plane arrays missing");
+    for (uint32_t k = 0; k < d.nbodies; ++k)
+      if (!(d.radius[k] > 0.0) || !(d.mass[3 * k] > 0.0) || !(d.mass[3 * k + 2] > 0.0))
+        return hg_fail(ctx, HG_ERR_INVALID, "radius, mass and inertia must be > 0");
+  }
+  HG_HIP(ctx, hipSetDevice(ctx->device));
+  std::unique_ptr<hg_dem> h(new hg_dem());
+  h->ctx = ctx;
+  h->precision = descs[0].precision;
+  h->nscenes = nscenes;
+  HG_HIP(ctx, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  uint64_t b_off = 0, cell_off = 0;
+  std::vector<uint32_t> h_b_off;
+  for (uint32_t s = 0; s < nscenes; ++s) {
+    const hg_dem_desc &d = descs[s];
+    SceneDev sd;
+    std::memset(&sd, 0, sizeof(sd));
+    sd.g0 = d.gravity[0];
+    sd.g1 = d.gravity[1];
+    sd.kn = d.kn;
+    sd.kt = d.kt;
+    sd.gamman = d.gamman;
+    sd.gammat = d.gammat;
+    sd.mu = d.mu;
+    double rmax = 0.0, x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
+    for (uint32_t k = 0; k < d.nbodies; ++k) {
+      rmax = std::max(rmax, d.radius[k]);
+      x0 = std::min(x0, d.q[3 * k]);
+      x1 = std::max(x1, d.q[3 * k]);
+      y0 = std::min(y0, d.q[3 * k + 1]);
+      y1 = std::max(y1, d.q[3 * k + 1]);
+    }
+    if (d.nbodies == 0) x0 = y0 = x1 = y1 = 0.0, rmax = 1.0;
+    const double cw = std::max(d.cell_width, 2.0 * rmax * (1.0 + 1e-9));
+    double lo0, lo1, hi0, hi1;
+    if (d.domain_max[0] > d.domain_min[0] && d.domain_max[1] > d.domain_min[1]) {
+      lo0 = d.domain_min[0];
+      lo1 = d.domain_min[1];
+      hi0 = d.domain_max[0];
+      hi1 = d.domain_max[1];
+    } else {
+      // room for the scene to spread: the bodies' box grown by twice its
+      // larger side, then clipped by axis-aligned planes the bodies sit inside
+      const double e = 2.0 * std::max(std::max(x1 - x0, y1 - y0), 8.0 * cw);
+      lo0 = x0 - e;
+      lo1 = y0 - e;
+      hi0 = x1 + e;
+      hi1 = y1 + e;
+      for (uint32_t k = 0; k < d.nplanes; ++k) {
+        const double n0 = d.plane_n[2 * k], n1 = d.plane_n[2 * k + 1];
+        const double px = d.plane_x[2 * k], py = d.plane_x[2 * k + 1];
+        if (n1 == 0.0 && n0 > 0.0 && px <= x0) lo0 = std::max(lo0, px - 2.0 * cw);
+        if (n1 == 0.0 && n0 < 0.0 && px >= x1) hi0 = std::min(hi0, px + 2.0 * cw);
+        if (n0 == 0.0 && n1 > 0.0 && py <= y0) lo1 = std::max(lo1, py - 2.0 * cw);
+        if (n0 == 0.0 && n1 < 0.0 && py >= y1) hi1 = std::min(hi1, py + 2.0 * cw);
+      }
+    }
+    // bodies that leave this box clamp to its rim cells (correct, slower)
+    sd.cmin0 = lo0;
+    sd.cmin1 = lo1;
+    sd.cw = cw;
+    sd.ncx = int32_t(std::min(std::max(std::ceil((hi0 - lo0) / cw), 1.0) + 1.0, 16384.0));
+    sd.ncy = int32_t(std::min(std::max(std::ceil((hi1 - lo1) / cw), 1.0) + 1.0, 16384.0));
+    sd.b_off = uint32_t(b_off);
+    sd.b_cnt = d.nbodies;
+    sd.cell_off = uint32_t(cell_off);
+    sd.nplanes = int32_t(d.nplanes);
+    for (uint32_t k = 0; k < d.nplanes; ++k) {
+      const double n0 = d.plane_n[2 * k], n1 = d.plane_n[2 * k + 1];
+      const double nn = std::sqrt(n0 * n0 + n1 * n1);
+      if (!(std::fabs(nn - 1.0) <= 1e-6)) return hg_fail(ctx, HG_ERR_INVALID, "plane normals must be unit");
+      sd.plane_x[2 * k] = d.plane_x[2 * k];
+      sd.plane_x[2 * k + 1] = d.plane_x[2 * k + 1];
+      sd.plane_n[2 * k] = n0;
+      sd.plane_n[2 * k + 1] = n1;
+    }
+    h_b_off.push_back(uint32_t(b_off));
+    b_off += d.nbodies;
+    cell_off += uint64_t(sd.ncx) * uint64_t(sd.ncy);
+    h->scenes.push_back(sd);
+  }
+  if (b_off >= (1ull << 31) || cell_off >= (1ull << 31))
+    return hg_fail(ctx, HG_ERR_UNSUPPORTED, "batch too large for 32-bit indexing");
+  h_b_off.push_back(uint32_t(b_off));
+  h->nbodies = uint32_t(b_off);
+  h->ncell = uint32_t(cell_off);
+  h->nscan = hg_scan_blocks(h->ncell);
+  if (h->nscan > uint32_t(kScanChunk)) return hg_fail(ctx, HG_ERR_UNSUPPORTED, "too many cells");
+  hg_dem *hp = h.get();
+  HG_TRY(dalloc(hp, &h->d_scenes, nscenes * sizeof(SceneDev)));
+  HG_TRY(dalloc(hp, &h->d_b_off, (nscenes + 1) * 4));
+  HG_TRY(dalloc(hp, &h->perm, size_t(h->nbodies) * 4));
+  HG_TRY(dalloc(hp, &h->key, size_t(h->nbodies) * 4));
+  HG_TRY(dalloc(hp, &h->slot, size_t(h->nbodies) * 4));
+  HG_TRY(dalloc(hp, &h->counts, size_t(h->ncell) * 4));
+  HG_TRY(dalloc(hp, &h->cell_start, size_t(h->ncell + 1) * 4));
+  HG_TRY(dalloc(hp, &h->block_sums, size_t(h->nscan) * 4));
+  HG_TRY(dalloc(hp, &h->err, nscenes * 4));
+  HG_TRY(dalloc(hp, &h->F_out, size_t(h->nbodies) * 24));
+  HG_HIP(ctx, hipMemcpy(h->d_scenes, h->scenes.data(), nscenes * sizeof(SceneDev), hipMemcpyHostToDevice));
+  HG_HIP(ctx, hipMemcpy(h->d_b_off, h_b_off.data(), (nscenes + 1) * 4, hipMemcpyHostToDevice));
+  HG_HIP(ctx, hipMemset(h->counts, 0, size_t(h->ncell) * 4));
+  HG_HIP(ctx, hipMemset(h->err, 0, nscenes * 4));
+  HG_HIP(ctx, hipMemset(h->F_out, 0, size_t(h->nbodies ? h->nbodies : 1) * 24));
+  if (h->precision == HG_FP64) HG_TRY(upload_bodies<double>(hp, descs));
+  else HG_TRY(upload_bodies<float>(hp, descs));
+  HG_HIP(ctx, hipDeviceSynchronize());
+  if (h->nbodies) {
+    BinArgs a;
+    a.scenes = h->d_scenes;
+    a.b_off = h->d_b_off;
+    a.nscenes = nscenes;
+    a.nbodies = h->nbodies;
+    a.x = h->b.x;
+    a.y = h->b.y;
+    a.key = h->key;
+    a.slot = h->slot;
+    a.counts = h->counts;
+    hipLaunchKernelGGL(dem_bin_kernel, dim3((h->nbodies + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                       h->stream, a);
+  }
+  hg_counting_sort(h->prof, h->stream, h->counts, h->ncell, h->block_sums, h->nscan, h->cell_start,
+                   h->key, h->slot, h->nbodies, h->perm);
+  HG_HIP(ctx, hipGetLastError());
+  HG_HIP(ctx, hipStreamSynchronize(h->stream));
+  *out = h.release();
+  return HG_OK;
+}
+
+static hg_status dem_check(hg_dem *h) {
+  HG_HIP(h->ctx, hipStreamSynchronize(h->stream));
+  std::vector<int32_t> e(h->nscenes);
+  HG_HIP(h->ctx, hipMemcpy(e.data(), h->err, h->nscenes * 4, hipMemcpyDeviceToHost));
+  for (uint32_t s = 0; s < h->nscenes; ++s) {
+    if (e[s] & 4) return hg_fail(h->ctx, HG_ERR_UNSUPPORTED, "scene " + std::to_string(s) + ": more than 8 contacts on one body");
+    if (e[s] & 2) return hg_fail(h->ctx, HG_ERR_STATE, "scene " + std::to_string(s) + ": non-finite body state");
+  }
+  return HG_OK;
+}
+
+extern "C" hg_status hg_dem_create(hg_ctx *ctx, const hg_dem_desc *desc, hg_dem **out) {
+  return dem_create_impl(ctx, desc, 1, out);
+}
+
+extern "C" hg_status hg_dem_create_batch(hg_ctx *ctx, const hg_dem_desc *descs, uint32_t nscenes,
+                                         hg_dem **out) {
+  return dem_create_impl(ctx, descs, nscenes, out);
+}
+
+extern "C" hg_status hg_dem_destroy(hg_dem *h) {
+  if (!h) return HG_OK;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->graph) hipGraphExecDestroy(h->graph);
+  for (void *p : h->allocs) hipFree(p);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return HG_OK;
+}
+
+extern "C" hg_status hg_dem_step(hg_dem *h, double dt, uint32_t nsteps) {
+  if (!h) return HG_ERR_INVALID;
+  if (!(dt > 0.0)) return hg_fail(h->ctx, HG_ERR_INVALID, "dt must be > 0");
+  HG_TRY(dem_upload_dt(h, dt));
+  for (uint32_t k = 0; k < nsteps; ++k) launch_step(h);
+  HG_HIP(h->ctx, hipGetLastError());
+  return HG_OK;
+}
+
+extern "C" hg_status hg_dem_step_graph(hg_dem *h, double dt, uint32_t nsteps) {
+  if (!h) return HG_ERR_INVALID;
+  if (!(dt > 0.0)) return hg_fail(h->ctx, HG_ERR_INVALID, "dt must be > 0");
+  if (h->prof.on) return hg_dem_step(h, dt, nsteps);
+  HG_TRY(dem_upload_dt(h, dt));
+  if (h->cur != 0 && nsteps) {
+    launch_step(h);
+    --nsteps;
+  }
+  const uint32_t pairs = nsteps / 2;
+  if (pairs) {
+    if (!h->graph) {
+      hipGraph_t g = nullptr;
+      HG_HIP(h->ctx, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      launch_step(h);
+      launch_step(h);
+      HG_HIP(h->ctx, hipStreamEndCapture(h->stream, &g));
+      HG_HIP(h->ctx, hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0));
+      HG_HIP(h->ctx, hipGraphDestroy(g));
+    }
+    for (uint32_t k = 0; k < pairs; ++k) HG_HIP(h->ctx, hipGraphLaunch(h->graph, h->stream));
+  }
+  if (nsteps & 1u) launch_step(h);
+  HG_HIP(h->ctx, hipGetLastError());
+  return HG_OK;
+}
+
+extern "C" hg_status hg_dem_sync(hg_dem *h) {
+  if (!h) return HG_ERR_INVALID;
+  return dem_check(h);
+}
+
+template <typename T>
+static hg_status set_weights_t(hg_dem *h, uint32_t scene, const double *w) {
+  const SceneDev &sc = h->scenes[scene];
+  std::vector<T> tw(sc.b_cnt);
+  for (uint32_t k = 0; k < sc.b_cnt; ++k) tw[k] = T(w[k]);
+  HG_HIP(h->ctx, hipStreamSynchronize(h->stream));
+  if (sc.b_cnt)
+    HG_HIP(h->ctx, hipMemcpy(static_cast<T *>(h->b.w) + sc.b_off, tw.data(), sc.b_cnt * sizeof(T),
+                             hipMemcpyHostToDevice));
+  return HG_OK;
+}
+
+extern "C" hg_status hg_dem_set_weights(hg_dem *h, uint32_t scene, const double *weights) {
+  if (!h || scene >= h->nscenes) return HG_ERR_INVALID;
+  HG_HIP(h->ctx, hipStreamSynchronize(h->stream));
+  h->scenes[scene].weighted = weights ? 1 : 0;
+  if (weights) {
+    if (h->precision == HG_FP64) HG_TRY(set_weights_t<double>(h, scene, weights));
+    else HG_TRY(set_weights_t<float>(h, scene, weights));
+  }
+  HG_HIP(h->ctx, hipMemcpy(h->d_scenes, h->scenes.data(), h->scenes.size() * sizeof(SceneDev),
+                           hipMemcpyHostToDevice));
+  if (h->graph) {
+    hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  return HG_OK;
+}
+
+template <typename T>
+static hg_status download_bodies_t(hg_dem *h, uint32_t scene, double *q, double *v) {
+  const SceneDev &sc = h->scenes[scene];
+  const uint32_t n = sc.b_cnt, off = sc.b_off;
+  if (!n) return HG_OK;
+  std::vector<double> x(n), y(n), th(n);
+  std::vector<T> vx(n), vy(n), om(n);
+  HG_HIP(h->ctx, hipMemcpy(x.data(), h->b.x + off, n * 8, hipMemcpyDeviceToHost));
+  HG_HIP(h->ctx, hipMemcpy(y.data(), h->b.y + off, n * 8, hipMemcpyDeviceToHost));
+  HG_HIP(h->ctx, hipMemcpy(th.data(), h->b.th + off, n * 8, hipMemcpyDeviceToHost));
+  HG_HIP(h->ctx, hipMemcpy(vx.data(), static_cast<T *>(h->b.vx) + off, n * sizeof(T), hipMemcpyDeviceToHost));
+  HG_HIP(h->ctx, hipMemcpy(vy.data(), static_cast<T *>(h->b.vy) + off, n * sizeof(T), hipMemcpyDeviceToHost));
+  HG_HIP(h->ctx, hipMemcpy(om.data(), static_cast<T *>(h->b.om) + off, n * sizeof(T), hipMemcpyDeviceToHost));
+  for (uint32_t k = 0; k < n; ++k) {
+    if (q) {
+      q[3 * k] = x[k];
+      q[3 * k + 1] = y[k];
+      q[3 * k + 2] = th[k];
+    }
+    if (v) {
+      v[3 * k] = double(vx[k]);
+      v[3 * k + 1] = double(vy[k]);
+      v[3 * k + 2] = double(om[k]);
+    }
+  }
+  return HG_OK;
+}
+
+extern "C" hg_status hg_dem_download_bodies(hg_dem *h, uint32_t scene, double *q, double *v) {
+  if (!h || scene >= h->nscenes) return HG_ERR_INVALID;
+  HG_TRY(dem_check(h));
+  if (h->precision == HG_FP64) return download_bodies_t<double>(h, scene, q, v);
+  return download_bodies_t<float>(h, scene, q, v);
+}
+
+template <typename T>
+static hg_status download_contacts_t(hg_dem *h, uint32_t scene, uint32_t max, int32_t *type,
+                                     int32_t *ci, int32_t *cj, double *delta_s, uint32_t *count) {
+  const SceneDev &sc = h->scenes[scene];
+  const uint32_t n = sc.b_cnt, off = sc.b_off;
+  const SlotsRaw &s = h->slots[h->cur];
+  std::vector<uint32_t> okey(size_t(n) * kMaxOwned);
+  std::vector<T> ods(size_t(n) * kMaxOwned * 2), pds(size_t(n) * kMaxPlanes * 2);
+  std::vector<uint8_t> ocnt(n), pact(size_t(n) * kMaxPlanes);
+  if (n) {
+    HG_HIP(h->ctx, hipMemcpy(okey.data(), s.okey + size_t(off) * kMaxOwned, okey.size() * 4, hipMemcpyDeviceToHost));
+    HG_HIP(h->ctx, hipMemcpy(ods.data(), static_cast<T *>(s.ods) + size_t(off) * kMaxOwned * 2, ods.size() * sizeof(T), hipMemcpyDeviceToHost));
+    HG_HIP(h->ctx, hipMemcpy(ocnt.data(), s.ocnt + off, n, hipMemcpyDeviceToHost));
+    HG_HIP(h->ctx, hipMemcpy(pact.data(), s.pact + size_t(off) * kMaxPlanes, pact.size(), hipMemcpyDeviceToHost));
+    HG_HIP(h->ctx, hipMemcpy(pds.data(), static_cast<T *>(s.pds) + size_t(off) * kMaxPlanes * 2, pds.size() * sizeof(T), hipMemcpyDeviceToHost));
+  }
+  uint32_t c = 0;
+  auto emit = [&](int t, uint32_t i, uint32_t j, double d0, double d1) {
+    if (c < max) {
+      if (type) type[c] = t;
+      if (ci) ci[c] = int32_t(i);
+      if (cj) cj[c] = int32_t(j);
+      if (delta_s) {
+        delta_s[2 * c] = d0;
+        delta_s[2 * c + 1] = d1;
+      }
+    }
+    ++c;
+  };
+  // reference iteration order: planes, circle pairs, kinematic pairs, each by owner
+  for (uint32_t i = 0; i < n; ++i)
+    for (int k = 0; k < sc.nplanes; ++k)
+      if (pact[size_t(i) * kMaxPlanes + k])
+        emit(0, i, uint32_t(k), double(pds[2 * (size_t(i) * kMaxPlanes + k)]),
+             double(pds[2 * (size_t(i) * kMaxPlanes + k) + 1]));
+  for (int kin = 0; kin < 2; ++kin)
+    for (uint32_t i = 0; i < n; ++i)
+      for (int q = 0; q < ocnt[i]; ++q) {
+        const size_t sl = size_t(i) * kMaxOwned + q;
+        const bool isk = (okey[sl] & kKinematic) != 0;
+        if (isk != (kin == 1)) continue;
+        emit(kin ? 2 : 1, i, okey[sl] & ~kKinematic, double(ods[2 * sl]), double(ods[2 * sl + 1]));
+      }
+  *count = c;
+  return HG_OK;
+}
+
+extern "C" hg_status hg_dem_download_contacts(hg_dem *h, uint32_t scene, uint32_t max,
+                                              int32_t *type, int32_t *i, int32_t *j,
+                                              double *delta_s, uint32_t *count) {
+  if (!h || scene >= h->nscenes || !count) return HG_ERR_INVALID;
+  HG_TRY(dem_check(h));
+  if (h->precision == HG_FP64) return download_contacts_t<double>(h, scene, max, type, i, j, delta_s, count);
+  return download_contacts_t<float>(h, scene, max, type, i, j, delta_s, count);
+}
+
+extern "C" hg_status hg_dem_keep_force(hg_dem *h, int32_t keep) {
+  if (!h) return HG_ERR_INVALID;
+  HG_HIP(h->ctx, hipStreamSynchronize(h->stream));
+  h->keep_force = keep != 0;
+  if (h->graph) {
+    hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  return HG_OK;
+}
+
+extern "C" hg_status hg_dem_download_force(hg_dem *h, uint32_t scene, double *F) {
+  if (!h || scene >= h->nscenes || !F) return HG_ERR_INVALID;
+  HG_TRY(dem_check(h));
+  const SceneDev &sc = h->scenes[scene];
+  if (sc.b_cnt)
+    HG_HIP(h->ctx, hipMemcpy(F, h->F_out + 3 * size_t(sc.b_off), size_t(sc.b_cnt) * 24, hipMemcpyDeviceToHost));
+  return HG_OK;
+}
+
+extern "C" hg_status hg_dem_set_profiling(hg_dem *h, int32_t on) {
+  if (!h) return HG_ERR_INVALID;
+  HG_HIP(h->ctx, hipStreamSynchronize(h->stream));
+  h->prof.reset();
+  h->prof.on = on != 0;
+  return HG_OK;
+}
+
+extern "C" hg_status hg_dem_kernel_times(hg_dem *h, int32_t max_kernels, char *names, double *ms,
+                                         int64_t *launches, int32_t *nkernels) {
+  if (!h) return HG_ERR_INVALID;
+  return hg_profiler_report(h->prof, h->ctx, max_kernels, names, ms, launches, nkernels);
+}

@@ -4,6 +4,7 @@
 // MaterialPoints / ConstitutiveModel routines it calls.  See
 // hg_mpm_kernels.h for the HBM layout.
 #include "hg_mpm_kernels.h"
+#include "hg_sort.h"
 
 #include <algorithm>
 #include <cmath>
@@ -15,9 +16,6 @@ using namespace hgmpm;
 
 namespace {
 
-constexpr int kScanThreads = 1024;
-constexpr int kScanItems = 4;
-constexpr int kScanChunk = kScanThreads * kScanItems;
 constexpr int kG2PThreads = 256;
 
 // ---------------------------------------------------------------------------
@@ -72,24 +70,23 @@ __device__ __forceinline__ bool point_key(const SceneDev &sc, double x, double y
 }
 
 // ---------------------------------------------------------------------------
-// P2G + grid update.  One workgroup per tile of TW x TH owned nodes.  Its
-// AW x AH = (TW+S-1) x (TH+S-1) threads each own one stencil-base cell of the
-// tile's apron; the apron's points lie in AH contiguous runs of the cell-sorted
-// order (one per cell row).
+// P2G + grid update.  One workgroup per tile of TW x TH owned nodes, one
+// thread per node.  The points whose stencils reach the tile have their base
+// cell in the apron [x0-S+1, x0+TW) x [y0-S+1, y0+TH); in the cell-sorted
+// order they form one run per apron row.
 //  1. stage (point-parallel): the apron points are loaded through the
 //     permutation, their Cauchy stress is evaluated (FP64), and a compact
-//     record (position, mass, momentum, -V J, sigma, id) goes to LDS.  Rows
-//     are staged in chunks of at most CAP points.
-//  2. accumulate (cell-parallel, from LDS): each thread sums its cell's points
-//     in id order -- so the sums do not depend on the sort's slot order --
-//     into S*S node accumulators in registers.
-//  3. S*S conflict-free phases add the accumulators into the owned-node LDS
-//     tile: in one phase every cell targets a different node, so no atomics
-//     are needed and each node's sum has a fixed order.  Contributions to
-//     nodes outside the tile are dropped (the neighbouring tile computes
-//     them), so each node is written once, with plain stores, by its owner.
-//  4. grid update of the owned nodes (forces, momentum, plane conditions,
-//     velocity and acceleration), written as one (v, a) vector per node.
+//     record goes to LDS; rows are staged in chunks of at most CAP points.
+//  2. order: each cell's records are ranked by point id, so every node adds
+//     a cell's points in id order whatever order the sort left them in.
+//  3. gather (node-parallel): each thread walks the S x S cells whose points
+//     can reach its node -- rows ascending, cells ascending, points in id
+//     order -- and sums mass, momentum and force in FP64 registers.  Chunks
+//     are whole rows taken in ascending order, so the sum order does not
+//     depend on the chunking: each node's value is a fixed function of the
+//     scene state (bitwise reproducible, identical alone or in a batch).
+//  4. grid update of the node (forces, momentum, plane conditions, velocity
+//     and acceleration), written as one (v, a) vector per node.
 //
 // Follows PhysicsGrid::rasterizePointMasses (PhysicsGrid.cpp:168-207),
 // rasterizePointMomentum (:274-311), MaterialPoints::computeHyperelasticCauchyStress
@@ -172,93 +169,66 @@ __device__ __forceinline__ P2GRec<T> p2g_record(const PtsRaw &pts, uint32_t p, c
   return r;
 }
 
-// Per-cell node accumulators.  FP64 path: FP64 sums.  FP32 path: FP32 sums
-// with the momentum carried as a compensated two-float (TwoSum) pair.
-template <typename T, int S> struct P2GAcc;
-
-template <int S> struct P2GAcc<double, S> {
-  double m[S][S], p0[S][S], p1[S][S], f0[S][S], f1[S][S];
-  __device__ __forceinline__ void zero() {
-#pragma unroll
-    for (int j = 0; j < S; ++j)
-#pragma unroll
-      for (int i = 0; i < S; ++i) m[j][i] = p0[j][i] = p1[j][i] = f0[j][i] = f1[j][i] = 0.0;
-  }
-  __device__ __forceinline__ void add_mom(int j, int i, double a, double b) {
-    p0[j][i] += a;
-    p1[j][i] += b;
-  }
-  __device__ __forceinline__ double mom0(int j, int i) const { return p0[j][i]; }
-  __device__ __forceinline__ double mom1(int j, int i) const { return p1[j][i]; }
-};
-
-__device__ __forceinline__ void two_sum(float &s, float &e, float b) {
-  const float a = s;
-  const float t = a + b;
-  const float bb = t - a;
-  e += (a - (t - bb)) + (b - bb);
-  s = t;
-}
-
-template <int S> struct P2GAcc<float, S> {
-  float m[S][S], p0[S][S], p0l[S][S], p1[S][S], p1l[S][S], f0[S][S], f1[S][S];
-  __device__ __forceinline__ void zero() {
-#pragma unroll
-    for (int j = 0; j < S; ++j)
-#pragma unroll
-      for (int i = 0; i < S; ++i)
-        m[j][i] = p0[j][i] = p0l[j][i] = p1[j][i] = p1l[j][i] = f0[j][i] = f1[j][i] = 0.0f;
-  }
-  __device__ __forceinline__ void add_mom(int j, int i, float a, float b) {
-    two_sum(p0[j][i], p0l[j][i], a);
-    two_sum(p1[j][i], p1l[j][i], b);
-  }
-  __device__ __forceinline__ double mom0(int j, int i) const { return double(p0[j][i]) + double(p0l[j][i]); }
-  __device__ __forceinline__ double mom1(int j, int i) const { return double(p1[j][i]) + double(p1l[j][i]); }
-};
-
-// Adds one point's contributions to the S x S nodes of stencil base (bx, by).
+// 1-D basis factor and its derivative for node offset k (0..S-1) from the
+// point's stencil base, masked past the stencil's count.
 template <typename T, int BASIS>
-__device__ __forceinline__ void p2g_accumulate(const P2GRec<T> &r, const SceneDev &sc, int bx,
-                                               int by, P2GAcc<T, BasisTraits<BASIS>::S> &acc) {
-  constexpr int S = BasisTraits<BASIS>::S;
-  const int cntx = int(r.cnt & 15u), cnty = int(r.cnt >> 4);
-  T axs[S], dxs[S], ays[S], dys[S];
-  basis_2d<T, BASIS>(r.x, r.y, r.hl, sc.min0, sc.min1, sc.h, T(sc.inv_h), bx, by, cntx, cnty, axs,
-                     dxs, ays, dys);
-#pragma unroll
-  for (int j = 0; j < S; ++j)
-#pragma unroll
-    for (int i = 0; i < S; ++i) {
-      const T w = axs[i] * ays[j];
-      const T gx = dxs[i] * ays[j];
-      const T gy = axs[i] * dys[j];
-      acc.m[j][i] += w * r.m;
-      acc.add_mom(j, i, w * r.mv0, w * r.mv1);
-      if (w != T(0)) {
-        const T sw0 = r.s00 * gx + r.s01 * gy;
-        const T sw1 = r.s01 * gx + r.s11 * gy;
-        acc.f0[j][i] += r.c * sw0;
-        acc.f1[j][i] += r.c * sw1;
-      }
+__device__ __forceinline__ void basis_factor(double x, T hl, double gmin, double h, T inv_h,
+                                             int node, int k, int cnt, T &a, T &d) {
+  if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) {
+    const double xi = h * double(node) + gmin;
+    const T r = T(x - xi);
+    const T w = T(h);
+    const T diff = fabs(r);
+    const T sgn = r >= T(0) ? T(1) : T(-1);
+    if (diff >= w + hl || k >= cnt) {
+      a = T(0);
+      d = T(0);
+    } else if (diff >= w - hl) {
+      const T u = w + hl - diff;
+      a = u * u * (T(0.5) * inv_h);
+      d = -sgn * u * inv_h;
+    } else if (diff >= hl) {
+      a = T(2) * hl * (T(1) - diff * inv_h);
+      d = -sgn * T(2) * hl * inv_h;
+    } else {
+      a = T(2) * hl - (hl * hl + diff * diff) * inv_h;
+      d = T(2) * (-r) * inv_h;
     }
+  } else if constexpr (BASIS == HG_BASIS_LINEAR) {
+    const T t = T((x - h * double(node) - gmin) * double(inv_h));
+    const T at = fabs(t);
+    a = at < T(1) ? T(1) - at : T(0);
+    d = (at <= T(1)) ? ((t >= T(0)) ? T(-1) : T(1)) : T(0);
+  } else {
+    const T t = T((x - h * double(node) - gmin) * double(inv_h));
+    const T at = fabs(t);
+    const T sg = (t >= T(0)) ? T(1) : T(-1);
+    if (at < T(1)) {
+      a = (T(0.5) * at - T(1)) * at * at + T(2.0 / 3.0);
+      d = sg * (T(1.5) * at - T(2)) * at;
+    } else if (at < T(2)) {
+      a = ((-at * T(1.0 / 6.0) + T(1)) * at - T(2)) * at + T(4.0 / 3.0);
+      d = sg * ((T(-0.5) * at + T(2)) * at - T(2));
+    } else {
+      a = T(0);
+      d = T(0);
+    }
+  }
 }
 
-template <typename T, int BASIS, int AW, int AH, int CAP>
-__global__ __launch_bounds__(AW *AH) __attribute__((amdgpu_waves_per_eu(3)))
-void mpm_p2g_kernel(P2GArgs args) {
+template <typename T, int BASIS, int TW, int TH, int CAP>
+__global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
   constexpr int S = BasisTraits<BASIS>::S;
-  constexpr int TW = AW - (S - 1);
-  constexpr int TH = AH - (S - 1);
-  constexpr int NT = AW * AH;
+  constexpr int AW = TW + S - 1;  // apron cells per row
+  constexpr int AH = TH + S - 1;  // apron rows
+  constexpr int NT = TW * TH;
   __shared__ double s_x[CAP], s_y[CAP];
   __shared__ T s_m[CAP], s_mv0[CAP], s_mv1[CAP], s_c[CAP], s_s00[CAP], s_s01[CAP], s_s11[CAP],
       s_hl[CAP];
   __shared__ uint32_t s_id[CAP];
+  __shared__ uint16_t s_ord[CAP];
   __shared__ uint8_t s_cnt[CAP];
-  __shared__ double s_nm[TW * TH], s_np0[TW * TH], s_np1[TW * TH], s_nf0[TW * TH],
-      s_nf1[TW * TH];
-  __shared__ uint32_t s_rs[AH], s_rc[AH];
+  __shared__ uint32_t s_rs[AH], s_rc[AH], s_cs[AH][AW + 1];
   __shared__ int s_any;
 
   const uint32_t tile = hg_xcd_remap(blockIdx.x, args.nblocks_padded);
@@ -270,47 +240,37 @@ void mpm_p2g_kernel(P2GArgs args) {
   const int y0 = int(ltile / uint32_t(sc.tiles_x)) * TH;
   const int t = threadIdx.x;
   const uint32_t *cs = args.cell_start + sc.cell_off;
+  const int ax0 = x0 - (S - 1), ay0 = y0 - (S - 1);
 
-  // This thread's apron cell and its run of the sorted order.
-  const int myrow = t / AW;
-  const int bx = x0 - (S - 1) + (t % AW);
-  const int by = y0 - (S - 1) + myrow;
-  uint32_t s0 = 0, s1 = 0;
-  if (bx >= 0 && by >= 0 && bx < sc.nnx && by < sc.nny) {
-    const uint32_t key = uint32_t(by) * uint32_t(sc.snx) + uint32_t(bx);
-    s0 = cs[key];
-    s1 = cs[key + 1];
-  }
-  // Row runs: cells of one apron row are consecutive keys.
-  if (t < AH) {
-    const int ry = y0 - (S - 1) + t;
-    const int xl = max(x0 - (S - 1), 0), xh = min(x0 - (S - 1) + AW - 1, sc.nnx - 1);
-    uint32_t rs = 0, rc = 0;
-    if (ry >= 0 && ry < sc.nny && xl <= xh) {
-      const uint32_t k0 = uint32_t(ry) * uint32_t(sc.snx) + uint32_t(xl);
-      const uint32_t k1 = uint32_t(ry) * uint32_t(sc.snx) + uint32_t(xh) + 1u;
-      rs = cs[k0];
-      rc = cs[k1] - rs;
+  // Apron cell starts, row by row (cells of one row are consecutive keys;
+  // cells off the grid hold no points and read as empty).
+  for (int q = t; q < AH * (AW + 1); q += NT) {
+    const int r = q / (AW + 1), c = q % (AW + 1);
+    const int ry = ay0 + r;
+    uint32_t v = 0;
+    if (ry >= 0 && ry < sc.nny) {
+      const int cx = min(max(ax0 + c, 0), sc.nnx);
+      v = cs[uint32_t(ry) * uint32_t(sc.snx) + uint32_t(cx)];
     }
-    s_rs[t] = rs;
-    s_rc[t] = rc;
+    s_cs[r][c] = v;
   }
   if (t == 0) s_any = 0;
-  for (int k = t; k < TW * TH; k += NT) {
-    s_nm[k] = 0.0;
-    s_np0[k] = 0.0;
-    s_np1[k] = 0.0;
-    s_nf0[k] = 0.0;
-    s_nf1[k] = 0.0;
-  }
   __syncthreads();
-  if (t < AH && s_rc[t] > 0) s_any = 1;
+  if (t < AH) {
+    s_rs[t] = s_cs[t][0];
+    s_rc[t] = s_cs[t][AW] - s_cs[t][0];
+    if (s_rc[t] > 0) s_any = 1;
+  }
   __syncthreads();
   const bool any = s_any != 0;
   if (t == 0) args.tile_active[tile] = any ? 1 : 0;
   if (!any) return;
 
-  const uint32_t n = s1 - s0;
+  const int lx = t % TW, ly = t / TW;
+  const int nx = x0 + lx, ny = y0 + ly;
+  const T inv_h = T(sc.inv_h);
+  double am = 0.0, ap0 = 0.0, ap1 = 0.0, af0 = 0.0, af1 = 0.0;
+
   int r0 = 0;
   while (r0 < AH) {
     // rows [r0, r1) are staged together (block-uniform decision); a single
@@ -323,8 +283,8 @@ void mpm_p2g_kernel(P2GArgs args) {
     }
     const bool staged = r1 > r0;
     if (!staged) r1 = r0 + 1;
-    // 1. stage the chunk's points
     if (staged) {
+      // 1. stage the chunk's points
       for (uint32_t k = uint32_t(t); k < tot; k += NT) {
         int r = r0;
         uint32_t pre = 0;
@@ -348,129 +308,157 @@ void mpm_p2g_kernel(P2GArgs args) {
         s_cnt[k] = uint8_t(rec.cnt);
       }
       __syncthreads();
+      // 2. rank each cell's records by id
+      for (int q = t; q < (r1 - r0) * AW; q += NT) {
+        const int r = r0 + q / AW, c = q % AW;
+        uint32_t base = s_cs[r][c] - s_rs[r];
+        for (int rr = r0; rr < r; ++rr) base += s_rc[rr];
+        const uint32_t n = s_cs[r][c + 1] - s_cs[r][c];
+        for (uint32_t u = 0; u < n; ++u) {
+          const uint32_t idu = s_id[base + u];
+          uint32_t rank = 0;
+          for (uint32_t v = 0; v < n; ++v) rank += (s_id[base + v] < idu) ? 1u : 0u;
+          s_ord[base + rank] = uint16_t(base + u);
+        }
+      }
+      __syncthreads();
     }
-    // 2. accumulate this thread's cell, in id order
-    const bool mine = myrow >= r0 && myrow < r1 && n > 0;
-    P2GAcc<T, S> acc;
-    acc.zero();
-    if (mine) {
-      uint32_t base = s0 - s_rs[myrow];
-      for (int r = r0; r < myrow; ++r) base += s_rc[r];
-      uint32_t last_id = 0;
-      for (uint32_t k = 0; k < n; ++k) {
-        uint32_t best = 0xffffffffu, bq = 0;
-        for (uint32_t q = 0; q < n; ++q) {
-          const uint32_t idv = staged ? s_id[base + q] : args.pts.id[args.perm[s0 + q]];
-          if ((k == 0 || idv > last_id) && idv <= best) {
-            best = idv;
-            bq = q;
+    // 3. gather: cells (ny - j, nx - i) of this chunk's rows, rows ascending
+    if (nx < sc.nnx && ny < sc.nny) {
+      for (int j = S - 1; j >= 0; --j) {
+        const int r = (ny - j) - ay0;  // apron row of the cells at ny - j
+        if (r < r0 || r >= r1) continue;
+        uint32_t rowbase = 0;
+        if (staged)
+          for (int rr = r0; rr < r; ++rr) rowbase += s_rc[rr];
+        for (int i = S - 1; i >= 0; --i) {
+          const int c = (nx - i) - ax0;  // apron column of the cells at nx - i
+          const uint32_t c0 = s_cs[r][c], c1 = s_cs[r][c + 1];
+          const uint32_t n = c1 - c0;
+          if (n == 0) continue;
+          const uint32_t base = rowbase + (c0 - s_rs[r]);
+          uint32_t last_id = 0;
+          for (uint32_t u = 0; u < n; ++u) {
+            P2GRec<T> rec;
+            if (staged) {
+              const uint32_t q = s_ord[base + u];
+              rec.x = s_x[q];
+              rec.y = s_y[q];
+              rec.m = s_m[q];
+              rec.mv0 = s_mv0[q];
+              rec.mv1 = s_mv1[q];
+              rec.c = s_c[q];
+              rec.s00 = s_s00[q];
+              rec.s01 = s_s01[q];
+              rec.s11 = s_s11[q];
+              rec.hl = s_hl[q];
+              rec.cnt = s_cnt[q];
+            } else {
+              // next point of the cell in id order, straight from HBM
+              uint32_t best = 0xffffffffu, bp = 0;
+              for (uint32_t v = c0; v < c1; ++v) {
+                const uint32_t pi = args.perm[v];
+                const uint32_t idv = args.pts.id[pi];
+                if ((u == 0 || idv > last_id) && idv <= best) {
+                  best = idv;
+                  bp = pi;
+                }
+              }
+              last_id = best;
+              rec = p2g_record<T, BASIS>(args.pts, bp, sc);
+            }
+            const int cntx = int(rec.cnt & 15u), cnty = int(rec.cnt >> 4);
+            T ax, dx, ay, dy;
+            basis_factor<T, BASIS>(rec.x, rec.hl, sc.min0, sc.h, inv_h, nx, i, cntx, ax, dx);
+            basis_factor<T, BASIS>(rec.y, rec.hl, sc.min1, sc.h, inv_h, ny, j, cnty, ay, dy);
+            T w, gx, gy;
+            if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) {
+              const T inv_vp = T(1) / (T(4) * rec.hl * rec.hl);
+              const T axv = ax * inv_vp;
+              w = axv * ay;
+              gx = (dx * inv_vp) * ay;
+              gy = axv * dy;
+            } else {
+              w = ax * ay;
+              gx = (dx * inv_h) * ay;
+              gy = ax * (dy * inv_h);
+            }
+            am += double(w * rec.m);
+            ap0 += double(w * rec.mv0);
+            ap1 += double(w * rec.mv1);
+            if (w != T(0)) {
+              af0 += double(rec.c * (rec.s00 * gx + rec.s01 * gy));
+              af1 += double(rec.c * (rec.s01 * gx + rec.s11 * gy));
+            }
           }
         }
-        last_id = best;
-        P2GRec<T> rec;
-        if (staged) {
-          const uint32_t q = base + bq;
-          rec.x = s_x[q];
-          rec.y = s_y[q];
-          rec.m = s_m[q];
-          rec.mv0 = s_mv0[q];
-          rec.mv1 = s_mv1[q];
-          rec.c = s_c[q];
-          rec.s00 = s_s00[q];
-          rec.s01 = s_s01[q];
-          rec.s11 = s_s11[q];
-          rec.hl = s_hl[q];
-          rec.cnt = s_cnt[q];
-        } else {
-          rec = p2g_record<T, BASIS>(args.pts, args.perm[s0 + bq], sc);
-        }
-        p2g_accumulate<T, BASIS>(rec, sc, bx, by, acc);
       }
     }
-    // 3. S*S conflict-free phases add the chunk's cell sums into the
-    //    owned-node LDS tile (the last barrier also frees the staging arrays)
-#pragma unroll
-    for (int j = 0; j < S; ++j)
-#pragma unroll
-      for (int i = 0; i < S; ++i) {
-        const int lx = bx + i - x0, ly = by + j - y0;
-        if (mine && lx >= 0 && lx < TW && ly >= 0 && ly < TH) {
-          const int q = ly * TW + lx;
-          s_nm[q] += double(acc.m[j][i]);
-          s_np0[q] += acc.mom0(j, i);
-          s_np1[q] += acc.mom1(j, i);
-          s_nf0[q] += double(acc.f0[j][i]);
-          s_nf1[q] += double(acc.f1[j][i]);
-        }
-        __syncthreads();
-      }
+    if (staged) __syncthreads();
     r0 = r1;
   }
 
-  // 4. grid update of the owned nodes.
+  // 4. grid update of the node.
+  if (nx >= sc.nnx || ny >= sc.nny) return;
   using T4 = typename Vec4T<T>::type;
   const double dt = sc.dt;
-  for (int q = t; q < TW * TH; q += NT) {
-    const int nx = x0 + (q % TW), ny = y0 + (q / TW);
-    if (nx >= sc.nnx || ny >= sc.nny) continue;
-    const uint32_t node = sc.node_off + uint32_t(ny) * uint32_t(sc.snx) + uint32_t(nx);
-    const double m = s_nm[q];
-    const double p0 = s_np0[q], p1 = s_np1[q];
-    double f0 = s_nf0[q], f1 = s_nf1[q];
-    f0 += m * sc.g0;
-    f1 += m * sc.g1;
-    double pn0 = 0.0, pn1 = 0.0;
+  const uint32_t node = sc.node_off + uint32_t(ny) * uint32_t(sc.snx) + uint32_t(nx);
+  const double m = am;
+  const double p0 = ap0, p1 = ap1;
+  double f0 = af0, f1 = af1;
+  f0 += m * sc.g0;
+  f1 += m * sc.g1;
+  double pn0 = 0.0, pn1 = 0.0;
+  if (m > 0.0) {
+    pn0 = p0 + dt * f0;
+    pn1 = p1 + dt * f1;
+  }
+  const double px0 = sc.min0 + sc.h * double(nx);
+  const double px1 = sc.min1 + sc.h * double(ny);
+  for (int k = 0; k < sc.nplanes; ++k) {
+    if (px1 < sc.plane_lower[k]) continue;
+    const double n0 = sc.plane_n[2 * k], n1 = sc.plane_n[2 * k + 1];
+    const double d = n0 * (px0 - sc.plane_x[2 * k]) + n1 * (px1 - sc.plane_x[2 * k + 1]);
+    if (d > 0.0) continue;
+    if (sc.plane_beh[k] == HG_PLANE_SLIDING) {
+      if (m <= 0.0) continue;
+      double u0 = pn0 / m, u1 = pn1 / m;
+      const double vn = n0 * u0 + n1 * u1;
+      if (vn > 0.0) continue;
+      u0 -= vn * n0;
+      u1 -= vn * n1;
+      pn0 = m * u0;
+      pn1 = m * u1;
+    } else {
+      pn0 = 0.0;
+      pn1 = 0.0;
+    }
+  }
+  if (args.store & kStoreVelAcc) {
+    T4 va;
     if (m > 0.0) {
-      pn0 = p0 + dt * f0;
-      pn1 = p1 + dt * f1;
+      const double dm = dt * m;
+      va.x = T(pn0 / m);
+      va.y = T(pn1 / m);
+      va.z = T((pn0 - p0) / dm);
+      va.w = T((pn1 - p1) / dm);
+    } else {
+      va.x = va.y = va.z = va.w = T(0);
     }
-    const double px0 = sc.min0 + sc.h * double(nx);
-    const double px1 = sc.min1 + sc.h * double(ny);
-    for (int k = 0; k < sc.nplanes; ++k) {
-      if (px1 < sc.plane_lower[k]) continue;
-      const double n0 = sc.plane_n[2 * k], n1 = sc.plane_n[2 * k + 1];
-      const double d = n0 * (px0 - sc.plane_x[2 * k]) + n1 * (px1 - sc.plane_x[2 * k + 1]);
-      if (d > 0.0) continue;
-      if (sc.plane_beh[k] == HG_PLANE_SLIDING) {
-        if (m <= 0.0) continue;
-        double u0 = pn0 / m, u1 = pn1 / m;
-        const double vn = n0 * u0 + n1 * u1;
-        if (vn > 0.0) continue;
-        u0 -= vn * n0;
-        u1 -= vn * n1;
-        pn0 = m * u0;
-        pn1 = m * u1;
-      } else {
-        pn0 = 0.0;
-        pn1 = 0.0;
-      }
-    }
-    if (args.store & kStoreVelAcc) {
-      T4 va;
-      if (m > 0.0) {
-        const double dm = dt * m;
-        va.x = T(pn0 / m);
-        va.y = T(pn1 / m);
-        va.z = T((pn0 - p0) / dm);
-        va.w = T((pn1 - p1) / dm);
-      } else {
-        va.x = va.y = va.z = va.w = T(0);
-      }
-      static_cast<T4 *>(args.gva)[node] = va;
-    }
-    if (args.store & kStoreMass) args.gm[node] = m;
-    if (args.store & kStoreMom) {
-      args.gp[2 * node] = p0;
-      args.gp[2 * node + 1] = p1;
-    }
-    if (args.store & kStoreMomNew) {
-      args.gpn[2 * node] = pn0;
-      args.gpn[2 * node + 1] = pn1;
-    }
-    if (args.store & kStoreForce) {
-      args.gf[2 * node] = f0;
-      args.gf[2 * node + 1] = f1;
-    }
+    static_cast<T4 *>(args.gva)[node] = va;
+  }
+  if (args.store & kStoreMass) args.gm[node] = m;
+  if (args.store & kStoreMom) {
+    args.gp[2 * node] = p0;
+    args.gp[2 * node + 1] = p1;
+  }
+  if (args.store & kStoreMomNew) {
+    args.gpn[2 * node] = pn0;
+    args.gpn[2 * node + 1] = pn1;
+  }
+  if (args.store & kStoreForce) {
+    args.gf[2 * node] = f0;
+    args.gf[2 * node + 1] = f1;
   }
 }
 
@@ -515,7 +503,9 @@ __global__ __launch_bounds__(256) void mpm_velacc_kernel(VelAccArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// G2P: one thread per point, in cell order.  MaterialPoints::computeVelGrad
+// G2P: one thread per point, in the buffer's physical order (which is the
+// previous sort's order); each point is written to its slot of the current
+// cell order in the other buffer (pinv), so the buffers stay cell-sorted.  MaterialPoints::computeVelGrad
 // (MaterialPoints.cpp:296-322), elasticPrediction (:324-348) with
 // ConstitutiveModel::elasticPrediction (ConstitutiveModel.cpp:19-22),
 // plasticCorrection (:350-369) with the Drucker-Prager return
@@ -531,7 +521,7 @@ struct G2PArgs {
   uint32_t npoints;
   uint32_t nblocks_padded;
   PtsRaw src, dst;
-  const uint32_t *perm;
+  const uint32_t *pinv;
   const void *gva;
   uint32_t *key, *slot, *counts;
   int32_t *err;
@@ -546,7 +536,8 @@ __global__ __launch_bounds__(kG2PThreads) void mpm_g2p_kernel(G2PArgs a) {
   if (i >= a.npoints) return;
   const uint32_t sid = scene_of_point(a.p_off, a.nscenes, i);
   const SceneDev &sc = a.scenes[sid];
-  const uint32_t p = a.perm[i];
+  const uint32_t p = i;
+  const uint32_t o = a.pinv[i];
 
   double x = a.src.x[p], y = a.src.y[p];
   T v0 = ld<T>(a.src.vx, p), v1 = ld<T>(a.src.vy, p);
@@ -665,20 +656,21 @@ __global__ __launch_bounds__(kG2PThreads) void mpm_g2p_kernel(G2PArgs a) {
     }
   }
 
-  // write the point to slot i of the other buffer and count its new cell
-  a.dst.x[i] = x;
-  a.dst.y[i] = y;
-  st<T>(a.dst.vx, i, v0);
-  st<T>(a.dst.vy, i, v1);
-  st<T>(a.dst.b00, i, T(c00));
-  st<T>(a.dst.b01, i, T(c01));
-  st<T>(a.dst.b11, i, T(c11));
-  st<T>(a.dst.J, i, T(J1));
-  st<T>(a.dst.m, i, ld<T>(a.src.m, p));
-  st<T>(a.dst.V, i, ld<T>(a.src.V, p));
-  a.dst.id[i] = a.src.id[p];
-  if (a.src.hl) st<T>(a.dst.hl, i, hl);
-  if (a.src.flags) a.dst.flags[i] = a.src.flags[p];
+  // write the point to its sorted slot o of the other buffer and count its
+  // new cell
+  a.dst.x[o] = x;
+  a.dst.y[o] = y;
+  st<T>(a.dst.vx, o, v0);
+  st<T>(a.dst.vy, o, v1);
+  st<T>(a.dst.b00, o, T(c00));
+  st<T>(a.dst.b01, o, T(c01));
+  st<T>(a.dst.b11, o, T(c11));
+  st<T>(a.dst.J, o, T(J1));
+  st<T>(a.dst.m, o, ld<T>(a.src.m, p));
+  st<T>(a.dst.V, o, ld<T>(a.src.V, p));
+  a.dst.id[o] = a.src.id[p];
+  if (a.src.hl) st<T>(a.dst.hl, o, hl);
+  if (a.src.flags) a.dst.flags[o] = a.src.flags[p];
 
   int nbx, nby, ncx, ncy;
   const bool ok = point_key<BASIS>(sc, x, y, double(hl), nbx, nby, ncx, ncy);
@@ -686,8 +678,8 @@ __global__ __launch_bounds__(kG2PThreads) void mpm_g2p_kernel(G2PArgs a) {
                       isfinite(double(v1)) && isfinite(J1);
   if (!ok || !finite) atomicOr(&a.err[sid], (ok ? 0 : 1) | (finite ? 0 : 2));
   const uint32_t k = sc.cell_off + uint32_t(nby) * uint32_t(sc.snx) + uint32_t(nbx);
-  a.key[i] = k;
-  a.slot[i] = atomicAdd(&a.counts[k], 1u);
+  a.key[o] = k;
+  a.slot[o] = atomicAdd(&a.counts[k], 1u);
 }
 
 // Initial binning of points in creation order.
@@ -715,93 +707,6 @@ __global__ __launch_bounds__(256) void mpm_bin_kernel(BinArgs a) {
   a.slot[i] = atomicAdd(&a.counts[k], 1u);
 }
 
-// ---------------------------------------------------------------------------
-// Counting sort: exclusive scan of the per-cell counts (three passes), then a
-// scatter of point indices.  scan_apply also zeroes the counts for the next
-// step's G2P.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t block_excl_scan_1024(uint32_t v, uint32_t &total) {
-  __shared__ uint32_t s_wave[16];
-  __shared__ uint32_t s_total;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t inc = hg_wave_incl_scan(v);
-  if (lane == 63) s_wave[wave] = inc;
-  __syncthreads();
-  if (wave == 0) {
-    const uint32_t wv = lane < 16 ? s_wave[lane] : 0u;
-    const uint32_t winc = hg_wave_incl_scan(wv);
-    if (lane < 16) s_wave[lane] = winc - wv;  // exclusive wave offsets
-    if (lane == 15) s_total = winc;
-  }
-  __syncthreads();
-  total = s_total;
-  return s_wave[wave] + inc - v;
-}
-
-__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t *counts,
-                                                                   uint32_t n,
-                                                                   uint32_t *block_sums) {
-  const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * kScanItems;
-  uint32_t s = 0;
-#pragma unroll
-  for (int k = 0; k < kScanItems; ++k)
-    if (base + k < n) s += counts[base + k];
-  uint32_t total = 0;
-  (void)block_excl_scan_1024(s, total);
-  if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
-}
-
-__global__ __launch_bounds__(kScanThreads) void scan_top_kernel(uint32_t *block_sums,
-                                                                uint32_t nb) {
-  uint32_t vals[kScanItems];
-  uint32_t s = 0;
-  const uint32_t base = threadIdx.x * kScanItems;
-#pragma unroll
-  for (int k = 0; k < kScanItems; ++k) {
-    vals[k] = (base + k < nb) ? block_sums[base + k] : 0u;
-    s += vals[k];
-  }
-  uint32_t total = 0;
-  uint32_t run = block_excl_scan_1024(s, total);
-#pragma unroll
-  for (int k = 0; k < kScanItems; ++k) {
-    if (base + k < nb) block_sums[base + k] = run;
-    run += vals[k];
-  }
-}
-
-__global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(uint32_t *counts, uint32_t n,
-                                                                  const uint32_t *block_off,
-                                                                  uint32_t *cell_start) {
-  const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * kScanItems;
-  uint32_t vals[kScanItems];
-  uint32_t s = 0;
-#pragma unroll
-  for (int k = 0; k < kScanItems; ++k) {
-    vals[k] = (base + k < n) ? counts[base + k] : 0u;
-    s += vals[k];
-  }
-  uint32_t total = 0;
-  uint32_t run = block_excl_scan_1024(s, total) + block_off[blockIdx.x];
-#pragma unroll
-  for (int k = 0; k < kScanItems; ++k) {
-    if (base + k < n) {
-      cell_start[base + k] = run;
-      counts[base + k] = 0u;
-      if (base + k == n - 1) cell_start[n] = run + vals[k];
-    }
-    run += vals[k];
-  }
-}
-
-__global__ __launch_bounds__(256) void scatter_kernel(const uint32_t *key, const uint32_t *slot,
-                                                      const uint32_t *cell_start, uint32_t n,
-                                                      uint32_t *perm) {
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  perm[cell_start[key[i]] + slot[i]] = i;
-}
-
 }  // namespace
 
 // ===========================================================================
@@ -817,7 +722,7 @@ struct hg_mpm {
   hipStream_t stream = nullptr;
   int precision = HG_FP64;
   int basis = HG_BASIS_UGIMP_LINEAR;
-  int aw = 16, ah = 16;  // P2G apron shape (threads per workgroup = aw*ah)
+  int aw = 16, ah = 16;  // P2G tile of owned nodes (threads per workgroup = aw*ah)
   std::vector<SceneDev> scenes;
   uint32_t nscenes = 0;
   uint32_t npoints = 0;
@@ -825,7 +730,7 @@ struct hg_mpm {
   bool has_hl = false, has_flags = false;
   PtsRaw buf[2] = {};
   int cur = 0;
-  uint32_t *perm = nullptr, *key = nullptr, *slot = nullptr, *counts = nullptr,
+  uint32_t *perm = nullptr, *pinv = nullptr, *key = nullptr, *slot = nullptr, *counts = nullptr,
            *cell_start = nullptr, *block_sums = nullptr;
   uint32_t nscan = 0;
   SceneDev *d_scenes = nullptr;
@@ -851,8 +756,8 @@ struct hg_mpm {
 static int stencil_width(int basis) {
   return basis == HG_BASIS_LINEAR ? 2 : (basis == HG_BASIS_CUBIC ? 4 : 3);
 }
-int hg_mpm::tw() const { return aw - (stencil_width(basis) - 1); }
-int hg_mpm::th() const { return ah - (stencil_width(basis) - 1); }
+int hg_mpm::tw() const { return aw; }
+int hg_mpm::th() const { return ah; }
 
 template <typename P>
 static hg_status dalloc(hg_mpm *h, P **out, size_t bytes) {
@@ -875,10 +780,10 @@ static hg_status dalloc(hg_mpm *h, P **out, size_t bytes) {
 // ---- kernel dispatch ----------------------------------------------------------
 
 template <typename T> struct P2GCap;
-template <> struct P2GCap<float> { static constexpr int value = 640; };
+template <> struct P2GCap<float> { static constexpr int value = 768; };
 template <> struct P2GCap<double> { static constexpr int value = 512; };
 
-template <typename T, int BASIS, int AW, int AH>
+template <typename T, int BASIS, int TW, int TH>
 static void launch_p2g_t(hg_mpm *h, uint32_t store) {
   P2GArgs a;
   a.scenes = h->d_scenes;
@@ -896,14 +801,14 @@ static void launch_p2g_t(hg_mpm *h, uint32_t store) {
   a.ntiles = h->ntiles;
   a.nblocks_padded = hg_pad8(h->ntiles);
   ProfScope ps(h->prof, "mpm_p2g", h->stream);
-  hipLaunchKernelGGL((mpm_p2g_kernel<T, BASIS, AW, AH, P2GCap<T>::value>), dim3(a.nblocks_padded), dim3(AW * AH),
-                     0, h->stream, a);
+  hipLaunchKernelGGL((mpm_p2g_kernel<T, BASIS, TW, TH, P2GCap<T>::value>), dim3(a.nblocks_padded),
+                     dim3(TW * TH), 0, h->stream, a);
 }
 
 template <typename T, int BASIS>
 static void launch_p2g_b(hg_mpm *h, uint32_t store) {
-  if (h->aw == 32 && h->ah == 16) launch_p2g_t<T, BASIS, 32, 16>(h, store);
-  else if (h->aw == 32 && h->ah == 8) launch_p2g_t<T, BASIS, 32, 8>(h, store);
+  if (h->tw() == 32 && h->th() == 8) launch_p2g_t<T, BASIS, 32, 8>(h, store);
+  else if (h->tw() == 16 && h->th() == 8) launch_p2g_t<T, BASIS, 16, 8>(h, store);
   else launch_p2g_t<T, BASIS, 16, 16>(h, store);
 }
 
@@ -933,7 +838,7 @@ static void launch_g2p_t(hg_mpm *h) {
   a.nblocks_padded = hg_pad8(nb);
   a.src = h->buf[h->cur];
   a.dst = h->buf[1 - h->cur];
-  a.perm = h->perm;
+  a.pinv = h->pinv;
   a.gva = h->gva;
   a.key = h->key;
   a.slot = h->slot;
@@ -979,26 +884,8 @@ static void launch_bin_p(hg_mpm *h) {
 }
 
 static void launch_sort(hg_mpm *h) {
-  {
-    ProfScope ps(h->prof, "scan_reduce", h->stream);
-    hipLaunchKernelGGL(scan_reduce_kernel, dim3(h->nscan), dim3(kScanThreads), 0, h->stream,
-                       h->counts, h->ncell, h->block_sums);
-  }
-  {
-    ProfScope ps(h->prof, "scan_top", h->stream);
-    hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(kScanThreads), 0, h->stream,
-                       h->block_sums, h->nscan);
-  }
-  {
-    ProfScope ps(h->prof, "scan_apply", h->stream);
-    hipLaunchKernelGGL(scan_apply_kernel, dim3(h->nscan), dim3(kScanThreads), 0, h->stream,
-                       h->counts, h->ncell, h->block_sums, h->cell_start);
-  }
-  if (h->npoints) {
-    ProfScope ps(h->prof, "scatter", h->stream);
-    hipLaunchKernelGGL(scatter_kernel, dim3((h->npoints + 255) / 256), dim3(256), 0, h->stream,
-                       h->key, h->slot, h->cell_start, h->npoints, h->perm);
-  }
+  hg_counting_sort(h->prof, h->stream, h->counts, h->ncell, h->block_sums, h->nscan,
+                   h->cell_start, h->key, h->slot, h->npoints, h->perm, h->pinv);
 }
 
 static void launch_g2p_and_sort(hg_mpm *h) {
@@ -1231,6 +1118,7 @@ static hg_status mpm_create_impl(hg_ctx *ctx, const hg_mpm_desc *descs, uint32_t
   HG_TRY(dalloc(hp, &h->d_tile_scene, h->ntiles * 4));
   HG_TRY(dalloc(hp, &h->tile_active, h->ntiles));
   HG_TRY(dalloc(hp, &h->perm, size_t(h->npoints) * 4));
+  HG_TRY(dalloc(hp, &h->pinv, size_t(h->npoints) * 4));
   HG_TRY(dalloc(hp, &h->key, size_t(h->npoints) * 4));
   HG_TRY(dalloc(hp, &h->slot, size_t(h->npoints) * 4));
   HG_TRY(dalloc(hp, &h->counts, size_t(h->ncell) * 4));
@@ -1494,8 +1382,8 @@ extern "C" hg_status hg_mpm_kernel_times(hg_mpm *h, int32_t max_kernels, char *n
 
 extern "C" hg_status hg_mpm_set_tile(hg_mpm *h, int32_t aw, int32_t ah) {
   if (!h) return HG_ERR_INVALID;
-  if (!((aw == 16 && ah == 16) || (aw == 32 && ah == 16) || (aw == 32 && ah == 8)))
-    return hg_fail(h->ctx, HG_ERR_UNSUPPORTED, "tile shape must be 16x16, 32x16 or 32x8");
+  if (!((aw == 16 && ah == 16) || (aw == 16 && ah == 8) || (aw == 32 && ah == 8)))
+    return hg_fail(h->ctx, HG_ERR_UNSUPPORTED, "tile shape must be 16x16, 16x8 or 32x8");
   if (h->phase_next != 0) return hg_fail(h->ctx, HG_ERR_INVALID, "a phased step is in progress");
   if (aw == h->aw && ah == h->ah) return HG_OK;
   // Tiles index the grid; rebuild the tile tables for the new shape.

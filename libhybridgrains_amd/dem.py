@@ -9,9 +9,13 @@ scene; ``GpuDem`` keeps scenes resident on the GPU and steps them with
 """
 from __future__ import annotations
 
+import ctypes as C
 from dataclasses import dataclass, field
 
 import numpy as np
+
+from . import _lib
+from ._lib import HG_FP64, DemDesc, ptr
 
 
 @dataclass
@@ -54,3 +58,123 @@ def circle_mass(rho: float, r: np.ndarray) -> np.ndarray:
     m = rho * np.pi * r * r
     I = 0.5 * m * r * r
     return np.stack([m, m, I], 1)
+
+
+def _kernel_times(ctx, fn, h) -> dict:
+    maxk = 32
+    names = C.create_string_buffer(32 * maxk)
+    ms = np.zeros(maxk)
+    launches = np.zeros(maxk, np.int64)
+    nk = C.c_int32()
+    ctx.check(fn(h, maxk, names, ptr(ms), launches.ctypes.data_as(C.POINTER(C.c_int64)),
+                 C.byref(nk)))
+    out = {}
+    raw = names.raw
+    for i in range(min(nk.value, maxk)):
+        nm = raw[32 * i: 32 * i + 32].split(b"\0", 1)[0].decode()
+        out[nm] = (float(ms[i]), int(launches[i]))
+    return out
+
+
+class _DemHold:
+    def __init__(self, s: DemScene, precision: int, cell_width=0.0, domain=None):
+        self.arrs = {}
+
+        def c(name, a, dt=np.float64):
+            a = np.ascontiguousarray(a, dtype=dt)
+            self.arrs[name] = a
+            return a
+
+        d = DemDesc()
+        d.precision = precision
+        d.nbodies = s.nbodies
+        d.q = ptr(c("q", s.q.reshape(-1)))
+        d.v = ptr(c("v", s.v.reshape(-1)))
+        d.mass = ptr(c("m", s.mass.reshape(-1)))
+        d.radius = ptr(c("r", s.radius))
+        d.fixed = ptr(c("f", s.fixed, np.uint8), C.c_uint8)
+        d.gravity[:] = [float(s.gravity[0]), float(s.gravity[1])]
+        d.nplanes = int(s.plane_x.shape[0])
+        d.plane_x = ptr(c("px", s.plane_x.reshape(-1)))
+        d.plane_n = ptr(c("pn", s.plane_n.reshape(-1)))
+        d.kn, d.kt, d.gamman, d.gammat, d.mu = (float(s.kn), float(s.kt), float(s.gamman),
+                                                float(s.gammat), float(s.mu))
+        d.cell_width = float(cell_width)
+        if domain is not None:
+            d.domain_min[:] = [float(domain[0][0]), float(domain[0][1])]
+            d.domain_max[:] = [float(domain[1][0]), float(domain[1][1])]
+        self.desc = d
+
+
+class GpuDem:
+    """One or more circle DEM scenes resident on a GPU."""
+
+    def __init__(self, scenes, precision: int = HG_FP64, ctx=None, cell_width: float = 0.0,
+                 domain=None):
+        if isinstance(scenes, DemScene):
+            scenes = [scenes]
+        self.ctx = ctx or _lib.default_context()
+        self.L = _lib.lib()
+        self.scenes = list(scenes)
+        holds = [_DemHold(s, precision, cell_width, domain) for s in self.scenes]
+        arr = (DemDesc * len(holds))(*[h.desc for h in holds])
+        h = C.c_void_p()
+        self.ctx.check(self.L.hg_dem_create_batch(self.ctx.handle, arr, len(holds), C.byref(h)))
+        self._h = h
+
+    def flow(self, dt: float, nsteps: int = 1, graph: bool = False) -> None:
+        f = self.L.hg_dem_step_graph if graph else self.L.hg_dem_step
+        self.ctx.check(f(self._h, float(dt), int(nsteps)))
+
+    def set_weights(self, weights, scene: int = 0) -> None:
+        w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+        self.ctx.check(self.L.hg_dem_set_weights(self._h, scene, ptr(w)))
+
+    def sync(self) -> None:
+        self.ctx.check(self.L.hg_dem_sync(self._h))
+
+    def state(self, scene: int = 0) -> dict:
+        n = self.scenes[scene].nbodies
+        q = np.zeros((n, 3))
+        v = np.zeros((n, 3))
+        self.ctx.check(self.L.hg_dem_download_bodies(self._h, scene, ptr(q), ptr(v)))
+        return {"q": q, "v": v}
+
+    def contacts(self, scene: int = 0) -> dict:
+        cnt = C.c_uint32()
+        self.ctx.check(self.L.hg_dem_download_contacts(self._h, scene, 0, None, None, None, None,
+                                                       C.byref(cnt)))
+        n = cnt.value
+        t = np.zeros(n, np.int32)
+        i = np.zeros(n, np.int32)
+        j = np.zeros(n, np.int32)
+        ds = np.zeros((n, 2))
+        self.ctx.check(self.L.hg_dem_download_contacts(
+            self._h, scene, n, ptr(t, C.c_int32), ptr(i, C.c_int32), ptr(j, C.c_int32), ptr(ds),
+            C.byref(cnt)))
+        return {"type": t, "i": i, "j": j, "delta_s": ds}
+
+    def keep_force(self, keep: bool) -> None:
+        self.ctx.check(self.L.hg_dem_keep_force(self._h, int(bool(keep))))
+
+    def force(self, scene: int = 0) -> np.ndarray:
+        F = np.zeros((self.scenes[scene].nbodies, 3))
+        self.ctx.check(self.L.hg_dem_download_force(self._h, scene, ptr(F)))
+        return F
+
+    def set_profiling(self, on: bool) -> None:
+        self.ctx.check(self.L.hg_dem_set_profiling(self._h, int(bool(on))))
+
+    def kernel_times(self) -> dict:
+        return _kernel_times(self.ctx, self.L.hg_dem_kernel_times, self._h)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self.L.hg_dem_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
