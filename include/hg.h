@@ -226,6 +226,40 @@ hg_status hg_dem_set_profiling(hg_dem *h, int32_t on);
 hg_status hg_dem_kernel_times(hg_dem *h, int32_t max_kernels, char *names, double *ms,
                               int64_t *launches, int32_t *nkernels);
 
+/* ---- hybrid: hybridgrains2dnew node-node coupling ---------------------------- */
+typedef struct hg_hybrid hg_hybrid;
+/*
+ * Replaces HybridGrains2DSim(HybridIntegratorState, RigidBody2DState,
+ * SimulationState) (hybridgrains2dnew/HybridGrains2DSim.h:19-21): one
+ * continuum scene and one grain scene sharing a stream.  dem_weights (B or
+ * NULL) are the grains' mass weights (RigidBody2DState::massWeights), which
+ * the reference sets from the hybrid zones.  Precisions must match.
+ */
+hg_status hg_hybrid_create(hg_ctx *ctx, const hg_mpm_desc *mpm, const hg_dem_desc *dem,
+                           const double *dem_weights, hg_hybrid **out);
+hg_status hg_hybrid_destroy(hg_hybrid *h);
+/* HybridGrains2DSim::stepSystem (HybridGrains2DSim.cpp:229-252) ->
+ * stepSystem_constraintSetUp_prediction_correction_advection_node_node
+ * (:116-227), nsteps times: continuum grid pass, grain mass rasterisation and
+ * coupling test, weighted DEM step, grain momentum rasterisation, node
+ * velocity exchange, FLIP transfer to the grains, continuum G2P and the
+ * coupled grains' position update (Coupling.cpp:22-591).  The periodic zone
+ * resampling is not part of the step. */
+hg_status hg_hybrid_step(hg_hybrid *h, double dt, uint32_t nsteps);
+/* The same steps replayed from a hipGraph of two steps (no reference
+ * counterpart). */
+hg_status hg_hybrid_step_graph(hg_hybrid *h, double dt, uint32_t nsteps);
+/* Waits; reports the MPM and DEM errors of hg_mpm_sync / hg_dem_sync. */
+hg_status hg_hybrid_sync(hg_hybrid *h);
+/* The continuum and grain handles (owned by the hybrid handle) for the
+ * download, kernel-time and profiling calls; stepping them on their own
+ * while they are part of a hybrid handle is not supported. */
+hg_status hg_hybrid_mpm(hg_hybrid *h, hg_mpm **out);
+hg_status hg_hybrid_dem(hg_hybrid *h, hg_dem **out);
+/* Per grain, 1 if the last step coupled it to the continuum
+ * (HybridCoupling2D's dem_particle_coupled). */
+hg_status hg_hybrid_download_coupling(hg_hybrid *h, uint8_t *coupled);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,5 +18,6 @@ from ._lib import (  # noqa: F401
 )
 from .mpm import GpuMpm, MpmScene  # noqa: F401
 from .dem import DemScene, GpuDem  # noqa: F401
+from .hybrid import GpuHybrid  # noqa: F401
 
 __version__ = "0.1.0"

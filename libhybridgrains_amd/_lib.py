@@ -144,6 +144,15 @@ def lib() -> C.CDLL:
         "hg_dem_set_profiling": ([vp, C.c_int32], C.c_int32),
         "hg_dem_kernel_times": (
             [vp, C.c_int32, C.c_char_p, _dp, C.POINTER(C.c_int64), _ip], C.c_int32),
+        "hg_hybrid_create": (
+            [vp, C.POINTER(MpmDesc), C.POINTER(DemDesc), _dp, C.POINTER(vp)], C.c_int32),
+        "hg_hybrid_destroy": ([vp], C.c_int32),
+        "hg_hybrid_step": ([vp, C.c_double, C.c_uint32], C.c_int32),
+        "hg_hybrid_step_graph": ([vp, C.c_double, C.c_uint32], C.c_int32),
+        "hg_hybrid_sync": ([vp], C.c_int32),
+        "hg_hybrid_mpm": ([vp, C.POINTER(vp)], C.c_int32),
+        "hg_hybrid_dem": ([vp, C.POINTER(vp)], C.c_int32),
+        "hg_hybrid_download_coupling": ([vp, C.POINTER(C.c_uint8)], C.c_int32),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)

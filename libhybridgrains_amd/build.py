@@ -33,7 +33,7 @@ HIP_FLAGS = [
     "-Wno-unused-value",
 ]
 
-HIP_SOURCES = ["hg_ctx.hip", "hg_sort.hip", "hg_mpm.hip", "hg_dem.hip"]
+HIP_SOURCES = ["hg_ctx.hip", "hg_sort.hip", "hg_mpm.hip", "hg_dem.hip", "hg_hybrid.hip"]
 
 
 def _run(cmd: list[str], cwd: Path | None = None) -> None:

@@ -24,7 +24,7 @@
 // gathers its forces in the reference's summation order -- gravity, planes,
 // pairs owned by smaller partners, own pairs, kinematic pairs -- then
 // symplectic Euler, then the new cell key), and the counting sort.
-#include "hg_internal.h"
+#include "hg_dem_handle.h"
 #include "hg_sort.h"
 
 #include <algorithm>
@@ -33,52 +33,6 @@
 #include <memory>
 
 namespace hgdem {
-
-constexpr int kMaxPlanes = 8;
-constexpr int kMaxOwned = 8;
-constexpr int kMaxIn = 8;
-constexpr uint32_t kKinematic = 0x80000000u;
-constexpr int kThreads = 256;
-
-struct SceneDev {
-  double g0, g1;
-  double kn, kt, gamman, gammat, mu;
-  double dt;
-  double cmin0, cmin1, cw;  // broad-phase grid origin and cell width
-  int32_t ncx, ncy;
-  uint32_t b_off, b_cnt, cell_off;
-  int32_t nplanes;
-  int32_t weighted;
-  double plane_x[2 * kMaxPlanes];
-  double plane_n[2 * kMaxPlanes];
-};
-
-struct BodiesRaw {
-  double *x, *y, *th;
-  void *vx, *vy, *om;
-  double *r;
-  void *im, *iI, *m;
-  uint8_t *fixed;
-  void *w;
-};
-
-struct SlotsRaw {
-  uint32_t *okey;  // [B][kMaxOwned]
-  void *ods;       // [B][kMaxOwned][2]
-  uint8_t *ocnt;   // [B]
-  uint8_t *pact;   // [B][kMaxPlanes]
-  void *pds;       // [B][kMaxPlanes][2]
-};
-
-struct ForceRaw {
-  void *oF;     // [B][kMaxOwned][2]
-  void *otau0;  // [B][kMaxOwned]
-  void *otau1;  // [B][kMaxOwned]
-  void *pF;     // [B][kMaxPlanes][2]
-  void *ptau;   // [B][kMaxPlanes]
-  uint32_t *ikey;  // [B][kMaxIn] owners of contacts with this body
-  uint8_t *icnt;
-};
 
 template <typename T>
 __device__ __forceinline__ T ld(const void *p, uint32_t i) {
@@ -360,6 +314,7 @@ struct IntegrateArgs {
   uint32_t *key, *slot, *counts;
   int32_t *err;
   double *F_out;  // optional: total force per body (3B)
+  int32_t rekey;  // produce the next step's cell keys
 };
 
 // Gathers a body's forces in the reference's summation order, then
@@ -426,6 +381,7 @@ __global__ __launch_bounds__(kThreads) void dem_integrate_kernel(IntegrateArgs a
   a.b.y[i] = y;
   a.b.th[i] = a.b.th[i] + dt * double(tv2);
   if (!(isfinite(x) && isfinite(y) && isfinite(v2))) atomicOr(&a.err[sid], 2);
+  if (!a.rekey) return;
   int cx, cy;
   cell_of(sc, x, y, cx, cy);
   const uint32_t k = sc.cell_off + uint32_t(cy) * uint32_t(sc.ncx) + uint32_t(cx);
@@ -459,30 +415,6 @@ using namespace hgdem;
 // ===========================================================================
 // host side
 // ===========================================================================
-
-struct hg_dem {
-  hg_ctx *ctx = nullptr;
-  hipStream_t stream = nullptr;
-  int precision = HG_FP64;
-  std::vector<SceneDev> scenes;
-  uint32_t nscenes = 0, nbodies = 0, ncell = 0, nscan = 0;
-  SceneDev *d_scenes = nullptr;
-  uint32_t *d_b_off = nullptr;
-  BodiesRaw b = {};
-  SlotsRaw slots[2] = {};
-  int cur = 0;  // slots[cur] holds the last step's contacts
-  ForceRaw f = {};
-  uint32_t *perm = nullptr, *key = nullptr, *slot = nullptr, *counts = nullptr,
-           *cell_start = nullptr, *block_sums = nullptr;
-  int32_t *err = nullptr;
-  double *F_out = nullptr;
-  bool keep_force = false;
-  double dt_dev = -1.0;
-  KernelProfiler prof;
-  hipGraphExec_t graph = nullptr;
-  std::vector<void *> allocs;
-  size_t tsize() const { return precision == HG_FP64 ? 8 : 4; }
-};
 
 template <typename P>
 static hg_status dalloc(hg_dem *h, P **out, size_t bytes) {
@@ -522,7 +454,7 @@ static void launch_contacts_t(hg_dem *h) {
 }
 
 template <typename T>
-static void launch_integrate_t(hg_dem *h) {
+static void launch_integrate_t(hg_dem *h, bool rekey) {
   IntegrateArgs a;
   a.scenes = h->d_scenes;
   a.b_off = h->d_b_off;
@@ -536,25 +468,49 @@ static void launch_integrate_t(hg_dem *h) {
   a.counts = h->counts;
   a.err = h->err;
   a.F_out = h->keep_force ? h->F_out : nullptr;
+  a.rekey = rekey ? 1 : 0;
   ProfScope ps(h->prof, "dem_integrate", h->stream);
   hipLaunchKernelGGL(dem_integrate_kernel<T>, dim3((h->nbodies + kThreads - 1) / kThreads),
                      dim3(kThreads), 0, h->stream, a);
 }
 
-static void launch_step(hg_dem *h) {
+void hg_dem_launch_step(hg_dem *h, bool rekey) {
   if (h->nbodies) {
     if (h->precision == HG_FP64) {
       launch_contacts_t<double>(h);
-      launch_integrate_t<double>(h);
+      launch_integrate_t<double>(h, rekey);
     } else {
       launch_contacts_t<float>(h);
-      launch_integrate_t<float>(h);
+      launch_integrate_t<float>(h, rekey);
     }
   }
   h->cur = 1 - h->cur;
+  if (rekey)
+    hg_counting_sort(h->prof, h->stream, h->counts, h->ncell, h->block_sums, h->nscan,
+                     h->cell_start, h->key, h->slot, h->nbodies, h->perm);
+}
+
+void hg_dem_launch_rebin(hg_dem *h) {
+  if (h->nbodies) {
+    BinArgs a;
+    a.scenes = h->d_scenes;
+    a.b_off = h->d_b_off;
+    a.nscenes = h->nscenes;
+    a.nbodies = h->nbodies;
+    a.x = h->b.x;
+    a.y = h->b.y;
+    a.key = h->key;
+    a.slot = h->slot;
+    a.counts = h->counts;
+    ProfScope ps(h->prof, "dem_bin", h->stream);
+    hipLaunchKernelGGL(dem_bin_kernel, dim3((h->nbodies + kThreads - 1) / kThreads), dim3(kThreads),
+                       0, h->stream, a);
+  }
   hg_counting_sort(h->prof, h->stream, h->counts, h->ncell, h->block_sums, h->nscan,
                    h->cell_start, h->key, h->slot, h->nbodies, h->perm);
 }
+
+static void launch_step(hg_dem *h) { hg_dem_launch_step(h, true); }
 
 static hg_status dem_upload_dt(hg_dem *h, double dt) {
   if (dt == h->dt_dev) return HG_OK;
@@ -762,22 +718,7 @@ static hg_status dem_create_impl(hg_ctx *ctx, const hg_dem_desc *descs, uint32_t
   if (h->precision == HG_FP64) HG_TRY(upload_bodies<double>(hp, descs));
   else HG_TRY(upload_bodies<float>(hp, descs));
   HG_HIP(ctx, hipDeviceSynchronize());
-  if (h->nbodies) {
-    BinArgs a;
-    a.scenes = h->d_scenes;
-    a.b_off = h->d_b_off;
-    a.nscenes = nscenes;
-    a.nbodies = h->nbodies;
-    a.x = h->b.x;
-    a.y = h->b.y;
-    a.key = h->key;
-    a.slot = h->slot;
-    a.counts = h->counts;
-    hipLaunchKernelGGL(dem_bin_kernel, dim3((h->nbodies + kThreads - 1) / kThreads), dim3(kThreads), 0,
-                       h->stream, a);
-  }
-  hg_counting_sort(h->prof, h->stream, h->counts, h->ncell, h->block_sums, h->nscan, h->cell_start,
-                   h->key, h->slot, h->nbodies, h->perm);
+  hg_dem_launch_rebin(hp);
   HG_HIP(ctx, hipGetLastError());
   HG_HIP(ctx, hipStreamSynchronize(h->stream));
   *out = h.release();
@@ -809,7 +750,7 @@ extern "C" hg_status hg_dem_destroy(hg_dem *h) {
   if (h->stream) hipStreamSynchronize(h->stream);
   if (h->graph) hipGraphExecDestroy(h->graph);
   for (void *p : h->allocs) hipFree(p);
-  if (h->stream) hipStreamDestroy(h->stream);
+  if (h->stream && h->own_stream) hipStreamDestroy(h->stream);
   delete h;
   return HG_OK;
 }

@@ -1,0 +1,91 @@
+// hg_dem_handle.h -- DEM device layout and the handle (host side), shared
+// with the hybrid module.
+#pragma once
+
+#include "hg_internal.h"
+
+namespace hgdem {
+
+constexpr int kMaxPlanes = 8;
+constexpr int kMaxOwned = 8;
+constexpr int kMaxIn = 8;
+constexpr uint32_t kKinematic = 0x80000000u;
+constexpr int kThreads = 256;
+
+struct SceneDev {
+  double g0, g1;
+  double kn, kt, gamman, gammat, mu;
+  double dt;
+  double cmin0, cmin1, cw;  // broad-phase grid origin and cell width
+  int32_t ncx, ncy;
+  uint32_t b_off, b_cnt, cell_off;
+  int32_t nplanes;
+  int32_t weighted;
+  double plane_x[2 * kMaxPlanes];
+  double plane_n[2 * kMaxPlanes];
+};
+
+struct BodiesRaw {
+  double *x, *y, *th;
+  void *vx, *vy, *om;
+  double *r;
+  void *im, *iI, *m;
+  uint8_t *fixed;
+  void *w;
+};
+
+struct SlotsRaw {
+  uint32_t *okey;  // [B][kMaxOwned]
+  void *ods;       // [B][kMaxOwned][2]
+  uint8_t *ocnt;   // [B]
+  uint8_t *pact;   // [B][kMaxPlanes]
+  void *pds;       // [B][kMaxPlanes][2]
+};
+
+struct ForceRaw {
+  void *oF;     // [B][kMaxOwned][2]
+  void *otau0;  // [B][kMaxOwned]
+  void *otau1;  // [B][kMaxOwned]
+  void *pF;     // [B][kMaxPlanes][2]
+  void *ptau;   // [B][kMaxPlanes]
+  uint32_t *ikey;  // [B][kMaxIn] owners of contacts with this body
+  uint8_t *icnt;
+};
+
+}  // namespace hgdem
+
+struct hg_dem {
+  using SceneDev = hgdem::SceneDev;
+  using BodiesRaw = hgdem::BodiesRaw;
+  using SlotsRaw = hgdem::SlotsRaw;
+  using ForceRaw = hgdem::ForceRaw;
+  hg_ctx *ctx = nullptr;
+  hipStream_t stream = nullptr;
+  bool own_stream = true;
+  int precision = HG_FP64;
+  std::vector<SceneDev> scenes;
+  uint32_t nscenes = 0, nbodies = 0, ncell = 0, nscan = 0;
+  SceneDev *d_scenes = nullptr;
+  uint32_t *d_b_off = nullptr;
+  BodiesRaw b = {};
+  SlotsRaw slots[2] = {};
+  int cur = 0;  // slots[cur] holds the last step's contacts
+  ForceRaw f = {};
+  uint32_t *perm = nullptr, *key = nullptr, *slot = nullptr, *counts = nullptr,
+           *cell_start = nullptr, *block_sums = nullptr;
+  int32_t *err = nullptr;
+  double *F_out = nullptr;
+  bool keep_force = false;
+  double dt_dev = -1.0;
+  KernelProfiler prof;
+  hipGraphExec_t graph = nullptr;
+  std::vector<void *> allocs;
+  size_t tsize() const { return precision == HG_FP64 ? 8 : 4; }
+};
+
+
+// Launches one DEM step (contacts, integrate); with rekey false the
+// integrate kernel does not produce cell keys and the caller re-bins with
+// hg_dem_launch_rebin after editing positions.
+void hg_dem_launch_step(hg_dem *h, bool rekey);
+void hg_dem_launch_rebin(hg_dem *h);

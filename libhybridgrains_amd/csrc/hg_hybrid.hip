@@ -1,0 +1,496 @@
+// hg_hybrid.hip -- the hybridgrains2dnew node-node coupled step on gfx950.
+//
+// Replaces HybridGrains2DSim::stepSystem_constraintSetUp_prediction_correction_
+// advection_node_node (hybridgrains2dnew/HybridGrains2DSim.cpp:116-227) and
+// HybridCoupling2D (Coupling.cpp:22-591); the periodic zone resampling
+// (updateHybridState) is outside the explicit step and not part of it.
+//
+// One MPM scene and one DEM scene share a stream.  The grain-side grid
+// buffers (grain mass, grain momentum, post-exchange velocity and velocity
+// increment) live on the MPM node storage; every node a coupled grain touches
+// is appended once to a list (a stamp per node), and the list's nodes are
+// cleared again at the end of the step, so nothing is swept over the whole
+// grid and the step replays correctly from a hipGraph.
+#include "hg_dem_handle.h"
+#include "hg_mpm_handle.h"
+
+#include <memory>
+
+namespace hghyb {
+
+constexpr int kThreads = 256;
+
+struct GrainGrid {
+  double *dmass;   // [G] grain mass per node
+  double *dmom;    // [2G] grain momentum per node
+  double *vafter;  // [2G] node velocity after the exchange
+  double *aafter;  // [2G] velocity increment lambda / m_dem
+  uint32_t *stamp; // [G] 1 while the node is on the list
+  uint32_t *list;  // touched nodes
+  uint32_t *count;
+  uint32_t max_list;
+};
+
+struct HybArgs {
+  const hgmpm::SceneDev *msc;  // MPM scene 0
+  const uint8_t *tile_active;
+  double *gm, *gpn;
+  int32_t tw, th;
+  double hl;  // grains use the first point's half width (Coupling.cpp:31)
+  hgdem::BodiesRaw b;
+  uint32_t nbodies;
+  double *prex, *prey;
+  uint8_t *coupled;
+  GrainGrid g;
+  int32_t *err;
+};
+
+template <typename T>
+__device__ __forceinline__ T ld(const void *p, uint32_t i) {
+  return static_cast<const T *>(p)[i];
+}
+
+// Stencil of a grain at (x, y) on the MPM grid, as node index ranges that may
+// reach off the grid; false if the coordinates are not finite or absurd.
+template <int BASIS>
+__device__ __forceinline__ bool grain_stencil(const hgmpm::SceneDev &sc, double x, double y,
+                                              double hl, int &lx, int &hx, int &ly, int &hy) {
+  double l0, h0, l1, h1;
+  hgmpm::stencil_1d<BASIS>(x, hl, sc.min0, sc.h, l0, h0);
+  hgmpm::stencil_1d<BASIS>(y, hl, sc.min1, sc.h, l1, h1);
+  const double lim = 1.0e9;
+  if (!(fabs(l0) < lim && fabs(h0) < lim && fabs(l1) < lim && fabs(h1) < lim)) return false;
+  lx = int(l0);
+  hx = int(h0);
+  ly = int(l1);
+  hy = int(h1);
+  return true;
+}
+
+template <int BASIS>
+__device__ __forceinline__ double grain_weight(const hgmpm::SceneDev &sc, double x, double y,
+                                               double hl, int ix, int iy, int lx, int hx, int ly,
+                                               int hy) {
+  double ax, dx, ay, dy;
+  hgmpm::basis_factor<double, BASIS>(x, hl, sc.min0, sc.h, sc.inv_h, ix, ix - lx, hx - lx + 1, ax,
+                                     dx);
+  hgmpm::basis_factor<double, BASIS>(y, hl, sc.min1, sc.h, sc.inv_h, iy, iy - ly, hy - ly + 1, ay,
+                                     dy);
+  if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) return (ax * (1.0 / (4.0 * hl * hl))) * ay;
+  return ax * ay;
+}
+
+__device__ __forceinline__ bool node_active(const HybArgs &a, int ix, int iy) {
+  const hgmpm::SceneDev &sc = *a.msc;
+  const uint32_t tile = sc.tile_off + uint32_t(iy / a.th) * uint32_t(sc.tiles_x) + uint32_t(ix / a.tw);
+  return a.tile_active[tile] != 0;
+}
+
+__global__ __launch_bounds__(kThreads) void hyb_copy_pre(HybArgs a) {
+  const uint32_t b = blockIdx.x * kThreads + threadIdx.x;
+  if (b == 0) *a.g.count = 0u;
+  if (b >= a.nbodies) return;
+  a.prex[b] = a.b.x[b];
+  a.prey[b] = a.b.y[b];
+}
+
+// HybridCoupling2D::rasterizeGrainMass (Coupling.cpp:22-129): a grain is
+// coupled when every on-grid node of its stencil (at its pre-step position)
+// carries more than 1e-6 of continuum mass; its mass is then spread with the
+// MPM weights.
+template <typename T, int BASIS>
+__global__ __launch_bounds__(kThreads) void hyb_grain_mass(HybArgs a) {
+  const uint32_t b = blockIdx.x * kThreads + threadIdx.x;
+  if (b >= a.nbodies) return;
+  a.coupled[b] = 0;
+  if (a.b.fixed[b]) return;
+  const hgmpm::SceneDev &sc = *a.msc;
+  const double x = a.prex[b], y = a.prey[b];
+  int lx, hx, ly, hy;
+  if (!grain_stencil<BASIS>(sc, x, y, a.hl, lx, hx, ly, hy)) return;
+  bool all_pos = true;
+  for (int iy = ly; iy <= hy; ++iy)
+    for (int ix = lx; ix <= hx; ++ix) {
+      if (ix < 0 || iy < 0 || ix >= sc.nnx || iy >= sc.nny) continue;
+      const uint32_t node = sc.node_off + uint32_t(iy) * uint32_t(sc.snx) + uint32_t(ix);
+      const double m = node_active(a, ix, iy) ? a.gm[node] : 0.0;
+      if (m <= 1.0e-6) all_pos = false;
+    }
+  if (!all_pos) return;
+  a.coupled[b] = 1;
+  const double mb = double(ld<T>(a.b.m, b));
+  for (int iy = ly; iy <= hy; ++iy)
+    for (int ix = lx; ix <= hx; ++ix) {
+      if (ix < 0 || iy < 0 || ix >= sc.nnx || iy >= sc.nny) continue;
+      const uint32_t n = uint32_t(iy) * uint32_t(sc.snx) + uint32_t(ix);
+      const double w = grain_weight<BASIS>(sc, x, y, a.hl, ix, iy, lx, hx, ly, hy);
+      atomicAdd(&a.g.dmass[n], w * mb);
+      if (atomicExch(&a.g.stamp[n], 1u) == 0u) {
+        const uint32_t k = atomicAdd(a.g.count, 1u);
+        if (k < a.g.max_list) a.g.list[k] = n;
+        else atomicOr(a.err, 8);
+      }
+    }
+}
+
+// HybridCoupling2D::rasterizeGrainMom (Coupling.cpp:287-466): post-step grain
+// velocity, pre-step position.
+template <typename T, int BASIS>
+__global__ __launch_bounds__(kThreads) void hyb_grain_mom(HybArgs a) {
+  const uint32_t b = blockIdx.x * kThreads + threadIdx.x;
+  if (b >= a.nbodies || !a.coupled[b]) return;
+  const hgmpm::SceneDev &sc = *a.msc;
+  const double x = a.prex[b], y = a.prey[b];
+  int lx, hx, ly, hy;
+  grain_stencil<BASIS>(sc, x, y, a.hl, lx, hx, ly, hy);
+  const double mb = double(ld<T>(a.b.m, b));
+  const double pm0 = mb * double(ld<T>(a.b.vx, b)), pm1 = mb * double(ld<T>(a.b.vy, b));
+  for (int iy = ly; iy <= hy; ++iy)
+    for (int ix = lx; ix <= hx; ++ix) {
+      if (ix < 0 || iy < 0 || ix >= sc.nnx || iy >= sc.nny) continue;
+      const uint32_t n = uint32_t(iy) * uint32_t(sc.snx) + uint32_t(ix);
+      const double w = grain_weight<BASIS>(sc, x, y, a.hl, ix, iy, lx, hx, ly, hy);
+      atomicAdd(&a.g.dmom[2 * n], w * pm0);
+      atomicAdd(&a.g.dmom[2 * n + 1], w * pm1);
+    }
+}
+
+// HybridCoupling2D::updateVelocitiesOnGrid (Coupling.cpp:533-579): the
+// multiplier that equalises the two velocities at a shared node while
+// conserving its momentum.
+__global__ __launch_bounds__(kThreads) void hyb_exchange(HybArgs a) {
+  const uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t cnt = min(*a.g.count, a.g.max_list);
+  if (k >= cnt) return;
+  const uint32_t n = a.g.list[k];
+  const double m_dem = a.g.dmass[n];
+  if (!(m_dem > 0.0)) return;
+  const uint32_t node = a.msc->node_off + n;
+  const double m_mpm = a.gm[node];
+  const double msum = m_mpm + m_dem;
+  const double l0 = (m_dem * a.gpn[2 * node] - m_mpm * a.g.dmom[2 * n]) / msum;
+  const double l1 = (m_dem * a.gpn[2 * node + 1] - m_mpm * a.g.dmom[2 * n + 1]) / msum;
+  a.gpn[2 * node] -= l0;
+  a.gpn[2 * node + 1] -= l1;
+  a.g.vafter[2 * n] = (a.g.dmom[2 * n] + l0) / m_dem;
+  a.g.vafter[2 * n + 1] = (a.g.dmom[2 * n + 1] + l1) / m_dem;
+  a.g.aafter[2 * n] = l0 / m_dem;
+  a.g.aafter[2 * n + 1] = l1 / m_dem;
+}
+
+// HybridCoupling2D::transferToGrain (Coupling.cpp:468-527), FLIP with alpha 1.
+template <typename T, int BASIS>
+__global__ __launch_bounds__(kThreads) void hyb_transfer(HybArgs a) {
+  const uint32_t b = blockIdx.x * kThreads + threadIdx.x;
+  if (b >= a.nbodies || !a.coupled[b]) return;
+  const hgmpm::SceneDev &sc = *a.msc;
+  const double x = a.prex[b], y = a.prey[b];
+  int lx, hx, ly, hy;
+  grain_stencil<BASIS>(sc, x, y, a.hl, lx, hx, ly, hy);
+  double vp0 = 0.0, vp1 = 0.0, a0 = 0.0, a1 = 0.0;
+  for (int iy = ly; iy <= hy; ++iy)
+    for (int ix = lx; ix <= hx; ++ix) {
+      if (ix < 0 || iy < 0 || ix >= sc.nnx || iy >= sc.nny) continue;
+      const uint32_t n = uint32_t(iy) * uint32_t(sc.snx) + uint32_t(ix);
+      const double w = grain_weight<BASIS>(sc, x, y, a.hl, ix, iy, lx, hx, ly, hy);
+      vp0 += w * a.g.vafter[2 * n];
+      vp1 += w * a.g.vafter[2 * n + 1];
+      a0 += w * a.g.aafter[2 * n];
+      a1 += w * a.g.aafter[2 * n + 1];
+    }
+  const double vf0 = double(ld<T>(a.b.vx, b)) + a0, vf1 = double(ld<T>(a.b.vy, b)) + a1;
+  static_cast<T *>(a.b.vx)[b] = T(vp0 * (1.0 - 1.0) + vf0 * 1.0);
+  static_cast<T *>(a.b.vy)[b] = T(vp1 * (1.0 - 1.0) + vf1 * 1.0);
+}
+
+// HybridCoupling2D::updatePositions_node_node (Coupling.cpp:581-591)
+template <typename T>
+__global__ __launch_bounds__(kThreads) void hyb_positions(HybArgs a, double dt) {
+  const uint32_t b = blockIdx.x * kThreads + threadIdx.x;
+  if (b >= a.nbodies || !a.coupled[b]) return;
+  a.b.x[b] = a.prex[b] + dt * double(ld<T>(a.b.vx, b));
+  a.b.y[b] = a.prey[b] + dt * double(ld<T>(a.b.vy, b));
+}
+
+__global__ __launch_bounds__(kThreads) void hyb_clear(HybArgs a) {
+  const uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t cnt = min(*a.g.count, a.g.max_list);
+  if (k >= cnt) return;
+  const uint32_t n = a.g.list[k];
+  a.g.dmass[n] = 0.0;
+  a.g.dmom[2 * n] = 0.0;
+  a.g.dmom[2 * n + 1] = 0.0;
+  a.g.vafter[2 * n] = 0.0;
+  a.g.vafter[2 * n + 1] = 0.0;
+  a.g.aafter[2 * n] = 0.0;
+  a.g.aafter[2 * n + 1] = 0.0;
+  a.g.stamp[n] = 0u;
+}
+
+}  // namespace hghyb
+
+using namespace hghyb;
+
+struct hg_hybrid {
+  hg_ctx *ctx = nullptr;
+  hg_mpm *mpm = nullptr;
+  hg_dem *dem = nullptr;
+  hipStream_t stream = nullptr;
+  GrainGrid g = {};
+  double *prex = nullptr, *prey = nullptr;
+  uint8_t *coupled = nullptr;
+  int32_t *err = nullptr;
+  hipGraphExec_t graph = nullptr;
+  double graph_dt = -1.0;
+  std::vector<void *> allocs;
+};
+
+template <typename P>
+static hg_status halloc(hg_hybrid *h, P **out, size_t bytes) {
+  void *p = nullptr;
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess)
+    return hg_fail(h->ctx, HG_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  h->allocs.push_back(p);
+  *out = static_cast<P *>(p);
+  return HG_OK;
+}
+
+#define HG_TRY(expr)            \
+  do {                          \
+    hg_status s_ = (expr);      \
+    if (s_ != HG_OK) return s_; \
+  } while (0)
+
+static HybArgs make_args(hg_hybrid *h) {
+  HybArgs a;
+  a.msc = h->mpm->d_scenes;
+  a.tile_active = h->mpm->tile_active;
+  a.gm = h->mpm->gm;
+  a.gpn = h->mpm->gpn;
+  a.tw = h->mpm->tw();
+  a.th = h->mpm->th();
+  a.hl = h->mpm->hl_first;
+  a.b = h->dem->b;
+  a.nbodies = h->dem->nbodies;
+  a.prex = h->prex;
+  a.prey = h->prey;
+  a.coupled = h->coupled;
+  a.g = h->g;
+  a.err = h->err;
+  return a;
+}
+
+template <typename T, int BASIS>
+static void launch_hybrid_step(hg_hybrid *h, double dt) {
+  HybArgs a = make_args(h);
+  const dim3 gb((std::max(a.nbodies, 1u) + kThreads - 1) / kThreads), tb(kThreads);
+  const dim3 gl((h->g.max_list + kThreads - 1) / kThreads);
+  KernelProfiler &prof = h->mpm->prof;
+  hipStream_t s = h->stream;
+  {
+    ProfScope ps(prof, "hyb_copy_pre", s);
+    hipLaunchKernelGGL(hyb_copy_pre, gb, tb, 0, s, a);
+  }
+  hg_mpm_launch_phase0(h->mpm, dt);
+  {
+    ProfScope ps(prof, "hyb_grain_mass", s);
+    hipLaunchKernelGGL((hyb_grain_mass<T, BASIS>), gb, tb, 0, s, a);
+  }
+  hg_dem_launch_step(h->dem, false);
+  {
+    ProfScope ps(prof, "hyb_grain_mom", s);
+    hipLaunchKernelGGL((hyb_grain_mom<T, BASIS>), gb, tb, 0, s, a);
+  }
+  {
+    ProfScope ps(prof, "hyb_exchange", s);
+    hipLaunchKernelGGL(hyb_exchange, gl, tb, 0, s, a);
+  }
+  {
+    ProfScope ps(prof, "hyb_transfer", s);
+    hipLaunchKernelGGL((hyb_transfer<T, BASIS>), gb, tb, 0, s, a);
+  }
+  hg_mpm_launch_phase2(h->mpm);
+  {
+    ProfScope ps(prof, "hyb_positions", s);
+    hipLaunchKernelGGL(hyb_positions<T>, gb, tb, 0, s, a, dt);
+  }
+  hg_dem_launch_rebin(h->dem);
+  {
+    ProfScope ps(prof, "hyb_clear", s);
+    hipLaunchKernelGGL(hyb_clear, gl, tb, 0, s, a);
+  }
+}
+
+static void hybrid_step(hg_hybrid *h, double dt) {
+  const bool f64 = h->mpm->precision == HG_FP64;
+  switch (h->mpm->basis) {
+    case HG_BASIS_LINEAR:
+      f64 ? launch_hybrid_step<double, HG_BASIS_LINEAR>(h, dt)
+          : launch_hybrid_step<float, HG_BASIS_LINEAR>(h, dt);
+      break;
+    case HG_BASIS_CUBIC:
+      f64 ? launch_hybrid_step<double, HG_BASIS_CUBIC>(h, dt)
+          : launch_hybrid_step<float, HG_BASIS_CUBIC>(h, dt);
+      break;
+    default:
+      f64 ? launch_hybrid_step<double, HG_BASIS_UGIMP_LINEAR>(h, dt)
+          : launch_hybrid_step<float, HG_BASIS_UGIMP_LINEAR>(h, dt);
+      break;
+  }
+}
+
+extern "C" hg_status hg_hybrid_destroy(hg_hybrid *h);
+
+extern "C" hg_status hg_hybrid_create(hg_ctx *ctx, const hg_mpm_desc *mpm_desc,
+                                      const hg_dem_desc *dem_desc, const double *dem_weights,
+                                      hg_hybrid **out) {
+  if (!ctx || !mpm_desc || !dem_desc || !out) return hg_fail(ctx, HG_ERR_INVALID, "null argument");
+  if (mpm_desc->precision != dem_desc->precision)
+    return hg_fail(ctx, HG_ERR_INVALID, "MPM and DEM precision must match");
+  std::unique_ptr<hg_hybrid> h(new hg_hybrid());
+  h->ctx = ctx;
+  HG_TRY(hg_mpm_create(ctx, mpm_desc, &h->mpm));
+  hg_status st = hg_dem_create(ctx, dem_desc, &h->dem);
+  if (st != HG_OK) {
+    hg_mpm_destroy(h->mpm);
+    return st;
+  }
+  // one stream for both: the phases interleave
+  HG_HIP(ctx, hipStreamSynchronize(h->dem->stream));
+  HG_HIP(ctx, hipStreamDestroy(h->dem->stream));
+  h->dem->stream = h->mpm->stream;
+  h->dem->own_stream = false;
+  h->stream = h->mpm->stream;
+  hg_hybrid *hp = h.get();
+  if (dem_weights) {
+    st = hg_dem_set_weights(h->dem, 0, dem_weights);
+    if (st != HG_OK) {
+      hg_hybrid_destroy(h.release());
+      return st;
+    }
+  }
+  const uint32_t G = h->mpm->nnode;
+  const int S = h->mpm->basis == HG_BASIS_LINEAR ? 2 : (h->mpm->basis == HG_BASIS_CUBIC ? 4 : 3);
+  const uint64_t ml = std::min<uint64_t>(uint64_t(G), uint64_t(S * S) * std::max(h->dem->nbodies, 1u));
+  h->g.max_list = uint32_t(ml);
+  const uint32_t B = std::max(h->dem->nbodies, 1u);
+  auto fail = [&](hg_status e) {
+    hg_hybrid_destroy(h.release());
+    return e;
+  };
+  hg_status e;
+  if ((e = halloc(hp, &h->g.dmass, size_t(G) * 8)) != HG_OK) return fail(e);
+  if ((e = halloc(hp, &h->g.dmom, size_t(G) * 16)) != HG_OK) return fail(e);
+  if ((e = halloc(hp, &h->g.vafter, size_t(G) * 16)) != HG_OK) return fail(e);
+  if ((e = halloc(hp, &h->g.aafter, size_t(G) * 16)) != HG_OK) return fail(e);
+  if ((e = halloc(hp, &h->g.stamp, size_t(G) * 4)) != HG_OK) return fail(e);
+  if ((e = halloc(hp, &h->g.list, size_t(h->g.max_list) * 4)) != HG_OK) return fail(e);
+  if ((e = halloc(hp, &h->g.count, 4)) != HG_OK) return fail(e);
+  if ((e = halloc(hp, &h->prex, size_t(B) * 8)) != HG_OK) return fail(e);
+  if ((e = halloc(hp, &h->prey, size_t(B) * 8)) != HG_OK) return fail(e);
+  if ((e = halloc(hp, &h->coupled, size_t(B))) != HG_OK) return fail(e);
+  if ((e = halloc(hp, &h->err, 4)) != HG_OK) return fail(e);
+  hipMemset(h->g.dmass, 0, size_t(G) * 8);
+  hipMemset(h->g.dmom, 0, size_t(G) * 16);
+  hipMemset(h->g.vafter, 0, size_t(G) * 16);
+  hipMemset(h->g.aafter, 0, size_t(G) * 16);
+  hipMemset(h->g.stamp, 0, size_t(G) * 4);
+  hipMemset(h->g.count, 0, 4);
+  hipMemset(h->coupled, 0, B);
+  hipMemset(h->err, 0, 4);
+  if (hipDeviceSynchronize() != hipSuccess) return fail(hg_fail(ctx, HG_ERR_HIP, "hybrid init"));
+  *out = h.release();
+  return HG_OK;
+}
+
+extern "C" hg_status hg_hybrid_destroy(hg_hybrid *h) {
+  if (!h) return HG_OK;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->graph) hipGraphExecDestroy(h->graph);
+  for (void *p : h->allocs) hipFree(p);
+  if (h->dem) hg_dem_destroy(h->dem);
+  if (h->mpm) hg_mpm_destroy(h->mpm);
+  delete h;
+  return HG_OK;
+}
+
+extern "C" hg_status hg_hybrid_step(hg_hybrid *h, double dt, uint32_t nsteps) {
+  if (!h) return HG_ERR_INVALID;
+  if (!(dt > 0.0)) return hg_fail(h->ctx, HG_ERR_INVALID, "dt must be > 0");
+  if (h->mpm->phase_next != 0) return hg_fail(h->ctx, HG_ERR_INVALID, "a phased MPM step is in progress");
+  // the DEM reads its dt from its scene table
+  HG_TRY(hg_dem_step(h->dem, dt, 0));
+  for (uint32_t k = 0; k < nsteps; ++k) hybrid_step(h, dt);
+  HG_HIP(h->ctx, hipGetLastError());
+  return HG_OK;
+}
+
+extern "C" hg_status hg_hybrid_step_graph(hg_hybrid *h, double dt, uint32_t nsteps) {
+  if (!h) return HG_ERR_INVALID;
+  if (!(dt > 0.0)) return hg_fail(h->ctx, HG_ERR_INVALID, "dt must be > 0");
+  if (h->mpm->phase_next != 0) return hg_fail(h->ctx, HG_ERR_INVALID, "a phased MPM step is in progress");
+  if (h->mpm->prof.on || h->dem->prof.on) return hg_hybrid_step(h, dt, nsteps);
+  HG_TRY(hg_dem_step(h->dem, dt, 0));
+  if (dt != h->graph_dt && h->graph) {
+    hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  // Both solvers alternate their buffers every step and always step
+  // together here, so a graph of two steps captured with the MPM points in
+  // buffer 0 replays from any later state with the MPM points in buffer 0.
+  if (h->mpm->cur != 0 && nsteps) {
+    hybrid_step(h, dt);
+    --nsteps;
+  }
+  const uint32_t pairs = nsteps / 2;
+  if (pairs) {
+    if (!h->graph) {
+      // the MPM dt upload synchronises; do it before capturing
+      HG_TRY(hg_mpm_step(h->mpm, dt, 0));
+      hipGraph_t g = nullptr;
+      HG_HIP(h->ctx, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      hybrid_step(h, dt);
+      hybrid_step(h, dt);
+      HG_HIP(h->ctx, hipStreamEndCapture(h->stream, &g));
+      HG_HIP(h->ctx, hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0));
+      HG_HIP(h->ctx, hipGraphDestroy(g));
+      h->graph_dt = dt;
+    }
+    for (uint32_t k = 0; k < pairs; ++k) HG_HIP(h->ctx, hipGraphLaunch(h->graph, h->stream));
+  }
+  if (nsteps & 1u) hybrid_step(h, dt);
+  HG_HIP(h->ctx, hipGetLastError());
+  return HG_OK;
+}
+
+extern "C" hg_status hg_hybrid_sync(hg_hybrid *h) {
+  if (!h) return HG_ERR_INVALID;
+  HG_HIP(h->ctx, hipStreamSynchronize(h->stream));
+  int32_t e = 0;
+  HG_HIP(h->ctx, hipMemcpy(&e, h->err, 4, hipMemcpyDeviceToHost));
+  if (e & 8) return hg_fail(h->ctx, HG_ERR_STATE, "coupling node list overflow");
+  HG_TRY(hg_dem_sync(h->dem));
+  return hg_mpm_sync(h->mpm);
+}
+
+extern "C" hg_status hg_hybrid_mpm(hg_hybrid *h, hg_mpm **out) {
+  if (!h || !out) return HG_ERR_INVALID;
+  *out = h->mpm;
+  return HG_OK;
+}
+
+extern "C" hg_status hg_hybrid_dem(hg_hybrid *h, hg_dem **out) {
+  if (!h || !out) return HG_ERR_INVALID;
+  *out = h->dem;
+  return HG_OK;
+}
+
+extern "C" hg_status hg_hybrid_download_coupling(hg_hybrid *h, uint8_t *coupled) {
+  if (!h) return HG_ERR_INVALID;
+  HG_TRY(hg_hybrid_sync(h));
+  if (coupled && h->dem->nbodies)
+    HG_HIP(h->ctx, hipMemcpy(coupled, h->coupled, h->dem->nbodies, hipMemcpyDeviceToHost));
+  return HG_OK;
+}

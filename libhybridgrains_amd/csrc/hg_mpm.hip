@@ -3,7 +3,7 @@
 // Replaces mpmgrains2d/ExplicitIntegrator.cpp:5-67 and the PhysicsGrid /
 // MaterialPoints / ConstitutiveModel routines it calls.  See
 // hg_mpm_kernels.h for the HBM layout.
-#include "hg_mpm_kernels.h"
+#include "hg_mpm_handle.h"
 #include "hg_sort.h"
 
 #include <algorithm>
@@ -167,53 +167,6 @@ __device__ __forceinline__ P2GRec<T> p2g_record(const PtsRaw &pts, uint32_t p, c
   r.mv1 = r.m * v1;
   r.cnt = stencil_counts<BASIS>(r.x, r.y, double(r.hl), sc);
   return r;
-}
-
-// 1-D basis factor and its derivative for node offset k (0..S-1) from the
-// point's stencil base, masked past the stencil's count.
-template <typename T, int BASIS>
-__device__ __forceinline__ void basis_factor(double x, T hl, double gmin, double h, T inv_h,
-                                             int node, int k, int cnt, T &a, T &d) {
-  if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) {
-    const double xi = h * double(node) + gmin;
-    const T r = T(x - xi);
-    const T w = T(h);
-    const T diff = fabs(r);
-    const T sgn = r >= T(0) ? T(1) : T(-1);
-    if (diff >= w + hl || k >= cnt) {
-      a = T(0);
-      d = T(0);
-    } else if (diff >= w - hl) {
-      const T u = w + hl - diff;
-      a = u * u * (T(0.5) * inv_h);
-      d = -sgn * u * inv_h;
-    } else if (diff >= hl) {
-      a = T(2) * hl * (T(1) - diff * inv_h);
-      d = -sgn * T(2) * hl * inv_h;
-    } else {
-      a = T(2) * hl - (hl * hl + diff * diff) * inv_h;
-      d = T(2) * (-r) * inv_h;
-    }
-  } else if constexpr (BASIS == HG_BASIS_LINEAR) {
-    const T t = T((x - h * double(node) - gmin) * double(inv_h));
-    const T at = fabs(t);
-    a = at < T(1) ? T(1) - at : T(0);
-    d = (at <= T(1)) ? ((t >= T(0)) ? T(-1) : T(1)) : T(0);
-  } else {
-    const T t = T((x - h * double(node) - gmin) * double(inv_h));
-    const T at = fabs(t);
-    const T sg = (t >= T(0)) ? T(1) : T(-1);
-    if (at < T(1)) {
-      a = (T(0.5) * at - T(1)) * at * at + T(2.0 / 3.0);
-      d = sg * (T(1.5) * at - T(2)) * at;
-    } else if (at < T(2)) {
-      a = ((-at * T(1.0 / 6.0) + T(1)) * at - T(2)) * at + T(4.0 / 3.0);
-      d = sg * ((T(-0.5) * at + T(2)) * at - T(2));
-    } else {
-      a = T(0);
-      d = T(0);
-    }
-  }
 }
 
 template <typename T, int BASIS, int TW, int TH, int CAP>
@@ -713,46 +666,6 @@ __global__ __launch_bounds__(256) void mpm_bin_kernel(BinArgs a) {
 // host side
 // ===========================================================================
 
-struct SceneHost {
-  SceneDev d;
-};
-
-struct hg_mpm {
-  hg_ctx *ctx = nullptr;
-  hipStream_t stream = nullptr;
-  int precision = HG_FP64;
-  int basis = HG_BASIS_UGIMP_LINEAR;
-  int aw = 16, ah = 16;  // P2G tile of owned nodes (threads per workgroup = aw*ah)
-  std::vector<SceneDev> scenes;
-  uint32_t nscenes = 0;
-  uint32_t npoints = 0;
-  uint32_t ncell = 0, nnode = 0, ntiles = 0;
-  bool has_hl = false, has_flags = false;
-  PtsRaw buf[2] = {};
-  int cur = 0;
-  uint32_t *perm = nullptr, *pinv = nullptr, *key = nullptr, *slot = nullptr, *counts = nullptr,
-           *cell_start = nullptr, *block_sums = nullptr;
-  uint32_t nscan = 0;
-  SceneDev *d_scenes = nullptr;
-  uint32_t *d_p_off = nullptr;
-  uint32_t *d_tile_scene = nullptr;
-  uint8_t *tile_active = nullptr;
-  void *gva = nullptr;
-  double *gm = nullptr, *gp = nullptr, *gpn = nullptr, *gf = nullptr;
-  int32_t *err = nullptr;
-  bool keep_all = false;
-  uint32_t last_store = 0;
-  int phase_next = 0;  // 0: flow or phase 0 allowed; 1: phase 1; 2: phase 2
-  double dt_dev = -1.0;  // dt currently in the device scene table
-  KernelProfiler prof;
-  hipGraphExec_t graph = nullptr;
-  std::vector<void *> allocs;
-
-  size_t tsize() const { return precision == HG_FP64 ? 8 : 4; }
-  int tw() const;
-  int th() const;
-};
-
 static int stencil_width(int basis) {
   return basis == HG_BASIS_LINEAR ? 2 : (basis == HG_BASIS_CUBIC ? 4 : 3);
 }
@@ -936,6 +849,19 @@ static uint32_t fused_store(const hg_mpm *h) {
                      : kStoreVelAcc;
 }
 
+hg_status hg_mpm_launch_phase0(hg_mpm *h, double dt) {
+  HG_TRY(upload_dt(h, dt));
+  launch_p2g(h, kStoreMass | kStoreMom | kStoreMomNew | kStoreForce);
+  return HG_OK;
+}
+
+hg_status hg_mpm_launch_phase2(hg_mpm *h) {
+  launch_velacc(h);
+  h->last_store |= kStoreVelAcc;
+  launch_g2p_and_sort(h);
+  return HG_OK;
+}
+
 static void launch_step(hg_mpm *h) {
   launch_p2g(h, fused_store(h));
   launch_g2p_and_sort(h);
@@ -1041,6 +967,7 @@ static hg_status mpm_create_impl(hg_ctx *ctx, const hg_mpm_desc *descs, uint32_t
   h->precision = descs[0].precision;
   h->basis = descs[0].basis;
   h->nscenes = nscenes;
+  h->hl_first = descs[0].npoints ? descs[0].hl[0] : 0.25 * descs[0].cell_width;
   HG_HIP(ctx, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
 
   // scene tables
@@ -1168,7 +1095,7 @@ static void mpm_free(hg_mpm *h) {
   if (h->stream) hipStreamSynchronize(h->stream);
   if (h->graph) hipGraphExecDestroy(h->graph);
   for (void *p : h->allocs) hipFree(p);
-  if (h->stream) hipStreamDestroy(h->stream);
+  if (h->stream && h->own_stream) hipStreamDestroy(h->stream);
   delete h;
 }
 

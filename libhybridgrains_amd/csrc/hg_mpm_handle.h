@@ -1,0 +1,49 @@
+// hg_mpm_handle.h -- the MPM handle (host side), shared with the hybrid module.
+#pragma once
+
+#include "hg_mpm_kernels.h"
+
+struct hg_mpm {
+  using SceneDev = hgmpm::SceneDev;
+  using PtsRaw = hgmpm::PtsRaw;
+  hg_ctx *ctx = nullptr;
+  hipStream_t stream = nullptr;
+  bool own_stream = true;
+  double hl_first = 0.0;  // first point's half width (the grains' stencil width)
+  int precision = HG_FP64;
+  int basis = HG_BASIS_UGIMP_LINEAR;
+  int aw = 16, ah = 16;  // P2G tile of owned nodes (threads per workgroup = aw*ah)
+  std::vector<SceneDev> scenes;
+  uint32_t nscenes = 0;
+  uint32_t npoints = 0;
+  uint32_t ncell = 0, nnode = 0, ntiles = 0;
+  bool has_hl = false, has_flags = false;
+  PtsRaw buf[2] = {};
+  int cur = 0;
+  uint32_t *perm = nullptr, *pinv = nullptr, *key = nullptr, *slot = nullptr, *counts = nullptr,
+           *cell_start = nullptr, *block_sums = nullptr;
+  uint32_t nscan = 0;
+  SceneDev *d_scenes = nullptr;
+  uint32_t *d_p_off = nullptr;
+  uint32_t *d_tile_scene = nullptr;
+  uint8_t *tile_active = nullptr;
+  void *gva = nullptr;
+  double *gm = nullptr, *gp = nullptr, *gpn = nullptr, *gf = nullptr;
+  int32_t *err = nullptr;
+  bool keep_all = false;
+  uint32_t last_store = 0;
+  int phase_next = 0;  // 0: flow or phase 0 allowed; 1: phase 1; 2: phase 2
+  double dt_dev = -1.0;  // dt currently in the device scene table
+  KernelProfiler prof;
+  hipGraphExec_t graph = nullptr;
+  std::vector<void *> allocs;
+
+  size_t tsize() const { return precision == HG_FP64 ? 8 : 4; }
+  int tw() const;
+  int th() const;
+};
+
+
+// Launch helpers the hybrid step reuses (defined in hg_mpm.hip).
+hg_status hg_mpm_launch_phase0(hg_mpm *h, double dt);
+hg_status hg_mpm_launch_phase2(hg_mpm *h);

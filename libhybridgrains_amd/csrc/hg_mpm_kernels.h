@@ -180,4 +180,51 @@ __device__ __forceinline__ void basis_2d(double x, double y, T hl, double min0, 
   }
 }
 
+// 1-D basis factor and its derivative for node offset k (0..S-1) from the
+// point's stencil base, masked past the stencil's count.
+template <typename T, int BASIS>
+__device__ __forceinline__ void basis_factor(double x, T hl, double gmin, double h, T inv_h,
+                                             int node, int k, int cnt, T &a, T &d) {
+  if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) {
+    const double xi = h * double(node) + gmin;
+    const T r = T(x - xi);
+    const T w = T(h);
+    const T diff = fabs(r);
+    const T sgn = r >= T(0) ? T(1) : T(-1);
+    if (diff >= w + hl || k >= cnt) {
+      a = T(0);
+      d = T(0);
+    } else if (diff >= w - hl) {
+      const T u = w + hl - diff;
+      a = u * u * (T(0.5) * inv_h);
+      d = -sgn * u * inv_h;
+    } else if (diff >= hl) {
+      a = T(2) * hl * (T(1) - diff * inv_h);
+      d = -sgn * T(2) * hl * inv_h;
+    } else {
+      a = T(2) * hl - (hl * hl + diff * diff) * inv_h;
+      d = T(2) * (-r) * inv_h;
+    }
+  } else if constexpr (BASIS == HG_BASIS_LINEAR) {
+    const T t = T((x - h * double(node) - gmin) * double(inv_h));
+    const T at = fabs(t);
+    a = at < T(1) ? T(1) - at : T(0);
+    d = (at <= T(1)) ? ((t >= T(0)) ? T(-1) : T(1)) : T(0);
+  } else {
+    const T t = T((x - h * double(node) - gmin) * double(inv_h));
+    const T at = fabs(t);
+    const T sg = (t >= T(0)) ? T(1) : T(-1);
+    if (at < T(1)) {
+      a = (T(0.5) * at - T(1)) * at * at + T(2.0 / 3.0);
+      d = sg * (T(1.5) * at - T(2)) * at;
+    } else if (at < T(2)) {
+      a = ((-at * T(1.0 / 6.0) + T(1)) * at - T(2)) * at + T(4.0 / 3.0);
+      d = sg * ((T(-0.5) * at + T(2)) * at - T(2));
+    } else {
+      a = T(0);
+      d = T(0);
+    }
+  }
+}
+
 }  // namespace hgmpm

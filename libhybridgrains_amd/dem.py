@@ -168,9 +168,19 @@ class GpuDem:
     def kernel_times(self) -> dict:
         return _kernel_times(self.ctx, self.L.hg_dem_kernel_times, self._h)
 
+    @classmethod
+    def _borrow(cls, handle, scenes, ctx) -> "GpuDem":
+        """A view of a handle another object owns (the hybrid's grains)."""
+        o = cls.__new__(cls)
+        o.ctx, o.L, o.scenes = ctx, _lib.lib(), list(scenes)
+        o._h = handle
+        o._borrowed = True
+        return o
+
     def close(self) -> None:
         if getattr(self, "_h", None):
-            self.L.hg_dem_destroy(self._h)
+            if not getattr(self, "_borrowed", False):
+                self.L.hg_dem_destroy(self._h)
             self._h = None
 
     def __del__(self):

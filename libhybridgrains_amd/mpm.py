@@ -196,9 +196,19 @@ class GpuMpm:
             out[nm] = (float(ms[i]), int(launches[i]))
         return out
 
+    @classmethod
+    def _borrow(cls, handle, scenes, precision, ctx) -> "GpuMpm":
+        """A view of a handle another object owns (the hybrid's continuum)."""
+        o = cls.__new__(cls)
+        o.ctx, o.L, o.scenes, o.precision = ctx, _lib.lib(), list(scenes), precision
+        o._h = handle
+        o._borrowed = True
+        return o
+
     def close(self) -> None:
         if getattr(self, "_h", None):
-            self.L.hg_mpm_destroy(self._h)
+            if not getattr(self, "_borrowed", False):
+                self.L.hg_mpm_destroy(self._h)
             self._h = None
 
     def __del__(self):

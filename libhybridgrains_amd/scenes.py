@@ -195,3 +195,73 @@ def dem_column(
     s.add_plane([0.0, 0.0], [1.0, 0.0])
     s.add_plane([box_width, 0.0], [-1.0, 0.0])
     return s
+
+
+# ---- hybrid --------------------------------------------------------------------
+
+def hybrid_column(
+    *,
+    domain=(0.4, 0.3),
+    h: float = 0.005,
+    column=(0.1, 0.08),
+    band_cells: int = 2,
+    grain_layer: float = 0.04,
+    r_mean: float = 1.25e-3,
+    r_spread: float = 0.1,
+    rho: float = 2500.0,
+    E: float = 1.0e6,
+    phi_deg: float = 30.0,
+    kn: float = 1.0e4,
+    restitution: float = 0.3,
+    mu: float = 0.3,
+    seed: int = 0,
+):
+    """A continuum column with a layer of grains on top.
+
+    The MPM column occupies the grid's floor corner up to height
+    ``column[1]``; circles fill the band from ``band_cells`` cells below the
+    column's free surface up to ``grain_layer`` above it.  Where the two
+    overlap (the coupling band) the grains and the points carry mass weight
+    1/2, as the reference's zone resampling assigns in a transition band
+    (HybridGrains2DSim.cpp:512-531); grains whose stencil nodes all carry
+    continuum mass are coupled each step.  Returns (mpm_scene, dem_scene,
+    weights), the DEM masses already scaled by the weights
+    (RigidBody2DState::updateCurrentMassUsingMassWeights).
+    """
+    from .dem import DemScene, circle_mass
+
+    rng = np.random.default_rng(seed)
+    mpm = column_collapse(domain=domain, h=h, column=column, seed=seed, E=E, phi_deg=phi_deg)
+    y_top = 2 * h + column[1]
+    y_band = y_top - band_cells * h
+    inband = mpm.q[:, 1] > y_band
+    mpm.m[inband] *= 0.5
+    mpm.volume[inband] *= 0.5
+    spacing = 2.0 * r_mean * (1.0 + r_spread) + 1e-5
+    dy = spacing * math.sqrt(3.0) / 2.0
+    x0 = 2 * h + r_mean * (1 + r_spread)
+    pts = []
+    j = 0
+    y = y_band + r_mean
+    while y < y_top + grain_layer:
+        off = 0.5 * spacing if (j % 2) else 0.0
+        x = x0 + off
+        while x < 2 * h + column[0] - r_mean:
+            pts.append((x, y))
+            x += spacing
+        y += dy
+        j += 1
+    B = len(pts)
+    q = np.zeros((B, 3))
+    q[:, :2] = np.asarray(pts) + rng.uniform(-0.02 * r_mean, 0.02 * r_mean, size=(B, 2))
+    radius = rng.uniform(r_mean * (1 - r_spread), r_mean * (1 + r_spread), B)
+    mass0 = circle_mass(rho, radius)
+    weights = np.where(q[:, 1] < y_top, 0.5, 1.0)
+    m_pair = 0.5 * rho * math.pi * r_mean * r_mean
+    gn = damping_from_restitution(restitution, kn, m_pair)
+    dem = DemScene(q=q, v=np.zeros((B, 3)), radius=radius, mass=mass0 * weights[:, None],
+                   fixed=np.zeros(B, np.uint8), gravity=mpm.gravity.copy(), kn=kn, kt=kn,
+                   gamman=gn, gammat=gn, mu=mu)
+    dem.add_plane([0.0, 2 * h], [0.0, 1.0])
+    dem.add_plane([2 * h, 0.0], [1.0, 0.0])
+    return mpm, dem, weights

@@ -285,3 +285,47 @@ class OracleDem:
 
     def state(self) -> dict:
         return {"q": self.q.reshape(-1, 3).copy(), "v": self.v.reshape(-1, 3).copy()}
+
+
+class OracleHybridC(C.Structure):
+    _fields_ = [
+        ("mpm", C.POINTER(OracleMpmC)), ("dem", C.POINTER(OracleDemC)),
+        ("dem_pos_before", _dp), ("dem_mass_grid", _dp), ("dem_mom_grid", _dp),
+        ("v_after", _dp), ("at_after", _dp), ("homog_factor", _dp),
+        ("coupled", _u8p),
+    ]
+
+
+class OracleHybrid:
+    """HybridGrains2DSim's coupled step (no zone resampling) on the host:
+    an OracleMpm and an OracleDem (with mass weights) plus the coupling
+    buffers."""
+
+    def __init__(self, mpm_scene, dem_scene, weights):
+        self.mpm = OracleMpm(mpm_scene)
+        self.dem = OracleDem(dem_scene, weights=weights)
+        L = self.mpm.L
+        G, B = self.mpm.G, self.dem.B
+        self.dem_pos_before = np.zeros(3 * B)
+        self.dem_mass_grid = np.zeros(G)
+        self.dem_mom_grid = np.zeros(2 * G)
+        self.v_after = np.zeros(2 * G)
+        self.at_after = np.zeros(2 * G)
+        self.homog_factor = np.zeros(G)
+        self.coupled = np.zeros(B, np.uint8)
+        c = OracleHybridC()
+        c.mpm = C.pointer(self.mpm.c)
+        c.dem = C.pointer(self.dem.c)
+        for k in ("dem_pos_before", "dem_mass_grid", "dem_mom_grid", "v_after", "at_after",
+                  "homog_factor"):
+            setattr(c, k, _p(getattr(self, k)))
+        c.coupled = _p(self.coupled, C.c_uint8)
+        self.c = c
+        P = C.POINTER(OracleHybridC)
+        L.oracle_hybrid_step.argtypes = [P, C.c_double, C.c_uint32]
+        L.oracle_hybrid_step.restype = C.c_int
+        self.L = L
+
+    def step(self, dt: float, nsteps: int = 1) -> None:
+        if self.L.oracle_hybrid_step(C.byref(self.c), float(dt), int(nsteps)) != 0:
+            raise MemoryError("oracle_hybrid_step")

@@ -163,6 +163,37 @@ uint32_t oracle_dem_pairs_bruteforce(const oracle_dem *s, int32_t *i, int32_t *j
 /* Total force per body of the last step (gravity + contacts, fixed zeroed). */
 void oracle_dem_last_force(const oracle_dem *s, double *F);
 
+/* ---- hybrid: hybridgrains2dnew coupling ------------------------------------ */
+
+/* HybridGrains2DSim's node-node coupled step without zone resampling
+ * (hybridgrains2dnew/HybridGrains2DSim.cpp:116-227): the MPM and DEM states
+ * plus HybridCoupling2D's per-step buffers (Coupling.h:16-70), owned by the
+ * caller. */
+typedef struct oracle_hybrid {
+  oracle_mpm *mpm;
+  oracle_dem *dem;          /* stepped with its weights (flowWithWeights) */
+  double *dem_pos_before;   /* 3B */
+  double *dem_mass_grid;    /* G */
+  double *dem_mom_grid;     /* 2G */
+  double *v_after;          /* 2G */
+  double *at_after;         /* 2G */
+  double *homog_factor;     /* G */
+  uint8_t *coupled;         /* B */
+} oracle_hybrid;
+
+/* Coupling.cpp:22-285 (serial branch, use_pre_position) */
+void oracle_hybrid_rasterize_grain_mass(oracle_hybrid *h);
+/* Coupling.cpp:287-466 */
+void oracle_hybrid_rasterize_grain_mom(oracle_hybrid *h);
+/* Coupling.cpp:533-579 */
+void oracle_hybrid_update_velocities_on_grid(oracle_hybrid *h);
+/* Coupling.cpp:468-527 (FLIP, alpha = 1) */
+void oracle_hybrid_transfer_to_grain(oracle_hybrid *h);
+/* Coupling.cpp:581-591 */
+void oracle_hybrid_update_positions(oracle_hybrid *h, double dt);
+/* HybridGrains2DSim.cpp:116-227 without updateHybridState, nsteps times */
+int oracle_hybrid_step(oracle_hybrid *h, double dt, uint32_t nsteps);
+
 #ifdef __cplusplus
 }
 #endif

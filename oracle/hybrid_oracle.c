@@ -1,0 +1,165 @@
+/*
+ * hybrid_oracle.c -- TEST INFRASTRUCTURE: serial FP64 CPU restatement of the
+ * hybridgrains2dnew node-node coupled step (HybridGrains2DSim.cpp:116-227)
+ * and HybridCoupling2D (Coupling.cpp:22-591), without the periodic zone
+ * resampling (updateHybridState), which is outside the explicit step.
+ * Checker only.
+ *
+ * Parity pins (tests/test_oracle_hybrid.py): the exchange conserves each
+ * node's momentum and leaves equal MPM and DEM node velocities (the
+ * reference's own Debug check, Coupling.cpp:566-577); grains the MPM mass does
+ * not surround stay uncoupled; a coupled free fall stays exact.
+ */
+#include "hg_oracle.h"
+
+#include <math.h>
+#include <string.h>
+
+static uint64_t nodes(const oracle_mpm *s) {
+  return (uint64_t)(s->cell_count[0] + 1) * (uint64_t)(s->cell_count[1] + 1);
+}
+
+static int node_valid(const oracle_mpm *s, int64_t ix, int64_t iy) {
+  return ix >= 0 && iy >= 0 && ix <= (int64_t)s->cell_count[0] && iy <= (int64_t)s->cell_count[1];
+}
+
+/* the grains use the MPM basis with the first point's half width
+ * (Coupling.cpp:31) */
+static double grain_hl(const oracle_hybrid *h) {
+  return h->mpm->npoints ? h->mpm->hl[0] : 0.25 * h->mpm->cell_width;
+}
+
+void oracle_hybrid_rasterize_grain_mass(oracle_hybrid *h) {
+  oracle_mpm *s = h->mpm;
+  oracle_dem *d = h->dem;
+  const uint32_t nx = s->cell_count[0] + 1;
+  const double hl = grain_hl(h);
+  memset(h->dem_mass_grid, 0, nodes(s) * sizeof(double));
+  if (s->hybridized) memset(s->hybridized, 0, nodes(s) * sizeof(int32_t));
+  for (uint32_t b = 0; b < d->nbodies; ++b) {
+    h->coupled[b] = 0;
+    if (d->fixed && d->fixed[b]) continue;
+    const double *qb = &h->dem_pos_before[3 * b];
+    int64_t lo[2], hi[2];
+    oracle_stencil(s, qb, hl, lo, hi);
+    int all_pos = 1;
+    for (int64_t iy = lo[1]; iy <= hi[1]; ++iy)
+      for (int64_t ix = lo[0]; ix <= hi[0]; ++ix) {
+        if (!node_valid(s, ix, iy)) continue;
+        if (s->mass[iy * nx + ix] <= 1.0e-6) all_pos = 0;
+      }
+    if (!all_pos) continue;
+    h->coupled[b] = 1;
+    const double m = d->mass[3 * b];
+    for (int64_t iy = lo[1]; iy <= hi[1]; ++iy)
+      for (int64_t ix = lo[0]; ix <= hi[0]; ++ix) {
+        if (!node_valid(s, ix, iy)) continue;
+        const double w = oracle_weight(s, qb, hl, ix, iy);
+        h->dem_mass_grid[iy * nx + ix] += w * m;
+      }
+  }
+}
+
+void oracle_hybrid_rasterize_grain_mom(oracle_hybrid *h) {
+  oracle_mpm *s = h->mpm;
+  oracle_dem *d = h->dem;
+  const uint32_t nx = s->cell_count[0] + 1;
+  const double hl = grain_hl(h);
+  memset(h->dem_mom_grid, 0, 2 * nodes(s) * sizeof(double));
+  for (uint32_t b = 0; b < d->nbodies; ++b) {
+    if (!h->coupled[b]) continue;
+    const double *qb = &h->dem_pos_before[3 * b];
+    int64_t lo[2], hi[2];
+    oracle_stencil(s, qb, hl, lo, hi);
+    const double m = d->mass[3 * b];
+    const double pm0 = m * d->v[3 * b], pm1 = m * d->v[3 * b + 1];
+    for (int64_t iy = lo[1]; iy <= hi[1]; ++iy)
+      for (int64_t ix = lo[0]; ix <= hi[0]; ++ix) {
+        if (!node_valid(s, ix, iy)) continue;
+        const double w = oracle_weight(s, qb, hl, ix, iy);
+        const int64_t n = iy * nx + ix;
+        h->dem_mom_grid[2 * n] += w * pm0;
+        h->dem_mom_grid[2 * n + 1] += w * pm1;
+      }
+  }
+}
+
+void oracle_hybrid_update_velocities_on_grid(oracle_hybrid *h) {
+  oracle_mpm *s = h->mpm;
+  const uint64_t G = nodes(s);
+  memset(h->v_after, 0, 2 * G * sizeof(double));
+  memset(h->at_after, 0, 2 * G * sizeof(double));
+  for (uint64_t n = 0; n < G; ++n) {
+    const double m_dem = h->dem_mass_grid[n];
+    if (!(m_dem > 0.0)) continue;
+    const double m_mpm = s->mass[n];
+    const double msum = m_mpm + m_dem;
+    const double l0 = (m_dem * s->mom_new[2 * n] - m_mpm * h->dem_mom_grid[2 * n]) / msum;
+    const double l1 = (m_dem * s->mom_new[2 * n + 1] - m_mpm * h->dem_mom_grid[2 * n + 1]) / msum;
+    s->mom_new[2 * n] -= l0;
+    s->mom_new[2 * n + 1] -= l1;
+    const double da0 = h->dem_mom_grid[2 * n] + l0, da1 = h->dem_mom_grid[2 * n + 1] + l1;
+    h->v_after[2 * n] = da0 / m_dem;
+    h->v_after[2 * n + 1] = da1 / m_dem;
+    h->at_after[2 * n] = l0 / m_dem;
+    h->at_after[2 * n + 1] = l1 / m_dem;
+  }
+}
+
+void oracle_hybrid_transfer_to_grain(oracle_hybrid *h) {
+  oracle_mpm *s = h->mpm;
+  oracle_dem *d = h->dem;
+  const uint32_t nx = s->cell_count[0] + 1;
+  const double hl = grain_hl(h);
+  for (uint32_t b = 0; b < d->nbodies; ++b) {
+    if (!h->coupled[b]) continue;
+    const double *qb = &h->dem_pos_before[3 * b];
+    int64_t lo[2], hi[2];
+    oracle_stencil(s, qb, hl, lo, hi);
+    double vp0 = 0.0, vp1 = 0.0, a0 = 0.0, a1 = 0.0;
+    for (int64_t iy = lo[1]; iy <= hi[1]; ++iy)
+      for (int64_t ix = lo[0]; ix <= hi[0]; ++ix) {
+        if (!node_valid(s, ix, iy)) continue;
+        const double w = oracle_weight(s, qb, hl, ix, iy);
+        const int64_t n = iy * nx + ix;
+        vp0 += w * h->v_after[2 * n];
+        vp1 += w * h->v_after[2 * n + 1];
+        a0 += w * h->at_after[2 * n];
+        a1 += w * h->at_after[2 * n + 1];
+      }
+    const double vf0 = d->v[3 * b] + a0, vf1 = d->v[3 * b + 1] + a1;
+    d->v[3 * b] = vp0 * (1.0 - 1.0) + vf0 * 1.0;
+    d->v[3 * b + 1] = vp1 * (1.0 - 1.0) + vf1 * 1.0;
+  }
+}
+
+void oracle_hybrid_update_positions(oracle_hybrid *h, double dt) {
+  oracle_dem *d = h->dem;
+  for (uint32_t b = 0; b < d->nbodies; ++b) {
+    if (!h->coupled[b]) continue;
+    d->q[3 * b] = h->dem_pos_before[3 * b] + dt * d->v[3 * b];
+    d->q[3 * b + 1] = h->dem_pos_before[3 * b + 1] + dt * d->v[3 * b + 1];
+  }
+}
+
+int oracle_hybrid_step(oracle_hybrid *h, double dt, uint32_t nsteps) {
+  oracle_mpm *s = h->mpm;
+  oracle_dem *d = h->dem;
+  for (uint32_t k = 0; k < nsteps; ++k) {
+    memcpy(h->dem_pos_before, d->q, 3 * (size_t)d->nbodies * sizeof(double));
+    oracle_mpm_flow_zero_phase(s);
+    oracle_hybrid_rasterize_grain_mass(h);
+    if (oracle_dem_flow(d, dt, 1)) return -1;
+    oracle_mpm_flow_first_phase(s, dt);
+    oracle_hybrid_rasterize_grain_mom(h);
+    oracle_hybrid_update_velocities_on_grid(h);
+    const uint64_t G = nodes(s);
+    if (h->homog_factor)
+      for (uint64_t n = 0; n < G; ++n)
+        h->homog_factor[n] = h->dem_mass_grid[n] > 0.0 ? 0.5 : (s->mass[n] > 0.0 ? 1.0 : 0.0);
+    oracle_hybrid_transfer_to_grain(h);
+    oracle_mpm_flow_second_phase(s, dt);
+    oracle_hybrid_update_positions(h, dt);
+  }
+  return 0;
+}
