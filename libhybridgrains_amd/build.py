@@ -20,7 +20,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -ffp-contract=off keeps every product and sum rounded as written, so the
 # FP64 path follows the same operation order as the CPU restatement.
-# Packed FP32 instructions are disabled (numerics).
+# Packed FP32 instructions (v_pk_*_f32) are disabled: some of their forms have
+# given wrong results on the target machines without any error.
 HIP_FLAGS = [
     "-O3",
     "-std=c++17",
@@ -34,6 +35,13 @@ HIP_FLAGS = [
 ]
 
 HIP_SOURCES = ["hg_ctx.hip", "hg_sort.hip", "hg_mpm.hip", "hg_dem.hip", "hg_hybrid.hip"]
+
+HOST_LIB = PKG / "libhg_host.so"
+HOST_SOURCES = ["hg_xml.cpp", "hg_scene.cpp", "hg_sim.cpp", "hg_host_capi.cpp"]
+CLIS = ["mpmgrains2d_cli", "rigidbody2d_cli", "hybridgrains2d_cli"]
+CXX = os.environ.get("CXX", "g++")
+ROCM_LIB = "/opt/rocm/lib"
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", f"-I{ROOT / 'include'}"]
 
 
 def _run(cmd: list[str], cwd: Path | None = None) -> None:
@@ -64,8 +72,27 @@ def build_lib(jobs: int = 8) -> Path:
     return LIB
 
 
+def build_host() -> Path:
+    """libhg_host.so (scene readers, reference-named stepping API) and the
+    command-line drivers under bin/; linked against libhg.so."""
+    tmp = HOST_LIB.with_suffix(".so.tmp")
+    _run([CXX, *CXX_FLAGS, "-shared", "-o", str(tmp), *[str(HOST / s) for s in HOST_SOURCES],
+          f"-L{PKG}", "-lhg", f"-L{ROCM_LIB}", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{ROCM_LIB}"])
+    os.replace(tmp, HOST_LIB)
+    bindir = PKG / "bin"
+    bindir.mkdir(exist_ok=True)
+    for name in CLIS:
+        out = bindir / name
+        _run([CXX, *CXX_FLAGS, "-o", str(out) + ".tmp", str(HOST / f"{name}.cpp"), f"-L{PKG}",
+              "-lhg_host", "-lhg", f"-L{ROCM_LIB}", "-Wl,-rpath,$ORIGIN/..",
+              f"-Wl,-rpath,{ROCM_LIB}"])
+        os.replace(str(out) + ".tmp", out)
+    return HOST_LIB
+
+
 def main() -> int:
     build_lib()
+    build_host()
     return 0
 
 
