@@ -164,6 +164,13 @@ hg_status hg_bench_copy(hg_ctx *ctx, uint64_t nbytes, int32_t iters, double *gbp
 /* Captures one step into a hipGraph and replays it nsteps times. */
 hg_status hg_mpm_step_graph(hg_mpm *h, double dt, uint32_t nsteps);
 
+/* ---- device memory (no reference counterpart) --------------------------------
+ * hg_*_estimate_bytes: an upper bound of the device memory a handle created
+ * from these descriptors allocates (admission by footprint); hg_*_device_bytes:
+ * what a handle holds. */
+hg_status hg_mpm_estimate_bytes(const hg_mpm_desc *descs, uint32_t nscenes, uint64_t *bytes);
+hg_status hg_mpm_device_bytes(hg_mpm *h, uint64_t *bytes);
+
 /* ---- DEM: rigidbody2d circles with PenaltyImpactFrictionMap ----------------- */
 /*
  * Replaces the circle state RigidBody2DSceneParser builds
@@ -222,6 +229,8 @@ hg_status hg_dem_download_contacts(hg_dem *h, uint32_t scene, uint32_t max, int3
 /* Keep the total per-body force (3B) of each step for download. */
 hg_status hg_dem_keep_force(hg_dem *h, int32_t keep);
 hg_status hg_dem_download_force(hg_dem *h, uint32_t scene, double *F);
+hg_status hg_dem_estimate_bytes(const hg_dem_desc *descs, uint32_t nscenes, uint64_t *bytes);
+hg_status hg_dem_device_bytes(hg_dem *h, uint64_t *bytes);
 hg_status hg_dem_set_profiling(hg_dem *h, int32_t on);
 hg_status hg_dem_kernel_times(hg_dem *h, int32_t max_kernels, char *names, double *ms,
                               int64_t *launches, int32_t *nkernels);
@@ -259,6 +268,9 @@ hg_status hg_hybrid_dem(hg_hybrid *h, hg_dem **out);
 /* Per grain, 1 if the last step coupled it to the continuum
  * (HybridCoupling2D's dem_particle_coupled). */
 hg_status hg_hybrid_download_coupling(hg_hybrid *h, uint8_t *coupled);
+hg_status hg_hybrid_estimate_bytes(const hg_mpm_desc *mpm, const hg_dem_desc *dem,
+                                   uint64_t *bytes);
+hg_status hg_hybrid_device_bytes(hg_hybrid *h, uint64_t *bytes);
 
 #ifdef __cplusplus
 }

@@ -92,6 +92,26 @@ class DemDesc(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("budget_bytes", C.c_uint64),
+        ("max_active", C.c_uint32),
+        ("slice_steps", C.c_uint32),
+        ("max_inflight", C.c_uint32),
+        ("max_bypass", C.c_uint32),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+        ("cancelled", C.c_uint64), ("active", C.c_uint64), ("queued", C.c_uint64),
+        ("bytes_in_use", C.c_uint64), ("p50_latency_s", C.c_double),
+        ("p99_latency_s", C.c_double), ("mean_latency_s", C.c_double),
+        ("max_latency_s", C.c_double), ("work_steps", C.c_double), ("span_s", C.c_double),
+    ]
+
+
 def lib() -> C.CDLL:
     """Loads libhg.so; raises if it has not been built."""
     global _lib
@@ -153,6 +173,33 @@ def lib() -> C.CDLL:
         "hg_hybrid_mpm": ([vp, C.POINTER(vp)], C.c_int32),
         "hg_hybrid_dem": ([vp, C.POINTER(vp)], C.c_int32),
         "hg_hybrid_download_coupling": ([vp, C.POINTER(C.c_uint8)], C.c_int32),
+        "hg_mpm_estimate_bytes": ([C.POINTER(MpmDesc), C.c_uint32, C.POINTER(C.c_uint64)], C.c_int32),
+        "hg_mpm_device_bytes": ([vp, C.POINTER(C.c_uint64)], C.c_int32),
+        "hg_dem_estimate_bytes": ([C.POINTER(DemDesc), C.c_uint32, C.POINTER(C.c_uint64)], C.c_int32),
+        "hg_dem_device_bytes": ([vp, C.POINTER(C.c_uint64)], C.c_int32),
+        "hg_hybrid_estimate_bytes": (
+            [C.POINTER(MpmDesc), C.POINTER(DemDesc), C.POINTER(C.c_uint64)], C.c_int32),
+        "hg_hybrid_device_bytes": ([vp, C.POINTER(C.c_uint64)], C.c_int32),
+        "hg_server_create": ([vp, C.POINTER(ServerConfig), C.POINTER(vp)], C.c_int32),
+        "hg_server_destroy": ([vp], C.c_int32),
+        "hg_server_submit_mpm": (
+            [vp, C.POINTER(MpmDesc), C.c_double, C.c_uint64, C.POINTER(C.c_uint64)], C.c_int32),
+        "hg_server_submit_dem": (
+            [vp, C.POINTER(DemDesc), _dp, C.c_double, C.c_uint64, C.POINTER(C.c_uint64)],
+            C.c_int32),
+        "hg_server_submit_hybrid": (
+            [vp, C.POINTER(MpmDesc), C.POINTER(DemDesc), _dp, C.c_double, C.c_uint64,
+             C.POINTER(C.c_uint64)], C.c_int32),
+        "hg_server_cancel": ([vp, C.c_uint64], C.c_int32),
+        "hg_server_poll": ([vp, C.c_uint64, _ip, C.POINTER(C.c_uint64)], C.c_int32),
+        "hg_server_wait": ([vp, C.c_uint64, C.c_double, _ip], C.c_int32),
+        "hg_server_drain": ([vp, C.c_double], C.c_int32),
+        "hg_server_result_points": ([vp, C.c_uint64, _dp, _dp, _dp, _dp], C.c_int32),
+        "hg_server_result_bodies": ([vp, C.c_uint64, _dp, _dp], C.c_int32),
+        "hg_server_latency": ([vp, C.c_uint64, _dp], C.c_int32),
+        "hg_server_error": ([vp, C.c_uint64, C.c_char_p, C.c_size_t], C.c_int32),
+        "hg_server_release": ([vp, C.c_uint64], C.c_int32),
+        "hg_server_stats_get": ([vp, C.POINTER(ServerStats)], C.c_int32),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)

@@ -424,6 +424,7 @@ static hg_status dalloc(hg_dem *h, P **out, size_t bytes) {
   if (e != hipSuccess)
     return hg_fail(h->ctx, HG_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
   h->allocs.push_back(p);
+  h->bytes += bytes;
   *out = static_cast<P *>(p);
   return HG_OK;
 }
@@ -597,6 +598,51 @@ static hg_status upload_bodies(hg_dem *h, const hg_dem_desc *descs) {
   return HG_OK;
 }
 
+// Broad-phase grid of one scene: the cell is at least the largest diameter
+// (the 3x3-cell search needs that); the box is the caller's domain or the
+// bodies' box grown to let the scene spread.
+static void dem_plan_grid(const hg_dem_desc &d, SceneDev &sd) {
+  double rmax = 0.0, x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
+  for (uint32_t k = 0; k < d.nbodies; ++k) {
+    rmax = std::max(rmax, d.radius[k]);
+    x0 = std::min(x0, d.q[3 * k]);
+    x1 = std::max(x1, d.q[3 * k]);
+    y0 = std::min(y0, d.q[3 * k + 1]);
+    y1 = std::max(y1, d.q[3 * k + 1]);
+  }
+  if (d.nbodies == 0) x0 = y0 = x1 = y1 = 0.0, rmax = 1.0;
+  const double cw = std::max(d.cell_width, 2.0 * rmax * (1.0 + 1e-9));
+  double lo0, lo1, hi0, hi1;
+  if (d.domain_max[0] > d.domain_min[0] && d.domain_max[1] > d.domain_min[1]) {
+    lo0 = d.domain_min[0];
+    lo1 = d.domain_min[1];
+    hi0 = d.domain_max[0];
+    hi1 = d.domain_max[1];
+  } else {
+    // room for the scene to spread: the bodies' box grown by twice its
+    // larger side, then clipped by axis-aligned planes the bodies sit inside
+    const double e = 2.0 * std::max(std::max(x1 - x0, y1 - y0), 8.0 * cw);
+    lo0 = x0 - e;
+    lo1 = y0 - e;
+    hi0 = x1 + e;
+    hi1 = y1 + e;
+    for (uint32_t k = 0; k < d.nplanes; ++k) {
+      const double n0 = d.plane_n[2 * k], n1 = d.plane_n[2 * k + 1];
+      const double px = d.plane_x[2 * k], py = d.plane_x[2 * k + 1];
+      if (n1 == 0.0 && n0 > 0.0 && px <= x0) lo0 = std::max(lo0, px - 2.0 * cw);
+      if (n1 == 0.0 && n0 < 0.0 && px >= x1) hi0 = std::min(hi0, px + 2.0 * cw);
+      if (n0 == 0.0 && n1 > 0.0 && py <= y0) lo1 = std::max(lo1, py - 2.0 * cw);
+      if (n0 == 0.0 && n1 < 0.0 && py >= y1) hi1 = std::min(hi1, py + 2.0 * cw);
+    }
+  }
+  // bodies that leave this box clamp to its rim cells (correct, slower)
+  sd.cmin0 = lo0;
+  sd.cmin1 = lo1;
+  sd.cw = cw;
+  sd.ncx = int32_t(std::min(std::max(std::ceil((hi0 - lo0) / cw), 1.0) + 1.0, 16384.0));
+  sd.ncy = int32_t(std::min(std::max(std::ceil((hi1 - lo1) / cw), 1.0) + 1.0, 16384.0));
+}
+
 static hg_status dem_create_impl(hg_ctx *ctx, const hg_dem_desc *descs, uint32_t nscenes,
                                  hg_dem **out) {
   if (!ctx || !descs || !out || nscenes == 0) return hg_fail(ctx, HG_ERR_INVALID, "null argument");
@@ -635,45 +681,7 @@ static hg_status dem_create_impl(hg_ctx *ctx, const hg_dem_desc *descs, uint32_t
     sd.gamman = d.gamman;
     sd.gammat = d.gammat;
     sd.mu = d.mu;
-    double rmax = 0.0, x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
-    for (uint32_t k = 0; k < d.nbodies; ++k) {
-      rmax = std::max(rmax, d.radius[k]);
-      x0 = std::min(x0, d.q[3 * k]);
-      x1 = std::max(x1, d.q[3 * k]);
-      y0 = std::min(y0, d.q[3 * k + 1]);
-      y1 = std::max(y1, d.q[3 * k + 1]);
-    }
-    if (d.nbodies == 0) x0 = y0 = x1 = y1 = 0.0, rmax = 1.0;
-    const double cw = std::max(d.cell_width, 2.0 * rmax * (1.0 + 1e-9));
-    double lo0, lo1, hi0, hi1;
-    if (d.domain_max[0] > d.domain_min[0] && d.domain_max[1] > d.domain_min[1]) {
-      lo0 = d.domain_min[0];
-      lo1 = d.domain_min[1];
-      hi0 = d.domain_max[0];
-      hi1 = d.domain_max[1];
-    } else {
-      // room for the scene to spread: the bodies' box grown by twice its
-      // larger side, then clipped by axis-aligned planes the bodies sit inside
-      const double e = 2.0 * std::max(std::max(x1 - x0, y1 - y0), 8.0 * cw);
-      lo0 = x0 - e;
-      lo1 = y0 - e;
-      hi0 = x1 + e;
-      hi1 = y1 + e;
-      for (uint32_t k = 0; k < d.nplanes; ++k) {
-        const double n0 = d.plane_n[2 * k], n1 = d.plane_n[2 * k + 1];
-        const double px = d.plane_x[2 * k], py = d.plane_x[2 * k + 1];
-        if (n1 == 0.0 && n0 > 0.0 && px <= x0) lo0 = std::max(lo0, px - 2.0 * cw);
-        if (n1 == 0.0 && n0 < 0.0 && px >= x1) hi0 = std::min(hi0, px + 2.0 * cw);
-        if (n0 == 0.0 && n1 > 0.0 && py <= y0) lo1 = std::max(lo1, py - 2.0 * cw);
-        if (n0 == 0.0 && n1 < 0.0 && py >= y1) hi1 = std::min(hi1, py + 2.0 * cw);
-      }
-    }
-    // bodies that leave this box clamp to its rim cells (correct, slower)
-    sd.cmin0 = lo0;
-    sd.cmin1 = lo1;
-    sd.cw = cw;
-    sd.ncx = int32_t(std::min(std::max(std::ceil((hi0 - lo0) / cw), 1.0) + 1.0, 16384.0));
-    sd.ncy = int32_t(std::min(std::max(std::ceil((hi1 - lo1) / cw), 1.0) + 1.0, 16384.0));
+    dem_plan_grid(d, sd);
     sd.b_off = uint32_t(b_off);
     sd.b_cnt = d.nbodies;
     sd.cell_off = uint32_t(cell_off);
@@ -948,4 +956,39 @@ extern "C" hg_status hg_dem_kernel_times(hg_dem *h, int32_t max_kernels, char *n
                                          int64_t *launches, int32_t *nkernels) {
   if (!h) return HG_ERR_INVALID;
   return hg_profiler_report(h->prof, h->ctx, max_kernels, names, ms, launches, nkernels);
+}
+
+// Device bytes per body of the arrays dalloc gives a handle (bodies, two
+// contact-slot sets, force records, sort keys), for the estimate below.
+static uint64_t dem_bytes_per_body(size_t T) {
+  const uint64_t bodies = 4 * 8 + 7 * T + 1;
+  const uint64_t slots = uint64_t(kMaxOwned) * 4 + uint64_t(kMaxOwned) * 2 * T + 1 +
+                         uint64_t(kMaxPlanes) + uint64_t(kMaxPlanes) * 2 * T;
+  const uint64_t forces = uint64_t(kMaxOwned) * 4 * T + uint64_t(kMaxPlanes) * 3 * T +
+                          uint64_t(kMaxIn) * 4 + 1;
+  return bodies + 2 * slots + forces + 3 * 4 + 24;
+}
+
+extern "C" hg_status hg_dem_estimate_bytes(const hg_dem_desc *descs, uint32_t nscenes,
+                                           uint64_t *bytes) {
+  if (!descs || !bytes || nscenes == 0) return HG_ERR_INVALID;
+  uint64_t B = 0, cells = 0;
+  for (uint32_t s = 0; s < nscenes; ++s) {
+    SceneDev sd;
+    std::memset(&sd, 0, sizeof(sd));
+    dem_plan_grid(descs[s], sd);
+    B += descs[s].nbodies;
+    cells += uint64_t(sd.ncx) * uint64_t(sd.ncy);
+  }
+  const size_t T = descs[0].precision == HG_FP64 ? 8 : 4;
+  const uint64_t nscan = (cells + kScanChunk - 1) / kScanChunk;
+  *bytes = B * dem_bytes_per_body(T) + cells * 8 + 4 + nscan * 4 +
+           uint64_t(nscenes) * (sizeof(SceneDev) + 16) + (1u << 20);
+  return HG_OK;
+}
+
+extern "C" hg_status hg_dem_device_bytes(hg_dem *h, uint64_t *bytes) {
+  if (!h || !bytes) return HG_ERR_INVALID;
+  *bytes = h->bytes;
+  return HG_OK;
 }

@@ -80,6 +80,7 @@ struct hg_dem {
   KernelProfiler prof;
   hipGraphExec_t graph = nullptr;
   std::vector<void *> allocs;
+  uint64_t bytes = 0;  // device bytes allocated
   size_t tsize() const { return precision == HG_FP64 ? 8 : 4; }
 };
 

@@ -11,25 +11,13 @@
 // is appended once to a list (a stamp per node), and the list's nodes are
 // cleared again at the end of the step, so nothing is swept over the whole
 // grid and the step replays correctly from a hipGraph.
-#include "hg_dem_handle.h"
-#include "hg_mpm_handle.h"
+#include "hg_hybrid_handle.h"
 
 #include <memory>
 
 namespace hghyb {
 
 constexpr int kThreads = 256;
-
-struct GrainGrid {
-  double *dmass;   // [G] grain mass per node
-  double *dmom;    // [2G] grain momentum per node
-  double *vafter;  // [2G] node velocity after the exchange
-  double *aafter;  // [2G] velocity increment lambda / m_dem
-  uint32_t *stamp; // [G] 1 while the node is on the list
-  uint32_t *list;  // touched nodes
-  uint32_t *count;
-  uint32_t max_list;
-};
 
 struct HybArgs {
   const hgmpm::SceneDev *msc;  // MPM scene 0
@@ -231,19 +219,6 @@ __global__ __launch_bounds__(kThreads) void hyb_clear(HybArgs a) {
 
 using namespace hghyb;
 
-struct hg_hybrid {
-  hg_ctx *ctx = nullptr;
-  hg_mpm *mpm = nullptr;
-  hg_dem *dem = nullptr;
-  hipStream_t stream = nullptr;
-  GrainGrid g = {};
-  double *prex = nullptr, *prey = nullptr;
-  uint8_t *coupled = nullptr;
-  int32_t *err = nullptr;
-  hipGraphExec_t graph = nullptr;
-  double graph_dt = -1.0;
-  std::vector<void *> allocs;
-};
 
 template <typename P>
 static hg_status halloc(hg_hybrid *h, P **out, size_t bytes) {
@@ -253,6 +228,7 @@ static hg_status halloc(hg_hybrid *h, P **out, size_t bytes) {
   if (e != hipSuccess)
     return hg_fail(h->ctx, HG_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
   h->allocs.push_back(p);
+  h->bytes += bytes;
   *out = static_cast<P *>(p);
   return HG_OK;
 }
@@ -492,5 +468,23 @@ extern "C" hg_status hg_hybrid_download_coupling(hg_hybrid *h, uint8_t *coupled)
   HG_TRY(hg_hybrid_sync(h));
   if (coupled && h->dem->nbodies)
     HG_HIP(h->ctx, hipMemcpy(coupled, h->coupled, h->dem->nbodies, hipMemcpyDeviceToHost));
+  return HG_OK;
+}
+
+extern "C" hg_status hg_hybrid_estimate_bytes(const hg_mpm_desc *mpm, const hg_dem_desc *dem,
+                                              uint64_t *bytes) {
+  if (!mpm || !dem || !bytes) return HG_ERR_INVALID;
+  uint64_t a = 0, b = 0;
+  HG_TRY(hg_mpm_estimate_bytes(mpm, 1, &a));
+  HG_TRY(hg_dem_estimate_bytes(dem, 1, &b));
+  const uint64_t G = (uint64_t(mpm->cell_count[0]) + 2) * (uint64_t(mpm->cell_count[1]) + 2);
+  const uint64_t B = std::max<uint64_t>(dem->nbodies, 1);
+  *bytes = a + b + G * (8 + 16 * 3 + 4 + 4) + B * (8 + 8 + 1) + 64;
+  return HG_OK;
+}
+
+extern "C" hg_status hg_hybrid_device_bytes(hg_hybrid *h, uint64_t *bytes) {
+  if (!h || !bytes) return HG_ERR_INVALID;
+  *bytes = h->bytes + h->mpm->bytes + h->dem->bytes;
   return HG_OK;
 }

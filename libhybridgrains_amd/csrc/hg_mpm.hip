@@ -680,6 +680,7 @@ static hg_status dalloc(hg_mpm *h, P **out, size_t bytes) {
   if (e != hipSuccess)
     return hg_fail(h->ctx, HG_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
   h->allocs.push_back(p);
+  h->bytes += bytes;
   *out = static_cast<P *>(p);
   return HG_OK;
 }
@@ -1334,6 +1335,7 @@ extern "C" hg_status hg_mpm_set_tile(hg_mpm *h, int32_t aw, int32_t ah) {
   HG_HIP(h->ctx, hipMalloc(&nta, h->ntiles));
   h->allocs.push_back(nts);
   h->allocs.push_back(nta);
+  h->bytes += uint64_t(h->ntiles) * 5;
   h->d_tile_scene = static_cast<uint32_t *>(nts);
   h->tile_active = static_cast<uint8_t *>(nta);
   HG_HIP(h->ctx, hipMemcpy(h->d_tile_scene, tile_scene.data(), h->ntiles * 4, hipMemcpyHostToDevice));
@@ -1374,5 +1376,32 @@ extern "C" hg_status hg_mpm_step_graph(hg_mpm *h, double dt, uint32_t nsteps) {
   }
   if (nsteps & 1u) launch_step(h);
   HG_HIP(h->ctx, hipGetLastError());
+  return HG_OK;
+}
+
+extern "C" hg_status hg_mpm_estimate_bytes(const hg_mpm_desc *descs, uint32_t nscenes,
+                                           uint64_t *bytes) {
+  if (!descs || !bytes || nscenes == 0) return HG_ERR_INVALID;
+  const uint64_t T = descs[0].precision == HG_FP64 ? 8 : 4;
+  uint64_t N = 0, nodes = 0, tiles = 0;
+  for (uint32_t s = 0; s < nscenes; ++s) {
+    const hg_mpm_desc &d = descs[s];
+    const uint64_t snx = uint64_t(d.cell_count[0]) + 2, sny = uint64_t(d.cell_count[1]) + 2;
+    N += d.npoints;
+    nodes += snx * sny;
+    // the smallest tile (16x8) gives the most tiles
+    tiles += ((snx + 15) / 16) * ((sny + 7) / 8);
+  }
+  const uint64_t per_point = 2 * (16 + 9 * T + 4 + 1) + 16;
+  const uint64_t per_node = 8 + 4 * T + 8 + 16 * 3;  // cell lists, grid fields
+  const uint64_t nscan = (nodes + kScanChunk - 1) / kScanChunk;
+  *bytes = N * per_point + nodes * per_node + 4 + nscan * 4 + tiles * 5 +
+           uint64_t(nscenes) * (sizeof(SceneDev) + 16) + (1u << 20);
+  return HG_OK;
+}
+
+extern "C" hg_status hg_mpm_device_bytes(hg_mpm *h, uint64_t *bytes) {
+  if (!h || !bytes) return HG_ERR_INVALID;
+  *bytes = h->bytes;
   return HG_OK;
 }

@@ -37,6 +37,7 @@ struct hg_mpm {
   KernelProfiler prof;
   hipGraphExec_t graph = nullptr;
   std::vector<void *> allocs;
+  uint64_t bytes = 0;  // device bytes allocated
 
   size_t tsize() const { return precision == HG_FP64 ? 8 : 4; }
   int tw() const;
