@@ -70,22 +70,23 @@ __device__ __forceinline__ bool point_key(const SceneDev &sc, double x, double y
 }
 
 // ---------------------------------------------------------------------------
-// P2G + grid update.  One workgroup per tile of TW x TH owned nodes, one
-// thread per node.  The points whose stencils reach the tile have their base
-// cell in the apron [x0-S+1, x0+TW) x [y0-S+1, y0+TH); in the cell-sorted
-// order they form one run per apron row.
+// P2G + grid update.  A workgroup takes tiles of TW x TH owned nodes from the
+// list of active tiles, one thread per node.  The points whose stencils
+// reach the tile have their base cell in the apron [x0-S+1, x0+TW) x
+// [y0-S+1, y0+TH); in the cell-sorted order they form one run per apron row,
+// and within a cell they are in point-id order (mpm_cell_order_kernel).
 //  1. stage (point-parallel): the apron points are loaded through the
-//     permutation, their Cauchy stress is evaluated (FP64), and a compact
-//     record goes to LDS; rows are staged in chunks of at most CAP points.
-//  2. order: each cell's records are ranked by point id, so every node adds
-//     a cell's points in id order whatever order the sort left them in.
-//  3. gather (node-parallel): each thread walks the S x S cells whose points
-//     can reach its node -- rows ascending, cells ascending, points in id
-//     order -- and sums mass, momentum and force in FP64 registers.  Chunks
-//     are whole rows taken in ascending order, so the sum order does not
-//     depend on the chunking: each node's value is a fixed function of the
+//     permutation; their Cauchy stress (FP64) and their 1-D stencil weights
+//     and weight derivatives for the S nodes of each axis go to LDS.  The
+//     apron's points are concatenated row by row and staged in chunks of at
+//     most CAP points (whole rows where they fit).
+//  2. gather (node-parallel): for each of the node's S cell rows, ascending,
+//     the node walks the contiguous run of points of its S cells in that row
+//     -- cells ascending, points in id order -- and sums mass, momentum and
+//     force in FP64 registers.  With whole-row chunks the sum order does not
+//     depend on the chunking, so each node's value is a fixed function of the
 //     scene state (bitwise reproducible, identical alone or in a batch).
-//  4. grid update of the node (forces, momentum, plane conditions, velocity
+//  3. grid update of the node (forces, momentum, plane conditions, velocity
 //     and acceleration), written as one (v, a) vector per node.
 //
 // Follows PhysicsGrid::rasterizePointMasses (PhysicsGrid.cpp:168-207),
@@ -98,22 +99,21 @@ __device__ __forceinline__ bool point_key(const SceneDev &sc, double x, double y
 struct P2GArgs {
   const SceneDev *scenes;
   const uint32_t *tile_scene;
+  const uint32_t *tile_list;
+  const uint32_t *tile_count;
   PtsRaw pts;
   const uint32_t *perm;
   const uint32_t *cell_start;
   void *gva;
   double *gm, *gp, *gpn, *gf;
-  uint8_t *tile_active;
   uint32_t store;
-  uint32_t ntiles, nblocks_padded;
 };
 
-// What P2G needs of one point.
+// What P2G needs of one point besides its weights.
 template <typename T>
 struct P2GRec {
   double x, y;
   T m, mv0, mv1, c, s00, s01, s11, hl;
-  uint32_t id;
   uint32_t cnt;  // stencil node counts: x in bits 0-3, y in bits 4-7
 };
 
@@ -143,7 +143,6 @@ __device__ __forceinline__ P2GRec<T> p2g_record(const PtsRaw &pts, uint32_t p, c
   const double b01 = double(ld<T>(pts.b01, p));
   const double b11 = double(ld<T>(pts.b11, p));
   r.hl = pts.hl ? ld<T>(pts.hl, p) : T(sc.hl_uniform);
-  r.id = pts.id[p];
   // Cauchy stress sigma = tau / J; tau = 0 when J > 1 (no tension).  FP64.
   double t00 = 0.0, t01 = 0.0, t11 = 0.0;
   if (J <= 1.0) {
@@ -175,243 +174,321 @@ __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
   constexpr int AW = TW + S - 1;  // apron cells per row
   constexpr int AH = TH + S - 1;  // apron rows
   constexpr int NT = TW * TH;
-  __shared__ double s_x[CAP], s_y[CAP];
-  __shared__ T s_m[CAP], s_mv0[CAP], s_mv1[CAP], s_c[CAP], s_s00[CAP], s_s01[CAP], s_s11[CAP],
-      s_hl[CAP];
-  __shared__ uint32_t s_id[CAP];
-  __shared__ uint16_t s_ord[CAP];
-  __shared__ uint8_t s_cnt[CAP];
-  __shared__ uint32_t s_rs[AH], s_rc[AH], s_cs[AH][AW + 1];
-  __shared__ int s_any;
+  // per staged point: x weights and x derivative factors of its S x nodes,
+  // the same for y, and the mass, momentum and stress terms
+  __shared__ T s_wx[S][CAP], s_gx[S][CAP], s_wy[S][CAP], s_gy[S][CAP];
+  __shared__ T s_m[CAP], s_mv0[CAP], s_mv1[CAP], s_c[CAP], s_s00[CAP], s_s01[CAP], s_s11[CAP];
+  __shared__ uint8_t s_col[CAP];               // apron column of the point's cell
+  __shared__ uint32_t s_cs[AH][AW + 1];        // cell starts of the apron rows
+  __shared__ uint32_t s_ro[AH + 1];            // row offsets in the concatenated list
 
-  const uint32_t tile = hg_xcd_remap(blockIdx.x, args.nblocks_padded);
-  if (tile >= args.ntiles) return;
-  const uint32_t sid = args.tile_scene[tile];
-  const SceneDev &sc = args.scenes[sid];
-  const uint32_t ltile = tile - sc.tile_off;
-  const int x0 = int(ltile % uint32_t(sc.tiles_x)) * TW;
-  const int y0 = int(ltile / uint32_t(sc.tiles_x)) * TH;
+  const uint32_t ntile_list = *args.tile_count;
   const int t = threadIdx.x;
-  const uint32_t *cs = args.cell_start + sc.cell_off;
-  const int ax0 = x0 - (S - 1), ay0 = y0 - (S - 1);
-
-  // Apron cell starts, row by row (cells of one row are consecutive keys;
-  // cells off the grid hold no points and read as empty).
-  for (int q = t; q < AH * (AW + 1); q += NT) {
-    const int r = q / (AW + 1), c = q % (AW + 1);
-    const int ry = ay0 + r;
-    uint32_t v = 0;
-    if (ry >= 0 && ry < sc.nny) {
-      const int cx = min(max(ax0 + c, 0), sc.nnx);
-      v = cs[uint32_t(ry) * uint32_t(sc.snx) + uint32_t(cx)];
-    }
-    s_cs[r][c] = v;
-  }
-  if (t == 0) s_any = 0;
-  __syncthreads();
-  if (t < AH) {
-    s_rs[t] = s_cs[t][0];
-    s_rc[t] = s_cs[t][AW] - s_cs[t][0];
-    if (s_rc[t] > 0) s_any = 1;
-  }
-  __syncthreads();
-  const bool any = s_any != 0;
-  if (t == 0) args.tile_active[tile] = any ? 1 : 0;
-  if (!any) return;
-
   const int lx = t % TW, ly = t / TW;
-  const int nx = x0 + lx, ny = y0 + ly;
-  const T inv_h = T(sc.inv_h);
-  double am = 0.0, ap0 = 0.0, ap1 = 0.0, af0 = 0.0, af1 = 0.0;
+  for (uint32_t li = blockIdx.x; li < ntile_list; li += gridDim.x) {
+    const uint32_t tile = args.tile_list[li];
+    const uint32_t sid = args.tile_scene[tile];
+    const SceneDev &sc = args.scenes[sid];
+    const uint32_t ltile = tile - sc.tile_off;
+    const int x0 = int(ltile % uint32_t(sc.tiles_x)) * TW;
+    const int y0 = int(ltile / uint32_t(sc.tiles_x)) * TH;
+    const uint32_t *cs = args.cell_start + sc.cell_off;
+    const int ax0 = x0 - (S - 1), ay0 = y0 - (S - 1);
 
-  int r0 = 0;
-  while (r0 < AH) {
-    // rows [r0, r1) are staged together (block-uniform decision); a single
-    // row holding more than CAP points is read straight from HBM instead
-    uint32_t tot = 0;
-    int r1 = r0;
-    while (r1 < AH && tot + s_rc[r1] <= uint32_t(CAP)) {
-      tot += s_rc[r1];
-      ++r1;
+    // Apron cell starts, row by row (cells of one row are consecutive keys;
+    // cells off the grid hold no points and read as empty).
+    for (int q = t; q < AH * (AW + 1); q += NT) {
+      const int r = q / (AW + 1), c = q % (AW + 1);
+      const int ry = ay0 + r;
+      uint32_t v = 0;
+      if (ry >= 0 && ry < sc.nny) {
+        const int cx = min(max(ax0 + c, 0), sc.nnx);
+        v = cs[uint32_t(ry) * uint32_t(sc.snx) + uint32_t(cx)];
+      }
+      s_cs[r][c] = v;
     }
-    const bool staged = r1 > r0;
-    if (!staged) r1 = r0 + 1;
-    if (staged) {
-      // 1. stage the chunk's points
-      for (uint32_t k = uint32_t(t); k < tot; k += NT) {
-        int r = r0;
-        uint32_t pre = 0;
-        while (k >= pre + s_rc[r]) {
-          pre += s_rc[r];
-          ++r;
-        }
-        const uint32_t p = args.perm[s_rs[r] + (k - pre)];
+    __syncthreads();
+    if (t == 0) {
+      uint32_t o = 0;
+      for (int r = 0; r < AH; ++r) {
+        s_ro[r] = o;
+        o += s_cs[r][AW] - s_cs[r][0];
+      }
+      s_ro[AH] = o;
+    }
+    __syncthreads();
+    const uint32_t total = s_ro[AH];
+
+    const int nx = x0 + lx, ny = y0 + ly;
+    // the node's run of points in each of its S cell rows (apron rows ly..ly+S-1)
+    uint32_t run0[S], run1[S];
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      const int r = ly + j;
+      run0[j] = s_ro[r] + (s_cs[r][lx] - s_cs[r][0]);
+      run1[j] = s_ro[r] + (s_cs[r][lx + S] - s_cs[r][0]);
+    }
+    double am = 0.0, ap0 = 0.0, ap1 = 0.0, af0 = 0.0, af1 = 0.0;
+
+    // Chunks are consecutive pieces of the concatenated list; each node walks
+    // its runs in ascending list order within every chunk, so its sum order
+    // is the list order whatever the chunk boundaries.
+    for (uint32_t k0 = 0; k0 < total; k0 += uint32_t(CAP)) {
+      const uint32_t k1 = min(k0 + uint32_t(CAP), total);
+
+      // 1. stage
+      for (uint32_t k = k0 + uint32_t(t); k < k1; k += NT) {
+        // apron row r: s_ro[r] <= k < s_ro[r + 1] (first such r)
+        int r = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1)
+          if (r + step <= AH - 1 && s_ro[r + step] <= k) r += step;
+        while (s_ro[r + 1] <= k) ++r;  // skip empty rows
+        const uint32_t pos = s_cs[r][0] + (k - s_ro[r]);
+        // apron column: s_cs[r][col] <= pos < s_cs[r][col + 1] (last such col)
+        int col = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1)
+          if (col + step <= AW - 1 && s_cs[r][col + step] <= pos) col += step;
+        const uint32_t p = args.perm[pos];
         const P2GRec<T> rec = p2g_record<T, BASIS>(args.pts, p, sc);
-        s_x[k] = rec.x;
-        s_y[k] = rec.y;
-        s_m[k] = rec.m;
-        s_mv0[k] = rec.mv0;
-        s_mv1[k] = rec.mv1;
-        s_c[k] = rec.c;
-        s_s00[k] = rec.s00;
-        s_s01[k] = rec.s01;
-        s_s11[k] = rec.s11;
-        s_hl[k] = rec.hl;
-        s_id[k] = rec.id;
-        s_cnt[k] = uint8_t(rec.cnt);
-      }
-      __syncthreads();
-      // 2. rank each cell's records by id
-      for (int q = t; q < (r1 - r0) * AW; q += NT) {
-        const int r = r0 + q / AW, c = q % AW;
-        uint32_t base = s_cs[r][c] - s_rs[r];
-        for (int rr = r0; rr < r; ++rr) base += s_rc[rr];
-        const uint32_t n = s_cs[r][c + 1] - s_cs[r][c];
-        for (uint32_t u = 0; u < n; ++u) {
-          const uint32_t idu = s_id[base + u];
-          uint32_t rank = 0;
-          for (uint32_t v = 0; v < n; ++v) rank += (s_id[base + v] < idu) ? 1u : 0u;
-          s_ord[base + rank] = uint16_t(base + u);
+        const uint32_t q = k - k0;
+        s_col[q] = uint8_t(col);
+        const int bx = ax0 + col, by = ay0 + r;
+        const T inv_h = T(sc.inv_h);
+        const int cntx = int(rec.cnt & 15u), cnty = int(rec.cnt >> 4);
+        T inv_vp = T(1);
+        if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) inv_vp = T(1) / (T(4) * rec.hl * rec.hl);
+#pragma unroll
+        for (int k2 = 0; k2 < S; ++k2) {
+          T ax, dx, ay, dy;
+          basis_factor<T, BASIS>(rec.x, rec.hl, sc.min0, sc.h, inv_h, bx + k2, k2, cntx, ax, dx);
+          basis_factor<T, BASIS>(rec.y, rec.hl, sc.min1, sc.h, inv_h, by + k2, k2, cnty, ay, dy);
+          if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) {
+            s_wx[k2][q] = ax * inv_vp;
+            s_gx[k2][q] = dx * inv_vp;
+            s_wy[k2][q] = ay;
+            s_gy[k2][q] = dy;
+          } else {
+            s_wx[k2][q] = ax;
+            s_gx[k2][q] = dx * inv_h;
+            s_wy[k2][q] = ay;
+            s_gy[k2][q] = dy * inv_h;
+          }
         }
+        s_m[q] = rec.m;
+        s_mv0[q] = rec.mv0;
+        s_mv1[q] = rec.mv1;
+        s_c[q] = rec.c;
+        s_s00[q] = rec.s00;
+        s_s01[q] = rec.s01;
+        s_s11[q] = rec.s11;
       }
       __syncthreads();
-    }
-    // 3. gather: cells (ny - j, nx - i) of this chunk's rows, rows ascending
-    if (nx < sc.nnx && ny < sc.nny) {
-      for (int j = S - 1; j >= 0; --j) {
-        const int r = (ny - j) - ay0;  // apron row of the cells at ny - j
-        if (r < r0 || r >= r1) continue;
-        uint32_t rowbase = 0;
-        if (staged)
-          for (int rr = r0; rr < r; ++rr) rowbase += s_rc[rr];
-        for (int i = S - 1; i >= 0; --i) {
-          const int c = (nx - i) - ax0;  // apron column of the cells at nx - i
-          const uint32_t c0 = s_cs[r][c], c1 = s_cs[r][c + 1];
-          const uint32_t n = c1 - c0;
-          if (n == 0) continue;
-          const uint32_t base = rowbase + (c0 - s_rs[r]);
-          uint32_t last_id = 0;
-          for (uint32_t u = 0; u < n; ++u) {
-            P2GRec<T> rec;
-            if (staged) {
-              const uint32_t q = s_ord[base + u];
-              rec.x = s_x[q];
-              rec.y = s_y[q];
-              rec.m = s_m[q];
-              rec.mv0 = s_mv0[q];
-              rec.mv1 = s_mv1[q];
-              rec.c = s_c[q];
-              rec.s00 = s_s00[q];
-              rec.s01 = s_s01[q];
-              rec.s11 = s_s11[q];
-              rec.hl = s_hl[q];
-              rec.cnt = s_cnt[q];
-            } else {
-              // next point of the cell in id order, straight from HBM
-              uint32_t best = 0xffffffffu, bp = 0;
-              for (uint32_t v = c0; v < c1; ++v) {
-                const uint32_t pi = args.perm[v];
-                const uint32_t idv = args.pts.id[pi];
-                if ((u == 0 || idv > last_id) && idv <= best) {
-                  best = idv;
-                  bp = pi;
-                }
-              }
-              last_id = best;
-              rec = p2g_record<T, BASIS>(args.pts, bp, sc);
-            }
-            const int cntx = int(rec.cnt & 15u), cnty = int(rec.cnt >> 4);
-            T ax, dx, ay, dy;
-            basis_factor<T, BASIS>(rec.x, rec.hl, sc.min0, sc.h, inv_h, nx, i, cntx, ax, dx);
-            basis_factor<T, BASIS>(rec.y, rec.hl, sc.min1, sc.h, inv_h, ny, j, cnty, ay, dy);
-            T w, gx, gy;
-            if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) {
-              const T inv_vp = T(1) / (T(4) * rec.hl * rec.hl);
-              const T axv = ax * inv_vp;
-              w = axv * ay;
-              gx = (dx * inv_vp) * ay;
-              gy = axv * dy;
-            } else {
-              w = ax * ay;
-              gx = (dx * inv_h) * ay;
-              gy = ax * (dy * inv_h);
-            }
-            am += double(w * rec.m);
-            ap0 += double(w * rec.mv0);
-            ap1 += double(w * rec.mv1);
+
+      // 2. gather: rows ascending, then the run (cells ascending, ids ascending)
+      if (nx < sc.nnx && ny < sc.nny) {
+#pragma unroll
+        for (int j0 = 0; j0 < S; ++j0) {
+          const int j = S - 1 - j0;  // cell row ny - j: weight index j
+          const uint32_t lo = max(run0[j0], k0), hi = min(run1[j0], k1);
+          for (uint32_t k = lo; k < hi; ++k) {
+            const uint32_t q = k - k0;
+            const int i = lx + (S - 1) - int(s_col[q]);  // node's x offset in the point's stencil
+            const T wx = s_wx[i][q], gxs = s_gx[i][q];
+            const T wy = s_wy[j][q], gys = s_gy[j][q];
+            const T w = wx * wy;
+            const T gx = gxs * wy;
+            const T gy = wx * gys;
+            am += double(w * s_m[q]);
+            ap0 += double(w * s_mv0[q]);
+            ap1 += double(w * s_mv1[q]);
             if (w != T(0)) {
-              af0 += double(rec.c * (rec.s00 * gx + rec.s01 * gy));
-              af1 += double(rec.c * (rec.s01 * gx + rec.s11 * gy));
+              const T c = s_c[q];
+              af0 += double(c * (s_s00[q] * gx + s_s01[q] * gy));
+              af1 += double(c * (s_s01[q] * gx + s_s11[q] * gy));
             }
           }
         }
       }
+      __syncthreads();
     }
-    if (staged) __syncthreads();
-    r0 = r1;
-  }
 
-  // 4. grid update of the node.
-  if (nx >= sc.nnx || ny >= sc.nny) return;
-  using T4 = typename Vec4T<T>::type;
-  const double dt = sc.dt;
-  const uint32_t node = sc.node_off + uint32_t(ny) * uint32_t(sc.snx) + uint32_t(nx);
-  const double m = am;
-  const double p0 = ap0, p1 = ap1;
-  double f0 = af0, f1 = af1;
-  f0 += m * sc.g0;
-  f1 += m * sc.g1;
-  double pn0 = 0.0, pn1 = 0.0;
-  if (m > 0.0) {
-    pn0 = p0 + dt * f0;
-    pn1 = p1 + dt * f1;
-  }
-  const double px0 = sc.min0 + sc.h * double(nx);
-  const double px1 = sc.min1 + sc.h * double(ny);
-  for (int k = 0; k < sc.nplanes; ++k) {
-    if (px1 < sc.plane_lower[k]) continue;
-    const double n0 = sc.plane_n[2 * k], n1 = sc.plane_n[2 * k + 1];
-    const double d = n0 * (px0 - sc.plane_x[2 * k]) + n1 * (px1 - sc.plane_x[2 * k + 1]);
-    if (d > 0.0) continue;
-    if (sc.plane_beh[k] == HG_PLANE_SLIDING) {
-      if (m <= 0.0) continue;
-      double u0 = pn0 / m, u1 = pn1 / m;
-      const double vn = n0 * u0 + n1 * u1;
-      if (vn > 0.0) continue;
-      u0 -= vn * n0;
-      u1 -= vn * n1;
-      pn0 = m * u0;
-      pn1 = m * u1;
-    } else {
-      pn0 = 0.0;
-      pn1 = 0.0;
+    // 3. grid update of the node.
+    if (nx < sc.nnx && ny < sc.nny) {
+      using T4 = typename Vec4T<T>::type;
+      const double dt = sc.dt;
+      const uint32_t node = sc.node_off + uint32_t(ny) * uint32_t(sc.snx) + uint32_t(nx);
+      const double m = am;
+      const double p0 = ap0, p1 = ap1;
+      double f0 = af0, f1 = af1;
+      f0 += m * sc.g0;
+      f1 += m * sc.g1;
+      double pn0 = 0.0, pn1 = 0.0;
+      if (m > 0.0) {
+        pn0 = p0 + dt * f0;
+        pn1 = p1 + dt * f1;
+      }
+      const double px0 = sc.min0 + sc.h * double(nx);
+      const double px1 = sc.min1 + sc.h * double(ny);
+      for (int k = 0; k < sc.nplanes; ++k) {
+        if (px1 < sc.plane_lower[k]) continue;
+        const double n0 = sc.plane_n[2 * k], n1 = sc.plane_n[2 * k + 1];
+        const double d = n0 * (px0 - sc.plane_x[2 * k]) + n1 * (px1 - sc.plane_x[2 * k + 1]);
+        if (d > 0.0) continue;
+        if (sc.plane_beh[k] == HG_PLANE_SLIDING) {
+          if (m <= 0.0) continue;
+          double u0 = pn0 / m, u1 = pn1 / m;
+          const double vn = n0 * u0 + n1 * u1;
+          if (vn > 0.0) continue;
+          u0 -= vn * n0;
+          u1 -= vn * n1;
+          pn0 = m * u0;
+          pn1 = m * u1;
+        } else {
+          pn0 = 0.0;
+          pn1 = 0.0;
+        }
+      }
+      if (args.store & kStoreVelAcc) {
+        T4 va;
+        if (m > 0.0) {
+          const double dm = dt * m;
+          va.x = T(pn0 / m);
+          va.y = T(pn1 / m);
+          va.z = T((pn0 - p0) / dm);
+          va.w = T((pn1 - p1) / dm);
+        } else {
+          va.x = va.y = va.z = va.w = T(0);
+        }
+        static_cast<T4 *>(args.gva)[node] = va;
+      }
+      if (args.store & kStoreMass) args.gm[node] = m;
+      if (args.store & kStoreMom) {
+        args.gp[2 * node] = p0;
+        args.gp[2 * node + 1] = p1;
+      }
+      if (args.store & kStoreMomNew) {
+        args.gpn[2 * node] = pn0;
+        args.gpn[2 * node + 1] = pn1;
+      }
+      if (args.store & kStoreForce) {
+        args.gf[2 * node] = f0;
+        args.gf[2 * node + 1] = f1;
+      }
     }
+    __syncthreads();  // LDS is reused by the next tile
   }
-  if (args.store & kStoreVelAcc) {
-    T4 va;
-    if (m > 0.0) {
-      const double dm = dt * m;
-      va.x = T(pn0 / m);
-      va.y = T(pn1 / m);
-      va.z = T((pn0 - p0) / dm);
-      va.w = T((pn1 - p1) / dm);
-    } else {
-      va.x = va.y = va.z = va.w = T(0);
+}
+
+// Points of each cell in point-id order: the sort's scatter leaves them in
+// the order its atomics handed out slots.  One thread per cell; perm and pinv
+// are permuted together.  Thread 0 also clears the active-tile count for the
+// tile list that follows.
+__device__ __forceinline__ void cmp_swap(uint32_t &ia, uint32_t &pa, uint32_t &ib, uint32_t &pb) {
+  if (ib < ia) {
+    const uint32_t ti = ia, tp = pa;
+    ia = ib;
+    pa = pb;
+    ib = ti;
+    pb = tp;
+  }
+}
+
+__global__ __launch_bounds__(256) void mpm_cell_order_kernel(const uint32_t *cell_start,
+                                                             uint32_t ncell, const uint32_t *ids,
+                                                             uint32_t *perm, uint32_t *pinv,
+                                                             uint32_t *tile_count) {
+  const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+  if (c == 0) *tile_count = 0u;
+  if (c >= ncell) return;
+  const uint32_t b = cell_start[c];
+  const uint32_t n = cell_start[c + 1] - b;
+  if (n < 2) return;
+  constexpr int R = 8;
+  if (n <= uint32_t(R)) {
+    uint32_t id[R], p[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      p[k] = (uint32_t(k) < n) ? perm[b + k] : 0u;
+      id[k] = (uint32_t(k) < n) ? ids[p[k]] : 0xffffffffu;
     }
-    static_cast<T4 *>(args.gva)[node] = va;
+    // odd-even transposition network (static indices: stays in registers)
+#pragma unroll
+    for (int round = 0; round < R; ++round) {
+#pragma unroll
+      for (int k = (round & 1); k + 1 < R; k += 2) cmp_swap(id[k], p[k], id[k + 1], p[k + 1]);
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k)
+      if (uint32_t(k) < n) {
+        perm[b + k] = p[k];
+        pinv[p[k]] = b + k;
+      }
+    return;
   }
-  if (args.store & kStoreMass) args.gm[node] = m;
-  if (args.store & kStoreMom) {
-    args.gp[2 * node] = p0;
-    args.gp[2 * node + 1] = p1;
+  // rare crowded cell: insertion sort in place
+  for (uint32_t k = 1; k < n; ++k) {
+    const uint32_t pk = perm[b + k], ik = ids[pk];
+    uint32_t m = k;
+    while (m > 0 && ids[perm[b + m - 1]] > ik) {
+      perm[b + m] = perm[b + m - 1];
+      --m;
+    }
+    perm[b + m] = pk;
   }
-  if (args.store & kStoreMomNew) {
-    args.gpn[2 * node] = pn0;
-    args.gpn[2 * node + 1] = pn1;
+  for (uint32_t k = 0; k < n; ++k) pinv[perm[b + k]] = b + k;
+}
+
+// Active tiles: a tile is active when any cell of its apron holds a point.
+// Writes the per-tile flags and appends active tiles to the list (in
+// ascending order within a workgroup; workgroups append in any order, which
+// P2G does not depend on).
+struct TileListArgs {
+  const SceneDev *scenes;
+  const uint32_t *tile_scene;
+  const uint32_t *cell_start;
+  uint8_t *tile_active;
+  uint32_t *tile_list, *tile_count;
+  uint32_t ntiles;
+  int32_t tw, th, S;
+};
+
+__global__ __launch_bounds__(256) void mpm_tile_list_kernel(TileListArgs a) {
+  __shared__ uint32_t s_wave[4], s_base;
+  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+  bool any = false;
+  if (t < a.ntiles) {
+    const SceneDev &sc = a.scenes[a.tile_scene[t]];
+    const uint32_t lt = t - sc.tile_off;
+    const int x0 = int(lt % uint32_t(sc.tiles_x)) * a.tw;
+    const int y0 = int(lt / uint32_t(sc.tiles_x)) * a.th;
+    const int ax0 = x0 - (a.S - 1), ay0 = y0 - (a.S - 1);
+    const int aw = a.tw + a.S - 1, ah = a.th + a.S - 1;
+    const uint32_t *cs = a.cell_start + sc.cell_off;
+    const int c0 = min(max(ax0, 0), sc.nnx), c1 = min(max(ax0 + aw, 0), sc.nnx);
+    for (int r = 0; r < ah && !any; ++r) {
+      const int ry = ay0 + r;
+      if (ry < 0 || ry >= sc.nny) continue;
+      const uint32_t row = uint32_t(ry) * uint32_t(sc.snx);
+      any = cs[row + uint32_t(c1)] > cs[row + uint32_t(c0)];
+    }
+    a.tile_active[t] = any ? 1 : 0;
   }
-  if (args.store & kStoreForce) {
-    args.gf[2 * node] = f0;
-    args.gf[2 * node + 1] = f1;
+  const uint64_t ballot = __ballot(any);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) s_wave[wave] = uint32_t(__popcll(ballot));
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t tot = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+    s_base = tot ? atomicAdd(a.tile_count, tot) : 0u;
+  }
+  __syncthreads();
+  if (any) {
+    uint32_t off = s_base;
+    for (int w = 0; w < wave; ++w) off += s_wave[w];
+    off += uint32_t(__popcll(ballot & ((1ull << lane) - 1ull)));
+    a.tile_list[off] = t;
   }
 }
 
@@ -421,37 +498,38 @@ __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
 struct VelAccArgs {
   const SceneDev *scenes;
   const uint32_t *tile_scene;
-  const uint8_t *tile_active;
+  const uint32_t *tile_list, *tile_count;
   const double *gm, *gp, *gpn;
   void *gva;
-  uint32_t ntiles;
   int32_t tw, th;
 };
 
 template <typename T>
 __global__ __launch_bounds__(256) void mpm_velacc_kernel(VelAccArgs a) {
-  const uint32_t tile = blockIdx.x;
-  if (tile >= a.ntiles || !a.tile_active[tile]) return;
-  const SceneDev &sc = a.scenes[a.tile_scene[tile]];
-  const uint32_t lt = tile - sc.tile_off;
-  const int x0 = int(lt % uint32_t(sc.tiles_x)) * a.tw, y0 = int(lt / uint32_t(sc.tiles_x)) * a.th;
+  const uint32_t n = *a.tile_count;
   using T4 = typename Vec4T<T>::type;
-  for (int q = threadIdx.x; q < a.tw * a.th; q += 256) {
-    const int nx = x0 + q % a.tw, ny = y0 + q / a.tw;
-    if (nx >= sc.nnx || ny >= sc.nny) continue;
-    const uint32_t node = sc.node_off + uint32_t(ny) * uint32_t(sc.snx) + uint32_t(nx);
-    const double m = a.gm[node];
-    T4 va;
-    if (m > 0.0) {
-      const double dm = sc.dt * m;
-      va.x = T(a.gpn[2 * node] / m);
-      va.y = T(a.gpn[2 * node + 1] / m);
-      va.z = T((a.gpn[2 * node] - a.gp[2 * node]) / dm);
-      va.w = T((a.gpn[2 * node + 1] - a.gp[2 * node + 1]) / dm);
-    } else {
-      va.x = va.y = va.z = va.w = T(0);
+  for (uint32_t li = blockIdx.x; li < n; li += gridDim.x) {
+    const uint32_t tile = a.tile_list[li];
+    const SceneDev &sc = a.scenes[a.tile_scene[tile]];
+    const uint32_t lt = tile - sc.tile_off;
+    const int x0 = int(lt % uint32_t(sc.tiles_x)) * a.tw, y0 = int(lt / uint32_t(sc.tiles_x)) * a.th;
+    for (int q = threadIdx.x; q < a.tw * a.th; q += 256) {
+      const int nx = x0 + q % a.tw, ny = y0 + q / a.tw;
+      if (nx >= sc.nnx || ny >= sc.nny) continue;
+      const uint32_t node = sc.node_off + uint32_t(ny) * uint32_t(sc.snx) + uint32_t(nx);
+      const double m = a.gm[node];
+      T4 va;
+      if (m > 0.0) {
+        const double dm = sc.dt * m;
+        va.x = T(a.gpn[2 * node] / m);
+        va.y = T(a.gpn[2 * node + 1] / m);
+        va.z = T((a.gpn[2 * node] - a.gp[2 * node]) / dm);
+        va.w = T((a.gpn[2 * node + 1] - a.gp[2 * node + 1]) / dm);
+      } else {
+        va.x = va.y = va.z = va.w = T(0);
+      }
+      static_cast<T4 *>(a.gva)[node] = va;
     }
-    static_cast<T4 *>(a.gva)[node] = va;
   }
 }
 
@@ -694,14 +772,20 @@ static hg_status dalloc(hg_mpm *h, P **out, size_t bytes) {
 // ---- kernel dispatch ----------------------------------------------------------
 
 template <typename T> struct P2GCap;
-template <> struct P2GCap<float> { static constexpr int value = 768; };
-template <> struct P2GCap<double> { static constexpr int value = 512; };
+template <> struct P2GCap<float> { static constexpr int value = 512; };
+template <> struct P2GCap<double> { static constexpr int value = 256; };
+
+// Workgroups of the P2G launch: enough to fill the chip several times over;
+// each takes active tiles from the list until it is exhausted.
+static uint32_t p2g_grid(const hg_mpm *h) { return std::max(1u, std::min(h->ntiles, 2048u)); }
 
 template <typename T, int BASIS, int TW, int TH>
 static void launch_p2g_t(hg_mpm *h, uint32_t store) {
   P2GArgs a;
   a.scenes = h->d_scenes;
   a.tile_scene = h->d_tile_scene;
+  a.tile_list = h->tile_list;
+  a.tile_count = h->tile_count;
   a.pts = h->buf[h->cur];
   a.perm = h->perm;
   a.cell_start = h->cell_start;
@@ -710,12 +794,9 @@ static void launch_p2g_t(hg_mpm *h, uint32_t store) {
   a.gp = h->gp;
   a.gpn = h->gpn;
   a.gf = h->gf;
-  a.tile_active = h->tile_active;
   a.store = store;
-  a.ntiles = h->ntiles;
-  a.nblocks_padded = hg_pad8(h->ntiles);
   ProfScope ps(h->prof, "mpm_p2g", h->stream);
-  hipLaunchKernelGGL((mpm_p2g_kernel<T, BASIS, TW, TH, P2GCap<T>::value>), dim3(a.nblocks_padded),
+  hipLaunchKernelGGL((mpm_p2g_kernel<T, BASIS, TW, TH, P2GCap<T>::value>), dim3(p2g_grid(h)),
                      dim3(TW * TH), 0, h->stream, a);
 }
 
@@ -797,9 +878,35 @@ static void launch_bin_p(hg_mpm *h) {
   }
 }
 
+static void launch_tile_list(hg_mpm *h) {
+  TileListArgs a;
+  a.scenes = h->d_scenes;
+  a.tile_scene = h->d_tile_scene;
+  a.cell_start = h->cell_start;
+  a.tile_active = h->tile_active;
+  a.tile_list = h->tile_list;
+  a.tile_count = h->tile_count;
+  a.ntiles = h->ntiles;
+  a.tw = h->tw();
+  a.th = h->th();
+  a.S = stencil_width(h->basis);
+  ProfScope ps(h->prof, "mpm_tile_list", h->stream);
+  hipLaunchKernelGGL(mpm_tile_list_kernel, dim3((h->ntiles + 255) / 256), dim3(256), 0, h->stream,
+                     a);
+}
+
+// Sorts the points of the current buffer by cell (perm, pinv, cell_start),
+// puts each cell's points in id order and lists the active tiles.
 static void launch_sort(hg_mpm *h) {
   hg_counting_sort(h->prof, h->stream, h->counts, h->ncell, h->block_sums, h->nscan,
                    h->cell_start, h->key, h->slot, h->npoints, h->perm, h->pinv);
+  {
+    ProfScope ps(h->prof, "mpm_cell_order", h->stream);
+    hipLaunchKernelGGL(mpm_cell_order_kernel, dim3((h->ncell + 255) / 256), dim3(256), 0,
+                       h->stream, h->cell_start, h->ncell, h->buf[h->cur].id, h->perm, h->pinv,
+                       h->tile_count);
+  }
+  launch_tile_list(h);
 }
 
 static void launch_g2p_and_sort(hg_mpm *h) {
@@ -807,27 +914,28 @@ static void launch_g2p_and_sort(hg_mpm *h) {
     if (h->precision == HG_FP64) launch_g2p_p<double>(h);
     else launch_g2p_p<float>(h);
   }
-  launch_sort(h);
+  // the points now live in the other buffer, which the sort orders
   h->cur = 1 - h->cur;
+  launch_sort(h);
 }
 
 static void launch_velacc(hg_mpm *h) {
   VelAccArgs a;
   a.scenes = h->d_scenes;
   a.tile_scene = h->d_tile_scene;
-  a.tile_active = h->tile_active;
+  a.tile_list = h->tile_list;
+  a.tile_count = h->tile_count;
   a.gm = h->gm;
   a.gp = h->gp;
   a.gpn = h->gpn;
   a.gva = h->gva;
-  a.ntiles = h->ntiles;
   a.tw = h->tw();
   a.th = h->th();
   ProfScope ps(h->prof, "mpm_velacc", h->stream);
   if (h->precision == HG_FP64)
-    hipLaunchKernelGGL(mpm_velacc_kernel<double>, dim3(h->ntiles), dim3(256), 0, h->stream, a);
+    hipLaunchKernelGGL(mpm_velacc_kernel<double>, dim3(p2g_grid(h)), dim3(256), 0, h->stream, a);
   else
-    hipLaunchKernelGGL(mpm_velacc_kernel<float>, dim3(h->ntiles), dim3(256), 0, h->stream, a);
+    hipLaunchKernelGGL(mpm_velacc_kernel<float>, dim3(p2g_grid(h)), dim3(256), 0, h->stream, a);
 }
 
 static hg_status upload_dt(hg_mpm *h, double dt) {
@@ -1045,6 +1153,8 @@ static hg_status mpm_create_impl(hg_ctx *ctx, const hg_mpm_desc *descs, uint32_t
   HG_TRY(dalloc(hp, &h->d_p_off, (nscenes + 1) * 4));
   HG_TRY(dalloc(hp, &h->d_tile_scene, h->ntiles * 4));
   HG_TRY(dalloc(hp, &h->tile_active, h->ntiles));
+  HG_TRY(dalloc(hp, &h->tile_list, size_t(h->ntiles) * 4));
+  HG_TRY(dalloc(hp, &h->tile_count, 4));
   HG_TRY(dalloc(hp, &h->perm, size_t(h->npoints) * 4));
   HG_TRY(dalloc(hp, &h->pinv, size_t(h->npoints) * 4));
   HG_TRY(dalloc(hp, &h->key, size_t(h->npoints) * 4));
@@ -1064,6 +1174,7 @@ static hg_status mpm_create_impl(hg_ctx *ctx, const hg_mpm_desc *descs, uint32_t
   // Zero everything a kernel may read before writing: masked stencil nodes
   // are read with weight zero and must hold finite values.
   HG_HIP(ctx, hipMemset(h->tile_active, 0, h->ntiles));
+  HG_HIP(ctx, hipMemset(h->tile_count, 0, 4));
   HG_HIP(ctx, hipMemset(h->counts, 0, size_t(h->ncell) * 4));
   HG_HIP(ctx, hipMemset(h->gva, 0, size_t(h->nnode) * 4 * hp->tsize()));
   HG_HIP(ctx, hipMemset(h->gm, 0, size_t(h->nnode) * 8));
@@ -1330,14 +1441,17 @@ extern "C" hg_status hg_mpm_set_tile(hg_mpm *h, int32_t aw, int32_t ah) {
     tile_off += uint64_t(sd.tiles_x) * uint64_t(sd.tiles_y);
   }
   h->ntiles = uint32_t(tile_off);
-  void *nts = nullptr, *nta = nullptr;
+  void *nts = nullptr, *nta = nullptr, *ntl = nullptr;
   HG_HIP(h->ctx, hipMalloc(&nts, h->ntiles * 4));
   HG_HIP(h->ctx, hipMalloc(&nta, h->ntiles));
+  HG_HIP(h->ctx, hipMalloc(&ntl, h->ntiles * 4));
   h->allocs.push_back(nts);
   h->allocs.push_back(nta);
-  h->bytes += uint64_t(h->ntiles) * 5;
+  h->allocs.push_back(ntl);
+  h->bytes += uint64_t(h->ntiles) * 9;
   h->d_tile_scene = static_cast<uint32_t *>(nts);
   h->tile_active = static_cast<uint8_t *>(nta);
+  h->tile_list = static_cast<uint32_t *>(ntl);
   HG_HIP(h->ctx, hipMemcpy(h->d_tile_scene, tile_scene.data(), h->ntiles * 4, hipMemcpyHostToDevice));
   HG_HIP(h->ctx, hipMemset(h->tile_active, 0, h->ntiles));
   HG_HIP(h->ctx, hipMemcpy(h->d_scenes, h->scenes.data(), h->scenes.size() * sizeof(SceneDev),
@@ -1346,6 +1460,10 @@ extern "C" hg_status hg_mpm_set_tile(hg_mpm *h, int32_t aw, int32_t ah) {
     hipGraphExecDestroy(h->graph);
     h->graph = nullptr;
   }
+  // the active-tile list of the current sort, for the new shape
+  HG_HIP(h->ctx, hipMemsetAsync(h->tile_count, 0, 4, h->stream));
+  launch_tile_list(h);
+  HG_HIP(h->ctx, hipStreamSynchronize(h->stream));
   return HG_OK;
 }
 

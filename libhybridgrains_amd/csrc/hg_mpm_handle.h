@@ -27,6 +27,7 @@ struct hg_mpm {
   uint32_t *d_p_off = nullptr;
   uint32_t *d_tile_scene = nullptr;
   uint8_t *tile_active = nullptr;
+  uint32_t *tile_list = nullptr, *tile_count = nullptr;  // active tiles of this step
   void *gva = nullptr;
   double *gm = nullptr, *gp = nullptr, *gpn = nullptr, *gf = nullptr;
   int32_t *err = nullptr;

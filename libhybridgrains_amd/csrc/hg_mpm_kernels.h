@@ -5,9 +5,11 @@
 //    [p_off, p_off + p_cnt)); x,y are FP64 in every precision, the rest is T
 //    (double for HG_FP64, float for HG_FP32); b_bar is stored as its three
 //    independent entries (b00, b01, b11) because the step keeps it exactly
-//    symmetric.  Two buffers: each step's G2P reads buffer A through the cell
-//    permutation and writes buffer B in cell order, so the points are
-//    physically re-sorted every step at no extra traffic.
+//    symmetric.  Two buffers: each step's G2P reads buffer A in its physical
+//    order and writes each point to its slot of the current cell order in
+//    buffer B, so the points are physically re-sorted every step at no extra
+//    traffic (and stay nearly sorted, which keeps the next sort's gathers
+//    coalesced).
 //  * cells: one key per grid node that can be a stencil base; key = base node
 //    (bx, by) as by*snx + bx, where snx = nnx + 1 pads one node column and row
 //    so the masked third uGIMP node of a two-node stencil is always in bounds.
@@ -15,9 +17,10 @@
 //    The fused step writes only (v, a) per node as one T4; split/keep modes
 //    also write FP64 mass, momentum, new momentum and force.
 //
-// Kernels of one step: p2g (tile-owned gather-scatter, also the whole grid
-// update), g2p (gather + return map + advection + re-key), then the counting
-// sort: scan_reduce, scan_top, scan_apply, scatter.
+// Kernels of one step: p2g (node gather over the active tiles, also the whole
+// grid update), g2p (gather + return map + advection + re-key), then the
+// counting sort (scan_reduce, scan_top, scan_apply, scatter), the in-cell
+// id order (cell_order) and the active-tile list (tile_list).
 #pragma once
 
 #include "hg_internal.h"

@@ -265,3 +265,55 @@ def hybrid_column(
     dem.add_plane([0.0, 2 * h], [0.0, 1.0])
     dem.add_plane([2 * h, 0.0], [1.0, 0.0])
     return mpm, dem, weights
+
+
+def hybrid_1m(seed: int = 0):
+    """The 1M-point hybrid scene: the 1M-point MPM column (2000 x 1000 cells
+    at h = 1 mm, 0.5 x 0.5 m at 4 points per cell) with a discrete zone of
+    circles (radius 0.5 mm +- 10 %) 0.1 m deep at its free surface,
+    overlapping the continuum by two cells.  A 10-cell-deep zone holds about
+    5k such circles and 200k would need a zone most of the column deep, so
+    the zone is fixed by depth (0.1 m, about 48k circles).  kn = 1e5 N/m,
+    restitution 0.3, mu = 0.3."""
+    return hybrid_column(domain=(2.0, 1.0), h=0.001, column=(0.5, 0.5), grain_layer=0.1,
+                         r_mean=0.5e-3, r_spread=0.1, kn=1.0e5, seed=seed)
+
+
+def served_requests(n: int, seed: int = 0, rate: float = 8.0, n_min: float = 1e4,
+                    n_max: float = 2e5, steps_min: int = 200, steps_max: int = 1000,
+                    mix=(0.5, 0.3, 0.2)):
+    """A Poisson stream of scene requests: (arrival_s, kind, payload, dt,
+    nsteps), kind 'mpm' (payload MpmScene), 'dem' (DemScene) or 'hybrid'
+    ((MpmScene, DemScene, weights)).  Sizes are log-uniform in [n_min, n_max]
+    particles (points + bodies) and horizons uniform in [steps_min,
+    steps_max]; each request is generated from its own seed."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for k in range(n):
+        t += rng.exponential(1.0 / rate)
+        u = rng.uniform()
+        kind = "mpm" if u < mix[0] else ("dem" if u < mix[0] + mix[1] else "hybrid")
+        N = math.exp(rng.uniform(math.log(n_min), math.log(n_max)))
+        nsteps = int(rng.integers(steps_min, steps_max + 1))
+        rs = seed * 100003 + k
+        if kind == "mpm":
+            h = 0.002
+            wc = max(8, int(round(math.sqrt(N / 4.0))))
+            hc = max(8, int(round(N / 4.0 / wc)))
+            s = column_collapse(domain=((wc + 3 * hc + 8) * h, (hc + 8) * h), h=h,
+                                column=(wc * h, hc * h), seed=rs)
+            out.append((t, kind, s, 1e-5, nsteps))
+        elif kind == "dem":
+            s = dem_column(n_bodies=int(N), width=max(0.02, 2.2e-3 * math.sqrt(N) / 2.0),
+                           seed=rs)
+            out.append((t, kind, s, 1e-5, nsteps))
+        else:
+            h = 0.002
+            wc = max(8, int(round(math.sqrt(N / 4.0))))
+            hc = max(8, int(round(0.8 * N / 4.0 / wc)))
+            m, d, w = hybrid_column(domain=((wc + 3 * hc + 8) * h, (hc + 40) * h), h=h,
+                                    column=(wc * h, hc * h), grain_layer=0.1 * hc * h,
+                                    r_mean=2.5e-3 * h / 0.005 * 1.0, seed=rs)
+            out.append((t, kind, (m, d, w), 1e-5, nsteps))
+    return out
