@@ -102,8 +102,8 @@ hg_status hg_mpm_step(hg_mpm *h, double dt, uint32_t nsteps);
  * the hybrid integrator does.  phase 0 rasterises mass and momentum; phase 1
  * adds the internal and gravity forces, updates the node momentum and applies
  * the grid plane conditions; phase 2 forms node velocity and acceleration and
- * runs the particle update.  Between phases 1 and 2 the node momenta may be
- * edited through hg_mpm_grid_device. */
+ * runs the particle update.  The hybrid step (hg_hybrid_step) edits the node
+ * momenta between phases 1 and 2 on the device. */
 hg_status hg_mpm_phase(hg_mpm *h, int32_t phase, double dt);
 
 /* Waits for the handle's stream and reports HG_ERR_STATE if any step so far

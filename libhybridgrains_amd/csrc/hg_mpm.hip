@@ -74,12 +74,14 @@ __device__ __forceinline__ bool point_key(const SceneDev &sc, double x, double y
 // list of active tiles, one thread per node.  The points whose stencils
 // reach the tile have their base cell in the apron [x0-S+1, x0+TW) x
 // [y0-S+1, y0+TH); in the cell-sorted order they form one run per apron row,
-// and within a cell they are in point-id order (mpm_cell_order_kernel).
+// and P2G stages each cell's points in point-id order.
 //  1. stage (point-parallel): the apron points are loaded through the
 //     permutation; their Cauchy stress (FP64) and their 1-D stencil weights
-//     and weight derivatives for the S nodes of each axis go to LDS.  The
-//     apron's points are concatenated row by row and staged in chunks of at
-//     most CAP points (whole rows where they fit).
+//     and weight derivatives for the S nodes of each axis go to LDS, each at
+//     its point-id rank within its cell (the sort leaves a cell's points in
+//     the order its atomics handed out slots).  The apron's points are
+//     concatenated row by row and staged in chunks of at most CAP points
+//     that end on cell boundaries.
 //  2. gather (node-parallel): for each of the node's S cell rows, ascending,
 //     the node walks the contiguous run of points of its S cells in that row
 //     -- cells ascending, points in id order -- and sums mass, momentum and
@@ -179,8 +181,11 @@ __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
   __shared__ T s_wx[S][CAP], s_gx[S][CAP], s_wy[S][CAP], s_gy[S][CAP];
   __shared__ T s_m[CAP], s_mv0[CAP], s_mv1[CAP], s_c[CAP], s_s00[CAP], s_s01[CAP], s_s11[CAP];
   __shared__ uint8_t s_col[CAP];               // apron column of the point's cell
+  __shared__ uint32_t s_p[CAP], s_id[CAP];     // point index and id, in sort order
+  __shared__ uint16_t s_rc[CAP];               // apron row and column, in sort order
   __shared__ uint32_t s_cs[AH][AW + 1];        // cell starts of the apron rows
   __shared__ uint32_t s_ro[AH + 1];            // row offsets in the concatenated list
+  __shared__ uint32_t s_k1;                    // end of the current chunk
 
   const uint32_t ntile_list = *args.tile_count;
   const int t = threadIdx.x;
@@ -230,30 +235,65 @@ __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
     }
     double am = 0.0, ap0 = 0.0, ap1 = 0.0, af0 = 0.0, af1 = 0.0;
 
-    // Chunks are consecutive pieces of the concatenated list; each node walks
-    // its runs in ascending list order within every chunk, so its sum order
-    // is the list order whatever the chunk boundaries.
-    for (uint32_t k0 = 0; k0 < total; k0 += uint32_t(CAP)) {
-      const uint32_t k1 = min(k0 + uint32_t(CAP), total);
+    // position k of the concatenated list -> apron row r and column col
+    auto locate = [&](uint32_t k, int &r, int &col) {
+      r = 0;
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1)
+        if (r + step <= AH - 1 && s_ro[r + step] <= k) r += step;
+      const uint32_t pos = s_cs[r][0] + (k - s_ro[r]);
+      col = 0;
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1)
+        if (col + step <= AW - 1 && s_cs[r][col + step] <= pos) col += step;
+    };
 
-      // 1. stage
+    // Chunks are consecutive pieces of the concatenated list that end on cell
+    // boundaries (a cell of more than CAP points is split).  Within a cell the
+    // points are staged in id order; each node walks its runs in ascending
+    // list order within every chunk, so its sum order is the list order
+    // (rows, cells, ids) whatever the chunk boundaries.
+    uint32_t k0 = 0;
+    while (k0 < total) {
+      if (t == 0) {
+        const uint32_t target = min(k0 + uint32_t(CAP), total);
+        uint32_t k1 = target;
+        if (target < total) {
+          int r, col;
+          locate(target, r, col);
+          const uint32_t cstart = s_ro[r] + (s_cs[r][col] - s_cs[r][0]);
+          if (cstart > k0) k1 = cstart;  // else: one cell fills the chunk; split it
+        }
+        s_k1 = k1;
+      }
+      __syncthreads();
+      const uint32_t k1 = s_k1;
+
+      // 1a. the chunk's point indices and ids
       for (uint32_t k = k0 + uint32_t(t); k < k1; k += NT) {
-        // apron row r: s_ro[r] <= k < s_ro[r + 1] (first such r)
-        int r = 0;
-#pragma unroll
-        for (int step = 32; step > 0; step >>= 1)
-          if (r + step <= AH - 1 && s_ro[r + step] <= k) r += step;
-        while (s_ro[r + 1] <= k) ++r;  // skip empty rows
-        const uint32_t pos = s_cs[r][0] + (k - s_ro[r]);
-        // apron column: s_cs[r][col] <= pos < s_cs[r][col + 1] (last such col)
-        int col = 0;
-#pragma unroll
-        for (int step = 32; step > 0; step >>= 1)
-          if (col + step <= AW - 1 && s_cs[r][col + step] <= pos) col += step;
-        const uint32_t p = args.perm[pos];
-        const P2GRec<T> rec = p2g_record<T, BASIS>(args.pts, p, sc);
+        int r, col;
+        locate(k, r, col);
+        const uint32_t p = args.perm[s_cs[r][0] + (k - s_ro[r])];
         const uint32_t q = k - k0;
-        s_col[q] = uint8_t(col);
+        s_p[q] = p;
+        s_id[q] = args.pts.id[p];
+        s_rc[q] = uint16_t((r << 8) | col);
+      }
+      __syncthreads();
+
+      // 1b. stage each point at its id rank within its cell
+      for (uint32_t k = k0 + uint32_t(t); k < k1; k += NT) {
+        const uint32_t q = k - k0;
+        const int r = int(s_rc[q] >> 8), col = int(s_rc[q] & 255u);
+        const uint32_t cb0 = s_ro[r] + (s_cs[r][col] - s_cs[r][0]);
+        const uint32_t ce0 = s_ro[r] + (s_cs[r][col + 1] - s_cs[r][0]);
+        const uint32_t cb = max(cb0, k0) - k0, ce = min(ce0, k1) - k0;
+        const uint32_t myid = s_id[q];
+        uint32_t rank = 0;
+        for (uint32_t v = cb; v < ce; ++v) rank += (s_id[v] < myid) ? 1u : 0u;
+        const uint32_t d = cb + rank;
+        const P2GRec<T> rec = p2g_record<T, BASIS>(args.pts, s_p[q], sc);
+        s_col[d] = uint8_t(col);
         const int bx = ax0 + col, by = ay0 + r;
         const T inv_h = T(sc.inv_h);
         const int cntx = int(rec.cnt & 15u), cnty = int(rec.cnt >> 4);
@@ -265,24 +305,24 @@ __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
           basis_factor<T, BASIS>(rec.x, rec.hl, sc.min0, sc.h, inv_h, bx + k2, k2, cntx, ax, dx);
           basis_factor<T, BASIS>(rec.y, rec.hl, sc.min1, sc.h, inv_h, by + k2, k2, cnty, ay, dy);
           if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) {
-            s_wx[k2][q] = ax * inv_vp;
-            s_gx[k2][q] = dx * inv_vp;
-            s_wy[k2][q] = ay;
-            s_gy[k2][q] = dy;
+            s_wx[k2][d] = ax * inv_vp;
+            s_gx[k2][d] = dx * inv_vp;
+            s_wy[k2][d] = ay;
+            s_gy[k2][d] = dy;
           } else {
-            s_wx[k2][q] = ax;
-            s_gx[k2][q] = dx * inv_h;
-            s_wy[k2][q] = ay;
-            s_gy[k2][q] = dy * inv_h;
+            s_wx[k2][d] = ax;
+            s_gx[k2][d] = dx * inv_h;
+            s_wy[k2][d] = ay;
+            s_gy[k2][d] = dy * inv_h;
           }
         }
-        s_m[q] = rec.m;
-        s_mv0[q] = rec.mv0;
-        s_mv1[q] = rec.mv1;
-        s_c[q] = rec.c;
-        s_s00[q] = rec.s00;
-        s_s01[q] = rec.s01;
-        s_s11[q] = rec.s11;
+        s_m[d] = rec.m;
+        s_mv0[d] = rec.mv0;
+        s_mv1[d] = rec.mv1;
+        s_c[d] = rec.c;
+        s_s00[d] = rec.s00;
+        s_s01[d] = rec.s01;
+        s_s11[d] = rec.s11;
       }
       __syncthreads();
 
@@ -312,6 +352,7 @@ __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
         }
       }
       __syncthreads();
+      k0 = k1;
     }
 
     // 3. grid update of the node.
@@ -381,83 +422,29 @@ __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
   }
 }
 
-// Points of each cell in point-id order: the sort's scatter leaves them in
-// the order its atomics handed out slots.  One thread per cell; perm and pinv
-// are permuted together.  Thread 0 also clears the active-tile count for the
-// tile list that follows.
-__device__ __forceinline__ void cmp_swap(uint32_t &ia, uint32_t &pa, uint32_t &ib, uint32_t &pb) {
-  if (ib < ia) {
-    const uint32_t ti = ia, tp = pa;
-    ia = ib;
-    pa = pb;
-    ib = ti;
-    pb = tp;
-  }
-}
-
-__global__ __launch_bounds__(256) void mpm_cell_order_kernel(const uint32_t *cell_start,
-                                                             uint32_t ncell, const uint32_t *ids,
-                                                             uint32_t *perm, uint32_t *pinv,
-                                                             uint32_t *tile_count) {
-  const uint32_t c = blockIdx.x * 256 + threadIdx.x;
-  if (c == 0) *tile_count = 0u;
-  if (c >= ncell) return;
-  const uint32_t b = cell_start[c];
-  const uint32_t n = cell_start[c + 1] - b;
-  if (n < 2) return;
-  constexpr int R = 8;
-  if (n <= uint32_t(R)) {
-    uint32_t id[R], p[R];
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      p[k] = (uint32_t(k) < n) ? perm[b + k] : 0u;
-      id[k] = (uint32_t(k) < n) ? ids[p[k]] : 0xffffffffu;
-    }
-    // odd-even transposition network (static indices: stays in registers)
-#pragma unroll
-    for (int round = 0; round < R; ++round) {
-#pragma unroll
-      for (int k = (round & 1); k + 1 < R; k += 2) cmp_swap(id[k], p[k], id[k + 1], p[k + 1]);
-    }
-#pragma unroll
-    for (int k = 0; k < R; ++k)
-      if (uint32_t(k) < n) {
-        perm[b + k] = p[k];
-        pinv[p[k]] = b + k;
-      }
-    return;
-  }
-  // rare crowded cell: insertion sort in place
-  for (uint32_t k = 1; k < n; ++k) {
-    const uint32_t pk = perm[b + k], ik = ids[pk];
-    uint32_t m = k;
-    while (m > 0 && ids[perm[b + m - 1]] > ik) {
-      perm[b + m] = perm[b + m - 1];
-      --m;
-    }
-    perm[b + m] = pk;
-  }
-  for (uint32_t k = 0; k < n; ++k) pinv[perm[b + k]] = b + k;
-}
-
 // Active tiles: a tile is active when any cell of its apron holds a point.
 // Writes the per-tile flags and appends active tiles to the list (in
 // ascending order within a workgroup; workgroups append in any order, which
-// P2G does not depend on).
+// P2G does not depend on).  The count alternates between two counters with
+// the point buffers: this list's counter was cleared by the previous list.
 struct TileListArgs {
   const SceneDev *scenes;
   const uint32_t *tile_scene;
   const uint32_t *cell_start;
   uint8_t *tile_active;
-  uint32_t *tile_list, *tile_count;
+  uint32_t *tile_list, *tile_count, *next_count;
   uint32_t ntiles;
   int32_t tw, th, S;
 };
 
+// One wave per tile: lane r checks apron row r (a row's cells are
+// consecutive keys, so two cell starts give its point count).  Four tiles per
+// workgroup; the workgroup appends its active tiles in ascending order.
 __global__ __launch_bounds__(256) void mpm_tile_list_kernel(TileListArgs a) {
-  __shared__ uint32_t s_wave[4], s_base;
-  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
-  bool any = false;
+  __shared__ uint32_t s_flag[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t t = blockIdx.x * 4u + uint32_t(wave);
+  bool nonempty = false;
   if (t < a.ntiles) {
     const SceneDev &sc = a.scenes[a.tile_scene[t]];
     const uint32_t lt = t - sc.tile_off;
@@ -465,30 +452,29 @@ __global__ __launch_bounds__(256) void mpm_tile_list_kernel(TileListArgs a) {
     const int y0 = int(lt / uint32_t(sc.tiles_x)) * a.th;
     const int ax0 = x0 - (a.S - 1), ay0 = y0 - (a.S - 1);
     const int aw = a.tw + a.S - 1, ah = a.th + a.S - 1;
-    const uint32_t *cs = a.cell_start + sc.cell_off;
-    const int c0 = min(max(ax0, 0), sc.nnx), c1 = min(max(ax0 + aw, 0), sc.nnx);
-    for (int r = 0; r < ah && !any; ++r) {
-      const int ry = ay0 + r;
-      if (ry < 0 || ry >= sc.nny) continue;
+    const int ry = ay0 + lane;
+    if (lane < ah && ry >= 0 && ry < sc.nny) {
+      const uint32_t *cs = a.cell_start + sc.cell_off;
+      const int c0 = min(max(ax0, 0), sc.nnx), c1 = min(max(ax0 + aw, 0), sc.nnx);
       const uint32_t row = uint32_t(ry) * uint32_t(sc.snx);
-      any = cs[row + uint32_t(c1)] > cs[row + uint32_t(c0)];
+      nonempty = cs[row + uint32_t(c1)] > cs[row + uint32_t(c0)];
     }
-    a.tile_active[t] = any ? 1 : 0;
   }
-  const uint64_t ballot = __ballot(any);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) s_wave[wave] = uint32_t(__popcll(ballot));
+  const bool any = __ballot(nonempty) != 0ull;
+  // the other parity's counter, for the next list (nothing reads it now)
+  if (blockIdx.x == 0 && threadIdx.x == 0) *a.next_count = 0u;
+  if (lane == 0) {
+    s_flag[wave] = any ? 1u : 0u;
+    if (t < a.ntiles) a.tile_active[t] = any ? 1 : 0;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const uint32_t tot = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
-    s_base = tot ? atomicAdd(a.tile_count, tot) : 0u;
-  }
-  __syncthreads();
-  if (any) {
-    uint32_t off = s_base;
-    for (int w = 0; w < wave; ++w) off += s_wave[w];
-    off += uint32_t(__popcll(ballot & ((1ull << lane) - 1ull)));
-    a.tile_list[off] = t;
+    const uint32_t tot = s_flag[0] + s_flag[1] + s_flag[2] + s_flag[3];
+    if (tot) {
+      uint32_t o = atomicAdd(a.tile_count, tot);
+      for (int w = 0; w < 4; ++w)
+        if (s_flag[w]) a.tile_list[o++] = blockIdx.x * 4u + uint32_t(w);
+    }
   }
 }
 
@@ -772,8 +758,8 @@ static hg_status dalloc(hg_mpm *h, P **out, size_t bytes) {
 // ---- kernel dispatch ----------------------------------------------------------
 
 template <typename T> struct P2GCap;
-template <> struct P2GCap<float> { static constexpr int value = 512; };
-template <> struct P2GCap<double> { static constexpr int value = 256; };
+template <> struct P2GCap<float> { static constexpr int value = 384; };
+template <> struct P2GCap<double> { static constexpr int value = 192; };
 
 // Workgroups of the P2G launch: enough to fill the chip several times over;
 // each takes active tiles from the list until it is exhausted.
@@ -785,7 +771,7 @@ static void launch_p2g_t(hg_mpm *h, uint32_t store) {
   a.scenes = h->d_scenes;
   a.tile_scene = h->d_tile_scene;
   a.tile_list = h->tile_list;
-  a.tile_count = h->tile_count;
+  a.tile_count = h->tile_count + h->cur;
   a.pts = h->buf[h->cur];
   a.perm = h->perm;
   a.cell_start = h->cell_start;
@@ -885,27 +871,21 @@ static void launch_tile_list(hg_mpm *h) {
   a.cell_start = h->cell_start;
   a.tile_active = h->tile_active;
   a.tile_list = h->tile_list;
-  a.tile_count = h->tile_count;
+  a.tile_count = h->tile_count + h->cur;
+  a.next_count = h->tile_count + (1 - h->cur);
   a.ntiles = h->ntiles;
   a.tw = h->tw();
   a.th = h->th();
   a.S = stencil_width(h->basis);
   ProfScope ps(h->prof, "mpm_tile_list", h->stream);
-  hipLaunchKernelGGL(mpm_tile_list_kernel, dim3((h->ntiles + 255) / 256), dim3(256), 0, h->stream,
-                     a);
+  hipLaunchKernelGGL(mpm_tile_list_kernel, dim3((h->ntiles + 3) / 4), dim3(256), 0, h->stream, a);
 }
 
-// Sorts the points of the current buffer by cell (perm, pinv, cell_start),
-// puts each cell's points in id order and lists the active tiles.
+// Sorts the points of the current buffer by cell (perm, pinv, cell_start)
+// and lists the active tiles.
 static void launch_sort(hg_mpm *h) {
   hg_counting_sort(h->prof, h->stream, h->counts, h->ncell, h->block_sums, h->nscan,
                    h->cell_start, h->key, h->slot, h->npoints, h->perm, h->pinv);
-  {
-    ProfScope ps(h->prof, "mpm_cell_order", h->stream);
-    hipLaunchKernelGGL(mpm_cell_order_kernel, dim3((h->ncell + 255) / 256), dim3(256), 0,
-                       h->stream, h->cell_start, h->ncell, h->buf[h->cur].id, h->perm, h->pinv,
-                       h->tile_count);
-  }
   launch_tile_list(h);
 }
 
@@ -924,7 +904,7 @@ static void launch_velacc(hg_mpm *h) {
   a.scenes = h->d_scenes;
   a.tile_scene = h->d_tile_scene;
   a.tile_list = h->tile_list;
-  a.tile_count = h->tile_count;
+  a.tile_count = h->tile_count + h->cur;
   a.gm = h->gm;
   a.gp = h->gp;
   a.gpn = h->gpn;
@@ -1154,7 +1134,7 @@ static hg_status mpm_create_impl(hg_ctx *ctx, const hg_mpm_desc *descs, uint32_t
   HG_TRY(dalloc(hp, &h->d_tile_scene, h->ntiles * 4));
   HG_TRY(dalloc(hp, &h->tile_active, h->ntiles));
   HG_TRY(dalloc(hp, &h->tile_list, size_t(h->ntiles) * 4));
-  HG_TRY(dalloc(hp, &h->tile_count, 4));
+  HG_TRY(dalloc(hp, &h->tile_count, 8));
   HG_TRY(dalloc(hp, &h->perm, size_t(h->npoints) * 4));
   HG_TRY(dalloc(hp, &h->pinv, size_t(h->npoints) * 4));
   HG_TRY(dalloc(hp, &h->key, size_t(h->npoints) * 4));
@@ -1174,7 +1154,7 @@ static hg_status mpm_create_impl(hg_ctx *ctx, const hg_mpm_desc *descs, uint32_t
   // Zero everything a kernel may read before writing: masked stencil nodes
   // are read with weight zero and must hold finite values.
   HG_HIP(ctx, hipMemset(h->tile_active, 0, h->ntiles));
-  HG_HIP(ctx, hipMemset(h->tile_count, 0, 4));
+  HG_HIP(ctx, hipMemset(h->tile_count, 0, 8));
   HG_HIP(ctx, hipMemset(h->counts, 0, size_t(h->ncell) * 4));
   HG_HIP(ctx, hipMemset(h->gva, 0, size_t(h->nnode) * 4 * hp->tsize()));
   HG_HIP(ctx, hipMemset(h->gm, 0, size_t(h->nnode) * 8));
@@ -1461,7 +1441,7 @@ extern "C" hg_status hg_mpm_set_tile(hg_mpm *h, int32_t aw, int32_t ah) {
     h->graph = nullptr;
   }
   // the active-tile list of the current sort, for the new shape
-  HG_HIP(h->ctx, hipMemsetAsync(h->tile_count, 0, 4, h->stream));
+  HG_HIP(h->ctx, hipMemsetAsync(h->tile_count, 0, 8, h->stream));
   launch_tile_list(h);
   HG_HIP(h->ctx, hipStreamSynchronize(h->stream));
   return HG_OK;

@@ -27,7 +27,8 @@ struct hg_mpm {
   uint32_t *d_p_off = nullptr;
   uint32_t *d_tile_scene = nullptr;
   uint8_t *tile_active = nullptr;
-  uint32_t *tile_list = nullptr, *tile_count = nullptr;  // active tiles of this step
+  uint32_t *tile_list = nullptr;
+  uint32_t *tile_count = nullptr;  // [2]: active tiles of the list of parity cur
   void *gva = nullptr;
   double *gm = nullptr, *gp = nullptr, *gpn = nullptr, *gf = nullptr;
   int32_t *err = nullptr;

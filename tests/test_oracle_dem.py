@@ -3,7 +3,9 @@ mathematics of the penalty map fixes: brute-force O(B^2) pair sets, the
 analytic linear spring-dashpot collision (restitution and contact time), a
 disk sliding on a plane (deceleration mu g, then rolling at 2/3 of the initial
 speed), exact conservation of linear and angular momentum by pair forces,
-the Coulomb cone, and exact free fall.
+the Coulomb cone, exact free fall, and the contact weights of flowWithWeights
+(a pair's mean weight and a plane contact's body weight scale every constant
+of the law).
 """
 import math
 
@@ -180,3 +182,47 @@ def test_fixed_body_contacts_are_kinematic(fixed_first):
     fx = 0 if fixed_first else 1
     # the fixed body keeps its velocity and drifts (only F is zeroed)
     assert st["v"][fx, 0] == s.v[fx, 0]
+
+
+def _scaled_constants(s, c):
+    t = s.copy()
+    t.kn, t.kt, t.gamman, t.gammat = s.kn * c, s.kt * c, s.gamman * c, s.gammat * c
+    return t
+
+
+def test_weighted_pair_uses_mean_weight():
+    """flowWithWeights scales every term of a pair's law by (w_i + w_j) / 2
+    (PenaltyImpactFrictionMap.cpp:414-416): with weights 0.2 and 0.6 the
+    collision equals the unweighted one with all four constants times 0.4,
+    including the Coulomb branch (a glancing impact)."""
+    s = _two_disks()
+    s.v[0, 1] = 0.8  # tangential slip: exercises the friction projection
+    w = np.array([0.2, 0.6])
+    a = oracle.OracleDem(s, weights=w)
+    b = oracle.OracleDem(_scaled_constants(s, 0.4))
+    a.flow(2e-7, 400)
+    b.flow(2e-7, 400)
+    sa, sb = a.state(), b.state()
+    assert np.abs(sa["v"] - sb["v"]).max() <= 1e-12 * np.abs(sb["v"]).max()
+    assert np.abs(sa["q"] - sb["q"]).max() <= 1e-15
+    # and it differs from the unweighted collision
+    c = oracle.OracleDem(s)
+    c.flow(2e-7, 400)
+    assert np.abs(c.state()["v"] - sb["v"]).max() > 1e-3
+
+
+def test_weighted_plane_uses_body_weight():
+    """A disk pressed onto a plane with weight 0.3 behaves as the unweighted
+    disk with constants times 0.3 (plane contacts take w_i)."""
+    r = 1e-3
+    radius = np.array([r])
+    mass = circle_mass(2500.0, radius)
+    s = DemScene(q=np.array([[0.0, 0.9 * r, 0.0]]), v=np.array([[0.3, -0.2, 0.0]]), radius=radius,
+                 mass=mass, fixed=np.zeros(1, np.uint8), gravity=np.array([0.0, -9.81]),
+                 kn=1e5, kt=1e5, gamman=0.5, gammat=0.5, mu=0.3)
+    s.add_plane([0.0, 0.0], [0.0, 1.0])
+    a = oracle.OracleDem(s, weights=np.array([0.3]))
+    b = oracle.OracleDem(_scaled_constants(s, 0.3))
+    a.flow(1e-6, 300)
+    b.flow(1e-6, 300)
+    assert np.abs(a.state()["v"] - b.state()["v"]).max() <= 1e-12 * np.abs(b.state()["v"]).max()
