@@ -53,6 +53,7 @@ def _time_hybrid(m, d, w, precision, steps, warmup, dt):
     from libhybridgrains_amd import GpuHybrid
 
     h = GpuHybrid(m, d, w, precision)
+    tune = h.continuum.autotune(dt)
     h.step(dt, warmup, graph=True)
     h.sync()
     t0 = time.perf_counter()
@@ -67,6 +68,7 @@ def _time_hybrid(m, d, w, precision, steps, warmup, dt):
     kt = dict(h.continuum.kernel_times())
     kt.update({"dem:" + k: v for k, v in h.discrete.kernel_times().items()})
     h.close()
+    kt["_tune"] = tune
     return wall, kt
 
 
@@ -78,6 +80,7 @@ def _time_batched(args, dt=1e-5):
     ss = scenes.mpm_batch_scenes(nscenes=64, seed=0)
     N = sum(s.npoints for s in ss)
     g = GpuMpm(ss, HG_FP32)
+    tune = g.autotune(dt)
     g.flow(dt, args.warmup, graph=True)
     g.sync()
     steps = max(10, args.steps // 2)
@@ -87,7 +90,8 @@ def _time_batched(args, dt=1e-5):
     wall = time.perf_counter() - t0
     g.close()
     return {"value": N * steps / wall, "unit": "particle-steps/s", "scenes": len(ss),
-            "points": N, "steps": steps, "ms_per_step": 1e3 * wall / steps}
+            "points": N, "steps": steps, "ms_per_step": 1e3 * wall / steps,
+            "autotune": tune}
 
 
 def _time_dem(args, dt=1e-5):
@@ -318,6 +322,7 @@ def main() -> int:
     for name, prec in (("fp32", HG_FP32), ("fp64", HG_FP64)):
         print(f"[bench] hybrid 1M {name}", file=sys.stderr, flush=True)
         wall, kt = _time_hybrid(m, d, w, prec, steps, args.warmup, DT_1M)
+        tune = kt.pop("_tune")
         transfer_ms = sum(kt[k][0] for k in ("mpm_p2g", "mpm_g2p")) / steps
         alg = ALG_BYTES[name] * N
         achieved = alg / (transfer_ms * 1e-3) / 1e9
@@ -325,6 +330,7 @@ def main() -> int:
             "value": (N + B) * steps / wall,
             "ms_per_step": 1e3 * wall / steps,
             "kernels_ms": {k: v[0] / v[1] for k, v in kt.items()},
+            "autotune": tune,
             "roofline": {
                 "bound": "hbm",
                 "kernel": "mpm_p2g+mpm_g2p",
@@ -381,7 +387,8 @@ def main() -> int:
         "roofline": f32["roofline"],
         "cpu_baseline": cpu,
         "fp64_plain": {"value": f64["value"], "ms_per_step": f64["ms_per_step"],
-                       "roofline": f64["roofline"]},
+                       "roofline": f64["roofline"], "autotune": f64["autotune"]},
+        "autotune": f32["autotune"],
         "fp32_over_fp64": f32["value"] / f64["value"],
         "batched_fp32": batched,
         "batched_fp32_over_plain_fp64": batched["value"] / f64["value"],

@@ -409,7 +409,7 @@ extern "C" hg_status hg_hybrid_step_graph(hg_hybrid *h, double dt, uint32_t nste
   if (h->mpm->phase_next != 0) return hg_fail(h->ctx, HG_ERR_INVALID, "a phased MPM step is in progress");
   if (h->mpm->prof.on || h->dem->prof.on) return hg_hybrid_step(h, dt, nsteps);
   HG_TRY(hg_dem_step(h->dem, dt, 0));
-  if (dt != h->graph_dt && h->graph) {
+  if (h->graph && (dt != h->graph_dt || h->mpm->aw != h->graph_aw || h->mpm->ah != h->graph_ah)) {
     hipGraphExecDestroy(h->graph);
     h->graph = nullptr;
   }
@@ -433,6 +433,8 @@ extern "C" hg_status hg_hybrid_step_graph(hg_hybrid *h, double dt, uint32_t nste
       HG_HIP(h->ctx, hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0));
       HG_HIP(h->ctx, hipGraphDestroy(g));
       h->graph_dt = dt;
+      h->graph_aw = h->mpm->aw;
+      h->graph_ah = h->mpm->ah;
     }
     for (uint32_t k = 0; k < pairs; ++k) HG_HIP(h->ctx, hipGraphLaunch(h->graph, h->stream));
   }

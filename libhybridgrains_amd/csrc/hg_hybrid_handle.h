@@ -31,6 +31,7 @@ struct hg_hybrid {
   int32_t *err = nullptr;
   hipGraphExec_t graph = nullptr;
   double graph_dt = -1.0;
+  int graph_aw = 0, graph_ah = 0;  // MPM tile shape the graph was captured with
   std::vector<void *> allocs;
   uint64_t bytes = 0;  // device bytes of the coupling buffers
 };

@@ -74,6 +74,33 @@ __device__ __forceinline__ uint32_t hg_xcd_remap(uint32_t b, uint32_t nblocks_pa
 }
 static inline uint32_t hg_pad8(uint32_t n) { return (n + 7u) & ~7u; }
 
+// Slot of this lane's item among the items of its key: one atomicAdd per
+// distinct key of the wavefront (by the key's lowest lane, for the whole
+// group), instead of one per lane on the same few counters.  Lanes of a key
+// get consecutive slots in lane order.  Call with the wavefront converged;
+// inactive lanes take no part.
+__device__ __forceinline__ uint32_t hg_wave_slot(uint32_t *counts, uint32_t key) {
+  const int lane = threadIdx.x & 63;
+  uint64_t pending = __ballot(1);
+  int leader = lane;
+  uint32_t rank = 0, group = 0;
+  while (pending) {
+    const int l = __ffsll((unsigned long long)pending) - 1;
+    const uint32_t kl = __shfl(key, l, 64);
+    const uint64_t same = __ballot(key == kl) & pending;
+    if ((same >> lane) & 1ull) {
+      leader = l;
+      rank = uint32_t(__popcll(same & ((1ull << lane) - 1ull)));
+    }
+    if (lane == l) group = uint32_t(__popcll(same));
+    pending &= ~same;
+  }
+  uint32_t base = 0;
+  if (leader == lane) base = atomicAdd(&counts[key], group);
+  base = __shfl(base, leader, 64);
+  return base + rank;
+}
+
 // Wavefront (64-lane) inclusive prefix sum of 32-bit values.
 __device__ __forceinline__ uint32_t hg_wave_incl_scan(uint32_t v) {
   const int lane = threadIdx.x & 63;

@@ -179,10 +179,9 @@ __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
   // per staged point: x weights and x derivative factors of its S x nodes,
   // the same for y, and the mass, momentum and stress terms
   __shared__ T s_wx[S][CAP], s_gx[S][CAP], s_wy[S][CAP], s_gy[S][CAP];
-  __shared__ T s_m[CAP], s_mv0[CAP], s_mv1[CAP], s_c[CAP], s_s00[CAP], s_s01[CAP], s_s11[CAP];
+  __shared__ T s_m[CAP], s_mv0[CAP], s_mv1[CAP], s_f00[CAP], s_f01[CAP], s_f11[CAP];
   __shared__ uint8_t s_col[CAP];               // apron column of the point's cell
-  __shared__ uint32_t s_p[CAP], s_id[CAP];     // point index and id, in sort order
-  __shared__ uint16_t s_rc[CAP];               // apron row and column, in sort order
+  __shared__ uint32_t s_id[CAP];               // point ids, in sort order
   __shared__ uint32_t s_cs[AH][AW + 1];        // cell starts of the apron rows
   __shared__ uint32_t s_ro[AH + 1];            // row offsets in the concatenated list
   __shared__ uint32_t s_k1;                    // end of the current chunk
@@ -269,60 +268,71 @@ __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
       __syncthreads();
       const uint32_t k1 = s_k1;
 
-      // 1a. the chunk's point indices and ids
-      for (uint32_t k = k0 + uint32_t(t); k < k1; k += NT) {
-        int r, col;
-        locate(k, r, col);
-        const uint32_t p = args.perm[s_cs[r][0] + (k - s_ro[r])];
-        const uint32_t q = k - k0;
-        s_p[q] = p;
-        s_id[q] = args.pts.id[p];
-        s_rc[q] = uint16_t((r << 8) | col);
+      // 1a. load the chunk's points (records stay in registers) and publish
+      //     their ids
+      constexpr int PER = (CAP + NT - 1) / NT;
+      P2GRec<T> rec[PER];
+      uint32_t myid[PER];
+      int rrow[PER], rcol[PER];
+#pragma unroll
+      for (int m = 0; m < PER; ++m) {
+        const uint32_t k = k0 + uint32_t(t) + uint32_t(m * NT);
+        if (k < k1) {
+          locate(k, rrow[m], rcol[m]);
+          const uint32_t p = args.perm[s_cs[rrow[m]][0] + (k - s_ro[rrow[m]])];
+          myid[m] = args.pts.id[p];
+          rec[m] = p2g_record<T, BASIS>(args.pts, p, sc);
+          s_id[k - k0] = myid[m];
+        }
       }
       __syncthreads();
 
       // 1b. stage each point at its id rank within its cell
-      for (uint32_t k = k0 + uint32_t(t); k < k1; k += NT) {
-        const uint32_t q = k - k0;
-        const int r = int(s_rc[q] >> 8), col = int(s_rc[q] & 255u);
-        const uint32_t cb0 = s_ro[r] + (s_cs[r][col] - s_cs[r][0]);
-        const uint32_t ce0 = s_ro[r] + (s_cs[r][col + 1] - s_cs[r][0]);
-        const uint32_t cb = max(cb0, k0) - k0, ce = min(ce0, k1) - k0;
-        const uint32_t myid = s_id[q];
-        uint32_t rank = 0;
-        for (uint32_t v = cb; v < ce; ++v) rank += (s_id[v] < myid) ? 1u : 0u;
-        const uint32_t d = cb + rank;
-        const P2GRec<T> rec = p2g_record<T, BASIS>(args.pts, s_p[q], sc);
-        s_col[d] = uint8_t(col);
-        const int bx = ax0 + col, by = ay0 + r;
-        const T inv_h = T(sc.inv_h);
-        const int cntx = int(rec.cnt & 15u), cnty = int(rec.cnt >> 4);
-        T inv_vp = T(1);
-        if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) inv_vp = T(1) / (T(4) * rec.hl * rec.hl);
 #pragma unroll
-        for (int k2 = 0; k2 < S; ++k2) {
-          T ax, dx, ay, dy;
-          basis_factor<T, BASIS>(rec.x, rec.hl, sc.min0, sc.h, inv_h, bx + k2, k2, cntx, ax, dx);
-          basis_factor<T, BASIS>(rec.y, rec.hl, sc.min1, sc.h, inv_h, by + k2, k2, cnty, ay, dy);
-          if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) {
-            s_wx[k2][d] = ax * inv_vp;
-            s_gx[k2][d] = dx * inv_vp;
-            s_wy[k2][d] = ay;
-            s_gy[k2][d] = dy;
-          } else {
-            s_wx[k2][d] = ax;
-            s_gx[k2][d] = dx * inv_h;
-            s_wy[k2][d] = ay;
-            s_gy[k2][d] = dy * inv_h;
+      for (int m = 0; m < PER; ++m) {
+        const uint32_t k = k0 + uint32_t(t) + uint32_t(m * NT);
+        if (k < k1) {
+          const int r = rrow[m], col = rcol[m];
+          const uint32_t cb0 = s_ro[r] + (s_cs[r][col] - s_cs[r][0]);
+          const uint32_t ce0 = s_ro[r] + (s_cs[r][col + 1] - s_cs[r][0]);
+          const uint32_t cb = max(cb0, k0) - k0, ce = min(ce0, k1) - k0;
+          uint32_t rank = 0;
+          for (uint32_t v = cb; v < ce; ++v) rank += (s_id[v] < myid[m]) ? 1u : 0u;
+          const uint32_t d = cb + rank;
+          s_col[d] = uint8_t(col);
+          const int bx = ax0 + col, by = ay0 + r;
+          const T inv_h = T(sc.inv_h);
+          const int cntx = int(rec[m].cnt & 15u), cnty = int(rec[m].cnt >> 4);
+          T inv_vp = T(1);
+          if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR)
+            inv_vp = T(1) / (T(4) * rec[m].hl * rec[m].hl);
+#pragma unroll
+          for (int k2 = 0; k2 < S; ++k2) {
+            T ax, dx, ay, dy;
+            basis_factor<T, BASIS>(rec[m].x, rec[m].hl, sc.min0, sc.h, inv_h, bx + k2, k2, cntx,
+                                   ax, dx);
+            basis_factor<T, BASIS>(rec[m].y, rec[m].hl, sc.min1, sc.h, inv_h, by + k2, k2, cnty,
+                                   ay, dy);
+            if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) {
+              s_wx[k2][d] = ax * inv_vp;
+              s_gx[k2][d] = dx * inv_vp;
+              s_wy[k2][d] = ay;
+              s_gy[k2][d] = dy;
+            } else {
+              s_wx[k2][d] = ax;
+              s_gx[k2][d] = dx * inv_h;
+              s_wy[k2][d] = ay;
+              s_gy[k2][d] = dy * inv_h;
+            }
           }
+          s_m[d] = rec[m].m;
+          s_mv0[d] = rec[m].mv0;
+          s_mv1[d] = rec[m].mv1;
+          // -V J sigma: the point's force per unit weight gradient
+          s_f00[d] = rec[m].c * rec[m].s00;
+          s_f01[d] = rec[m].c * rec[m].s01;
+          s_f11[d] = rec[m].c * rec[m].s11;
         }
-        s_m[d] = rec.m;
-        s_mv0[d] = rec.mv0;
-        s_mv1[d] = rec.mv1;
-        s_c[d] = rec.c;
-        s_s00[d] = rec.s00;
-        s_s01[d] = rec.s01;
-        s_s11[d] = rec.s11;
       }
       __syncthreads();
 
@@ -344,9 +354,9 @@ __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
             ap0 += double(w * s_mv0[q]);
             ap1 += double(w * s_mv1[q]);
             if (w != T(0)) {
-              const T c = s_c[q];
-              af0 += double(c * (s_s00[q] * gx + s_s01[q] * gy));
-              af1 += double(c * (s_s01[q] * gx + s_s11[q] * gy));
+              const T f01 = s_f01[q];
+              af0 += double(s_f00[q] * gx + f01 * gy);
+              af1 += double(f01 * gx + s_f11[q] * gy);
             }
           }
         }
@@ -696,7 +706,7 @@ __global__ __launch_bounds__(kG2PThreads) void mpm_g2p_kernel(G2PArgs a) {
   if (!ok || !finite) atomicOr(&a.err[sid], (ok ? 0 : 1) | (finite ? 0 : 2));
   const uint32_t k = sc.cell_off + uint32_t(nby) * uint32_t(sc.snx) + uint32_t(nbx);
   a.key[o] = k;
-  a.slot[o] = atomicAdd(&a.counts[k], 1u);
+  a.slot[o] = hg_wave_slot(a.counts, k);
 }
 
 // Initial binning of points in creation order.
@@ -1501,5 +1511,55 @@ extern "C" hg_status hg_mpm_estimate_bytes(const hg_mpm_desc *descs, uint32_t ns
 extern "C" hg_status hg_mpm_device_bytes(hg_mpm *h, uint64_t *bytes) {
   if (!h || !bytes) return HG_ERR_INVALID;
   *bytes = h->bytes;
+  return HG_OK;
+}
+
+// Tile-shape autotuner: times the P2G pass (which reads the point state and
+// writes only grid fields that every step recomputes) for each supported
+// tile shape on the handle's own scenes and keeps the fastest.  The choice
+// changes no result: a node's sum order does not depend on the tile shape.
+extern "C" hg_status hg_mpm_autotune(hg_mpm *h, double dt, int32_t reps, int32_t *aw_out,
+                                     int32_t *ah_out, double *us /* [3] or NULL */) {
+  if (!h) return HG_ERR_INVALID;
+  if (!(dt > 0.0)) return hg_fail(h->ctx, HG_ERR_INVALID, "dt must be > 0");
+  if (h->phase_next != 0) return hg_fail(h->ctx, HG_ERR_INVALID, "a phased step is in progress");
+  if (reps < 1) reps = 1;
+  HG_TRY(upload_dt(h, dt));
+  const int shapes[3][2] = {{16, 16}, {16, 8}, {32, 8}};
+  const bool was_on = h->prof.on;
+  h->prof.on = false;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HG_HIP(h->ctx, hipEventCreate(&e0));
+  HG_HIP(h->ctx, hipEventCreate(&e1));
+  double best = 1e300;
+  int bi = 0;
+  hg_status st = HG_OK;
+  for (int s = 0; s < 3 && st == HG_OK; ++s) {
+    st = hg_mpm_set_tile(h, shapes[s][0], shapes[s][1]);
+    if (st != HG_OK) break;
+    launch_p2g(h, kStoreVelAcc);  // warm-up
+    hipEventRecord(e0, h->stream);
+    for (int r = 0; r < reps; ++r) launch_p2g(h, kStoreVelAcc);
+    hipEventRecord(e1, h->stream);
+    if (hipEventSynchronize(e1) != hipSuccess) {
+      st = hg_fail(h->ctx, HG_ERR_HIP, "autotune timing failed");
+      break;
+    }
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    const double t = 1e3 * double(ms) / reps;
+    if (us) us[s] = t;
+    if (t < best) {
+      best = t;
+      bi = s;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  h->prof.on = was_on;
+  HG_TRY(st);
+  HG_TRY(hg_mpm_set_tile(h, shapes[bi][0], shapes[bi][1]));
+  if (aw_out) *aw_out = shapes[bi][0];
+  if (ah_out) *ah_out = shapes[bi][1];
   return HG_OK;
 }

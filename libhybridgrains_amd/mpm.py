@@ -147,6 +147,16 @@ class GpuMpm:
     def sync(self) -> None:
         self.ctx.check(self.L.hg_mpm_sync(self._h))
 
+    def autotune(self, dt: float, reps: int = 5) -> dict:
+        """Picks the fastest P2G tile shape for these scenes (results do not
+        depend on it); returns the choice and the timings."""
+        aw, ah = C.c_int32(), C.c_int32()
+        us = np.zeros(3)
+        self.ctx.check(self.L.hg_mpm_autotune(self._h, float(dt), int(reps), C.byref(aw),
+                                              C.byref(ah), ptr(us)))
+        return {"tile": (aw.value, ah.value),
+                "us": {"16x16": us[0], "16x8": us[1], "32x8": us[2]}}
+
     def scene_error(self, scene: int) -> int:
         f = C.c_int32()
         self.ctx.check(self.L.hg_mpm_scene_error(self._h, scene, C.byref(f)))

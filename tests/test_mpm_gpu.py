@@ -171,3 +171,19 @@ def test_point_leaving_grid_is_reported(hg_ctx):
         gpu.sync()
     assert ei.value.code == -4
     assert gpu.scene_error(0) & 1
+
+
+def test_autotune_picks_a_shape_and_changes_no_result(hg_ctx):
+    s = _small_scene(seed=9)
+    a = GpuMpm(s, HG_FP64, ctx=hg_ctx)
+    b = GpuMpm(s, HG_FP64, ctx=hg_ctx)
+    a.flow(DT, 3)
+    b.flow(DT, 3)
+    tune = b.autotune(DT, reps=2)
+    assert tune["tile"] in ((16, 16), (16, 8), (32, 8))
+    assert all(v > 0 for v in tune["us"].values())
+    a.flow(DT, 5, graph=True)
+    b.flow(DT, 5, graph=True)
+    pa, pb = a.points(), b.points()
+    for k in pa:
+        assert np.array_equal(pa[k], pb[k]), k
