@@ -11,6 +11,7 @@
 #include <cstring>
 #include <memory>
 #include <numeric>
+#include <type_traits>
 
 using namespace hgmpm;
 
@@ -132,19 +133,41 @@ __device__ __forceinline__ uint32_t stencil_counts(double x, double y, double hl
   return uint32_t(cntx) | (uint32_t(cnty) << 4);
 }
 
-template <typename T, int BASIS>
-__device__ __forceinline__ P2GRec<T> p2g_record(const PtsRaw &pts, uint32_t p, const SceneDev &sc) {
-  P2GRec<T> r;
+// A point's fields as P2G loads them.
+template <typename T>
+struct P2GRaw {
+  double x, y;
+  T m, v0, v1, V, J, b00, b01, b11, hl;
+  uint32_t id;
+};
+
+template <typename T>
+__device__ __forceinline__ P2GRaw<T> p2g_load(const PtsRaw &pts, uint32_t p, const SceneDev &sc) {
+  P2GRaw<T> r;
   r.x = pts.x[p];
   r.y = pts.y[p];
   r.m = ld<T>(pts.m, p);
-  const T v0 = ld<T>(pts.vx, p), v1 = ld<T>(pts.vy, p);
-  const T V = ld<T>(pts.V, p);
-  const double J = double(ld<T>(pts.J, p));
-  const double b00 = double(ld<T>(pts.b00, p));
-  const double b01 = double(ld<T>(pts.b01, p));
-  const double b11 = double(ld<T>(pts.b11, p));
+  r.v0 = ld<T>(pts.vx, p);
+  r.v1 = ld<T>(pts.vy, p);
+  r.V = ld<T>(pts.V, p);
+  r.J = ld<T>(pts.J, p);
+  r.b00 = ld<T>(pts.b00, p);
+  r.b01 = ld<T>(pts.b01, p);
+  r.b11 = ld<T>(pts.b11, p);
   r.hl = pts.hl ? ld<T>(pts.hl, p) : T(sc.hl_uniform);
+  r.id = pts.id[p];
+  return r;
+}
+
+template <typename T, int BASIS>
+__device__ __forceinline__ P2GRec<T> p2g_record(const P2GRaw<T> &w, const SceneDev &sc) {
+  P2GRec<T> r;
+  r.x = w.x;
+  r.y = w.y;
+  r.m = w.m;
+  r.hl = w.hl;
+  const double J = double(w.J);
+  const double b00 = double(w.b00), b01 = double(w.b01), b11 = double(w.b11);
   // Cauchy stress sigma = tau / J; tau = 0 when J > 1 (no tension).  FP64.
   double t00 = 0.0, t01 = 0.0, t11 = 0.0;
   if (J <= 1.0) {
@@ -163,12 +186,16 @@ __device__ __forceinline__ P2GRec<T> p2g_record(const PtsRaw &pts, uint32_t p, c
   r.s00 = T(t00 * iJ);
   r.s01 = T(t01 * iJ);
   r.s11 = T(t11 * iJ);
-  r.c = T(-double(V) * J);
-  r.mv0 = r.m * v0;
-  r.mv1 = r.m * v1;
+  r.c = T(-double(w.V) * J);
+  r.mv0 = r.m * w.v0;
+  r.mv1 = r.m * w.v1;
   r.cnt = stencil_counts<BASIS>(r.x, r.y, double(r.hl), sc);
   return r;
 }
+
+// Apron point indices kept in LDS per tile (larger aprons read them from HBM
+// chunk by chunk).
+constexpr int kP2GPermCap = 2048;
 
 template <typename T, int BASIS, int TW, int TH, int CAP>
 __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
@@ -182,9 +209,9 @@ __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
   __shared__ T s_m[CAP], s_mv0[CAP], s_mv1[CAP], s_f00[CAP], s_f01[CAP], s_f11[CAP];
   __shared__ uint8_t s_col[CAP];               // apron column of the point's cell
   __shared__ uint32_t s_id[CAP];               // point ids, in sort order
+  __shared__ uint32_t s_perm[kP2GPermCap];     // the apron's point indices, when they fit
   __shared__ uint32_t s_cs[AH][AW + 1];        // cell starts of the apron rows
   __shared__ uint32_t s_ro[AH + 1];            // row offsets in the concatenated list
-  __shared__ uint32_t s_k1;                    // end of the current chunk
 
   const uint32_t ntile_list = *args.tile_count;
   const int t = threadIdx.x;
@@ -247,43 +274,60 @@ __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
         if (col + step <= AW - 1 && s_cs[r][col + step] <= pos) col += step;
     };
 
-    // Chunks are consecutive pieces of the concatenated list that end on cell
-    // boundaries (a cell of more than CAP points is split).  Within a cell the
-    // points are staged in id order; each node walks its runs in ascending
-    // list order within every chunk, so its sum order is the list order
-    // (rows, cells, ids) whatever the chunk boundaries.
-    uint32_t k0 = 0;
-    while (k0 < total) {
-      if (t == 0) {
-        const uint32_t target = min(k0 + uint32_t(CAP), total);
-        uint32_t k1 = target;
-        if (target < total) {
-          int r, col;
-          locate(target, r, col);
-          const uint32_t cstart = s_ro[r] + (s_cs[r][col] - s_cs[r][0]);
-          if (cstart > k0) k1 = cstart;  // else: one cell fills the chunk; split it
-        }
-        s_k1 = k1;
+    // the apron's point indices, in list order, when they fit in LDS: every
+    // chunk then needs one round trip to HBM instead of two
+    const bool perm_lds = total <= uint32_t(kP2GPermCap);
+    if (perm_lds) {
+      for (uint32_t k = uint32_t(t); k < total; k += NT) {
+        int r, col;
+        locate(k, r, col);
+        s_perm[k] = args.perm[s_cs[r][0] + (k - s_ro[r])];
       }
       __syncthreads();
-      const uint32_t k1 = s_k1;
+    }
 
-      // 1a. load the chunk's points (records stay in registers) and publish
-      //     their ids
-      constexpr int PER = (CAP + NT - 1) / NT;
-      P2GRec<T> rec[PER];
-      uint32_t myid[PER];
-      int rrow[PER], rcol[PER];
+    // end of the chunk that starts at k0 (every thread computes it): at most
+    // CAP points, ending on a cell boundary unless one cell fills the chunk
+    auto chunk_end = [&](uint32_t k0) {
+      const uint32_t target = min(k0 + uint32_t(CAP), total);
+      if (target >= total) return total;
+      int r, col;
+      locate(target, r, col);
+      const uint32_t cstart = s_ro[r] + (s_cs[r][col] - s_cs[r][0]);
+      return cstart > k0 ? cstart : target;
+    };
+    constexpr int PER = (CAP + NT - 1) / NT;
+    // a chunk's points as loaded (through the permutation) for one thread
+    auto load_chunk = [&](uint32_t k0, uint32_t k1, P2GRaw<T> (&raw)[PER], int (&rrow)[PER],
+                          int (&rcol)[PER]) {
 #pragma unroll
       for (int m = 0; m < PER; ++m) {
         const uint32_t k = k0 + uint32_t(t) + uint32_t(m * NT);
         if (k < k1) {
           locate(k, rrow[m], rcol[m]);
-          const uint32_t p = args.perm[s_cs[rrow[m]][0] + (k - s_ro[rrow[m]])];
-          myid[m] = args.pts.id[p];
-          rec[m] = p2g_record<T, BASIS>(args.pts, p, sc);
-          s_id[k - k0] = myid[m];
+          const uint32_t p =
+              perm_lds ? s_perm[k] : args.perm[s_cs[rrow[m]][0] + (k - s_ro[rrow[m]])];
+          raw[m] = p2g_load<T>(args.pts, p, sc);
         }
+      }
+    };
+
+    // Chunks are consecutive pieces of the concatenated list that end on cell
+    // boundaries (a cell of more than CAP points is split).  Within a cell the
+    // points are staged in id order; each node walks its runs in ascending
+    // list order within every chunk, so its sum order is the list order
+    // (rows, cells, ids) whatever the chunk boundaries.  The next chunk's
+    // points are loaded while the current one is gathered.
+    P2GRaw<T> raw[PER];
+    int rrow[PER], rcol[PER];
+    uint32_t k0 = 0, k1 = chunk_end(0);
+    if (k0 < total) load_chunk(k0, k1, raw, rrow, rcol);
+    while (k0 < total) {
+      // 1a. publish the chunk's ids
+#pragma unroll
+      for (int m = 0; m < PER; ++m) {
+        const uint32_t k = k0 + uint32_t(t) + uint32_t(m * NT);
+        if (k < k1) s_id[k - k0] = raw[m].id;
       }
       __syncthreads();
 
@@ -297,22 +341,22 @@ __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
           const uint32_t ce0 = s_ro[r] + (s_cs[r][col + 1] - s_cs[r][0]);
           const uint32_t cb = max(cb0, k0) - k0, ce = min(ce0, k1) - k0;
           uint32_t rank = 0;
-          for (uint32_t v = cb; v < ce; ++v) rank += (s_id[v] < myid[m]) ? 1u : 0u;
+          for (uint32_t v = cb; v < ce; ++v) rank += (s_id[v] < raw[m].id) ? 1u : 0u;
           const uint32_t d = cb + rank;
+          const P2GRec<T> rec = p2g_record<T, BASIS>(raw[m], sc);
           s_col[d] = uint8_t(col);
           const int bx = ax0 + col, by = ay0 + r;
           const T inv_h = T(sc.inv_h);
-          const int cntx = int(rec[m].cnt & 15u), cnty = int(rec[m].cnt >> 4);
+          const int cntx = int(rec.cnt & 15u), cnty = int(rec.cnt >> 4);
           T inv_vp = T(1);
-          if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR)
-            inv_vp = T(1) / (T(4) * rec[m].hl * rec[m].hl);
+          if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) inv_vp = T(1) / (T(4) * rec.hl * rec.hl);
 #pragma unroll
           for (int k2 = 0; k2 < S; ++k2) {
             T ax, dx, ay, dy;
-            basis_factor<T, BASIS>(rec[m].x, rec[m].hl, sc.min0, sc.h, inv_h, bx + k2, k2, cntx,
-                                   ax, dx);
-            basis_factor<T, BASIS>(rec[m].y, rec[m].hl, sc.min1, sc.h, inv_h, by + k2, k2, cnty,
-                                   ay, dy);
+            basis_factor<T, BASIS>(rec.x, rec.hl, sc.min0, sc.h, inv_h, bx + k2, k2, cntx, ax,
+                                   dx);
+            basis_factor<T, BASIS>(rec.y, rec.hl, sc.min1, sc.h, inv_h, by + k2, k2, cnty, ay,
+                                   dy);
             if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) {
               s_wx[k2][d] = ax * inv_vp;
               s_gx[k2][d] = dx * inv_vp;
@@ -325,19 +369,34 @@ __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
               s_gy[k2][d] = dy * inv_h;
             }
           }
-          s_m[d] = rec[m].m;
-          s_mv0[d] = rec[m].mv0;
-          s_mv1[d] = rec[m].mv1;
+          s_m[d] = rec.m;
+          s_mv0[d] = rec.mv0;
+          s_mv1[d] = rec.mv1;
           // -V J sigma: the point's force per unit weight gradient
-          s_f00[d] = rec[m].c * rec[m].s00;
-          s_f01[d] = rec[m].c * rec[m].s01;
-          s_f11[d] = rec[m].c * rec[m].s11;
+          s_f00[d] = rec.c * rec.s00;
+          s_f01[d] = rec.c * rec.s01;
+          s_f11[d] = rec.c * rec.s11;
         }
       }
       __syncthreads();
 
-      // 2. gather: rows ascending, then the run (cells ascending, ids ascending)
+      // the next chunk's loads run under this chunk's gather
+      const uint32_t nk0 = k1, nk1 = nk0 < total ? chunk_end(nk0) : total;
+      if (nk0 < total) load_chunk(nk0, nk1, raw, rrow, rcol);
+
+      // 2. gather: rows ascending, then the run (cells ascending, ids ascending).
+      //    FP64 mode sums every term into the FP64 node sums; FP32 mode sums a
+      //    chunk's terms in FP32 and adds that partial to the FP64 node sums.
       if (nx < sc.nnx && ny < sc.nny) {
+        using Acc = T;
+        Acc cm = 0, cp0 = 0, cp1 = 0, cf0 = 0, cf1 = 0;
+        if constexpr (std::is_same<T, double>::value) {
+          cm = am;
+          cp0 = ap0;
+          cp1 = ap1;
+          cf0 = af0;
+          cf1 = af1;
+        }
 #pragma unroll
         for (int j0 = 0; j0 < S; ++j0) {
           const int j = S - 1 - j0;  // cell row ny - j: weight index j
@@ -350,19 +409,33 @@ __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
             const T w = wx * wy;
             const T gx = gxs * wy;
             const T gy = wx * gys;
-            am += double(w * s_m[q]);
-            ap0 += double(w * s_mv0[q]);
-            ap1 += double(w * s_mv1[q]);
+            cm += w * s_m[q];
+            cp0 += w * s_mv0[q];
+            cp1 += w * s_mv1[q];
             if (w != T(0)) {
               const T f01 = s_f01[q];
-              af0 += double(s_f00[q] * gx + f01 * gy);
-              af1 += double(f01 * gx + s_f11[q] * gy);
+              cf0 += s_f00[q] * gx + f01 * gy;
+              cf1 += f01 * gx + s_f11[q] * gy;
             }
           }
         }
+        if constexpr (std::is_same<T, double>::value) {
+          am = cm;
+          ap0 = cp0;
+          ap1 = cp1;
+          af0 = cf0;
+          af1 = cf1;
+        } else {
+          am += double(cm);
+          ap0 += double(cp0);
+          ap1 += double(cp1);
+          af0 += double(cf0);
+          af1 += double(cf1);
+        }
       }
       __syncthreads();
-      k0 = k1;
+      k0 = nk0;
+      k1 = nk1;
     }
 
     // 3. grid update of the node.

@@ -19,8 +19,8 @@
 //
 // Kernels of one step: p2g (node gather over the active tiles, also the whole
 // grid update), g2p (gather + return map + advection + re-key), then the
-// counting sort (scan_reduce, scan_top, scan_apply, scatter), the in-cell
-// id order (cell_order) and the active-tile list (tile_list).
+// counting sort (scan_reduce, scan_apply, scatter) and the active-tile list
+// (tile_list).
 #pragma once
 
 #include "hg_internal.h"

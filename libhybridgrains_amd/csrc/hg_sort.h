@@ -14,7 +14,7 @@ constexpr int kScanThreads = 1024;
 constexpr int kScanItems = 4;
 constexpr int kScanChunk = kScanThreads * kScanItems;
 
-// Kernels: scan_reduce, scan_top, scan_apply, scatter (4 launches).
+// Kernels: scan_reduce, scan_apply, scatter (3 launches).
 void hg_counting_sort(KernelProfiler &prof, hipStream_t stream, uint32_t *counts, uint32_t ncell,
                       uint32_t *block_sums, uint32_t nscan, uint32_t *cell_start,
                       const uint32_t *key, const uint32_t *slot, uint32_t n, uint32_t *perm,

@@ -5,9 +5,10 @@
 namespace {
 
 // ---------------------------------------------------------------------------
-// Counting sort: exclusive scan of the per-cell counts (three passes), then a
-// scatter of point indices.  scan_apply also zeroes the counts for the next
-// step's G2P.
+// Counting sort: exclusive scan of the per-cell counts (two passes: block
+// totals, then block scans offset by the totals before them), then a scatter
+// of point indices.  scan_apply also zeroes the counts for the next step's
+// G2P.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t block_excl_scan_1024(uint32_t v, uint32_t &total) {
   __shared__ uint32_t s_wave[16];
@@ -40,28 +41,28 @@ __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_
   if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(kScanThreads) void scan_top_kernel(uint32_t *block_sums,
-                                                                uint32_t nb) {
-  uint32_t vals[kScanItems];
-  uint32_t s = 0;
-  const uint32_t base = threadIdx.x * kScanItems;
-#pragma unroll
-  for (int k = 0; k < kScanItems; ++k) {
-    vals[k] = (base + k < nb) ? block_sums[base + k] : 0u;
-    s += vals[k];
-  }
-  uint32_t total = 0;
-  uint32_t run = block_excl_scan_1024(s, total);
-#pragma unroll
-  for (int k = 0; k < kScanItems; ++k) {
-    if (base + k < nb) block_sums[base + k] = run;
-    run += vals[k];
-  }
-}
-
+// Each block first sums the totals of the blocks before it (at most
+// kScanChunk of them, one per thread), so no separate pass over the block
+// totals is needed.
 __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(uint32_t *counts, uint32_t n,
-                                                                  const uint32_t *block_off,
+                                                                  const uint32_t *block_sums,
                                                                   uint32_t *cell_start) {
+  __shared__ uint32_t s_part[kScanThreads / 64];
+  uint32_t before = 0;
+  for (uint32_t b = threadIdx.x; b < blockIdx.x; b += kScanThreads) before += block_sums[b];
+  {
+    // block sum of 'before'
+    uint32_t v = before;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t tot = 0;
+#pragma unroll
+    for (int w = 0; w < kScanThreads / 64; ++w) tot += s_part[w];
+    before = tot;
+    __syncthreads();
+  }
   const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * kScanItems;
   uint32_t vals[kScanItems];
   uint32_t s = 0;
@@ -71,7 +72,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(uint32_t *coun
     s += vals[k];
   }
   uint32_t total = 0;
-  uint32_t run = block_excl_scan_1024(s, total) + block_off[blockIdx.x];
+  uint32_t run = block_excl_scan_1024(s, total) + before;
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
     if (base + k < n) {
@@ -103,10 +104,6 @@ void hg_counting_sort(KernelProfiler &prof, hipStream_t stream, uint32_t *counts
     ProfScope ps(prof, "scan_reduce", stream);
     hipLaunchKernelGGL(scan_reduce_kernel, dim3(nscan), dim3(kScanThreads), 0, stream, counts,
                        ncell, block_sums);
-  }
-  {
-    ProfScope ps(prof, "scan_top", stream);
-    hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(kScanThreads), 0, stream, block_sums, nscan);
   }
   {
     ProfScope ps(prof, "scan_apply", stream);
