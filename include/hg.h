@@ -201,8 +201,9 @@ typedef struct hg_dem_desc {
   const double *plane_n;   /* 2P, unit */
   double kn, kt, gamman, gammat, mu;
   double cell_width;       /* broad-phase cell; at least 2 * max radius (0: that) */
-  double domain_min[2];    /* broad-phase box; bodies outside clamp to its rim */
-  double domain_max[2];    /* (max <= min: derived from the bodies and planes) */
+  double domain_min[2];    /* extent the broad-phase bucket table spans; bodies */
+  double domain_max[2];    /* outside it stay correct (the table wraps); max <= min:
+                              the bodies' box with as much room again to spread */
 } hg_dem_desc;
 
 hg_status hg_dem_create(hg_ctx *ctx, const hg_dem_desc *desc, hg_dem **out);

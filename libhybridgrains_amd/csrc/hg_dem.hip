@@ -9,8 +9,13 @@
 //  * bodies: x, y, theta (FP64), vx, vy, omega (T), radius (FP64: the pair
 //    set is decided in FP64 in every precision), 1/m, 1/I, m (T), fixed, and
 //    optional contact weights (T).
-//  * broad phase: a uniform cell grid of width >= the largest diameter per
-//    scene, points counting-sorted by cell every step (hg_sort).
+//  * broad phase: cells of width >= the largest diameter, folded onto a
+//    wrapped table of Wx x Wy buckets per scene (powers of two, each >= 4):
+//    bucket = (cx mod Wx) + Wx (cy mod Wy).  The table covers the bodies' box
+//    with room to spread, so the buckets of a compact scene are its cells;
+//    cells a table width apart share a bucket, which only adds candidates
+//    the distance test rejects (the 3x3 neighbour buckets stay distinct).
+//    Bodies are counting-sorted by bucket every step (hg_sort).
 //  * contacts: each contact is owned by one body -- the smaller index of a
 //    circle pair, or the simulated body of a pair with a fixed body, as the
 //    reference keys its CollisionTypeCache.  A body keeps up to kMaxOwned
@@ -52,13 +57,6 @@ __device__ __forceinline__ uint32_t scene_of(const uint32_t *b_off, uint32_t ns,
   return lo;
 }
 
-__device__ __forceinline__ void cell_of(const SceneDev &sc, double x, double y, int &cx, int &cy) {
-  const double fx = floor((x - sc.cmin0) / sc.cw), fy = floor((y - sc.cmin1) / sc.cw);
-  // clamping keeps neighbouring bodies in neighbouring cells, so the 3x3
-  // search stays complete for bodies that leave the grid's box
-  cx = (fx >= 0.0) ? int(fmin(fx, double(sc.ncx - 1))) : 0;
-  cy = (fy >= 0.0) ? int(fmin(fy, double(sc.ncy - 1))) : 0;
-}
 
 // One contact's penalty force with the Coulomb projection of the tangential
 // spring: PenaltyImpactFrictionMap::updateAnchorsAndAccumulateCollisionForces
@@ -157,44 +155,51 @@ __global__ __launch_bounds__(kThreads) void dem_contacts_kernel(ContactArgs a) {
   bool overflow = false;
 
   int cx, cy;
-  cell_of(sc, xi, yi, cx, cy);
+  dem_cell_of(sc, xi, yi, cx, cy);
   const uint32_t *cs = a.cell_start + sc.cell_off;
-  for (int yy = max(cy - 1, 0); yy <= min(cy + 1, sc.ncy - 1); ++yy) {
-    const uint32_t k0 = uint32_t(yy) * uint32_t(sc.ncx) + uint32_t(max(cx - 1, 0));
-    const uint32_t k1 = uint32_t(yy) * uint32_t(sc.ncx) + uint32_t(min(cx + 1, sc.ncx - 1)) + 1u;
-    for (uint32_t t = cs[k0]; t < cs[k1]; ++t) {
-      const uint32_t j = a.perm[t];
-      if (j == i) continue;
-      const bool fj = a.b.fixed[j] != 0;
-      if (fi && fj) continue;
-      const double dx = xi - a.b.x[j], dy = yi - a.b.y[j];
-      const double rs = ri + a.b.r[j];
-      const double c0 = (dx * dx + dy * dy) - rs * rs;
-      if (!(c0 <= 0.0)) continue;
-      if (!fi && (fj || i < j)) {
-        const uint32_t key = (fj ? kKinematic : 0u) | (j - sc.b_off);
-        if (nown < kMaxOwned) {
-          // insertion into the sorted list
-          int q = nown++;
-          while (q > 0 && own[q - 1] > key) {
-            own[q] = own[q - 1];
-            --q;
+  const uint32_t wx = uint32_t(sc.ncx), bx0 = uint32_t(cx - 1) & (wx - 1u);
+  // the buckets of the 3x3 cells: per row one run of three consecutive
+  // buckets, or two runs where the row wraps (the second is empty otherwise)
+  const uint32_t len0 = min(bx0 + 3u, wx) - bx0;
+  for (int oy = -1; oy <= 1; ++oy) {
+    const uint32_t row = (uint32_t(cy + oy) & uint32_t(sc.ncy - 1)) * wx;
+    for (int part = 0; part < 2; ++part) {
+      const uint32_t k0 = part ? row : row + bx0;
+      const uint32_t k1 = part ? row + (3u - len0) : row + bx0 + len0;
+      for (uint32_t t = cs[k0]; t < cs[k1]; ++t) {
+        const uint32_t j = a.perm[t];
+        if (j == i) continue;
+        const bool fj = a.b.fixed[j] != 0;
+        if (fi && fj) continue;
+        const double dx = xi - a.b.x[j], dy = yi - a.b.y[j];
+        const double rs = ri + a.b.r[j];
+        const double c0 = (dx * dx + dy * dy) - rs * rs;
+        if (!(c0 <= 0.0)) continue;
+        if (!fi && (fj || i < j)) {
+          const uint32_t key = (fj ? kKinematic : 0u) | (j - sc.b_off);
+          if (nown < kMaxOwned) {
+            // insertion into the sorted list
+            int q = nown++;
+            while (q > 0 && own[q - 1] > key) {
+              own[q] = own[q - 1];
+              --q;
+            }
+            own[q] = key;
+          } else {
+            overflow = true;
           }
-          own[q] = key;
-        } else {
-          overflow = true;
-        }
-      } else if (!fi && !fj) {
-        const uint32_t key = j - sc.b_off;
-        if (nin < kMaxIn) {
-          int q = nin++;
-          while (q > 0 && inl[q - 1] > key) {
-            inl[q] = inl[q - 1];
-            --q;
+        } else if (!fi && !fj) {
+          const uint32_t key = j - sc.b_off;
+          if (nin < kMaxIn) {
+            int q = nin++;
+            while (q > 0 && inl[q - 1] > key) {
+              inl[q] = inl[q - 1];
+              --q;
+            }
+            inl[q] = key;
+          } else {
+            overflow = true;
           }
-          inl[q] = key;
-        } else {
-          overflow = true;
         }
       }
     }
@@ -382,9 +387,7 @@ __global__ __launch_bounds__(kThreads) void dem_integrate_kernel(IntegrateArgs a
   a.b.th[i] = a.b.th[i] + dt * double(tv2);
   if (!(isfinite(x) && isfinite(y) && isfinite(v2))) atomicOr(&a.err[sid], 2);
   if (!a.rekey) return;
-  int cx, cy;
-  cell_of(sc, x, y, cx, cy);
-  const uint32_t k = sc.cell_off + uint32_t(cy) * uint32_t(sc.ncx) + uint32_t(cx);
+  const uint32_t k = dem_bucket(sc, x, y);
   a.key[i] = k;
   a.slot[i] = atomicAdd(&a.counts[k], 1u);
 }
@@ -401,9 +404,7 @@ __global__ __launch_bounds__(kThreads) void dem_bin_kernel(BinArgs a) {
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= a.nbodies) return;
   const SceneDev &sc = a.scenes[scene_of(a.b_off, a.nscenes, i)];
-  int cx, cy;
-  cell_of(sc, a.x[i], a.y[i], cx, cy);
-  const uint32_t k = sc.cell_off + uint32_t(cy) * uint32_t(sc.ncx) + uint32_t(cx);
+  const uint32_t k = dem_bucket(sc, a.x[i], a.y[i]);
   a.key[i] = k;
   a.slot[i] = atomicAdd(&a.counts[k], 1u);
 }
@@ -507,6 +508,10 @@ void hg_dem_launch_rebin(hg_dem *h) {
     hipLaunchKernelGGL(dem_bin_kernel, dim3((h->nbodies + kThreads - 1) / kThreads), dim3(kThreads),
                        0, h->stream, a);
   }
+  hg_dem_launch_sort(h);
+}
+
+void hg_dem_launch_sort(hg_dem *h) {
   hg_counting_sort(h->prof, h->stream, h->counts, h->ncell, h->block_sums, h->nscan,
                    h->cell_start, h->key, h->slot, h->nbodies, h->perm);
 }
@@ -598,9 +603,16 @@ static hg_status upload_bodies(hg_dem *h, const hg_dem_desc *descs) {
   return HG_OK;
 }
 
-// Broad-phase grid of one scene: the cell is at least the largest diameter
-// (the 3x3-cell search needs that); the box is the caller's domain or the
-// bodies' box grown to let the scene spread.
+// Broad-phase table of one scene: the cell is at least the largest diameter
+// (the 3x3-cell search needs that); the table spans the caller's domain, or
+// the bodies' box with as much room again on every side to spread, rounded up
+// to powers of two and kept within 8 buckets per body.
+static uint32_t pow2_at_least(double v) {
+  uint32_t w = 4;
+  while (w < 65536u && double(w) < v) w <<= 1;
+  return w;
+}
+
 static void dem_plan_grid(const hg_dem_desc &d, SceneDev &sd) {
   double rmax = 0.0, x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
   for (uint32_t k = 0; k < d.nbodies; ++k) {
@@ -612,35 +624,31 @@ static void dem_plan_grid(const hg_dem_desc &d, SceneDev &sd) {
   }
   if (d.nbodies == 0) x0 = y0 = x1 = y1 = 0.0, rmax = 1.0;
   const double cw = std::max(d.cell_width, 2.0 * rmax * (1.0 + 1e-9));
-  double lo0, lo1, hi0, hi1;
+  double lo0, lo1, ex, ey;
   if (d.domain_max[0] > d.domain_min[0] && d.domain_max[1] > d.domain_min[1]) {
     lo0 = d.domain_min[0];
     lo1 = d.domain_min[1];
-    hi0 = d.domain_max[0];
-    hi1 = d.domain_max[1];
+    ex = (d.domain_max[0] - lo0) / cw + 2.0;
+    ey = (d.domain_max[1] - lo1) / cw + 2.0;
   } else {
-    // room for the scene to spread: the bodies' box grown by twice its
-    // larger side, then clipped by axis-aligned planes the bodies sit inside
-    const double e = 2.0 * std::max(std::max(x1 - x0, y1 - y0), 8.0 * cw);
-    lo0 = x0 - e;
-    lo1 = y0 - e;
-    hi0 = x1 + e;
-    hi1 = y1 + e;
-    for (uint32_t k = 0; k < d.nplanes; ++k) {
-      const double n0 = d.plane_n[2 * k], n1 = d.plane_n[2 * k + 1];
-      const double px = d.plane_x[2 * k], py = d.plane_x[2 * k + 1];
-      if (n1 == 0.0 && n0 > 0.0 && px <= x0) lo0 = std::max(lo0, px - 2.0 * cw);
-      if (n1 == 0.0 && n0 < 0.0 && px >= x1) hi0 = std::min(hi0, px + 2.0 * cw);
-      if (n0 == 0.0 && n1 > 0.0 && py <= y0) lo1 = std::max(lo1, py - 2.0 * cw);
-      if (n0 == 0.0 && n1 < 0.0 && py >= y1) hi1 = std::min(hi1, py + 2.0 * cw);
-    }
+    const double bx = (x1 - x0) / cw + 1.0, by = (y1 - y0) / cw + 1.0;
+    lo0 = x0 - 0.5 * bx * cw;
+    lo1 = y0 - 0.5 * by * cw;
+    ex = 2.0 * bx + 2.0;
+    ey = 2.0 * by + 2.0;
   }
-  // bodies that leave this box clamp to its rim cells (correct, slower)
+  uint32_t wx = pow2_at_least(ex), wy = pow2_at_least(ey);
+  const uint64_t cap = std::max<uint64_t>(8ull * std::max<uint32_t>(d.nbodies, 1u), 4096ull);
+  while (uint64_t(wx) * wy > cap && (wx > 4 || wy > 4)) {
+    if (wx >= wy && wx > 4) wx >>= 1;
+    else wy >>= 1;
+  }
   sd.cmin0 = lo0;
   sd.cmin1 = lo1;
   sd.cw = cw;
-  sd.ncx = int32_t(std::min(std::max(std::ceil((hi0 - lo0) / cw), 1.0) + 1.0, 16384.0));
-  sd.ncy = int32_t(std::min(std::max(std::ceil((hi1 - lo1) / cw), 1.0) + 1.0, 16384.0));
+  sd.icw = 1.0 / cw;
+  sd.ncx = int32_t(wx);
+  sd.ncy = int32_t(wy);
 }
 
 static hg_status dem_create_impl(hg_ctx *ctx, const hg_dem_desc *descs, uint32_t nscenes,

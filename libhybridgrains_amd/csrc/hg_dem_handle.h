@@ -10,14 +10,17 @@ constexpr int kMaxPlanes = 8;
 constexpr int kMaxOwned = 8;
 constexpr int kMaxIn = 8;
 constexpr uint32_t kKinematic = 0x80000000u;
-constexpr int kThreads = 256;
+// One wavefront per workgroup: scenes of tens of thousands of bodies then
+// still spread over every CU.
+constexpr int kThreads = 64;
 
 struct SceneDev {
   double g0, g1;
   double kn, kt, gamman, gammat, mu;
   double dt;
-  double cmin0, cmin1, cw;  // broad-phase grid origin and cell width
-  int32_t ncx, ncy;
+  double cmin0, cmin1, cw;  // broad-phase cell origin and width
+  double icw;               // 1 / cw
+  int32_t ncx, ncy;         // wrapped bucket table: Wx x Wy, powers of two >= 4
   uint32_t b_off, b_cnt, cell_off;
   int32_t nplanes;
   int32_t weighted;
@@ -51,6 +54,25 @@ struct ForceRaw {
   uint32_t *ikey;  // [B][kMaxIn] owners of contacts with this body
   uint8_t *icnt;
 };
+
+// Broad-phase cell of a position (unbounded; clamped to +-2^30 so that any
+// finite or non-finite coordinate gives a valid integer).
+__device__ __forceinline__ void dem_cell_of(const SceneDev &sc, double x, double y, int &cx,
+                                            int &cy) {
+  const double lim = 1073741824.0;
+  const double fx = fmin(fmax(floor((x - sc.cmin0) * sc.icw), -lim), lim);
+  const double fy = fmin(fmax(floor((y - sc.cmin1) * sc.icw), -lim), lim);
+  cx = int(fx);
+  cy = int(fy);
+}
+
+// The cell's bucket in the scene's wrapped table (a key of the batch).
+__device__ __forceinline__ uint32_t dem_bucket(const SceneDev &sc, double x, double y) {
+  int cx, cy;
+  dem_cell_of(sc, x, y, cx, cy);
+  return sc.cell_off + (uint32_t(cx) & uint32_t(sc.ncx - 1)) +
+         uint32_t(sc.ncx) * (uint32_t(cy) & uint32_t(sc.ncy - 1));
+}
 
 }  // namespace hgdem
 
@@ -90,3 +112,5 @@ struct hg_dem {
 // hg_dem_launch_rebin after editing positions.
 void hg_dem_launch_step(hg_dem *h, bool rekey);
 void hg_dem_launch_rebin(hg_dem *h);
+// The counting sort alone, for a caller that wrote the cell keys and slots.
+void hg_dem_launch_sort(hg_dem *h);

@@ -10,26 +10,42 @@
 // increment) live on the MPM node storage; every node a coupled grain touches
 // is appended once to a list (a stamp per node), and the list's nodes are
 // cleared again at the end of the step, so nothing is swept over the whole
-// grid and the step replays correctly from a hipGraph.
+// grid and the step replays correctly from a hipGraph.  The list length
+// alternates between two counters: each step's last kernel clears the one
+// the next step uses.
+//
+// Kernels of one step: P2G with the grid update (MPM), grain_mass (also saves
+// the pre-step grain positions), the DEM contacts and integration,
+// grain_mom, exchange (also refreshes the node velocity and acceleration P2G
+// wrote at the nodes it changes), transfer, G2P with the MPM sort, finish
+// (coupled grain positions, the grains' broad-phase keys, list clean-up),
+// and the DEM sort.
 #include "hg_hybrid_handle.h"
 
 #include <memory>
 
 namespace hghyb {
 
-constexpr int kThreads = 256;
+// Grain kernels run one wavefront per workgroup: the hybrid zone holds tens
+// of thousands of grains, and small workgroups spread them over every CU.
+constexpr int kThreads = 64;
 
 struct HybArgs {
   const hgmpm::SceneDev *msc;  // MPM scene 0
   const uint8_t *tile_active;
   double *gm, *gpn;
+  const double *gp;
+  void *gva;
   int32_t tw, th;
   double hl;  // grains use the first point's half width (Coupling.cpp:31)
   hgdem::BodiesRaw b;
   uint32_t nbodies;
+  const hgdem::SceneDev *dsc;  // DEM scene 0 (broad-phase keys)
+  uint32_t *dkey, *dslot, *dcounts;
   double *prex, *prey;
   uint8_t *coupled;
   GrainGrid g;
+  uint32_t *count, *next_count;  // this step's list length; the next step's
   int32_t *err;
 };
 
@@ -74,26 +90,21 @@ __device__ __forceinline__ bool node_active(const HybArgs &a, int ix, int iy) {
   return a.tile_active[tile] != 0;
 }
 
-__global__ __launch_bounds__(kThreads) void hyb_copy_pre(HybArgs a) {
-  const uint32_t b = blockIdx.x * kThreads + threadIdx.x;
-  if (b == 0) *a.g.count = 0u;
-  if (b >= a.nbodies) return;
-  a.prex[b] = a.b.x[b];
-  a.prey[b] = a.b.y[b];
-}
-
 // HybridCoupling2D::rasterizeGrainMass (Coupling.cpp:22-129): a grain is
 // coupled when every on-grid node of its stencil (at its pre-step position)
 // carries more than 1e-6 of continuum mass; its mass is then spread with the
-// MPM weights.
+// MPM weights.  Every grain's pre-step position is saved for the position
+// update at the end of the step.
 template <typename T, int BASIS>
 __global__ __launch_bounds__(kThreads) void hyb_grain_mass(HybArgs a) {
   const uint32_t b = blockIdx.x * kThreads + threadIdx.x;
   if (b >= a.nbodies) return;
+  const double x = a.b.x[b], y = a.b.y[b];
+  a.prex[b] = x;
+  a.prey[b] = y;
   a.coupled[b] = 0;
   if (a.b.fixed[b]) return;
   const hgmpm::SceneDev &sc = *a.msc;
-  const double x = a.prex[b], y = a.prey[b];
   int lx, hx, ly, hy;
   if (!grain_stencil<BASIS>(sc, x, y, a.hl, lx, hx, ly, hy)) return;
   bool all_pos = true;
@@ -113,10 +124,22 @@ __global__ __launch_bounds__(kThreads) void hyb_grain_mass(HybArgs a) {
       const uint32_t n = uint32_t(iy) * uint32_t(sc.snx) + uint32_t(ix);
       const double w = grain_weight<BASIS>(sc, x, y, a.hl, ix, iy, lx, hx, ly, hy);
       atomicAdd(&a.g.dmass[n], w * mb);
-      if (atomicExch(&a.g.stamp[n], 1u) == 0u) {
-        const uint32_t k = atomicAdd(a.g.count, 1u);
-        if (k < a.g.max_list) a.g.list[k] = n;
-        else atomicOr(a.err, 8);
+      // the first grain to stamp a node appends it to the list; the lanes of
+      // a wave that do so take consecutive entries with one counter update
+      bool fresh = false;
+      if (a.g.stamp[n] == 0u) fresh = atomicExch(&a.g.stamp[n], 1u) == 0u;
+      const uint64_t mask = __ballot(fresh);
+      if (mask) {
+        const int lane = int(threadIdx.x & 63u);
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(a.count, uint32_t(__popcll(mask)));
+        base = __shfl(base, leader, 64);
+        if (fresh) {
+          const uint32_t k = base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+          if (k < a.g.max_list) a.g.list[k] = n;
+          else atomicOr(a.err, 8);
+        }
       }
     }
 }
@@ -145,25 +168,43 @@ __global__ __launch_bounds__(kThreads) void hyb_grain_mom(HybArgs a) {
 
 // HybridCoupling2D::updateVelocitiesOnGrid (Coupling.cpp:533-579): the
 // multiplier that equalises the two velocities at a shared node while
-// conserving its momentum.
+// conserving its momentum.  The node's velocity and acceleration are then
+// recomputed from its new momentum as the grid update computes them
+// (PhysicsGrid::lumpedMassVelocityAndAccelerationUpdate,
+// PhysicsGrid.cpp:588-609); nodes the exchange leaves alone keep the values
+// P2G wrote.
+template <typename T>
 __global__ __launch_bounds__(kThreads) void hyb_exchange(HybArgs a) {
-  const uint32_t k = blockIdx.x * kThreads + threadIdx.x;
-  const uint32_t cnt = min(*a.g.count, a.g.max_list);
-  if (k >= cnt) return;
-  const uint32_t n = a.g.list[k];
-  const double m_dem = a.g.dmass[n];
-  if (!(m_dem > 0.0)) return;
-  const uint32_t node = a.msc->node_off + n;
-  const double m_mpm = a.gm[node];
-  const double msum = m_mpm + m_dem;
-  const double l0 = (m_dem * a.gpn[2 * node] - m_mpm * a.g.dmom[2 * n]) / msum;
-  const double l1 = (m_dem * a.gpn[2 * node + 1] - m_mpm * a.g.dmom[2 * n + 1]) / msum;
-  a.gpn[2 * node] -= l0;
-  a.gpn[2 * node + 1] -= l1;
-  a.g.vafter[2 * n] = (a.g.dmom[2 * n] + l0) / m_dem;
-  a.g.vafter[2 * n + 1] = (a.g.dmom[2 * n + 1] + l1) / m_dem;
-  a.g.aafter[2 * n] = l0 / m_dem;
-  a.g.aafter[2 * n + 1] = l1 / m_dem;
+  using T4 = typename hgmpm::Vec4T<T>::type;
+  const uint32_t cnt = min(*a.count, a.g.max_list);
+  for (uint32_t k = blockIdx.x * kThreads + threadIdx.x; k < cnt; k += gridDim.x * kThreads) {
+    const uint32_t n = a.g.list[k];
+    const double m_dem = a.g.dmass[n];
+    if (!(m_dem > 0.0)) continue;
+    const uint32_t node = a.msc->node_off + n;
+    const double m_mpm = a.gm[node];
+    const double msum = m_mpm + m_dem;
+    const double l0 = (m_dem * a.gpn[2 * node] - m_mpm * a.g.dmom[2 * n]) / msum;
+    const double l1 = (m_dem * a.gpn[2 * node + 1] - m_mpm * a.g.dmom[2 * n + 1]) / msum;
+    const double pn0 = a.gpn[2 * node] - l0, pn1 = a.gpn[2 * node + 1] - l1;
+    a.gpn[2 * node] = pn0;
+    a.gpn[2 * node + 1] = pn1;
+    a.g.vafter[2 * n] = (a.g.dmom[2 * n] + l0) / m_dem;
+    a.g.vafter[2 * n + 1] = (a.g.dmom[2 * n + 1] + l1) / m_dem;
+    a.g.aafter[2 * n] = l0 / m_dem;
+    a.g.aafter[2 * n + 1] = l1 / m_dem;
+    T4 va;
+    if (m_mpm > 0.0) {
+      const double dm = a.msc->dt * m_mpm;
+      va.x = T(pn0 / m_mpm);
+      va.y = T(pn1 / m_mpm);
+      va.z = T((pn0 - a.gp[2 * node]) / dm);
+      va.w = T((pn1 - a.gp[2 * node + 1]) / dm);
+    } else {
+      va.x = va.y = va.z = va.w = T(0);
+    }
+    static_cast<T4 *>(a.gva)[node] = va;
+  }
 }
 
 // HybridCoupling2D::transferToGrain (Coupling.cpp:468-527), FLIP with alpha 1.
@@ -191,28 +232,39 @@ __global__ __launch_bounds__(kThreads) void hyb_transfer(HybArgs a) {
   static_cast<T *>(a.b.vy)[b] = T(vp1 * (1.0 - 1.0) + vf1 * 1.0);
 }
 
-// HybridCoupling2D::updatePositions_node_node (Coupling.cpp:581-591)
+// End of the step: HybridCoupling2D::updatePositions_node_node
+// (Coupling.cpp:581-591) for the coupled grains, every grain's broad-phase
+// key and slot for the DEM sort, and the clean-up of the touched nodes and
+// of the next step's list counter.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void hyb_positions(HybArgs a, double dt) {
-  const uint32_t b = blockIdx.x * kThreads + threadIdx.x;
-  if (b >= a.nbodies || !a.coupled[b]) return;
-  a.b.x[b] = a.prex[b] + dt * double(ld<T>(a.b.vx, b));
-  a.b.y[b] = a.prey[b] + dt * double(ld<T>(a.b.vy, b));
-}
-
-__global__ __launch_bounds__(kThreads) void hyb_clear(HybArgs a) {
-  const uint32_t k = blockIdx.x * kThreads + threadIdx.x;
-  const uint32_t cnt = min(*a.g.count, a.g.max_list);
-  if (k >= cnt) return;
-  const uint32_t n = a.g.list[k];
-  a.g.dmass[n] = 0.0;
-  a.g.dmom[2 * n] = 0.0;
-  a.g.dmom[2 * n + 1] = 0.0;
-  a.g.vafter[2 * n] = 0.0;
-  a.g.vafter[2 * n + 1] = 0.0;
-  a.g.aafter[2 * n] = 0.0;
-  a.g.aafter[2 * n + 1] = 0.0;
-  a.g.stamp[n] = 0u;
+__global__ __launch_bounds__(kThreads) void hyb_finish(HybArgs a, double dt) {
+  const uint32_t tid = blockIdx.x * kThreads + threadIdx.x;
+  if (tid == 0) *a.next_count = 0u;
+  if (tid < a.nbodies) {
+    const uint32_t b = tid;
+    double x = a.b.x[b], y = a.b.y[b];
+    if (a.coupled[b]) {
+      x = a.prex[b] + dt * double(ld<T>(a.b.vx, b));
+      y = a.prey[b] + dt * double(ld<T>(a.b.vy, b));
+      a.b.x[b] = x;
+      a.b.y[b] = y;
+    }
+    const uint32_t k = hgdem::dem_bucket(*a.dsc, x, y);
+    a.dkey[b] = k;
+    a.dslot[b] = atomicAdd(&a.dcounts[k], 1u);
+  }
+  const uint32_t cnt = min(*a.count, a.g.max_list);
+  for (uint32_t k = tid; k < cnt; k += gridDim.x * kThreads) {
+    const uint32_t n = a.g.list[k];
+    a.g.dmass[n] = 0.0;
+    a.g.dmom[2 * n] = 0.0;
+    a.g.dmom[2 * n + 1] = 0.0;
+    a.g.vafter[2 * n] = 0.0;
+    a.g.vafter[2 * n + 1] = 0.0;
+    a.g.aafter[2 * n] = 0.0;
+    a.g.aafter[2 * n + 1] = 0.0;
+    a.g.stamp[n] = 0u;
+  }
 }
 
 }  // namespace hghyb
@@ -245,15 +297,23 @@ static HybArgs make_args(hg_hybrid *h) {
   a.tile_active = h->mpm->tile_active;
   a.gm = h->mpm->gm;
   a.gpn = h->mpm->gpn;
+  a.gp = h->mpm->gp;
+  a.gva = h->mpm->gva;
   a.tw = h->mpm->tw();
   a.th = h->mpm->th();
   a.hl = h->mpm->hl_first;
   a.b = h->dem->b;
   a.nbodies = h->dem->nbodies;
+  a.dsc = h->dem->d_scenes;
+  a.dkey = h->dem->key;
+  a.dslot = h->dem->slot;
+  a.dcounts = h->dem->counts;
   a.prex = h->prex;
   a.prey = h->prey;
   a.coupled = h->coupled;
   a.g = h->g;
+  a.count = h->g.count + h->par;
+  a.next_count = h->g.count + (1 - h->par);
   a.err = h->err;
   return a;
 }
@@ -262,14 +322,9 @@ template <typename T, int BASIS>
 static void launch_hybrid_step(hg_hybrid *h, double dt) {
   HybArgs a = make_args(h);
   const dim3 gb((std::max(a.nbodies, 1u) + kThreads - 1) / kThreads), tb(kThreads);
-  const dim3 gl((h->g.max_list + kThreads - 1) / kThreads);
   KernelProfiler &prof = h->mpm->prof;
   hipStream_t s = h->stream;
-  {
-    ProfScope ps(prof, "hyb_copy_pre", s);
-    hipLaunchKernelGGL(hyb_copy_pre, gb, tb, 0, s, a);
-  }
-  hg_mpm_launch_phase0(h->mpm, dt);
+  hg_mpm_launch_coupled_p2g(h->mpm, dt);
   {
     ProfScope ps(prof, "hyb_grain_mass", s);
     hipLaunchKernelGGL((hyb_grain_mass<T, BASIS>), gb, tb, 0, s, a);
@@ -280,23 +335,22 @@ static void launch_hybrid_step(hg_hybrid *h, double dt) {
     hipLaunchKernelGGL((hyb_grain_mom<T, BASIS>), gb, tb, 0, s, a);
   }
   {
+    // the list holds at most S*S nodes per grain: a grid over the grains
+    // walks it in strides
     ProfScope ps(prof, "hyb_exchange", s);
-    hipLaunchKernelGGL(hyb_exchange, gl, tb, 0, s, a);
+    hipLaunchKernelGGL(hyb_exchange<T>, gb, tb, 0, s, a);
   }
   {
     ProfScope ps(prof, "hyb_transfer", s);
     hipLaunchKernelGGL((hyb_transfer<T, BASIS>), gb, tb, 0, s, a);
   }
-  hg_mpm_launch_phase2(h->mpm);
+  hg_mpm_launch_g2p(h->mpm);
   {
-    ProfScope ps(prof, "hyb_positions", s);
-    hipLaunchKernelGGL(hyb_positions<T>, gb, tb, 0, s, a, dt);
+    ProfScope ps(prof, "hyb_finish", s);
+    hipLaunchKernelGGL(hyb_finish<T>, gb, tb, 0, s, a, dt);
   }
-  hg_dem_launch_rebin(h->dem);
-  {
-    ProfScope ps(prof, "hyb_clear", s);
-    hipLaunchKernelGGL(hyb_clear, gl, tb, 0, s, a);
-  }
+  hg_dem_launch_sort(h->dem);
+  h->par = 1 - h->par;
 }
 
 static void hybrid_step(hg_hybrid *h, double dt) {
@@ -363,7 +417,7 @@ extern "C" hg_status hg_hybrid_create(hg_ctx *ctx, const hg_mpm_desc *mpm_desc,
   if ((e = halloc(hp, &h->g.aafter, size_t(G) * 16)) != HG_OK) return fail(e);
   if ((e = halloc(hp, &h->g.stamp, size_t(G) * 4)) != HG_OK) return fail(e);
   if ((e = halloc(hp, &h->g.list, size_t(h->g.max_list) * 4)) != HG_OK) return fail(e);
-  if ((e = halloc(hp, &h->g.count, 4)) != HG_OK) return fail(e);
+  if ((e = halloc(hp, &h->g.count, 8)) != HG_OK) return fail(e);
   if ((e = halloc(hp, &h->prex, size_t(B) * 8)) != HG_OK) return fail(e);
   if ((e = halloc(hp, &h->prey, size_t(B) * 8)) != HG_OK) return fail(e);
   if ((e = halloc(hp, &h->coupled, size_t(B))) != HG_OK) return fail(e);
@@ -373,7 +427,7 @@ extern "C" hg_status hg_hybrid_create(hg_ctx *ctx, const hg_mpm_desc *mpm_desc,
   hipMemset(h->g.vafter, 0, size_t(G) * 16);
   hipMemset(h->g.aafter, 0, size_t(G) * 16);
   hipMemset(h->g.stamp, 0, size_t(G) * 4);
-  hipMemset(h->g.count, 0, 4);
+  hipMemset(h->g.count, 0, 8);
   hipMemset(h->coupled, 0, B);
   hipMemset(h->err, 0, 4);
   if (hipDeviceSynchronize() != hipSuccess) return fail(hg_fail(ctx, HG_ERR_HIP, "hybrid init"));
@@ -413,14 +467,20 @@ extern "C" hg_status hg_hybrid_step_graph(hg_hybrid *h, double dt, uint32_t nste
     hipGraphExecDestroy(h->graph);
     h->graph = nullptr;
   }
-  // Both solvers alternate their buffers every step and always step
-  // together here, so a graph of two steps captured with the MPM points in
-  // buffer 0 replays from any later state with the MPM points in buffer 0.
-  if (h->mpm->cur != 0 && nsteps) {
+  // The solvers and the list counters alternate every step, so a graph of two
+  // steps captured with the MPM points in buffer 0 and the list counter 0
+  // replays from any later state with both back at 0.  (Stepping the MPM
+  // handle alone in between breaks that alignment; the steps then run
+  // without the graph.)
+  if ((h->mpm->cur != 0 || h->par != 0) && nsteps) {
     hybrid_step(h, dt);
     --nsteps;
   }
-  const uint32_t pairs = nsteps / 2;
+  const bool aligned = h->mpm->cur == 0 && h->par == 0;
+  const uint32_t pairs = aligned ? nsteps / 2 : 0;
+  if (!aligned) {
+    for (; nsteps; --nsteps) hybrid_step(h, dt);
+  }
   if (pairs) {
     if (!h->graph) {
       // the MPM dt upload synchronises; do it before capturing

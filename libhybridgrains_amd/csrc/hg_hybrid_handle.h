@@ -14,7 +14,7 @@ struct GrainGrid {
   double *aafter;  // [2G] velocity increment lambda / m_dem
   uint32_t *stamp; // [G] 1 while the node is on the list
   uint32_t *list;  // touched nodes
-  uint32_t *count;
+  uint32_t *count; // [2]: list length of the steps of each parity
   uint32_t max_list;
 };
 
@@ -29,6 +29,7 @@ struct hg_hybrid {
   double *prex = nullptr, *prey = nullptr;
   uint8_t *coupled = nullptr;
   int32_t *err = nullptr;
+  int par = 0;  // parity of the next step (which list counter it uses)
   hipGraphExec_t graph = nullptr;
   double graph_dt = -1.0;
   int graph_aw = 0, graph_ah = 0;  // MPM tile shape the graph was captured with

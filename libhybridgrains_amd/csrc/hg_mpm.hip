@@ -83,12 +83,13 @@ __device__ __forceinline__ bool point_key(const SceneDev &sc, double x, double y
 //     the order its atomics handed out slots).  The apron's points are
 //     concatenated row by row and staged in chunks of at most CAP points
 //     that end on cell boundaries.
-//  2. gather (node-parallel): for each of the node's S cell rows, ascending,
-//     the node walks the contiguous run of points of its S cells in that row
-//     -- cells ascending, points in id order -- and sums mass, momentum and
-//     force in FP64 registers.  With whole-row chunks the sum order does not
-//     depend on the chunking, so each node's value is a fixed function of the
-//     scene state (bitwise reproducible, identical alone or in a batch).
+//  2. gather (node-parallel): for each of the node's S x S cells -- rows
+//     ascending, cells ascending -- the node walks the cell's points in id
+//     order and sums mass, momentum and force (FP64 mode: into FP64 sums;
+//     FP32 mode: a FP32 sum per cell, added to FP64 node sums).  The sum
+//     order does not depend on the chunking or the tile shape, so each
+//     node's value is a fixed function of the scene state (bitwise
+//     reproducible, identical alone or in a batch, for any tile shape).
 //  3. grid update of the node (forces, momentum, plane conditions, velocity
 //     and acceleration), written as one (v, a) vector per node.
 //
@@ -207,7 +208,6 @@ __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
   // the same for y, and the mass, momentum and stress terms
   __shared__ T s_wx[S][CAP], s_gx[S][CAP], s_wy[S][CAP], s_gy[S][CAP];
   __shared__ T s_m[CAP], s_mv0[CAP], s_mv1[CAP], s_f00[CAP], s_f01[CAP], s_f11[CAP];
-  __shared__ uint8_t s_col[CAP];               // apron column of the point's cell
   __shared__ uint32_t s_id[CAP];               // point ids, in sort order
   __shared__ uint32_t s_perm[kP2GPermCap];     // the apron's point indices, when they fit
   __shared__ uint32_t s_cs[AH][AW + 1];        // cell starts of the apron rows
@@ -251,14 +251,6 @@ __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
     const uint32_t total = s_ro[AH];
 
     const int nx = x0 + lx, ny = y0 + ly;
-    // the node's run of points in each of its S cell rows (apron rows ly..ly+S-1)
-    uint32_t run0[S], run1[S];
-#pragma unroll
-    for (int j = 0; j < S; ++j) {
-      const int r = ly + j;
-      run0[j] = s_ro[r] + (s_cs[r][lx] - s_cs[r][0]);
-      run1[j] = s_ro[r] + (s_cs[r][lx + S] - s_cs[r][0]);
-    }
     double am = 0.0, ap0 = 0.0, ap1 = 0.0, af0 = 0.0, af1 = 0.0;
 
     // position k of the concatenated list -> apron row r and column col
@@ -344,7 +336,6 @@ __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
           for (uint32_t v = cb; v < ce; ++v) rank += (s_id[v] < raw[m].id) ? 1u : 0u;
           const uint32_t d = cb + rank;
           const P2GRec<T> rec = p2g_record<T, BASIS>(raw[m], sc);
-          s_col[d] = uint8_t(col);
           const int bx = ax0 + col, by = ay0 + r;
           const T inv_h = T(sc.inv_h);
           const int cntx = int(rec.cnt & 15u), cnty = int(rec.cnt >> 4);
@@ -384,53 +375,65 @@ __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
       const uint32_t nk0 = k1, nk1 = nk0 < total ? chunk_end(nk0) : total;
       if (nk0 < total) load_chunk(nk0, nk1, raw, rrow, rcol);
 
-      // 2. gather: rows ascending, then the run (cells ascending, ids ascending).
-      //    FP64 mode sums every term into the FP64 node sums; FP32 mode sums a
-      //    chunk's terms in FP32 and adds that partial to the FP64 node sums.
+      // 2. gather: rows ascending, cells ascending, ids ascending.  The cell
+      //    fixes the node's offset in the point's stencil, so every LDS read of
+      //    an iteration is independent of the others.  Force terms count only
+      //    where the weight is nonzero.  FP64 mode sums every term into the
+      //    FP64 node sums; FP32 mode sums each cell's terms in FP32 and adds
+      //    that partial to the FP64 node sums (chunks end on cell boundaries,
+      //    so the partials do not depend on the chunking unless one cell holds
+      //    more than a chunk).
       if (nx < sc.nnx && ny < sc.nny) {
-        using Acc = T;
-        Acc cm = 0, cp0 = 0, cp1 = 0, cf0 = 0, cf1 = 0;
-        if constexpr (std::is_same<T, double>::value) {
-          cm = am;
-          cp0 = ap0;
-          cp1 = ap1;
-          cf0 = af0;
-          cf1 = af1;
-        }
+        constexpr bool f64 = std::is_same<T, double>::value;
 #pragma unroll
         for (int j0 = 0; j0 < S; ++j0) {
           const int j = S - 1 - j0;  // cell row ny - j: weight index j
-          const uint32_t lo = max(run0[j0], k0), hi = min(run1[j0], k1);
-          for (uint32_t k = lo; k < hi; ++k) {
-            const uint32_t q = k - k0;
-            const int i = lx + (S - 1) - int(s_col[q]);  // node's x offset in the point's stencil
-            const T wx = s_wx[i][q], gxs = s_gx[i][q];
-            const T wy = s_wy[j][q], gys = s_gy[j][q];
-            const T w = wx * wy;
-            const T gx = gxs * wy;
-            const T gy = wx * gys;
-            cm += w * s_m[q];
-            cp0 += w * s_mv0[q];
-            cp1 += w * s_mv1[q];
-            if (w != T(0)) {
-              const T f01 = s_f01[q];
-              cf0 += s_f00[q] * gx + f01 * gy;
-              cf1 += f01 * gx + s_f11[q] * gy;
+          const int r = ly + j0;
+          const uint32_t rb = s_ro[r] - s_cs[r][0];  // list position of the row's cell starts
+#pragma unroll
+          for (int i0 = 0; i0 < S; ++i0) {
+            const int i = S - 1 - i0;  // cell column nx - i: weight index i
+            const uint32_t lo = max(rb + s_cs[r][lx + i0], k0);
+            const uint32_t hi = min(rb + s_cs[r][lx + i0 + 1], k1);
+            T cm = 0, cp0 = 0, cp1 = 0, cf0 = 0, cf1 = 0;
+            if constexpr (f64) {
+              cm = am;
+              cp0 = ap0;
+              cp1 = ap1;
+              cf0 = af0;
+              cf1 = af1;
+            }
+            for (uint32_t k = lo; k < hi; ++k) {
+              const uint32_t q = k - k0;
+              const T wx = s_wx[i][q], gxs = s_gx[i][q];
+              const T wy = s_wy[j][q], gys = s_gy[j][q];
+              const T pm = s_m[q], pv0 = s_mv0[q], pv1 = s_mv1[q];
+              const T f00 = s_f00[q], f01 = s_f01[q], f11 = s_f11[q];
+              const T w = wx * wy;
+              const T gx = gxs * wy;
+              const T gy = wx * gys;
+              cm += w * pm;
+              cp0 += w * pv0;
+              cp1 += w * pv1;
+              const T t0 = f00 * gx + f01 * gy;
+              const T t1 = f01 * gx + f11 * gy;
+              cf0 += (w != T(0)) ? t0 : T(0);
+              cf1 += (w != T(0)) ? t1 : T(0);
+            }
+            if constexpr (f64) {
+              am = cm;
+              ap0 = cp0;
+              ap1 = cp1;
+              af0 = cf0;
+              af1 = cf1;
+            } else if (hi > lo) {
+              am += double(cm);
+              ap0 += double(cp0);
+              ap1 += double(cp1);
+              af0 += double(cf0);
+              af1 += double(cf1);
             }
           }
-        }
-        if constexpr (std::is_same<T, double>::value) {
-          am = cm;
-          ap0 = cp0;
-          ap1 = cp1;
-          af0 = cf0;
-          af1 = cf1;
-        } else {
-          am += double(cm);
-          ap0 += double(cp0);
-          ap1 += double(cp1);
-          af0 += double(cf0);
-          af1 += double(cf1);
         }
       }
       __syncthreads();
@@ -1021,18 +1024,14 @@ static uint32_t fused_store(const hg_mpm *h) {
                      : kStoreVelAcc;
 }
 
-hg_status hg_mpm_launch_phase0(hg_mpm *h, double dt) {
+hg_status hg_mpm_launch_coupled_p2g(hg_mpm *h, double dt) {
   HG_TRY(upload_dt(h, dt));
-  launch_p2g(h, kStoreMass | kStoreMom | kStoreMomNew | kStoreForce);
+  launch_p2g(h, kStoreVelAcc | kStoreMass | kStoreMom | kStoreMomNew |
+                    (h->keep_all ? kStoreForce : 0u));
   return HG_OK;
 }
 
-hg_status hg_mpm_launch_phase2(hg_mpm *h) {
-  launch_velacc(h);
-  h->last_store |= kStoreVelAcc;
-  launch_g2p_and_sort(h);
-  return HG_OK;
-}
+void hg_mpm_launch_g2p(hg_mpm *h) { launch_g2p_and_sort(h); }
 
 static void launch_step(hg_mpm *h) {
   launch_p2g(h, fused_store(h));

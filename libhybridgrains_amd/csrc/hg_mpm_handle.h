@@ -47,6 +47,8 @@ struct hg_mpm {
 };
 
 
-// Launch helpers the hybrid step reuses (defined in hg_mpm.hip).
-hg_status hg_mpm_launch_phase0(hg_mpm *h, double dt);
-hg_status hg_mpm_launch_phase2(hg_mpm *h);
+// The two MPM halves of the hybrid step (defined in hg_mpm.hip): P2G with
+// the grid update, storing node mass, both momenta and (v, a); then G2P and
+// the sort.
+hg_status hg_mpm_launch_coupled_p2g(hg_mpm *h, double dt);
+void hg_mpm_launch_g2p(hg_mpm *h);

@@ -48,6 +48,27 @@ def test_pair_set_bit_exact_against_bruteforce(hg_ctx):
     assert len(got) > 1000
 
 
+def test_wrapped_table_keeps_pairs_and_results(hg_ctx):
+    # a table of 4 x 4 buckets over a scene hundreds of cells wide: cells a
+    # table width apart share buckets; the pair set is still the brute-force
+    # one, and 20 steps agree bit for bit with the default table
+    s = _scene(n=1500, seed=7, fixed_every=31)
+    cw = 2.0 * float(s.radius.max()) * 1.01
+    x0, y0 = float(s.q[:, 0].min()), float(s.q[:, 1].min())
+    tiny = GpuDem(s, HG_FP64, ctx=hg_ctx, cell_width=cw, domain=((x0, y0), (x0 + cw, y0 + cw)))
+    wide = GpuDem(s, HG_FP64, ctx=hg_ctx, cell_width=cw)
+    tiny.flow(DT, 1)
+    bf = {tuple(p) for p in oracle.OracleDem(s).pairs_bruteforce().tolist()}
+    c = tiny.contacts()
+    assert {(min(i, j), max(i, j)) for t, i, j in zip(c["type"], c["i"], c["j"]) if t in (1, 2)} == bf
+    wide.flow(DT, 1)
+    tiny.flow(DT, 19)
+    wide.flow(DT, 19)
+    a, b = tiny.state(), wide.state()
+    assert np.array_equal(a["q"], b["q"]) and np.array_equal(a["v"], b["v"])
+    assert _pairset(tiny.contacts()) == _pairset(wide.contacts())
+
+
 def test_one_step_fp64_matches_oracle(hg_ctx):
     s = _scene(seed=2, fixed_every=41)
     g = GpuDem(s, HG_FP64, ctx=hg_ctx)

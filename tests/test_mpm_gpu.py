@@ -144,6 +144,20 @@ def test_tile_shapes_agree(hg_ctx, tile):
     _point_fields_close(gpu.points(), ref.points(), 1e-11)
 
 
+def test_fp32_tile_shapes_agree_bitwise(hg_ctx):
+    # FP32 mode sums a cell's terms in FP32 before the FP64 node sums; the
+    # cells, not the tiles or chunks, set that order
+    s = _small_scene(seed=10)
+    out = []
+    for tile in ((16, 16), (16, 8), (32, 8)):
+        g = GpuMpm(s, HG_FP32, ctx=hg_ctx, tile=tile)
+        g.flow(DT, 10)
+        out.append(g.points())
+    for p in out[1:]:
+        for k in p:
+            assert np.array_equal(p[k], out[0][k]), k
+
+
 def test_fp32_hundred_steps(hg_ctx):
     s = _small_scene(seed=7)
     gpu = GpuMpm(s, HG_FP32, ctx=hg_ctx)
