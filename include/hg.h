@@ -147,12 +147,13 @@ hg_status hg_mpm_download_grid_scene(hg_mpm *h, uint32_t scene, double *mass, do
 hg_status hg_mpm_scene_error(hg_mpm *h, uint32_t scene, int32_t *flags);
 
 /* P2G tile: aw x ah grid nodes per workgroup, one thread per node.
- * Supported: 16x16 (the default), 16x8, 32x8.  Results do not depend on it. */
+ * Supported: 16x16 (the default), 16x8, 32x8, 64x4.  Results do not depend
+ * on it (bitwise, unless one cell holds more points than a staging chunk). */
 hg_status hg_mpm_set_tile(hg_mpm *h, int32_t aw, int32_t ah);
 /* Autotuner (no reference counterpart): times the P2G pass for each tile
  * shape on the handle's own scenes (reps launches each; the point state is
- * not changed), keeps the fastest and returns it; us (3 entries, or NULL)
- * receives the microseconds per launch of 16x16, 16x8 and 32x8. */
+ * not changed), keeps the fastest and returns it; us (4 entries, or NULL)
+ * receives the microseconds per launch of 16x16, 16x8, 32x8 and 64x4. */
 hg_status hg_mpm_autotune(hg_mpm *h, double dt, int32_t reps, int32_t *aw, int32_t *ah,
                           double *us);
 

@@ -198,6 +198,13 @@ __device__ __forceinline__ P2GRec<T> p2g_record(const P2GRaw<T> &w, const SceneD
 // chunk by chunk).
 constexpr int kP2GPermCap = 2048;
 
+// Largest power of two <= n (n >= 1; 0 for n < 1).
+constexpr int top_bit(int n) {
+  int b = 1;
+  while (b * 2 <= n) b *= 2;
+  return n >= 1 ? b : 0;
+}
+
 template <typename T, int BASIS, int TW, int TH, int CAP>
 __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
   constexpr int S = BasisTraits<BASIS>::S;
@@ -254,15 +261,16 @@ __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
     double am = 0.0, ap0 = 0.0, ap1 = 0.0, af0 = 0.0, af1 = 0.0;
 
     // position k of the concatenated list -> apron row r and column col
+    // (binary searches over the row offsets and the row's cell starts)
     auto locate = [&](uint32_t k, int &r, int &col) {
       r = 0;
 #pragma unroll
-      for (int step = 32; step > 0; step >>= 1)
+      for (int step = top_bit(AH - 1); step > 0; step >>= 1)
         if (r + step <= AH - 1 && s_ro[r + step] <= k) r += step;
       const uint32_t pos = s_cs[r][0] + (k - s_ro[r]);
       col = 0;
 #pragma unroll
-      for (int step = 32; step > 0; step >>= 1)
+      for (int step = top_bit(AW - 1); step > 0; step >>= 1)
         if (col + step <= AW - 1 && s_cs[r][col + step] <= pos) col += step;
     };
 
@@ -876,6 +884,7 @@ template <typename T, int BASIS>
 static void launch_p2g_b(hg_mpm *h, uint32_t store) {
   if (h->tw() == 32 && h->th() == 8) launch_p2g_t<T, BASIS, 32, 8>(h, store);
   else if (h->tw() == 16 && h->th() == 8) launch_p2g_t<T, BASIS, 16, 8>(h, store);
+  else if (h->tw() == 64 && h->th() == 4) launch_p2g_t<T, BASIS, 64, 4>(h, store);
   else launch_p2g_t<T, BASIS, 16, 16>(h, store);
 }
 
@@ -1483,8 +1492,9 @@ extern "C" hg_status hg_mpm_kernel_times(hg_mpm *h, int32_t max_kernels, char *n
 
 extern "C" hg_status hg_mpm_set_tile(hg_mpm *h, int32_t aw, int32_t ah) {
   if (!h) return HG_ERR_INVALID;
-  if (!((aw == 16 && ah == 16) || (aw == 16 && ah == 8) || (aw == 32 && ah == 8)))
-    return hg_fail(h->ctx, HG_ERR_UNSUPPORTED, "tile shape must be 16x16, 16x8 or 32x8");
+  if (!((aw == 16 && ah == 16) || (aw == 16 && ah == 8) || (aw == 32 && ah == 8) ||
+        (aw == 64 && ah == 4)))
+    return hg_fail(h->ctx, HG_ERR_UNSUPPORTED, "tile shape must be 16x16, 16x8, 32x8 or 64x4");
   if (h->phase_next != 0) return hg_fail(h->ctx, HG_ERR_INVALID, "a phased step is in progress");
   if (aw == h->aw && ah == h->ah) return HG_OK;
   // Tiles index the grid; rebuild the tile tables for the new shape.
@@ -1591,13 +1601,13 @@ extern "C" hg_status hg_mpm_device_bytes(hg_mpm *h, uint64_t *bytes) {
 // tile shape on the handle's own scenes and keeps the fastest.  The choice
 // changes no result: a node's sum order does not depend on the tile shape.
 extern "C" hg_status hg_mpm_autotune(hg_mpm *h, double dt, int32_t reps, int32_t *aw_out,
-                                     int32_t *ah_out, double *us /* [3] or NULL */) {
+                                     int32_t *ah_out, double *us /* [4] or NULL */) {
   if (!h) return HG_ERR_INVALID;
   if (!(dt > 0.0)) return hg_fail(h->ctx, HG_ERR_INVALID, "dt must be > 0");
   if (h->phase_next != 0) return hg_fail(h->ctx, HG_ERR_INVALID, "a phased step is in progress");
   if (reps < 1) reps = 1;
   HG_TRY(upload_dt(h, dt));
-  const int shapes[3][2] = {{16, 16}, {16, 8}, {32, 8}};
+  const int shapes[4][2] = {{16, 16}, {16, 8}, {32, 8}, {64, 4}};
   const bool was_on = h->prof.on;
   h->prof.on = false;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1606,7 +1616,7 @@ extern "C" hg_status hg_mpm_autotune(hg_mpm *h, double dt, int32_t reps, int32_t
   double best = 1e300;
   int bi = 0;
   hg_status st = HG_OK;
-  for (int s = 0; s < 3 && st == HG_OK; ++s) {
+  for (int s = 0; s < 4 && st == HG_OK; ++s) {
     st = hg_mpm_set_tile(h, shapes[s][0], shapes[s][1]);
     if (st != HG_OK) break;
     launch_p2g(h, kStoreVelAcc);  // warm-up

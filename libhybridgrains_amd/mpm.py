@@ -151,11 +151,11 @@ class GpuMpm:
         """Picks the fastest P2G tile shape for these scenes (results do not
         depend on it); returns the choice and the timings."""
         aw, ah = C.c_int32(), C.c_int32()
-        us = np.zeros(3)
+        us = np.zeros(4)
         self.ctx.check(self.L.hg_mpm_autotune(self._h, float(dt), int(reps), C.byref(aw),
                                               C.byref(ah), ptr(us)))
         return {"tile": (aw.value, ah.value),
-                "us": {"16x16": us[0], "16x8": us[1], "32x8": us[2]}}
+                "us": {"16x16": us[0], "16x8": us[1], "32x8": us[2], "64x4": us[3]}}
 
     def scene_error(self, scene: int) -> int:
         f = C.c_int32()

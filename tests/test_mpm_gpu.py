@@ -134,7 +134,7 @@ def test_batch_invariance_bitwise(hg_ctx):
             assert np.array_equal(pa[f], pb[f]), (k, f)
 
 
-@pytest.mark.parametrize("tile", [(16, 16), (16, 8), (32, 8)])
+@pytest.mark.parametrize("tile", [(16, 16), (16, 8), (32, 8), (64, 4)])
 def test_tile_shapes_agree(hg_ctx, tile):
     s = _small_scene(seed=6)
     ref = oracle.OracleMpm(s)
@@ -149,7 +149,7 @@ def test_fp32_tile_shapes_agree_bitwise(hg_ctx):
     # cells, not the tiles or chunks, set that order
     s = _small_scene(seed=10)
     out = []
-    for tile in ((16, 16), (16, 8), (32, 8)):
+    for tile in ((16, 16), (16, 8), (32, 8), (64, 4)):
         g = GpuMpm(s, HG_FP32, ctx=hg_ctx, tile=tile)
         g.flow(DT, 10)
         out.append(g.points())
@@ -194,7 +194,7 @@ def test_autotune_picks_a_shape_and_changes_no_result(hg_ctx):
     a.flow(DT, 3)
     b.flow(DT, 3)
     tune = b.autotune(DT, reps=2)
-    assert tune["tile"] in ((16, 16), (16, 8), (32, 8))
+    assert tune["tile"] in ((16, 16), (16, 8), (32, 8), (64, 4))
     assert all(v > 0 for v in tune["us"].values())
     a.flow(DT, 5, graph=True)
     b.flow(DT, 5, graph=True)
