@@ -60,9 +60,11 @@ def _time_hybrid(m, d, w, precision, steps, warmup, dt):
     h.step(dt, steps, graph=True)
     h.sync()
     wall = time.perf_counter() - t0
-    # per-kernel durations over the same number of steps (events on the stream)
+    # per-kernel durations over the same number of steps (events on the
+    # stream); the steps queue up behind a hold so that they run back to back
     h.continuum.set_profiling(True)
     h.discrete.set_profiling(True)
+    h.continuum.hold(3e5)
     h.step(dt, steps)
     h.sync()
     kt = dict(h.continuum.kernel_times())
@@ -109,6 +111,7 @@ def _time_dem(args, dt=1e-5):
         g.sync()
         wall = time.perf_counter() - t0
         g.set_profiling(True)
+        g.hold(2e5)
         g.flow(dt, args.steps)
         g.sync()
         kt = g.kernel_times()
@@ -174,8 +177,8 @@ def _copy_gbs(_lib, nbytes=1 << 30, iters=20):
 
 def _pmc(counter, mode, extra=()):
     """Runs this script under rocprofv3 --pmc <counter> in a child process and
-    returns {kernel: mean counter value per launch}; None if the profiler is
-    unavailable or fails."""
+    returns {kernel: (mean counter value per launch, launches)}; None if the
+    profiler is unavailable or fails."""
     prof = shutil.which("rocprofv3")
     if not prof:
         return None
@@ -192,25 +195,26 @@ def _pmc(counter, mode, extra=()):
         with open(path) as f:
             for r in csv.DictReader(f):
                 agg.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
-        return {k: sum(v) / len(v) for k, v in agg.items()}
+        return {k: (sum(v) / len(v), len(v)) for k, v in agg.items()}
     except Exception:
         return None
     finally:
         shutil.rmtree(d, ignore_errors=True)
 
 
-def _traffic():
-    """HBM bytes per P2G + G2P pair of the headline step, from FETCH_SIZE and
-    WRITE_SIZE (KiB per dispatch), each scaled by its reading for the copy
-    kernel that reads and writes 1 GiB."""
+def _traffic(tile):
+    """HBM bytes per P2G + G2P pair of the headline step (P2G tile as chosen
+    by the headline run), from FETCH_SIZE and WRITE_SIZE (KiB per dispatch),
+    each scaled by its reading for the copy kernel that reads and writes 1 GiB."""
+    t = ("--tile", "%dx%d" % tuple(tile))
     copy_f, copy_w = _pmc("FETCH_SIZE", "copy"), _pmc("WRITE_SIZE", "copy")
-    run_f, run_w = _pmc("FETCH_SIZE", "hyb32"), _pmc("WRITE_SIZE", "hyb32")
+    run_f, run_w = _pmc("FETCH_SIZE", "hyb32", t), _pmc("WRITE_SIZE", "hyb32", t)
     if not (copy_f and copy_w and run_f and run_w):
         return None, None
 
     def pick(d, key):
-        vals = [v for k, v in d.items() if key in k]
-        return vals[0] if vals else None
+        vals = sorted((n, v) for k, (v, n) in d.items() if key in k)
+        return vals[-1][1] if vals else None
 
     gib_kib = float(1 << 20)
     cf, cw = pick(copy_f, "hg_copy_kernel"), pick(copy_w, "hg_copy_kernel")
@@ -280,6 +284,8 @@ def main() -> int:
     ap.add_argument("--served-requests", type=int, default=40)
     ap.add_argument("--served-rate", type=float, default=8.0)
     ap.add_argument("--served-full", action="store_true")
+    ap.add_argument("--tile", default="", help="P2G tile WxH for the profiling modes "
+                                               "(default: autotune)")
     ap.add_argument("--mode", default="full",
                     choices=["full", "hyb32", "hyb64", "mpm32", "mpm64", "copy"],
                     help="full: the JSON line; hyb32/hyb64/mpm32/mpm64: only those steps "
@@ -298,13 +304,22 @@ def main() -> int:
         return 0
     if args.mode in ("mpm32", "mpm64", "hyb32", "hyb64"):
         prec = HG_FP32 if args.mode.endswith("32") else HG_FP64
+        # the same set-up as the full run: the P2G tile it chose (--tile) or
+        # the autotuner's choice, hipGraph steps
         if args.mode.startswith("mpm"):
             g = GpuMpm(scenes.mpm_1m(), prec)
+            cont = g
             step = lambda n: g.flow(DT_1M, n, graph=True)  # noqa: E731
         else:
             m, d, w = scenes.hybrid_1m()
             g = GpuHybrid(m, d, w, prec)
+            cont = g.continuum
             step = lambda n: g.step(DT_1M, n, graph=True)  # noqa: E731
+        if args.tile:
+            tw, th = (int(v) for v in args.tile.split("x"))
+            cont.ctx.check(cont.L.hg_mpm_set_tile(cont._h, tw, th))
+        else:
+            cont.autotune(DT_1M)
         step(args.warmup)
         g.sync()
         t0 = time.perf_counter()
@@ -356,7 +371,7 @@ def main() -> int:
     traffic_detail = None
     if not args.no_traffic:
         print("[bench] traffic (rocprofv3)", file=sys.stderr, flush=True)
-        traffic, traffic_detail = _traffic()
+        traffic, traffic_detail = _traffic(f32["autotune"]["tile"])
         f32["roofline"]["traffic"] = traffic
     cpu = None
     if not args.no_cpu:

@@ -164,6 +164,11 @@ hg_status hg_mpm_autotune(hg_mpm *h, double dt, int32_t reps, int32_t *aw, int32
 hg_status hg_mpm_set_profiling(hg_mpm *h, int32_t on);
 hg_status hg_mpm_kernel_times(hg_mpm *h, int32_t max_kernels, char *names /* max_kernels*32 */,
                               double *ms, int64_t *launches, int32_t *nkernels);
+/* Occupies the handle's stream for about us microseconds (a one-wavefront
+ * kernel that waits on the device clock; at most one second).  Steps
+ * submitted right after it queue up behind it, so profiled kernels then run
+ * back to back and their event times hold no host submission gaps. */
+hg_status hg_mpm_hold(hg_mpm *h, double us);
 /* Streams nbytes from one device buffer to another iters times and returns
  * the achieved read+write bandwidth; the reference point for HBM counters and
  * the measured copy roofline of the box. */
@@ -240,6 +245,7 @@ hg_status hg_dem_download_force(hg_dem *h, uint32_t scene, double *F);
 hg_status hg_dem_estimate_bytes(const hg_dem_desc *descs, uint32_t nscenes, uint64_t *bytes);
 hg_status hg_dem_device_bytes(hg_dem *h, uint64_t *bytes);
 hg_status hg_dem_set_profiling(hg_dem *h, int32_t on);
+hg_status hg_dem_hold(hg_dem *h, double us);  /* as hg_mpm_hold */
 hg_status hg_dem_kernel_times(hg_dem *h, int32_t max_kernels, char *names, double *ms,
                               int64_t *launches, int32_t *nkernels);
 

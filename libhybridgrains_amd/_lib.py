@@ -143,6 +143,7 @@ def lib() -> C.CDLL:
         "hg_mpm_download_grid_scene": ([vp, C.c_uint32, _dp, _dp, _dp, _dp, _dp, _dp], C.c_int32),
         "hg_mpm_counts": ([vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)], C.c_int32),
         "hg_mpm_set_profiling": ([vp, C.c_int32], C.c_int32),
+        "hg_mpm_hold": ([vp, C.c_double], C.c_int32),
         "hg_mpm_kernel_times": (
             [vp, C.c_int32, C.c_char_p, _dp, C.POINTER(C.c_int64), _ip],
             C.c_int32,
@@ -163,6 +164,7 @@ def lib() -> C.CDLL:
         "hg_dem_keep_force": ([vp, C.c_int32], C.c_int32),
         "hg_dem_download_force": ([vp, C.c_uint32, _dp], C.c_int32),
         "hg_dem_set_profiling": ([vp, C.c_int32], C.c_int32),
+        "hg_dem_hold": ([vp, C.c_double], C.c_int32),
         "hg_dem_kernel_times": (
             [vp, C.c_int32, C.c_char_p, _dp, C.POINTER(C.c_int64), _ip], C.c_int32),
         "hg_hybrid_create": (

@@ -118,6 +118,27 @@ hg_status hg_profiler_report(KernelProfiler &prof, hg_ctx *ctx, int32_t max_kern
   return HG_OK;
 }
 
+// ---- stream hold (for profiling) ---------------------------------------------
+
+// One wavefront waits until the constant-rate clock has advanced by ticks;
+// the wait ends whatever the clock's start value (unsigned differences).
+__global__ __launch_bounds__(64) void hg_hold_kernel(uint64_t ticks) {
+  const uint64_t t0 = wall_clock64();
+  while (wall_clock64() - t0 < ticks) __builtin_amdgcn_s_sleep(100);
+}
+
+hg_status hg_stream_hold(hg_ctx *ctx, hipStream_t s, double us) {
+  if (!(us >= 0.0)) return hg_fail(ctx, HG_ERR_INVALID, "hold time must be >= 0");
+  us = us > 1.0e6 ? 1.0e6 : us;
+  int khz = 0;
+  HG_HIP(ctx, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
+  if (khz <= 0) return hg_fail(ctx, HG_ERR_HIP, "device clock rate unknown");
+  const uint64_t ticks = uint64_t(us * 1e-3 * double(khz));
+  hipLaunchKernelGGL(hg_hold_kernel, dim3(1), dim3(64), 0, s, ticks);
+  HG_HIP(ctx, hipGetLastError());
+  return HG_OK;
+}
+
 // ---- streaming copy (bandwidth reference) -------------------------------------
 
 __global__ __launch_bounds__(256) void hg_copy_kernel(const float4 *__restrict__ src,

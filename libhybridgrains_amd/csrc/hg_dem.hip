@@ -960,6 +960,11 @@ extern "C" hg_status hg_dem_set_profiling(hg_dem *h, int32_t on) {
   return HG_OK;
 }
 
+extern "C" hg_status hg_dem_hold(hg_dem *h, double us) {
+  if (!h) return HG_ERR_INVALID;
+  return hg_stream_hold(h->ctx, h->stream, us);
+}
+
 extern "C" hg_status hg_dem_kernel_times(hg_dem *h, int32_t max_kernels, char *names, double *ms,
                                          int64_t *launches, int32_t *nkernels) {
   if (!h) return HG_ERR_INVALID;

@@ -58,6 +58,10 @@ struct ProfScope {
   ~ProfScope() { p.end(rec, s); }
 };
 
+// Enqueues a kernel that keeps the stream busy for about us microseconds
+// (at most one second), measured on the device's constant-rate clock.
+hg_status hg_stream_hold(hg_ctx *ctx, hipStream_t s, double us);
+
 hg_status hg_profiler_report(KernelProfiler &prof, hg_ctx *ctx, int32_t max_kernels,
                              char *names, double *ms, int64_t *launches,
                              int32_t *nkernels);

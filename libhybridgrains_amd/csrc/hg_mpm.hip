@@ -1484,6 +1484,11 @@ extern "C" hg_status hg_mpm_set_profiling(hg_mpm *h, int32_t on) {
   return HG_OK;
 }
 
+extern "C" hg_status hg_mpm_hold(hg_mpm *h, double us) {
+  if (!h) return HG_ERR_INVALID;
+  return hg_stream_hold(h->ctx, h->stream, us);
+}
+
 extern "C" hg_status hg_mpm_kernel_times(hg_mpm *h, int32_t max_kernels, char *names, double *ms,
                                          int64_t *launches, int32_t *nkernels) {
   if (!h) return HG_ERR_INVALID;

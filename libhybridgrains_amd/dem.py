@@ -165,6 +165,10 @@ class GpuDem:
     def set_profiling(self, on: bool) -> None:
         self.ctx.check(self.L.hg_dem_set_profiling(self._h, int(bool(on))))
 
+    def hold(self, us: float) -> None:
+        """As GpuMpm.hold."""
+        self.ctx.check(self.L.hg_dem_hold(self._h, float(us)))
+
     def kernel_times(self) -> dict:
         return _kernel_times(self.ctx, self.L.hg_dem_kernel_times, self._h)
 

@@ -190,6 +190,11 @@ class GpuMpm:
     def set_profiling(self, on: bool) -> None:
         self.ctx.check(self.L.hg_mpm_set_profiling(self._h, int(bool(on))))
 
+    def hold(self, us: float) -> None:
+        """Keeps the stream busy for about us microseconds, so that the steps
+        submitted next run back to back (clean per-kernel event times)."""
+        self.ctx.check(self.L.hg_mpm_hold(self._h, float(us)))
+
     def kernel_times(self) -> dict:
         maxk = 32
         names = C.create_string_buffer(32 * maxk)

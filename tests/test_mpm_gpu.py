@@ -187,6 +187,19 @@ def test_point_leaving_grid_is_reported(hg_ctx):
     assert gpu.scene_error(0) & 1
 
 
+def test_hold_occupies_the_stream(hg_ctx):
+    import time
+    g = GpuMpm(_small_scene(seed=12), HG_FP64, ctx=hg_ctx)
+    g.sync()
+    t0 = time.perf_counter()
+    g.hold(30000.0)
+    g.flow(DT, 1)
+    g.sync()
+    assert time.perf_counter() - t0 >= 0.025
+    with pytest.raises(HgError):
+        g.hold(-1.0)
+
+
 def test_autotune_picks_a_shape_and_changes_no_result(hg_ctx):
     s = _small_scene(seed=9)
     a = GpuMpm(s, HG_FP64, ctx=hg_ctx)
