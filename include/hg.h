@@ -282,6 +282,22 @@ hg_status hg_hybrid_dem(hg_hybrid *h, hg_dem **out);
 /* Per grain, 1 if the last step coupled it to the continuum
  * (HybridCoupling2D's dem_particle_coupled). */
 hg_status hg_hybrid_download_coupling(hg_hybrid *h, uint8_t *coupled);
+/* Per-step homogenisation of the grains onto the continuum grid, which
+ * HybridGrains2DSim::updateHybridStates_poorman runs after every step when a
+ * scene sets homogenize_stress / homogenize_velocity with their
+ * grid_smoothing flags (HybridGrains2DSim.cpp:545-556): each step adds
+ * ResamplingTools::homogenizeStress (ResamplingTools.cpp:744-801, the circle
+ * contacts' stress spread from their contact points) and homogenizeVelocity
+ * (:818-874, the grains' nodal velocity) to node accumulators.  Off by
+ * default; switching it on allocates the accumulators (zeroed). */
+hg_status hg_hybrid_set_homogenize(hg_hybrid *h, int32_t stress, int32_t velocity);
+/* The accumulators, nodes in the reference's flat order (PhysicsGrid
+ * flatNodeIndex): stress 4 per node (column-major 2x2), velocity 2 per node;
+ * NULL skips one.  Zeros while homogenisation was never switched on. */
+hg_status hg_hybrid_download_homogenized(hg_hybrid *h, double *stress, double *velocity);
+/* ResamplingTools::clearHomogenizedStress / clearHomogenizedVelocity
+ * (ResamplingTools.cpp:736-742, :810-816). */
+hg_status hg_hybrid_clear_homogenized(hg_hybrid *h);
 hg_status hg_hybrid_estimate_bytes(const hg_mpm_desc *mpm, const hg_dem_desc *dem,
                                    uint64_t *bytes);
 hg_status hg_hybrid_device_bytes(hg_hybrid *h, uint64_t *bytes);

@@ -168,6 +168,15 @@ class HybridGrains2DSim {
   const RigidBody2DState &discreteState();
   const SimulationState &continuumState();
   std::vector<std::uint8_t> coupledGrains();
+  // ResamplingTools' homogenised fields (ResamplingTools.cpp:744-884), which
+  // every step adds to when the scene's coupling_force sets homogenize_* and
+  // grid_smoothing_homogenized_* (HybridGrains2DSim.cpp:545-556): 4 entries
+  // per node (column-major 2x2) and 2 per node, nodes in PhysicsGrid's flat
+  // order.
+  std::vector<double> homogenizedStress();
+  std::vector<double> homogenizedVelocity();
+  // ResamplingTools::clearHomogenizedStress / clearHomogenizedVelocity
+  void clearHomogenized();
   hg_hybrid *handle() const { return h_; }
 
  private:

@@ -176,6 +176,9 @@ def lib() -> C.CDLL:
         "hg_hybrid_mpm": ([vp, C.POINTER(vp)], C.c_int32),
         "hg_hybrid_dem": ([vp, C.POINTER(vp)], C.c_int32),
         "hg_hybrid_download_coupling": ([vp, C.POINTER(C.c_uint8)], C.c_int32),
+        "hg_hybrid_set_homogenize": ([vp, C.c_int32, C.c_int32], C.c_int32),
+        "hg_hybrid_download_homogenized": ([vp, _dp, _dp], C.c_int32),
+        "hg_hybrid_clear_homogenized": ([vp], C.c_int32),
         "hg_mpm_estimate_bytes": ([C.POINTER(MpmDesc), C.c_uint32, C.POINTER(C.c_uint64)], C.c_int32),
         "hg_mpm_device_bytes": ([vp, C.POINTER(C.c_uint64)], C.c_int32),
         "hg_dem_estimate_bytes": ([C.POINTER(DemDesc), C.c_uint32, C.POINTER(C.c_uint64)], C.c_int32),

@@ -135,6 +135,7 @@ struct ContactArgs {
   ForceRaw f;
   const uint32_t *perm, *cell_start;
   int32_t *err;
+  double *cpoint;  // optional: contact point per owned slot (circle pairs)
 };
 
 template <typename T>
@@ -242,6 +243,10 @@ __global__ __launch_bounds__(kThreads) void dem_contacts_kernel(ContactArgs a) {
       vr0 = vxi + omi * (-r01) - vxj - omj * (-r11);
       vr1 = vyi + omi * r00 - vyj - omj * r10;
       w = sc.weighted ? (wi + double(ld<T>(a.b.w, j))) * 0.5 : 1.0;
+      if (a.cpoint) {
+        a.cpoint[2 * (obase + uint32_t(k))] = p0;
+        a.cpoint[2 * (obase + uint32_t(k)) + 1] = p1;
+      }
     } else {
       // KinematicCircleCircleConstraint (KinematicObjectCircleConstraint.cpp:121-199)
       r00 = -ri * n0;
@@ -451,6 +456,7 @@ static void launch_contacts_t(hg_dem *h) {
   a.perm = h->perm;
   a.cell_start = h->cell_start;
   a.err = h->err;
+  a.cpoint = h->cpoint;
   ProfScope ps(h->prof, "dem_contacts", h->stream);
   hipLaunchKernelGGL(dem_contacts_kernel<T>, dim3(a.nblocks_padded), dim3(kThreads), 0, h->stream, a);
 }

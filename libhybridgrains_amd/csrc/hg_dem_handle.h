@@ -97,6 +97,7 @@ struct hg_dem {
            *cell_start = nullptr, *block_sums = nullptr;
   int32_t *err = nullptr;
   double *F_out = nullptr;
+  double *cpoint = nullptr;  // contact points of the owned circle pairs (kept on request)
   bool keep_force = false;
   double dt_dev = -1.0;
   KernelProfiler prof;

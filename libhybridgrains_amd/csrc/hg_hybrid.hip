@@ -267,6 +267,108 @@ __global__ __launch_bounds__(kThreads) void hyb_finish(HybArgs a, double dt) {
   }
 }
 
+// ---- per-step homogenisation (ResamplingTools.cpp:744-884) -------------------
+
+struct HomogArgs {
+  HomogGrid hg;
+  hgdem::SlotsRaw s;   // the contacts this step wrote
+  const void *oF;      // their forces on the owner
+  const double *cpoint;
+  uint32_t *count, *next_count;
+  int32_t stress, vel;
+};
+
+// Spreads one quantity set from position (x, y) with the MPM weights of half
+// width hl = h/4, as ResamplingTools does for both fields.
+template <int BASIS>
+__device__ __forceinline__ bool homog_stencil(const hgmpm::SceneDev &sc, double x, double y,
+                                              int &lx, int &hx, int &ly, int &hy) {
+  return grain_stencil<BASIS>(sc, x, y, 0.25 * sc.h, lx, hx, ly, hy);
+}
+
+// Per grain: ResamplingTools::homogenizeStress (ResamplingTools.cpp:744-801)
+// for the circle contacts it owns -- stress -(f r^T + r f^T)/2 with r at the
+// end-of-step positions, spread from the contact point and divided by the
+// cell area -- and the mass and momentum half of homogenizeVelocity
+// (:818-866) for the grain itself.
+template <typename T, int BASIS>
+__global__ __launch_bounds__(kThreads) void hyb_homog_grains(HybArgs a, HomogArgs g) {
+  const uint32_t b = blockIdx.x * kThreads + threadIdx.x;
+  if (b >= a.nbodies) return;
+  const hgmpm::SceneDev &sc = *a.msc;
+  const double hl = 0.25 * sc.h;
+  const double x = a.b.x[b], y = a.b.y[b];
+  if (g.stress) {
+    const double inv_area = 1.0 / (sc.h * sc.h);
+    const int n = g.s.ocnt[b];
+    for (int k = 0; k < n; ++k) {
+      const uint32_t sl = b * uint32_t(hgdem::kMaxOwned) + uint32_t(k);
+      const uint32_t key = g.s.okey[sl];
+      if (key & hgdem::kKinematic) continue;
+      const uint32_t j = key;  // one DEM scene: bodies are numbered from 0
+      const double f0 = double(ld<T>(g.oF, 2 * sl)), f1 = double(ld<T>(g.oF, 2 * sl + 1));
+      const double r0 = x - a.b.x[j], r1 = y - a.b.y[j];
+      const double sxx = -0.5 * (f0 * r0 + r0 * f0);
+      const double sxy = -0.5 * (f1 * r0 + r1 * f0);
+      const double syy = -0.5 * (f1 * r1 + r1 * f1);
+      const double px = g.cpoint[2 * sl], py = g.cpoint[2 * sl + 1];
+      int lx, hx, ly, hy;
+      if (!homog_stencil<BASIS>(sc, px, py, lx, hx, ly, hy)) continue;
+      for (int iy = ly; iy <= hy; ++iy)
+        for (int ix = lx; ix <= hx; ++ix) {
+          if (ix < 0 || iy < 0 || ix >= sc.nnx || iy >= sc.nny) continue;
+          const uint32_t node = uint32_t(iy) * uint32_t(sc.snx) + uint32_t(ix);
+          const double w = grain_weight<BASIS>(sc, px, py, hl, ix, iy, lx, hx, ly, hy);
+          atomicAdd(&g.hg.stress[3 * node], w * sxx * inv_area);
+          atomicAdd(&g.hg.stress[3 * node + 1], w * sxy * inv_area);
+          atomicAdd(&g.hg.stress[3 * node + 2], w * syy * inv_area);
+        }
+    }
+  }
+  if (g.vel && !a.b.fixed[b]) {
+    const double m = double(ld<T>(a.b.m, b));
+    const double v0 = double(ld<T>(a.b.vx, b)), v1 = double(ld<T>(a.b.vy, b));
+    int lx, hx, ly, hy;
+    if (homog_stencil<BASIS>(sc, x, y, lx, hx, ly, hy)) {
+      for (int iy = ly; iy <= hy; ++iy)
+        for (int ix = lx; ix <= hx; ++ix) {
+          if (ix < 0 || iy < 0 || ix >= sc.nnx || iy >= sc.nny) continue;
+          const uint32_t node = uint32_t(iy) * uint32_t(sc.snx) + uint32_t(ix);
+          const double w = grain_weight<BASIS>(sc, x, y, hl, ix, iy, lx, hx, ly, hy);
+          atomicAdd(&g.hg.mom[2 * node], w * m * v0);
+          atomicAdd(&g.hg.mom[2 * node + 1], w * m * v1);
+          atomicAdd(&g.hg.mass[node], w * m);
+          if (g.hg.stamp[node] == 0u && atomicExch(&g.hg.stamp[node], 1u) == 0u) {
+            const uint32_t e = atomicAdd(g.count, 1u);
+            if (e < g.hg.max_list) g.hg.list[e] = node;
+            else atomicOr(a.err, 8);
+          }
+        }
+    }
+  }
+}
+
+// The rest of homogenizeVelocity (ResamplingTools.cpp:868-873): each reached
+// node with positive grain mass adds its velocity; then the step's mass and
+// momentum are cleared, and the next step's list counter.
+__global__ __launch_bounds__(kThreads) void hyb_homog_nodes(HomogArgs g) {
+  const uint32_t tid = blockIdx.x * kThreads + threadIdx.x;
+  if (tid == 0) *g.next_count = 0u;
+  const uint32_t cnt = min(*g.count, g.hg.max_list);
+  for (uint32_t k = tid; k < cnt; k += gridDim.x * kThreads) {
+    const uint32_t n = g.hg.list[k];
+    const double m = g.hg.mass[n];
+    if (m > 0.0) {
+      g.hg.vel[2 * n] += g.hg.mom[2 * n] / m;
+      g.hg.vel[2 * n + 1] += g.hg.mom[2 * n + 1] / m;
+    }
+    g.hg.mass[n] = 0.0;
+    g.hg.mom[2 * n] = 0.0;
+    g.hg.mom[2 * n + 1] = 0.0;
+    g.hg.stamp[n] = 0u;
+  }
+}
+
 }  // namespace hghyb
 
 using namespace hghyb;
@@ -350,6 +452,25 @@ static void launch_hybrid_step(hg_hybrid *h, double dt) {
     hipLaunchKernelGGL(hyb_finish<T>, gb, tb, 0, s, a, dt);
   }
   hg_dem_launch_sort(h->dem);
+  if (h->homog_stress || h->homog_vel) {
+    HomogArgs g;
+    g.hg = h->hg;
+    g.s = h->dem->slots[h->dem->cur];
+    g.oF = h->dem->f.oF;
+    g.cpoint = h->dem->cpoint;
+    g.count = h->hg.count + h->par;
+    g.next_count = h->hg.count + (1 - h->par);
+    g.stress = h->homog_stress ? 1 : 0;
+    g.vel = h->homog_vel ? 1 : 0;
+    {
+      ProfScope ps(prof, "hyb_homog_grains", s);
+      hipLaunchKernelGGL((hyb_homog_grains<T, BASIS>), gb, tb, 0, s, a, g);
+    }
+    if (h->homog_vel) {
+      ProfScope ps(prof, "hyb_homog_nodes", s);
+      hipLaunchKernelGGL(hyb_homog_nodes, gb, tb, 0, s, g);
+    }
+  }
   h->par = 1 - h->par;
 }
 
@@ -463,7 +584,9 @@ extern "C" hg_status hg_hybrid_step_graph(hg_hybrid *h, double dt, uint32_t nste
   if (h->mpm->phase_next != 0) return hg_fail(h->ctx, HG_ERR_INVALID, "a phased MPM step is in progress");
   if (h->mpm->prof.on || h->dem->prof.on) return hg_hybrid_step(h, dt, nsteps);
   HG_TRY(hg_dem_step(h->dem, dt, 0));
-  if (h->graph && (dt != h->graph_dt || h->mpm->aw != h->graph_aw || h->mpm->ah != h->graph_ah)) {
+  if (h->graph && (dt != h->graph_dt || h->mpm->aw != h->graph_aw || h->mpm->ah != h->graph_ah ||
+                   h->homog_stress != h->graph_homog_stress ||
+                   h->homog_vel != h->graph_homog_vel)) {
     hipGraphExecDestroy(h->graph);
     h->graph = nullptr;
   }
@@ -495,6 +618,8 @@ extern "C" hg_status hg_hybrid_step_graph(hg_hybrid *h, double dt, uint32_t nste
       h->graph_dt = dt;
       h->graph_aw = h->mpm->aw;
       h->graph_ah = h->mpm->ah;
+      h->graph_homog_stress = h->homog_stress;
+      h->graph_homog_vel = h->homog_vel;
     }
     for (uint32_t k = 0; k < pairs; ++k) HG_HIP(h->ctx, hipGraphLaunch(h->graph, h->stream));
   }
@@ -530,6 +655,77 @@ extern "C" hg_status hg_hybrid_download_coupling(hg_hybrid *h, uint8_t *coupled)
   HG_TRY(hg_hybrid_sync(h));
   if (coupled && h->dem->nbodies)
     HG_HIP(h->ctx, hipMemcpy(coupled, h->coupled, h->dem->nbodies, hipMemcpyDeviceToHost));
+  return HG_OK;
+}
+
+extern "C" hg_status hg_hybrid_set_homogenize(hg_hybrid *h, int32_t stress, int32_t velocity) {
+  if (!h) return HG_ERR_INVALID;
+  HG_HIP(h->ctx, hipStreamSynchronize(h->stream));
+  if ((stress || velocity) && !h->hg.stress) {
+    const uint32_t G = h->mpm->nnode;
+    const uint32_t B = std::max(h->dem->nbodies, 1u);
+    const int S = h->mpm->basis == HG_BASIS_LINEAR ? 2 : (h->mpm->basis == HG_BASIS_CUBIC ? 4 : 3);
+    h->hg.max_list = uint32_t(std::min<uint64_t>(uint64_t(G), uint64_t(S * S) * B));
+    HG_TRY(halloc(h, &h->hg.stress, size_t(G) * 24));
+    HG_TRY(halloc(h, &h->hg.vel, size_t(G) * 16));
+    HG_TRY(halloc(h, &h->hg.mass, size_t(G) * 8));
+    HG_TRY(halloc(h, &h->hg.mom, size_t(G) * 16));
+    HG_TRY(halloc(h, &h->hg.stamp, size_t(G) * 4));
+    HG_TRY(halloc(h, &h->hg.list, size_t(h->hg.max_list) * 4));
+    HG_TRY(halloc(h, &h->hg.count, 8));
+    HG_TRY(halloc(h, &h->dem->cpoint, size_t(B) * hgdem::kMaxOwned * 16));
+    HG_HIP(h->ctx, hipMemset(h->hg.stress, 0, size_t(G) * 24));
+    HG_HIP(h->ctx, hipMemset(h->hg.vel, 0, size_t(G) * 16));
+    HG_HIP(h->ctx, hipMemset(h->hg.mass, 0, size_t(G) * 8));
+    HG_HIP(h->ctx, hipMemset(h->hg.mom, 0, size_t(G) * 16));
+    HG_HIP(h->ctx, hipMemset(h->hg.stamp, 0, size_t(G) * 4));
+    HG_HIP(h->ctx, hipMemset(h->hg.count, 0, 8));
+    HG_HIP(h->ctx, hipDeviceSynchronize());
+  }
+  h->homog_stress = stress != 0;
+  h->homog_vel = velocity != 0;
+  return HG_OK;
+}
+
+extern "C" hg_status hg_hybrid_clear_homogenized(hg_hybrid *h) {
+  if (!h) return HG_ERR_INVALID;
+  HG_HIP(h->ctx, hipStreamSynchronize(h->stream));
+  if (!h->hg.stress) return HG_OK;
+  const uint32_t G = h->mpm->nnode;
+  HG_HIP(h->ctx, hipMemset(h->hg.stress, 0, size_t(G) * 24));
+  HG_HIP(h->ctx, hipMemset(h->hg.vel, 0, size_t(G) * 16));
+  HG_HIP(h->ctx, hipDeviceSynchronize());
+  return HG_OK;
+}
+
+extern "C" hg_status hg_hybrid_download_homogenized(hg_hybrid *h, double *stress,
+                                                    double *velocity) {
+  if (!h) return HG_ERR_INVALID;
+  HG_TRY(hg_hybrid_sync(h));
+  // one MPM scene: its nodes start at 0 with the padded pitch snx
+  const hgmpm::SceneDev &sc = h->mpm->scenes[0];
+  const size_t nx = size_t(sc.nnx), ny = size_t(sc.nny), G = h->mpm->nnode;
+  std::vector<double> st(h->hg.stress ? 3 * G : 0), ve(h->hg.vel ? 2 * G : 0);
+  if (h->hg.stress) {
+    HG_HIP(h->ctx, hipMemcpy(st.data(), h->hg.stress, 3 * G * 8, hipMemcpyDeviceToHost));
+    HG_HIP(h->ctx, hipMemcpy(ve.data(), h->hg.vel, 2 * G * 8, hipMemcpyDeviceToHost));
+  }
+  for (size_t iy = 0; iy < ny; ++iy)
+    for (size_t ix = 0; ix < nx; ++ix) {
+      const size_t r = iy * nx + ix, d = iy * size_t(sc.snx) + ix;
+      if (stress) {
+        const bool on = !st.empty();
+        stress[4 * r] = on ? st[3 * d] : 0.0;
+        stress[4 * r + 1] = on ? st[3 * d + 1] : 0.0;
+        stress[4 * r + 2] = on ? st[3 * d + 1] : 0.0;
+        stress[4 * r + 3] = on ? st[3 * d + 2] : 0.0;
+      }
+      if (velocity) {
+        const bool on = !ve.empty();
+        velocity[2 * r] = on ? ve[2 * d] : 0.0;
+        velocity[2 * r + 1] = on ? ve[2 * d + 1] : 0.0;
+      }
+    }
   return HG_OK;
 }
 

@@ -18,6 +18,18 @@ struct GrainGrid {
   uint32_t max_list;
 };
 
+// Per-step homogenisation accumulators on the MPM node storage.
+struct HomogGrid {
+  double *stress;  // [3G] xx, xy, yy
+  double *vel;     // [2G]
+  double *mass;    // [G]  this step's grain mass per node
+  double *mom;     // [2G] this step's grain momentum per node
+  uint32_t *stamp; // [G]  1 while the node is on the list
+  uint32_t *list;  // nodes the grains reached this step
+  uint32_t *count; // [2]
+  uint32_t max_list;
+};
+
 }  // namespace hghyb
 
 struct hg_hybrid {
@@ -30,6 +42,9 @@ struct hg_hybrid {
   uint8_t *coupled = nullptr;
   int32_t *err = nullptr;
   int par = 0;  // parity of the next step (which list counter it uses)
+  hghyb::HomogGrid hg = {};
+  bool homog_stress = false, homog_vel = false;
+  bool graph_homog_stress = false, graph_homog_vel = false;  // as captured
   hipGraphExec_t graph = nullptr;
   double graph_dt = -1.0;
   int graph_aw = 0, graph_ah = 0;  // MPM tile shape the graph was captured with

@@ -260,6 +260,29 @@ HybridGrains2DSim::HybridGrains2DSim(const HybridIntegratorSettings &settings,
   const hg_dem_desc dd = dem_desc(weighted, discrete.penalty, 1.0, precision, px, pn);
   const hg_mpm_desc md = continuum_.desc(precision);
   dev_.check(hg_hybrid_create(dev_.ctx(), &md, &dd, dem_weights.data(), &h_), "hg_hybrid_create");
+  // updateHybridStates_poorman runs after every step once coupling_force is
+  // present; its homogenisation depends on these flags
+  const PoormanEnrichmentSettings &p = settings_.poorman_settings;
+  const bool hs = p.enabled && p.homogenize_stress && p.grid_smoothing_homogenized_stress;
+  const bool hv = p.enabled && p.homogenize_velocity && p.grid_smoothing_homogenized_velocity;
+  if (hs || hv)
+    dev_.check(hg_hybrid_set_homogenize(h_, hs ? 1 : 0, hv ? 1 : 0), "hg_hybrid_set_homogenize");
+}
+
+std::vector<double> HybridGrains2DSim::homogenizedStress() {
+  std::vector<double> s(4 * continuum_.physics_grid.numGridPoints());
+  dev_.check(hg_hybrid_download_homogenized(h_, s.data(), nullptr), "hg_hybrid_download_homogenized");
+  return s;
+}
+
+std::vector<double> HybridGrains2DSim::homogenizedVelocity() {
+  std::vector<double> v(2 * continuum_.physics_grid.numGridPoints());
+  dev_.check(hg_hybrid_download_homogenized(h_, nullptr, v.data()), "hg_hybrid_download_homogenized");
+  return v;
+}
+
+void HybridGrains2DSim::clearHomogenized() {
+  dev_.check(hg_hybrid_clear_homogenized(h_), "hg_hybrid_clear_homogenized");
 }
 
 HybridGrains2DSim::~HybridGrains2DSim() {

@@ -53,6 +53,26 @@ class GpuHybrid:
         self.ctx.check(self.L.hg_hybrid_download_coupling(self._h, ptr(c, C.c_uint8)))
         return c
 
+    # -- per-step homogenisation (ResamplingTools.cpp:744-884) -----------------
+    def set_homogenize(self, stress: bool = True, velocity: bool = True) -> None:
+        """Accumulate the grains' homogenised stress and velocity on the grid
+        after every step, as updateHybridStates_poorman does when the scene
+        sets homogenize_* with grid_smoothing_homogenized_*."""
+        self.ctx.check(self.L.hg_hybrid_set_homogenize(self._h, int(bool(stress)),
+                                                       int(bool(velocity))))
+
+    def homogenized(self) -> dict:
+        """The accumulators: stress (G x 4, column-major 2x2) and velocity
+        (G x 2), nodes in the reference's flat order."""
+        s = self.mpm_scene
+        G = (int(s.cell_count[0]) + 1) * (int(s.cell_count[1]) + 1)
+        st, ve = np.zeros((G, 4)), np.zeros((G, 2))
+        self.ctx.check(self.L.hg_hybrid_download_homogenized(self._h, ptr(st), ptr(ve)))
+        return {"stress": st, "velocity": ve}
+
+    def clear_homogenized(self) -> None:
+        self.ctx.check(self.L.hg_hybrid_clear_homogenized(self._h))
+
     def points(self) -> dict:
         return self.continuum.points()
 

@@ -122,6 +122,8 @@ def lib() -> C.CDLL:
     L.oracle_dem_pairs_bruteforce.restype = C.c_uint32
     L.oracle_dem_last_force.argtypes = [PD, _dp]
     L.oracle_dem_last_force.restype = None
+    L.oracle_dem_circle_contacts.argtypes = [PD, _ip, _ip, _dp, _dp, C.c_uint32]
+    L.oracle_dem_circle_contacts.restype = C.c_uint32
     _lib = L
     return L
 
@@ -271,6 +273,18 @@ class OracleDem:
                                    _p(j, C.c_int32), _p(ds), n)
         return {"type": t, "i": i, "j": j, "delta_s": ds}
 
+    def circle_contacts(self) -> dict:
+        """Circle-circle contacts of the last step: owner i, partner j, contact
+        point p and the normal + friction force f on i."""
+        n = self.L.oracle_dem_circle_contacts(C.byref(self.c), None, None, None, None, 0)
+        i = np.zeros(n, np.int32)
+        j = np.zeros(n, np.int32)
+        p = np.zeros((n, 2))
+        f = np.zeros((n, 2))
+        self.L.oracle_dem_circle_contacts(C.byref(self.c), _p(i, C.c_int32), _p(j, C.c_int32),
+                                          _p(p), _p(f), n)
+        return {"i": i, "j": j, "p": p, "f": f}
+
     def pairs_bruteforce(self) -> np.ndarray:
         n = self.L.oracle_dem_pairs_bruteforce(C.byref(self.c), None, None, 0)
         i = np.zeros(n, np.int32)
@@ -292,7 +306,7 @@ class OracleHybridC(C.Structure):
         ("mpm", C.POINTER(OracleMpmC)), ("dem", C.POINTER(OracleDemC)),
         ("dem_pos_before", _dp), ("dem_mass_grid", _dp), ("dem_mom_grid", _dp),
         ("v_after", _dp), ("at_after", _dp), ("homog_factor", _dp),
-        ("coupled", _u8p),
+        ("coupled", _u8p), ("homog_stress", _dp), ("homog_velocity", _dp),
     ]
 
 
@@ -301,7 +315,7 @@ class OracleHybrid:
     an OracleMpm and an OracleDem (with mass weights) plus the coupling
     buffers."""
 
-    def __init__(self, mpm_scene, dem_scene, weights):
+    def __init__(self, mpm_scene, dem_scene, weights, homogenize: bool = False):
         self.mpm = OracleMpm(mpm_scene)
         self.dem = OracleDem(dem_scene, weights=weights)
         L = self.mpm.L
@@ -320,6 +334,11 @@ class OracleHybrid:
                   "homog_factor"):
             setattr(c, k, _p(getattr(self, k)))
         c.coupled = _p(self.coupled, C.c_uint8)
+        self.homog_stress = np.zeros(4 * G) if homogenize else None
+        self.homog_velocity = np.zeros(2 * G) if homogenize else None
+        if homogenize:
+            c.homog_stress = _p(self.homog_stress)
+            c.homog_velocity = _p(self.homog_velocity)
         self.c = c
         P = C.POINTER(OracleHybridC)
         L.oracle_hybrid_step.argtypes = [P, C.c_double, C.c_uint32]
@@ -329,3 +348,9 @@ class OracleHybrid:
     def step(self, dt: float, nsteps: int = 1) -> None:
         if self.L.oracle_hybrid_step(C.byref(self.c), float(dt), int(nsteps)) != 0:
             raise MemoryError("oracle_hybrid_step")
+
+    def homogenized(self) -> dict:
+        """The accumulated homogenised stress (G x 4, column-major 2x2) and
+        velocity (G x 2), nodes in the reference's order."""
+        return {"stress": self.homog_stress.reshape(-1, 4).copy(),
+                "velocity": self.homog_velocity.reshape(-1, 2).copy()}

@@ -37,6 +37,8 @@ typedef struct contact {
   double vrel[2];
   double r0[2], r1[2];
   double w;
+  double p[2]; /* circle pairs: the contact point (CircleCircleConstraint.cpp:214) */
+  double f[2]; /* normal + friction force on body i (set by the force pass) */
 } contact;
 
 typedef struct hist {
@@ -339,6 +341,8 @@ static int dem_step(oracle_dem *s, double dt) {
       c.r0[1] = p[1] - qa[1];
       c.r1[0] = p[0] - qb[0];
       c.r1[1] = p[1] - qb[1];
+      c.p[0] = p[0];
+      c.p[1] = p[1];
       const double t0[2] = {-c.r0[1], c.r0[0]};
       const double t1[2] = {-c.r1[1], c.r1[0]};
       const double *va = &s->v[3 * a], *vb = &s->v[3 * b];
@@ -411,6 +415,8 @@ static int dem_step(oracle_dem *s, double dt) {
     contact *c = &h->c[k];
     double tf[2];
     contact_force(s, c, tf);
+    c->f[0] = tf[0];
+    c->f[1] = tf[1];
     const uint32_t a = (uint32_t)c->i;
     F[3 * a] += tf[0];
     F[3 * a + 1] += tf[1];
@@ -453,6 +459,31 @@ uint32_t oracle_dem_contacts(const oracle_dem *s, int32_t *type, int32_t *i, int
     }
   }
   return h->n;
+}
+
+uint32_t oracle_dem_circle_contacts(const oracle_dem *s, int32_t *i, int32_t *j, double *p,
+                                    double *f, uint32_t max) {
+  const hist *h = (const hist *)s->hist;
+  if (!h) return 0;
+  uint32_t n = 0;
+  for (uint32_t k = 0; k < h->n; ++k) {
+    const contact *c = &h->c[k];
+    if (c->type != ORACLE_CONTACT_CIRCLE) continue;
+    if (n < max) {
+      if (i) i[n] = c->i;
+      if (j) j[n] = c->j;
+      if (p) {
+        p[2 * n] = c->p[0];
+        p[2 * n + 1] = c->p[1];
+      }
+      if (f) {
+        f[2 * n] = c->f[0];
+        f[2 * n + 1] = c->f[1];
+      }
+    }
+    ++n;
+  }
+  return n;
 }
 
 void oracle_dem_last_force(const oracle_dem *s, double *F) {

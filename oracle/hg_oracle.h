@@ -162,6 +162,11 @@ uint32_t oracle_dem_contacts(const oracle_dem *s, int32_t *type, int32_t *i, int
 uint32_t oracle_dem_pairs_bruteforce(const oracle_dem *s, int32_t *i, int32_t *j, uint32_t max);
 /* Total force per body of the last step (gravity + contacts, fixed zeroed). */
 void oracle_dem_last_force(const oracle_dem *s, double *F);
+/* The circle-circle entries of the contact cache in its order, with each
+ * contact's point and its normal + friction force on body i, as the cache
+ * holds them after a step (PenaltyImpactFrictionMap.h:59-87, :843-844). */
+uint32_t oracle_dem_circle_contacts(const oracle_dem *s, int32_t *i, int32_t *j, double *p,
+                                    double *f, uint32_t max);
 
 /* ---- hybrid: hybridgrains2dnew coupling ------------------------------------ */
 
@@ -179,6 +184,8 @@ typedef struct oracle_hybrid {
   double *at_after;         /* 2G */
   double *homog_factor;     /* G */
   uint8_t *coupled;         /* B */
+  double *homog_stress;     /* 4G column-major 2x2 accumulators, NULL: not homogenised */
+  double *homog_velocity;   /* 2G accumulators, NULL: not homogenised */
 } oracle_hybrid;
 
 /* Coupling.cpp:22-285 (serial branch, use_pre_position) */
@@ -191,7 +198,16 @@ void oracle_hybrid_update_velocities_on_grid(oracle_hybrid *h);
 void oracle_hybrid_transfer_to_grain(oracle_hybrid *h);
 /* Coupling.cpp:581-591 */
 void oracle_hybrid_update_positions(oracle_hybrid *h, double dt);
-/* HybridGrains2DSim.cpp:116-227 without updateHybridState, nsteps times */
+/* ResamplingTools::homogenizeStress (ResamplingTools.cpp:744-801): adds every
+ * circle contact's stress -(f r^T + r f^T)/2, spread with the MPM weights
+ * (hl = h/4) from its contact point, divided by the cell area. */
+void oracle_hybrid_homogenize_stress(oracle_hybrid *h);
+/* ResamplingTools::homogenizeVelocity (ResamplingTools.cpp:818-874): adds the
+ * grains' mass-weighted nodal velocity at every node they reach. */
+int oracle_hybrid_homogenize_velocity(oracle_hybrid *h);
+/* HybridGrains2DSim.cpp:116-227 without the zone resampling, nsteps times;
+ * with homog_stress / homog_velocity set, each step then homogenises as
+ * updateHybridStates_poorman does every step (HybridGrains2DSim.cpp:545-556). */
 int oracle_hybrid_step(oracle_hybrid *h, double dt, uint32_t nsteps);
 
 #ifdef __cplusplus

@@ -5,14 +5,22 @@
  * resampling (updateHybridState), which is outside the explicit step.
  * Checker only.
  *
+ * Also the per-step homogenisation of the grains (ResamplingTools.cpp:744-884).
+ *
  * Parity pins (tests/test_oracle_hybrid.py): the exchange conserves each
  * node's momentum and leaves equal MPM and DEM node velocities (the
  * reference's own Debug check, Coupling.cpp:566-577); grains the MPM mass does
- * not surround stay uncoupled; a coupled free fall stays exact.
+ * not surround stay uncoupled; a coupled free fall stays exact.  Homogenised
+ * stress: its node sum times the cell area equals the sum of the contacts'
+ * stress tensors computed independently from the contact list (partition of
+ * unity), and a compressed head-on pair gives -|f||r| along the axis;
+ * homogenised velocity: grains moving together give that velocity at every
+ * node they reach, once per step.
  */
 #include "hg_oracle.h"
 
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 static uint64_t nodes(const oracle_mpm *s) {
@@ -142,6 +150,93 @@ void oracle_hybrid_update_positions(oracle_hybrid *h, double dt) {
   }
 }
 
+/* ResamplingTools.cpp:744-801.  The cache's circle contacts in its order;
+ * r is taken at the current (end-of-step) positions, as the reference calls
+ * this after the step. */
+void oracle_hybrid_homogenize_stress(oracle_hybrid *h) {
+  oracle_mpm *s = h->mpm;
+  oracle_dem *d = h->dem;
+  const uint32_t nx = s->cell_count[0] + 1;
+  const double cw = s->cell_width;
+  const double cell_area = cw * cw;
+  const double hl = cw * 0.25;
+  const uint32_t nc = oracle_dem_circle_contacts(d, NULL, NULL, NULL, NULL, 0);
+  if (nc == 0) return;
+  int32_t *ci = (int32_t *)malloc(2 * (size_t)nc * sizeof(int32_t));
+  double *pf = (double *)malloc(4 * (size_t)nc * sizeof(double));
+  if (!ci || !pf) {
+    free(ci);
+    free(pf);
+    return;
+  }
+  oracle_dem_circle_contacts(d, ci, ci + nc, pf, pf + 2 * nc, nc);
+  for (uint32_t k = 0; k < nc; ++k) {
+    const uint32_t a = (uint32_t)ci[k], b = (uint32_t)ci[nc + k];
+    const double *p = &pf[2 * k], *f = &pf[2 * nc + 2 * k];
+    const double r[2] = {d->q[3 * a] - d->q[3 * b], d->q[3 * a + 1] - d->q[3 * b + 1]};
+    /* -0.5 (f r^T + r f^T), column-major */
+    double st[4];
+    st[0] = -0.5 * (f[0] * r[0] + r[0] * f[0]);
+    st[1] = -0.5 * (f[1] * r[0] + r[1] * f[0]);
+    st[2] = -0.5 * (f[0] * r[1] + r[0] * f[1]);
+    st[3] = -0.5 * (f[1] * r[1] + r[1] * f[1]);
+    int64_t lo[2], hi[2];
+    oracle_stencil(s, p, hl, lo, hi);
+    for (int64_t iy = lo[1]; iy <= hi[1]; ++iy)
+      for (int64_t ix = lo[0]; ix <= hi[0]; ++ix) {
+        if (!node_valid(s, ix, iy)) continue;
+        const double w = oracle_weight(s, p, hl, ix, iy);
+        const int64_t n = iy * nx + ix;
+        for (int e = 0; e < 4; ++e) h->homog_stress[4 * n + e] += w * st[e] / cell_area;
+      }
+  }
+  free(ci);
+  free(pf);
+}
+
+/* ResamplingTools.cpp:818-874: per-step nodal mass and momentum of the
+ * non-fixed circles (end-of-step positions and velocities), then the nodal
+ * velocity added wherever the mass is positive. */
+int oracle_hybrid_homogenize_velocity(oracle_hybrid *h) {
+  oracle_mpm *s = h->mpm;
+  oracle_dem *d = h->dem;
+  const uint32_t nx = s->cell_count[0] + 1;
+  const uint64_t G = nodes(s);
+  const double hl = s->cell_width * 0.25;
+  double *mom = (double *)calloc(2 * G, sizeof(double));
+  double *mass = (double *)calloc(G, sizeof(double));
+  if (!mom || !mass) {
+    free(mom);
+    free(mass);
+    return -1;
+  }
+  for (uint32_t b = 0; b < d->nbodies; ++b) {
+    if (d->fixed && d->fixed[b]) continue;
+    const double m = d->mass[3 * b];
+    const double v0 = d->v[3 * b], v1 = d->v[3 * b + 1];
+    const double *q = &d->q[3 * b];
+    int64_t lo[2], hi[2];
+    oracle_stencil(s, q, hl, lo, hi);
+    for (int64_t iy = lo[1]; iy <= hi[1]; ++iy)
+      for (int64_t ix = lo[0]; ix <= hi[0]; ++ix) {
+        if (!node_valid(s, ix, iy)) continue;
+        const double w = oracle_weight(s, q, hl, ix, iy);
+        const int64_t n = iy * nx + ix;
+        mom[2 * n] += w * m * v0;
+        mom[2 * n + 1] += w * m * v1;
+        mass[n] += w * m;
+      }
+  }
+  for (uint64_t n = 0; n < G; ++n)
+    if (mass[n] > 0.0) {
+      h->homog_velocity[2 * n] += mom[2 * n] / mass[n];
+      h->homog_velocity[2 * n + 1] += mom[2 * n + 1] / mass[n];
+    }
+  free(mom);
+  free(mass);
+  return 0;
+}
+
 int oracle_hybrid_step(oracle_hybrid *h, double dt, uint32_t nsteps) {
   oracle_mpm *s = h->mpm;
   oracle_dem *d = h->dem;
@@ -160,6 +255,8 @@ int oracle_hybrid_step(oracle_hybrid *h, double dt, uint32_t nsteps) {
     oracle_hybrid_transfer_to_grain(h);
     oracle_mpm_flow_second_phase(s, dt);
     oracle_hybrid_update_positions(h, dt);
+    if (h->homog_stress) oracle_hybrid_homogenize_stress(h);
+    if (h->homog_velocity && oracle_hybrid_homogenize_velocity(h)) return -1;
   }
   return 0;
 }

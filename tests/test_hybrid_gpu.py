@@ -121,3 +121,54 @@ def test_fp32_tracks_fp64(hg_ctx):
     assert (c == o.coupled).mean() > 0.99
     assert np.abs(g.points()["q"] - o.mpm.points()["q"]).max() < 1e-8
     assert np.abs(g.bodies()["q"][:, :2] - o.dem.state()["q"][:, :2]).max() < 1e-8
+
+
+def _agitated(seed):
+    m, d, w = scenes.hybrid_column(seed=seed)
+    d.v[:, :2] = np.random.default_rng(seed).normal(scale=0.3, size=(d.nbodies, 2))
+    return m, d, w
+
+
+def test_homogenization_matches_oracle(hg_ctx):
+    """Per-step homogenised stress and velocity (ResamplingTools.cpp:744-884)
+    accumulated over 30 steps against the restatement; the node sums are FP64
+    atomics (order varies by a few ulps)."""
+    m, d, w = _agitated(7)
+    g = GpuHybrid(m, d, w, HG_FP64, ctx=hg_ctx)
+    g.set_homogenize(True, True)
+    o = oracle.OracleHybrid(m, d, w, homogenize=True)
+    g.step(DT, 30)
+    o.step(DT, 30)
+    assert len(o.dem.circle_contacts()["i"]) > 100
+    gh, oh = g.homogenized(), o.homogenized()
+    assert np.abs(oh["stress"]).max() > 0 and np.abs(oh["velocity"]).max() > 0
+    assert_close(gh["stress"], oh["stress"], 1e-10, "homogenized stress")
+    assert_close(gh["velocity"], oh["velocity"], 1e-10, "homogenized velocity")
+    _compare(g, o, 1e-11, 1e-10)
+
+
+def test_homogenization_graph_clear_and_off(hg_ctx):
+    m, d, w = _agitated(8)
+    a = GpuHybrid(m, d, w, HG_FP64, ctx=hg_ctx)
+    b = GpuHybrid(m, d, w, HG_FP64, ctx=hg_ctx)
+    c = GpuHybrid(m, d, w, HG_FP64, ctx=hg_ctx)
+    a.set_homogenize(True, True)
+    b.set_homogenize(True, True)
+    a.step(DT, 12)
+    b.step(DT, 12, graph=True)
+    c.step(DT, 12, graph=True)
+    ha, hb = a.homogenized(), b.homogenized()
+    for k in ha:
+        assert np.abs(ha[k]).max() > 0
+        assert_close(hb[k], ha[k], 1e-12, k)
+    # homogenisation only reads the state: the step itself is unchanged
+    pa, pc = a.points(), c.points()
+    for k in pa:
+        assert_close(pc[k], pa[k], 1e-12, k)
+    hc = c.homogenized()
+    assert not hc["stress"].any() and not hc["velocity"].any()
+    b.clear_homogenized()
+    hb = b.homogenized()
+    assert not hb["stress"].any() and not hb["velocity"].any()
+    b.step(DT, 4, graph=True)
+    assert np.abs(b.homogenized()["velocity"]).max() > 0

@@ -65,3 +65,85 @@ def test_coupled_free_fall_is_exact():
     g = m.gravity
     assert np.abs(o.mpm.v.reshape(-1, 2) - n * dt * g).max() < 1e-13
     assert np.abs(o.dem.v.reshape(-1, 3)[:, :2] - n * dt * g).max() < 1e-13
+
+
+# ---- per-step homogenisation (ResamplingTools.cpp:744-884) ------------------
+
+def _contact_stress_sum(o):
+    """Sum over the cache's circle contacts of -(f r^T + r f^T)/2, with r at
+    the end-of-step positions, computed here from the contact list."""
+    c = o.dem.circle_contacts()
+    q = o.dem.q.reshape(-1, 3)[:, :2]
+    r = q[c["i"]] - q[c["j"]]
+    f = c["f"]
+    S = -0.5 * (np.einsum("ka,kb->ab", f, r) + np.einsum("ka,kb->ab", r, f))
+    return S, len(c["i"])
+
+
+def _agitated(seed):
+    # grains with random velocities, so that contacts form within a few steps
+    m, d, w = scenes.hybrid_column(seed=seed)
+    d.v[:, :2] = np.random.default_rng(seed).normal(scale=0.3, size=(d.nbodies, 2))
+    return m, d, w
+
+
+def test_homogenized_stress_spreads_each_contact_stress_once():
+    m, d, w = _agitated(4)
+    o = oracle.OracleHybrid(m, d, w, homogenize=True)
+    o.step(1e-5, 30)
+    area = m.cell_width ** 2
+    h0 = o.homogenized()["stress"]
+    o.step(1e-5, 1)
+    S, n = _contact_stress_sum(o)
+    assert n > 100
+    hs = o.homogenized()["stress"]
+    got = (hs - h0).sum(0).reshape(2, 2, order="F") * area
+    assert np.abs(got - S).max() <= 1e-11 * np.abs(S).max()
+    assert np.array_equal(hs[:, 1], hs[:, 2])  # symmetric, entry by entry
+
+
+def test_homogenized_stress_of_a_compressed_pair():
+    # two grains at rest overlapping by delta along x, far from the column:
+    # the first step's contact force on the lower index is -kn delta along x
+    # (no spring or damping yet), so the spread stress sums to -kn delta r_x
+    from libhybridgrains_amd.dem import DemScene, circle_mass
+
+    m, _, _ = scenes.hybrid_column(seed=5)
+    r0, delta = 1.2e-3, 2.0e-5
+    x0, y0 = 0.3, 0.2
+    q = np.array([[x0, y0, 0.0], [x0 + 2 * r0 - delta, y0, 0.0]])
+    rad = np.array([r0, r0])
+    d = DemScene(q=q, v=np.zeros((2, 3)), radius=rad, mass=circle_mass(2500.0, rad),
+                 fixed=np.zeros(2, np.uint8), gravity=np.zeros(2), kn=1.0e4, kt=1.0e4,
+                 gamman=0.0, gammat=0.0, mu=0.3)
+    o = oracle.OracleHybrid(m, d, np.ones(2), homogenize=True)
+    o.step(1e-6, 1)
+    qe = o.dem.q.reshape(-1, 3)
+    rx = qe[0, 0] - qe[1, 0]
+    fx = -1.0e4 * delta
+    hs = o.homogenized()["stress"].sum(0) * m.cell_width ** 2
+    assert abs(hs[0] - (-fx * rx)) <= 1e-12 * abs(fx * rx)
+    assert hs[0] < 0.0  # compression
+    assert abs(hs[3]) <= 1e-20 and abs(hs[1]) <= 1e-20
+
+
+def test_homogenized_velocity_of_grains_moving_together():
+    m, d, w = scenes.hybrid_column(seed=6)
+    for s in (m, d):
+        s.plane_x = np.zeros((0, 2))
+        s.plane_n = np.zeros((0, 2))
+    m.plane_lower = np.zeros(0)
+    m.plane_behavior = np.zeros(0, np.int32)
+    m.gravity = np.zeros(2)
+    d.gravity = np.zeros(2)
+    d.radius = d.radius * 0.5  # no grain contacts
+    d.q[:, 1] += 0.05  # no coupling
+    d.v[:, :2] = [0.3, -0.2]
+    d.v[:, 2] = 0.0
+    o = oracle.OracleHybrid(m, d, w, homogenize=True)
+    o.step(1e-5, 3)
+    assert o.coupled.sum() == 0
+    hv = o.homogenized()["velocity"]
+    hit = np.abs(hv).sum(1) > 0
+    assert hit.sum() > 50
+    assert np.abs(hv[hit] - 3 * np.array([0.3, -0.2])).max() <= 1e-14
