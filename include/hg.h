@@ -302,6 +302,25 @@ hg_status hg_hybrid_estimate_bytes(const hg_mpm_desc *mpm, const hg_dem_desc *de
                                    uint64_t *bytes);
 hg_status hg_hybrid_device_bytes(hg_hybrid *h, uint64_t *bytes);
 
+/* ---- snapshot and resume -------------------------------------------------------
+ * The reference's CLIs serialise their host state for a bit-identical resume
+ * (mpmgrains2d_cli.cpp:154-247, the -s/-r options); the DEM contact history
+ * and the hybrid's homogenisation accumulators are beyond what they save.
+ * hg_*_snapshot writes a handle's evolving state into buf: *bytes holds the
+ * buffer size on entry and the snapshot's size on return (buf NULL: size
+ * only).  hg_*_restore loads it into a handle created from the same
+ * descriptors (same scenes, sizes and precision; DEM mass weights are set as
+ * for the original handle); stepping then continues as the snapshotted
+ * handle would -- bitwise for MPM and DEM, to the hybrid's own run-to-run
+ * spread (its grain-side node sums are FP64 atomics).  A blob of another
+ * kind, precision or size is refused with HG_ERR_INVALID. */
+hg_status hg_mpm_snapshot(hg_mpm *h, void *buf, uint64_t *bytes);
+hg_status hg_mpm_restore(hg_mpm *h, const void *buf, uint64_t bytes);
+hg_status hg_dem_snapshot(hg_dem *h, void *buf, uint64_t *bytes);
+hg_status hg_dem_restore(hg_dem *h, const void *buf, uint64_t bytes);
+hg_status hg_hybrid_snapshot(hg_hybrid *h, void *buf, uint64_t *bytes);
+hg_status hg_hybrid_restore(hg_hybrid *h, const void *buf, uint64_t bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,6 +77,13 @@ void markHostModified(SimulationState &state);
 void setResidency(SimulationState &state, Residency r, int precision = HG_FP64);
 void syncToHost(SimulationState &state);
 
+// Resumable snapshots of the device state (hg_mpm_snapshot / hg_mpm_restore;
+// the reference serialises its host state instead, mpmgrains2d_cli.cpp:154-247).
+// restoreDeviceState loads a snapshot taken from a state read from the same
+// scene; the next flow continues exactly where the snapshot was taken.
+std::vector<std::uint8_t> snapshotDeviceState(SimulationState &state);
+void restoreDeviceState(SimulationState &state, const std::vector<std::uint8_t> &blob);
+
 namespace ExplicitIntegrator {
 void flow(const double &dt, SimulationState &state);
 void flow_zero_phase(SimulationState &state);
@@ -133,6 +140,9 @@ class RigidBody2DSim {
   // RigidBody2DSim::flowWithWeight (RigidBody2DSim.cpp:1041-1069)
   void flowWithWeight(unsigned iteration, const Rational &dt,
                       const PenaltyImpactFrictionMap &ifmap, bool reduce_bandwidth = false);
+  // Bodies and contact history (hg_dem_snapshot / hg_dem_restore).
+  std::vector<std::uint8_t> snapshot(const PenaltyImpactFrictionMap &ifmap);
+  void restore(const std::vector<std::uint8_t> &blob, const PenaltyImpactFrictionMap &ifmap);
   hg_dem *handle() const { return h_; }
 
  private:
@@ -177,6 +187,10 @@ class HybridGrains2DSim {
   std::vector<double> homogenizedVelocity();
   // ResamplingTools::clearHomogenizedStress / clearHomogenizedVelocity
   void clearHomogenized();
+  // Continuum, grains with their contact history, and the homogenisation
+  // accumulators (hg_hybrid_snapshot / hg_hybrid_restore).
+  std::vector<std::uint8_t> snapshot();
+  void restore(const std::vector<std::uint8_t> &blob);
   hg_hybrid *handle() const { return h_; }
 
  private:

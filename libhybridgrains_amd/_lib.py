@@ -179,6 +179,12 @@ def lib() -> C.CDLL:
         "hg_hybrid_set_homogenize": ([vp, C.c_int32, C.c_int32], C.c_int32),
         "hg_hybrid_download_homogenized": ([vp, _dp, _dp], C.c_int32),
         "hg_hybrid_clear_homogenized": ([vp], C.c_int32),
+        "hg_mpm_snapshot": ([vp, C.c_void_p, C.POINTER(C.c_uint64)], C.c_int32),
+        "hg_mpm_restore": ([vp, C.c_void_p, C.c_uint64], C.c_int32),
+        "hg_dem_snapshot": ([vp, C.c_void_p, C.POINTER(C.c_uint64)], C.c_int32),
+        "hg_dem_restore": ([vp, C.c_void_p, C.c_uint64], C.c_int32),
+        "hg_hybrid_snapshot": ([vp, C.c_void_p, C.POINTER(C.c_uint64)], C.c_int32),
+        "hg_hybrid_restore": ([vp, C.c_void_p, C.c_uint64], C.c_int32),
         "hg_mpm_estimate_bytes": ([C.POINTER(MpmDesc), C.c_uint32, C.POINTER(C.c_uint64)], C.c_int32),
         "hg_mpm_device_bytes": ([vp, C.POINTER(C.c_uint64)], C.c_int32),
         "hg_dem_estimate_bytes": ([C.POINTER(DemDesc), C.c_uint32, C.POINTER(C.c_uint64)], C.c_int32),
@@ -263,3 +269,18 @@ def default_context() -> Context:
     if _default_ctx is None:
         _default_ctx = Context(0)
     return _default_ctx
+
+
+def snapshot_blob(ctx, fn, handle) -> bytes:
+    """Calls an hg_*_snapshot entry point twice (size, then data)."""
+    n = C.c_uint64(0)
+    ctx.check(fn(handle, None, C.byref(n)))
+    buf = C.create_string_buffer(n.value)
+    ctx.check(fn(handle, buf, C.byref(n)))
+    return buf.raw[: n.value]
+
+
+def restore_blob(ctx, fn, handle, blob: bytes) -> None:
+    """Calls an hg_*_restore entry point with a snapshot blob."""
+    buf = C.create_string_buffer(bytes(blob), len(blob))
+    ctx.check(fn(handle, buf, len(blob)))

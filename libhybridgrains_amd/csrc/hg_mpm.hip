@@ -1042,6 +1042,17 @@ hg_status hg_mpm_launch_coupled_p2g(hg_mpm *h, double dt) {
 
 void hg_mpm_launch_g2p(hg_mpm *h) { launch_g2p_and_sort(h); }
 
+hg_status hg_mpm_rebin(hg_mpm *h) {
+  if (h->npoints) {
+    if (h->precision == HG_FP64) launch_bin_p<double>(h);
+    else launch_bin_p<float>(h);
+  }
+  launch_sort(h);
+  HG_HIP(h->ctx, hipGetLastError());
+  HG_HIP(h->ctx, hipStreamSynchronize(h->stream));
+  return HG_OK;
+}
+
 static void launch_step(hg_mpm *h) {
   launch_p2g(h, fused_store(h));
   launch_g2p_and_sort(h);

@@ -52,3 +52,6 @@ struct hg_mpm {
 // the sort.
 hg_status hg_mpm_launch_coupled_p2g(hg_mpm *h, double dt);
 void hg_mpm_launch_g2p(hg_mpm *h);
+// Re-keys and re-sorts the points of the current buffer, as at creation,
+// and waits for it (used after the buffer was rewritten).
+hg_status hg_mpm_rebin(hg_mpm *h);

@@ -165,6 +165,13 @@ class GpuDem:
     def set_profiling(self, on: bool) -> None:
         self.ctx.check(self.L.hg_dem_set_profiling(self._h, int(bool(on))))
 
+    def snapshot(self) -> bytes:
+        """Bodies and contact history as a blob (hg_dem_snapshot)."""
+        return _lib.snapshot_blob(self.ctx, self.L.hg_dem_snapshot, self._h)
+
+    def restore(self, blob: bytes) -> None:
+        _lib.restore_blob(self.ctx, self.L.hg_dem_restore, self._h, blob)
+
     def hold(self, us: float) -> None:
         """As GpuMpm.hold."""
         self.ctx.check(self.L.hg_dem_hold(self._h, float(us)))

@@ -86,6 +86,26 @@ static Device &device_of(SimulationState &state) {
   return b.device ? *b.device : Device::defaultDevice();
 }
 
+template <typename H, typename F>
+static std::vector<std::uint8_t> snapshot_of(Device &dev, H *h, F fn, const char *where) {
+  std::uint64_t n = 0;
+  dev.check(fn(h, nullptr, &n), where);
+  std::vector<std::uint8_t> blob(n);
+  dev.check(fn(h, blob.data(), &n), where);
+  blob.resize(n);
+  return blob;
+}
+
+std::vector<std::uint8_t> snapshotDeviceState(SimulationState &state) {
+  return snapshot_of(device_of(state), bound(state), hg_mpm_snapshot, "hg_mpm_snapshot");
+}
+
+void restoreDeviceState(SimulationState &state, const std::vector<std::uint8_t> &blob) {
+  hg_mpm *h = bound(state);
+  device_of(state).check(hg_mpm_restore(h, blob.data(), blob.size()), "hg_mpm_restore");
+  MpmBinding::of(state).gpu->download(state);
+}
+
 static void after_call(SimulationState &state) {
   MpmBinding &b = MpmBinding::of(state);
   if (b.residency == Residency::SyncEveryCall) b.gpu->download(state);
@@ -227,6 +247,18 @@ void RigidBody2DSim::flowWithWeight(unsigned, const Rational &dt,
   step(dt.value(), true);
 }
 
+std::vector<std::uint8_t> RigidBody2DSim::snapshot(const PenaltyImpactFrictionMap &ifmap) {
+  ensure(ifmap);
+  return snapshot_of(dev_, h_, hg_dem_snapshot, "hg_dem_snapshot");
+}
+
+void RigidBody2DSim::restore(const std::vector<std::uint8_t> &blob,
+                             const PenaltyImpactFrictionMap &ifmap) {
+  ensure(ifmap);
+  dev_.check(hg_dem_restore(h_, blob.data(), blob.size()), "hg_dem_restore");
+  dev_.check(hg_dem_download_bodies(h_, 0, state_.q.data(), state_.v.data()), "hg_dem_restore");
+}
+
 // ---- hybrid -------------------------------------------------------------------
 
 HybridGrains2DSim::HybridGrains2DSim(const HybridIntegratorSettings &settings,
@@ -283,6 +315,15 @@ std::vector<double> HybridGrains2DSim::homogenizedVelocity() {
 
 void HybridGrains2DSim::clearHomogenized() {
   dev_.check(hg_hybrid_clear_homogenized(h_), "hg_hybrid_clear_homogenized");
+}
+
+std::vector<std::uint8_t> HybridGrains2DSim::snapshot() {
+  return snapshot_of(dev_, h_, hg_hybrid_snapshot, "hg_hybrid_snapshot");
+}
+
+void HybridGrains2DSim::restore(const std::vector<std::uint8_t> &blob) {
+  dev_.check(hg_hybrid_restore(h_, blob.data(), blob.size()), "hg_hybrid_restore");
+  host_stale_ = true;
 }
 
 HybridGrains2DSim::~HybridGrains2DSim() {

@@ -34,12 +34,22 @@ int main(int argc, char **argv) {
   const long n = hgcli::step_count(o, dt, hs.end_time);
   std::vector<double> w(dem.state.nbodies(), 1.0);
   hg::HybridGrains2DSim sim(hs, dem, mpm, w, o.precision);
+  long it0 = 0;
+  if (!o.resume.empty()) {
+    std::vector<std::uint8_t> blob;
+    if (!hgcli::read_snapshot(o.resume, hgcli::kSnapHybrid, dt, it0, blob)) return 1;
+    sim.restore(blob);
+  }
   const auto t0 = std::chrono::steady_clock::now();
   const long chunk = o.report_every > 0 ? o.report_every : std::max(n, 1L);
-  for (long it = 0; it < n;) {
+  for (long it = it0; it < n;) {
     const long k = std::min(chunk, n - it);
     sim.stepSystem(static_cast<unsigned>(k));
     it += k;
+    if (o.serialize >= 0 &&
+        !hgcli::write_snapshot(hgcli::snapshot_path(o.serialize, it), hgcli::kSnapHybrid, it, dt,
+                               sim.snapshot()))
+      return 1;
     if (o.report_every > 0) {
       const auto c = sim.coupledGrains();
       std::printf("step %ld t %.9g coupled_grains %ld\n", it, double(it) * dt,
@@ -49,7 +59,7 @@ int main(int argc, char **argv) {
   const hg::RigidBody2DState &d = sim.discreteState();
   const hg::SimulationState &c = sim.continuumState();
   const double secs = hgcli::seconds_since(t0);
-  std::printf("Simulation complete: %ld steps in %.3f s; %zu grains, %zu points\n", n, secs,
-              d.nbodies(), c.material_points.npoints());
+  std::printf("Simulation complete: %ld steps in %.3f s; %zu grains, %zu points\n",
+              std::max(n - it0, 0L), secs, d.nbodies(), c.material_points.npoints());
   return 0;
 }

@@ -37,13 +37,23 @@ int main(int argc, char **argv) {
     return 1;
   }
   hg::setResidency(st, hg::Residency::Resident, o.precision);
-  report(st, 0, dt);
+  long it0 = 0;
+  if (!o.resume.empty()) {
+    std::vector<std::uint8_t> blob;
+    if (!hgcli::read_snapshot(o.resume, hgcli::kSnapMpm, dt, it0, blob)) return 1;
+    hg::restoreDeviceState(st, blob);
+  }
+  report(st, it0, dt);
   const auto t0 = std::chrono::steady_clock::now();
   const long chunk = o.report_every > 0 ? o.report_every : std::max(n, 1L);
-  for (long it = 0; it < n;) {
+  for (long it = it0; it < n;) {
     const long k = std::min(chunk, n - it);
     hg::ExplicitIntegrator::flow_n(dt, static_cast<unsigned>(k), st);
     it += k;
+    if (o.serialize >= 0 &&
+        !hgcli::write_snapshot(hgcli::snapshot_path(o.serialize, it), hgcli::kSnapMpm, it, dt,
+                               hg::snapshotDeviceState(st)))
+      return 1;
     if (o.report_every > 0 && it < n) {
       hg::syncToHost(st);
       report(st, it, dt);
@@ -51,8 +61,9 @@ int main(int argc, char **argv) {
   }
   hg::syncToHost(st);
   const double secs = hgcli::seconds_since(t0);
-  report(st, n, dt);
-  std::printf("Simulation complete: %ld steps in %.3f s (%.4g point-steps/s)\n", n, secs,
-              secs > 0 ? double(n) * double(st.material_points.npoints()) / secs : 0.0);
+  const long done = std::max(n - it0, 0L);
+  report(st, std::max(n, it0), dt);
+  std::printf("Simulation complete: %ld steps in %.3f s (%.4g point-steps/s)\n", done, secs,
+              secs > 0 ? double(done) * double(st.material_points.npoints()) / secs : 0.0);
   return 0;
 }

@@ -38,10 +38,22 @@ int main(int argc, char **argv) {
   hg::RigidBody2DSim sim(sc.state, sc.spatial_grid_scale, o.precision);
   hg::PenaltyImpactFrictionMap ifmap(sc.penalty);
   sim.setResidency(hg::Residency::Resident);
-  report(sim.getState(), 0, dt);
+  long it0 = 0;
+  if (!o.resume.empty()) {
+    std::vector<std::uint8_t> blob;
+    if (!hgcli::read_snapshot(o.resume, hgcli::kSnapDem, dt, it0, blob)) return 1;
+    sim.restore(blob, ifmap);
+  }
+  report(sim.getState(), it0, dt);
   const auto t0 = std::chrono::steady_clock::now();
-  for (long it = 0; it < n; ++it) {
+  const long every = o.report_every > 0 ? o.report_every : std::max(n, 1L);
+  for (long it = it0; it < n; ++it) {
     sim.flow(static_cast<unsigned>(it), sc.dt, ifmap, sc.reduce_bandwidth);
+    const bool mark = (it + 1) % every == 0 || it + 1 == n;
+    if (mark && o.serialize >= 0 &&
+        !hgcli::write_snapshot(hgcli::snapshot_path(o.serialize, it + 1), hgcli::kSnapDem,
+                               it + 1, dt, sim.snapshot(ifmap)))
+      return 1;
     if (o.report_every > 0 && (it + 1) % o.report_every == 0 && it + 1 < n) {
       sim.syncToHost();
       report(sim.getState(), it + 1, dt);
@@ -49,8 +61,9 @@ int main(int argc, char **argv) {
   }
   sim.syncToHost();
   const double secs = hgcli::seconds_since(t0);
-  report(sim.getState(), n, dt);
-  std::printf("Simulation complete: %ld steps in %.3f s (%.4g body-steps/s)\n", n, secs,
-              secs > 0 ? double(n) * double(sc.state.nbodies()) / secs : 0.0);
+  const long done = std::max(n - it0, 0L);
+  report(sim.getState(), std::max(n, it0), dt);
+  std::printf("Simulation complete: %ld steps in %.3f s (%.4g body-steps/s)\n", done, secs,
+              secs > 0 ? double(done) * double(sc.state.nbodies()) / secs : 0.0);
   return 0;
 }

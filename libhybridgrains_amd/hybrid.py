@@ -73,6 +73,14 @@ class GpuHybrid:
     def clear_homogenized(self) -> None:
         self.ctx.check(self.L.hg_hybrid_clear_homogenized(self._h))
 
+    def snapshot(self) -> bytes:
+        """Continuum, grains (with contact history) and the homogenisation
+        accumulators as a blob (hg_hybrid_snapshot)."""
+        return _lib.snapshot_blob(self.ctx, self.L.hg_hybrid_snapshot, self._h)
+
+    def restore(self, blob: bytes) -> None:
+        _lib.restore_blob(self.ctx, self.L.hg_hybrid_restore, self._h, blob)
+
     def points(self) -> dict:
         return self.continuum.points()
 

@@ -190,6 +190,14 @@ class GpuMpm:
     def set_profiling(self, on: bool) -> None:
         self.ctx.check(self.L.hg_mpm_set_profiling(self._h, int(bool(on))))
 
+    def snapshot(self) -> bytes:
+        """The evolving device state as a blob (hg_mpm_snapshot)."""
+        return _lib.snapshot_blob(self.ctx, self.L.hg_mpm_snapshot, self._h)
+
+    def restore(self, blob: bytes) -> None:
+        """Loads a snapshot of a handle made from the same scenes."""
+        _lib.restore_blob(self.ctx, self.L.hg_mpm_restore, self._h, blob)
+
     def hold(self, us: float) -> None:
         """Keeps the stream busy for about us microseconds, so that the steps
         submitted next run back to back (clean per-kernel event times)."""
