@@ -105,6 +105,34 @@ __device__ __forceinline__ uint32_t hg_wave_slot(uint32_t *counts, uint32_t key)
   return base + rank;
 }
 
+// As hg_wave_slot, for items whose equal keys mostly sit in runs of
+// neighbouring lanes (cell-sorted items): one atomicAdd per run of equal keys
+// (by the run's first lane, for the whole run) instead of one pass per
+// distinct key.  Lanes of a run get consecutive slots in lane order; a key
+// that appears in several runs takes one range per run.  Inactive lanes take
+// no part and end runs.
+__device__ __forceinline__ uint32_t hg_run_slot(uint32_t *counts, uint32_t key) {
+  const int lane = int(threadIdx.x & 63u);
+  const uint64_t active = __ballot(1);
+  const uint32_t prev = uint32_t(__shfl_up(int(key), 1, 64));
+  const bool prev_active = lane > 0 && ((active >> (lane - 1)) & 1ull);
+  const bool head = !prev_active || prev != key;
+  const uint64_t heads = __ballot(head) & active;
+  // the run's first lane: the highest head at or below this lane
+  const uint64_t upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+  const int first = 63 - __clzll((long long)(heads & upto));
+  // the next run's first lane (64 if none)
+  const uint64_t after = heads & ~upto;
+  const int next = after ? (__ffsll((unsigned long long)after) - 1) : 64;
+  const uint64_t below_first = (first == 0) ? 0ull : ((1ull << first) - 1ull);
+  const uint64_t run = (next == 64 ? ~0ull : ((1ull << next) - 1ull)) & ~below_first;
+  uint32_t base = 0;
+  if (lane == first) base = atomicAdd(&counts[key], uint32_t(__popcll(run & active)));
+  base = uint32_t(__shfl(int(base), first, 64));
+  const uint64_t before = ((lane == 0) ? 0ull : ((1ull << lane) - 1ull)) & ~below_first;
+  return base + uint32_t(__popcll(before & active));
+}
+
 // Wavefront (64-lane) inclusive prefix sum of 32-bit values.
 __device__ __forceinline__ uint32_t hg_wave_incl_scan(uint32_t v) {
   const int lane = threadIdx.x & 63;

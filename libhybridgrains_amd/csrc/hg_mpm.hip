@@ -790,7 +790,7 @@ __global__ __launch_bounds__(kG2PThreads) void mpm_g2p_kernel(G2PArgs a) {
   if (!ok || !finite) atomicOr(&a.err[sid], (ok ? 0 : 1) | (finite ? 0 : 2));
   const uint32_t k = sc.cell_off + uint32_t(nby) * uint32_t(sc.snx) + uint32_t(nbx);
   a.key[o] = k;
-  a.slot[o] = hg_wave_slot(a.counts, k);
+  a.slot[o] = hg_run_slot(a.counts, k);
 }
 
 // Initial binning of points in creation order.
