@@ -14,12 +14,20 @@
 // alternates between two counters: each step's last kernel clears the one
 // the next step uses.
 //
-// Kernels of one step: P2G with the grid update (MPM), grain_mass (also saves
-// the pre-step grain positions), the DEM contacts and integration,
-// grain_mom, exchange (also refreshes the node velocity and acceleration P2G
-// wrote at the nodes it changes), transfer, G2P with the MPM sort, finish
-// (coupled grain positions, the grains' broad-phase keys, list clean-up),
-// and the DEM sort.
+// Kernels of one step, in two branches that fork and join on events:
+//   continuum stream       | grain stream
+//   P2G + grid update      | DEM contacts, DEM integration
+//   grain_mass             |
+//   ---- join: grain_mom, exchange (also refreshes the node velocity and
+//        acceleration P2G wrote at the nodes it changes) ----
+//   G2P, MPM sort, tiles   | transfer, finish (coupled grain positions, the
+//                          | grains' broad-phase keys, next step's pre-step
+//                          | positions, list clean-up), DEM sort,
+//                          | homogenisation when on
+//   ---- join ----
+// The pre-step positions the coupling uses are saved by the previous step's
+// finish (and at creation), so the DEM integration may move the grains while
+// grain_mass still reads them.
 #include "hg_hybrid_handle.h"
 
 #include <memory>
@@ -93,15 +101,12 @@ __device__ __forceinline__ bool node_active(const HybArgs &a, int ix, int iy) {
 // HybridCoupling2D::rasterizeGrainMass (Coupling.cpp:22-129): a grain is
 // coupled when every on-grid node of its stencil (at its pre-step position)
 // carries more than 1e-6 of continuum mass; its mass is then spread with the
-// MPM weights.  Every grain's pre-step position is saved for the position
-// update at the end of the step.
+// MPM weights.
 template <typename T, int BASIS>
 __global__ __launch_bounds__(kThreads) void hyb_grain_mass(HybArgs a) {
   const uint32_t b = blockIdx.x * kThreads + threadIdx.x;
   if (b >= a.nbodies) return;
-  const double x = a.b.x[b], y = a.b.y[b];
-  a.prex[b] = x;
-  a.prey[b] = y;
+  const double x = a.prex[b], y = a.prey[b];
   a.coupled[b] = 0;
   if (a.b.fixed[b]) return;
   const hgmpm::SceneDev &sc = *a.msc;
@@ -234,8 +239,9 @@ __global__ __launch_bounds__(kThreads) void hyb_transfer(HybArgs a) {
 
 // End of the step: HybridCoupling2D::updatePositions_node_node
 // (Coupling.cpp:581-591) for the coupled grains, every grain's broad-phase
-// key and slot for the DEM sort, and the clean-up of the touched nodes and
-// of the next step's list counter.
+// key and slot for the DEM sort and its position for the next step's
+// coupling, and the clean-up of the touched nodes and of the next step's
+// list counter.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void hyb_finish(HybArgs a, double dt) {
   const uint32_t tid = blockIdx.x * kThreads + threadIdx.x;
@@ -249,6 +255,8 @@ __global__ __launch_bounds__(kThreads) void hyb_finish(HybArgs a, double dt) {
       a.b.x[b] = x;
       a.b.y[b] = y;
     }
+    a.prex[b] = x;
+    a.prey[b] = y;
     const uint32_t k = hgdem::dem_bucket(*a.dsc, x, y);
     a.dkey[b] = k;
     a.dslot[b] = atomicAdd(&a.dcounts[k], 1u);
@@ -420,18 +428,36 @@ static HybArgs make_args(hg_hybrid *h) {
   return a;
 }
 
+// concurrent: the grain branch on the second stream (eager steps); a graph
+// capture keeps one stream, since graph replays with parallel branches were
+// slower on the target than the same kernels in order.
 template <typename T, int BASIS>
-static void launch_hybrid_step(hg_hybrid *h, double dt) {
+static void launch_hybrid_step(hg_hybrid *h, double dt, bool concurrent) {
   HybArgs a = make_args(h);
   const dim3 gb((std::max(a.nbodies, 1u) + kThreads - 1) / kThreads), tb(kThreads);
   KernelProfiler &prof = h->mpm->prof;
-  hipStream_t s = h->stream;
+  hipStream_t s = h->stream, side = concurrent ? h->side : h->stream;
+  auto fork = [&]() {
+    if (side == s) return;
+    hipEventRecord(h->fork_ev, s);
+    hipStreamWaitEvent(side, h->fork_ev, 0);
+  };
+  auto join = [&]() {
+    if (side == s) return;
+    hipEventRecord(h->join_ev, side);
+    hipStreamWaitEvent(s, h->join_ev, 0);
+  };
+  // fork: the grain stream starts after everything queued so far
+  fork();
+  h->dem->stream = side;
+  hg_dem_launch_step(h->dem, false);
+  h->dem->stream = s;
   hg_mpm_launch_coupled_p2g(h->mpm, dt);
   {
     ProfScope ps(prof, "hyb_grain_mass", s);
     hipLaunchKernelGGL((hyb_grain_mass<T, BASIS>), gb, tb, 0, s, a);
   }
-  hg_dem_launch_step(h->dem, false);
+  join();
   {
     ProfScope ps(prof, "hyb_grain_mom", s);
     hipLaunchKernelGGL((hyb_grain_mom<T, BASIS>), gb, tb, 0, s, a);
@@ -442,16 +468,18 @@ static void launch_hybrid_step(hg_hybrid *h, double dt) {
     ProfScope ps(prof, "hyb_exchange", s);
     hipLaunchKernelGGL(hyb_exchange<T>, gb, tb, 0, s, a);
   }
+  fork();
   {
-    ProfScope ps(prof, "hyb_transfer", s);
-    hipLaunchKernelGGL((hyb_transfer<T, BASIS>), gb, tb, 0, s, a);
+    ProfScope ps(prof, "hyb_transfer", side);
+    hipLaunchKernelGGL((hyb_transfer<T, BASIS>), gb, tb, 0, side, a);
   }
-  hg_mpm_launch_g2p(h->mpm);
   {
-    ProfScope ps(prof, "hyb_finish", s);
-    hipLaunchKernelGGL(hyb_finish<T>, gb, tb, 0, s, a, dt);
+    ProfScope ps(prof, "hyb_finish", side);
+    hipLaunchKernelGGL(hyb_finish<T>, gb, tb, 0, side, a, dt);
   }
+  h->dem->stream = side;
   hg_dem_launch_sort(h->dem);
+  h->dem->stream = s;
   if (h->homog_stress || h->homog_vel) {
     HomogArgs g;
     g.hg = h->hg;
@@ -463,31 +491,34 @@ static void launch_hybrid_step(hg_hybrid *h, double dt) {
     g.stress = h->homog_stress ? 1 : 0;
     g.vel = h->homog_vel ? 1 : 0;
     {
-      ProfScope ps(prof, "hyb_homog_grains", s);
-      hipLaunchKernelGGL((hyb_homog_grains<T, BASIS>), gb, tb, 0, s, a, g);
+      ProfScope ps(prof, "hyb_homog_grains", side);
+      hipLaunchKernelGGL((hyb_homog_grains<T, BASIS>), gb, tb, 0, side, a, g);
     }
     if (h->homog_vel) {
-      ProfScope ps(prof, "hyb_homog_nodes", s);
-      hipLaunchKernelGGL(hyb_homog_nodes, gb, tb, 0, s, g);
+      ProfScope ps(prof, "hyb_homog_nodes", side);
+      hipLaunchKernelGGL(hyb_homog_nodes, gb, tb, 0, side, g);
     }
   }
+  hg_mpm_launch_g2p(h->mpm);
+  // join: the step ends when both branches have
+  join();
   h->par = 1 - h->par;
 }
 
-static void hybrid_step(hg_hybrid *h, double dt) {
+static void hybrid_step(hg_hybrid *h, double dt, bool concurrent) {
   const bool f64 = h->mpm->precision == HG_FP64;
   switch (h->mpm->basis) {
     case HG_BASIS_LINEAR:
-      f64 ? launch_hybrid_step<double, HG_BASIS_LINEAR>(h, dt)
-          : launch_hybrid_step<float, HG_BASIS_LINEAR>(h, dt);
+      f64 ? launch_hybrid_step<double, HG_BASIS_LINEAR>(h, dt, concurrent)
+          : launch_hybrid_step<float, HG_BASIS_LINEAR>(h, dt, concurrent);
       break;
     case HG_BASIS_CUBIC:
-      f64 ? launch_hybrid_step<double, HG_BASIS_CUBIC>(h, dt)
-          : launch_hybrid_step<float, HG_BASIS_CUBIC>(h, dt);
+      f64 ? launch_hybrid_step<double, HG_BASIS_CUBIC>(h, dt, concurrent)
+          : launch_hybrid_step<float, HG_BASIS_CUBIC>(h, dt, concurrent);
       break;
     default:
-      f64 ? launch_hybrid_step<double, HG_BASIS_UGIMP_LINEAR>(h, dt)
-          : launch_hybrid_step<float, HG_BASIS_UGIMP_LINEAR>(h, dt);
+      f64 ? launch_hybrid_step<double, HG_BASIS_UGIMP_LINEAR>(h, dt, concurrent)
+          : launch_hybrid_step<float, HG_BASIS_UGIMP_LINEAR>(h, dt, concurrent);
       break;
   }
 }
@@ -551,6 +582,15 @@ extern "C" hg_status hg_hybrid_create(hg_ctx *ctx, const hg_mpm_desc *mpm_desc,
   hipMemset(h->g.count, 0, 8);
   hipMemset(h->coupled, 0, B);
   hipMemset(h->err, 0, 4);
+  // the first step's pre-step positions (later steps get them from finish)
+  if (h->dem->nbodies) {
+    hipMemcpy(h->prex, h->dem->b.x, size_t(h->dem->nbodies) * 8, hipMemcpyDeviceToDevice);
+    hipMemcpy(h->prey, h->dem->b.y, size_t(h->dem->nbodies) * 8, hipMemcpyDeviceToDevice);
+  }
+  if (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->join_ev, hipEventDisableTiming) != hipSuccess)
+    return fail(hg_fail(ctx, HG_ERR_HIP, "hybrid streams"));
   if (hipDeviceSynchronize() != hipSuccess) return fail(hg_fail(ctx, HG_ERR_HIP, "hybrid init"));
   *out = h.release();
   return HG_OK;
@@ -559,6 +599,12 @@ extern "C" hg_status hg_hybrid_create(hg_ctx *ctx, const hg_mpm_desc *mpm_desc,
 extern "C" hg_status hg_hybrid_destroy(hg_hybrid *h) {
   if (!h) return HG_OK;
   if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->side) {
+    hipStreamSynchronize(h->side);
+    hipStreamDestroy(h->side);
+  }
+  if (h->fork_ev) hipEventDestroy(h->fork_ev);
+  if (h->join_ev) hipEventDestroy(h->join_ev);
   if (h->graph) hipGraphExecDestroy(h->graph);
   for (void *p : h->allocs) hipFree(p);
   if (h->dem) hg_dem_destroy(h->dem);
@@ -573,7 +619,7 @@ extern "C" hg_status hg_hybrid_step(hg_hybrid *h, double dt, uint32_t nsteps) {
   if (h->mpm->phase_next != 0) return hg_fail(h->ctx, HG_ERR_INVALID, "a phased MPM step is in progress");
   // the DEM reads its dt from its scene table
   HG_TRY(hg_dem_step(h->dem, dt, 0));
-  for (uint32_t k = 0; k < nsteps; ++k) hybrid_step(h, dt);
+  for (uint32_t k = 0; k < nsteps; ++k) hybrid_step(h, dt, true);
   HG_HIP(h->ctx, hipGetLastError());
   return HG_OK;
 }
@@ -596,13 +642,13 @@ extern "C" hg_status hg_hybrid_step_graph(hg_hybrid *h, double dt, uint32_t nste
   // handle alone in between breaks that alignment; the steps then run
   // without the graph.)
   if ((h->mpm->cur != 0 || h->par != 0) && nsteps) {
-    hybrid_step(h, dt);
+    hybrid_step(h, dt, true);
     --nsteps;
   }
   const bool aligned = h->mpm->cur == 0 && h->par == 0;
   const uint32_t pairs = aligned ? nsteps / 2 : 0;
   if (!aligned) {
-    for (; nsteps; --nsteps) hybrid_step(h, dt);
+    for (; nsteps; --nsteps) hybrid_step(h, dt, true);
   }
   if (pairs) {
     if (!h->graph) {
@@ -610,8 +656,8 @@ extern "C" hg_status hg_hybrid_step_graph(hg_hybrid *h, double dt, uint32_t nste
       HG_TRY(hg_mpm_step(h->mpm, dt, 0));
       hipGraph_t g = nullptr;
       HG_HIP(h->ctx, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-      hybrid_step(h, dt);
-      hybrid_step(h, dt);
+      hybrid_step(h, dt, false);
+      hybrid_step(h, dt, false);
       HG_HIP(h->ctx, hipStreamEndCapture(h->stream, &g));
       HG_HIP(h->ctx, hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0));
       HG_HIP(h->ctx, hipGraphDestroy(g));
@@ -623,7 +669,7 @@ extern "C" hg_status hg_hybrid_step_graph(hg_hybrid *h, double dt, uint32_t nste
     }
     for (uint32_t k = 0; k < pairs; ++k) HG_HIP(h->ctx, hipGraphLaunch(h->graph, h->stream));
   }
-  if (nsteps & 1u) hybrid_step(h, dt);
+  if (nsteps & 1u) hybrid_step(h, dt, true);
   HG_HIP(h->ctx, hipGetLastError());
   return HG_OK;
 }

@@ -37,6 +37,11 @@ struct hg_hybrid {
   hg_mpm *mpm = nullptr;
   hg_dem *dem = nullptr;
   hipStream_t stream = nullptr;
+  // The grain side runs on a second stream beside the continuum: the DEM
+  // contacts and integration beside P2G, the grain transfer, positions,
+  // broad-phase keys and DEM sort beside G2P (fork and join by events).
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   hghyb::GrainGrid g = {};
   double *prex = nullptr, *prey = nullptr;
   uint8_t *coupled = nullptr;

@@ -257,6 +257,13 @@ extern "C" hg_status hg_hybrid_restore(hg_hybrid *h, const void *buf, uint64_t b
   if (bd > uint64_t(end - p)) return hg_fail(h->ctx, HG_ERR_INVALID, "corrupt hybrid snapshot");
   HG_TRY(hg_dem_restore(h->dem, p, bd));
   p += bd;
+  // the next step's pre-step grain positions
+  if (h->dem->nbodies) {
+    HG_HIP(h->ctx, hipMemcpy(h->prex, h->dem->b.x, size_t(h->dem->nbodies) * 8,
+                             hipMemcpyDeviceToDevice));
+    HG_HIP(h->ctx, hipMemcpy(h->prey, h->dem->b.y, size_t(h->dem->nbodies) * 8,
+                             hipMemcpyDeviceToDevice));
+  }
   if (uint64_t(end - p) < 4) return hg_fail(h->ctx, HG_ERR_INVALID, "corrupt hybrid snapshot");
   uint32_t flags = 0;
   std::memcpy(&flags, p, 4);
