@@ -14,20 +14,21 @@
 // alternates between two counters: each step's last kernel clears the one
 // the next step uses.
 //
-// Kernels of one step, in two branches that fork and join on events:
+// Kernels of one step:
 //   continuum stream       | grain stream
 //   P2G + grid update      | DEM contacts, DEM integration
 //   grain_mass             |
-//   ---- join: grain_mom, exchange (also refreshes the node velocity and
-//        acceleration P2G wrote at the nodes it changes) ----
+//   grain_mom, exchange (also refreshes the node velocity and acceleration
+//   P2G wrote at the nodes it changes)
 //   G2P, MPM sort, tiles   | transfer, finish (coupled grain positions, the
-//                          | grains' broad-phase keys, next step's pre-step
-//                          | positions, list clean-up), DEM sort,
-//                          | homogenisation when on
-//   ---- join ----
-// The pre-step positions the coupling uses are saved by the previous step's
-// finish (and at creation), so the DEM integration may move the grains while
-// grain_mass still reads them.
+//                          | grains' broad-phase keys, the next step's
+//                          | pre-step positions, list clean-up), DEM sort,
+//                          | homogenisation when on, then the next step's
+//                          | DEM contacts and integration
+// with events where one stream needs the other's results.  The pre-step
+// positions the coupling uses are saved by the previous step's finish (and
+// at creation), so the DEM integration may move the grains while grain_mass
+// still reads them.  A graph capture queues the same kernels on one stream.
 #include "hg_hybrid_handle.h"
 
 #include <memory>
@@ -428,97 +429,108 @@ static HybArgs make_args(hg_hybrid *h) {
   return a;
 }
 
-// concurrent: the grain branch on the second stream (eager steps); a graph
-// capture keeps one stream, since graph replays with parallel branches were
-// slower on the target than the same kernels in order.
+// n steps.  concurrent: the grain branch runs on the second stream (eager
+// steps), and each step's DEM contacts and integration are queued on it as
+// soon as the previous step's grain work is queued, so they overlap the
+// continuum's G2P and sort; a graph capture keeps one stream, since graph
+// replays with parallel branches were slower on the target than the same
+// kernels in order.
 template <typename T, int BASIS>
-static void launch_hybrid_step(hg_hybrid *h, double dt, bool concurrent) {
-  HybArgs a = make_args(h);
-  const dim3 gb((std::max(a.nbodies, 1u) + kThreads - 1) / kThreads), tb(kThreads);
+static void launch_hybrid_steps(hg_hybrid *h, double dt, uint32_t n, bool concurrent) {
+  if (n == 0) return;
   KernelProfiler &prof = h->mpm->prof;
-  hipStream_t s = h->stream, side = concurrent ? h->side : h->stream;
-  auto fork = [&]() {
-    if (side == s) return;
-    hipEventRecord(h->fork_ev, s);
-    hipStreamWaitEvent(side, h->fork_ev, 0);
+  const hipStream_t s = h->stream, side = concurrent ? h->side : h->stream;
+  const uint32_t nb = std::max(h->dem->nbodies, 1u);
+  const dim3 gb((nb + kThreads - 1) / kThreads), tb(kThreads);
+  // work queued on stream from before this point precedes what stream to
+  // queues after it
+  auto order = [&](hipEvent_t ev, hipStream_t from, hipStream_t to) {
+    if (from == to) return;
+    hipEventRecord(ev, from);
+    hipStreamWaitEvent(to, ev, 0);
   };
-  auto join = [&]() {
-    if (side == s) return;
-    hipEventRecord(h->join_ev, side);
-    hipStreamWaitEvent(s, h->join_ev, 0);
+  auto dem_step = [&]() {
+    h->dem->stream = side;
+    hg_dem_launch_step(h->dem, false);
+    h->dem->stream = s;
   };
-  // fork: the grain stream starts after everything queued so far
-  fork();
-  h->dem->stream = side;
-  hg_dem_launch_step(h->dem, false);
-  h->dem->stream = s;
-  hg_mpm_launch_coupled_p2g(h->mpm, dt);
-  {
-    ProfScope ps(prof, "hyb_grain_mass", s);
-    hipLaunchKernelGGL((hyb_grain_mass<T, BASIS>), gb, tb, 0, s, a);
-  }
-  join();
-  {
-    ProfScope ps(prof, "hyb_grain_mom", s);
-    hipLaunchKernelGGL((hyb_grain_mom<T, BASIS>), gb, tb, 0, s, a);
-  }
-  {
-    // the list holds at most S*S nodes per grain: a grid over the grains
-    // walks it in strides
-    ProfScope ps(prof, "hyb_exchange", s);
-    hipLaunchKernelGGL(hyb_exchange<T>, gb, tb, 0, s, a);
-  }
-  fork();
-  {
-    ProfScope ps(prof, "hyb_transfer", side);
-    hipLaunchKernelGGL((hyb_transfer<T, BASIS>), gb, tb, 0, side, a);
-  }
-  {
-    ProfScope ps(prof, "hyb_finish", side);
-    hipLaunchKernelGGL(hyb_finish<T>, gb, tb, 0, side, a, dt);
-  }
-  h->dem->stream = side;
-  hg_dem_launch_sort(h->dem);
-  h->dem->stream = s;
-  if (h->homog_stress || h->homog_vel) {
-    HomogArgs g;
-    g.hg = h->hg;
-    g.s = h->dem->slots[h->dem->cur];
-    g.oF = h->dem->f.oF;
-    g.cpoint = h->dem->cpoint;
-    g.count = h->hg.count + h->par;
-    g.next_count = h->hg.count + (1 - h->par);
-    g.stress = h->homog_stress ? 1 : 0;
-    g.vel = h->homog_vel ? 1 : 0;
+  order(h->fork_ev, s, side);
+  dem_step();
+  for (uint32_t k = 0; k < n; ++k) {
+    HybArgs a = make_args(h);
+    hg_mpm_launch_coupled_p2g(h->mpm, dt);
+    // grain_mass needs the previous step's finish: pre-step positions,
+    // cleared node lists
+    if (k > 0 && side != s) hipStreamWaitEvent(s, h->fin_ev, 0);
     {
-      ProfScope ps(prof, "hyb_homog_grains", side);
-      hipLaunchKernelGGL((hyb_homog_grains<T, BASIS>), gb, tb, 0, side, a, g);
+      ProfScope ps(prof, "hyb_grain_mass", s);
+      hipLaunchKernelGGL((hyb_grain_mass<T, BASIS>), gb, tb, 0, s, a);
     }
-    if (h->homog_vel) {
-      ProfScope ps(prof, "hyb_homog_nodes", side);
-      hipLaunchKernelGGL(hyb_homog_nodes, gb, tb, 0, side, g);
+    order(h->join_ev, side, s);  // this step's DEM step
+    {
+      ProfScope ps(prof, "hyb_grain_mom", s);
+      hipLaunchKernelGGL((hyb_grain_mom<T, BASIS>), gb, tb, 0, s, a);
     }
+    {
+      // the list holds at most S*S nodes per grain: a grid over the grains
+      // walks it in strides
+      ProfScope ps(prof, "hyb_exchange", s);
+      hipLaunchKernelGGL(hyb_exchange<T>, gb, tb, 0, s, a);
+    }
+    order(h->fork_ev, s, side);
+    {
+      ProfScope ps(prof, "hyb_transfer", side);
+      hipLaunchKernelGGL((hyb_transfer<T, BASIS>), gb, tb, 0, side, a);
+    }
+    {
+      ProfScope ps(prof, "hyb_finish", side);
+      hipLaunchKernelGGL(hyb_finish<T>, gb, tb, 0, side, a, dt);
+    }
+    if (side != s) hipEventRecord(h->fin_ev, side);
+    h->dem->stream = side;
+    hg_dem_launch_sort(h->dem);
+    h->dem->stream = s;
+    if (h->homog_stress || h->homog_vel) {
+      HomogArgs g;
+      g.hg = h->hg;
+      g.s = h->dem->slots[h->dem->cur];
+      g.oF = h->dem->f.oF;
+      g.cpoint = h->dem->cpoint;
+      g.count = h->hg.count + h->par;
+      g.next_count = h->hg.count + (1 - h->par);
+      g.stress = h->homog_stress ? 1 : 0;
+      g.vel = h->homog_vel ? 1 : 0;
+      {
+        ProfScope ps(prof, "hyb_homog_grains", side);
+        hipLaunchKernelGGL((hyb_homog_grains<T, BASIS>), gb, tb, 0, side, a, g);
+      }
+      if (h->homog_vel) {
+        ProfScope ps(prof, "hyb_homog_nodes", side);
+        hipLaunchKernelGGL(hyb_homog_nodes, gb, tb, 0, side, g);
+      }
+    }
+    if (k + 1 < n) dem_step();  // the next step's DEM step, ahead
+    hg_mpm_launch_g2p(h->mpm);
+    h->par = 1 - h->par;
   }
-  hg_mpm_launch_g2p(h->mpm);
-  // join: the step ends when both branches have
-  join();
-  h->par = 1 - h->par;
+  // the call ends when both streams have
+  order(h->join_ev, side, s);
 }
 
-static void hybrid_step(hg_hybrid *h, double dt, bool concurrent) {
+static void hybrid_steps(hg_hybrid *h, double dt, uint32_t n, bool concurrent) {
   const bool f64 = h->mpm->precision == HG_FP64;
   switch (h->mpm->basis) {
     case HG_BASIS_LINEAR:
-      f64 ? launch_hybrid_step<double, HG_BASIS_LINEAR>(h, dt, concurrent)
-          : launch_hybrid_step<float, HG_BASIS_LINEAR>(h, dt, concurrent);
+      f64 ? launch_hybrid_steps<double, HG_BASIS_LINEAR>(h, dt, n, concurrent)
+          : launch_hybrid_steps<float, HG_BASIS_LINEAR>(h, dt, n, concurrent);
       break;
     case HG_BASIS_CUBIC:
-      f64 ? launch_hybrid_step<double, HG_BASIS_CUBIC>(h, dt, concurrent)
-          : launch_hybrid_step<float, HG_BASIS_CUBIC>(h, dt, concurrent);
+      f64 ? launch_hybrid_steps<double, HG_BASIS_CUBIC>(h, dt, n, concurrent)
+          : launch_hybrid_steps<float, HG_BASIS_CUBIC>(h, dt, n, concurrent);
       break;
     default:
-      f64 ? launch_hybrid_step<double, HG_BASIS_UGIMP_LINEAR>(h, dt, concurrent)
-          : launch_hybrid_step<float, HG_BASIS_UGIMP_LINEAR>(h, dt, concurrent);
+      f64 ? launch_hybrid_steps<double, HG_BASIS_UGIMP_LINEAR>(h, dt, n, concurrent)
+          : launch_hybrid_steps<float, HG_BASIS_UGIMP_LINEAR>(h, dt, n, concurrent);
       break;
   }
 }
@@ -589,7 +601,8 @@ extern "C" hg_status hg_hybrid_create(hg_ctx *ctx, const hg_mpm_desc *mpm_desc,
   }
   if (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&h->join_ev, hipEventDisableTiming) != hipSuccess)
+      hipEventCreateWithFlags(&h->join_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->fin_ev, hipEventDisableTiming) != hipSuccess)
     return fail(hg_fail(ctx, HG_ERR_HIP, "hybrid streams"));
   if (hipDeviceSynchronize() != hipSuccess) return fail(hg_fail(ctx, HG_ERR_HIP, "hybrid init"));
   *out = h.release();
@@ -605,6 +618,7 @@ extern "C" hg_status hg_hybrid_destroy(hg_hybrid *h) {
   }
   if (h->fork_ev) hipEventDestroy(h->fork_ev);
   if (h->join_ev) hipEventDestroy(h->join_ev);
+  if (h->fin_ev) hipEventDestroy(h->fin_ev);
   if (h->graph) hipGraphExecDestroy(h->graph);
   for (void *p : h->allocs) hipFree(p);
   if (h->dem) hg_dem_destroy(h->dem);
@@ -619,7 +633,7 @@ extern "C" hg_status hg_hybrid_step(hg_hybrid *h, double dt, uint32_t nsteps) {
   if (h->mpm->phase_next != 0) return hg_fail(h->ctx, HG_ERR_INVALID, "a phased MPM step is in progress");
   // the DEM reads its dt from its scene table
   HG_TRY(hg_dem_step(h->dem, dt, 0));
-  for (uint32_t k = 0; k < nsteps; ++k) hybrid_step(h, dt, true);
+  hybrid_steps(h, dt, nsteps, true);
   HG_HIP(h->ctx, hipGetLastError());
   return HG_OK;
 }
@@ -642,13 +656,14 @@ extern "C" hg_status hg_hybrid_step_graph(hg_hybrid *h, double dt, uint32_t nste
   // handle alone in between breaks that alignment; the steps then run
   // without the graph.)
   if ((h->mpm->cur != 0 || h->par != 0) && nsteps) {
-    hybrid_step(h, dt, true);
+    hybrid_steps(h, dt, 1, true);
     --nsteps;
   }
   const bool aligned = h->mpm->cur == 0 && h->par == 0;
   const uint32_t pairs = aligned ? nsteps / 2 : 0;
   if (!aligned) {
-    for (; nsteps; --nsteps) hybrid_step(h, dt, true);
+    hybrid_steps(h, dt, nsteps, true);
+    nsteps = 0;
   }
   if (pairs) {
     if (!h->graph) {
@@ -656,8 +671,8 @@ extern "C" hg_status hg_hybrid_step_graph(hg_hybrid *h, double dt, uint32_t nste
       HG_TRY(hg_mpm_step(h->mpm, dt, 0));
       hipGraph_t g = nullptr;
       HG_HIP(h->ctx, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-      hybrid_step(h, dt, false);
-      hybrid_step(h, dt, false);
+      hybrid_steps(h, dt, 1, false);
+      hybrid_steps(h, dt, 1, false);
       HG_HIP(h->ctx, hipStreamEndCapture(h->stream, &g));
       HG_HIP(h->ctx, hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0));
       HG_HIP(h->ctx, hipGraphDestroy(g));
@@ -669,7 +684,7 @@ extern "C" hg_status hg_hybrid_step_graph(hg_hybrid *h, double dt, uint32_t nste
     }
     for (uint32_t k = 0; k < pairs; ++k) HG_HIP(h->ctx, hipGraphLaunch(h->graph, h->stream));
   }
-  if (nsteps & 1u) hybrid_step(h, dt, true);
+  if (nsteps & 1u) hybrid_steps(h, dt, 1, true);
   HG_HIP(h->ctx, hipGetLastError());
   return HG_OK;
 }

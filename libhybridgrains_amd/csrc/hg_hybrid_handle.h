@@ -37,11 +37,12 @@ struct hg_hybrid {
   hg_mpm *mpm = nullptr;
   hg_dem *dem = nullptr;
   hipStream_t stream = nullptr;
-  // The grain side runs on a second stream beside the continuum: the DEM
-  // contacts and integration beside P2G, the grain transfer, positions,
-  // broad-phase keys and DEM sort beside G2P (fork and join by events).
+  // The grain side runs on a second stream beside the continuum (eager
+  // steps): the grain transfer, positions, broad-phase keys, DEM sort and the
+  // next step's DEM contacts and integration beside G2P, the MPM sort and
+  // P2G; events order the two.
   hipStream_t side = nullptr;
-  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr, fin_ev = nullptr;
   hghyb::GrainGrid g = {};
   double *prex = nullptr, *prey = nullptr;
   uint8_t *coupled = nullptr;
