@@ -7,8 +7,10 @@ Headline (value): the 1M-point hybrid scene -- the MPM column of 1,000,000
 points on 2000 x 1000 cells (h = 1 mm, 4 points per cell) with a 0.1 m deep
 discrete zone of about 48k circles at its free surface -- stepped with the
 node-node coupled step (MPM + weighted DEM + exchange), FP32 bulk with FP64
-positions and return map, every step a hipGraph replay.  value =
-(points + circles) x steps / wall time, inputs resident in HBM.
+positions and return map; the grain branch runs on its own stream beside the
+continuum (the engine's hg_hybrid_step).  value = (points + circles) x steps
+/ wall time, inputs resident in HBM.  The same steps as hipGraph replays (one
+stream) are reported as graph_ms_per_step.
 
 Beside it: the same scene on the plain FP64 GPU path (fp64_plain), the
 batched FP32 form (64 independent MPM scenes, 5k-500k points each, in one
@@ -50,16 +52,21 @@ DT_1M = 5.0e-6
 
 
 def _time_hybrid(m, d, w, precision, steps, warmup, dt):
+    """Eager steps (the grain branch on its own stream, overlapping the
+    continuum) and hipGraph steps (one stream); returns both walls."""
     from libhybridgrains_amd import GpuHybrid
 
     h = GpuHybrid(m, d, w, precision)
     tune = h.continuum.autotune(dt)
-    h.step(dt, warmup, graph=True)
-    h.sync()
-    t0 = time.perf_counter()
-    h.step(dt, steps, graph=True)
-    h.sync()
-    wall = time.perf_counter() - t0
+    walls = {}
+    for graph in (True, False):
+        h.step(dt, warmup, graph=graph)
+        h.sync()
+        t0 = time.perf_counter()
+        h.step(dt, steps, graph=graph)
+        h.sync()
+        walls["graph" if graph else "eager"] = time.perf_counter() - t0
+    wall = walls["eager"]
     # per-kernel durations over the same number of steps (events on the
     # stream); the steps queue up behind a hold so that they run back to back
     h.continuum.set_profiling(True)
@@ -71,6 +78,7 @@ def _time_hybrid(m, d, w, precision, steps, warmup, dt):
     kt.update({"dem:" + k: v for k, v in h.discrete.kernel_times().items()})
     h.close()
     kt["_tune"] = tune
+    kt["_graph_wall"] = walls["graph"]
     return wall, kt
 
 
@@ -338,12 +346,14 @@ def main() -> int:
         print(f"[bench] hybrid 1M {name}", file=sys.stderr, flush=True)
         wall, kt = _time_hybrid(m, d, w, prec, steps, args.warmup, DT_1M)
         tune = kt.pop("_tune")
+        graph_wall = kt.pop("_graph_wall")
         transfer_ms = sum(kt[k][0] for k in ("mpm_p2g", "mpm_g2p")) / steps
         alg = ALG_BYTES[name] * N
         achieved = alg / (transfer_ms * 1e-3) / 1e9
         legs[name] = {
             "value": (N + B) * steps / wall,
             "ms_per_step": 1e3 * wall / steps,
+            "graph_ms_per_step": 1e3 * graph_wall / steps,
             "kernels_ms": {k: v[0] / v[1] for k, v in kt.items()},
             "autotune": tune,
             "roofline": {
@@ -394,7 +404,8 @@ def main() -> int:
         "config": {
             "workload": "hybrid_column_1M: 1,000,000 MPM points (2000x1000 cells, h=1mm, 4 ppc, "
                         "0.5x0.5 m column) + 47,700 circles (r=0.5mm+-10%, 0.1 m discrete zone), "
-                        "node-node coupled step, dt=5e-6, hipGraph step",
+                        "node-node coupled step, dt=5e-6, grain branch on a second stream",
+            "graph_ms_per_step": f32["graph_ms_per_step"],
             "points": N,
             "circles": B,
             "precision": "fp32 bulk, fp64 positions and return map",
@@ -402,6 +413,7 @@ def main() -> int:
         "roofline": f32["roofline"],
         "cpu_baseline": cpu,
         "fp64_plain": {"value": f64["value"], "ms_per_step": f64["ms_per_step"],
+                       "graph_ms_per_step": f64["graph_ms_per_step"],
                        "roofline": f64["roofline"], "autotune": f64["autotune"]},
         "autotune": f32["autotune"],
         "fp32_over_fp64": f32["value"] / f64["value"],
