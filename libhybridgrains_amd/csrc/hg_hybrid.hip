@@ -93,6 +93,52 @@ __device__ __forceinline__ double grain_weight(const hgmpm::SceneDev &sc, double
   return ax * ay;
 }
 
+// A grain's stencil with its 1-D weights: node (lx + i, ly + j) of the S x S
+// box has weight wx[i] * wy[j] (the x factor carries uGIMP's 1/(4 hl^2), as
+// grain_weight applies it) when bit j*S + i of `on` is set (inside the
+// stencil and on the grid).
+template <int BASIS>
+struct GrainStencil {
+  static constexpr int S = hgmpm::BasisTraits<BASIS>::S;
+  int lx, ly;
+  uint32_t on;
+  double wx[S], wy[S];
+};
+
+template <int BASIS>
+__device__ __forceinline__ bool grain_stencil_weights(const hgmpm::SceneDev &sc, double x,
+                                                      double y, double hl,
+                                                      GrainStencil<BASIS> &g) {
+  constexpr int S = GrainStencil<BASIS>::S;
+  int lx, hx, ly, hy;
+  g.on = 0u;
+  if (!grain_stencil<BASIS>(sc, x, y, hl, lx, hx, ly, hy)) return false;
+  g.lx = lx;
+  g.ly = ly;
+  const double vnorm = (BASIS == HG_BASIS_UGIMP_LINEAR) ? 1.0 / (4.0 * hl * hl) : 1.0;
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    double ax = 0.0, ay = 0.0, d;
+    if (lx + i <= hx)
+      hgmpm::basis_factor<double, BASIS>(x, hl, sc.min0, sc.h, sc.inv_h, lx + i, i, hx - lx + 1,
+                                         ax, d);
+    if (ly + i <= hy)
+      hgmpm::basis_factor<double, BASIS>(y, hl, sc.min1, sc.h, sc.inv_h, ly + i, i, hy - ly + 1,
+                                         ay, d);
+    g.wx[i] = (BASIS == HG_BASIS_UGIMP_LINEAR) ? ax * vnorm : ax;
+    g.wy[i] = ay;
+  }
+#pragma unroll
+  for (int j = 0; j < S; ++j)
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      const int ix = lx + i, iy = ly + j;
+      if (ix <= hx && iy <= hy && ix >= 0 && iy >= 0 && ix < sc.nnx && iy < sc.nny)
+        g.on |= 1u << (j * S + i);
+    }
+  return true;
+}
+
 __device__ __forceinline__ bool node_active(const HybArgs &a, int ix, int iy) {
   const hgmpm::SceneDev &sc = *a.msc;
   const uint32_t tile = sc.tile_off + uint32_t(iy / a.th) * uint32_t(sc.tiles_x) + uint32_t(ix / a.tw);
@@ -105,49 +151,62 @@ __device__ __forceinline__ bool node_active(const HybArgs &a, int ix, int iy) {
 // MPM weights.
 template <typename T, int BASIS>
 __global__ __launch_bounds__(kThreads) void hyb_grain_mass(HybArgs a) {
+  constexpr int S = GrainStencil<BASIS>::S;
   const uint32_t b = blockIdx.x * kThreads + threadIdx.x;
-  if (b >= a.nbodies) return;
-  const double x = a.prex[b], y = a.prey[b];
-  a.coupled[b] = 0;
-  if (a.b.fixed[b]) return;
-  const hgmpm::SceneDev &sc = *a.msc;
-  int lx, hx, ly, hy;
-  if (!grain_stencil<BASIS>(sc, x, y, a.hl, lx, hx, ly, hy)) return;
-  bool all_pos = true;
-  for (int iy = ly; iy <= hy; ++iy)
-    for (int ix = lx; ix <= hx; ++ix) {
-      if (ix < 0 || iy < 0 || ix >= sc.nnx || iy >= sc.nny) continue;
-      const uint32_t node = sc.node_off + uint32_t(iy) * uint32_t(sc.snx) + uint32_t(ix);
-      const double m = node_active(a, ix, iy) ? a.gm[node] : 0.0;
-      if (m <= 1.0e-6) all_pos = false;
-    }
-  if (!all_pos) return;
-  a.coupled[b] = 1;
-  const double mb = double(ld<T>(a.b.m, b));
-  for (int iy = ly; iy <= hy; ++iy)
-    for (int ix = lx; ix <= hx; ++ix) {
-      if (ix < 0 || iy < 0 || ix >= sc.nnx || iy >= sc.nny) continue;
-      const uint32_t n = uint32_t(iy) * uint32_t(sc.snx) + uint32_t(ix);
-      const double w = grain_weight<BASIS>(sc, x, y, a.hl, ix, iy, lx, hx, ly, hy);
-      atomicAdd(&a.g.dmass[n], w * mb);
-      // the first grain to stamp a node appends it to the list; the lanes of
-      // a wave that do so take consecutive entries with one counter update
-      bool fresh = false;
-      if (a.g.stamp[n] == 0u) fresh = atomicExch(&a.g.stamp[n], 1u) == 0u;
-      const uint64_t mask = __ballot(fresh);
-      if (mask) {
-        const int lane = int(threadIdx.x & 63u);
-        const int leader = __ffsll((unsigned long long)mask) - 1;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(a.count, uint32_t(__popcll(mask)));
-        base = __shfl(base, leader, 64);
-        if (fresh) {
-          const uint32_t k = base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
-          if (k < a.g.max_list) a.g.list[k] = n;
-          else atomicOr(a.err, 8);
+  const int lane = int(threadIdx.x & 63u);
+  // every lane of a wave reaches the list append below
+  uint32_t fresh = 0u;  // stencil nodes this grain stamped first
+  GrainStencil<BASIS> g;
+  g.on = 0u;
+  if (b < a.nbodies) {
+    const double x = a.prex[b], y = a.prey[b];
+    a.coupled[b] = 0;
+    const hgmpm::SceneDev &sc = *a.msc;
+    if (!a.b.fixed[b] && grain_stencil_weights<BASIS>(sc, x, y, a.hl, g)) {
+      bool all_pos = true;
+#pragma unroll
+      for (int q = 0; q < S * S; ++q) {
+        if (!((g.on >> q) & 1u)) continue;
+        const int ix = g.lx + q % S, iy = g.ly + q / S;
+        const uint32_t node = sc.node_off + uint32_t(iy) * uint32_t(sc.snx) + uint32_t(ix);
+        const double m = node_active(a, ix, iy) ? a.gm[node] : 0.0;
+        if (m <= 1.0e-6) all_pos = false;
+      }
+      if (all_pos) {
+        a.coupled[b] = 1;
+        const double mb = double(ld<T>(a.b.m, b));
+        // mass spread and stamps, all nodes at once
+#pragma unroll
+        for (int q = 0; q < S * S; ++q) {
+          if (!((g.on >> q) & 1u)) continue;
+          const uint32_t n = uint32_t(g.ly + q / S) * uint32_t(sc.snx) + uint32_t(g.lx + q % S);
+          const double w = g.wx[q % S] * g.wy[q / S];
+          atomicAdd(&a.g.dmass[n], w * mb);
+          if (atomicExch(&a.g.stamp[n], 1u) == 0u) fresh |= 1u << q;
         }
       }
     }
+  }
+  // the first grain to stamp a node appends it to the list: one counter
+  // update per wave, consecutive entries per lane
+  const uint32_t mine = uint32_t(__popc(fresh));
+  const uint32_t incl = hg_wave_incl_scan(mine);
+  const uint32_t total = uint32_t(__shfl(int(incl), 63, 64));
+  if (total == 0u) return;
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(a.count, total);
+  base = uint32_t(__shfl(int(base), 0, 64)) + (incl - mine);
+  if (fresh) {
+    const hgmpm::SceneDev &sc = *a.msc;
+#pragma unroll
+    for (int q = 0; q < S * S; ++q) {
+      if (!((fresh >> q) & 1u)) continue;
+      const uint32_t n = uint32_t(g.ly + q / S) * uint32_t(sc.snx) + uint32_t(g.lx + q % S);
+      if (base < a.g.max_list) a.g.list[base] = n;
+      else atomicOr(a.err, 8);
+      ++base;
+    }
+  }
 }
 
 // HybridCoupling2D::rasterizeGrainMom (Coupling.cpp:287-466): post-step grain
@@ -156,20 +215,20 @@ template <typename T, int BASIS>
 __global__ __launch_bounds__(kThreads) void hyb_grain_mom(HybArgs a) {
   const uint32_t b = blockIdx.x * kThreads + threadIdx.x;
   if (b >= a.nbodies || !a.coupled[b]) return;
+  constexpr int S = GrainStencil<BASIS>::S;
   const hgmpm::SceneDev &sc = *a.msc;
-  const double x = a.prex[b], y = a.prey[b];
-  int lx, hx, ly, hy;
-  grain_stencil<BASIS>(sc, x, y, a.hl, lx, hx, ly, hy);
+  GrainStencil<BASIS> g;
+  grain_stencil_weights<BASIS>(sc, a.prex[b], a.prey[b], a.hl, g);
   const double mb = double(ld<T>(a.b.m, b));
   const double pm0 = mb * double(ld<T>(a.b.vx, b)), pm1 = mb * double(ld<T>(a.b.vy, b));
-  for (int iy = ly; iy <= hy; ++iy)
-    for (int ix = lx; ix <= hx; ++ix) {
-      if (ix < 0 || iy < 0 || ix >= sc.nnx || iy >= sc.nny) continue;
-      const uint32_t n = uint32_t(iy) * uint32_t(sc.snx) + uint32_t(ix);
-      const double w = grain_weight<BASIS>(sc, x, y, a.hl, ix, iy, lx, hx, ly, hy);
-      atomicAdd(&a.g.dmom[2 * n], w * pm0);
-      atomicAdd(&a.g.dmom[2 * n + 1], w * pm1);
-    }
+#pragma unroll
+  for (int q = 0; q < S * S; ++q) {
+    if (!((g.on >> q) & 1u)) continue;
+    const uint32_t n = uint32_t(g.ly + q / S) * uint32_t(sc.snx) + uint32_t(g.lx + q % S);
+    const double w = g.wx[q % S] * g.wy[q / S];
+    atomicAdd(&a.g.dmom[2 * n], w * pm0);
+    atomicAdd(&a.g.dmom[2 * n + 1], w * pm1);
+  }
 }
 
 // HybridCoupling2D::updateVelocitiesOnGrid (Coupling.cpp:533-579): the
@@ -218,21 +277,21 @@ template <typename T, int BASIS>
 __global__ __launch_bounds__(kThreads) void hyb_transfer(HybArgs a) {
   const uint32_t b = blockIdx.x * kThreads + threadIdx.x;
   if (b >= a.nbodies || !a.coupled[b]) return;
+  constexpr int S = GrainStencil<BASIS>::S;
   const hgmpm::SceneDev &sc = *a.msc;
-  const double x = a.prex[b], y = a.prey[b];
-  int lx, hx, ly, hy;
-  grain_stencil<BASIS>(sc, x, y, a.hl, lx, hx, ly, hy);
+  GrainStencil<BASIS> g;
+  grain_stencil_weights<BASIS>(sc, a.prex[b], a.prey[b], a.hl, g);
   double vp0 = 0.0, vp1 = 0.0, a0 = 0.0, a1 = 0.0;
-  for (int iy = ly; iy <= hy; ++iy)
-    for (int ix = lx; ix <= hx; ++ix) {
-      if (ix < 0 || iy < 0 || ix >= sc.nnx || iy >= sc.nny) continue;
-      const uint32_t n = uint32_t(iy) * uint32_t(sc.snx) + uint32_t(ix);
-      const double w = grain_weight<BASIS>(sc, x, y, a.hl, ix, iy, lx, hx, ly, hy);
-      vp0 += w * a.g.vafter[2 * n];
-      vp1 += w * a.g.vafter[2 * n + 1];
-      a0 += w * a.g.aafter[2 * n];
-      a1 += w * a.g.aafter[2 * n + 1];
-    }
+#pragma unroll
+  for (int q = 0; q < S * S; ++q) {
+    if (!((g.on >> q) & 1u)) continue;
+    const uint32_t n = uint32_t(g.ly + q / S) * uint32_t(sc.snx) + uint32_t(g.lx + q % S);
+    const double w = g.wx[q % S] * g.wy[q / S];
+    vp0 += w * a.g.vafter[2 * n];
+    vp1 += w * a.g.vafter[2 * n + 1];
+    a0 += w * a.g.aafter[2 * n];
+    a1 += w * a.g.aafter[2 * n + 1];
+  }
   const double vf0 = double(ld<T>(a.b.vx, b)) + a0, vf1 = double(ld<T>(a.b.vy, b)) + a1;
   static_cast<T *>(a.b.vx)[b] = T(vp0 * (1.0 - 1.0) + vf0 * 1.0);
   static_cast<T *>(a.b.vy)[b] = T(vp1 * (1.0 - 1.0) + vf1 * 1.0);
@@ -633,7 +692,9 @@ extern "C" hg_status hg_hybrid_step(hg_hybrid *h, double dt, uint32_t nsteps) {
   if (h->mpm->phase_next != 0) return hg_fail(h->ctx, HG_ERR_INVALID, "a phased MPM step is in progress");
   // the DEM reads its dt from its scene table
   HG_TRY(hg_dem_step(h->dem, dt, 0));
-  hybrid_steps(h, dt, nsteps, true);
+  // profiled steps keep one stream, so that each event pair times one kernel
+  // running alone
+  hybrid_steps(h, dt, nsteps, !(h->mpm->prof.on || h->dem->prof.on));
   HG_HIP(h->ctx, hipGetLastError());
   return HG_OK;
 }
