@@ -32,6 +32,11 @@ __device__ __forceinline__ void st(void *p, uint32_t i, T v) {
   static_cast<T *>(p)[i] = v;
 }
 
+// a * b + c: one fused operation in FP32 mode; in FP64 a rounded product and
+// a rounded sum, the operations the CPU FP64 restatement performs.
+__device__ __forceinline__ float madd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double madd(double a, double b, double c) { return a * b + c; }
+
 __device__ __forceinline__ uint32_t scene_of_point(const uint32_t *p_off, uint32_t nscenes,
                                                    uint32_t i) {
   uint32_t lo = 0, hi = nscenes;  // p_off[lo] <= i < p_off[hi]
@@ -121,17 +126,22 @@ struct P2GRec {
   uint32_t cnt;  // stencil node counts: x in bits 0-3, y in bits 4-7
 };
 
+// Stencil node counts of a point whose stencil base node (bx, by) is its
+// cell key (as the sort binned it): only uGIMP's count varies, and its last
+// node is ceil((x + hl - min) / h) (uGIMPLinearBasisFunctions.cpp:68-90).
 template <int BASIS>
-__device__ __forceinline__ uint32_t stencil_counts(double x, double y, double hl, const SceneDev &sc) {
+__device__ __forceinline__ uint32_t stencil_counts(double x, double y, double hl, int bx, int by,
+                                                   const SceneDev &sc) {
   constexpr int S = BasisTraits<BASIS>::S;
-  double lo_d, hi_d;
-  stencil_1d<BASIS>(x, hl, sc.min0, sc.h, lo_d, hi_d);
-  const double cxd = hi_d - lo_d + 1.0;
-  const int cntx = (cxd >= 1.0 && cxd <= double(S)) ? int(cxd) : S;
-  stencil_1d<BASIS>(y, hl, sc.min1, sc.h, lo_d, hi_d);
-  const double cyd = hi_d - lo_d + 1.0;
-  const int cnty = (cyd >= 1.0 && cyd <= double(S)) ? int(cyd) : S;
-  return uint32_t(cntx) | (uint32_t(cnty) << 4);
+  if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) {
+    const double cxd = ceil(((x + hl) - sc.min0) / sc.h) - double(bx) + 1.0;
+    const int cntx = (cxd >= 1.0 && cxd <= double(S)) ? int(cxd) : S;
+    const double cyd = ceil(((y + hl) - sc.min1) / sc.h) - double(by) + 1.0;
+    const int cnty = (cyd >= 1.0 && cyd <= double(S)) ? int(cyd) : S;
+    return uint32_t(cntx) | (uint32_t(cnty) << 4);
+  } else {
+    return uint32_t(S) | (uint32_t(S) << 4);
+  }
 }
 
 // A point's fields as P2G loads them.
@@ -161,7 +171,8 @@ __device__ __forceinline__ P2GRaw<T> p2g_load(const PtsRaw &pts, uint32_t p, con
 }
 
 template <typename T, int BASIS>
-__device__ __forceinline__ P2GRec<T> p2g_record(const P2GRaw<T> &w, const SceneDev &sc) {
+__device__ __forceinline__ P2GRec<T> p2g_record(const P2GRaw<T> &w, int bx, int by,
+                                                const SceneDev &sc) {
   P2GRec<T> r;
   r.x = w.x;
   r.y = w.y;
@@ -190,7 +201,7 @@ __device__ __forceinline__ P2GRec<T> p2g_record(const P2GRaw<T> &w, const SceneD
   r.c = T(-double(w.V) * J);
   r.mv0 = r.m * w.v0;
   r.mv1 = r.m * w.v1;
-  r.cnt = stencil_counts<BASIS>(r.x, r.y, double(r.hl), sc);
+  r.cnt = stencil_counts<BASIS>(r.x, r.y, double(r.hl), bx, by, sc);
   return r;
 }
 
@@ -343,12 +354,17 @@ __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
           uint32_t rank = 0;
           for (uint32_t v = cb; v < ce; ++v) rank += (s_id[v] < raw[m].id) ? 1u : 0u;
           const uint32_t d = cb + rank;
-          const P2GRec<T> rec = p2g_record<T, BASIS>(raw[m], sc);
           const int bx = ax0 + col, by = ay0 + r;
+          const P2GRec<T> rec = p2g_record<T, BASIS>(raw[m], bx, by, sc);
           const T inv_h = T(sc.inv_h);
           const int cntx = int(rec.cnt & 15u), cnty = int(rec.cnt >> 4);
           T inv_vp = T(1);
           if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) inv_vp = T(1) / (T(4) * rec.hl * rec.hl);
+          // A node's force term counts only where its weight wx*wy is nonzero
+          // (PhysicsGrid.cpp:413-473 skips zero-weight nodes): an axis's
+          // derivative factor is staged as zero where that axis's weight is
+          // zero, so both gradient components gxs*wy and wx*gys vanish
+          // exactly when one factor of the weight does.
 #pragma unroll
           for (int k2 = 0; k2 < S; ++k2) {
             T ax, dx, ay, dy;
@@ -356,17 +372,20 @@ __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
                                    dx);
             basis_factor<T, BASIS>(rec.y, rec.hl, sc.min1, sc.h, inv_h, by + k2, k2, cnty, ay,
                                    dy);
+            T wx, gx, gy;
             if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) {
-              s_wx[k2][d] = ax * inv_vp;
-              s_gx[k2][d] = dx * inv_vp;
-              s_wy[k2][d] = ay;
-              s_gy[k2][d] = dy;
+              wx = ax * inv_vp;
+              gx = dx * inv_vp;
+              gy = dy;
             } else {
-              s_wx[k2][d] = ax;
-              s_gx[k2][d] = dx * inv_h;
-              s_wy[k2][d] = ay;
-              s_gy[k2][d] = dy * inv_h;
+              wx = ax;
+              gx = dx * inv_h;
+              gy = dy * inv_h;
             }
+            s_wx[k2][d] = wx;
+            s_gx[k2][d] = (wx != T(0)) ? gx : T(0);
+            s_wy[k2][d] = ay;
+            s_gy[k2][d] = (ay != T(0)) ? gy : T(0);
           }
           s_m[d] = rec.m;
           s_mv0[d] = rec.mv0;
@@ -385,12 +404,12 @@ __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
 
       // 2. gather: rows ascending, cells ascending, ids ascending.  The cell
       //    fixes the node's offset in the point's stencil, so every LDS read of
-      //    an iteration is independent of the others.  Force terms count only
-      //    where the weight is nonzero.  FP64 mode sums every term into the
-      //    FP64 node sums; FP32 mode sums each cell's terms in FP32 and adds
-      //    that partial to the FP64 node sums (chunks end on cell boundaries,
-      //    so the partials do not depend on the chunking unless one cell holds
-      //    more than a chunk).
+      //    an iteration is independent of the others.  The staged derivative
+      //    factors already vanish where the weight does.  FP64 mode sums every
+      //    term into the FP64 node sums; FP32 mode sums each cell's terms in
+      //    FP32 (fused multiply-adds) and adds that partial to the FP64 node
+      //    sums (chunks end on cell boundaries, so the partials do not depend
+      //    on the chunking unless one cell holds more than a chunk).
       if (nx < sc.nnx && ny < sc.nny) {
         constexpr bool f64 = std::is_same<T, double>::value;
 #pragma unroll
@@ -420,13 +439,16 @@ __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
               const T w = wx * wy;
               const T gx = gxs * wy;
               const T gy = wx * gys;
-              cm += w * pm;
-              cp0 += w * pv0;
-              cp1 += w * pv1;
-              const T t0 = f00 * gx + f01 * gy;
-              const T t1 = f01 * gx + f11 * gy;
-              cf0 += (w != T(0)) ? t0 : T(0);
-              cf1 += (w != T(0)) ? t1 : T(0);
+              cm = madd(w, pm, cm);
+              cp0 = madd(w, pv0, cp0);
+              cp1 = madd(w, pv1, cp1);
+              if constexpr (f64) {
+                cf0 += f00 * gx + f01 * gy;
+                cf1 += f01 * gx + f11 * gy;
+              } else {
+                cf0 = madd(f01, gy, madd(f00, gx, cf0));
+                cf1 = madd(f11, gy, madd(f01, gx, cf1));
+              }
             }
             if constexpr (f64) {
               am = cm;
@@ -669,23 +691,52 @@ __global__ __launch_bounds__(kG2PThreads) void mpm_g2p_kernel(G2PArgs a) {
   const T4 *g = static_cast<const T4 *>(a.gva) + sc.node_off;
   T L00 = 0, L10 = 0, L01 = 0, L11 = 0;
   T vp0 = 0, vp1 = 0, ac0 = 0, ac1 = 0;
+  if constexpr (std::is_same<T, float>::value) {
+    // FP32: the weights are products ax[i] ay[j], so each node row's sums
+    // over i are formed first and then weighted by the row's y factors
+    // (fused multiply-adds throughout).
 #pragma unroll
-  for (int j = 0; j < S; ++j) {
-    const uint32_t row = uint32_t(by + j) * uint32_t(sc.snx) + uint32_t(bx);
+    for (int j = 0; j < S; ++j) {
+      const uint32_t row = uint32_t(by + j) * uint32_t(sc.snx) + uint32_t(bx);
+      T dvx = 0, dvy = 0, wvx = 0, wvy = 0, wax = 0, way = 0;
 #pragma unroll
-    for (int ii = 0; ii < S; ++ii) {
-      const T4 nv = g[row + ii];
-      const T w = axs[ii] * ays[j];
-      const T gx = dxs[ii] * ays[j];
-      const T gy = axs[ii] * dys[j];
-      L00 += nv.x * gx;
-      L10 += nv.y * gx;
-      L01 += nv.x * gy;
-      L11 += nv.y * gy;
-      vp0 += w * nv.x;
-      vp1 += w * nv.y;
-      ac0 += w * nv.z;
-      ac1 += w * nv.w;
+      for (int ii = 0; ii < S; ++ii) {
+        const T4 nv = g[row + ii];
+        dvx = madd(nv.x, dxs[ii], dvx);
+        dvy = madd(nv.y, dxs[ii], dvy);
+        wvx = madd(nv.x, axs[ii], wvx);
+        wvy = madd(nv.y, axs[ii], wvy);
+        wax = madd(nv.z, axs[ii], wax);
+        way = madd(nv.w, axs[ii], way);
+      }
+      L00 = madd(ays[j], dvx, L00);
+      L10 = madd(ays[j], dvy, L10);
+      L01 = madd(dys[j], wvx, L01);
+      L11 = madd(dys[j], wvy, L11);
+      vp0 = madd(ays[j], wvx, vp0);
+      vp1 = madd(ays[j], wvy, vp1);
+      ac0 = madd(ays[j], wax, ac0);
+      ac1 = madd(ays[j], way, ac1);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      const uint32_t row = uint32_t(by + j) * uint32_t(sc.snx) + uint32_t(bx);
+#pragma unroll
+      for (int ii = 0; ii < S; ++ii) {
+        const T4 nv = g[row + ii];
+        const T w = axs[ii] * ays[j];
+        const T gx = dxs[ii] * ays[j];
+        const T gy = axs[ii] * dys[j];
+        L00 += nv.x * gx;
+        L10 += nv.y * gx;
+        L01 += nv.x * gy;
+        L11 += nv.y * gy;
+        vp0 += w * nv.x;
+        vp1 += w * nv.y;
+        ac0 += w * nv.z;
+        ac1 += w * nv.w;
+      }
     }
   }
 
@@ -701,10 +752,16 @@ __global__ __launch_bounds__(kG2PThreads) void mpm_g2p_kernel(G2PArgs a) {
   const double e00 = n00 + dt * (Lb00 + bL00);
   const double e01 = n01 + dt * (Lb10 + bL10);
   const double e11 = n11 + dt * (Lb11 + bL11);
+  // FP32 mode stores J and b_bar in FP32, so it divides by a shared
+  // reciprocal (a last-bit FP64 difference); FP64 mode divides as written.
+  constexpr bool f32 = std::is_same<T, float>::value;
   double detE = e00 * e11 - e01 * e01;
   double J1 = sqrt(detE);
   double nrm = sqrt(detE);
-  double c00 = e00 / nrm, c01 = e01 / nrm, c11 = e11 / nrm;
+  const double inrm = f32 ? 1.0 / nrm : 0.0;
+  double c00 = f32 ? e00 * inrm : e00 / nrm;
+  double c01 = f32 ? e01 * inrm : e01 / nrm;
+  double c11 = f32 ? e11 * inrm : e11 / nrm;
 
   // Drucker-Prager return on b* = J b_bar
   {
@@ -715,7 +772,10 @@ __global__ __launch_bounds__(kG2PThreads) void mpm_g2p_kernel(G2PArgs a) {
     if (ys < 0.0) ys = 0.0;
     const double half_tr = 0.5 * (s00 + s11);
     const double d00 = s00 - half_tr, d01 = s01 - 0.0, d11 = s11 - half_tr;
-    const double q00 = d00 / Jr, q01 = d01 / Jr, q11 = d11 / Jr;
+    const double iJr = f32 ? 1.0 / Jr : 0.0;
+    const double q00 = f32 ? d00 * iJr : d00 / Jr;
+    const double q01 = f32 ? d01 * iJr : d01 / Jr;
+    const double q11 = f32 ? d11 * iJr : d11 / Jr;
     const double fn = sqrt(q00 * q00 + q01 * q01 + q01 * q01 + q11 * q11);
     double r00 = s00, r01 = s01, r11 = s11;
     if (sc.mu * fn > ys) {
@@ -730,9 +790,10 @@ __global__ __launch_bounds__(kG2PThreads) void mpm_g2p_kernel(G2PArgs a) {
     const double detR = r00 * r11 - r01 * r01;
     J1 = sqrt(detR);
     const double rn = sqrt(detR);
-    c00 = r00 / rn;
-    c01 = r01 / rn;
-    c11 = r11 / rn;
+    const double irn = f32 ? 1.0 / rn : 0.0;
+    c00 = f32 ? r00 * irn : r00 / rn;
+    c01 = f32 ? r01 * irn : r01 / rn;
+    c11 = f32 ? r11 * irn : r11 / rn;
   }
 
   // PIC/FLIP velocity and advection
