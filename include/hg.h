@@ -143,7 +143,9 @@ hg_status hg_mpm_download_grid_scene(hg_mpm *h, uint32_t scene, double *mass, do
                                      double *mom_new, double *force, double *vel,
                                      double *acc);
 /* Per-scene device error bits after the steps queued so far (waits for the
- * stream): 1 = a point's stencil left the grid, 2 = non-finite point state. */
+ * stream): 1 = a point's stencil left the grid, 2 = non-finite point state;
+ * 0x100 (scene 0 only) = the cell sort's per-block totals disagreed with its
+ * cell counts (never expected; the step's results are void). */
 hg_status hg_mpm_scene_error(hg_mpm *h, uint32_t scene, int32_t *flags);
 
 /* P2G tile: aw x ah grid nodes per workgroup, one thread per node.

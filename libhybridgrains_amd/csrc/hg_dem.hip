@@ -321,7 +321,7 @@ struct IntegrateArgs {
   BodiesRaw b;
   SlotsRaw s;  // the slots this step wrote
   ForceRaw f;
-  uint32_t *key, *slot, *counts;
+  uint32_t *key, *slot, *counts, *totals;
   int32_t *err;
   double *F_out;  // optional: total force per body (3B)
   int32_t rekey;  // produce the next step's cell keys
@@ -395,6 +395,7 @@ __global__ __launch_bounds__(kThreads) void dem_integrate_kernel(IntegrateArgs a
   const uint32_t k = dem_bucket(sc, x, y);
   a.key[i] = k;
   a.slot[i] = atomicAdd(&a.counts[k], 1u);
+  hg_scan_block_count(a.totals, k, blockIdx.x);
 }
 
 struct BinArgs {
@@ -402,7 +403,7 @@ struct BinArgs {
   const uint32_t *b_off;
   uint32_t nscenes, nbodies;
   const double *x, *y;
-  uint32_t *key, *slot, *counts;
+  uint32_t *key, *slot, *counts, *totals;
 };
 
 __global__ __launch_bounds__(kThreads) void dem_bin_kernel(BinArgs a) {
@@ -412,6 +413,7 @@ __global__ __launch_bounds__(kThreads) void dem_bin_kernel(BinArgs a) {
   const uint32_t k = dem_bucket(sc, a.x[i], a.y[i]);
   a.key[i] = k;
   a.slot[i] = atomicAdd(&a.counts[k], 1u);
+  hg_scan_block_count(a.totals, k, blockIdx.x);
 }
 
 }  // namespace hgdem
@@ -474,6 +476,8 @@ static void launch_integrate_t(hg_dem *h, bool rekey) {
   a.key = h->key;
   a.slot = h->slot;
   a.counts = h->counts;
+  // read by the sort that follows this step, after slots flip
+  a.totals = h->scan_totals + size_t(1 - h->cur) * hg_scan_totals_words(h->nscan);
   a.err = h->err;
   a.F_out = h->keep_force ? h->F_out : nullptr;
   a.rekey = rekey ? 1 : 0;
@@ -493,12 +497,12 @@ void hg_dem_launch_step(hg_dem *h, bool rekey) {
     }
   }
   h->cur = 1 - h->cur;
-  if (rekey)
-    hg_counting_sort(h->prof, h->stream, h->counts, h->ncell, h->block_sums, h->nscan,
-                     h->cell_start, h->key, h->slot, h->nbodies, h->perm);
+  if (rekey) hg_dem_launch_sort(h);
 }
 
 void hg_dem_launch_rebin(hg_dem *h) {
+  uint32_t *totals = h->scan_totals + size_t(h->cur) * hg_scan_totals_words(h->nscan);
+  hg_zero_words(h->stream, totals, hg_scan_totals_words(h->nscan));
   if (h->nbodies) {
     BinArgs a;
     a.scenes = h->d_scenes;
@@ -510,6 +514,7 @@ void hg_dem_launch_rebin(hg_dem *h) {
     a.key = h->key;
     a.slot = h->slot;
     a.counts = h->counts;
+    a.totals = totals;
     ProfScope ps(h->prof, "dem_bin", h->stream);
     hipLaunchKernelGGL(dem_bin_kernel, dim3((h->nbodies + kThreads - 1) / kThreads), dim3(kThreads),
                        0, h->stream, a);
@@ -518,8 +523,10 @@ void hg_dem_launch_rebin(hg_dem *h) {
 }
 
 void hg_dem_launch_sort(hg_dem *h) {
-  hg_counting_sort(h->prof, h->stream, h->counts, h->ncell, h->block_sums, h->nscan,
-                   h->cell_start, h->key, h->slot, h->nbodies, h->perm);
+  hg_counting_sort(h->prof, h->stream, h->counts, h->ncell,
+                   h->scan_totals + size_t(h->cur) * hg_scan_totals_words(h->nscan),
+                   h->scan_totals + size_t(1 - h->cur) * hg_scan_totals_words(h->nscan), h->nscan, h->cell_start,
+                   h->key, h->slot, h->nbodies, h->perm, h->err);
 }
 
 static void launch_step(hg_dem *h) { hg_dem_launch_step(h, true); }
@@ -729,12 +736,14 @@ static hg_status dem_create_impl(hg_ctx *ctx, const hg_dem_desc *descs, uint32_t
   HG_TRY(dalloc(hp, &h->slot, size_t(h->nbodies) * 4));
   HG_TRY(dalloc(hp, &h->counts, size_t(h->ncell) * 4));
   HG_TRY(dalloc(hp, &h->cell_start, size_t(h->ncell + 1) * 4));
-  HG_TRY(dalloc(hp, &h->block_sums, size_t(h->nscan) * 4));
+  HG_TRY(dalloc(hp, &h->scan_totals, size_t(hg_scan_totals_words(h->nscan)) * 8));
   HG_TRY(dalloc(hp, &h->err, nscenes * 4));
   HG_TRY(dalloc(hp, &h->F_out, size_t(h->nbodies) * 24));
   HG_HIP(ctx, hipMemcpy(h->d_scenes, h->scenes.data(), nscenes * sizeof(SceneDev), hipMemcpyHostToDevice));
   HG_HIP(ctx, hipMemcpy(h->d_b_off, h_b_off.data(), (nscenes + 1) * 4, hipMemcpyHostToDevice));
   HG_HIP(ctx, hipMemset(h->counts, 0, size_t(h->ncell) * 4));
+  HG_HIP(ctx, hipMemset(h->scan_totals, 0, size_t(hg_scan_totals_words(h->nscan)) * 8));
+  HG_HIP(ctx, hipMemset(h->perm, 0, size_t(h->nbodies) * 4));
   HG_HIP(ctx, hipMemset(h->err, 0, nscenes * 4));
   HG_HIP(ctx, hipMemset(h->F_out, 0, size_t(h->nbodies ? h->nbodies : 1) * 24));
   if (h->precision == HG_FP64) HG_TRY(upload_bodies<double>(hp, descs));
@@ -752,6 +761,7 @@ static hg_status dem_check(hg_dem *h) {
   std::vector<int32_t> e(h->nscenes);
   HG_HIP(h->ctx, hipMemcpy(e.data(), h->err, h->nscenes * 4, hipMemcpyDeviceToHost));
   for (uint32_t s = 0; s < h->nscenes; ++s) {
+    if (e[s] & kSortTotalsBit) return hg_fail(h->ctx, HG_ERR_STATE, "bucket sort: block totals do not match the counts");
     if (e[s] & 4) return hg_fail(h->ctx, HG_ERR_UNSUPPORTED, "scene " + std::to_string(s) + ": more than 8 contacts on one body");
     if (e[s] & 2) return hg_fail(h->ctx, HG_ERR_STATE, "scene " + std::to_string(s) + ": non-finite body state");
   }
@@ -1001,7 +1011,7 @@ extern "C" hg_status hg_dem_estimate_bytes(const hg_dem_desc *descs, uint32_t ns
   }
   const size_t T = descs[0].precision == HG_FP64 ? 8 : 4;
   const uint64_t nscan = (cells + kScanChunk - 1) / kScanChunk;
-  *bytes = B * dem_bytes_per_body(T) + cells * 8 + 4 + nscan * 4 +
+  *bytes = B * dem_bytes_per_body(T) + cells * 8 + 4 + nscan * 8 * kScanSubTotals +
            uint64_t(nscenes) * (sizeof(SceneDev) + 16) + (1u << 20);
   return HG_OK;
 }

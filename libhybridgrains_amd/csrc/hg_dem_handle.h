@@ -94,7 +94,10 @@ struct hg_dem {
   int cur = 0;  // slots[cur] holds the last step's contacts
   ForceRaw f = {};
   uint32_t *perm = nullptr, *key = nullptr, *slot = nullptr, *counts = nullptr,
-           *cell_start = nullptr, *block_sums = nullptr;
+           *cell_start = nullptr;
+  // per-scan-block totals of the bucket keys that the sort after a step
+  // with slots[c] current reads, at scan_totals + c * hg_scan_totals_words(nscan)
+  uint32_t *scan_totals = nullptr;
   int32_t *err = nullptr;
   double *F_out = nullptr;
   double *cpoint = nullptr;  // contact points of the owned circle pairs (kept on request)

@@ -30,6 +30,7 @@
 // at creation), so the DEM integration may move the grains while grain_mass
 // still reads them.  A graph capture queues the same kernels on one stream.
 #include "hg_hybrid_handle.h"
+#include "hg_sort.h"
 
 #include <memory>
 
@@ -50,7 +51,7 @@ struct HybArgs {
   hgdem::BodiesRaw b;
   uint32_t nbodies;
   const hgdem::SceneDev *dsc;  // DEM scene 0 (broad-phase keys)
-  uint32_t *dkey, *dslot, *dcounts;
+  uint32_t *dkey, *dslot, *dcounts, *dtotals;
   double *prex, *prey;
   uint8_t *coupled;
   GrainGrid g;
@@ -320,6 +321,7 @@ __global__ __launch_bounds__(kThreads) void hyb_finish(HybArgs a, double dt) {
     const uint32_t k = hgdem::dem_bucket(*a.dsc, x, y);
     a.dkey[b] = k;
     a.dslot[b] = atomicAdd(&a.dcounts[k], 1u);
+    hg_scan_block_count(a.dtotals, k, blockIdx.x);
   }
   const uint32_t cnt = min(*a.count, a.g.max_list);
   for (uint32_t k = tid; k < cnt; k += gridDim.x * kThreads) {
@@ -478,6 +480,9 @@ static HybArgs make_args(hg_hybrid *h) {
   a.dkey = h->dem->key;
   a.dslot = h->dem->slot;
   a.dcounts = h->dem->counts;
+  // read by the DEM sort after this step's finish (the DEM step has flipped
+  // its slots already)
+  a.dtotals = h->dem->scan_totals + size_t(h->dem->cur) * hg_scan_totals_words(h->dem->nscan);
   a.prex = h->prex;
   a.prey = h->prey;
   a.coupled = h->coupled;

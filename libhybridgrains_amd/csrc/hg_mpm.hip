@@ -656,7 +656,7 @@ struct G2PArgs {
   PtsRaw src, dst;
   const uint32_t *pinv;
   const void *gva;
-  uint32_t *key, *slot, *counts;
+  uint32_t *key, *slot, *counts, *totals;
   int32_t *err;
 };
 
@@ -852,6 +852,7 @@ __global__ __launch_bounds__(kG2PThreads) void mpm_g2p_kernel(G2PArgs a) {
   const uint32_t k = sc.cell_off + uint32_t(nby) * uint32_t(sc.snx) + uint32_t(nbx);
   a.key[o] = k;
   a.slot[o] = hg_run_slot(a.counts, k);
+  hg_scan_block_count(a.totals, k, blk);
 }
 
 // Initial binning of points in creation order.
@@ -860,7 +861,7 @@ struct BinArgs {
   const uint32_t *p_off;
   uint32_t nscenes, npoints;
   PtsRaw pts;
-  uint32_t *key, *slot, *counts;
+  uint32_t *key, *slot, *counts, *totals;
   int32_t *err;
 };
 
@@ -877,6 +878,7 @@ __global__ __launch_bounds__(256) void mpm_bin_kernel(BinArgs a) {
   const uint32_t k = sc.cell_off + uint32_t(by) * uint32_t(sc.snx) + uint32_t(bx);
   a.key[i] = k;
   a.slot[i] = atomicAdd(&a.counts[k], 1u);
+  hg_scan_block_count(a.totals, k, blockIdx.x);
 }
 
 }  // namespace
@@ -980,6 +982,7 @@ static void launch_g2p_t(hg_mpm *h) {
   a.key = h->key;
   a.slot = h->slot;
   a.counts = h->counts;
+  a.totals = h->scan_totals + size_t(1 - h->cur) * hg_scan_totals_words(h->nscan);  // the keys of buf[1 - cur]
   a.err = h->err;
   ProfScope ps(h->prof, "mpm_g2p", h->stream);
   hipLaunchKernelGGL((mpm_g2p_kernel<T, BASIS>), dim3(a.nblocks_padded), dim3(kG2PThreads), 0,
@@ -1006,6 +1009,7 @@ static void launch_bin_t(hg_mpm *h) {
   a.key = h->key;
   a.slot = h->slot;
   a.counts = h->counts;
+  a.totals = h->scan_totals + size_t(h->cur) * hg_scan_totals_words(h->nscan);
   a.err = h->err;
   const uint32_t nb = (h->npoints + 255) / 256;
   if (nb) hipLaunchKernelGGL((mpm_bin_kernel<T, BASIS>), dim3(nb), dim3(256), 0, h->stream, a);
@@ -1040,8 +1044,10 @@ static void launch_tile_list(hg_mpm *h) {
 // Sorts the points of the current buffer by cell (perm, pinv, cell_start)
 // and lists the active tiles.
 static void launch_sort(hg_mpm *h) {
-  hg_counting_sort(h->prof, h->stream, h->counts, h->ncell, h->block_sums, h->nscan,
-                   h->cell_start, h->key, h->slot, h->npoints, h->perm, h->pinv);
+  hg_counting_sort(h->prof, h->stream, h->counts, h->ncell,
+                   h->scan_totals + size_t(h->cur) * hg_scan_totals_words(h->nscan),
+                   h->scan_totals + size_t(1 - h->cur) * hg_scan_totals_words(h->nscan), h->nscan, h->cell_start,
+                   h->key, h->slot, h->npoints, h->perm, h->err, h->pinv);
   launch_tile_list(h);
 }
 
@@ -1103,12 +1109,20 @@ hg_status hg_mpm_launch_coupled_p2g(hg_mpm *h, double dt) {
 
 void hg_mpm_launch_g2p(hg_mpm *h) { launch_g2p_and_sort(h); }
 
-hg_status hg_mpm_rebin(hg_mpm *h) {
+// Bins the points of buf[cur] afresh (their block totals start from zero)
+// and sorts them.
+static void launch_rebin(hg_mpm *h) {
+  hg_zero_words(h->stream, h->scan_totals + size_t(h->cur) * hg_scan_totals_words(h->nscan),
+                hg_scan_totals_words(h->nscan));
   if (h->npoints) {
     if (h->precision == HG_FP64) launch_bin_p<double>(h);
     else launch_bin_p<float>(h);
   }
   launch_sort(h);
+}
+
+hg_status hg_mpm_rebin(hg_mpm *h) {
+  launch_rebin(h);
   HG_HIP(h->ctx, hipGetLastError());
   HG_HIP(h->ctx, hipStreamSynchronize(h->stream));
   return HG_OK;
@@ -1304,7 +1318,7 @@ static hg_status mpm_create_impl(hg_ctx *ctx, const hg_mpm_desc *descs, uint32_t
   HG_TRY(dalloc(hp, &h->slot, size_t(h->npoints) * 4));
   HG_TRY(dalloc(hp, &h->counts, size_t(h->ncell) * 4));
   HG_TRY(dalloc(hp, &h->cell_start, size_t(h->ncell + 1) * 4));
-  HG_TRY(dalloc(hp, &h->block_sums, size_t(h->nscan) * 4));
+  HG_TRY(dalloc(hp, &h->scan_totals, size_t(hg_scan_totals_words(h->nscan)) * 8));
   HG_TRY(dalloc(hp, &h->gva, size_t(h->nnode) * 4 * hp->tsize()));
   HG_TRY(dalloc(hp, &h->gm, size_t(h->nnode) * 8));
   HG_TRY(dalloc(hp, &h->gp, size_t(h->nnode) * 16));
@@ -1319,6 +1333,8 @@ static hg_status mpm_create_impl(hg_ctx *ctx, const hg_mpm_desc *descs, uint32_t
   HG_HIP(ctx, hipMemset(h->tile_active, 0, h->ntiles));
   HG_HIP(ctx, hipMemset(h->tile_count, 0, 8));
   HG_HIP(ctx, hipMemset(h->counts, 0, size_t(h->ncell) * 4));
+  HG_HIP(ctx, hipMemset(h->scan_totals, 0, size_t(hg_scan_totals_words(h->nscan)) * 8));
+  HG_HIP(ctx, hipMemset(h->perm, 0, size_t(h->npoints) * 4));
   HG_HIP(ctx, hipMemset(h->gva, 0, size_t(h->nnode) * 4 * hp->tsize()));
   HG_HIP(ctx, hipMemset(h->gm, 0, size_t(h->nnode) * 8));
   HG_HIP(ctx, hipMemset(h->gp, 0, size_t(h->nnode) * 16));
@@ -1332,9 +1348,7 @@ static hg_status mpm_create_impl(hg_ctx *ctx, const hg_mpm_desc *descs, uint32_t
   HG_HIP(ctx, hipDeviceSynchronize());
 
   // initial sort
-  if (h->precision == HG_FP64) launch_bin_p<double>(hp);
-  else launch_bin_p<float>(hp);
-  launch_sort(hp);
+  launch_rebin(hp);
   HG_HIP(ctx, hipGetLastError());
   HG_HIP(ctx, hipStreamSynchronize(h->stream));
   std::vector<int32_t> e(nscenes);
@@ -1359,6 +1373,7 @@ static hg_status check_err(hg_mpm *h) {
   std::vector<int32_t> e(h->nscenes);
   HG_HIP(h->ctx, hipMemcpy(e.data(), h->err, h->nscenes * 4, hipMemcpyDeviceToHost));
   for (uint32_t s = 0; s < h->nscenes; ++s) {
+    if (e[s] & kSortTotalsBit) return hg_fail(h->ctx, HG_ERR_STATE, "cell sort: block totals do not match the counts");
     if (e[s] & 1) return hg_fail(h->ctx, HG_ERR_STATE, "scene " + std::to_string(s) + ": a material point left the grid");
     if (e[s] & 2) return hg_fail(h->ctx, HG_ERR_STATE, "scene " + std::to_string(s) + ": non-finite material point state");
   }
@@ -1662,7 +1677,7 @@ extern "C" hg_status hg_mpm_estimate_bytes(const hg_mpm_desc *descs, uint32_t ns
   const uint64_t per_point = 2 * (16 + 9 * T + 4 + 1) + 16;
   const uint64_t per_node = 8 + 4 * T + 8 + 16 * 3;  // cell lists, grid fields
   const uint64_t nscan = (nodes + kScanChunk - 1) / kScanChunk;
-  *bytes = N * per_point + nodes * per_node + 4 + nscan * 4 + tiles * 5 +
+  *bytes = N * per_point + nodes * per_node + 4 + nscan * 8 * kScanSubTotals + tiles * 5 +
            uint64_t(nscenes) * (sizeof(SceneDev) + 16) + (1u << 20);
   return HG_OK;
 }

@@ -21,7 +21,10 @@ struct hg_mpm {
   PtsRaw buf[2] = {};
   int cur = 0;
   uint32_t *perm = nullptr, *pinv = nullptr, *key = nullptr, *slot = nullptr, *counts = nullptr,
-           *cell_start = nullptr, *block_sums = nullptr;
+           *cell_start = nullptr;
+  // per-scan-block totals of the cell keys of the points in buf[c], at
+  // scan_totals + c * hg_scan_totals_words(nscan)
+  uint32_t *scan_totals = nullptr;
   uint32_t nscan = 0;
   SceneDev *d_scenes = nullptr;
   uint32_t *d_p_off = nullptr;

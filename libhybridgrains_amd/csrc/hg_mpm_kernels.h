@@ -19,7 +19,7 @@
 //
 // Kernels of one step: p2g (node gather over the active tiles, also the whole
 // grid update), g2p (gather + return map + advection + re-key), then the
-// counting sort (scan_reduce, scan_apply, scatter) and the active-tile list
+// counting sort (scan, scatter) and the active-tile list
 // (tile_list).
 #pragma once
 

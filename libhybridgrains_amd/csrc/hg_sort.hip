@@ -1,87 +1,116 @@
-// hg_sort.hip -- counting sort of items by cell key: a three-pass exclusive
-// scan of the per-cell counts and a scatter of item indices.
+// hg_sort.hip -- counting sort of items by cell key: a single-pass exclusive
+// scan of the per-cell counts (block offsets from the producer's block
+// totals) and a scatter of item indices.
 #include "hg_sort.h"
 
 namespace {
 
 // ---------------------------------------------------------------------------
-// Counting sort: exclusive scan of the per-cell counts (two passes: block
-// totals, then block scans offset by the totals before them), then a scatter
-// of point indices.  scan_apply also zeroes the counts for the next step's
-// G2P.
+// Single-pass exclusive scan of the per-cell counts.  Block b of the scan
+// covers cells [b kScanChunk, (b + 1) kScanChunk); the producer of the keys
+// already summed each block's items, spread over kScanSubTotals counters per
+// block (totals[b * kScanSubTotals + k]), so block b's offset is the sum of
+// the first b * kScanSubTotals counters.  Every thread reads a few of them
+// (all loads in flight together, along with the block's own counts, 16 bytes
+// per thread) and the block reduces them with its scan.  The block writes
+// cell_start, zeroes its counts for the next round of slot atomics and its
+// counters of the other totals buffer for the next producer.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t block_excl_scan_1024(uint32_t v, uint32_t &total) {
-  __shared__ uint32_t s_wave[16];
-  __shared__ uint32_t s_total;
+
+// Exclusive scan of v over the block, the block's total of v, and the block's
+// sum of u.
+__device__ __forceinline__ uint32_t block_excl_scan_1024(uint32_t v, uint32_t u, uint32_t &total,
+                                                         uint32_t &usum) {
+  __shared__ uint32_t s_wave[16], s_wu[16];
+  __shared__ uint32_t s_total, s_usum;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t inc = hg_wave_incl_scan(v);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) u += __shfl_down(u, off, 64);
   if (lane == 63) s_wave[wave] = inc;
+  if (lane == 0) s_wu[wave] = u;
   __syncthreads();
   if (wave == 0) {
     const uint32_t wv = lane < 16 ? s_wave[lane] : 0u;
     const uint32_t winc = hg_wave_incl_scan(wv);
+    uint32_t wu = lane < 16 ? s_wu[lane] : 0u;
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) wu += __shfl_down(wu, off, 64);
     if (lane < 16) s_wave[lane] = winc - wv;  // exclusive wave offsets
     if (lane == 15) s_total = winc;
+    if (lane == 0) s_usum = wu;
   }
   __syncthreads();
   total = s_total;
+  usum = s_usum;
   return s_wave[wave] + inc - v;
 }
 
-__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t *counts,
-                                                                   uint32_t n,
-                                                                   uint32_t *block_sums) {
-  const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * kScanItems;
+__global__ __launch_bounds__(kScanThreads) void scan_kernel(uint32_t *counts, uint32_t n,
+                                                            const uint32_t *totals,
+                                                            uint32_t *totals_clear,
+                                                            uint32_t *cell_start, uint32_t nitems,
+                                                            int32_t *err) {
+  const uint32_t bid = blockIdx.x;
+  const uint32_t base = bid * uint32_t(kScanChunk) + threadIdx.x * uint32_t(kScanItems);
+  const bool full = base + uint32_t(kScanItems) <= n;
+  uint32_t v[kScanItems];
+  if (full) {
+    const uint4 q = *reinterpret_cast<const uint4 *>(counts + base);
+    v[0] = q.x;
+    v[1] = q.y;
+    v[2] = q.z;
+    v[3] = q.w;
+  } else {
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) v[k] = (base + k < n) ? counts[base + k] : 0u;
+  }
+  // this thread's share of the counters of the blocks before this one
+  const uint32_t nbefore = bid * uint32_t(kScanSubTotals);
+  uint32_t before = 0;
+  for (uint32_t j = threadIdx.x; j < nbefore; j += uint32_t(kScanThreads)) before += totals[j];
   uint32_t s = 0;
 #pragma unroll
-  for (int k = 0; k < kScanItems; ++k)
-    if (base + k < n) s += counts[base + k];
-  uint32_t total = 0;
-  (void)block_excl_scan_1024(s, total);
-  if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+  for (int k = 0; k < kScanItems; ++k) s += v[k];
+  uint32_t total = 0, offset = 0;
+  const uint32_t excl = block_excl_scan_1024(s, before, total, offset);
+  if (threadIdx.x < uint32_t(kScanSubTotals)) {
+    // the block's own counters must add up to its counts
+    uint32_t mine = totals[nbefore + threadIdx.x];
+#pragma unroll
+    for (int off = kScanSubTotals / 2; off > 0; off >>= 1) mine += __shfl_down(mine, off, 64);
+    if (threadIdx.x == 0 && mine != total) atomicOr(err, kSortTotalsBit);
+    totals_clear[nbefore + threadIdx.x] = 0u;
+  }
+
+  // Starts are clamped to the item count (a no-op unless the totals were
+  // wrong), so every later index through cell_start stays in bounds.
+  uint32_t run = offset + excl;
+  if (full) {
+    uint4 q;
+    q.x = min(run, nitems);
+    q.y = min(run + v[0], nitems);
+    q.z = min(run + v[0] + v[1], nitems);
+    q.w = min(run + v[0] + v[1] + v[2], nitems);
+    *reinterpret_cast<uint4 *>(cell_start + base) = q;
+    *reinterpret_cast<uint4 *>(counts + base) = make_uint4(0u, 0u, 0u, 0u);
+    if (base + uint32_t(kScanItems) == n) cell_start[n] = min(run + s, nitems);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      if (base + k < n) {
+        cell_start[base + k] = min(run, nitems);
+        counts[base + k] = 0u;
+        if (base + k == n - 1) cell_start[n] = min(run + v[k], nitems);
+      }
+      run += v[k];
+    }
+  }
 }
 
-// Each block first sums the totals of the blocks before it (at most
-// kScanChunk of them, one per thread), so no separate pass over the block
-// totals is needed.
-__global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(uint32_t *counts, uint32_t n,
-                                                                  const uint32_t *block_sums,
-                                                                  uint32_t *cell_start) {
-  __shared__ uint32_t s_part[kScanThreads / 64];
-  uint32_t before = 0;
-  for (uint32_t b = threadIdx.x; b < blockIdx.x; b += kScanThreads) before += block_sums[b];
-  {
-    // block sum of 'before'
-    uint32_t v = before;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = v;
-    __syncthreads();
-    uint32_t tot = 0;
-#pragma unroll
-    for (int w = 0; w < kScanThreads / 64; ++w) tot += s_part[w];
-    before = tot;
-    __syncthreads();
-  }
-  const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * kScanItems;
-  uint32_t vals[kScanItems];
-  uint32_t s = 0;
-#pragma unroll
-  for (int k = 0; k < kScanItems; ++k) {
-    vals[k] = (base + k < n) ? counts[base + k] : 0u;
-    s += vals[k];
-  }
-  uint32_t total = 0;
-  uint32_t run = block_excl_scan_1024(s, total) + before;
-#pragma unroll
-  for (int k = 0; k < kScanItems; ++k) {
-    if (base + k < n) {
-      cell_start[base + k] = run;
-      counts[base + k] = 0u;
-      if (base + k == n - 1) cell_start[n] = run + vals[k];
-    }
-    run += vals[k];
-  }
+__global__ __launch_bounds__(256) void zero_kernel(uint32_t *p, uint32_t n) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = 0u;
 }
 
 __global__ __launch_bounds__(256) void scatter_kernel(const uint32_t *key, const uint32_t *slot,
@@ -90,29 +119,29 @@ __global__ __launch_bounds__(256) void scatter_kernel(const uint32_t *key, const
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const uint32_t pos = cell_start[key[i]] + slot[i];
-  perm[pos] = i;
-  if (pinv) pinv[i] = pos;
+  // out of range only if the scan's totals were wrong (flagged there)
+  if (pos < n) perm[pos] = i;
+  if (pinv) pinv[i] = pos < n ? pos : i;
 }
 
 }  // namespace
 
 void hg_counting_sort(KernelProfiler &prof, hipStream_t stream, uint32_t *counts, uint32_t ncell,
-                      uint32_t *block_sums, uint32_t nscan, uint32_t *cell_start,
-                      const uint32_t *key, const uint32_t *slot, uint32_t n, uint32_t *perm,
-                      uint32_t *pinv) {
-  {
-    ProfScope ps(prof, "scan_reduce", stream);
-    hipLaunchKernelGGL(scan_reduce_kernel, dim3(nscan), dim3(kScanThreads), 0, stream, counts,
-                       ncell, block_sums);
-  }
-  {
-    ProfScope ps(prof, "scan_apply", stream);
-    hipLaunchKernelGGL(scan_apply_kernel, dim3(nscan), dim3(kScanThreads), 0, stream, counts,
-                       ncell, block_sums, cell_start);
+                      const uint32_t *totals_in, uint32_t *totals_clear, uint32_t nscan,
+                      uint32_t *cell_start, const uint32_t *key, const uint32_t *slot, uint32_t n,
+                      uint32_t *perm, int32_t *err, uint32_t *pinv) {
+  if (nscan) {
+    ProfScope ps(prof, "scan", stream);
+    hipLaunchKernelGGL(scan_kernel, dim3(nscan), dim3(kScanThreads), 0, stream, counts, ncell,
+                       totals_in, totals_clear, cell_start, n, err);
   }
   if (n) {
     ProfScope ps(prof, "scatter", stream);
     hipLaunchKernelGGL(scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, key, slot,
                        cell_start, n, perm, pinv);
   }
+}
+
+void hg_zero_words(hipStream_t stream, uint32_t *p, uint32_t n) {
+  if (n) hipLaunchKernelGGL(zero_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, p, n);
 }
