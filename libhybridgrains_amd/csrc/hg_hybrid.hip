@@ -17,9 +17,9 @@
 // Kernels of one step:
 //   continuum stream       | grain stream
 //   P2G + grid update      | DEM contacts, DEM integration
-//   grain_mass             |
-//   grain_mom, exchange (also refreshes the node velocity and acceleration
-//   P2G wrote at the nodes it changes)
+//   grain_mass (coupling test, grain mass and momentum spread)
+//   exchange (also refreshes the node velocity and acceleration P2G wrote
+//   at the nodes it changes)
 //   G2P, MPM sort, tiles   | transfer, finish (coupled grain positions, the
 //                          | grains' broad-phase keys, the next step's
 //                          | pre-step positions, list clean-up), DEM sort,
@@ -149,7 +149,9 @@ __device__ __forceinline__ bool node_active(const HybArgs &a, int ix, int iy) {
 // HybridCoupling2D::rasterizeGrainMass (Coupling.cpp:22-129): a grain is
 // coupled when every on-grid node of its stencil (at its pre-step position)
 // carries more than 1e-6 of continuum mass; its mass is then spread with the
-// MPM weights.
+// MPM weights.  The same pass spreads its momentum at the post-DEM-step
+// velocity (rasterizeGrainMom, Coupling.cpp:287-466), so the step's DEM step
+// must be complete when it runs.
 template <typename T, int BASIS>
 __global__ __launch_bounds__(kThreads) void hyb_grain_mass(HybArgs a) {
   constexpr int S = GrainStencil<BASIS>::S;
@@ -176,13 +178,16 @@ __global__ __launch_bounds__(kThreads) void hyb_grain_mass(HybArgs a) {
       if (all_pos) {
         a.coupled[b] = 1;
         const double mb = double(ld<T>(a.b.m, b));
-        // mass spread and stamps, all nodes at once
+        const double pm0 = mb * double(ld<T>(a.b.vx, b)), pm1 = mb * double(ld<T>(a.b.vy, b));
+        // mass and momentum spread and stamps, all nodes at once
 #pragma unroll
         for (int q = 0; q < S * S; ++q) {
           if (!((g.on >> q) & 1u)) continue;
           const uint32_t n = uint32_t(g.ly + q / S) * uint32_t(sc.snx) + uint32_t(g.lx + q % S);
           const double w = g.wx[q % S] * g.wy[q / S];
           atomicAdd(&a.g.dmass[n], w * mb);
+          atomicAdd(&a.g.dmom[2 * n], w * pm0);
+          atomicAdd(&a.g.dmom[2 * n + 1], w * pm1);
           if (atomicExch(&a.g.stamp[n], 1u) == 0u) fresh |= 1u << q;
         }
       }
@@ -207,28 +212,6 @@ __global__ __launch_bounds__(kThreads) void hyb_grain_mass(HybArgs a) {
       else atomicOr(a.err, 8);
       ++base;
     }
-  }
-}
-
-// HybridCoupling2D::rasterizeGrainMom (Coupling.cpp:287-466): post-step grain
-// velocity, pre-step position.
-template <typename T, int BASIS>
-__global__ __launch_bounds__(kThreads) void hyb_grain_mom(HybArgs a) {
-  const uint32_t b = blockIdx.x * kThreads + threadIdx.x;
-  if (b >= a.nbodies || !a.coupled[b]) return;
-  constexpr int S = GrainStencil<BASIS>::S;
-  const hgmpm::SceneDev &sc = *a.msc;
-  GrainStencil<BASIS> g;
-  grain_stencil_weights<BASIS>(sc, a.prex[b], a.prey[b], a.hl, g);
-  const double mb = double(ld<T>(a.b.m, b));
-  const double pm0 = mb * double(ld<T>(a.b.vx, b)), pm1 = mb * double(ld<T>(a.b.vy, b));
-#pragma unroll
-  for (int q = 0; q < S * S; ++q) {
-    if (!((g.on >> q) & 1u)) continue;
-    const uint32_t n = uint32_t(g.ly + q / S) * uint32_t(sc.snx) + uint32_t(g.lx + q % S);
-    const double w = g.wx[q % S] * g.wy[q / S];
-    atomicAdd(&a.g.dmom[2 * n], w * pm0);
-    atomicAdd(&a.g.dmom[2 * n + 1], w * pm1);
   }
 }
 
@@ -523,17 +506,13 @@ static void launch_hybrid_steps(hg_hybrid *h, double dt, uint32_t n, bool concur
   for (uint32_t k = 0; k < n; ++k) {
     HybArgs a = make_args(h);
     hg_mpm_launch_coupled_p2g(h->mpm, dt);
-    // grain_mass needs the previous step's finish: pre-step positions,
-    // cleared node lists
-    if (k > 0 && side != s) hipStreamWaitEvent(s, h->fin_ev, 0);
+    // grain_mass needs this step's DEM step (post-step velocities) and, queued
+    // before it on the grain stream, the previous step's finish (pre-step
+    // positions, cleared node lists)
+    order(h->join_ev, side, s);
     {
       ProfScope ps(prof, "hyb_grain_mass", s);
       hipLaunchKernelGGL((hyb_grain_mass<T, BASIS>), gb, tb, 0, s, a);
-    }
-    order(h->join_ev, side, s);  // this step's DEM step
-    {
-      ProfScope ps(prof, "hyb_grain_mom", s);
-      hipLaunchKernelGGL((hyb_grain_mom<T, BASIS>), gb, tb, 0, s, a);
     }
     {
       // the list holds at most S*S nodes per grain: a grid over the grains
@@ -550,7 +529,6 @@ static void launch_hybrid_steps(hg_hybrid *h, double dt, uint32_t n, bool concur
       ProfScope ps(prof, "hyb_finish", side);
       hipLaunchKernelGGL(hyb_finish<T>, gb, tb, 0, side, a, dt);
     }
-    if (side != s) hipEventRecord(h->fin_ev, side);
     h->dem->stream = side;
     hg_dem_launch_sort(h->dem);
     h->dem->stream = s;
@@ -665,8 +643,7 @@ extern "C" hg_status hg_hybrid_create(hg_ctx *ctx, const hg_mpm_desc *mpm_desc,
   }
   if (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&h->join_ev, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&h->fin_ev, hipEventDisableTiming) != hipSuccess)
+      hipEventCreateWithFlags(&h->join_ev, hipEventDisableTiming) != hipSuccess)
     return fail(hg_fail(ctx, HG_ERR_HIP, "hybrid streams"));
   if (hipDeviceSynchronize() != hipSuccess) return fail(hg_fail(ctx, HG_ERR_HIP, "hybrid init"));
   *out = h.release();
@@ -682,7 +659,6 @@ extern "C" hg_status hg_hybrid_destroy(hg_hybrid *h) {
   }
   if (h->fork_ev) hipEventDestroy(h->fork_ev);
   if (h->join_ev) hipEventDestroy(h->join_ev);
-  if (h->fin_ev) hipEventDestroy(h->fin_ev);
   if (h->graph) hipGraphExecDestroy(h->graph);
   for (void *p : h->allocs) hipFree(p);
   if (h->dem) hg_dem_destroy(h->dem);

@@ -42,7 +42,7 @@ struct hg_hybrid {
   // next step's DEM contacts and integration beside G2P, the MPM sort and
   // P2G; events order the two.
   hipStream_t side = nullptr;
-  hipEvent_t fork_ev = nullptr, join_ev = nullptr, fin_ev = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   hghyb::GrainGrid g = {};
   double *prex = nullptr, *prey = nullptr;
   uint8_t *coupled = nullptr;
