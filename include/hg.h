@@ -241,6 +241,19 @@ hg_status hg_dem_download_bodies(hg_dem *h, uint32_t scene, double *q, double *v
  * most max entries are written. */
 hg_status hg_dem_download_contacts(hg_dem *h, uint32_t scene, uint32_t max, int32_t *type,
                                    int32_t *i, int32_t *j, double *delta_s, uint32_t *count);
+/* The last step's force on the owner and contact point of each contact, in
+ * hg_dem_download_contacts' order (planes give zeros; points are kept only
+ * while a hybrid homogenises, else zeros): what the reference's collision
+ * cache holds for ResamplingTools::homogenizeStress (ResamplingTools.cpp:
+ * 744-801). */
+hg_status hg_dem_download_contact_forces(hg_dem *h, uint32_t scene, uint32_t max, double *force,
+                                         double *point, uint32_t *count);
+/* Replaces the contact history with count contacts in hg_dem_download_contacts'
+ * form (pairs with i < j), as PenaltyImpactFrictionMap's caches hold them after
+ * RigidBody2DState::removeBodies relabels them (PenaltyImpactFrictionMap.cpp:
+ * 1199-1274). */
+hg_status hg_dem_upload_contacts(hg_dem *h, uint32_t scene, uint32_t count, const int32_t *type,
+                                 const int32_t *i, const int32_t *j, const double *delta_s);
 /* Keep the total per-body force (3B) of each step for download. */
 hg_status hg_dem_keep_force(hg_dem *h, int32_t keep);
 hg_status hg_dem_download_force(hg_dem *h, uint32_t scene, double *F);
@@ -297,6 +310,9 @@ hg_status hg_hybrid_set_homogenize(hg_hybrid *h, int32_t stress, int32_t velocit
  * flatNodeIndex): stress 4 per node (column-major 2x2), velocity 2 per node;
  * NULL skips one.  Zeros while homogenisation was never switched on. */
 hg_status hg_hybrid_download_homogenized(hg_hybrid *h, double *stress, double *velocity);
+/* Sets the homogenised accumulators from hg_hybrid_download_homogenized's
+ * layout (the zone resampling's clear and accumulate steps run on the host). */
+hg_status hg_hybrid_upload_homogenized(hg_hybrid *h, const double *stress, const double *velocity);
 /* ResamplingTools::clearHomogenizedStress / clearHomogenizedVelocity
  * (ResamplingTools.cpp:736-742, :810-816). */
 hg_status hg_hybrid_clear_homogenized(hg_hybrid *h);

@@ -51,6 +51,49 @@ hg_status hgx_hybrid_scene_times(const hgx_hybrid_scene *s, double *dt, double *
 const char *hgx_hybrid_scene_style(const hgx_hybrid_scene *s);
 void hgx_hybrid_scene_free(hgx_hybrid_scene *s);
 
+/* ---- zone resampling on the host (hg_resample.hpp) ----------------------------
+ * The pieces of ResamplingTools / ZoneTools / CAvoidAVoid (hybridgrains2dnew/
+ * ResamplingTools.cpp, ZoneTools.cpp, AvoidAvoid.cpp, random.c,
+ * CircleRectangleIntersection.cpp) as plain calls, and one resampling pass on
+ * a hybrid scene without any device work. */
+
+/* random.c: n tempered words of a stream seeded with seed (randomMT_i), and
+ * n single-precision reals of another such stream (randomMT). */
+hg_status hgx_mt1998(uint32_t seed, uint32_t n, uint32_t *words, float *reals);
+/* CircleRectangleIntersection.cpp: area of [x0,x1]x[y0,y1] inside the circle. */
+double hgx_circle_rect_area(double x0, double x1, double y0, double y1, double cx, double cy,
+                            double r);
+/* ZoneTools on given circles (2 + 1 per circle) and points (2 per point):
+ * packing fraction (ls cells), signed distance (ls nodes), level-set and
+ * MPM-cell zone indicators (0 continuum, 1 discrete, 2 hybrid) for a grid of
+ * cells[0] x cells[1] MPM cells of width h at gmin and level-set cells of
+ * width ls_h.  Output sizes: ls_cells = ceil(cells * h / ls_h) per axis. */
+hg_status hgx_zone_tools(const double gmin[2], double h, const uint32_t cells[2], double ls_h,
+                         uint32_t ncircles, const double *circle_pos, const double *circle_r,
+                         uint32_t npoints, const double *points, double window, int32_t spc,
+                         double phi_threshold, double level_set, double half_thickness,
+                         int32_t allow_direct, double *phi, double *dist, int32_t *ls_indicator,
+                         int32_t *mpm_indicator);
+
+/* One pass of the resampling (HybridGrains2DSim::updateHybridStates_poorman's
+ * resampling block) on a hybrid scene as HybridGrains2DSim starts it: the
+ * continuum's points dropped, the grains as read, no contacts, zero
+ * homogenised fields.  Passes can be repeated on the result. */
+typedef struct hgx_resample hgx_resample;
+hg_status hgx_resample_create(const hgx_hybrid_scene *s, hgx_resample **out);
+/* report: dem before, after sampling, after deletion; mpm the same; zones */
+hg_status hgx_resample_run(hgx_resample *r, int64_t report[7]);
+hg_status hgx_resample_counts(const hgx_resample *r, uint32_t *npoints, uint32_t *nbodies);
+/* any pointer may be NULL */
+hg_status hgx_resample_points(const hgx_resample *r, double *q, double *v, double *m,
+                              double *volume, double *m0, double *volume0, double *weight,
+                              double *J, double *be_bar, double *hl);
+hg_status hgx_resample_bodies(const hgx_resample *r, double *q, double *v, double *M,
+                              double *M0, double *weight, double *radius, uint8_t *fixed);
+/* MPM-cell zone indicators of the last pass (cell_count[0] x cell_count[1]) */
+hg_status hgx_resample_indicators(const hgx_resample *r, int32_t *mpm_indicator);
+void hgx_resample_free(hgx_resample *r);
+
 #ifdef __cplusplus
 }
 #endif

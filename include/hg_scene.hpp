@@ -94,6 +94,9 @@ struct SimulationState {
   double near_earth_gravity[2] = {0, 0};
   double alpha = 0;  // PIC/FLIP blend
   int basis = HG_BASIS_UGIMP_LINEAR;
+  // the first rectangle's point spacing (cell width / samples per side) and
+  // density (InitialSimulationState.cpp:203-224), for the hybrid resampling
+  double initial_particle_size = 0, material_density = 0;
   // scene-file settings (InitialSimulationState: dt, dt_string, end_time)
   Rational dt;
   std::string dt_string;

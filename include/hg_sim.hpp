@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "hg.h"
+#include "hg_resample.hpp"
 #include "hg_scene.hpp"
 
 namespace hg {
@@ -163,9 +164,12 @@ class RigidBody2DSim {
 // ---- hybrid -------------------------------------------------------------------
 
 // HybridGrains2DSim (hybridgrains2dnew/HybridGrains2DSim.h:15-76): one
-// continuum and one grain scene.  The grains' mass weights come from the
-// caller (the reference's zone resampling sets them; that resampling is not
-// part of the step).
+// continuum and one grain scene.  Without a coupling_force element the
+// grains' mass weights come from the caller.  With one, the zone resampling
+// runs as the reference runs it (hg_resample.hpp): the continuum's points
+// are dropped and re-seeded, the constructor performs initializeAvoidAVoid
+// (two resampling passes), and every avoidAVoidFreq + 1 steps the state is
+// downloaded, resampled on the host and stepped on from new device handles.
 class HybridGrains2DSim {
  public:
   HybridGrains2DSim(const HybridIntegratorSettings &settings, const RigidBody2DScene &discrete,
@@ -189,17 +193,33 @@ class HybridGrains2DSim {
   void clearHomogenized();
   // Continuum, grains with their contact history, and the homogenisation
   // accumulators (hg_hybrid_snapshot / hg_hybrid_restore).
+  // With resampling on, the blob also carries the resampler's host state
+  // (counter, random streams, zones, reference masses and weights).
   std::vector<std::uint8_t> snapshot();
   void restore(const std::vector<std::uint8_t> &blob);
   hg_hybrid *handle() const { return h_; }
+  bool resamplingEnabled() const { return rs_ != nullptr; }
+  // counts of the last resampling pass, and the number of passes so far
+  ResampleReport lastResample() const;
+  long resampleCount() const;
 
  private:
+  struct Resampler;
+  void rebuildHandle();
+  void pullState();
+  void pushHomogenized();
+  void poormanCall(bool after_step);
+  void initializeAvoidAVoid();
+
   Device &dev_;
   HybridIntegratorSettings settings_;
   RigidBody2DState discrete_;
   SimulationState continuum_;
+  PenaltyImpactFrictionMapSettings penalty_;
+  int precision_ = HG_FP64;
   bool host_stale_ = false;
   hg_hybrid *h_ = nullptr;
+  std::unique_ptr<Resampler> rs_;
 };
 
 }  // namespace hg

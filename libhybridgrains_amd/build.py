@@ -38,7 +38,7 @@ HIP_SOURCES = ["hg_ctx.hip", "hg_sort.hip", "hg_mpm.hip", "hg_dem.hip", "hg_hybr
                "hg_snapshot.hip"]
 
 HOST_LIB = PKG / "libhg_host.so"
-HOST_SOURCES = ["hg_xml.cpp", "hg_scene.cpp", "hg_sim.cpp", "hg_host_capi.cpp"]
+HOST_SOURCES = ["hg_xml.cpp", "hg_scene.cpp", "hg_sim.cpp", "hg_resample.cpp", "hg_host_capi.cpp"]
 CLIS = ["mpmgrains2d_cli", "rigidbody2d_cli", "hybridgrains2d_cli"]
 CXX = os.environ.get("CXX", "g++")
 ROCM_LIB = "/opt/rocm/lib"

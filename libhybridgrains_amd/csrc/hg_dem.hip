@@ -948,6 +948,124 @@ extern "C" hg_status hg_dem_download_contacts(hg_dem *h, uint32_t scene, uint32_
   return download_contacts_t<float>(h, scene, max, type, i, j, delta_s, count);
 }
 
+template <typename T>
+static hg_status contact_forces_t(hg_dem *h, uint32_t scene, uint32_t max, double *force,
+                                  double *point, uint32_t *count) {
+  const SceneDev &sc = h->scenes[scene];
+  const uint32_t n = sc.b_cnt, off = sc.b_off;
+  const SlotsRaw &s = h->slots[h->cur];
+  std::vector<uint32_t> okey(size_t(n) * kMaxOwned);
+  std::vector<T> oF(size_t(n) * kMaxOwned * 2);
+  std::vector<double> cp(h->cpoint ? size_t(n) * kMaxOwned * 2 : 0);
+  std::vector<uint8_t> ocnt(n), pact(size_t(n) * kMaxPlanes);
+  if (n) {
+    HG_HIP(h->ctx, hipMemcpy(okey.data(), s.okey + size_t(off) * kMaxOwned, okey.size() * 4, hipMemcpyDeviceToHost));
+    HG_HIP(h->ctx, hipMemcpy(ocnt.data(), s.ocnt + off, n, hipMemcpyDeviceToHost));
+    HG_HIP(h->ctx, hipMemcpy(pact.data(), s.pact + size_t(off) * kMaxPlanes, pact.size(), hipMemcpyDeviceToHost));
+    HG_HIP(h->ctx, hipMemcpy(oF.data(), static_cast<T *>(h->f.oF) + size_t(off) * kMaxOwned * 2, oF.size() * sizeof(T), hipMemcpyDeviceToHost));
+    if (!cp.empty())
+      HG_HIP(h->ctx, hipMemcpy(cp.data(), h->cpoint + size_t(off) * kMaxOwned * 2, cp.size() * 8, hipMemcpyDeviceToHost));
+  }
+  uint32_t c = 0;
+  auto emit = [&](double f0, double f1, double p0, double p1) {
+    if (c < max) {
+      if (force) {
+        force[2 * c] = f0;
+        force[2 * c + 1] = f1;
+      }
+      if (point) {
+        point[2 * c] = p0;
+        point[2 * c + 1] = p1;
+      }
+    }
+    ++c;
+  };
+  // the order of hg_dem_download_contacts
+  for (uint32_t i = 0; i < n; ++i)
+    for (int k = 0; k < sc.nplanes; ++k)
+      if (pact[size_t(i) * kMaxPlanes + k]) emit(0.0, 0.0, 0.0, 0.0);
+  for (int kin = 0; kin < 2; ++kin)
+    for (uint32_t i = 0; i < n; ++i)
+      for (int q = 0; q < ocnt[i]; ++q) {
+        const size_t sl = size_t(i) * kMaxOwned + q;
+        if (((okey[sl] & kKinematic) != 0) != (kin == 1)) continue;
+        emit(double(oF[2 * sl]), double(oF[2 * sl + 1]), cp.empty() ? 0.0 : cp[2 * sl],
+             cp.empty() ? 0.0 : cp[2 * sl + 1]);
+      }
+  *count = c;
+  return HG_OK;
+}
+
+extern "C" hg_status hg_dem_download_contact_forces(hg_dem *h, uint32_t scene, uint32_t max,
+                                                    double *force, double *point,
+                                                    uint32_t *count) {
+  if (!h || scene >= h->nscenes || !count) return HG_ERR_INVALID;
+  HG_TRY(dem_check(h));
+  if (h->precision == HG_FP64) return contact_forces_t<double>(h, scene, max, force, point, count);
+  return contact_forces_t<float>(h, scene, max, force, point, count);
+}
+
+template <typename T>
+static hg_status upload_contacts_t(hg_dem *h, uint32_t scene, uint32_t count, const int32_t *type,
+                                   const int32_t *ci, const int32_t *cj, const double *ds) {
+  const SceneDev &sc = h->scenes[scene];
+  const uint32_t n = sc.b_cnt, off = sc.b_off;
+  std::vector<uint32_t> okey(size_t(n) * kMaxOwned, 0u);
+  std::vector<T> ods(size_t(n) * kMaxOwned * 2, T(0)), pds(size_t(n) * kMaxPlanes * 2, T(0));
+  std::vector<uint8_t> ocnt(n, 0), pact(size_t(n) * kMaxPlanes, 0);
+  for (uint32_t c = 0; c < count; ++c) {
+    const int32_t t = type[c], i = ci[c], j = cj[c];
+    if (i < 0 || uint32_t(i) >= n)
+      return hg_fail(h->ctx, HG_ERR_INVALID, "contact owner out of range");
+    if (t == 0) {
+      if (j < 0 || j >= sc.nplanes) return hg_fail(h->ctx, HG_ERR_INVALID, "contact plane out of range");
+      const size_t sl = size_t(i) * kMaxPlanes + size_t(j);
+      pact[sl] = 1;
+      pds[2 * sl] = T(ds[2 * c]);
+      pds[2 * sl + 1] = T(ds[2 * c + 1]);
+    } else if (t == 1 || t == 2) {
+      if (j < 0 || uint32_t(j) >= n || (t == 1 && j <= i))
+        return hg_fail(h->ctx, HG_ERR_INVALID, "contact partner out of range");
+      if (ocnt[i] >= kMaxOwned)
+        return hg_fail(h->ctx, HG_ERR_UNSUPPORTED, "more than 8 contacts on one body");
+      const size_t sl = size_t(i) * kMaxOwned + ocnt[i]++;
+      okey[sl] = uint32_t(j) | (t == 2 ? kKinematic : 0u);
+      ods[2 * sl] = T(ds[2 * c]);
+      ods[2 * sl + 1] = T(ds[2 * c + 1]);
+    } else {
+      return hg_fail(h->ctx, HG_ERR_INVALID, "unknown contact type");
+    }
+  }
+  // each body's contacts sorted by partner key, as the step keeps them
+  for (uint32_t i = 0; i < n; ++i) {
+    const size_t b = size_t(i) * kMaxOwned;
+    for (int a = 1; a < ocnt[i]; ++a)
+      for (int k = a; k > 0 && okey[b + k - 1] > okey[b + k]; --k) {
+        std::swap(okey[b + k - 1], okey[b + k]);
+        std::swap(ods[2 * (b + k - 1)], ods[2 * (b + k)]);
+        std::swap(ods[2 * (b + k - 1) + 1], ods[2 * (b + k) + 1]);
+      }
+  }
+  const SlotsRaw &s = h->slots[h->cur];
+  if (n) {
+    HG_HIP(h->ctx, hipMemcpy(s.okey + size_t(off) * kMaxOwned, okey.data(), okey.size() * 4, hipMemcpyHostToDevice));
+    HG_HIP(h->ctx, hipMemcpy(static_cast<T *>(s.ods) + size_t(off) * kMaxOwned * 2, ods.data(), ods.size() * sizeof(T), hipMemcpyHostToDevice));
+    HG_HIP(h->ctx, hipMemcpy(s.ocnt + off, ocnt.data(), n, hipMemcpyHostToDevice));
+    HG_HIP(h->ctx, hipMemcpy(s.pact + size_t(off) * kMaxPlanes, pact.data(), pact.size(), hipMemcpyHostToDevice));
+    HG_HIP(h->ctx, hipMemcpy(static_cast<T *>(s.pds) + size_t(off) * kMaxPlanes * 2, pds.data(), pds.size() * sizeof(T), hipMemcpyHostToDevice));
+  }
+  return HG_OK;
+}
+
+extern "C" hg_status hg_dem_upload_contacts(hg_dem *h, uint32_t scene, uint32_t count,
+                                            const int32_t *type, const int32_t *i,
+                                            const int32_t *j, const double *delta_s) {
+  if (!h || scene >= h->nscenes || (count && (!type || !i || !j || !delta_s))) return HG_ERR_INVALID;
+  HG_HIP(h->ctx, hipStreamSynchronize(h->stream));
+  if (h->precision == HG_FP64) return upload_contacts_t<double>(h, scene, count, type, i, j, delta_s);
+  return upload_contacts_t<float>(h, scene, count, type, i, j, delta_s);
+}
+
 extern "C" hg_status hg_dem_keep_force(hg_dem *h, int32_t keep) {
   if (!h) return HG_ERR_INVALID;
   HG_HIP(h->ctx, hipStreamSynchronize(h->stream));

@@ -832,6 +832,42 @@ extern "C" hg_status hg_hybrid_download_homogenized(hg_hybrid *h, double *stress
   return HG_OK;
 }
 
+// Sets the homogenised accumulators (the layout hg_hybrid_download_homogenized
+// returns; stress must be symmetric, its (1,0) entry is not kept).
+// Homogenisation buffers are allocated if they are not yet.
+extern "C" hg_status hg_hybrid_upload_homogenized(hg_hybrid *h, const double *stress,
+                                                  const double *velocity) {
+  if (!h) return HG_ERR_INVALID;
+  HG_TRY(hg_hybrid_sync(h));
+  if (!h->hg.stress) {
+    const bool s = h->homog_stress, v = h->homog_vel;
+    HG_TRY(hg_hybrid_set_homogenize(h, 1, 1));
+    h->homog_stress = s;
+    h->homog_vel = v;
+  }
+  const hgmpm::SceneDev &sc = h->mpm->scenes[0];
+  const size_t nx = size_t(sc.nnx), ny = size_t(sc.nny), G = h->mpm->nnode;
+  std::vector<double> st(3 * G, 0.0), ve(2 * G, 0.0);
+  HG_HIP(h->ctx, hipMemcpy(st.data(), h->hg.stress, 3 * G * 8, hipMemcpyDeviceToHost));
+  HG_HIP(h->ctx, hipMemcpy(ve.data(), h->hg.vel, 2 * G * 8, hipMemcpyDeviceToHost));
+  for (size_t iy = 0; iy < ny; ++iy)
+    for (size_t ix = 0; ix < nx; ++ix) {
+      const size_t r = iy * nx + ix, d = iy * size_t(sc.snx) + ix;
+      if (stress) {
+        st[3 * d] = stress[4 * r];
+        st[3 * d + 1] = stress[4 * r + 2];
+        st[3 * d + 2] = stress[4 * r + 3];
+      }
+      if (velocity) {
+        ve[2 * d] = velocity[2 * r];
+        ve[2 * d + 1] = velocity[2 * r + 1];
+      }
+    }
+  HG_HIP(h->ctx, hipMemcpy(h->hg.stress, st.data(), 3 * G * 8, hipMemcpyHostToDevice));
+  HG_HIP(h->ctx, hipMemcpy(h->hg.vel, ve.data(), 2 * G * 8, hipMemcpyHostToDevice));
+  return HG_OK;
+}
+
 extern "C" hg_status hg_hybrid_estimate_bytes(const hg_mpm_desc *mpm, const hg_dem_desc *dem,
                                               uint64_t *bytes) {
   if (!mpm || !dem || !bytes) return HG_ERR_INVALID;

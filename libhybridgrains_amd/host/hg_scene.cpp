@@ -363,6 +363,10 @@ static bool read_mpm_root(const xml::Node &root, SimulationState &st, std::strin
 
   st.physics_grid.setDimensions(gmin, gmax, cw);
   if (rects.empty()) return set_err(error, "Provide at least one rectangle region.");
+  // InitialSimulationState::generateSimulationState (:203-224): the first
+  // rectangle sets the particle size and density the resampling uses
+  st.initial_particle_size = st.physics_grid.cell_width / double(rects[0].npoints_per_dim);
+  st.material_density = rects[0].rho;
   return rasterizeRectangles(rects, st, error);
 }
 

@@ -3,8 +3,10 @@
 // the hybrid scene and the two scenes it names, step with
 // HybridGrains2DSim::stepSystem, report.
 //
-// The zone resampling that assigns mass weights is not part of the step; this
-// driver gives every grain weight 1.
+// Grains start with weight 1.  A scene with a coupling_force element runs
+// the zone resampling as the reference does (initializeAvoidAVoid, then every
+// avoidAVoidFreq + 1 steps), which re-seeds the continuum and re-weights both
+// sides.
 #include <algorithm>
 #include <cstdio>
 
@@ -52,8 +54,14 @@ int main(int argc, char **argv) {
       return 1;
     if (o.report_every > 0) {
       const auto c = sim.coupledGrains();
-      std::printf("step %ld t %.9g coupled_grains %ld\n", it, double(it) * dt,
-                  static_cast<long>(std::count(c.begin(), c.end(), 1)));
+      const hg::RigidBody2DState &d = sim.discreteState();
+      const hg::SimulationState &cs = sim.continuumState();
+      double mass = 0.0;
+      for (double m : cs.material_points.m) mass += m;
+      std::printf("step %ld t %.9g coupled_grains %ld grains %zu points %zu resamplings %ld "
+                  "point_mass %.9g\n",
+                  it, double(it) * dt, static_cast<long>(std::count(c.begin(), c.end(), 1)),
+                  d.nbodies(), cs.material_points.npoints(), sim.resampleCount(), mass);
     }
   }
   const hg::RigidBody2DState &d = sim.discreteState();
