@@ -161,6 +161,9 @@ def lib() -> C.CDLL:
         "hg_dem_download_bodies": ([vp, C.c_uint32, _dp, _dp], C.c_int32),
         "hg_dem_download_contacts": (
             [vp, C.c_uint32, C.c_uint32, _ip, _ip, _ip, _dp, C.POINTER(C.c_uint32)], C.c_int32),
+        "hg_dem_download_contact_forces": (
+            [vp, C.c_uint32, C.c_uint32, _dp, _dp, C.POINTER(C.c_uint32)], C.c_int32),
+        "hg_dem_upload_contacts": ([vp, C.c_uint32, C.c_uint32, _ip, _ip, _ip, _dp], C.c_int32),
         "hg_dem_keep_force": ([vp, C.c_int32], C.c_int32),
         "hg_dem_download_force": ([vp, C.c_uint32, _dp], C.c_int32),
         "hg_dem_set_profiling": ([vp, C.c_int32], C.c_int32),

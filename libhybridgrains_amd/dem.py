@@ -154,6 +154,30 @@ class GpuDem:
             C.byref(cnt)))
         return {"type": t, "i": i, "j": j, "delta_s": ds}
 
+    def upload_contacts(self, c: dict, scene: int = 0) -> None:
+        """Replace the contact history with contacts in contacts()' form
+        (hg_dem_upload_contacts)."""
+        t = np.ascontiguousarray(c["type"], np.int32)
+        i = np.ascontiguousarray(c["i"], np.int32)
+        j = np.ascontiguousarray(c["j"], np.int32)
+        ds = np.ascontiguousarray(c["delta_s"], np.float64).reshape(-1, 2)
+        self.ctx.check(self.L.hg_dem_upload_contacts(self._h, scene, len(t), ptr(t, C.c_int32),
+                                                      ptr(i, C.c_int32), ptr(j, C.c_int32),
+                                                      ptr(ds)))
+
+    def contact_forces(self, scene: int = 0) -> dict:
+        """The last step's force on each contact's owner and its contact
+        point, in contacts()' order (hg_dem_download_contact_forces)."""
+        cnt = C.c_uint32()
+        self.ctx.check(self.L.hg_dem_download_contact_forces(self._h, scene, 0, None, None,
+                                                             C.byref(cnt)))
+        n = cnt.value
+        f = np.zeros((n, 2))
+        p = np.zeros((n, 2))
+        self.ctx.check(self.L.hg_dem_download_contact_forces(self._h, scene, n, ptr(f), ptr(p),
+                                                             C.byref(cnt)))
+        return {"force": f, "point": p}
+
     def keep_force(self, keep: bool) -> None:
         self.ctx.check(self.L.hg_dem_keep_force(self._h, int(bool(keep))))
 

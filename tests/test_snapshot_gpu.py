@@ -66,6 +66,37 @@ def test_dem_resume_keeps_the_contact_history(hg_ctx):
     assert not np.array_equal(fresh.state()["v"], sa["v"])
 
 
+def test_uploaded_contact_history_continues_the_run(hg_ctx):
+    # hg_dem_upload_contacts: a handle made from another run's positions and
+    # velocities, given that run's contact history, continues bit for bit
+    s = scenes.dem_column(n_bodies=2000, seed=5)
+    s.v[:, :2] = np.random.default_rng(5).normal(scale=0.2, size=(s.nbodies, 2))
+    a = GpuDem(s, HG_FP64, ctx=hg_ctx)
+    a.flow(DT, 60)
+    st, ca = a.state(), a.contacts()
+    assert len(ca["i"]) > 300 and (ca["type"] == 0).any() and (ca["type"] == 1).any()
+    qv = s.copy()
+    qv.q, qv.v = st["q"], st["v"]
+    b = GpuDem(qv, HG_FP64, ctx=hg_ctx)
+    b.upload_contacts(ca)
+    cb = b.contacts()
+    for k in ca:
+        assert np.array_equal(ca[k], cb[k]), k
+    a.flow(DT, 25)
+    b.flow(DT, 25)
+    sa, sb = a.state(), b.state()
+    assert np.array_equal(sa["q"], sb["q"]) and np.array_equal(sa["v"], sb["v"])
+    # the owners' forces are those of the contacts' last step
+    f = a.contact_forces()
+    assert f["force"].shape == (len(a.contacts()["i"]), 2)
+    assert np.abs(f["force"][a.contacts()["type"] == 1]).max() > 0
+    # ill-formed histories are refused
+    bad = {k: v.copy() for k, v in ca.items()}
+    bad["j"][np.nonzero(bad["type"] == 1)[0][0]] = -1
+    with pytest.raises(HgError):
+        b.upload_contacts(bad)
+
+
 def test_hybrid_resume_with_homogenization(hg_ctx):
     m, d, w = scenes.hybrid_column(seed=3)
     d.v[:, :2] = np.random.default_rng(3).normal(scale=0.3, size=(d.nbodies, 2))
