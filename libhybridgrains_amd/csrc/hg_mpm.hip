@@ -551,13 +551,26 @@ struct TileListArgs {
   uint32_t *tile_list, *tile_count, *next_count;
   uint32_t ntiles;
   int32_t tw, th, S;
+  // the sort's scatter, run by the workgroups past the tile list's
+  const uint32_t *key, *slot;
+  uint32_t *perm, *pinv;
+  uint32_t npoints;
 };
 
-// One wave per tile: lane r checks apron row r (a row's cells are
-// consecutive keys, so two cell starts give its point count).  Four tiles per
-// workgroup; the workgroup appends its active tiles in ascending order.
+// Workgroups [0, ceil(ntiles / 4)) list the active tiles, one wave per tile:
+// lane r checks apron row r (a row's cells are consecutive keys, so two cell
+// starts give its point count), and the workgroup appends its active tiles in
+// ascending order.  The workgroups after them scatter the sorted points into
+// perm and pinv, 256 points each (both halves read only the scan's
+// cell_start, so one launch does both).
 __global__ __launch_bounds__(256) void mpm_tile_list_kernel(TileListArgs a) {
   __shared__ uint32_t s_flag[4];
+  const uint32_t ntb = (a.ntiles + 3u) / 4u;
+  if (blockIdx.x >= ntb) {
+    const uint32_t i = (blockIdx.x - ntb) * 256u + threadIdx.x;
+    if (i < a.npoints) hg_scatter_item(a.key, a.slot, a.cell_start, a.npoints, a.perm, a.pinv, i);
+    return;
+  }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t t = blockIdx.x * 4u + uint32_t(wave);
   bool nonempty = false;
@@ -1024,7 +1037,7 @@ static void launch_bin_p(hg_mpm *h) {
   }
 }
 
-static void launch_tile_list(hg_mpm *h) {
+static void launch_tile_list(hg_mpm *h, bool scatter) {
   TileListArgs a;
   a.scenes = h->d_scenes;
   a.tile_scene = h->d_tile_scene;
@@ -1037,18 +1050,25 @@ static void launch_tile_list(hg_mpm *h) {
   a.tw = h->tw();
   a.th = h->th();
   a.S = stencil_width(h->basis);
+  a.key = h->key;
+  a.slot = h->slot;
+  a.perm = h->perm;
+  a.pinv = h->pinv;
+  a.npoints = scatter ? h->npoints : 0u;
+  const uint32_t blocks = (h->ntiles + 3) / 4 + (a.npoints + 255) / 256;
   ProfScope ps(h->prof, "mpm_tile_list", h->stream);
-  hipLaunchKernelGGL(mpm_tile_list_kernel, dim3((h->ntiles + 3) / 4), dim3(256), 0, h->stream, a);
+  hipLaunchKernelGGL(mpm_tile_list_kernel, dim3(blocks), dim3(256), 0, h->stream, a);
 }
 
 // Sorts the points of the current buffer by cell (perm, pinv, cell_start)
-// and lists the active tiles.
+// and lists the active tiles: the scan, then one launch for the scatter and
+// the tile list.
 static void launch_sort(hg_mpm *h) {
-  hg_counting_sort(h->prof, h->stream, h->counts, h->ncell,
+  hg_counting_scan(h->prof, h->stream, h->counts, h->ncell,
                    h->scan_totals + size_t(h->cur) * hg_scan_totals_words(h->nscan),
-                   h->scan_totals + size_t(1 - h->cur) * hg_scan_totals_words(h->nscan), h->nscan, h->cell_start,
-                   h->key, h->slot, h->npoints, h->perm, h->err, h->pinv);
-  launch_tile_list(h);
+                   h->scan_totals + size_t(1 - h->cur) * hg_scan_totals_words(h->nscan), h->nscan,
+                   h->cell_start, h->npoints, h->err);
+  launch_tile_list(h, true);
 }
 
 static void launch_g2p_and_sort(hg_mpm *h) {
@@ -1626,7 +1646,7 @@ extern "C" hg_status hg_mpm_set_tile(hg_mpm *h, int32_t aw, int32_t ah) {
   }
   // the active-tile list of the current sort, for the new shape
   HG_HIP(h->ctx, hipMemsetAsync(h->tile_count, 0, 8, h->stream));
-  launch_tile_list(h);
+  launch_tile_list(h, false);
   HG_HIP(h->ctx, hipStreamSynchronize(h->stream));
   return HG_OK;
 }

@@ -32,7 +32,12 @@ constexpr int kScanSubTotals = 8;
 // sort's results are then wrong.  Never expected.
 constexpr int32_t kSortTotalsBit = 0x100;
 
-// Kernels: scan, scatter (2 launches).
+// Kernels: scan, scatter (2 launches).  hg_counting_scan is the first of the
+// two alone, for a caller that scatters with hg_scatter_item inside a kernel
+// of its own.
+void hg_counting_scan(KernelProfiler &prof, hipStream_t stream, uint32_t *counts, uint32_t ncell,
+                      const uint32_t *totals_in, uint32_t *totals_clear, uint32_t nscan,
+                      uint32_t *cell_start, uint32_t n, int32_t *err);
 void hg_counting_sort(KernelProfiler &prof, hipStream_t stream, uint32_t *counts, uint32_t ncell,
                       const uint32_t *totals_in, uint32_t *totals_clear, uint32_t nscan,
                       uint32_t *cell_start, const uint32_t *key, const uint32_t *slot, uint32_t n,
@@ -66,4 +71,15 @@ __device__ __forceinline__ void hg_scan_block_count(uint32_t *totals, uint32_t k
     if (lane == l) atomicAdd(&totals[bl], uint32_t(__popcll(same)));
     pending &= ~same;
   }
+}
+
+// Places item i (< n) at cell_start[key[i]] + slot[i] of perm, and that
+// position in pinv (if given).  A position past n arises only from wrong
+// totals (flagged by the scan) and is dropped.
+__device__ __forceinline__ void hg_scatter_item(const uint32_t *key, const uint32_t *slot,
+                                                const uint32_t *cell_start, uint32_t n,
+                                                uint32_t *perm, uint32_t *pinv, uint32_t i) {
+  const uint32_t pos = cell_start[key[i]] + slot[i];
+  if (pos < n) perm[pos] = i;
+  if (pinv) pinv[i] = pos < n ? pos : i;
 }

@@ -117,24 +117,26 @@ __global__ __launch_bounds__(256) void scatter_kernel(const uint32_t *key, const
                                                       const uint32_t *cell_start, uint32_t n,
                                                       uint32_t *perm, uint32_t *pinv) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t pos = cell_start[key[i]] + slot[i];
-  // out of range only if the scan's totals were wrong (flagged there)
-  if (pos < n) perm[pos] = i;
-  if (pinv) pinv[i] = pos < n ? pos : i;
+  if (i < n) hg_scatter_item(key, slot, cell_start, n, perm, pinv, i);
 }
 
 }  // namespace
 
-void hg_counting_sort(KernelProfiler &prof, hipStream_t stream, uint32_t *counts, uint32_t ncell,
+void hg_counting_scan(KernelProfiler &prof, hipStream_t stream, uint32_t *counts, uint32_t ncell,
                       const uint32_t *totals_in, uint32_t *totals_clear, uint32_t nscan,
-                      uint32_t *cell_start, const uint32_t *key, const uint32_t *slot, uint32_t n,
-                      uint32_t *perm, int32_t *err, uint32_t *pinv) {
+                      uint32_t *cell_start, uint32_t n, int32_t *err) {
   if (nscan) {
     ProfScope ps(prof, "scan", stream);
     hipLaunchKernelGGL(scan_kernel, dim3(nscan), dim3(kScanThreads), 0, stream, counts, ncell,
                        totals_in, totals_clear, cell_start, n, err);
   }
+}
+
+void hg_counting_sort(KernelProfiler &prof, hipStream_t stream, uint32_t *counts, uint32_t ncell,
+                      const uint32_t *totals_in, uint32_t *totals_clear, uint32_t nscan,
+                      uint32_t *cell_start, const uint32_t *key, const uint32_t *slot, uint32_t n,
+                      uint32_t *perm, int32_t *err, uint32_t *pinv) {
+  hg_counting_scan(prof, stream, counts, ncell, totals_in, totals_clear, nscan, cell_start, n, err);
   if (n) {
     ProfScope ps(prof, "scatter", stream);
     hipLaunchKernelGGL(scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, key, slot,
