@@ -70,8 +70,8 @@ template <int BASIS>
 __device__ __forceinline__ bool grain_stencil(const hgmpm::SceneDev &sc, double x, double y,
                                               double hl, int &lx, int &hx, int &ly, int &hy) {
   double l0, h0, l1, h1;
-  hgmpm::stencil_1d<BASIS>(x, hl, sc.min0, sc.h, l0, h0);
-  hgmpm::stencil_1d<BASIS>(y, hl, sc.min1, sc.h, l1, h1);
+  hgmpm::stencil_1d<BASIS>(x, hl, sc.min0, sc.h, sc.inv_h, l0, h0);
+  hgmpm::stencil_1d<BASIS>(y, hl, sc.min1, sc.h, sc.inv_h, l1, h1);
   const double lim = 1.0e9;
   if (!(fabs(l0) < lim && fabs(h0) < lim && fabs(l1) < lim && fabs(h1) < lim)) return false;
   lx = int(l0);

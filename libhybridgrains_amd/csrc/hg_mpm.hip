@@ -53,8 +53,8 @@ template <int BASIS>
 __device__ __forceinline__ bool point_key(const SceneDev &sc, double x, double y, double hl,
                                           int &bx, int &by, int &cx, int &cy) {
   double lx, hx, ly, hy;
-  stencil_1d<BASIS>(x, hl, sc.min0, sc.h, lx, hx);
-  stencil_1d<BASIS>(y, hl, sc.min1, sc.h, ly, hy);
+  stencil_1d<BASIS>(x, hl, sc.min0, sc.h, sc.inv_h, lx, hx);
+  stencil_1d<BASIS>(y, hl, sc.min1, sc.h, sc.inv_h, ly, hy);
   const bool ok = (lx >= 0.0) && (ly >= 0.0) && (hx <= double(sc.nnx - 1)) &&
                   (hy <= double(sc.nny - 1));
   constexpr int S = BasisTraits<BASIS>::S;
@@ -134,9 +134,9 @@ __device__ __forceinline__ uint32_t stencil_counts(double x, double y, double hl
                                                    const SceneDev &sc) {
   constexpr int S = BasisTraits<BASIS>::S;
   if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) {
-    const double cxd = ceil(((x + hl) - sc.min0) / sc.h) - double(bx) + 1.0;
+    const double cxd = ceil(div_rn((x + hl) - sc.min0, sc.h, sc.inv_h)) - double(bx) + 1.0;
     const int cntx = (cxd >= 1.0 && cxd <= double(S)) ? int(cxd) : S;
-    const double cyd = ceil(((y + hl) - sc.min1) / sc.h) - double(by) + 1.0;
+    const double cyd = ceil(div_rn((y + hl) - sc.min1, sc.h, sc.inv_h)) - double(by) + 1.0;
     const int cnty = (cyd >= 1.0 && cyd <= double(S)) ? int(cyd) : S;
     return uint32_t(cntx) | (uint32_t(cnty) << 4);
   } else {
@@ -234,7 +234,14 @@ __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
   const uint32_t ntile_list = *args.tile_count;
   const int t = threadIdx.x;
   const int lx = t % TW, ly = t / TW;
-  for (uint32_t li = blockIdx.x; li < ntile_list; li += gridDim.x) {
+  // Workgroup b runs on XCD b % 8 (the grid is a multiple of 8).  Each XCD
+  // takes one contiguous eighth of the list, whose tiles are mostly
+  // neighbours in the grid, so the apron points two tiles share are fetched
+  // from HBM into that XCD's L2 once.
+  const uint32_t per_xcd = (ntile_list + 7u) >> 3, xcd = blockIdx.x & 7u;
+  for (uint32_t j = blockIdx.x >> 3; j < per_xcd; j += gridDim.x >> 3) {
+    const uint32_t li = xcd * per_xcd + j;
+    if (li >= ntile_list) break;
     const uint32_t tile = args.tile_list[li];
     const uint32_t sid = args.tile_scene[tile];
     const SceneDev &sc = args.scenes[sid];
@@ -557,23 +564,24 @@ struct TileListArgs {
   uint32_t npoints;
 };
 
-// Workgroups [0, ceil(ntiles / 4)) list the active tiles, one wave per tile:
-// lane r checks apron row r (a row's cells are consecutive keys, so two cell
-// starts give its point count), and the workgroup appends its active tiles in
-// ascending order.  The workgroups after them scatter the sorted points into
-// perm and pinv, 256 points each (both halves read only the scan's
-// cell_start, so one launch does both).
+// Workgroups [0, ceil(ntiles / 256)) list the active tiles, one lane per
+// tile: the lane checks the tile's apron rows (a row's cells are consecutive
+// keys, so two cell starts give its point count), and the workgroup appends
+// its active tiles in ascending order with one atomic.  The workgroups after
+// them scatter the sorted points into perm and pinv, 256 points each (both
+// parts read only the scan's cell_start, so one launch does both).
 __global__ __launch_bounds__(256) void mpm_tile_list_kernel(TileListArgs a) {
-  __shared__ uint32_t s_flag[4];
-  const uint32_t ntb = (a.ntiles + 3u) / 4u;
+  __shared__ uint32_t s_wave[4];
+  __shared__ uint32_t s_base;
+  const uint32_t ntb = (a.ntiles + 255u) / 256u;
   if (blockIdx.x >= ntb) {
     const uint32_t i = (blockIdx.x - ntb) * 256u + threadIdx.x;
     if (i < a.npoints) hg_scatter_item(a.key, a.slot, a.cell_start, a.npoints, a.perm, a.pinv, i);
     return;
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t t = blockIdx.x * 4u + uint32_t(wave);
-  bool nonempty = false;
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  bool active = false;
   if (t < a.ntiles) {
     const SceneDev &sc = a.scenes[a.tile_scene[t]];
     const uint32_t lt = t - sc.tile_off;
@@ -581,29 +589,34 @@ __global__ __launch_bounds__(256) void mpm_tile_list_kernel(TileListArgs a) {
     const int y0 = int(lt / uint32_t(sc.tiles_x)) * a.th;
     const int ax0 = x0 - (a.S - 1), ay0 = y0 - (a.S - 1);
     const int aw = a.tw + a.S - 1, ah = a.th + a.S - 1;
-    const int ry = ay0 + lane;
-    if (lane < ah && ry >= 0 && ry < sc.nny) {
-      const uint32_t *cs = a.cell_start + sc.cell_off;
-      const int c0 = min(max(ax0, 0), sc.nnx), c1 = min(max(ax0 + aw, 0), sc.nnx);
+    const uint32_t *cs = a.cell_start + sc.cell_off;
+    const uint32_t c0 = uint32_t(min(max(ax0, 0), sc.nnx));
+    const uint32_t c1 = uint32_t(min(max(ax0 + aw, 0), sc.nnx));
+    const int r0 = max(ay0, 0), r1 = min(ay0 + ah, sc.nny);
+    // every row's loads issue together (no early exit)
+    uint32_t nonempty = 0;
+#pragma unroll 8
+    for (int ry = r0; ry < r1; ++ry) {
       const uint32_t row = uint32_t(ry) * uint32_t(sc.snx);
-      nonempty = cs[row + uint32_t(c1)] > cs[row + uint32_t(c0)];
+      nonempty |= uint32_t(cs[row + c1] > cs[row + c0]);
     }
+    active = nonempty != 0u;
+    a.tile_active[t] = active ? 1 : 0;
   }
-  const bool any = __ballot(nonempty) != 0ull;
   // the other parity's counter, for the next list (nothing reads it now)
   if (blockIdx.x == 0 && threadIdx.x == 0) *a.next_count = 0u;
-  if (lane == 0) {
-    s_flag[wave] = any ? 1u : 0u;
-    if (t < a.ntiles) a.tile_active[t] = any ? 1 : 0;
-  }
+  const uint64_t bal = __ballot(active);
+  if (lane == 0) s_wave[wave] = uint32_t(__popcll(bal));
   __syncthreads();
   if (threadIdx.x == 0) {
-    const uint32_t tot = s_flag[0] + s_flag[1] + s_flag[2] + s_flag[3];
-    if (tot) {
-      uint32_t o = atomicAdd(a.tile_count, tot);
-      for (int w = 0; w < 4; ++w)
-        if (s_flag[w]) a.tile_list[o++] = blockIdx.x * 4u + uint32_t(w);
-    }
+    const uint32_t tot = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+    s_base = tot ? atomicAdd(a.tile_count, tot) : 0u;
+  }
+  __syncthreads();
+  if (active) {
+    uint32_t o = s_base + uint32_t(__popcll(bal & ((1ull << lane) - 1ull)));
+    for (int w = 0; w < wave; ++w) o += s_wave[w];
+    a.tile_list[o] = t;
   }
 }
 
@@ -933,7 +946,7 @@ template <> struct P2GCap<double> { static constexpr int value = 192; };
 
 // Workgroups of the P2G launch: enough to fill the chip several times over;
 // each takes active tiles from the list until it is exhausted.
-static uint32_t p2g_grid(const hg_mpm *h) { return std::max(1u, std::min(h->ntiles, 2048u)); }
+static uint32_t p2g_grid(const hg_mpm *h) { return hg_pad8(std::max(1u, std::min(h->ntiles, 2048u))); }
 
 template <typename T, int BASIS, int TW, int TH>
 static void launch_p2g_t(hg_mpm *h, uint32_t store) {
@@ -1055,7 +1068,7 @@ static void launch_tile_list(hg_mpm *h, bool scatter) {
   a.perm = h->perm;
   a.pinv = h->pinv;
   a.npoints = scatter ? h->npoints : 0u;
-  const uint32_t blocks = (h->ntiles + 3) / 4 + (a.npoints + 255) / 256;
+  const uint32_t blocks = (h->ntiles + 255) / 256 + (a.npoints + 255) / 256;
   ProfScope ps(h->prof, "mpm_tile_list", h->stream);
   hipLaunchKernelGGL(mpm_tile_list_kernel, dim3(blocks), dim3(256), 0, h->stream, a);
 }

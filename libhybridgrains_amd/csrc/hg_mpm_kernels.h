@@ -73,21 +73,38 @@ template <typename T> struct Vec4T;
 template <> struct Vec4T<double> { using type = double4; };
 template <> struct Vec4T<float> { using type = float4; };
 
+// a / h rounded to nearest (the IEEE quotient), given inv_h = 1 / h rounded
+// to nearest, in five FP64 operations instead of a division sequence.
+// q = a inv_h is within two ulps of a / h; each step q += (a - q h) inv_h,
+// whose remainder the fused multiply-add computes exactly, brings it closer.
+// After the first step q is within an ulp, and Markstein's theorem (P.
+// Markstein, IBM J. Res. Dev. 34(1), 1990; Muller et al., Handbook of
+// Floating-Point Arithmetic, the chapter on FMA division) makes the second
+// the correctly rounded quotient.  A zero quotient may come out as +0 where
+// the division gives -0, which floor, ceil and every comparison treat alike.
+// Overflowed or non-finite operands give NaN where the division may give an
+// infinity; both fail the callers' range checks.
+__device__ __forceinline__ double div_rn(double a, double h, double inv_h) {
+  double q = a * inv_h;
+  q = fma(fma(-q, h, a), inv_h, q);
+  return fma(fma(-q, h, a), inv_h, q);
+}
+
 // Stencil of one axis: first node lo and last node hi (inclusive), as double
 // so an off-grid or NaN coordinate can be detected before any integer cast.
 // uGIMPLinearBasisFunctions.cpp:68-90, LinearBasisFunctions.cpp:48-60,
-// CubicBasisFunctions.cpp:61-75.
+// CubicBasisFunctions.cpp:61-75.  The divisions by h are div_rn's.
 template <int BASIS>
 __device__ __forceinline__ void stencil_1d(double x, double hl, double gmin, double h,
-                                           double &lo, double &hi) {
+                                           double inv_h, double &lo, double &hi) {
   if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) {
-    lo = floor(((x - hl) - gmin) / h);
-    hi = ceil(((x + hl) - gmin) / h);
+    lo = floor(div_rn((x - hl) - gmin, h, inv_h));
+    hi = ceil(div_rn((x + hl) - gmin, h, inv_h));
   } else if constexpr (BASIS == HG_BASIS_LINEAR) {
-    lo = floor((x - gmin) / h);
+    lo = floor(div_rn(x - gmin, h, inv_h));
     hi = lo + 1.0;
   } else {
-    lo = floor((x - gmin) / h) - 1.0;
+    lo = floor(div_rn(x - gmin, h, inv_h)) - 1.0;
     hi = lo + 3.0;
   }
 }
