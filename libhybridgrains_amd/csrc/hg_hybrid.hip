@@ -505,11 +505,14 @@ static void launch_hybrid_steps(hg_hybrid *h, double dt, uint32_t n, bool concur
   dem_step();
   for (uint32_t k = 0; k < n; ++k) {
     HybArgs a = make_args(h);
-    hg_mpm_launch_coupled_p2g(h->mpm, dt);
     // grain_mass needs this step's DEM step (post-step velocities) and, queued
     // before it on the grain stream, the previous step's finish (pre-step
-    // positions, cleared node lists)
+    // positions, cleared node lists).  The join comes before the P2G: the
+    // P2G's tiles fill every CU in a single wave of workgroups, so a grain
+    // kernel still holding a CU when it starts would delay some of its tiles
+    // by a whole tile time.
     order(h->join_ev, side, s);
+    hg_mpm_launch_coupled_p2g(h->mpm, dt);
     {
       ProfScope ps(prof, "hyb_grain_mass", s);
       hipLaunchKernelGGL((hyb_grain_mass<T, BASIS>), gb, tb, 0, s, a);
