@@ -316,6 +316,16 @@ hg_status hg_hybrid_upload_homogenized(hg_hybrid *h, const double *stress, const
 /* ResamplingTools::clearHomogenizedStress / clearHomogenizedVelocity
  * (ResamplingTools.cpp:736-742, :810-816). */
 hg_status hg_hybrid_clear_homogenized(hg_hybrid *h);
+/* The continuum grid's coupling weights (PhysicsGrid::homog_factor), which
+ * HybridGrains2DSim's node-node step saves every step (HybridGrains2DSim.cpp:
+ * 169-197): 1 at nodes with continuum mass after the step's P2G, 0 elsewhere,
+ * 0.5 at nodes that hold grain mass.  Output only; no step reads them.  While
+ * on, every step records them (one pass over the nodes on the continuum
+ * stream, about 2 us at 2M nodes); they start at 1 everywhere, as the
+ * reference's grid does (PhysicsGrid.cpp:52).  The download returns the last
+ * recorded values, nny x nnx row-major; HG_ERR_INVALID if never switched on. */
+hg_status hg_hybrid_set_homog_factor(hg_hybrid *h, int32_t on);
+hg_status hg_hybrid_download_homog_factor(hg_hybrid *h, double *out);
 hg_status hg_hybrid_estimate_bytes(const hg_mpm_desc *mpm, const hg_dem_desc *dem,
                                    uint64_t *bytes);
 hg_status hg_hybrid_device_bytes(hg_hybrid *h, uint64_t *bytes);

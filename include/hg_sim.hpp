@@ -191,6 +191,10 @@ class HybridGrains2DSim {
   std::vector<double> homogenizedVelocity();
   // ResamplingTools::clearHomogenizedStress / clearHomogenizedVelocity
   void clearHomogenized();
+  // PhysicsGrid::homog_factor as the last step saved it
+  // (HybridGrains2DSim.cpp:169-197): 1 at nodes with continuum mass, 0.5 at
+  // nodes with grain mass, else 0; 1 everywhere before the first step.
+  std::vector<double> homogFactor();
   // Continuum, grains with their contact history, and the homogenisation
   // accumulators (hg_hybrid_snapshot / hg_hybrid_restore).
   // With resampling on, the blob also carries the resampler's host state

@@ -181,6 +181,8 @@ def lib() -> C.CDLL:
         "hg_hybrid_download_coupling": ([vp, C.POINTER(C.c_uint8)], C.c_int32),
         "hg_hybrid_set_homogenize": ([vp, C.c_int32, C.c_int32], C.c_int32),
         "hg_hybrid_download_homogenized": ([vp, _dp, _dp], C.c_int32),
+        "hg_hybrid_set_homog_factor": ([vp, C.c_int32], C.c_int32),
+        "hg_hybrid_download_homog_factor": ([vp, _dp], C.c_int32),
         "hg_hybrid_clear_homogenized": ([vp], C.c_int32),
         "hg_mpm_snapshot": ([vp, C.c_void_p, C.POINTER(C.c_uint64)], C.c_int32),
         "hg_mpm_restore": ([vp, C.c_void_p, C.c_uint64], C.c_int32),

@@ -57,6 +57,7 @@ struct HybArgs {
   GrainGrid g;
   uint32_t *count, *next_count;  // this step's list length; the next step's
   int32_t *err;
+  uint8_t *hfac;  // coupling weights per node in halves (0, 1, 2), or null
 };
 
 template <typename T>
@@ -238,6 +239,7 @@ __global__ __launch_bounds__(kThreads) void hyb_exchange(HybArgs a) {
     const double pn0 = a.gpn[2 * node] - l0, pn1 = a.gpn[2 * node + 1] - l1;
     a.gpn[2 * node] = pn0;
     a.gpn[2 * node + 1] = pn1;
+    if (a.hfac) a.hfac[n] = 1u;  // 0.5: the node holds grain mass
     a.g.vafter[2 * n] = (a.g.dmom[2 * n] + l0) / m_dem;
     a.g.vafter[2 * n + 1] = (a.g.dmom[2 * n + 1] + l1) / m_dem;
     a.g.aafter[2 * n] = l0 / m_dem;
@@ -254,6 +256,21 @@ __global__ __launch_bounds__(kThreads) void hyb_exchange(HybArgs a) {
     }
     static_cast<T4 *>(a.gva)[node] = va;
   }
+}
+
+// The continuum grid's coupling weights as HybridGrains2DSim's node-node step
+// saves them (HybridGrains2DSim.cpp:169-197), in halves: 2 (1.0) at nodes with
+// continuum mass after this step's P2G, 0 elsewhere; hyb_exchange then marks
+// the nodes that hold grain mass 1 (0.5).  Output only.  Nodes of tiles
+// without points carry no mass (their P2G values are not this step's).
+__global__ __launch_bounds__(kThreads) void hyb_homog_factor(HybArgs a) {
+  const hgmpm::SceneDev &sc = *a.msc;
+  const uint32_t idx = blockIdx.x * kThreads + threadIdx.x;
+  if (idx >= uint32_t(sc.nny) * uint32_t(sc.snx)) return;
+  const int ix = int(idx % uint32_t(sc.snx)), iy = int(idx / uint32_t(sc.snx));
+  if (ix >= sc.nnx) return;
+  const double m = node_active(a, ix, iy) ? a.gm[sc.node_off + idx] : 0.0;
+  a.hfac[idx] = m > 0.0 ? 2u : 0u;
 }
 
 // HybridCoupling2D::transferToGrain (Coupling.cpp:468-527), FLIP with alpha 1.
@@ -473,6 +490,7 @@ static HybArgs make_args(hg_hybrid *h) {
   a.count = h->g.count + h->par;
   a.next_count = h->g.count + (1 - h->par);
   a.err = h->err;
+  a.hfac = h->hfac_on ? h->hfac : nullptr;
   return a;
 }
 
@@ -516,6 +534,12 @@ static void launch_hybrid_steps(hg_hybrid *h, double dt, uint32_t n, bool concur
     {
       ProfScope ps(prof, "hyb_grain_mass", s);
       hipLaunchKernelGGL((hyb_grain_mass<T, BASIS>), gb, tb, 0, s, a);
+    }
+    if (a.hfac) {
+      const hgmpm::SceneDev &msc = h->mpm->scenes[0];
+      const uint32_t nn = uint32_t(msc.nny) * uint32_t(msc.snx);
+      ProfScope ps(prof, "hyb_homog_factor", s);
+      hipLaunchKernelGGL(hyb_homog_factor, dim3((nn + kThreads - 1) / kThreads), tb, 0, s, a);
     }
     {
       // the list holds at most S*S nodes per grain: a grid over the grains
@@ -691,7 +715,7 @@ extern "C" hg_status hg_hybrid_step_graph(hg_hybrid *h, double dt, uint32_t nste
   HG_TRY(hg_dem_step(h->dem, dt, 0));
   if (h->graph && (dt != h->graph_dt || h->mpm->aw != h->graph_aw || h->mpm->ah != h->graph_ah ||
                    h->homog_stress != h->graph_homog_stress ||
-                   h->homog_vel != h->graph_homog_vel)) {
+                   h->homog_vel != h->graph_homog_vel || h->hfac_on != h->graph_hfac_on)) {
     hipGraphExecDestroy(h->graph);
     h->graph = nullptr;
   }
@@ -726,6 +750,7 @@ extern "C" hg_status hg_hybrid_step_graph(hg_hybrid *h, double dt, uint32_t nste
       h->graph_ah = h->mpm->ah;
       h->graph_homog_stress = h->homog_stress;
       h->graph_homog_vel = h->homog_vel;
+      h->graph_hfac_on = h->hfac_on;
     }
     for (uint32_t k = 0; k < pairs; ++k) HG_HIP(h->ctx, hipGraphLaunch(h->graph, h->stream));
   }
@@ -832,6 +857,37 @@ extern "C" hg_status hg_hybrid_download_homogenized(hg_hybrid *h, double *stress
         velocity[2 * r + 1] = on ? ve[2 * d + 1] : 0.0;
       }
     }
+  return HG_OK;
+}
+
+// Coupling weights (HybridGrains2DSim.cpp:169-197) recorded by every later
+// step while on.  The buffer starts at 1.0 everywhere, as the reference's grid
+// does (PhysicsGrid.cpp:52).
+extern "C" hg_status hg_hybrid_set_homog_factor(hg_hybrid *h, int32_t on) {
+  if (!h) return HG_ERR_INVALID;
+  HG_TRY(hg_hybrid_sync(h));
+  if (on && !h->hfac) {
+    const size_t G = h->mpm->nnode;
+    HG_TRY(halloc(h, &h->hfac, G));
+    HG_HIP(h->ctx, hipMemset(h->hfac, 2, G));
+    HG_HIP(h->ctx, hipDeviceSynchronize());
+  }
+  h->hfac_on = on != 0;
+  return HG_OK;
+}
+
+// The last recorded coupling weights, nny x nnx row-major (0, 0.5 or 1).
+extern "C" hg_status hg_hybrid_download_homog_factor(hg_hybrid *h, double *out) {
+  if (!h || !out) return HG_ERR_INVALID;
+  if (!h->hfac)
+    return hg_fail(h->ctx, HG_ERR_INVALID, "coupling weights were never switched on");
+  HG_TRY(hg_hybrid_sync(h));
+  const hgmpm::SceneDev &sc = h->mpm->scenes[0];
+  const size_t nx = size_t(sc.nnx), ny = size_t(sc.nny);
+  std::vector<uint8_t> c(h->mpm->nnode);
+  HG_HIP(h->ctx, hipMemcpy(c.data(), h->hfac, c.size(), hipMemcpyDeviceToHost));
+  for (size_t iy = 0; iy < ny; ++iy)
+    for (size_t ix = 0; ix < nx; ++ix) out[iy * nx + ix] = 0.5 * double(c[iy * size_t(sc.snx) + ix]);
   return HG_OK;
 }
 

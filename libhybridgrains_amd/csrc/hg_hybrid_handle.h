@@ -51,6 +51,10 @@ struct hg_hybrid {
   hghyb::HomogGrid hg = {};
   bool homog_stress = false, homog_vel = false;
   bool graph_homog_stress = false, graph_homog_vel = false;  // as captured
+  // coupling weights per MPM node in halves (hyb_homog_factor), recorded
+  // while hfac_on
+  uint8_t *hfac = nullptr;
+  bool hfac_on = false, graph_hfac_on = false;
   hipGraphExec_t graph = nullptr;
   double graph_dt = -1.0;
   int graph_aw = 0, graph_ah = 0;  // MPM tile shape the graph was captured with

@@ -346,6 +346,8 @@ HybridGrains2DSim::HybridGrains2DSim(const HybridIntegratorSettings &settings,
   const hg_dem_desc dd = dem_desc(weighted, discrete.penalty, 1.0, precision, px, pn);
   const hg_mpm_desc md = continuum_.desc(precision);
   dev_.check(hg_hybrid_create(dev_.ctx(), &md, &dd, dem_weights.data(), &h_), "hg_hybrid_create");
+  // the reference saves the coupling weights every step
+  dev_.check(hg_hybrid_set_homog_factor(h_, 1), "hg_hybrid_set_homog_factor");
 }
 
 // New device handles from the resampled state: points and grains with their
@@ -374,6 +376,7 @@ void HybridGrains2DSim::rebuildHandle() {
              "hg_dem_upload_contacts");
   if (r.ss || r.sv)
     dev_.check(hg_hybrid_set_homogenize(h_, r.ss ? 1 : 0, r.sv ? 1 : 0), "hg_hybrid_set_homogenize");
+  dev_.check(hg_hybrid_set_homog_factor(h_, 1), "hg_hybrid_set_homog_factor");
   host_stale_ = false;
 }
 
@@ -481,6 +484,12 @@ std::vector<double> HybridGrains2DSim::homogenizedStress() {
 std::vector<double> HybridGrains2DSim::homogenizedVelocity() {
   std::vector<double> v(2 * continuum_.physics_grid.numGridPoints());
   dev_.check(hg_hybrid_download_homogenized(h_, nullptr, v.data()), "hg_hybrid_download_homogenized");
+  return v;
+}
+
+std::vector<double> HybridGrains2DSim::homogFactor() {
+  std::vector<double> v(continuum_.physics_grid.numGridPoints());
+  dev_.check(hg_hybrid_download_homog_factor(h_, v.data()), "hg_hybrid_download_homog_factor");
   return v;
 }
 

@@ -70,6 +70,21 @@ class GpuHybrid:
         self.ctx.check(self.L.hg_hybrid_download_homogenized(self._h, ptr(st), ptr(ve)))
         return {"stress": st, "velocity": ve}
 
+    def set_homog_factor(self, on: bool = True) -> None:
+        """Record the continuum grid's coupling weights in every later step,
+        as HybridGrains2DSim's node-node step saves them
+        (HybridGrains2DSim.cpp:169-197)."""
+        self.ctx.check(self.L.hg_hybrid_set_homog_factor(self._h, int(bool(on))))
+
+    def homog_factor(self) -> np.ndarray:
+        """The last recorded coupling weights (G values in the reference's
+        flat node order): 1 with continuum mass, 0.5 with grain mass, else 0."""
+        s = self.mpm_scene
+        G = (int(s.cell_count[0]) + 1) * (int(s.cell_count[1]) + 1)
+        out = np.zeros(G)
+        self.ctx.check(self.L.hg_hybrid_download_homog_factor(self._h, ptr(out)))
+        return out
+
     def clear_homogenized(self) -> None:
         self.ctx.check(self.L.hg_hybrid_clear_homogenized(self._h))
 

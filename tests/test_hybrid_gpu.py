@@ -147,6 +147,40 @@ def test_homogenization_matches_oracle(hg_ctx):
     _compare(g, o, 1e-11, 1e-10)
 
 
+def test_coupling_weights_match_oracle(hg_ctx):
+    """PhysicsGrid::homog_factor as the node-node step saves it
+    (HybridGrains2DSim.cpp:169-197): 1 with continuum mass, 0.5 with grain
+    mass, 0 elsewhere.  Both sets depend only on which weights are nonzero,
+    so the values match the restatement exactly, in eager and graph steps;
+    recording them leaves the step unchanged."""
+    from libhybridgrains_amd._lib import HgError
+    m, d, w = _agitated(9)
+    g = GpuHybrid(m, d, w, HG_FP64, ctx=hg_ctx)
+    gg = GpuHybrid(m, d, w, HG_FP64, ctx=hg_ctx)
+    c = GpuHybrid(m, d, w, HG_FP64, ctx=hg_ctx)
+    o = oracle.OracleHybrid(m, d, w)
+    with pytest.raises(HgError):
+        g.homog_factor()
+    g.set_homog_factor(True)
+    gg.set_homog_factor(True)
+    assert np.all(g.homog_factor() == 1.0)
+    for n in (1, 5):
+        g.step(DT, n)
+        gg.step(DT, n, graph=True)
+        c.step(DT, n)
+        o.step(DT, n)
+        hf = g.homog_factor()
+        assert np.array_equal(hf, o.homog_factor)
+        assert np.array_equal(gg.homog_factor(), o.homog_factor)
+    assert (hf == 0.5).sum() > 50 and (hf == 1.0).sum() > 50 and (hf == 0.0).sum() > 50
+    pg, pc = g.points(), c.points()
+    for k in pg:
+        assert_close(pg[k], pc[k], 1e-12, k)
+    g.set_homog_factor(False)
+    g.step(DT, 2)
+    assert np.array_equal(g.homog_factor(), hf)
+
+
 def test_homogenization_graph_clear_and_off(hg_ctx):
     m, d, w = _agitated(8)
     a = GpuHybrid(m, d, w, HG_FP64, ctx=hg_ctx)
