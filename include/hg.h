@@ -320,8 +320,9 @@ hg_status hg_hybrid_clear_homogenized(hg_hybrid *h);
  * HybridGrains2DSim's node-node step saves every step (HybridGrains2DSim.cpp:
  * 169-197): 1 at nodes with continuum mass after the step's P2G, 0 elsewhere,
  * 0.5 at nodes that hold grain mass.  Output only; no step reads them.  While
- * on, every step records them (one pass over the nodes on the continuum
- * stream, about 2 us at 2M nodes); they start at 1 everywhere, as the
+ * on, every step records them (one pass over the nodes on the grain stream;
+ * the 1M-point hybrid step takes about 5-6 us longer); they start at 1
+ * everywhere, as the
  * reference's grid does (PhysicsGrid.cpp:52).  The download returns the last
  * recorded values, nny x nnx row-major; HG_ERR_INVALID if never switched on. */
 hg_status hg_hybrid_set_homog_factor(hg_hybrid *h, int32_t on);

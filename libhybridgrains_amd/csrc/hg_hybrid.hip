@@ -239,7 +239,6 @@ __global__ __launch_bounds__(kThreads) void hyb_exchange(HybArgs a) {
     const double pn0 = a.gpn[2 * node] - l0, pn1 = a.gpn[2 * node + 1] - l1;
     a.gpn[2 * node] = pn0;
     a.gpn[2 * node + 1] = pn1;
-    if (a.hfac) a.hfac[n] = 1u;  // 0.5: the node holds grain mass
     a.g.vafter[2 * n] = (a.g.dmom[2 * n] + l0) / m_dem;
     a.g.vafter[2 * n + 1] = (a.g.dmom[2 * n + 1] + l1) / m_dem;
     a.g.aafter[2 * n] = l0 / m_dem;
@@ -259,18 +258,35 @@ __global__ __launch_bounds__(kThreads) void hyb_exchange(HybArgs a) {
 }
 
 // The continuum grid's coupling weights as HybridGrains2DSim's node-node step
-// saves them (HybridGrains2DSim.cpp:169-197), in halves: 2 (1.0) at nodes with
-// continuum mass after this step's P2G, 0 elsewhere; hyb_exchange then marks
-// the nodes that hold grain mass 1 (0.5).  Output only.  Nodes of tiles
+// saves them (HybridGrains2DSim.cpp:169-197), in halves: 1 (0.5) at nodes
+// holding grain mass, else 2 (1.0) at nodes with continuum mass after this
+// step's P2G, else 0.  Output only; it runs on the grain stream after the
+// exchange and before the finish clears the grain mass.  Nodes of tiles
 // without points carry no mass (their P2G values are not this step's).
 __global__ __launch_bounds__(kThreads) void hyb_homog_factor(HybArgs a) {
+  // four nodes of one row per thread (tile widths are multiples of four);
+  // blockIdx.y is the node row
   const hgmpm::SceneDev &sc = *a.msc;
-  const uint32_t idx = blockIdx.x * kThreads + threadIdx.x;
-  if (idx >= uint32_t(sc.nny) * uint32_t(sc.snx)) return;
-  const int ix = int(idx % uint32_t(sc.snx)), iy = int(idx / uint32_t(sc.snx));
-  if (ix >= sc.nnx) return;
-  const double m = node_active(a, ix, iy) ? a.gm[sc.node_off + idx] : 0.0;
-  a.hfac[idx] = m > 0.0 ? 2u : 0u;
+  const int iy = int(blockIdx.y);
+  const int ix0 = int(blockIdx.x * kThreads + threadIdx.x) * 4;
+  if (ix0 >= sc.nnx) return;
+  const uint32_t row = uint32_t(iy) * uint32_t(sc.snx);
+  const bool active = node_active(a, ix0, iy);
+  uint8_t c[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t n = row + uint32_t(ix0 + k);
+    c[k] = 0u;
+    // grain mass only reaches nodes with continuum mass, so only nodes of
+    // tiles with points are read
+    if (active && ix0 + k < sc.nnx) {
+      if (a.g.dmass[n] > 0.0) c[k] = 1u;
+      else if (a.gm[sc.node_off + n] > 0.0) c[k] = 2u;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (ix0 + k < sc.nnx) a.hfac[row + uint32_t(ix0 + k)] = c[k];
 }
 
 // HybridCoupling2D::transferToGrain (Coupling.cpp:468-527), FLIP with alpha 1.
@@ -535,12 +551,6 @@ static void launch_hybrid_steps(hg_hybrid *h, double dt, uint32_t n, bool concur
       ProfScope ps(prof, "hyb_grain_mass", s);
       hipLaunchKernelGGL((hyb_grain_mass<T, BASIS>), gb, tb, 0, s, a);
     }
-    if (a.hfac) {
-      const hgmpm::SceneDev &msc = h->mpm->scenes[0];
-      const uint32_t nn = uint32_t(msc.nny) * uint32_t(msc.snx);
-      ProfScope ps(prof, "hyb_homog_factor", s);
-      hipLaunchKernelGGL(hyb_homog_factor, dim3((nn + kThreads - 1) / kThreads), tb, 0, s, a);
-    }
     {
       // the list holds at most S*S nodes per grain: a grid over the grains
       // walks it in strides
@@ -548,6 +558,12 @@ static void launch_hybrid_steps(hg_hybrid *h, double dt, uint32_t n, bool concur
       hipLaunchKernelGGL(hyb_exchange<T>, gb, tb, 0, s, a);
     }
     order(h->fork_ev, s, side);
+    if (a.hfac) {
+      const hgmpm::SceneDev &msc = h->mpm->scenes[0];
+      const uint32_t per_row = (uint32_t(msc.nnx) + 4 * kThreads - 1) / (4 * kThreads);
+      ProfScope ps(prof, "hyb_homog_factor", side);
+      hipLaunchKernelGGL(hyb_homog_factor, dim3(per_row, uint32_t(msc.nny)), tb, 0, side, a);
+    }
     {
       ProfScope ps(prof, "hyb_transfer", side);
       hipLaunchKernelGGL((hyb_transfer<T, BASIS>), gb, tb, 0, side, a);
