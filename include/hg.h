@@ -322,9 +322,9 @@ hg_status hg_hybrid_clear_homogenized(hg_hybrid *h);
  * 0.5 at nodes that hold grain mass.  Output only; no step reads them.  While
  * on, every step records them (one pass over the nodes on the grain stream;
  * the 1M-point hybrid step takes about 5-6 us longer); they start at 1
- * everywhere, as the
- * reference's grid does (PhysicsGrid.cpp:52).  The download returns the last
- * recorded values, nny x nnx row-major; HG_ERR_INVALID if never switched on. */
+ * everywhere, as the reference's grid does (PhysicsGrid.cpp:52).  The
+ * download returns the last recorded values, nny x nnx row-major;
+ * HG_ERR_INVALID if never switched on. */
 hg_status hg_hybrid_set_homog_factor(hg_hybrid *h, int32_t on);
 hg_status hg_hybrid_download_homog_factor(hg_hybrid *h, double *out);
 hg_status hg_hybrid_estimate_bytes(const hg_mpm_desc *mpm, const hg_dem_desc *dem,
