@@ -231,7 +231,12 @@ hg_status hg_dem_step_graph(hg_dem *h, double dt, uint32_t nsteps);
  * (PenaltyImpactFrictionMap.cpp:296-599).  NULL returns to unweighted flow. */
 hg_status hg_dem_set_weights(hg_dem *h, uint32_t scene, const double *weights);
 /* Waits; HG_ERR_STATE on non-finite bodies, HG_ERR_UNSUPPORTED when a body had
- * more than 8 circle contacts (the step's results are then void). */
+ * more pair contacts than the handle's per-body capacity (the step's results
+ * are then void).  The capacity is 8, 16 or 32, sized at creation from the
+ * radii: the number of partners that fit around the largest body while no two
+ * of them overlap by more than half the smallest radius (16 for equal radii).
+ * The reference keeps unbounded per-body lists
+ * (rigidbody2d/PenaltyImpactFrictionMap.h:272). */
 hg_status hg_dem_sync(hg_dem *h);
 hg_status hg_dem_download_bodies(hg_dem *h, uint32_t scene, double *q, double *v);
 /* The contact cache after the last step in the reference's iteration order

@@ -18,8 +18,9 @@
 //    Bodies are counting-sorted by bucket every step (hg_sort).
 //  * contacts: each contact is owned by one body -- the smaller index of a
 //    circle pair, or the simulated body of a pair with a fixed body, as the
-//    reference keys its CollisionTypeCache.  A body keeps up to kMaxOwned
-//    owned slots (partner key, tangential spring delta_s, force, torques),
+//    reference keys its CollisionTypeCache.  A body keeps up to cap owned
+//    slots (partner key, tangential spring delta_s, force, torques; cap is
+//    8, 16 or 32 per handle, from the scene's radii),
 //    sorted by partner key, double-buffered so that the next step's
 //    contacts join their spring history by partner; per-plane slots likewise.
 //
@@ -138,10 +139,10 @@ struct ContactArgs {
   double *cpoint;  // optional: contact point per owned slot (circle pairs)
 };
 
-template <typename T>
+template <typename T, int CAP>
 __global__ __launch_bounds__(kThreads) void dem_contacts_kernel(ContactArgs a) {
-  __shared__ uint32_t s_own[kThreads][kMaxOwned + 1];
-  __shared__ uint32_t s_in[kThreads][kMaxIn + 1];
+  __shared__ uint32_t s_own[kThreads][CAP + 1];
+  __shared__ uint32_t s_in[kThreads][CAP + 1];
   const uint32_t blk = hg_xcd_remap(blockIdx.x, a.nblocks_padded);
   const uint32_t s = blk * kThreads + threadIdx.x;
   if (s >= a.nbodies) return;
@@ -154,6 +155,10 @@ __global__ __launch_bounds__(kThreads) void dem_contacts_kernel(ContactArgs a) {
   uint32_t *inl = s_in[threadIdx.x];
   int nown = 0, nin = 0;
   bool overflow = false;
+  // A body whose state is no longer finite (flagged by the integration that
+  // produced it) looks for no partners: every such body sorts into one
+  // bucket, and scanning it would cost the square of their number.
+  const bool finite_i = isfinite(xi) && isfinite(yi);
 
   int cx, cy;
   dem_cell_of(sc, xi, yi, cx, cy);
@@ -162,7 +167,7 @@ __global__ __launch_bounds__(kThreads) void dem_contacts_kernel(ContactArgs a) {
   // the buckets of the 3x3 cells: per row one run of three consecutive
   // buckets, or two runs where the row wraps (the second is empty otherwise)
   const uint32_t len0 = min(bx0 + 3u, wx) - bx0;
-  for (int oy = -1; oy <= 1; ++oy) {
+  for (int oy = -1; oy <= (finite_i ? 1 : -2); ++oy) {
     const uint32_t row = (uint32_t(cy + oy) & uint32_t(sc.ncy - 1)) * wx;
     for (int part = 0; part < 2; ++part) {
       const uint32_t k0 = part ? row : row + bx0;
@@ -173,12 +178,14 @@ __global__ __launch_bounds__(kThreads) void dem_contacts_kernel(ContactArgs a) {
         const bool fj = a.b.fixed[j] != 0;
         if (fi && fj) continue;
         const double dx = xi - a.b.x[j], dy = yi - a.b.y[j];
+        // a partner that left the finite range is no candidate
+        if (!isfinite(dx) || !isfinite(dy)) continue;
         const double rs = ri + a.b.r[j];
         const double c0 = (dx * dx + dy * dy) - rs * rs;
         if (!(c0 <= 0.0)) continue;
         if (!fi && (fj || i < j)) {
           const uint32_t key = (fj ? kKinematic : 0u) | (j - sc.b_off);
-          if (nown < kMaxOwned) {
+          if (nown < CAP) {
             // insertion into the sorted list
             int q = nown++;
             while (q > 0 && own[q - 1] > key) {
@@ -191,7 +198,7 @@ __global__ __launch_bounds__(kThreads) void dem_contacts_kernel(ContactArgs a) {
           }
         } else if (!fi && !fj) {
           const uint32_t key = j - sc.b_off;
-          if (nin < kMaxIn) {
+          if (nin < CAP) {
             int q = nin++;
             while (q > 0 && inl[q - 1] > key) {
               inl[q] = inl[q - 1];
@@ -211,7 +218,7 @@ __global__ __launch_bounds__(kThreads) void dem_contacts_kernel(ContactArgs a) {
   const double vxi = double(ld<T>(a.b.vx, i)), vyi = double(ld<T>(a.b.vy, i));
   const double omi = double(ld<T>(a.b.om, i));
   const double wi = sc.weighted ? double(ld<T>(a.b.w, i)) : 1.0;
-  const uint32_t obase = i * kMaxOwned;
+  const uint32_t obase = i * uint32_t(CAP);
   const int old_n = a.old_s.ocnt[i];
   for (int k = 0; k < nown; ++k) {
     const uint32_t key = own[k];
@@ -281,7 +288,7 @@ __global__ __launch_bounds__(kThreads) void dem_contacts_kernel(ContactArgs a) {
     st<T>(a.f.otau1, sl, T(r10 * F1 - r11 * F0));
   }
   a.new_s.ocnt[i] = uint8_t(nown);
-  for (int k = 0; k < nin; ++k) a.f.ikey[i * kMaxIn + uint32_t(k)] = inl[k];
+  for (int k = 0; k < nin; ++k) a.f.ikey[i * uint32_t(CAP) + uint32_t(k)] = inl[k];
   a.f.icnt[i] = uint8_t(nin);
 
   // static planes (RigidBody2DSim.cpp:777-842; StaticPlaneCircleConstraint.cpp)
@@ -325,6 +332,7 @@ struct IntegrateArgs {
   int32_t *err;
   double *F_out;  // optional: total force per body (3B)
   int32_t rekey;  // produce the next step's cell keys
+  uint32_t cap;   // contact slots per body
 };
 
 // Gathers a body's forces in the reference's summation order, then
@@ -350,10 +358,10 @@ __global__ __launch_bounds__(kThreads) void dem_integrate_kernel(IntegrateArgs a
   const uint32_t lid = i - sc.b_off;
   const int nin = a.f.icnt[i];
   for (int k = 0; k < nin; ++k) {
-    const uint32_t j = a.f.ikey[i * kMaxIn + uint32_t(k)] + sc.b_off;
+    const uint32_t j = a.f.ikey[i * a.cap + uint32_t(k)] + sc.b_off;
     const int nj = a.s.ocnt[j];
     for (int q = 0; q < nj; ++q) {
-      const uint32_t sl = j * kMaxOwned + uint32_t(q);
+      const uint32_t sl = j * a.cap + uint32_t(q);
       if (a.s.okey[sl] == lid) {
         F0 -= double(ld<T>(a.f.oF, 2 * sl));
         F1 -= double(ld<T>(a.f.oF, 2 * sl + 1));
@@ -365,7 +373,7 @@ __global__ __launch_bounds__(kThreads) void dem_integrate_kernel(IntegrateArgs a
   // own pairs, then kinematic pairs (the key order)
   const int nown = a.s.ocnt[i];
   for (int q = 0; q < nown; ++q) {
-    const uint32_t sl = i * kMaxOwned + uint32_t(q);
+    const uint32_t sl = i * a.cap + uint32_t(q);
     F0 += double(ld<T>(a.f.oF, 2 * sl));
     F1 += double(ld<T>(a.f.oF, 2 * sl + 1));
     F2 += double(ld<T>(a.f.otau0, sl));
@@ -460,7 +468,10 @@ static void launch_contacts_t(hg_dem *h) {
   a.err = h->err;
   a.cpoint = h->cpoint;
   ProfScope ps(h->prof, "dem_contacts", h->stream);
-  hipLaunchKernelGGL(dem_contacts_kernel<T>, dim3(a.nblocks_padded), dim3(kThreads), 0, h->stream, a);
+  const dim3 g(a.nblocks_padded), b(kThreads);
+  if (h->cap <= 8) hipLaunchKernelGGL((dem_contacts_kernel<T, 8>), g, b, 0, h->stream, a);
+  else if (h->cap <= 16) hipLaunchKernelGGL((dem_contacts_kernel<T, 16>), g, b, 0, h->stream, a);
+  else hipLaunchKernelGGL((dem_contacts_kernel<T, 32>), g, b, 0, h->stream, a);
 }
 
 template <typename T>
@@ -481,6 +492,7 @@ static void launch_integrate_t(hg_dem *h, bool rekey) {
   a.err = h->err;
   a.F_out = h->keep_force ? h->F_out : nullptr;
   a.rekey = rekey ? 1 : 0;
+  a.cap = uint32_t(h->cap);
   ProfScope ps(h->prof, "dem_integrate", h->stream);
   hipLaunchKernelGGL(dem_integrate_kernel<T>, dim3((h->nbodies + kThreads - 1) / kThreads),
                      dim3(kThreads), 0, h->stream, a);
@@ -597,8 +609,8 @@ static hg_status upload_bodies(hg_dem *h, const hg_dem_desc *descs) {
   }
   for (int k = 0; k < 2; ++k) {
     SlotsRaw &s = h->slots[k];
-    HG_TRY(dalloc(h, &s.okey, size_t(B) * kMaxOwned * 4));
-    HG_TRY(dalloc(h, &s.ods, size_t(B) * kMaxOwned * 2 * sizeof(T)));
+    HG_TRY(dalloc(h, &s.okey, size_t(B) * h->cap * 4));
+    HG_TRY(dalloc(h, &s.ods, size_t(B) * h->cap * 2 * sizeof(T)));
     HG_TRY(dalloc(h, &s.ocnt, B));
     HG_TRY(dalloc(h, &s.pact, size_t(B) * kMaxPlanes));
     HG_TRY(dalloc(h, &s.pds, size_t(B) * kMaxPlanes * 2 * sizeof(T)));
@@ -606,14 +618,33 @@ static hg_status upload_bodies(hg_dem *h, const hg_dem_desc *descs) {
     HG_HIP(h->ctx, hipMemset(s.pact, 0, size_t(B ? B : 1) * kMaxPlanes));
   }
   ForceRaw &f = h->f;
-  HG_TRY(dalloc(h, &f.oF, size_t(B) * kMaxOwned * 2 * sizeof(T)));
-  HG_TRY(dalloc(h, &f.otau0, size_t(B) * kMaxOwned * sizeof(T)));
-  HG_TRY(dalloc(h, &f.otau1, size_t(B) * kMaxOwned * sizeof(T)));
+  HG_TRY(dalloc(h, &f.oF, size_t(B) * h->cap * 2 * sizeof(T)));
+  HG_TRY(dalloc(h, &f.otau0, size_t(B) * h->cap * sizeof(T)));
+  HG_TRY(dalloc(h, &f.otau1, size_t(B) * h->cap * sizeof(T)));
   HG_TRY(dalloc(h, &f.pF, size_t(B) * kMaxPlanes * 2 * sizeof(T)));
   HG_TRY(dalloc(h, &f.ptau, size_t(B) * kMaxPlanes * sizeof(T)));
-  HG_TRY(dalloc(h, &f.ikey, size_t(B) * kMaxIn * 4));
+  HG_TRY(dalloc(h, &f.ikey, size_t(B) * h->cap * 4));
   HG_TRY(dalloc(h, &f.icnt, B));
   return HG_OK;
+}
+
+// Contact slots per body.  A body of radius R touches partners whose centres
+// lie within R + r_max of its own.  While no two partners overlap each other
+// by more than half the smallest radius, their centres stay at least
+// 1.5 r_min apart, so about 2 pi (R + r_max) / (1.5 r_min) of them fit
+// around it.  The capacity is the first of 8, 16 and 32 at or above that
+// bound for R = r_max (16 for equal radii); a body with more contacts than
+// its capacity sets the overflow error bit.
+int hg_dem_contact_capacity(const hg_dem_desc *descs, uint32_t nscenes) {
+  double rmin = INFINITY, rmax = 0.0;
+  for (uint32_t s = 0; s < nscenes; ++s)
+    for (uint32_t k = 0; k < descs[s].nbodies; ++k) {
+      rmin = std::min(rmin, descs[s].radius[k]);
+      rmax = std::max(rmax, descs[s].radius[k]);
+    }
+  if (!(rmax > 0.0) || !(rmin > 0.0)) return kCapMin;
+  const double bound = std::ceil(2.0 * M_PI * (2.0 * rmax) / (1.5 * rmin));
+  return bound <= 8.0 ? 8 : (bound <= 16.0 ? 16 : kCapMax);
 }
 
 // Broad-phase table of one scene: the cell is at least the largest diameter
@@ -688,6 +719,7 @@ static hg_status dem_create_impl(hg_ctx *ctx, const hg_dem_desc *descs, uint32_t
   h->ctx = ctx;
   h->precision = descs[0].precision;
   h->nscenes = nscenes;
+  h->cap = hg_dem_contact_capacity(descs, nscenes);
   HG_HIP(ctx, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   uint64_t b_off = 0, cell_off = 0;
   std::vector<uint32_t> h_b_off;
@@ -748,7 +780,9 @@ static hg_status dem_create_impl(hg_ctx *ctx, const hg_dem_desc *descs, uint32_t
   HG_HIP(ctx, hipMemset(h->F_out, 0, size_t(h->nbodies ? h->nbodies : 1) * 24));
   if (h->precision == HG_FP64) HG_TRY(upload_bodies<double>(hp, descs));
   else HG_TRY(upload_bodies<float>(hp, descs));
-  HG_HIP(ctx, hipDeviceSynchronize());
+  // the copies and memsets above ran on the null stream, which the handle's
+  // non-blocking stream does not wait for; other handles' work is not waited on
+  HG_HIP(ctx, hipStreamSynchronize(nullptr));
   hg_dem_launch_rebin(hp);
   HG_HIP(ctx, hipGetLastError());
   HG_HIP(ctx, hipStreamSynchronize(h->stream));
@@ -762,7 +796,9 @@ static hg_status dem_check(hg_dem *h) {
   HG_HIP(h->ctx, hipMemcpy(e.data(), h->err, h->nscenes * 4, hipMemcpyDeviceToHost));
   for (uint32_t s = 0; s < h->nscenes; ++s) {
     if (e[s] & kSortTotalsBit) return hg_fail(h->ctx, HG_ERR_STATE, "bucket sort: block totals do not match the counts");
-    if (e[s] & 4) return hg_fail(h->ctx, HG_ERR_UNSUPPORTED, "scene " + std::to_string(s) + ": more than 8 contacts on one body");
+    if (e[s] & 4)
+      return hg_fail(h->ctx, HG_ERR_UNSUPPORTED, "scene " + std::to_string(s) + ": more than " +
+                                                     std::to_string(h->cap) + " contacts on one body");
     if (e[s] & 2) return hg_fail(h->ctx, HG_ERR_STATE, "scene " + std::to_string(s) + ": non-finite body state");
   }
   return HG_OK;
@@ -897,13 +933,14 @@ static hg_status download_contacts_t(hg_dem *h, uint32_t scene, uint32_t max, in
                                      int32_t *ci, int32_t *cj, double *delta_s, uint32_t *count) {
   const SceneDev &sc = h->scenes[scene];
   const uint32_t n = sc.b_cnt, off = sc.b_off;
+  const size_t C = size_t(h->cap);
   const SlotsRaw &s = h->slots[h->cur];
-  std::vector<uint32_t> okey(size_t(n) * kMaxOwned);
-  std::vector<T> ods(size_t(n) * kMaxOwned * 2), pds(size_t(n) * kMaxPlanes * 2);
+  std::vector<uint32_t> okey(size_t(n) * C);
+  std::vector<T> ods(size_t(n) * C * 2), pds(size_t(n) * kMaxPlanes * 2);
   std::vector<uint8_t> ocnt(n), pact(size_t(n) * kMaxPlanes);
   if (n) {
-    HG_HIP(h->ctx, hipMemcpy(okey.data(), s.okey + size_t(off) * kMaxOwned, okey.size() * 4, hipMemcpyDeviceToHost));
-    HG_HIP(h->ctx, hipMemcpy(ods.data(), static_cast<T *>(s.ods) + size_t(off) * kMaxOwned * 2, ods.size() * sizeof(T), hipMemcpyDeviceToHost));
+    HG_HIP(h->ctx, hipMemcpy(okey.data(), s.okey + size_t(off) * C, okey.size() * 4, hipMemcpyDeviceToHost));
+    HG_HIP(h->ctx, hipMemcpy(ods.data(), static_cast<T *>(s.ods) + size_t(off) * C * 2, ods.size() * sizeof(T), hipMemcpyDeviceToHost));
     HG_HIP(h->ctx, hipMemcpy(ocnt.data(), s.ocnt + off, n, hipMemcpyDeviceToHost));
     HG_HIP(h->ctx, hipMemcpy(pact.data(), s.pact + size_t(off) * kMaxPlanes, pact.size(), hipMemcpyDeviceToHost));
     HG_HIP(h->ctx, hipMemcpy(pds.data(), static_cast<T *>(s.pds) + size_t(off) * kMaxPlanes * 2, pds.size() * sizeof(T), hipMemcpyDeviceToHost));
@@ -930,7 +967,7 @@ static hg_status download_contacts_t(hg_dem *h, uint32_t scene, uint32_t max, in
   for (int kin = 0; kin < 2; ++kin)
     for (uint32_t i = 0; i < n; ++i)
       for (int q = 0; q < ocnt[i]; ++q) {
-        const size_t sl = size_t(i) * kMaxOwned + q;
+        const size_t sl = size_t(i) * C + q;
         const bool isk = (okey[sl] & kKinematic) != 0;
         if (isk != (kin == 1)) continue;
         emit(kin ? 2 : 1, i, okey[sl] & ~kKinematic, double(ods[2 * sl]), double(ods[2 * sl + 1]));
@@ -953,18 +990,19 @@ static hg_status contact_forces_t(hg_dem *h, uint32_t scene, uint32_t max, doubl
                                   double *point, uint32_t *count) {
   const SceneDev &sc = h->scenes[scene];
   const uint32_t n = sc.b_cnt, off = sc.b_off;
+  const size_t C = size_t(h->cap);
   const SlotsRaw &s = h->slots[h->cur];
-  std::vector<uint32_t> okey(size_t(n) * kMaxOwned);
-  std::vector<T> oF(size_t(n) * kMaxOwned * 2);
-  std::vector<double> cp(h->cpoint ? size_t(n) * kMaxOwned * 2 : 0);
+  std::vector<uint32_t> okey(size_t(n) * C);
+  std::vector<T> oF(size_t(n) * C * 2);
+  std::vector<double> cp(h->cpoint ? size_t(n) * C * 2 : 0);
   std::vector<uint8_t> ocnt(n), pact(size_t(n) * kMaxPlanes);
   if (n) {
-    HG_HIP(h->ctx, hipMemcpy(okey.data(), s.okey + size_t(off) * kMaxOwned, okey.size() * 4, hipMemcpyDeviceToHost));
+    HG_HIP(h->ctx, hipMemcpy(okey.data(), s.okey + size_t(off) * C, okey.size() * 4, hipMemcpyDeviceToHost));
     HG_HIP(h->ctx, hipMemcpy(ocnt.data(), s.ocnt + off, n, hipMemcpyDeviceToHost));
     HG_HIP(h->ctx, hipMemcpy(pact.data(), s.pact + size_t(off) * kMaxPlanes, pact.size(), hipMemcpyDeviceToHost));
-    HG_HIP(h->ctx, hipMemcpy(oF.data(), static_cast<T *>(h->f.oF) + size_t(off) * kMaxOwned * 2, oF.size() * sizeof(T), hipMemcpyDeviceToHost));
+    HG_HIP(h->ctx, hipMemcpy(oF.data(), static_cast<T *>(h->f.oF) + size_t(off) * C * 2, oF.size() * sizeof(T), hipMemcpyDeviceToHost));
     if (!cp.empty())
-      HG_HIP(h->ctx, hipMemcpy(cp.data(), h->cpoint + size_t(off) * kMaxOwned * 2, cp.size() * 8, hipMemcpyDeviceToHost));
+      HG_HIP(h->ctx, hipMemcpy(cp.data(), h->cpoint + size_t(off) * C * 2, cp.size() * 8, hipMemcpyDeviceToHost));
   }
   uint32_t c = 0;
   auto emit = [&](double f0, double f1, double p0, double p1) {
@@ -987,7 +1025,7 @@ static hg_status contact_forces_t(hg_dem *h, uint32_t scene, uint32_t max, doubl
   for (int kin = 0; kin < 2; ++kin)
     for (uint32_t i = 0; i < n; ++i)
       for (int q = 0; q < ocnt[i]; ++q) {
-        const size_t sl = size_t(i) * kMaxOwned + q;
+        const size_t sl = size_t(i) * C + q;
         if (((okey[sl] & kKinematic) != 0) != (kin == 1)) continue;
         emit(double(oF[2 * sl]), double(oF[2 * sl + 1]), cp.empty() ? 0.0 : cp[2 * sl],
              cp.empty() ? 0.0 : cp[2 * sl + 1]);
@@ -1010,8 +1048,9 @@ static hg_status upload_contacts_t(hg_dem *h, uint32_t scene, uint32_t count, co
                                    const int32_t *ci, const int32_t *cj, const double *ds) {
   const SceneDev &sc = h->scenes[scene];
   const uint32_t n = sc.b_cnt, off = sc.b_off;
-  std::vector<uint32_t> okey(size_t(n) * kMaxOwned, 0u);
-  std::vector<T> ods(size_t(n) * kMaxOwned * 2, T(0)), pds(size_t(n) * kMaxPlanes * 2, T(0));
+  const size_t C = size_t(h->cap);
+  std::vector<uint32_t> okey(size_t(n) * C, 0u);
+  std::vector<T> ods(size_t(n) * C * 2, T(0)), pds(size_t(n) * kMaxPlanes * 2, T(0));
   std::vector<uint8_t> ocnt(n, 0), pact(size_t(n) * kMaxPlanes, 0);
   for (uint32_t c = 0; c < count; ++c) {
     const int32_t t = type[c], i = ci[c], j = cj[c];
@@ -1026,9 +1065,9 @@ static hg_status upload_contacts_t(hg_dem *h, uint32_t scene, uint32_t count, co
     } else if (t == 1 || t == 2) {
       if (j < 0 || uint32_t(j) >= n || (t == 1 && j <= i))
         return hg_fail(h->ctx, HG_ERR_INVALID, "contact partner out of range");
-      if (ocnt[i] >= kMaxOwned)
-        return hg_fail(h->ctx, HG_ERR_UNSUPPORTED, "more than 8 contacts on one body");
-      const size_t sl = size_t(i) * kMaxOwned + ocnt[i]++;
+      if (ocnt[i] >= C)
+        return hg_fail(h->ctx, HG_ERR_UNSUPPORTED, "more contacts on one body than its capacity (" + std::to_string(C) + ")");
+      const size_t sl = size_t(i) * C + ocnt[i]++;
       okey[sl] = uint32_t(j) | (t == 2 ? kKinematic : 0u);
       ods[2 * sl] = T(ds[2 * c]);
       ods[2 * sl + 1] = T(ds[2 * c + 1]);
@@ -1038,7 +1077,7 @@ static hg_status upload_contacts_t(hg_dem *h, uint32_t scene, uint32_t count, co
   }
   // each body's contacts sorted by partner key, as the step keeps them
   for (uint32_t i = 0; i < n; ++i) {
-    const size_t b = size_t(i) * kMaxOwned;
+    const size_t b = size_t(i) * C;
     for (int a = 1; a < ocnt[i]; ++a)
       for (int k = a; k > 0 && okey[b + k - 1] > okey[b + k]; --k) {
         std::swap(okey[b + k - 1], okey[b + k]);
@@ -1048,8 +1087,8 @@ static hg_status upload_contacts_t(hg_dem *h, uint32_t scene, uint32_t count, co
   }
   const SlotsRaw &s = h->slots[h->cur];
   if (n) {
-    HG_HIP(h->ctx, hipMemcpy(s.okey + size_t(off) * kMaxOwned, okey.data(), okey.size() * 4, hipMemcpyHostToDevice));
-    HG_HIP(h->ctx, hipMemcpy(static_cast<T *>(s.ods) + size_t(off) * kMaxOwned * 2, ods.data(), ods.size() * sizeof(T), hipMemcpyHostToDevice));
+    HG_HIP(h->ctx, hipMemcpy(s.okey + size_t(off) * C, okey.data(), okey.size() * 4, hipMemcpyHostToDevice));
+    HG_HIP(h->ctx, hipMemcpy(static_cast<T *>(s.ods) + size_t(off) * C * 2, ods.data(), ods.size() * sizeof(T), hipMemcpyHostToDevice));
     HG_HIP(h->ctx, hipMemcpy(s.ocnt + off, ocnt.data(), n, hipMemcpyHostToDevice));
     HG_HIP(h->ctx, hipMemcpy(s.pact + size_t(off) * kMaxPlanes, pact.data(), pact.size(), hipMemcpyHostToDevice));
     HG_HIP(h->ctx, hipMemcpy(static_cast<T *>(s.pds) + size_t(off) * kMaxPlanes * 2, pds.data(), pds.size() * sizeof(T), hipMemcpyHostToDevice));
@@ -1107,12 +1146,11 @@ extern "C" hg_status hg_dem_kernel_times(hg_dem *h, int32_t max_kernels, char *n
 
 // Device bytes per body of the arrays dalloc gives a handle (bodies, two
 // contact-slot sets, force records, sort keys), for the estimate below.
-static uint64_t dem_bytes_per_body(size_t T) {
+static uint64_t dem_bytes_per_body(size_t T, int cap) {
+  const uint64_t C = uint64_t(cap);
   const uint64_t bodies = 4 * 8 + 7 * T + 1;
-  const uint64_t slots = uint64_t(kMaxOwned) * 4 + uint64_t(kMaxOwned) * 2 * T + 1 +
-                         uint64_t(kMaxPlanes) + uint64_t(kMaxPlanes) * 2 * T;
-  const uint64_t forces = uint64_t(kMaxOwned) * 4 * T + uint64_t(kMaxPlanes) * 3 * T +
-                          uint64_t(kMaxIn) * 4 + 1;
+  const uint64_t slots = C * 4 + C * 2 * T + 1 + uint64_t(kMaxPlanes) + uint64_t(kMaxPlanes) * 2 * T;
+  const uint64_t forces = C * 4 * T + uint64_t(kMaxPlanes) * 3 * T + C * 4 + 1;
   return bodies + 2 * slots + forces + 3 * 4 + 24;
 }
 
@@ -1129,7 +1167,7 @@ extern "C" hg_status hg_dem_estimate_bytes(const hg_dem_desc *descs, uint32_t ns
   }
   const size_t T = descs[0].precision == HG_FP64 ? 8 : 4;
   const uint64_t nscan = (cells + kScanChunk - 1) / kScanChunk;
-  *bytes = B * dem_bytes_per_body(T) + cells * 8 + 4 + nscan * 8 * kScanSubTotals +
+  *bytes = B * dem_bytes_per_body(T, hg_dem_contact_capacity(descs, nscenes)) + cells * 8 + 4 + nscan * 8 * kScanSubTotals +
            uint64_t(nscenes) * (sizeof(SceneDev) + 16) + (1u << 20);
   return HG_OK;
 }

@@ -7,8 +7,11 @@
 namespace hgdem {
 
 constexpr int kMaxPlanes = 8;
-constexpr int kMaxOwned = 8;
-constexpr int kMaxIn = 8;
+// Per-body contact capacity (owned pair contacts, and incoming ones): a
+// handle picks 8, 16 or 32 from its scenes' radii (hg_dem_contact_capacity);
+// a body with more contacts sets the overflow error bit.
+constexpr int kCapMin = 8;
+constexpr int kCapMax = 32;
 constexpr uint32_t kKinematic = 0x80000000u;
 // One wavefront per workgroup: scenes of tens of thousands of bodies then
 // still spread over every CU.
@@ -38,20 +41,20 @@ struct BodiesRaw {
 };
 
 struct SlotsRaw {
-  uint32_t *okey;  // [B][kMaxOwned]
-  void *ods;       // [B][kMaxOwned][2]
+  uint32_t *okey;  // [B][cap]
+  void *ods;       // [B][cap][2]
   uint8_t *ocnt;   // [B]
   uint8_t *pact;   // [B][kMaxPlanes]
   void *pds;       // [B][kMaxPlanes][2]
 };
 
 struct ForceRaw {
-  void *oF;     // [B][kMaxOwned][2]
-  void *otau0;  // [B][kMaxOwned]
-  void *otau1;  // [B][kMaxOwned]
+  void *oF;     // [B][cap][2]
+  void *otau0;  // [B][cap]
+  void *otau1;  // [B][cap]
   void *pF;     // [B][kMaxPlanes][2]
   void *ptau;   // [B][kMaxPlanes]
-  uint32_t *ikey;  // [B][kMaxIn] owners of contacts with this body
+  uint32_t *ikey;  // [B][cap] owners of contacts with this body
   uint8_t *icnt;
 };
 
@@ -85,6 +88,7 @@ struct hg_dem {
   hipStream_t stream = nullptr;
   bool own_stream = true;
   int precision = HG_FP64;
+  int cap = hgdem::kCapMin;  // contact slots per body (owned, and incoming)
   std::vector<SceneDev> scenes;
   uint32_t nscenes = 0, nbodies = 0, ncell = 0, nscan = 0;
   SceneDev *d_scenes = nullptr;
@@ -110,6 +114,9 @@ struct hg_dem {
   size_t tsize() const { return precision == HG_FP64 ? 8 : 4; }
 };
 
+
+// Contact slots per body for a batch of scenes: 8, 16 or 32 (see hg_dem.hip).
+int hg_dem_contact_capacity(const hg_dem_desc *descs, uint32_t nscenes);
 
 // Launches one DEM step (contacts, integrate); with rekey false the
 // integrate kernel does not produce cell keys and the caller re-bins with

@@ -260,8 +260,10 @@ __global__ __launch_bounds__(kThreads) void hyb_exchange(HybArgs a) {
 // The continuum grid's coupling weights as HybridGrains2DSim's node-node step
 // saves them (HybridGrains2DSim.cpp:169-197), in halves: 1 (0.5) at nodes
 // holding grain mass, else 2 (1.0) at nodes with continuum mass after this
-// step's P2G, else 0.  Output only; it runs on the grain stream after the
-// exchange and before the finish clears the grain mass.  Nodes of tiles
+// step's P2G, else 0.  Output only.  It runs on the continuum stream after
+// the exchange and before the grain branch forks off, so it precedes both
+// the finish (which clears the grain mass) and the sort after G2P (which
+// rewrites the tile flags it reads for the next step).  Nodes of tiles
 // without points carry no mass (their P2G values are not this step's).
 __global__ __launch_bounds__(kThreads) void hyb_homog_factor(HybArgs a) {
   // four nodes of one row per thread (tile widths are multiples of four);
@@ -362,6 +364,7 @@ struct HomogArgs {
   const double *cpoint;
   uint32_t *count, *next_count;
   int32_t stress, vel;
+  uint32_t cap;        // contact slots per body
 };
 
 // Spreads one quantity set from position (x, y) with the MPM weights of half
@@ -388,7 +391,7 @@ __global__ __launch_bounds__(kThreads) void hyb_homog_grains(HybArgs a, HomogArg
     const double inv_area = 1.0 / (sc.h * sc.h);
     const int n = g.s.ocnt[b];
     for (int k = 0; k < n; ++k) {
-      const uint32_t sl = b * uint32_t(hgdem::kMaxOwned) + uint32_t(k);
+      const uint32_t sl = b * g.cap + uint32_t(k);
       const uint32_t key = g.s.okey[sl];
       if (key & hgdem::kKinematic) continue;
       const uint32_t j = key;  // one DEM scene: bodies are numbered from 0
@@ -557,13 +560,15 @@ static void launch_hybrid_steps(hg_hybrid *h, double dt, uint32_t n, bool concur
       ProfScope ps(prof, "hyb_exchange", s);
       hipLaunchKernelGGL(hyb_exchange<T>, gb, tb, 0, s, a);
     }
-    order(h->fork_ev, s, side);
     if (a.hfac) {
+      // on the continuum stream, before the fork: it reads the tile flags,
+      // which the sort after G2P rewrites on this stream for the next step
       const hgmpm::SceneDev &msc = h->mpm->scenes[0];
       const uint32_t per_row = (uint32_t(msc.nnx) + 4 * kThreads - 1) / (4 * kThreads);
-      ProfScope ps(prof, "hyb_homog_factor", side);
-      hipLaunchKernelGGL(hyb_homog_factor, dim3(per_row, uint32_t(msc.nny)), tb, 0, side, a);
+      ProfScope ps(prof, "hyb_homog_factor", s);
+      hipLaunchKernelGGL(hyb_homog_factor, dim3(per_row, uint32_t(msc.nny)), tb, 0, s, a);
     }
+    order(h->fork_ev, s, side);
     {
       ProfScope ps(prof, "hyb_transfer", side);
       hipLaunchKernelGGL((hyb_transfer<T, BASIS>), gb, tb, 0, side, a);
@@ -585,6 +590,7 @@ static void launch_hybrid_steps(hg_hybrid *h, double dt, uint32_t n, bool concur
       g.next_count = h->hg.count + (1 - h->par);
       g.stress = h->homog_stress ? 1 : 0;
       g.vel = h->homog_vel ? 1 : 0;
+      g.cap = uint32_t(h->dem->cap);
       {
         ProfScope ps(prof, "hyb_homog_grains", side);
         hipLaunchKernelGGL((hyb_homog_grains<T, BASIS>), gb, tb, 0, side, a, g);
@@ -688,7 +694,9 @@ extern "C" hg_status hg_hybrid_create(hg_ctx *ctx, const hg_mpm_desc *mpm_desc,
       hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&h->join_ev, hipEventDisableTiming) != hipSuccess)
     return fail(hg_fail(ctx, HG_ERR_HIP, "hybrid streams"));
-  if (hipDeviceSynchronize() != hipSuccess) return fail(hg_fail(ctx, HG_ERR_HIP, "hybrid init"));
+  // the memsets and copies above ran on the null stream (other handles' work
+  // is not waited for)
+  if (hipStreamSynchronize(nullptr) != hipSuccess) return fail(hg_fail(ctx, HG_ERR_HIP, "hybrid init"));
   *out = h.release();
   return HG_OK;
 }
@@ -820,14 +828,14 @@ extern "C" hg_status hg_hybrid_set_homogenize(hg_hybrid *h, int32_t stress, int3
     HG_TRY(halloc(h, &h->hg.stamp, size_t(G) * 4));
     HG_TRY(halloc(h, &h->hg.list, size_t(h->hg.max_list) * 4));
     HG_TRY(halloc(h, &h->hg.count, 8));
-    HG_TRY(halloc(h, &h->dem->cpoint, size_t(B) * hgdem::kMaxOwned * 16));
+    HG_TRY(halloc(h, &h->dem->cpoint, size_t(B) * size_t(h->dem->cap) * 16));
     HG_HIP(h->ctx, hipMemset(h->hg.stress, 0, size_t(G) * 24));
     HG_HIP(h->ctx, hipMemset(h->hg.vel, 0, size_t(G) * 16));
     HG_HIP(h->ctx, hipMemset(h->hg.mass, 0, size_t(G) * 8));
     HG_HIP(h->ctx, hipMemset(h->hg.mom, 0, size_t(G) * 16));
     HG_HIP(h->ctx, hipMemset(h->hg.stamp, 0, size_t(G) * 4));
     HG_HIP(h->ctx, hipMemset(h->hg.count, 0, 8));
-    HG_HIP(h->ctx, hipDeviceSynchronize());
+    HG_HIP(h->ctx, hipStreamSynchronize(nullptr));
   }
   h->homog_stress = stress != 0;
   h->homog_vel = velocity != 0;
@@ -841,7 +849,7 @@ extern "C" hg_status hg_hybrid_clear_homogenized(hg_hybrid *h) {
   const uint32_t G = h->mpm->nnode;
   HG_HIP(h->ctx, hipMemset(h->hg.stress, 0, size_t(G) * 24));
   HG_HIP(h->ctx, hipMemset(h->hg.vel, 0, size_t(G) * 16));
-  HG_HIP(h->ctx, hipDeviceSynchronize());
+  HG_HIP(h->ctx, hipStreamSynchronize(nullptr));
   return HG_OK;
 }
 
@@ -886,7 +894,7 @@ extern "C" hg_status hg_hybrid_set_homog_factor(hg_hybrid *h, int32_t on) {
     const size_t G = h->mpm->nnode;
     HG_TRY(halloc(h, &h->hfac, G));
     HG_HIP(h->ctx, hipMemset(h->hfac, 2, G));
-    HG_HIP(h->ctx, hipDeviceSynchronize());
+    HG_HIP(h->ctx, hipStreamSynchronize(nullptr));
   }
   h->hfac_on = on != 0;
   return HG_OK;

@@ -1376,9 +1376,10 @@ static hg_status mpm_create_impl(hg_ctx *ctx, const hg_mpm_desc *descs, uint32_t
   HG_HIP(ctx, hipMemset(h->err, 0, nscenes * 4));
   if (h->precision == HG_FP64) HG_TRY(upload_points<double>(hp, descs));
   else HG_TRY(upload_points<float>(hp, descs));
-  // the memsets above run on the null stream; the engine's stream does not
-  // wait for it, so finish them before the first kernel
-  HG_HIP(ctx, hipDeviceSynchronize());
+  // the memsets and copies above run on the null stream; the engine's
+  // non-blocking stream does not wait for it, so finish them before the first
+  // kernel (without waiting for other handles' streams)
+  HG_HIP(ctx, hipStreamSynchronize(nullptr));
 
   // initial sort
   launch_rebin(hp);

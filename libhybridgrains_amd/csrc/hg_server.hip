@@ -84,6 +84,9 @@ struct Req {
   hg_hybrid *h = nullptr;
   hipStream_t stream = nullptr;
   std::deque<hipEvent_t> events;
+  // pinned host copies of the handles' device error words (MPM scene 0, DEM
+  // scene 0, hybrid coupling), refreshed after every slice
+  int32_t *herr = nullptr;
   // results
   std::vector<double> pq, pv, pb, pJ, bq, bv;
   double latency = 0.0;
@@ -94,6 +97,7 @@ class GpuExecutor : public hg::sched::Executor {
   explicit GpuExecutor(hg_ctx *ctx) : ctx_(ctx) {}
   ~GpuExecutor() override {
     for (hipEvent_t e : pool_) hipEventDestroy(e);
+    for (int32_t *p : pinned_) hipHostFree(p);
   }
 
   bool start(Job &j) override {
@@ -124,8 +128,13 @@ class GpuExecutor : public hg::sched::Executor {
         }
         break;
     }
-    if (st != HG_OK) {
+    if (st == HG_OK && !(r.herr = pinned())) {
+      st = HG_ERR_NOMEM;
+      j.error = "pinned host memory for the error words";
+    } else if (st != HG_OK) {
       j.error = hg_last_error(ctx_);
+    }
+    if (st != HG_OK) {
       release(r);
       return false;
     }
@@ -134,16 +143,36 @@ class GpuExecutor : public hg::sched::Executor {
     return true;
   }
 
-  void launch(Job &j, uint32_t k) override {
+  bool launch(Job &j, uint32_t k) override {
     Req &r = *static_cast<Req *>(j.payload.get());
+    hg_status st = HG_OK;
     switch (r.kind) {
-      case Kind::Mpm: hg_mpm_step_graph(r.m, r.dt, k); break;
-      case Kind::Dem: hg_dem_step_graph(r.d, r.dt, k); break;
-      case Kind::Hybrid: hg_hybrid_step_graph(r.h, r.dt, k); break;
+      case Kind::Mpm: st = hg_mpm_step_graph(r.m, r.dt, k); break;
+      case Kind::Dem: st = hg_dem_step_graph(r.d, r.dt, k); break;
+      case Kind::Hybrid: st = hg_hybrid_step_graph(r.h, r.dt, k); break;
     }
+    if (st != HG_OK) {
+      j.error = std::string("slice not queued: ") + hg_last_error(ctx_);
+      return false;
+    }
+    // the handles' error words as they stand after this slice, read by poll
+    // once the slice's event has completed
+    const int32_t *src[3] = {r.m ? r.m->err : (r.h ? r.h->mpm->err : nullptr),
+                             r.d ? r.d->err : (r.h ? r.h->dem->err : nullptr),
+                             r.h ? r.h->err : nullptr};
+    for (int w = 0; w < 3; ++w)
+      if (src[w] && hipMemcpyAsync(r.herr + w, src[w], 4, hipMemcpyDeviceToHost, r.stream) != hipSuccess) {
+        j.error = "error-word copy not queued";
+        return false;
+      }
     hipEvent_t e = event();
-    hipEventRecord(e, r.stream);
+    if (!e || hipEventRecord(e, r.stream) != hipSuccess) {
+      j.error = "slice event not recorded";
+      if (e) pool_.push_back(e);
+      return false;
+    }
     r.events.push_back(e);
+    return true;
   }
 
   int poll(Job &j) override {
@@ -159,6 +188,15 @@ class GpuExecutor : public hg::sched::Executor {
       pool_.push_back(r.events.front());
       r.events.pop_front();
       ++n;
+    }
+    // a finished slice left an error: retire the request now rather than
+    // running its remaining steps on a void state
+    if (n > 0) {
+      const std::string e = decode_errors(r.herr);
+      if (!e.empty()) {
+        j.error = e;
+        return -1;
+      }
     }
     return n;
   }
@@ -209,10 +247,35 @@ class GpuExecutor : public hg::sched::Executor {
     hipEventCreateWithFlags(&e, hipEventDisableTiming);
     return e;
   }
+  // The handles' error bits (hg_mpm_scene_error, hg_dem_sync, hg_hybrid_sync)
+  // as a message; empty when none is set.
+  static std::string decode_errors(const int32_t *w) {
+    const int32_t m = w[0], d = w[1], c = w[2];
+    if ((m | d) & 0x100) return "cell sort: block totals do not match the counts";
+    if (m & 1) return "a material point left the grid";
+    if (m & 2) return "non-finite material point state";
+    if (d & 4) return "more contacts on one body than its capacity";
+    if (d & 2) return "non-finite body state";
+    if (c & 8) return "coupling node list overflow";
+    return std::string();
+  }
+  int32_t *pinned() {
+    int32_t *p = nullptr;
+    if (!pinned_.empty()) {
+      p = pinned_.back();
+      pinned_.pop_back();
+    } else if (hipHostMalloc(reinterpret_cast<void **>(&p), 16, hipHostMallocDefault) != hipSuccess) {
+      return nullptr;
+    }
+    p[0] = p[1] = p[2] = p[3] = 0;
+    return p;
+  }
   void release(Req &r) {
     if (r.stream) hipStreamSynchronize(r.stream);
     for (hipEvent_t e : r.events) pool_.push_back(e);
     r.events.clear();
+    if (r.herr) pinned_.push_back(r.herr);
+    r.herr = nullptr;
     if (r.m) hg_mpm_destroy(r.m);
     if (r.d) hg_dem_destroy(r.d);
     if (r.h) hg_hybrid_destroy(r.h);
@@ -223,6 +286,7 @@ class GpuExecutor : public hg::sched::Executor {
   }
   hg_ctx *ctx_;
   std::vector<hipEvent_t> pool_;
+  std::vector<int32_t *> pinned_;
 };
 
 }  // namespace
@@ -303,10 +367,12 @@ static hg_status submit(hg_server *s, std::shared_ptr<Req> r, uint64_t bytes, ui
   j.steps_total = nsteps;
   j.work_per_step = work;
   j.payload = r;
+  // the hand-over time, taken before waiting for the worker's lock
+  const double t = s->ex.now();
   uint64_t got;
   {
     std::lock_guard<std::mutex> lk(s->mu);
-    got = s->sched.submit(std::move(j));
+    got = s->sched.submit(std::move(j), t);
   }
   s->wake.notify_all();
   if (id) *id = got;

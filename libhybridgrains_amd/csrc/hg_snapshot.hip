@@ -122,7 +122,7 @@ SnapHeader mpm_header(const hg_mpm *h, uint64_t payload) {
 
 std::vector<Sec> dem_secs(hg_dem *h) {
   const size_t B = h->nbodies, T = h->tsize();
-  const size_t O = size_t(hgdem::kMaxOwned), P = size_t(hgdem::kMaxPlanes);
+  const size_t O = size_t(h->cap), P = size_t(hgdem::kMaxPlanes);
   const hgdem::SlotsRaw &sl = h->slots[h->cur];
   return {{h->b.x, B * 8},       {h->b.y, B * 8},          {h->b.th, B * 8},
           {h->b.vx, B * T},      {h->b.vy, B * T},         {h->b.om, B * T},

@@ -15,8 +15,10 @@
 //    run concurrently (one stream each).
 //  - Completion: when all steps of a request have finished, its results are
 //    copied out, its device memory is released and its latency (submit to
-//    completion) is recorded.  A request whose slice fails (its state left
-//    the valid domain, a device error) is retired alone with the error.
+//    completion) is recorded.  A request whose slice fails to launch, or
+//    whose finished slice reports an error (its state left the valid domain,
+//    a contact overflow, a device error), is retired alone with the error as
+//    soon as that is seen, without running its remaining steps.
 #pragma once
 
 #include <algorithm>
@@ -53,8 +55,9 @@ class Executor {
   virtual ~Executor() = default;
   // Allocates and uploads the job; false (with job.error) on failure.
   virtual bool start(Job &job) = 0;
-  // Queues k more steps of the job on the device.
-  virtual void launch(Job &job, uint32_t k) = 0;
+  // Queues k more steps of the job on the device; false (with job.error) if
+  // they could not be queued.
+  virtual bool launch(Job &job, uint32_t k) = 0;
   // Number of queued slices of the job that have finished since the last
   // call (in launch order); -1 with job.error if the job failed.
   virtual int poll(Job &job) = 0;
@@ -85,9 +88,13 @@ class Scheduler {
  public:
   Scheduler(Executor &ex, const Config &cfg) : ex_(ex), cfg_(cfg) {}
 
-  uint64_t submit(Job job) {
+  uint64_t submit(Job job) { return submit(std::move(job), ex_.now()); }
+
+  // submit_t: when the caller handed the request over (before any wait for
+  // the scheduler), so that latencies include that wait.
+  uint64_t submit(Job job, double submit_t) {
     job.id = ++next_id_;
-    job.submit_t = ex_.now();
+    job.submit_t = submit_t;
     job.state = State::Queued;
     if (stats_.submitted == 0) stats_.first_submit_t = job.submit_t;
     ++stats_.submitted;
@@ -204,15 +211,29 @@ class Scheduler {
   }
 
   void feed() {
+    std::vector<uint64_t> failed;
     for (uint64_t id : active_) {
       Job &j = jobs_[id];
       while (j.inflight < cfg_.max_inflight && j.steps_launched < j.steps_total) {
         const uint64_t k = std::min<uint64_t>(cfg_.slice_steps, j.steps_total - j.steps_launched);
-        ex_.launch(j, uint32_t(k));
+        if (!ex_.launch(j, uint32_t(k))) {
+          failed.push_back(id);
+          break;
+        }
         slices_[id].push_back(k);
         j.steps_launched += k;
         ++j.inflight;
       }
+    }
+    for (uint64_t id : failed) {
+      Job &j = jobs_[id];
+      ex_.abort(j);
+      j.state = State::Failed;
+      j.done_t = ex_.now();
+      ++stats_.failed;
+      bytes_ -= j.bytes;
+      slices_.erase(id);
+      active_.erase(std::remove(active_.begin(), active_.end(), id), active_.end());
     }
   }
 

@@ -173,15 +173,15 @@ def dem_column(
     per_row = int(width / spacing)
     dy = spacing * math.sqrt(3.0) / 2.0
     rows = (n_bodies + per_row - 1) // per_row
-    pts = []
-    for j in range(rows):
-        off = 0.5 * spacing if (j % 2) else 0.0
-        for i in range(per_row):
-            if len(pts) == n_bodies:
-                break
-            pts.append((spacing * (i + 0.5) + off, r_mean + r_spread + 1e-4 + j * dy))
+    if spacing * (per_row + 0.5) + r_mean + r_spread > box_width:
+        raise ValueError("the column is wider than the box: bodies would start behind the right wall")
+    # lattice sites row by row, the first n_bodies of them
+    k = np.arange(n_bodies)
+    j, i = k // per_row, k % per_row
+    pts = np.stack([spacing * (i + 0.5) + 0.5 * spacing * (j % 2),
+                    r_mean + r_spread + 1e-4 + j * dy], axis=1)
     q = np.zeros((n_bodies, 3))
-    q[:, :2] = np.asarray(pts) + rng.uniform(-jitter, jitter, size=(n_bodies, 2))
+    q[:, :2] = pts + rng.uniform(-jitter, jitter, size=(n_bodies, 2))
     q[:, 1] = np.maximum(q[:, 1], r_mean + r_spread)
     q[:, 0] = np.maximum(q[:, 0], r_mean + r_spread)
     radius = rng.uniform(r_mean - r_spread, r_mean + r_spread, n_bodies)
@@ -281,12 +281,13 @@ def hybrid_1m(seed: int = 0):
 
 def served_requests(n: int, seed: int = 0, rate: float = 8.0, n_min: float = 1e4,
                     n_max: float = 2e5, steps_min: int = 200, steps_max: int = 1000,
-                    mix=(0.5, 0.3, 0.2)):
+                    mix=(0.5, 0.3, 0.2), dem_rows: int = 300):
     """A Poisson stream of scene requests: (arrival_s, kind, payload, dt,
     nsteps), kind 'mpm' (payload MpmScene), 'dem' (DemScene) or 'hybrid'
     ((MpmScene, DemScene, weights)).  Sizes are log-uniform in [n_min, n_max]
     particles (points + bodies) and horizons uniform in [steps_min,
-    steps_max]; each request is generated from its own seed."""
+    steps_max]; each request is generated from its own seed.  DEM columns
+    are at most dem_rows rows tall and as wide as their count needs."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -305,7 +306,15 @@ def served_requests(n: int, seed: int = 0, rate: float = 8.0, n_min: float = 1e4
                                 column=(wc * h, hc * h), seed=rs)
             out.append((t, kind, s, 1e-5, nsteps))
         elif kind == "dem":
-            s = dem_column(n_bodies=int(N), width=max(0.02, 2.2e-3 * math.sqrt(N) / 2.0),
+            # at most dem_rows rows: a taller column of 1 mm circles at
+            # kn = 1e5 N/m overlaps at its foot by more than a radius; the box
+            # leaves room to run out four column heights past the column
+            spacing = 2.2e-3
+            per_row = max(8, -(-int(N) // dem_rows))
+            rows = -(-int(N) // per_row)
+            width = spacing * (per_row + 0.25)
+            s = dem_column(n_bodies=int(N), width=width,
+                           box_width=width + spacing + 4.0 * rows * spacing * math.sqrt(3.0) / 2.0,
                            seed=rs)
             out.append((t, kind, s, 1e-5, nsteps))
         else:

@@ -30,7 +30,11 @@ struct FakeExecutor : Executor {
   double step_cost = 1.0;
   std::map<uint64_t, std::vector<double>> ends;  // finish time of each queued slice
   std::map<uint64_t, double> stream_free;
-  std::map<uint64_t, bool> fail_start, fail_poll;
+  std::map<uint64_t, bool> fail_start, fail_poll, fail_launch;
+  // a slice whose end passes this time reports an error (a state that left
+  // the valid domain), as the GPU executor's per-slice error words do
+  std::map<uint64_t, double> error_after;
+  std::map<uint64_t, uint64_t> steps_queued;
   std::map<uint64_t, int> max_inflight_seen;
   std::vector<uint64_t> started, finished, aborted;
 
@@ -43,13 +47,19 @@ struct FakeExecutor : Executor {
     stream_free[j.id] = t;
     return true;
   }
-  void launch(Job &j, uint32_t k) override {
+  bool launch(Job &j, uint32_t k) override {
+    if (fail_launch[j.id]) {
+      j.error = "slice not queued";
+      return false;
+    }
+    steps_queued[j.id] += k;
     const double s = std::max(t, stream_free[j.id]);
     const double e = s + step_cost * k;
     stream_free[j.id] = e;
     ends[j.id].push_back(e);
     const int inflight = int(ends[j.id].size());
     max_inflight_seen[j.id] = std::max(max_inflight_seen[j.id], inflight);
+    return true;
   }
   int poll(Job &j) override {
     if (fail_poll[j.id]) {
@@ -59,6 +69,11 @@ struct FakeExecutor : Executor {
     int n = 0;
     auto &v = ends[j.id];
     while (!v.empty() && v.front() <= t) {
+      const auto it = error_after.find(j.id);
+      if (it != error_after.end() && v.front() > it->second) {
+        j.error = "non-finite body state";
+        return -1;
+      }
       v.erase(v.begin());
       ++n;
     }
@@ -226,6 +241,30 @@ static void test_continuous_admission() {
   CHECK(s.job(d)->done_t < s.job(a)->done_t);
 }
 
+static void test_slice_errors_retire_early() {
+  // an error seen after a slice retires the request without running the
+  // rest of its steps; a slice that cannot be queued retires it at once
+  FakeExecutor ex;
+  Config c;
+  c.slice_steps = 10;
+  Scheduler s(ex, c);
+  const uint64_t a = s.submit(make(1, 1000));
+  const uint64_t b = s.submit(make(1, 1000));
+  const uint64_t d = s.submit(make(1, 60));
+  ex.error_after[a] = 25.0;  // the slice ending at 30 carries the error
+  ex.fail_launch[b] = true;
+  run_until_idle(s, ex);
+  CHECK(s.job(a)->state == State::Failed && s.job(a)->error == "non-finite body state");
+  CHECK(s.job(a)->steps_done == 20);
+  CHECK(ex.steps_queued[a] <= 40);  // at most max_inflight slices past the error
+  CHECK(s.job(b)->state == State::Failed && s.job(b)->error == "slice not queued");
+  CHECK(ex.steps_queued[b] == 0);
+  CHECK(s.job(d)->state == State::Done);
+  const auto st = s.stats();
+  CHECK(st.failed == 2 && st.completed == 1 && st.bytes_in_use == 0 && st.active == 0);
+  CHECK(ex.aborted.size() == 2);
+}
+
 int main() {
   test_admission_by_budget();
   test_bypass_is_bounded();
@@ -233,6 +272,7 @@ int main() {
   test_cancel_and_failure_are_isolated();
   test_latency_quantiles();
   test_continuous_admission();
+  test_slice_errors_retire_early();
   if (failures) {
     std::printf("%d checks failed\n", failures);
     return 1;
