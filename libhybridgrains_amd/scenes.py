@@ -197,6 +197,41 @@ def dem_column(
     return s
 
 
+def dem_rosettes(nx: int = 6, ny: int = 4, R: float = 4e-3, r: float = 1e-3,
+                 overlap: float = 0.05e-3, kn: float = 1.0e5, seed: int = 0):
+    """Polydisperse contact clusters: nx x ny large circles of radius R, each
+    ringed by as many circles of radius r as fit around it (15 for the
+    defaults), every one overlapping the large circle by ``overlap``, in the
+    air above a floor and beside a wall.  Each large circle starts with more
+    than eight partners; the ring's small circles do not touch each other."""
+    from .dem import DemScene, circle_mass
+
+    rng = np.random.default_rng(seed)
+    n = int(2.0 * math.pi * (R + r) / (2.0 * r))
+    pitch = 2.0 * (R + 2.0 * r) + 2e-3
+    pts, rad = [], []
+    for a in range(nx):
+        for b in range(ny):
+            c = np.array([R + 2 * r + 1e-3 + a * pitch, R + 2 * r + 5e-3 + b * pitch])
+            pts.append(c)
+            rad.append(R)
+            th0 = rng.uniform(0.0, 2.0 * math.pi)
+            for i in range(n):
+                th = th0 + 2.0 * math.pi * i / n
+                pts.append(c + (R + r - overlap) * np.array([math.cos(th), math.sin(th)]))
+                rad.append(r)
+    q = np.zeros((len(pts), 3))
+    q[:, :2] = np.asarray(pts)
+    radius = np.asarray(rad)
+    gn = damping_from_restitution(0.3, kn, 0.5 * 2500.0 * math.pi * r * r)
+    s = DemScene(q=q, v=np.zeros_like(q), radius=radius, mass=circle_mass(2500.0, radius),
+                 fixed=np.zeros(len(pts), np.uint8), gravity=np.array([0.0, -9.81]), kn=kn,
+                 kt=kn, gamman=gn, gammat=gn, mu=0.3)
+    s.add_plane([0.0, 0.0], [0.0, 1.0])
+    s.add_plane([0.0, 0.0], [1.0, 0.0])
+    return s
+
+
 # ---- hybrid --------------------------------------------------------------------
 
 def hybrid_column(

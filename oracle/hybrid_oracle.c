@@ -15,7 +15,11 @@
  * stress tensors computed independently from the contact list (partition of
  * unity), and a compressed head-on pair gives -|f||r| along the axis;
  * homogenised velocity: grains moving together give that velocity at every
- * node they reach, once per step.
+ * node they reach, once per step.  In tests/test_oracle_pins.py: the FLIP
+ * transfer gives the grains the exchange multipliers' sum (continuum plus grain
+ * momentum unchanged from before the exchange to after the transfer), coupled
+ * positions advance from the pre-step position with the transferred velocity,
+ * and one stencil node at or below 1e-6 keeps a grain uncoupled.
  */
 #include "hg_oracle.h"
 

@@ -11,7 +11,11 @@
  * conservation (PhysicsGrid.cpp:207, :316-327), zero net internal force
  * (PhysicsGrid.cpp:545), exact free fall, plane sticking/sliding closed forms,
  * the return-map invariants (det preserved, yield surface hit exactly,
- * ConstitutiveModel.cpp:62-77) and the no-tension branch.
+ * ConstitutiveModel.cpp:62-77) and the no-tension branch.  In
+ * tests/test_oracle_pins.py: the internal force's sign and size (a uniformly
+ * compressed block's nodal forces are its pressure integrated over its edges:
+ * zero inside, -tau H outward) and the side of the PIC/FLIP blend (alpha 0
+ * and 1 on linear grid fields).
  */
 #include "hg_oracle.h"
 
