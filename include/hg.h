@@ -336,6 +336,22 @@ hg_status hg_hybrid_estimate_bytes(const hg_mpm_desc *mpm, const hg_dem_desc *de
                                    uint64_t *bytes);
 hg_status hg_hybrid_device_bytes(hg_hybrid *h, uint64_t *bytes);
 
+/* ---- observables (no reference counterpart: the reference writes its state
+ * to HDF5 and leaves measurement to scripts) ------------------------------------
+ * What a long granular collapse is judged by, from the resident state of one
+ * scene: out[0] the kinetic energy sum 1/2 m |v|^2 (translational, per unit
+ * depth), out[1] the largest x of the points (bodies), out[2] the largest y,
+ * and out[3 + k] for k < nbins the largest y of the points whose x lies in
+ * [x0 + k bin_width, x0 + (k + 1) bin_width), NaN for an empty bin (the pile
+ * profile).  out holds 3 + nbins doubles.  Waits for the handle's stream;
+ * results are bitwise reproducible.  The handle keeps a device scratch buffer
+ * for the per-workgroup partials and the bins, allocated on the first call
+ * and grown when a call asks for more bins. */
+hg_status hg_mpm_observables(hg_mpm *h, uint32_t scene, double x0, double bin_width,
+                             uint32_t nbins, double *out);
+hg_status hg_dem_observables(hg_dem *h, uint32_t scene, double x0, double bin_width,
+                             uint32_t nbins, double *out);
+
 /* ---- snapshot and resume -------------------------------------------------------
  * The reference's CLIs serialise their host state for a bit-identical resume
  * (mpmgrains2d_cli.cpp:154-247, the -s/-r options); the DEM contact history

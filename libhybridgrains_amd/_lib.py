@@ -217,6 +217,10 @@ def lib() -> C.CDLL:
         "hg_server_error": ([vp, C.c_uint64, C.c_char_p, C.c_size_t], C.c_int32),
         "hg_server_release": ([vp, C.c_uint64], C.c_int32),
         "hg_server_stats_get": ([vp, C.POINTER(ServerStats)], C.c_int32),
+        "hg_mpm_observables": ([vp, C.c_uint32, C.c_double, C.c_double, C.c_uint32, _dp],
+                               C.c_int32),
+        "hg_dem_observables": ([vp, C.c_uint32, C.c_double, C.c_double, C.c_uint32, _dp],
+                               C.c_int32),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)

@@ -35,7 +35,7 @@ HIP_FLAGS = [
 ]
 
 HIP_SOURCES = ["hg_ctx.hip", "hg_sort.hip", "hg_mpm.hip", "hg_dem.hip", "hg_hybrid.hip", "hg_server.hip",
-               "hg_snapshot.hip"]
+               "hg_snapshot.hip", "hg_observe.hip"]
 
 HOST_LIB = PKG / "libhg_host.so"
 HOST_SOURCES = ["hg_xml.cpp", "hg_scene.cpp", "hg_sim.cpp", "hg_resample.cpp", "hg_host_capi.cpp"]

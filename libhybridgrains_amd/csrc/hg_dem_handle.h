@@ -109,6 +109,8 @@ struct hg_dem {
   double dt_dev = -1.0;
   KernelProfiler prof;
   hipGraphExec_t graph = nullptr;
+  uint64_t *obs = nullptr;  // scratch of hg_*_observables (allocated on first use)
+  size_t obs_words = 0;
   std::vector<void *> allocs;
   uint64_t bytes = 0;  // device bytes allocated
   size_t tsize() const { return precision == HG_FP64 ? 8 : 4; }

@@ -41,6 +41,8 @@ struct hg_mpm {
   double dt_dev = -1.0;  // dt currently in the device scene table
   KernelProfiler prof;
   hipGraphExec_t graph = nullptr;
+  uint64_t *obs = nullptr;  // scratch of hg_*_observables (allocated on first use)
+  size_t obs_words = 0;
   std::vector<void *> allocs;
   uint64_t bytes = 0;  // device bytes allocated
 

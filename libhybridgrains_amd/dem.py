@@ -200,6 +200,17 @@ class GpuDem:
         """As GpuMpm.hold."""
         self.ctx.check(self.L.hg_dem_hold(self._h, float(us)))
 
+    def observables(self, x0: float = 0.0, bin_width: float = 1.0, nbins: int = 0,
+                    scene: int = 0) -> dict:
+        """Kinetic energy, front (largest x), top (largest y) and the height
+        profile over nbins x-bins from x0 (NaN where a bin is empty), computed
+        on the device from the resident state."""
+        out = np.zeros(3 + int(nbins))
+        self.ctx.check(self.L.hg_dem_observables(self._h, scene, float(x0), float(bin_width), int(nbins),
+                                              ptr(out)))
+        return {"ke": float(out[0]), "front": float(out[1]), "top": float(out[2]),
+                "profile": out[3:].copy()}
+
     def kernel_times(self) -> dict:
         return _kernel_times(self.ctx, self.L.hg_dem_kernel_times, self._h)
 

@@ -214,3 +214,46 @@ def test_served_full_size_range_alone_equals_mix(hg_ctx):
     st = srv.stats()
     assert st["completed"] == len(reqs) and st["failed"] == 0
     srv.close()
+
+
+def _host_observables(q, v, m, x0, w, nbins):
+    ke = float(np.sum(0.5 * m * (v[:, 0] ** 2 + v[:, 1] ** 2)))
+    prof = np.full(nbins, np.nan)
+    k = np.floor((q[:, 0] - x0) / w)
+    ok = (k >= 0) & (k < nbins)
+    for b in np.unique(k[ok]).astype(int):
+        prof[b] = q[ok & (k == b), 1].max()
+    return ke, q[:, 0].max(), q[:, 1].max(), prof
+
+
+@pytest.mark.parametrize("precision", [HG_FP64, HG_FP32])
+def test_observables_match_the_downloaded_state(hg_ctx, precision):
+    s = scenes.mpm_config0(seed=4)
+    g = GpuMpm(s, precision, ctx=hg_ctx)
+    g.flow(1e-5, 300)
+    p = g.points()
+    # the masses as the handle stores them (FP32 mode rounds them)
+    cast = np.float32 if precision == HG_FP32 else np.float64
+    m = np.asarray(s.m).astype(cast).astype(float)
+    ob = g.observables(x0=0.0, bin_width=0.01, nbins=100)
+    ke, fx, ty, prof = _host_observables(p["q"], p["v"], m, 0.0, 0.01, 100)
+    assert ke > 0 and abs(ob["ke"] - ke) <= 1e-12 * ke
+    assert ob["front"] == fx and ob["top"] == ty
+    assert np.array_equal(np.isnan(ob["profile"]), np.isnan(prof))
+    assert np.array_equal(ob["profile"][~np.isnan(prof)], prof[~np.isnan(prof)])
+    # reproducible, and fewer bins reuse the same scratch
+    again = g.observables(x0=0.0, bin_width=0.01, nbins=100)
+    assert again["ke"] == ob["ke"] and np.array_equal(again["profile"], ob["profile"],
+                                                      equal_nan=True)
+    few = g.observables(x0=0.05, bin_width=0.02, nbins=7)
+    _, _, _, prof7 = _host_observables(p["q"], p["v"], m, 0.05, 0.02, 7)
+    assert np.array_equal(few["profile"], prof7, equal_nan=True)
+    d = scenes.dem_column(n_bodies=3000, seed=4)
+    gd = GpuDem(d, precision, ctx=hg_ctx)
+    gd.flow(1e-5, 200)
+    st = gd.state()
+    od = gd.observables(x0=0.0, bin_width=0.005, nbins=64)
+    ke, fx, ty, prof = _host_observables(st["q"][:, :2], st["v"][:, :2],
+                                         d.mass[:, 0].astype(cast).astype(float), 0.0, 0.005, 64)
+    assert abs(od["ke"] - ke) <= 1e-12 * ke and od["front"] == fx and od["top"] == ty
+    assert np.array_equal(od["profile"], prof, equal_nan=True)
