@@ -351,6 +351,15 @@ hg_status hg_mpm_observables(hg_mpm *h, uint32_t scene, double x0, double bin_wi
                              uint32_t nbins, double *out);
 hg_status hg_dem_observables(hg_dem *h, uint32_t scene, double x0, double bin_width,
                              uint32_t nbins, double *out);
+/* hg_mpm_observables without allocating: no reference counterpart; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same results,
+ * bitwise.  Its scratch is a static device array of 90 KiB on each GPU, held
+ * until the process ends; nbins above 8192 is refused with HG_ERR_INVALID.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (from several threads or streams) race. */
+hg_status hg_mpm_observables_noalloc(hg_mpm *h, uint32_t scene, double x0, double bin_width,
+                                     uint32_t nbins, double *out);
 
 /* ---- snapshot and resume -------------------------------------------------------
  * The reference's CLIs serialise their host state for a bit-identical resume

@@ -221,6 +221,8 @@ def lib() -> C.CDLL:
                                C.c_int32),
         "hg_dem_observables": ([vp, C.c_uint32, C.c_double, C.c_double, C.c_uint32, _dp],
                                C.c_int32),
+        "hg_mpm_observables_noalloc": ([vp, C.c_uint32, C.c_double, C.c_double, C.c_uint32,
+                                        _dp], C.c_int32),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)

@@ -16,6 +16,11 @@ extern "C" hg_status hg_ctx_create(int32_t device, hg_ctx **out) {
   if (hipSetDevice(device) != hipSuccess) return HG_ERR_HIP;
   hg_ctx *c = new hg_ctx();
   c->device = device;
+  const hg_status st = hg_observe_init(c);
+  if (st != HG_OK) {
+    delete c;
+    return st;
+  }
   *out = c;
   return HG_OK;
 }

@@ -16,6 +16,10 @@ struct hg_ctx {
 
 hg_status hg_fail(hg_ctx *ctx, hg_status code, const std::string &msg);
 
+// Per-device set-up of the observables' static scratch (hg_observe.hip);
+// hg_ctx_create calls it with the context's device current.
+hg_status hg_observe_init(hg_ctx *ctx);
+
 #define HG_HIP(ctx, expr)                                                        \
   do {                                                                           \
     hipError_t e_ = (expr);                                                      \

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #define HG_TRY_OBS(expr)          \
   do {                            \
@@ -196,7 +197,36 @@ ObsArgs mpm_args(hg_mpm *h, uint32_t scene, double x0, double bin_width, uint32_
   return a;
 }
 
+// ---- allocation-free variant ---------------------------------------------------
+// Largest bin count hg_mpm_observables_noalloc accepts.
+constexpr uint32_t kObsStaticBins = 8192;
+constexpr int kObsMaxDevices = 64;
+
 }  // namespace
+
+// Scratch of hg_mpm_observables_noalloc: 90 KiB on every GPU the library runs
+// on, held until the process ends.  In host code this name is a host-side
+// stand-in, not a device address: host code uses the per-device address
+// hipGetSymbolAddress returns (obs_static), never the name.
+static __device__ __align__(256) uint64_t scratch[3 * kObsBlocks + 4 + kObsStaticBins];
+
+static uint64_t *obs_static[kObsMaxDevices];
+static std::mutex obs_static_mu;
+
+// Looks up this device's copy of the scratch array once per device (the
+// device is current: hg_ctx_create calls this after hipSetDevice).
+hg_status hg_observe_init(hg_ctx *ctx) {
+  if (ctx->device < 0 || ctx->device >= kObsMaxDevices) return HG_OK;
+  std::lock_guard<std::mutex> lk(obs_static_mu);
+  if (obs_static[ctx->device]) return HG_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return hg_fail(ctx, HG_ERR_HIP, std::string("observables scratch: hipGetSymbolAddress: ") +
+                                        hipGetErrorString(e));
+  obs_static[ctx->device] = static_cast<uint64_t *>(p);
+  return HG_OK;
+}
 
 extern "C" hg_status hg_mpm_observables(hg_mpm *h, uint32_t scene, double x0, double bin_width,
                                         uint32_t nbins, double *out) {
@@ -228,6 +258,27 @@ extern "C" hg_status hg_dem_observables(hg_dem *h, uint32_t scene, double x0, do
   a.inv_w = nbins ? 1.0 / bin_width : 0.0;
   a.nbins = nbins;
   a.s = h->obs;
+  return h->precision == HG_FP64 ? obs_run<double>(h->ctx, h->stream, a, out)
+                                 : obs_run<float>(h->ctx, h->stream, a, out);
+}
+
+// hg_mpm_observables without any allocation: the same kernels on the
+// device's static scratch array.  Calls that overlap on one GPU (handles on
+// different streams or threads) share that array and race.
+extern "C" hg_status hg_mpm_observables_noalloc(hg_mpm *h, uint32_t scene, double x0,
+                                                double bin_width, uint32_t nbins, double *out) {
+  if (!h || scene >= h->nscenes) return HG_ERR_INVALID;
+  HG_TRY_OBS(check_args(h->ctx, bin_width, nbins, out));
+  if (nbins > kObsStaticBins)
+    return hg_fail(h->ctx, HG_ERR_INVALID,
+                   "at most " + std::to_string(kObsStaticBins) + " bins without allocation (asked " +
+                       std::to_string(nbins) + "); hg_mpm_observables takes more");
+  if (h->phase_next != 0) return hg_fail(h->ctx, HG_ERR_INVALID, "a phased step is in progress");
+  const int dev = h->ctx->device;
+  uint64_t *scr = (dev >= 0 && dev < kObsMaxDevices) ? obs_static[dev] : nullptr;
+  if (!scr) return hg_fail(h->ctx, HG_ERR_STATE, "observables scratch not set up on this device");
+  ObsArgs a = mpm_args(h, scene, x0, bin_width, nbins);
+  a.s = scr;
   return h->precision == HG_FP64 ? obs_run<double>(h->ctx, h->stream, a, out)
                                  : obs_run<float>(h->ctx, h->stream, a, out);
 }
