@@ -119,7 +119,11 @@ hg_status hg_mpm_download_points(hg_mpm *h, double *q, double *v, double *be_bar
  * (flat index y*(cell_count[0]+1)+x, PhysicsGrid.cpp:76-80).  Fields not kept
  * by the last step's mode are returned as zeros; hg_mpm_set_keep_grid(h, 1)
  * keeps all of them (mass, momentum, momentum_new, force, velocity,
- * acceleration), as PhysicsGrid holds after flow(). */
+ * acceleration), as PhysicsGrid holds after flow().  Without it an FP32
+ * handle runs the fused step, which ends each step with the next step's grid
+ * update: its velocity and acceleration are then those of the grid of the
+ * current points (the step that would come next), and the other fields are
+ * zeros. */
 hg_status hg_mpm_set_keep_grid(hg_mpm *h, int32_t keep_all);
 hg_status hg_mpm_download_grid(hg_mpm *h, double *mass, double *mom,
                                double *mom_new, double *force, double *vel,

@@ -762,6 +762,8 @@ extern "C" hg_status hg_hybrid_step_graph(hg_hybrid *h, double dt, uint32_t nste
     if (!h->graph) {
       // the MPM dt upload synchronises; do it before capturing
       HG_TRY(hg_mpm_step(h->mpm, dt, 0));
+      // each captured step finds the continuum's node sums ready
+      hg_mpm_prime(h->mpm);
       hipGraph_t g = nullptr;
       HG_HIP(h->ctx, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
       hybrid_steps(h, dt, 1, false);

@@ -118,6 +118,75 @@ struct P2GArgs {
   uint32_t store;
 };
 
+// The grid update of one node from its sums (mass m, momentum p, internal
+// force f): gravity, PhysicsGrid::updateMomentum (PhysicsGrid.cpp:565-586),
+// resolvePlaneCollisions (:611-673) and
+// lumpedMassVelocityAndAccelerationUpdate (:588-609), writing what store
+// asks for.
+template <typename T>
+__device__ __forceinline__ void grid_node_update(const SceneDev &sc, int nx, int ny, double m,
+                                                 double p0, double p1, double f0, double f1,
+                                                 uint32_t store, void *gva, double *gm,
+                                                 double *gp, double *gpn, double *gf) {
+  using T4 = typename Vec4T<T>::type;
+  const double dt = sc.dt;
+  const uint32_t node = sc.node_off + uint32_t(ny) * uint32_t(sc.snx) + uint32_t(nx);
+  f0 += m * sc.g0;
+  f1 += m * sc.g1;
+  double pn0 = 0.0, pn1 = 0.0;
+  if (m > 0.0) {
+    pn0 = p0 + dt * f0;
+    pn1 = p1 + dt * f1;
+  }
+  const double px0 = sc.min0 + sc.h * double(nx);
+  const double px1 = sc.min1 + sc.h * double(ny);
+  for (int k = 0; k < sc.nplanes; ++k) {
+    if (px1 < sc.plane_lower[k]) continue;
+    const double n0 = sc.plane_n[2 * k], n1 = sc.plane_n[2 * k + 1];
+    const double d = n0 * (px0 - sc.plane_x[2 * k]) + n1 * (px1 - sc.plane_x[2 * k + 1]);
+    if (d > 0.0) continue;
+    if (sc.plane_beh[k] == HG_PLANE_SLIDING) {
+      if (m <= 0.0) continue;
+      double u0 = pn0 / m, u1 = pn1 / m;
+      const double vn = n0 * u0 + n1 * u1;
+      if (vn > 0.0) continue;
+      u0 -= vn * n0;
+      u1 -= vn * n1;
+      pn0 = m * u0;
+      pn1 = m * u1;
+    } else {
+      pn0 = 0.0;
+      pn1 = 0.0;
+    }
+  }
+  if (store & kStoreVelAcc) {
+    T4 va;
+    if (m > 0.0) {
+      const double dm = dt * m;
+      va.x = T(pn0 / m);
+      va.y = T(pn1 / m);
+      va.z = T((pn0 - p0) / dm);
+      va.w = T((pn1 - p1) / dm);
+    } else {
+      va.x = va.y = va.z = va.w = T(0);
+    }
+    static_cast<T4 *>(gva)[node] = va;
+  }
+  if (store & kStoreMass) gm[node] = m;
+  if (store & kStoreMom) {
+    gp[2 * node] = p0;
+    gp[2 * node + 1] = p1;
+  }
+  if (store & kStoreMomNew) {
+    gpn[2 * node] = pn0;
+    gpn[2 * node + 1] = pn1;
+  }
+  if (store & kStoreForce) {
+    gf[2 * node] = f0;
+    gf[2 * node + 1] = f1;
+  }
+}
+
 // What P2G needs of one point besides its weights.
 template <typename T>
 struct P2GRec {
@@ -178,8 +247,8 @@ __device__ __forceinline__ P2GRec<T> p2g_record(const P2GRaw<T> &w, int bx, int 
   r.y = w.y;
   r.m = w.m;
   r.hl = w.hl;
-  const double J = double(w.J);
-  const double b00 = double(w.b00), b01 = double(w.b01), b11 = double(w.b11);
+  const double J = strain_load<T>(w.J);
+  const double b00 = strain_load<T>(w.b00), b01 = double(w.b01), b11 = strain_load<T>(w.b11);
   // Cauchy stress sigma = tau / J; tau = 0 when J > 1 (no tension).  FP64.
   double t00 = 0.0, t01 = 0.0, t11 = 0.0;
   if (J <= 1.0) {
@@ -479,68 +548,9 @@ __global__ __launch_bounds__(TW *TH) void mpm_p2g_kernel(P2GArgs args) {
     }
 
     // 3. grid update of the node.
-    if (nx < sc.nnx && ny < sc.nny) {
-      using T4 = typename Vec4T<T>::type;
-      const double dt = sc.dt;
-      const uint32_t node = sc.node_off + uint32_t(ny) * uint32_t(sc.snx) + uint32_t(nx);
-      const double m = am;
-      const double p0 = ap0, p1 = ap1;
-      double f0 = af0, f1 = af1;
-      f0 += m * sc.g0;
-      f1 += m * sc.g1;
-      double pn0 = 0.0, pn1 = 0.0;
-      if (m > 0.0) {
-        pn0 = p0 + dt * f0;
-        pn1 = p1 + dt * f1;
-      }
-      const double px0 = sc.min0 + sc.h * double(nx);
-      const double px1 = sc.min1 + sc.h * double(ny);
-      for (int k = 0; k < sc.nplanes; ++k) {
-        if (px1 < sc.plane_lower[k]) continue;
-        const double n0 = sc.plane_n[2 * k], n1 = sc.plane_n[2 * k + 1];
-        const double d = n0 * (px0 - sc.plane_x[2 * k]) + n1 * (px1 - sc.plane_x[2 * k + 1]);
-        if (d > 0.0) continue;
-        if (sc.plane_beh[k] == HG_PLANE_SLIDING) {
-          if (m <= 0.0) continue;
-          double u0 = pn0 / m, u1 = pn1 / m;
-          const double vn = n0 * u0 + n1 * u1;
-          if (vn > 0.0) continue;
-          u0 -= vn * n0;
-          u1 -= vn * n1;
-          pn0 = m * u0;
-          pn1 = m * u1;
-        } else {
-          pn0 = 0.0;
-          pn1 = 0.0;
-        }
-      }
-      if (args.store & kStoreVelAcc) {
-        T4 va;
-        if (m > 0.0) {
-          const double dm = dt * m;
-          va.x = T(pn0 / m);
-          va.y = T(pn1 / m);
-          va.z = T((pn0 - p0) / dm);
-          va.w = T((pn1 - p1) / dm);
-        } else {
-          va.x = va.y = va.z = va.w = T(0);
-        }
-        static_cast<T4 *>(args.gva)[node] = va;
-      }
-      if (args.store & kStoreMass) args.gm[node] = m;
-      if (args.store & kStoreMom) {
-        args.gp[2 * node] = p0;
-        args.gp[2 * node + 1] = p1;
-      }
-      if (args.store & kStoreMomNew) {
-        args.gpn[2 * node] = pn0;
-        args.gpn[2 * node + 1] = pn1;
-      }
-      if (args.store & kStoreForce) {
-        args.gf[2 * node] = f0;
-        args.gf[2 * node + 1] = f1;
-      }
-    }
+    if (nx < sc.nnx && ny < sc.nny)
+      grid_node_update<T>(sc, nx, ny, am, ap0, ap1, af0, af1, args.store, args.gva, args.gm,
+                          args.gp, args.gpn, args.gf);
     __syncthreads();  // LDS is reused by the next tile
   }
 }
@@ -686,35 +696,29 @@ struct G2PArgs {
   int32_t *err;
 };
 
-template <typename T, int BASIS>
-__global__ __launch_bounds__(kG2PThreads) void mpm_g2p_kernel(G2PArgs a) {
+// The particle half of the step for one point: gathers the node (v, a) of
+// its stencil (fetch(j, i) gives node (bx + i, by + j)), then
+// MaterialPoints::computeVelGrad (MaterialPoints.cpp:296-322),
+// elasticPrediction (:324-348) with ConstitutiveModel::elasticPrediction
+// (ConstitutiveModel.cpp:19-22), plasticCorrection (:350-369) with the
+// Drucker-Prager return (ConstitutiveModel.cpp:26-81), updateVelocities
+// (:204-245), updatePositions (:247-288) and resolvePlaneCollisions
+// (:371-423).  J and b_bar come in and go out in FP64; the return map runs in
+// FP64 in every precision.
+template <typename T, int BASIS, typename Fetch>
+__device__ __forceinline__ void g2p_point(const SceneDev &sc, const Fetch &fetch, int bx, int by,
+                                          int cx, int cy, T hl, bool scripted, double &x,
+                                          double &y, T &v0, T &v1, double &J1, double &c00,
+                                          double &c01, double &c11) {
   constexpr int S = BasisTraits<BASIS>::S;
   using T4 = typename Vec4T<T>::type;
-  const uint32_t blk = hg_xcd_remap(blockIdx.x, a.nblocks_padded);
-  const uint32_t i = blk * kG2PThreads + threadIdx.x;
-  if (i >= a.npoints) return;
-  const uint32_t sid = scene_of_point(a.p_off, a.nscenes, i);
-  const SceneDev &sc = a.scenes[sid];
-  const uint32_t p = i;
-  const uint32_t o = a.pinv[i];
-
-  double x = a.src.x[p], y = a.src.y[p];
-  T v0 = ld<T>(a.src.vx, p), v1 = ld<T>(a.src.vy, p);
-  const double J0 = double(ld<T>(a.src.J, p));
-  const double b00 = double(ld<T>(a.src.b00, p));
-  const double b01 = double(ld<T>(a.src.b01, p));
-  const double b11 = double(ld<T>(a.src.b11, p));
-  const T hl = a.src.hl ? ld<T>(a.src.hl, p) : T(sc.hl_uniform);
-  const bool scripted = a.src.flags ? (a.src.flags[p] != 0) : false;
+  const double J0 = J1;
+  const double b00 = c00, b01 = c01, b11 = c11;
   const double dt = sc.dt;
-
-  int bx, by, cx, cy;
-  point_key<BASIS>(sc, x, y, double(hl), bx, by, cx, cy);
   T axs[S], dxs[S], ays[S], dys[S];
   basis_2d<T, BASIS>(x, y, hl, sc.min0, sc.min1, sc.h, T(sc.inv_h), bx, by, cx, cy, axs, dxs, ays,
                      dys);
 
-  const T4 *g = static_cast<const T4 *>(a.gva) + sc.node_off;
   T L00 = 0, L10 = 0, L01 = 0, L11 = 0;
   T vp0 = 0, vp1 = 0, ac0 = 0, ac1 = 0;
   if constexpr (std::is_same<T, float>::value) {
@@ -723,11 +727,10 @@ __global__ __launch_bounds__(kG2PThreads) void mpm_g2p_kernel(G2PArgs a) {
     // (fused multiply-adds throughout).
 #pragma unroll
     for (int j = 0; j < S; ++j) {
-      const uint32_t row = uint32_t(by + j) * uint32_t(sc.snx) + uint32_t(bx);
       T dvx = 0, dvy = 0, wvx = 0, wvy = 0, wax = 0, way = 0;
 #pragma unroll
       for (int ii = 0; ii < S; ++ii) {
-        const T4 nv = g[row + ii];
+        const T4 nv = fetch(j, ii);
         dvx = madd(nv.x, dxs[ii], dvx);
         dvy = madd(nv.y, dxs[ii], dvy);
         wvx = madd(nv.x, axs[ii], wvx);
@@ -747,10 +750,9 @@ __global__ __launch_bounds__(kG2PThreads) void mpm_g2p_kernel(G2PArgs a) {
   } else {
 #pragma unroll
     for (int j = 0; j < S; ++j) {
-      const uint32_t row = uint32_t(by + j) * uint32_t(sc.snx) + uint32_t(bx);
 #pragma unroll
       for (int ii = 0; ii < S; ++ii) {
-        const T4 nv = g[row + ii];
+        const T4 nv = fetch(j, ii);
         const T w = axs[ii] * ays[j];
         const T gx = dxs[ii] * ays[j];
         const T gy = axs[ii] * dys[j];
@@ -782,12 +784,12 @@ __global__ __launch_bounds__(kG2PThreads) void mpm_g2p_kernel(G2PArgs a) {
   // reciprocal (a last-bit FP64 difference); FP64 mode divides as written.
   constexpr bool f32 = std::is_same<T, float>::value;
   double detE = e00 * e11 - e01 * e01;
-  double J1 = sqrt(detE);
+  J1 = sqrt(detE);
   double nrm = sqrt(detE);
   const double inrm = f32 ? 1.0 / nrm : 0.0;
-  double c00 = f32 ? e00 * inrm : e00 / nrm;
-  double c01 = f32 ? e01 * inrm : e01 / nrm;
-  double c11 = f32 ? e11 * inrm : e11 / nrm;
+  c00 = f32 ? e00 * inrm : e00 / nrm;
+  c01 = f32 ? e01 * inrm : e01 / nrm;
+  c11 = f32 ? e11 * inrm : e11 / nrm;
 
   // Drucker-Prager return on b* = J b_bar
   {
@@ -853,6 +855,35 @@ __global__ __launch_bounds__(kG2PThreads) void mpm_g2p_kernel(G2PArgs a) {
       v1 = T(0);
     }
   }
+}
+
+template <typename T, int BASIS>
+__global__ __launch_bounds__(kG2PThreads) void mpm_g2p_kernel(G2PArgs a) {
+  using T4 = typename Vec4T<T>::type;
+  const uint32_t blk = hg_xcd_remap(blockIdx.x, a.nblocks_padded);
+  const uint32_t i = blk * kG2PThreads + threadIdx.x;
+  if (i >= a.npoints) return;
+  const uint32_t sid = scene_of_point(a.p_off, a.nscenes, i);
+  const SceneDev &sc = a.scenes[sid];
+  const uint32_t p = i;
+  const uint32_t o = a.pinv[i];
+
+  double x = a.src.x[p], y = a.src.y[p];
+  T v0 = ld<T>(a.src.vx, p), v1 = ld<T>(a.src.vy, p);
+  double J1 = strain_load<T>(ld<T>(a.src.J, p));
+  double c00 = strain_load<T>(ld<T>(a.src.b00, p));
+  double c01 = double(ld<T>(a.src.b01, p));
+  double c11 = strain_load<T>(ld<T>(a.src.b11, p));
+  const T hl = a.src.hl ? ld<T>(a.src.hl, p) : T(sc.hl_uniform);
+  const bool scripted = a.src.flags ? (a.src.flags[p] != 0) : false;
+
+  int bx, by, cx, cy;
+  point_key<BASIS>(sc, x, y, double(hl), bx, by, cx, cy);
+  const T4 *g = static_cast<const T4 *>(a.gva) + sc.node_off;
+  auto fetch = [&](int j, int ii) {
+    return g[uint32_t(by + j) * uint32_t(sc.snx) + uint32_t(bx) + uint32_t(ii)];
+  };
+  g2p_point<T, BASIS>(sc, fetch, bx, by, cx, cy, hl, scripted, x, y, v0, v1, J1, c00, c01, c11);
 
   // write the point to its sorted slot o of the other buffer and count its
   // new cell
@@ -860,10 +891,10 @@ __global__ __launch_bounds__(kG2PThreads) void mpm_g2p_kernel(G2PArgs a) {
   a.dst.y[o] = y;
   st<T>(a.dst.vx, o, v0);
   st<T>(a.dst.vy, o, v1);
-  st<T>(a.dst.b00, o, T(c00));
+  st<T>(a.dst.b00, o, strain_store<T>(c00));
   st<T>(a.dst.b01, o, T(c01));
-  st<T>(a.dst.b11, o, T(c11));
-  st<T>(a.dst.J, o, T(J1));
+  st<T>(a.dst.b11, o, strain_store<T>(c11));
+  st<T>(a.dst.J, o, strain_store<T>(J1));
   st<T>(a.dst.m, o, ld<T>(a.src.m, p));
   st<T>(a.dst.V, o, ld<T>(a.src.V, p));
   a.dst.id[o] = a.src.id[p];
@@ -905,6 +936,411 @@ __global__ __launch_bounds__(256) void mpm_bin_kernel(BinArgs a) {
   a.key[i] = k;
   a.slot[i] = atomicAdd(&a.counts[k], 1u);
   hg_scan_block_count(a.totals, k, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// FP32 step, fused: G2P of step n and the rasterisation (P2G) of step n + 1
+// in one pass over each active tile, then a grid kernel.
+//
+// mpm_fused_kernel, per tile of TW x TH owned cells (persistent workgroups
+// over the active-tile list, one contiguous range of it per XCD):
+//  * the tile's points are the TH runs of its owned cells in the current
+//    cell order (perm); they are loaded through perm (the buffer is ordered
+//    by the previous sort, so the loads are nearly contiguous);
+//  * G2P reads the node (v, a) of its stencil from an LDS copy of the
+//    tile's node window, runs the particle update (g2p_point) and writes the
+//    point once, to its position in the current order in the other buffer,
+//    with its new cell key for the next sort;
+//  * from the updated point it forms its rasterisation terms for the next
+//    step -- mass w m, momentum w m v and internal force -V J sigma grad w at
+//    each node of its new stencil (PhysicsGrid::rasterizePointMasses
+//    :168-207, rasterizePointMomentum :274-311, computeForces :413-473, with
+//    MaterialPoints::computeHyperelasticCauchyStress, MaterialPoints.cpp:
+//    183-202) -- in FP32, rounds each down to a whole number of the scene's
+//    quantum (fx_shift) and adds it into LDS accumulators covering the
+//    tile's nodes one cell beyond its points' reach; a point that moved
+//    further adds straight into the node sums in HBM after the pass;
+//  * the accumulators are then added into the node sums in HBM (atomics, at
+//    the tile borders shared with the neighbour tiles).
+// The sums are whole numbers of quanta in 64-bit integers, so every addition
+// is exact and the sums do not depend on the order of the additions: each
+// node's sum is a fixed function of the scene state -- bitwise reproducible,
+// the same alone or in a batch and for any tile shape.  In prime mode the kernel only rasterises the current points (no
+// particle update).
+//
+// mpm_grid_kernel, per node of the active tiles (the new sort's): converts
+// the sums back (fx_quantum), clears them for the next step and runs the
+// grid update (grid_node_update).
+// ---------------------------------------------------------------------------
+struct FusedArgs {
+  const SceneDev *scenes;
+  const uint32_t *tile_scene, *tile_list, *tile_count;
+  PtsRaw src, dst;
+  const uint32_t *perm, *cell_start;
+  const float4 *gva;
+  unsigned long long *acc;  // kFxQuantities arrays of nnode node sums (whole quanta)
+  uint32_t nnode;
+  uint32_t *key, *slot, *counts, *totals;
+  int32_t *err;
+};
+
+// A term already scaled to quanta, rounded down to a whole number of them, as
+// a two's-complement 64-bit integer (the caller bounds it below 2^50).  In
+// FP64 (full rate on this GPU): with t = scaled 2^-32, H = floor(t) is the
+// high word and (t - H) 2^32, in [0, 2^32), the low word; every step is exact.
+__device__ __forceinline__ unsigned long long fx_of(float scaled) {
+  const double t = ldexp(double(scaled), -32);
+  const double H = floor(t);
+  const uint32_t lo = uint32_t(ldexp(t - H, 32));
+  const int32_t hi = int32_t(H);
+  return (static_cast<unsigned long long>(static_cast<uint32_t>(hi)) << 32) | lo;
+}
+
+// A point's rasterisation terms in FP32: per axis the S weights and masked
+// weight-derivative factors of its stencil (base node bx, by; cntx, cnty
+// stencil nodes per axis), and its mass, momentum and -V J sigma.  The stress
+// is formed in FP64, as in P2G.
+template <int BASIS>
+struct P2GTerms {
+  static constexpr int S = BasisTraits<BASIS>::S;
+  float wx[S], gx[S], wy[S], gy[S];
+  // mass, momentum and -V J sigma, each times its quantity's 2^fx_shift
+  // (an exact power-of-two scaling)
+  float m, mv0, mv1, f00, f01, f11;
+};
+
+template <int BASIS>
+__device__ __forceinline__ void p2g_terms(const SceneDev &sc, double x, double y, float v0,
+                                          float v1, float m, float V, float hl, double J,
+                                          double b00, double b01, double b11, int bx, int by,
+                                          int cntx, int cnty, P2GTerms<BASIS> &o) {
+  constexpr int S = BasisTraits<BASIS>::S;
+  double t00 = 0.0, t01 = 0.0, t11 = 0.0;
+  if (J <= 1.0) {
+    const double e00 = b00 * J, e01 = b01 * J, e11 = b11 * J;
+    const double det = e00 * e11 - e01 * e01;
+    const double Jt = sqrt(det);
+    const double inrm = 1.0 / Jt;
+    const double bb00 = e00 * inrm, bb01 = e01 * inrm, bb11 = e11 * inrm;
+    const double half_tr = (bb00 + bb11) * 0.5;
+    const double c = sc.kappa * 0.5 * (Jt + 1.0) * (Jt - 1.0);
+    t00 = c + sc.mu * (bb00 - half_tr);
+    t01 = sc.mu * bb01;
+    t11 = c + sc.mu * (bb11 - half_tr);
+  }
+  const double iJ = 1.0 / J;
+  const float c = float(-double(V) * J);
+  const int sf = sc.fx_shift[kFxF0], sp = sc.fx_shift[kFxMom0];
+  o.f00 = ldexpf(c * float(t00 * iJ), sf);
+  o.f01 = ldexpf(c * float(t01 * iJ), sf);
+  o.f11 = ldexpf(c * float(t11 * iJ), sf);
+  o.m = ldexpf(m, sc.fx_shift[kFxMass]);
+  o.mv0 = ldexpf(m * v0, sp);
+  o.mv1 = ldexpf(m * v1, sp);
+  const float inv_h = float(sc.inv_h);
+  float inv_vp = 1.0f;
+  if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) inv_vp = 1.0f / (4.0f * hl * hl);
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    float ax, dx, ay, dy;
+    basis_factor<float, BASIS>(x, hl, sc.min0, sc.h, inv_h, bx + k, k, cntx, ax, dx);
+    basis_factor<float, BASIS>(y, hl, sc.min1, sc.h, inv_h, by + k, k, cnty, ay, dy);
+    float wxk, gxk, gyk;
+    if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) {
+      wxk = ax * inv_vp;
+      gxk = dx * inv_vp;
+      gyk = dy;
+    } else {
+      wxk = ax;
+      gxk = dx * inv_h;
+      gyk = dy * inv_h;
+    }
+    // a node's terms count only where its weight's factors are nonzero
+    // (PhysicsGrid.cpp:413-473 skips zero-weight nodes); with a zero factor
+    // every term of the node is then exactly zero
+    o.wx[k] = wxk;
+    o.gx[k] = (wxk != 0.0f) ? gxk : 0.0f;
+    o.wy[k] = ay;
+    o.gy[k] = (ay != 0.0f) ? gyk : 0.0f;
+  }
+}
+
+// False when a scaled term could reach 2^50 quanta (|w| <= 1, so the terms
+// are bounded by m, |m v| and (|f00| + 2 |f01| + |f11|) times the largest
+// weight-derivative factor), or is not finite.
+template <int BASIS>
+__device__ __forceinline__ bool p2g_terms_in_range(const P2GTerms<BASIS> &o) {
+  constexpr int S = BasisTraits<BASIS>::S;
+  float g = 0.0f;
+#pragma unroll
+  for (int k = 0; k < S; ++k) g = fmaxf(g, fmaxf(fabsf(o.gx[k]), fabsf(o.gy[k])));
+  const float fb = (fabsf(o.f00) + 2.0f * fabsf(o.f01) + fabsf(o.f11)) * g;
+  const float lim = 1.0e15f;  // below 2^50
+  return fabsf(o.m) < lim && fmaxf(fabsf(o.mv0), fabsf(o.mv1)) < lim && fb < lim;
+}
+
+// Adds a point's terms at the nodes of its stencil: add(q, i, j, value) for
+// quantity q at node (bx + i, by + j).
+template <int BASIS, typename Add>
+__device__ __forceinline__ void p2g_scatter(const P2GTerms<BASIS> &o, const Add &add) {
+  constexpr int S = BasisTraits<BASIS>::S;
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      const float w = o.wx[i] * o.wy[j];
+      const float gx = o.gx[i] * o.wy[j];
+      const float gy = o.wx[i] * o.gy[j];
+      add(kFxMass, i, j, fx_of(w * o.m));
+      add(kFxMom0, i, j, fx_of(w * o.mv0));
+      add(kFxMom1, i, j, fx_of(w * o.mv1));
+      add(kFxF0, i, j, fx_of(__builtin_fmaf(o.f01, gy, o.f00 * gx)));
+      add(kFxF1, i, j, fx_of(__builtin_fmaf(o.f11, gy, o.f01 * gx)));
+    }
+  }
+}
+
+// A point's stored state, as the fused kernel loads it.
+struct FusedPoint {
+  double x, y, J, c00, c01, c11;
+  float v0, v1, m, V, hl;
+};
+
+__device__ __forceinline__ FusedPoint fused_load(const PtsRaw &p, uint32_t i, const SceneDev &sc) {
+  FusedPoint q;
+  q.x = p.x[i];
+  q.y = p.y[i];
+  q.v0 = ld<float>(p.vx, i);
+  q.v1 = ld<float>(p.vy, i);
+  q.J = strain_load<float>(ld<float>(p.J, i));
+  q.c00 = strain_load<float>(ld<float>(p.b00, i));
+  q.c01 = double(ld<float>(p.b01, i));
+  q.c11 = strain_load<float>(ld<float>(p.b11, i));
+  q.m = ld<float>(p.m, i);
+  q.V = ld<float>(p.V, i);
+  q.hl = p.hl ? ld<float>(p.hl, i) : float(sc.hl_uniform);
+  return q;
+}
+
+// Points of a tile that moved more than one cell since the sort, whose terms
+// are added after the tile's pass straight into the node sums.
+constexpr int kFarCap = 256;
+
+template <int BASIS, int TW, int TH, bool PRIME>
+__global__ __launch_bounds__(TW *TH) void mpm_fused_kernel(FusedArgs a) {
+  constexpr int S = BasisTraits<BASIS>::S;
+  constexpr int NT = TW * TH;
+  constexpr int GW = TW + S - 1, GH = TH + S - 1;  // G2P node window
+  constexpr int RW = TW + S + 1, RH = TH + S + 1;  // rasterisation window
+  constexpr int RN = RW * RH;
+  __shared__ float4 s_va[PRIME ? 1 : GW * GH];
+  __shared__ unsigned long long s_acc[kFxQuantities][RN];
+  __shared__ uint32_t s_rb[TH], s_ro[TH + 1];
+  __shared__ uint32_t s_far[kFarCap];
+  __shared__ uint32_t s_nfar;
+
+  const uint32_t ntile_list = *a.tile_count;
+  const int t = threadIdx.x;
+  // Workgroup b runs on XCD b % 8 (the grid is a multiple of 8); each XCD
+  // takes one contiguous eighth of the list, so neighbouring tiles' shared
+  // nodes stay in one L2.
+  const uint32_t per_xcd = (ntile_list + 7u) >> 3, xcd = blockIdx.x & 7u;
+  for (uint32_t jt = blockIdx.x >> 3; jt < per_xcd; jt += gridDim.x >> 3) {
+    const uint32_t li = xcd * per_xcd + jt;
+    if (li >= ntile_list) break;
+    const uint32_t tile = a.tile_list[li];
+    const uint32_t sid = a.tile_scene[tile];
+    const SceneDev &sc = a.scenes[sid];
+    const uint32_t ltile = tile - sc.tile_off;
+    const int x0 = int(ltile % uint32_t(sc.tiles_x)) * TW;
+    const int y0 = int(ltile / uint32_t(sc.tiles_x)) * TH;
+    const uint32_t *cs = a.cell_start + sc.cell_off;
+    // the owned cells' points: one run of the current order per cell row
+    if (t < TH) {
+      const int ry = y0 + t;
+      uint32_t b = 0, e = 0;
+      if (ry < sc.nny) {
+        const uint32_t row = uint32_t(ry) * uint32_t(sc.snx);
+        b = cs[row + uint32_t(min(x0, sc.nnx))];
+        e = cs[row + uint32_t(min(x0 + TW, sc.nnx))];
+      }
+      s_rb[t] = b;
+      s_ro[t + 1] = e - b;
+    }
+    if (t == 0) s_nfar = 0;
+    if constexpr (!PRIME) {
+      for (int q = t; q < GW * GH; q += NT) {
+        const int nx = x0 + q % GW, ny = y0 + q / GW;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (nx < sc.snx && ny < sc.sny)
+          v = a.gva[sc.node_off + uint32_t(ny) * uint32_t(sc.snx) + uint32_t(nx)];
+        s_va[q] = v;
+      }
+    }
+    for (int q = t; q < kFxQuantities * RN; q += NT) (&s_acc[0][0])[q] = 0ull;
+    __syncthreads();
+    if (t == 0) {
+      uint32_t o = 0;
+      s_ro[0] = 0;
+      for (int r = 0; r < TH; ++r) {
+        o += s_ro[r + 1];
+        s_ro[r + 1] = o;
+      }
+    }
+    __syncthreads();
+    const uint32_t total = s_ro[TH];
+    const int wx0 = x0 - 1, wy0 = y0 - 1;  // origin of the rasterisation window
+
+    for (uint32_t k0 = 0; k0 < total; k0 += uint32_t(NT)) {
+      const uint32_t k = k0 + uint32_t(t);
+      if (k < total) {
+        int r = 0;
+#pragma unroll
+        for (int q = 1; q < TH; ++q) r += (s_ro[q] <= k) ? 1 : 0;
+        const uint32_t sp = s_rb[r] + (k - s_ro[r]);  // position in the current order
+        const uint32_t p = a.perm[sp];
+        FusedPoint q = fused_load(a.src, p, sc);
+        int bx, by, cx, cy;
+        bool ok = point_key<BASIS>(sc, q.x, q.y, double(q.hl), bx, by, cx, cy);
+        if constexpr (!PRIME) {
+          const bool scripted = a.src.flags ? (a.src.flags[p] != 0) : false;
+          const bool inwin = bx >= x0 && bx < x0 + TW && by >= y0 && by < y0 + TH;
+          const float4 *g = a.gva + sc.node_off;
+          auto fetch = [&](int j, int ii) {
+            return inwin ? s_va[(by - y0 + j) * GW + (bx - x0 + ii)]
+                         : g[uint32_t(by + j) * uint32_t(sc.snx) + uint32_t(bx + ii)];
+          };
+          g2p_point<float, BASIS>(sc, fetch, bx, by, cx, cy, q.hl, scripted, q.x, q.y, q.v0, q.v1,
+                                  q.J, q.c00, q.c01, q.c11);
+          // the point's one write, at its position in the current order; the
+          // next step's terms below use the stored (rounded) strain, as a
+          // rasterisation of the stored state (after a restore) does
+          const float sJ = strain_store<float>(q.J), s00 = strain_store<float>(q.c00),
+                      s01 = float(q.c01), s11 = strain_store<float>(q.c11);
+          q.J = strain_load<float>(sJ);
+          q.c00 = strain_load<float>(s00);
+          q.c01 = double(s01);
+          q.c11 = strain_load<float>(s11);
+          a.dst.x[sp] = q.x;
+          a.dst.y[sp] = q.y;
+          st<float>(a.dst.vx, sp, q.v0);
+          st<float>(a.dst.vy, sp, q.v1);
+          st<float>(a.dst.b00, sp, s00);
+          st<float>(a.dst.b01, sp, s01);
+          st<float>(a.dst.b11, sp, s11);
+          st<float>(a.dst.J, sp, sJ);
+          st<float>(a.dst.m, sp, q.m);
+          st<float>(a.dst.V, sp, q.V);
+          a.dst.id[sp] = a.src.id[p];
+          if (a.src.hl) st<float>(a.dst.hl, sp, q.hl);
+          if (a.src.flags) a.dst.flags[sp] = scripted ? 1 : 0;
+          ok = point_key<BASIS>(sc, q.x, q.y, double(q.hl), bx, by, cx, cy);
+          const bool finite = isfinite(q.x) && isfinite(q.y) && isfinite(double(q.v0)) &&
+                              isfinite(double(q.v1)) && isfinite(q.J);
+          if (!ok || !finite) atomicOr(&a.err[sid], (ok ? 0 : 1) | (finite ? 0 : 2));
+          const uint32_t key = sc.cell_off + uint32_t(by) * uint32_t(sc.snx) + uint32_t(bx);
+          a.key[sp] = key;
+          a.slot[sp] = hg_run_slot(a.counts, key);
+          hg_scan_block_count(a.totals, key, blockIdx.x);
+        }
+        // the point's terms for the next step, at its new stencil
+        if (bx >= wx0 && bx + S <= wx0 + RW && by >= wy0 && by + S <= wy0 + RH) {
+          P2GTerms<BASIS> o;
+          p2g_terms<BASIS>(sc, q.x, q.y, q.v0, q.v1, q.m, q.V, q.hl, q.J, q.c00, q.c01, q.c11, bx,
+                           by, cx, cy, o);
+          if (!p2g_terms_in_range<BASIS>(o)) atomicOr(&a.err[sid], kMpmFixedRangeBit);
+          unsigned long long *base = &s_acc[0][0] + (by - wy0) * RW + (bx - wx0);
+          p2g_scatter<BASIS>(o, [&](int f, int i, int j, unsigned long long v) {
+            atomicAdd(base + f * RN + j * RW + i, v);
+          });
+        } else {
+          // moved more than a cell since the sort (or left the grid): its
+          // terms go straight into the node sums after the pass
+          const uint32_t slot = atomicAdd(&s_nfar, 1u);
+          if (slot < uint32_t(kFarCap)) s_far[slot] = PRIME ? p : sp;
+          else atomicOr(&a.err[sid], kMpmFixedRangeBit);
+        }
+      }
+    }
+    __syncthreads();
+    // the window's sums into the node sums (nodes off the grid get nothing)
+    for (int q = t; q < RN; q += NT) {
+      const int nx = wx0 + q % RW, ny = wy0 + q / RW;
+      if (nx < 0 || ny < 0 || nx >= sc.nnx || ny >= sc.nny) continue;
+      const uint32_t node = sc.node_off + uint32_t(ny) * uint32_t(sc.snx) + uint32_t(nx);
+#pragma unroll
+      for (int f = 0; f < kFxQuantities; ++f) {
+        const unsigned long long v = s_acc[f][q];
+        if (v) atomicAdd(&a.acc[size_t(f) * a.nnode + node], v);
+      }
+    }
+    // the far points, from the state this pass wrote (or, priming, read)
+    const uint32_t nfar = min(s_nfar, uint32_t(kFarCap));
+    for (uint32_t f = uint32_t(t); f < nfar; f += uint32_t(NT)) {
+      const uint32_t i = s_far[f];
+      const FusedPoint q = fused_load(PRIME ? a.src : a.dst, i, sc);
+      int bx, by, cx, cy;
+      point_key<BASIS>(sc, q.x, q.y, double(q.hl), bx, by, cx, cy);
+      P2GTerms<BASIS> o;
+      p2g_terms<BASIS>(sc, q.x, q.y, q.v0, q.v1, q.m, q.V, q.hl, q.J, q.c00, q.c01, q.c11, bx, by,
+                       cx, cy, o);
+      if (!p2g_terms_in_range<BASIS>(o)) atomicOr(&a.err[sid], kMpmFixedRangeBit);
+      const uint32_t nb = sc.node_off + uint32_t(by) * uint32_t(sc.snx) + uint32_t(bx);
+      p2g_scatter<BASIS>(o, [&](int fq, int i2, int j2, unsigned long long v) {
+        atomicAdd(&a.acc[size_t(fq) * a.nnode + nb + uint32_t(j2) * uint32_t(sc.snx) + uint32_t(i2)],
+                  v);
+      });
+    }
+    __syncthreads();  // LDS is reused by the next tile
+  }
+}
+
+struct GridArgs {
+  const SceneDev *scenes;
+  const uint32_t *tile_scene, *tile_list, *tile_count;
+  unsigned long long *acc;
+  uint32_t nnode;
+  void *gva;
+  double *gm, *gp, *gpn, *gf;
+  uint32_t store;
+  int32_t *err;
+  int32_t tw, th;
+};
+
+// Node sums of 2^62 quanta or more are flagged: further additions could wrap
+// around.
+__global__ __launch_bounds__(256) void mpm_grid_kernel(GridArgs a) {
+  const uint32_t ntile_list = *a.tile_count;
+  const uint32_t per_xcd = (ntile_list + 7u) >> 3, xcd = blockIdx.x & 7u;
+  const int nt = a.tw * a.th;
+  for (uint32_t jt = blockIdx.x >> 3; jt < per_xcd; jt += gridDim.x >> 3) {
+    const uint32_t li = xcd * per_xcd + jt;
+    if (li >= ntile_list) break;
+    const uint32_t tile = a.tile_list[li];
+    const uint32_t sid = a.tile_scene[tile];
+    const SceneDev &sc = a.scenes[sid];
+    const uint32_t ltile = tile - sc.tile_off;
+    const int x0 = int(ltile % uint32_t(sc.tiles_x)) * a.tw;
+    const int y0 = int(ltile / uint32_t(sc.tiles_x)) * a.th;
+    for (int q = threadIdx.x; q < nt; q += 256) {
+      const int nx = x0 + q % a.tw, ny = y0 + q / a.tw;
+      if (nx >= sc.nnx || ny >= sc.nny) continue;
+      const uint32_t node = sc.node_off + uint32_t(ny) * uint32_t(sc.snx) + uint32_t(nx);
+      double v[kFxQuantities];
+      bool big = false;
+#pragma unroll
+      for (int f = 0; f < kFxQuantities; ++f) {
+        unsigned long long &w = a.acc[size_t(f) * a.nnode + node];
+        const long long iv = (long long)w;
+        w = 0ull;
+        big |= iv > (1ll << 62) || iv < -(1ll << 62);
+        v[f] = double(iv) * sc.fx_quantum[f];
+      }
+      if (big) atomicOr(&a.err[sid], kMpmFixedRangeBit);
+      grid_node_update<float>(sc, nx, ny, v[kFxMass], v[kFxMom0], v[kFxMom1], v[kFxF0], v[kFxF1],
+                              a.store, a.gva, a.gm, a.gp, a.gpn, a.gf);
+    }
+  }
 }
 
 }  // namespace
@@ -1113,6 +1549,120 @@ static void launch_velacc(hg_mpm *h) {
     hipLaunchKernelGGL(mpm_velacc_kernel<float>, dim3(p2g_grid(h)), dim3(256), 0, h->stream, a);
 }
 
+// ---- FP32 fused step ------------------------------------------------------------
+
+template <int BASIS, int TW, int TH>
+static void launch_fused_t(hg_mpm *h, bool prime) {
+  FusedArgs a;
+  a.scenes = h->d_scenes;
+  a.tile_scene = h->d_tile_scene;
+  a.tile_list = h->tile_list;
+  a.tile_count = h->tile_count + h->cur;
+  a.src = h->buf[h->cur];
+  a.dst = h->buf[1 - h->cur];
+  a.perm = h->perm;
+  a.cell_start = h->cell_start;
+  a.gva = static_cast<const float4 *>(h->gva);
+  a.acc = h->acc;
+  a.nnode = h->nnode;
+  a.key = h->key;
+  a.slot = h->slot;
+  a.counts = h->counts;
+  a.totals = h->scan_totals + size_t(1 - h->cur) * hg_scan_totals_words(h->nscan);  // the keys of buf[1 - cur]
+  a.err = h->err;
+  ProfScope ps(h->prof, prime ? "mpm_raster" : "mpm_fused", h->stream);
+  if (prime)
+    hipLaunchKernelGGL((mpm_fused_kernel<BASIS, TW, TH, true>), dim3(p2g_grid(h)), dim3(TW * TH),
+                       0, h->stream, a);
+  else
+    hipLaunchKernelGGL((mpm_fused_kernel<BASIS, TW, TH, false>), dim3(p2g_grid(h)), dim3(TW * TH),
+                       0, h->stream, a);
+}
+
+template <int BASIS>
+static void launch_fused_b(hg_mpm *h, bool prime) {
+  if (h->tw() == 32 && h->th() == 8) launch_fused_t<BASIS, 32, 8>(h, prime);
+  else if (h->tw() == 16 && h->th() == 8) launch_fused_t<BASIS, 16, 8>(h, prime);
+  else if (h->tw() == 64 && h->th() == 4) launch_fused_t<BASIS, 64, 4>(h, prime);
+  else launch_fused_t<BASIS, 16, 16>(h, prime);
+}
+
+// The fused kernel over the current sort: prime rasterises the points of
+// buf[cur] into the node sums; otherwise G2P moves them into buf[1 - cur]
+// (keyed for the next sort) and rasterises them there.
+static void launch_fused(hg_mpm *h, bool prime) {
+  switch (h->basis) {
+    case HG_BASIS_LINEAR: launch_fused_b<HG_BASIS_LINEAR>(h, prime); break;
+    case HG_BASIS_CUBIC: launch_fused_b<HG_BASIS_CUBIC>(h, prime); break;
+    default: launch_fused_b<HG_BASIS_UGIMP_LINEAR>(h, prime); break;
+  }
+}
+
+// The grid kernel over the current active tiles: node sums -> grid update,
+// sums cleared.
+static void launch_grid(hg_mpm *h, uint32_t store) {
+  GridArgs a;
+  a.scenes = h->d_scenes;
+  a.tile_scene = h->d_tile_scene;
+  a.tile_list = h->tile_list;
+  a.tile_count = h->tile_count + h->cur;
+  a.acc = h->acc;
+  a.nnode = h->nnode;
+  a.gva = h->gva;
+  a.gm = h->gm;
+  a.gp = h->gp;
+  a.gpn = h->gpn;
+  a.gf = h->gf;
+  a.store = store;
+  a.err = h->err;
+  a.tw = h->tw();
+  a.th = h->th();
+  ProfScope ps(h->prof, "mpm_grid", h->stream);
+  hipLaunchKernelGGL(mpm_grid_kernel, dim3(p2g_grid(h)), dim3(256), 0, h->stream, a);
+  h->acc_ready = false;
+  h->grid_ready = true;
+  h->grid_store = store;
+  h->last_store = store;
+}
+
+static bool fused_mode(const hg_mpm *h) { return h->acc && !h->keep_all; }
+
+// Consumes node sums still pending (they would otherwise leak into a later
+// rasterisation) and marks the grid as no longer the current points'.  Called
+// before anything that changes the points or the grid outside the fused step.
+static void fused_drain(hg_mpm *h) {
+  if (!h->acc) return;
+  if (h->acc_ready) launch_grid(h, kStoreVelAcc);
+  h->grid_ready = false;
+}
+
+// The node (v, a) of the current points, with at least the fields in store.
+static void fused_ensure_grid(hg_mpm *h, uint32_t store) {
+  if (h->grid_ready && (h->grid_store & store) == store) return;
+  if (!h->acc_ready) launch_fused(h, true);
+  launch_grid(h, store);
+}
+
+// One FP32 step: G2P (with the next step's rasterisation), the sort, the
+// grid update of the next step.
+static void fused_step(hg_mpm *h) {
+  fused_ensure_grid(h, kStoreVelAcc);
+  launch_fused(h, false);
+  h->cur = 1 - h->cur;
+  launch_sort(h);
+  launch_grid(h, kStoreVelAcc);
+}
+
+// (The node sums are all zero whenever acc_ready is false: only the grid
+// kernel clears acc_ready, and it clears every sum of the active tiles,
+// which hold all nonzero sums.)
+void hg_mpm_prime(hg_mpm *h) {
+  if (fused_mode(h) && !h->acc_ready) {
+    launch_fused(h, true);
+    h->acc_ready = true;
+  }
+}
+
 static hg_status upload_dt(hg_mpm *h, double dt) {
   if (dt == h->dt_dev) return HG_OK;
   // the table is read by queued kernels: wait for them before rewriting it
@@ -1121,6 +1671,8 @@ static hg_status upload_dt(hg_mpm *h, double dt) {
   HG_HIP(h->ctx, hipMemcpy(h->d_scenes, h->scenes.data(), h->scenes.size() * sizeof(SceneDev),
                            hipMemcpyHostToDevice));
   h->dt_dev = dt;
+  // the node (v, a) depend on dt; the node sums do not
+  h->grid_ready = false;
   if (h->graph) {
     hipGraphExecDestroy(h->graph);
     h->graph = nullptr;
@@ -1135,18 +1687,40 @@ static uint32_t fused_store(const hg_mpm *h) {
 
 hg_status hg_mpm_launch_coupled_p2g(hg_mpm *h, double dt) {
   HG_TRY(upload_dt(h, dt));
-  launch_p2g(h, kStoreVelAcc | kStoreMass | kStoreMom | kStoreMomNew |
-                    (h->keep_all ? kStoreForce : 0u));
+  const uint32_t store = kStoreVelAcc | kStoreMass | kStoreMom | kStoreMomNew |
+                         (h->keep_all ? kStoreForce : 0u);
+  if (fused_mode(h)) {
+    // the sums of the previous step's fused pass, or a rasterisation now
+    if (!h->acc_ready) launch_fused(h, true);
+    launch_grid(h, store);
+    return HG_OK;
+  }
+  fused_drain(h);
+  launch_p2g(h, store);
   return HG_OK;
 }
 
-void hg_mpm_launch_g2p(hg_mpm *h) { launch_g2p_and_sort(h); }
+void hg_mpm_launch_g2p(hg_mpm *h) {
+  if (fused_mode(h)) {
+    launch_fused(h, false);
+    h->cur = 1 - h->cur;
+    launch_sort(h);
+    h->acc_ready = true;
+    h->grid_ready = false;
+    return;
+  }
+  launch_g2p_and_sort(h);
+}
 
 // Bins the points of buf[cur] afresh (their block totals start from zero)
 // and sorts them.
 static void launch_rebin(hg_mpm *h) {
+  fused_drain(h);
   hg_zero_words(h->stream, h->scan_totals + size_t(h->cur) * hg_scan_totals_words(h->nscan),
                 hg_scan_totals_words(h->nscan));
+  // the list is rebuilt at the same parity: its counter holds the previous
+  // list's length, which the tile-list kernel would append to
+  hg_zero_words(h->stream, h->tile_count + h->cur, 1);
   if (h->npoints) {
     if (h->precision == HG_FP64) launch_bin_p<double>(h);
     else launch_bin_p<float>(h);
@@ -1162,6 +1736,11 @@ hg_status hg_mpm_rebin(hg_mpm *h) {
 }
 
 static void launch_step(hg_mpm *h) {
+  if (fused_mode(h)) {
+    fused_step(h);
+    return;
+  }
+  fused_drain(h);
   launch_p2g(h, fused_store(h));
   launch_g2p_and_sort(h);
 }
@@ -1203,10 +1782,10 @@ static void fill_points(const hg_mpm_desc &d, uint32_t id0, std::vector<double> 
     y.push_back(d.q[2 * i + 1]);
     f[0].push_back(T(d.v[2 * i]));
     f[1].push_back(T(d.v[2 * i + 1]));
-    f[2].push_back(T(d.be_bar[4 * i]));
+    f[2].push_back(strain_store<T>(d.be_bar[4 * i]));
     f[3].push_back(T(0.5 * (d.be_bar[4 * i + 1] + d.be_bar[4 * i + 2])));
-    f[4].push_back(T(d.be_bar[4 * i + 3]));
-    f[5].push_back(T(d.J[i]));
+    f[4].push_back(strain_store<T>(d.be_bar[4 * i + 3]));
+    f[5].push_back(strain_store<T>(d.J[i]));
     f[6].push_back(T(d.m[i]));
     f[7].push_back(T(d.volume[i]));
     f[8].push_back(T(d.hl[i]));
@@ -1250,6 +1829,40 @@ static hg_status upload_points(hg_mpm *h, const hg_mpm_desc *descs) {
     if (p.flags) HG_HIP(h->ctx, hipMemcpy(p.flags, flags.data(), N, hipMemcpyHostToDevice));
   }
   return HG_OK;
+}
+
+// Fixed-point scales of a scene's node sums (FP32 step).  Each quantity has
+// a unit that bounds one point's contribution at a node: mass m_max;
+// momentum m_max V with V the larger of the elastic wave speed
+// sqrt((kappa + 4 mu / 3) / rho_min) and the speed of a fall across the
+// domain; internal force V0_max (kappa + mu) / h (the return map keeps the
+// Kirchhoff stress within about kappa, and |grad w| h stays below 2).  The
+// quantum is 2^-40 of the unit rounded up to a power of two: a term stays
+// below 2^50 quanta (2^10 units) and a node sum below 2^62 (2^22 units).
+static void plan_fixed_point(const hg_mpm_desc &d, SceneDev &sd) {
+  double mmax = 0.0, vmax = 0.0, rho_min = INFINITY;
+  for (uint32_t i = 0; i < d.npoints; ++i) {
+    mmax = std::max(mmax, d.m[i]);
+    vmax = std::max(vmax, d.volume[i]);
+    rho_min = std::min(rho_min, d.m[i] / d.volume[i]);
+  }
+  const double wave = (rho_min > 0.0 && std::isfinite(rho_min))
+                          ? std::sqrt(std::max(0.0, d.bulk_modulus + 4.0 / 3.0 * d.shear_modulus) / rho_min)
+                          : 0.0;
+  const double L = d.cell_width * double(std::max(d.cell_count[0], d.cell_count[1]));
+  const double fall = std::sqrt(2.0 * std::hypot(d.gravity[0], d.gravity[1]) * L);
+  double V = std::max(wave, fall);
+  if (!(V > 0.0) || !std::isfinite(V)) V = 1.0;
+  double fu = vmax * (d.bulk_modulus + d.shear_modulus) / d.cell_width;
+  if (!(fu > 0.0) || !std::isfinite(fu)) fu = mmax > 0.0 ? mmax : 1.0;
+  const double unit[kFxQuantities] = {mmax > 0.0 ? mmax : 1.0, (mmax > 0.0 ? mmax : 1.0) * V,
+                                      (mmax > 0.0 ? mmax : 1.0) * V, fu, fu};
+  for (int q = 0; q < kFxQuantities; ++q) {
+    int e = 0;
+    std::frexp(unit[q], &e);  // unit <= 2^e
+    sd.fx_shift[q] = 40 - e;
+    sd.fx_quantum[q] = std::ldexp(1.0, e - 40);
+  }
 }
 
 static hg_status mpm_create_impl(hg_ctx *ctx, const hg_mpm_desc *descs, uint32_t nscenes,
@@ -1307,6 +1920,7 @@ static hg_status mpm_create_impl(hg_ctx *ctx, const hg_mpm_desc *descs, uint32_t
     sd.node_off = uint32_t(node_off);
     sd.tile_off = uint32_t(tile_off);
     sd.nplanes = int32_t(d.nplanes);
+    plan_fixed_point(d, sd);
     for (uint32_t k = 0; k < d.nplanes; ++k) {
       const double n0 = d.plane_n[2 * k], n1 = d.plane_n[2 * k + 1];
       const double nn = std::sqrt(n0 * n0 + n1 * n1);
@@ -1358,6 +1972,10 @@ static hg_status mpm_create_impl(hg_ctx *ctx, const hg_mpm_desc *descs, uint32_t
   HG_TRY(dalloc(hp, &h->gpn, size_t(h->nnode) * 16));
   HG_TRY(dalloc(hp, &h->gf, size_t(h->nnode) * 16));
   HG_TRY(dalloc(hp, &h->err, nscenes * 4));
+  if (h->precision == HG_FP32) {
+    HG_TRY(dalloc(hp, &h->acc, size_t(h->nnode) * kFxQuantities * 8));
+    HG_HIP(ctx, hipMemset(h->acc, 0, size_t(h->nnode) * kFxQuantities * 8));
+  }
   HG_HIP(ctx, hipMemcpy(h->d_scenes, h->scenes.data(), nscenes * sizeof(SceneDev), hipMemcpyHostToDevice));
   HG_HIP(ctx, hipMemcpy(h->d_p_off, h_p_off.data(), (nscenes + 1) * 4, hipMemcpyHostToDevice));
   HG_HIP(ctx, hipMemcpy(h->d_tile_scene, tile_scene.data(), h->ntiles * 4, hipMemcpyHostToDevice));
@@ -1410,6 +2028,8 @@ static hg_status check_err(hg_mpm *h) {
     if (e[s] & kSortTotalsBit) return hg_fail(h->ctx, HG_ERR_STATE, "cell sort: block totals do not match the counts");
     if (e[s] & 1) return hg_fail(h->ctx, HG_ERR_STATE, "scene " + std::to_string(s) + ": a material point left the grid");
     if (e[s] & 2) return hg_fail(h->ctx, HG_ERR_STATE, "scene " + std::to_string(s) + ": non-finite material point state");
+    if (e[s] & kMpmFixedRangeBit)
+      return hg_fail(h->ctx, HG_ERR_STATE, "scene " + std::to_string(s) + ": node sums outside the fixed-point range");
   }
   return HG_OK;
 }
@@ -1449,6 +2069,7 @@ extern "C" hg_status hg_mpm_phase(hg_mpm *h, int32_t phase, double dt) {
     // Mass and momentum; the forces and the grid update need only the point
     // state this phase leaves unchanged, so they are computed in the same pass
     // and kept until phase 1 hands them over.
+    fused_drain(h);
     h->phase_next = 1;
     HG_TRY(upload_dt(h, dt > 0.0 ? dt : (h->dt_dev > 0.0 ? h->dt_dev : 1.0)));
     launch_p2g(h, kStoreMass | kStoreMom | kStoreMomNew | kStoreForce);
@@ -1507,12 +2128,12 @@ static hg_status download_points_t(hg_mpm *h, uint32_t scene, double *q, double 
     if (q) { q[2 * i] = x[s]; q[2 * i + 1] = y[s]; }
     if (v) { v[2 * i] = double(f[0][s]); v[2 * i + 1] = double(f[1][s]); }
     if (be_bar) {
-      be_bar[4 * i] = double(f[2][s]);
+      be_bar[4 * i] = strain_load<T>(f[2][s]);
       be_bar[4 * i + 1] = double(f[3][s]);
       be_bar[4 * i + 2] = double(f[3][s]);
-      be_bar[4 * i + 3] = double(f[4][s]);
+      be_bar[4 * i + 3] = strain_load<T>(f[4][s]);
     }
-    if (J) J[i] = double(f[5][s]);
+    if (J) J[i] = strain_load<T>(f[5][s]);
   }
   return HG_OK;
 }
@@ -1533,6 +2154,7 @@ extern "C" hg_status hg_mpm_download_points(hg_mpm *h, double *q, double *v, dou
 
 extern "C" hg_status hg_mpm_set_keep_grid(hg_mpm *h, int32_t keep_all) {
   if (!h) return HG_ERR_INVALID;
+  if ((keep_all != 0) != h->keep_all) fused_drain(h);
   h->keep_all = keep_all != 0;
   if (h->graph) {
     hipGraphExecDestroy(h->graph);
@@ -1671,6 +2293,8 @@ extern "C" hg_status hg_mpm_step_graph(hg_mpm *h, double dt, uint32_t nsteps) {
   if (h->phase_next != 0) return hg_fail(h->ctx, HG_ERR_INVALID, "a phased step is in progress");
   if (h->prof.on) return hg_mpm_step(h, dt, nsteps);  // events are not captured
   HG_TRY(upload_dt(h, dt));
+  // every captured fused step expects the grid of its points ready
+  if (fused_mode(h)) fused_ensure_grid(h, kStoreVelAcc);
   // The point buffers alternate every step, so one graph holds a pair of
   // steps that starts and ends on buffer 0.
   if (h->cur != 0 && nsteps) {
@@ -1709,7 +2333,8 @@ extern "C" hg_status hg_mpm_estimate_bytes(const hg_mpm_desc *descs, uint32_t ns
     tiles += ((snx + 15) / 16) * ((sny + 7) / 8);
   }
   const uint64_t per_point = 2 * (16 + 9 * T + 4 + 1) + 16;
-  const uint64_t per_node = 8 + 4 * T + 8 + 16 * 3;  // cell lists, grid fields
+  // cell lists, grid fields, and in FP32 the fixed-point node sums
+  const uint64_t per_node = 8 + 4 * T + 8 + 16 * 3 + (T == 4 ? 8 * kFxQuantities : 0);
   const uint64_t nscan = (nodes + kScanChunk - 1) / kScanChunk;
   *bytes = N * per_point + nodes * per_node + 4 + nscan * 8 * kScanSubTotals + tiles * 5 +
            uint64_t(nscenes) * (sizeof(SceneDev) + 16) + (1u << 20);
@@ -1742,12 +2367,25 @@ extern "C" hg_status hg_mpm_autotune(hg_mpm *h, double dt, int32_t reps, int32_t
   double best = 1e300;
   int bi = 0;
   hg_status st = HG_OK;
+  // Fused FP32 handles time a rasterisation of the current points with its
+  // grid update, which leaves the state as it was (the node sums are exact,
+  // so the grid comes out the same); pending sums are consumed first.
+  const bool fused = fused_mode(h);
+  if (fused) fused_drain(h);
+  auto pass = [&]() {
+    if (fused) {
+      launch_fused(h, true);
+      launch_grid(h, kStoreVelAcc);
+    } else {
+      launch_p2g(h, kStoreVelAcc);
+    }
+  };
   for (int s = 0; s < 4 && st == HG_OK; ++s) {
     st = hg_mpm_set_tile(h, shapes[s][0], shapes[s][1]);
     if (st != HG_OK) break;
-    launch_p2g(h, kStoreVelAcc);  // warm-up
+    pass();  // warm-up
     hipEventRecord(e0, h->stream);
-    for (int r = 0; r < reps; ++r) launch_p2g(h, kStoreVelAcc);
+    for (int r = 0; r < reps; ++r) pass();
     hipEventRecord(e1, h->stream);
     if (hipEventSynchronize(e1) != hipSuccess) {
       st = hg_fail(h->ctx, HG_ERR_HIP, "autotune timing failed");

@@ -34,6 +34,13 @@ struct hg_mpm {
   uint32_t *tile_count = nullptr;  // [2]: active tiles of the list of parity cur
   void *gva = nullptr;
   double *gm = nullptr, *gp = nullptr, *gpn = nullptr, *gf = nullptr;
+  // FP32 handles: node sums in whole quanta (kFxQuantities arrays of nnode
+  // 64-bit integers) for the fused step.  acc_ready: they hold the next rasterisation
+  // of the current points; grid_ready: the node (v, a) (and grid_store's
+  // other fields) belong to the current points.
+  unsigned long long *acc = nullptr;
+  bool acc_ready = false, grid_ready = false;
+  uint32_t grid_store = 0;
   int32_t *err = nullptr;
   bool keep_all = false;
   uint32_t last_store = 0;
@@ -57,6 +64,9 @@ struct hg_mpm {
 // the sort.
 hg_status hg_mpm_launch_coupled_p2g(hg_mpm *h, double dt);
 void hg_mpm_launch_g2p(hg_mpm *h);
+// Before capturing steps into a graph: FP32 handles rasterise the current
+// points now if no step has (so every captured step finds the sums ready).
+void hg_mpm_prime(hg_mpm *h);
 // Re-keys and re-sorts the points of the current buffer, as at creation,
 // and waits for it (used after the buffer was rewritten).
 hg_status hg_mpm_rebin(hg_mpm *h);

@@ -25,6 +25,8 @@
 
 #include "hg_internal.h"
 
+#include <type_traits>
+
 namespace hgmpm {
 
 constexpr int kMaxPlanes = 8;
@@ -36,6 +38,17 @@ enum : uint32_t {
   kStoreMomNew = 8u,
   kStoreForce = 16u,
 };
+
+// The FP32 step's node sums are exact: each point-node contribution, computed
+// in FP32 and scaled by 2^shift (a per-scene power of two), is rounded down
+// to a whole number of quanta and summed as a 64-bit integer, so the sum does
+// not depend on the order in which the contributions arrive.  Quantities:
+// node mass, the two momentum components, the two internal-force
+// components.
+enum : int { kFxMass = 0, kFxMom0 = 1, kFxMom1 = 2, kFxF0 = 3, kFxF1 = 4, kFxQuantities = 5 };
+// MPM error bit: a contribution or a node sum outside the fixed-point range
+// (never expected for physical states; the step's results are then void).
+constexpr int32_t kMpmFixedRangeBit = 4;
 
 struct SceneDev {
   double min0, min1, h;
@@ -55,7 +68,26 @@ struct SceneDev {
   double plane_n[2 * kMaxPlanes];
   double plane_lower[kMaxPlanes];
   int32_t plane_beh[kMaxPlanes];
+  // node sums of the FP32 step: contribution * 2^fx_shift[q] is summed in
+  // whole quanta; fx_quantum[q] = 2^-fx_shift[q] converts back
+  int32_t fx_shift[kFxQuantities];
+  double fx_quantum[kFxQuantities];
 };
+
+// Stored strain.  FP32 mode keeps J - 1 and the diagonal of b_bar minus 1
+// (b01 as it is), so that FP32 resolves the strain relative to its own size
+// rather than to 1; FP64 mode keeps J and b_bar.  The conversions are exact in
+// FP64 for the stored FP32 values.
+template <typename T>
+__host__ __device__ __forceinline__ double strain_load(T v) {
+  if constexpr (std::is_same<T, float>::value) return 1.0 + double(v);
+  else return v;
+}
+template <typename T>
+__host__ __device__ __forceinline__ T strain_store(double v) {
+  if constexpr (std::is_same<T, float>::value) return float(v - 1.0);
+  else return v;
+}
 
 struct PtsRaw {
   double *x, *y;

@@ -5,7 +5,8 @@
 // homogenisation accumulators are not part of what it can save.  Here a
 // snapshot is a host blob:
 //
-//   header (magic "HGSNAP01", kind, precision, two counts, payload bytes)
+//   header (magic "HGSNAP02", kind, precision, two counts, payload bytes; format 02
+//   stores FP32 points' J and b_bar diagonal as their deviations from 1)
 //   payload: the arrays listed per kind below, back to back.
 //
 //  * MPM: every per-point array of the current buffer in its physical order
@@ -32,7 +33,7 @@
 
 namespace {
 
-constexpr char kMagic[8] = {'H', 'G', 'S', 'N', 'A', 'P', '0', '1'};
+constexpr char kMagic[8] = {'H', 'G', 'S', 'N', 'A', 'P', '0', '2'};
 enum : uint32_t { kKindMpm = 1, kKindDem = 2, kKindHybrid = 3 };
 
 struct SnapHeader {
