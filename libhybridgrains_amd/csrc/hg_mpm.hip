@@ -1104,7 +1104,16 @@ __device__ __forceinline__ void p2g_scatter(const P2GTerms<BASIS> &o, const Add 
 struct FusedPoint {
   double x, y, J, c00, c01, c11;
   float v0, v1, m, V, hl;
+  float sJ, s00, s01, s11;  // the strain as stored (FP32 deviations)
 };
+
+// The strain of a point from its stored form.
+__device__ __forceinline__ void fused_strain(FusedPoint &q) {
+  q.J = strain_load<float>(q.sJ);
+  q.c00 = strain_load<float>(q.s00);
+  q.c01 = double(q.s01);
+  q.c11 = strain_load<float>(q.s11);
+}
 
 __device__ __forceinline__ FusedPoint fused_load(const PtsRaw &p, uint32_t i, const SceneDev &sc) {
   FusedPoint q;
@@ -1112,10 +1121,11 @@ __device__ __forceinline__ FusedPoint fused_load(const PtsRaw &p, uint32_t i, co
   q.y = p.y[i];
   q.v0 = ld<float>(p.vx, i);
   q.v1 = ld<float>(p.vy, i);
-  q.J = strain_load<float>(ld<float>(p.J, i));
-  q.c00 = strain_load<float>(ld<float>(p.b00, i));
-  q.c01 = double(ld<float>(p.b01, i));
-  q.c11 = strain_load<float>(ld<float>(p.b11, i));
+  q.sJ = ld<float>(p.J, i);
+  q.s00 = ld<float>(p.b00, i);
+  q.s01 = ld<float>(p.b01, i);
+  q.s11 = ld<float>(p.b11, i);
+  fused_strain(q);
   q.m = ld<float>(p.m, i);
   q.V = ld<float>(p.V, i);
   q.hl = p.hl ? ld<float>(p.hl, i) : float(sc.hl_uniform);
@@ -1126,10 +1136,12 @@ __device__ __forceinline__ FusedPoint fused_load(const PtsRaw &p, uint32_t i, co
 // are added after the tile's pass straight into the node sums.
 constexpr int kFarCap = 256;
 
+template <int TW, int TH> constexpr int fused_threads() { return TW * TH; }
+
 template <int BASIS, int TW, int TH, bool PRIME>
-__global__ __launch_bounds__(TW *TH) void mpm_fused_kernel(FusedArgs a) {
+__global__ __launch_bounds__((fused_threads<TW, TH>())) void mpm_fused_kernel(FusedArgs a) {
   constexpr int S = BasisTraits<BASIS>::S;
-  constexpr int NT = TW * TH;
+  constexpr int NT = fused_threads<TW, TH>();
   constexpr int GW = TW + S - 1, GH = TH + S - 1;  // G2P node window
   constexpr int RW = TW + S + 1, RH = TH + S + 1;  // rasterisation window
   constexpr int RN = RW * RH;
@@ -1138,6 +1150,10 @@ __global__ __launch_bounds__(TW *TH) void mpm_fused_kernel(FusedArgs a) {
   __shared__ uint32_t s_rb[TH], s_ro[TH + 1];
   __shared__ uint32_t s_far[kFarCap];
   __shared__ uint32_t s_nfar;
+  // one pass's points after G2P (or as loaded, priming), for the terms
+  __shared__ double s_px[NT], s_py[NT];
+  __shared__ float s_pf[9][NT];
+  __shared__ uint32_t s_pi[NT];
 
   const uint32_t ntile_list = *a.tile_count;
   const int t = threadIdx.x;
@@ -1191,8 +1207,18 @@ __global__ __launch_bounds__(TW *TH) void mpm_fused_kernel(FusedArgs a) {
     const uint32_t total = s_ro[TH];
     const int wx0 = x0 - 1, wy0 = y0 - 1;  // origin of the rasterisation window
 
+    // Each pass takes NT points of the tile in two phases.  G2P runs on
+    // consecutive lanes (neighbouring points share node gathers and slot
+    // atomics) and leaves each moved point in LDS; the terms then run with
+    // lane l of wave w on point l * NW + w, so that the points of one cell
+    // (consecutive in the order) go to different waves and the lanes of one
+    // LDS atomic add to distinct nodes instead of queueing on one address.
+    constexpr int NW = NT / 64;
+    static_assert(NT % 64 == 0, "whole waves per tile");
+    const uint32_t lane_pt = uint32_t((t & 63) * NW + (t >> 6));
     for (uint32_t k0 = 0; k0 < total; k0 += uint32_t(NT)) {
       const uint32_t k = k0 + uint32_t(t);
+      uint32_t tag = 0xffffffffu;  // no point
       if (k < total) {
         int r = 0;
 #pragma unroll
@@ -1200,9 +1226,9 @@ __global__ __launch_bounds__(TW *TH) void mpm_fused_kernel(FusedArgs a) {
         const uint32_t sp = s_rb[r] + (k - s_ro[r]);  // position in the current order
         const uint32_t p = a.perm[sp];
         FusedPoint q = fused_load(a.src, p, sc);
-        int bx, by, cx, cy;
-        bool ok = point_key<BASIS>(sc, q.x, q.y, double(q.hl), bx, by, cx, cy);
         if constexpr (!PRIME) {
+          int bx, by, cx, cy;
+          point_key<BASIS>(sc, q.x, q.y, double(q.hl), bx, by, cx, cy);
           const bool scripted = a.src.flags ? (a.src.flags[p] != 0) : false;
           const bool inwin = bx >= x0 && bx < x0 + TW && by >= y0 && by < y0 + TH;
           const float4 *g = a.gva + sc.node_off;
@@ -1213,28 +1239,26 @@ __global__ __launch_bounds__(TW *TH) void mpm_fused_kernel(FusedArgs a) {
           g2p_point<float, BASIS>(sc, fetch, bx, by, cx, cy, q.hl, scripted, q.x, q.y, q.v0, q.v1,
                                   q.J, q.c00, q.c01, q.c11);
           // the point's one write, at its position in the current order; the
-          // next step's terms below use the stored (rounded) strain, as a
+          // next step's terms use the stored (rounded) strain, as a
           // rasterisation of the stored state (after a restore) does
-          const float sJ = strain_store<float>(q.J), s00 = strain_store<float>(q.c00),
-                      s01 = float(q.c01), s11 = strain_store<float>(q.c11);
-          q.J = strain_load<float>(sJ);
-          q.c00 = strain_load<float>(s00);
-          q.c01 = double(s01);
-          q.c11 = strain_load<float>(s11);
+          q.sJ = strain_store<float>(q.J);
+          q.s00 = strain_store<float>(q.c00);
+          q.s01 = float(q.c01);
+          q.s11 = strain_store<float>(q.c11);
           a.dst.x[sp] = q.x;
           a.dst.y[sp] = q.y;
           st<float>(a.dst.vx, sp, q.v0);
           st<float>(a.dst.vy, sp, q.v1);
-          st<float>(a.dst.b00, sp, s00);
-          st<float>(a.dst.b01, sp, s01);
-          st<float>(a.dst.b11, sp, s11);
-          st<float>(a.dst.J, sp, sJ);
+          st<float>(a.dst.b00, sp, q.s00);
+          st<float>(a.dst.b01, sp, q.s01);
+          st<float>(a.dst.b11, sp, q.s11);
+          st<float>(a.dst.J, sp, q.sJ);
           st<float>(a.dst.m, sp, q.m);
           st<float>(a.dst.V, sp, q.V);
           a.dst.id[sp] = a.src.id[p];
           if (a.src.hl) st<float>(a.dst.hl, sp, q.hl);
           if (a.src.flags) a.dst.flags[sp] = scripted ? 1 : 0;
-          ok = point_key<BASIS>(sc, q.x, q.y, double(q.hl), bx, by, cx, cy);
+          const bool ok = point_key<BASIS>(sc, q.x, q.y, double(q.hl), bx, by, cx, cy);
           const bool finite = isfinite(q.x) && isfinite(q.y) && isfinite(double(q.v0)) &&
                               isfinite(double(q.v1)) && isfinite(q.J);
           if (!ok || !finite) atomicOr(&a.err[sid], (ok ? 0 : 1) | (finite ? 0 : 2));
@@ -1243,7 +1267,39 @@ __global__ __launch_bounds__(TW *TH) void mpm_fused_kernel(FusedArgs a) {
           a.slot[sp] = hg_run_slot(a.counts, key);
           hg_scan_block_count(a.totals, key, blockIdx.x);
         }
-        // the point's terms for the next step, at its new stencil
+        s_px[t] = q.x;
+        s_py[t] = q.y;
+        s_pf[0][t] = q.v0;
+        s_pf[1][t] = q.v1;
+        s_pf[2][t] = q.m;
+        s_pf[3][t] = q.V;
+        s_pf[4][t] = q.hl;
+        s_pf[5][t] = q.sJ;
+        s_pf[6][t] = q.s00;
+        s_pf[7][t] = q.s01;
+        s_pf[8][t] = q.s11;
+        tag = PRIME ? p : sp;
+      }
+      s_pi[t] = tag;
+      __syncthreads();
+      // the moved points' terms for the next step, at their new stencils
+      const uint32_t ptag = s_pi[lane_pt];
+      if (ptag != 0xffffffffu) {
+        FusedPoint q;
+        q.x = s_px[lane_pt];
+        q.y = s_py[lane_pt];
+        q.v0 = s_pf[0][lane_pt];
+        q.v1 = s_pf[1][lane_pt];
+        q.m = s_pf[2][lane_pt];
+        q.V = s_pf[3][lane_pt];
+        q.hl = s_pf[4][lane_pt];
+        q.sJ = s_pf[5][lane_pt];
+        q.s00 = s_pf[6][lane_pt];
+        q.s01 = s_pf[7][lane_pt];
+        q.s11 = s_pf[8][lane_pt];
+        fused_strain(q);
+        int bx, by, cx, cy;
+        point_key<BASIS>(sc, q.x, q.y, double(q.hl), bx, by, cx, cy);
         if (bx >= wx0 && bx + S <= wx0 + RW && by >= wy0 && by + S <= wy0 + RH) {
           P2GTerms<BASIS> o;
           p2g_terms<BASIS>(sc, q.x, q.y, q.v0, q.v1, q.m, q.V, q.hl, q.J, q.c00, q.c01, q.c11, bx,
@@ -1255,12 +1311,13 @@ __global__ __launch_bounds__(TW *TH) void mpm_fused_kernel(FusedArgs a) {
           });
         } else {
           // moved more than a cell since the sort (or left the grid): its
-          // terms go straight into the node sums after the pass
+          // terms go straight into the node sums after the tile
           const uint32_t slot = atomicAdd(&s_nfar, 1u);
-          if (slot < uint32_t(kFarCap)) s_far[slot] = PRIME ? p : sp;
+          if (slot < uint32_t(kFarCap)) s_far[slot] = ptag;
           else atomicOr(&a.err[sid], kMpmFixedRangeBit);
         }
       }
+      __syncthreads();  // the stash is refilled by the next pass
     }
     __syncthreads();
     // the window's sums into the node sums (nodes off the grid get nothing)
@@ -1572,10 +1629,12 @@ static void launch_fused_t(hg_mpm *h, bool prime) {
   a.err = h->err;
   ProfScope ps(h->prof, prime ? "mpm_raster" : "mpm_fused", h->stream);
   if (prime)
-    hipLaunchKernelGGL((mpm_fused_kernel<BASIS, TW, TH, true>), dim3(p2g_grid(h)), dim3(TW * TH),
+    hipLaunchKernelGGL((mpm_fused_kernel<BASIS, TW, TH, true>), dim3(p2g_grid(h)),
+                       dim3(fused_threads<TW, TH>()),
                        0, h->stream, a);
   else
-    hipLaunchKernelGGL((mpm_fused_kernel<BASIS, TW, TH, false>), dim3(p2g_grid(h)), dim3(TW * TH),
+    hipLaunchKernelGGL((mpm_fused_kernel<BASIS, TW, TH, false>), dim3(p2g_grid(h)),
+                       dim3(fused_threads<TW, TH>()),
                        0, h->stream, a);
 }
 
@@ -2348,9 +2407,10 @@ extern "C" hg_status hg_mpm_device_bytes(hg_mpm *h, uint64_t *bytes) {
 }
 
 // Tile-shape autotuner: times the P2G pass (which reads the point state and
-// writes only grid fields that every step recomputes) for each supported
-// tile shape on the handle's own scenes and keeps the fastest.  The choice
-// changes no result: a node's sum order does not depend on the tile shape.
+// writes only grid fields that every step recomputes), or in FP32 whole fused
+// steps, for each supported tile shape on the handle's own scenes and keeps
+// the fastest.  The choice changes no result: a node's sum order does not
+// depend on the tile shape, and FP32 node sums are exact.
 extern "C" hg_status hg_mpm_autotune(hg_mpm *h, double dt, int32_t reps, int32_t *aw_out,
                                      int32_t *ah_out, double *us /* [4] or NULL */) {
   if (!h) return HG_ERR_INVALID;
@@ -2367,18 +2427,23 @@ extern "C" hg_status hg_mpm_autotune(hg_mpm *h, double dt, int32_t reps, int32_t
   double best = 1e300;
   int bi = 0;
   hg_status st = HG_OK;
-  // Fused FP32 handles time a rasterisation of the current points with its
-  // grid update, which leaves the state as it was (the node sums are exact,
-  // so the grid comes out the same); pending sums are consumed first.
+  // Fused FP32 handles time whole steps (G2P and rasterisation favour
+  // different tiles) and then return to the state they started from: the
+  // points and the error words are saved first and restored after, and a
+  // restored handle continues bitwise as the uninterrupted one would.
   const bool fused = fused_mode(h);
-  if (fused) fused_drain(h);
+  std::vector<uint8_t> saved;
+  std::vector<int32_t> saved_err(h->nscenes);
+  if (fused) {
+    uint64_t nb = 0;
+    HG_TRY(hg_mpm_snapshot(h, nullptr, &nb));
+    saved.resize(nb);
+    HG_TRY(hg_mpm_snapshot(h, saved.data(), &nb));
+    HG_HIP(h->ctx, hipMemcpy(saved_err.data(), h->err, h->nscenes * 4, hipMemcpyDeviceToHost));
+  }
   auto pass = [&]() {
-    if (fused) {
-      launch_fused(h, true);
-      launch_grid(h, kStoreVelAcc);
-    } else {
-      launch_p2g(h, kStoreVelAcc);
-    }
+    if (fused) launch_step(h);
+    else launch_p2g(h, kStoreVelAcc);
   };
   for (int s = 0; s < 4 && st == HG_OK; ++s) {
     st = hg_mpm_set_tile(h, shapes[s][0], shapes[s][1]);
@@ -2403,6 +2468,10 @@ extern "C" hg_status hg_mpm_autotune(hg_mpm *h, double dt, int32_t reps, int32_t
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   h->prof.on = was_on;
+  if (fused) {
+    HG_TRY(hg_mpm_restore(h, saved.data(), saved.size()));
+    HG_HIP(h->ctx, hipMemcpy(h->err, saved_err.data(), h->nscenes * 4, hipMemcpyHostToDevice));
+  }
   HG_TRY(st);
   HG_TRY(hg_mpm_set_tile(h, shapes[bi][0], shapes[bi][1]));
   if (aw_out) *aw_out = shapes[bi][0];

@@ -343,11 +343,20 @@ class OracleHybrid:
         P = C.POINTER(OracleHybridC)
         L.oracle_hybrid_step.argtypes = [P, C.c_double, C.c_uint32]
         L.oracle_hybrid_step.restype = C.c_int
+        L.oracle_hybrid_step_omp.argtypes = [P, C.c_double, C.c_uint32, C.c_int]
+        L.oracle_hybrid_step_omp.restype = C.c_int
         self.L = L
 
     def step(self, dt: float, nsteps: int = 1) -> None:
         if self.L.oracle_hybrid_step(C.byref(self.c), float(dt), int(nsteps)) != 0:
             raise MemoryError("oracle_hybrid_step")
+
+    def step_omp(self, dt: float, nsteps: int, nthreads: int) -> None:
+        """The same step with the MPM part on nthreads OpenMP threads (the
+        reference's per-thread grid buffers); no homogenisation."""
+        if self.L.oracle_hybrid_step_omp(C.byref(self.c), float(dt), int(nsteps),
+                                         int(nthreads)) != 0:
+            raise MemoryError("oracle_hybrid_step_omp")
 
     def homogenized(self) -> dict:
         """The accumulated homogenised stress (G x 4, column-major 2x2) and

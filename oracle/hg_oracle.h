@@ -119,6 +119,12 @@ int oracle_mpm_is_consistent(const oracle_mpm *s);
  * the multi-core CPU baseline; results differ from the serial oracle by
  * summation order only. */
 void oracle_mpm_flow_n_omp(oracle_mpm *s, double dt, uint32_t nsteps, int nthreads);
+/* The OpenMP step's phases (CPU baseline only): scratch holds T * 5 * G
+ * doubles; oracle_omp_threads(0) is OpenMP's default thread count. */
+int oracle_omp_threads(int nthreads);
+void oracle_mpm_omp_scatter(oracle_mpm *s, int T, double *scratch);
+void oracle_mpm_omp_grid_update(oracle_mpm *s, double dt, int T);
+void oracle_mpm_omp_point_update(oracle_mpm *s, double dt, int T);
 
 /* ---- DEM: rigidbody2d circles with PenaltyImpactFrictionMap -------------- */
 
@@ -209,6 +215,9 @@ int oracle_hybrid_homogenize_velocity(oracle_hybrid *h);
  * with homog_stress / homog_velocity set, each step then homogenises as
  * updateHybridStates_poorman does every step (HybridGrains2DSim.cpp:545-556). */
 int oracle_hybrid_step(oracle_hybrid *h, double dt, uint32_t nsteps);
+/* The same step with the MPM part in the reference's OpenMP structure on
+ * nthreads threads (CPU baseline only; the grain work stays serial). */
+int oracle_hybrid_step_omp(oracle_hybrid *h, double dt, uint32_t nsteps, int nthreads);
 
 #ifdef __cplusplus
 }

@@ -264,3 +264,33 @@ int oracle_hybrid_step(oracle_hybrid *h, double dt, uint32_t nsteps) {
   }
   return 0;
 }
+
+int oracle_hybrid_step_omp(oracle_hybrid *h, double dt, uint32_t nsteps, int nthreads) {
+  oracle_mpm *s = h->mpm;
+  oracle_dem *d = h->dem;
+  const int T = oracle_omp_threads(nthreads);
+  const uint64_t G = nodes(s);
+  double *scratch = (double *)malloc((size_t)T * 5 * G * sizeof(double));
+  if (!scratch) return -1;
+  int rc = 0;
+  for (uint32_t k = 0; k < nsteps && rc == 0; ++k) {
+    memcpy(h->dem_pos_before, d->q, 3 * (size_t)d->nbodies * sizeof(double));
+    oracle_mpm_omp_scatter(s, T, scratch);
+    oracle_hybrid_rasterize_grain_mass(h);
+    if (oracle_dem_flow(d, dt, 1)) {
+      rc = -1;
+      break;
+    }
+    oracle_mpm_omp_grid_update(s, dt, T);
+    oracle_hybrid_rasterize_grain_mom(h);
+    oracle_hybrid_update_velocities_on_grid(h);
+    if (h->homog_factor)
+      for (uint64_t n = 0; n < G; ++n)
+        h->homog_factor[n] = h->dem_mass_grid[n] > 0.0 ? 0.5 : (s->mass[n] > 0.0 ? 1.0 : 0.0);
+    oracle_hybrid_transfer_to_grain(h);
+    oracle_mpm_omp_point_update(s, dt, T);
+    oracle_hybrid_update_positions(h, dt);
+  }
+  free(scratch);
+  return rc;
+}

@@ -578,105 +578,146 @@ int oracle_mpm_is_consistent(const oracle_mpm *s) {
 /* ---- OpenMP CPU baseline -------------------------------------------------- */
 /* The reference's OpenMP structure: per-thread full-grid scatter buffers that
  * are zeroed, filled and reduced every transfer (PhysicsGrid.cpp:214-265), and
- * parallel-for loops over points and nodes elsewhere. */
-void oracle_mpm_flow_n_omp(oracle_mpm *s, double dt, uint32_t nsteps, int nthreads) {
+ * parallel-for loops over points and nodes elsewhere.  A step is three
+ * phases, which the hybrid baseline interleaves with the grain work. */
+
+/* Cleared grid, stress, and the mass, momentum and internal-force scatters
+ * through T per-thread buffers (scratch: T * 5 * G doubles) and their
+ * reduction.  The stress comes first so that the three scatters share one
+ * pass over the buffers; it depends only on b and J. */
+void oracle_mpm_omp_scatter(oracle_mpm *s, int T, double *scratch) {
 #ifdef _OPENMP
   const uint32_t nx = nx_nodes(s);
   const uint64_t G = num_nodes(s);
-  const int T = nthreads > 0 ? nthreads : omp_get_max_threads();
-  double *scratch = (double *)malloc((size_t)T * 5 * G * sizeof(double));
-  for (uint32_t step = 0; step < nsteps; ++step) {
-    oracle_mpm_clear_rasterized_data(s);
+  oracle_mpm_clear_rasterized_data(s);
 #pragma omp parallel for num_threads(T)
-    for (int t = 0; t < T; ++t) memset(scratch + (size_t)t * 5 * G, 0, 5 * G * sizeof(double));
-    /* stress first so the mass, momentum and force scatters share one pass
-     * over the per-thread buffers; the stress depends only on b and J. */
+  for (int t = 0; t < T; ++t) memset(scratch + (size_t)t * 5 * G, 0, 5 * G * sizeof(double));
 #pragma omp parallel for num_threads(T)
-    for (uint32_t p = 0; p < s->npoints; ++p) {
-      double tau[4] = {0.0, 0.0, 0.0, 0.0};
-      if (s->J[p] <= 1.0) {
-        double be[4];
-        for (int k = 0; k < 4; ++k) be[k] = s->be_bar[4 * p + k] * s->J[p];
-        oracle_compute_tau(be, s->bulk_modulus, s->shear_modulus, 0, tau);
-      }
-      for (int k = 0; k < 4; ++k) s->sigma[4 * p + k] = tau[k] / s->J[p];
+  for (uint32_t p = 0; p < s->npoints; ++p) {
+    double tau[4] = {0.0, 0.0, 0.0, 0.0};
+    if (s->J[p] <= 1.0) {
+      double be[4];
+      for (int k = 0; k < 4; ++k) be[k] = s->be_bar[4 * p + k] * s->J[p];
+      oracle_compute_tau(be, s->bulk_modulus, s->shear_modulus, 0, tau);
     }
+    for (int k = 0; k < 4; ++k) s->sigma[4 * p + k] = tau[k] / s->J[p];
+  }
 #pragma omp parallel num_threads(T)
-    {
-      double *buf = scratch + (size_t)omp_get_thread_num() * 5 * G;
+  {
+    double *buf = scratch + (size_t)omp_get_thread_num() * 5 * G;
 #pragma omp for
-      for (uint32_t p = 0; p < s->npoints; ++p) {
-        const double *xp = &s->q[2 * p];
-        const double *sg = &s->sigma[4 * p];
-        int64_t lo[2], hi[2];
-        oracle_stencil(s, xp, s->hl[p], lo, hi);
-        const double pm0 = s->m[p] * s->v[2 * p], pm1 = s->m[p] * s->v[2 * p + 1];
-        const double c = -s->volume[p] * s->J[p];
-        for (int64_t iy = lo[1]; iy <= hi[1]; ++iy)
-          for (int64_t ix = lo[0]; ix <= hi[0]; ++ix) {
-            const double w = oracle_weight(s, xp, s->hl[p], ix, iy);
-            const int64_t n = iy * nx + ix;
-            buf[n] += w * s->m[p];
-            buf[G + 2 * n] += w * pm0;
-            buf[G + 2 * n + 1] += w * pm1;
-            if (w == 0.0) continue;
-            double g[2];
-            oracle_weight_grad(s, xp, s->hl[p], ix, iy, g);
-            buf[3 * G + 2 * n] += c * (sg[0] * g[0] + sg[2] * g[1]);
-            buf[3 * G + 2 * n + 1] += c * (sg[1] * g[0] + sg[3] * g[1]);
-          }
-      }
-    }
-#pragma omp parallel for num_threads(T)
-    for (uint64_t i = 0; i < G; ++i) {
-      for (int t = 0; t < T; ++t) {
-        const double *buf = scratch + (size_t)t * 5 * G;
-        s->mass[i] += buf[i];
-        s->mom[2 * i] += buf[G + 2 * i];
-        s->mom[2 * i + 1] += buf[G + 2 * i + 1];
-        s->force[2 * i] += buf[3 * G + 2 * i];
-        s->force[2 * i + 1] += buf[3 * G + 2 * i + 1];
-      }
-      s->force[2 * i] += s->mass[i] * s->gravity[0];
-      s->force[2 * i + 1] += s->mass[i] * s->gravity[1];
-      if (s->mass[i] > 0.0) {
-        s->mom_new[2 * i] = s->mom[2 * i] + dt * s->force[2 * i];
-        s->mom_new[2 * i + 1] = s->mom[2 * i + 1] + dt * s->force[2 * i + 1];
-      }
-    }
-    oracle_mpm_grid_plane_collisions(s);
-#pragma omp parallel for num_threads(T)
-    for (uint64_t i = 0; i < G; ++i) {
-      if (s->mass[i] > 0.0) {
-        s->vel[2 * i] = s->mom_new[2 * i] / s->mass[i];
-        s->vel[2 * i + 1] = s->mom_new[2 * i + 1] / s->mass[i];
-        const double dm = dt * s->mass[i];
-        s->acc[2 * i] = (s->mom_new[2 * i] - s->mom[2 * i]) / dm;
-        s->acc[2 * i + 1] = (s->mom_new[2 * i + 1] - s->mom[2 * i + 1]) / dm;
-      }
-    }
-    /* the per-point passes of flow_second_phase, each a parallel loop in the
-     * reference (MaterialPoints.cpp:299, :328, :352, :208, :250, :373) */
-#pragma omp parallel for num_threads(T)
     for (uint32_t p = 0; p < s->npoints; ++p) {
-      oracle_mpm one = *s;
-      one.npoints = 1;
-      one.q += 2 * p; one.v += 2 * p; one.m += p; one.volume += p; one.hl += p;
-      one.be_bar += 4 * p; one.J += p; one.sigma += 4 * p; one.vel_grad += 4 * p;
-      one.just_homogenized = NULL;
-      one.kinematically_scripted =
-          s->kinematically_scripted ? s->kinematically_scripted + p : NULL;
-      oracle_mpm_compute_vel_grad(&one);
-      oracle_mpm_elastic_prediction(&one, dt);
-      oracle_mpm_plastic_correction(&one);
-      oracle_mpm_update_velocities(&one, dt);
-      oracle_mpm_update_positions(&one, dt);
-      oracle_mpm_point_plane_collisions(&one);
+      const double *xp = &s->q[2 * p];
+      const double *sg = &s->sigma[4 * p];
+      int64_t lo[2], hi[2];
+      oracle_stencil(s, xp, s->hl[p], lo, hi);
+      const double pm0 = s->m[p] * s->v[2 * p], pm1 = s->m[p] * s->v[2 * p + 1];
+      const double c = -s->volume[p] * s->J[p];
+      for (int64_t iy = lo[1]; iy <= hi[1]; ++iy)
+        for (int64_t ix = lo[0]; ix <= hi[0]; ++ix) {
+          const double w = oracle_weight(s, xp, s->hl[p], ix, iy);
+          const int64_t n = iy * nx + ix;
+          buf[n] += w * s->m[p];
+          buf[G + 2 * n] += w * pm0;
+          buf[G + 2 * n + 1] += w * pm1;
+          if (w == 0.0) continue;
+          double g[2];
+          oracle_weight_grad(s, xp, s->hl[p], ix, iy, g);
+          buf[3 * G + 2 * n] += c * (sg[0] * g[0] + sg[2] * g[1]);
+          buf[3 * G + 2 * n + 1] += c * (sg[1] * g[0] + sg[3] * g[1]);
+        }
     }
   }
-  free(scratch);
+#pragma omp parallel for num_threads(T)
+  for (uint64_t i = 0; i < G; ++i) {
+    for (int t = 0; t < T; ++t) {
+      const double *buf = scratch + (size_t)t * 5 * G;
+      s->mass[i] += buf[i];
+      s->mom[2 * i] += buf[G + 2 * i];
+      s->mom[2 * i + 1] += buf[G + 2 * i + 1];
+      s->force[2 * i] += buf[3 * G + 2 * i];
+      s->force[2 * i + 1] += buf[3 * G + 2 * i + 1];
+    }
+  }
+#else
+  (void)T;
+  (void)scratch;
+  oracle_mpm_flow_zero_phase(s);
+  oracle_mpm_compute_cauchy_stress(s);
+  oracle_mpm_compute_forces(s);
+#endif
+}
+
+/* Gravity (halved at hybridised nodes), the coupling force, the momentum
+ * update and the grid plane conditions (PhysicsGrid.cpp:544-673). */
+void oracle_mpm_omp_grid_update(oracle_mpm *s, double dt, int T) {
+  const uint64_t G = num_nodes(s);
+#pragma omp parallel for num_threads(T)
+  for (uint64_t i = 0; i < G; ++i) {
+    const double gs = (s->hybridized && s->hybridized[i] > 0) ? 0.5 : 1.0;
+    s->force[2 * i] += gs * s->mass[i] * s->gravity[0];
+    s->force[2 * i + 1] += gs * s->mass[i] * s->gravity[1];
+    if (s->coupling_force) {
+      s->force[2 * i] += s->coupling_force[2 * i];
+      s->force[2 * i + 1] += s->coupling_force[2 * i + 1];
+    }
+    if (s->mass[i] > 0.0) {
+      s->mom_new[2 * i] = s->mom[2 * i] + dt * s->force[2 * i];
+      s->mom_new[2 * i + 1] = s->mom[2 * i + 1] + dt * s->force[2 * i + 1];
+    }
+  }
+  oracle_mpm_grid_plane_collisions(s);
+}
+
+/* Nodal velocity and acceleration, then the per-point passes of
+ * flow_second_phase, each a parallel loop in the reference
+ * (MaterialPoints.cpp:299, :328, :352, :208, :250, :373). */
+void oracle_mpm_omp_point_update(oracle_mpm *s, double dt, int T) {
+  const uint64_t G = num_nodes(s);
+#pragma omp parallel for num_threads(T)
+  for (uint64_t i = 0; i < G; ++i) {
+    if (s->mass[i] > 0.0) {
+      s->vel[2 * i] = s->mom_new[2 * i] / s->mass[i];
+      s->vel[2 * i + 1] = s->mom_new[2 * i + 1] / s->mass[i];
+      const double dm = dt * s->mass[i];
+      s->acc[2 * i] = (s->mom_new[2 * i] - s->mom[2 * i]) / dm;
+      s->acc[2 * i + 1] = (s->mom_new[2 * i + 1] - s->mom[2 * i + 1]) / dm;
+    }
+  }
+#pragma omp parallel for num_threads(T)
+  for (uint32_t p = 0; p < s->npoints; ++p) {
+    oracle_mpm one = *s;
+    one.npoints = 1;
+    one.q += 2 * p; one.v += 2 * p; one.m += p; one.volume += p; one.hl += p;
+    one.be_bar += 4 * p; one.J += p; one.sigma += 4 * p; one.vel_grad += 4 * p;
+    one.just_homogenized = NULL;
+    one.kinematically_scripted =
+        s->kinematically_scripted ? s->kinematically_scripted + p : NULL;
+    oracle_mpm_compute_vel_grad(&one);
+    oracle_mpm_elastic_prediction(&one, dt);
+    oracle_mpm_plastic_correction(&one);
+    oracle_mpm_update_velocities(&one, dt);
+    oracle_mpm_update_positions(&one, dt);
+    oracle_mpm_point_plane_collisions(&one);
+  }
+}
+
+int oracle_omp_threads(int nthreads) {
+#ifdef _OPENMP
+  return nthreads > 0 ? nthreads : omp_get_max_threads();
 #else
   (void)nthreads;
-  oracle_mpm_flow_n(s, dt, nsteps);
+  return 1;
 #endif
+}
+
+void oracle_mpm_flow_n_omp(oracle_mpm *s, double dt, uint32_t nsteps, int nthreads) {
+  const int T = oracle_omp_threads(nthreads);
+  double *scratch = (double *)malloc((size_t)T * 5 * num_nodes(s) * sizeof(double));
+  for (uint32_t step = 0; step < nsteps; ++step) {
+    oracle_mpm_omp_scatter(s, T, scratch);
+    oracle_mpm_omp_grid_update(s, dt, T);
+    oracle_mpm_omp_point_update(s, dt, T);
+  }
+  free(scratch);
 }

@@ -200,10 +200,12 @@ def test_hold_occupies_the_stream(hg_ctx):
         g.hold(-1.0)
 
 
-def test_autotune_picks_a_shape_and_changes_no_result(hg_ctx):
+@pytest.mark.parametrize("prec", [HG_FP64, HG_FP32])
+def test_autotune_picks_a_shape_and_changes_no_result(hg_ctx, prec):
+    # FP32 times whole fused steps and restores the saved state afterwards
     s = _small_scene(seed=9)
-    a = GpuMpm(s, HG_FP64, ctx=hg_ctx)
-    b = GpuMpm(s, HG_FP64, ctx=hg_ctx)
+    a = GpuMpm(s, prec, ctx=hg_ctx)
+    b = GpuMpm(s, prec, ctx=hg_ctx)
     a.flow(DT, 3)
     b.flow(DT, 3)
     tune = b.autotune(DT, reps=2)
