@@ -52,10 +52,10 @@ const char *hgx_hybrid_scene_style(const hgx_hybrid_scene *s);
 void hgx_hybrid_scene_free(hgx_hybrid_scene *s);
 
 /* ---- zone resampling on the host (hg_resample.hpp) ----------------------------
- * The pieces of ResamplingTools / ZoneTools / CAvoidAVoid (hybridgrains2dnew/
- * ResamplingTools.cpp, ZoneTools.cpp, AvoidAvoid.cpp, random.c,
- * CircleRectangleIntersection.cpp) as plain calls, and one resampling pass on
- * a hybrid scene without any device work. */
+ * The pieces of the zone resampling (what hybridgrains2dnew/ResamplingTools.cpp,
+ * ZoneTools.cpp, AvoidAvoid.cpp, random.c and CircleRectangleIntersection.cpp
+ * compute) as plain calls, and one resampling pass on a hybrid scene without
+ * any device work. */
 
 /* random.c: n tempered words of a stream seeded with seed (randomMT_i), and
  * n single-precision reals of another such stream (randomMT). */

@@ -7,11 +7,11 @@
 // 301-559); HybridGrains2DSim (hg_sim.hpp) runs it between device steps on
 // downloaded state and rebuilds the device handles from the result.
 //
-// Names follow the reference: ZoneTools (ZoneTools.cpp), ResamplingTools
-// (ResamplingTools.cpp), CAvoidAVoid (AvoidAvoid.cpp), CUniformGridH2D
-// (uniformgrid.cpp), DistGrid and redistanceGrid (distgrid.cpp,
-// levelsetutil.cpp), circleRectangleIntersectionArea
-// (CircleRectangleIntersection.cpp) and the MT19937 stream of random.c.
+// The zone and resampling classes keep the reference's names, ZoneTools
+// (ZoneTools.cpp) and ResamplingTools (ResamplingTools.cpp), with DistGrid and
+// redistanceGrid (distgrid.cpp, levelsetutil.cpp) and the MT19937 stream of
+// random.c; the geometry and search structures behind them are this
+// library's own.
 #pragma once
 
 #include <cstdint>
@@ -47,31 +47,11 @@ class MersenneTwister1998 {
   int mti_ = 625;
 };
 
-// ---- CircleRectangleIntersection.cpp ------------------------------------------
-// Area of [x0, x1] x [y0, y1] inside the circle of radius r at (cx, cy).
-double circleRectangleIntersectionArea(double x0, double x1, double y0, double y1, double cx,
-                                       double cy, double r);
-
-// ---- uniformgrid.cpp -------------------------------------------------------------
-// Buckets of ids over a uniform grid; coordinates outside are clamped to the
-// border cells.
-class CUniformGridH2D {
- public:
-  CUniformGridH2D(const double min_coords[2], const std::uint32_t res[2], double cell_width);
-  void registerData(const double min_c[2], const double max_c[2], int id);
-  void registerPointData(const double x[2], int id);
-  void getGridID(const double x[2], std::uint32_t idx[2]) const;
-  void getGridIDRange(const double min_c[2], const double max_c[2], std::uint32_t lo[2],
-                      std::uint32_t hi[2]) const;
-  // ids of a cell (empty outside the grid)
-  const std::vector<int> &getIDs(std::uint32_t i, std::uint32_t j) const;
-
- private:
-  std::uint32_t res_[2];
-  double min_[2], h_;
-  std::vector<std::vector<int>> ids_;
-  std::vector<int> empty_;
-};
+// ---- geometry ---------------------------------------------------------------------
+// Area of [x0, x1] x [y0, y1] inside the circle of radius r at (cx, cy) (what
+// the reference's CircleRectangleIntersection.cpp computes).
+double boxCircleOverlapArea(double x0, double x1, double y0, double y1, double cx, double cy,
+                            double r);
 
 // ---- distgrid.cpp, levelsetutil.cpp -------------------------------------------------
 struct DistGrid {
@@ -168,34 +148,6 @@ class ZoneTools {
   std::vector<CellIndex> rzone_idx_, direct_idx_;
 };
 
-// ---- AvoidAvoid.cpp ----------------------------------------------------------------
-// Fills zones with new disks that overlap no existing disk by more than the
-// reduce rate allows: 16 uniform trials per zone, in zone order.
-class CAvoidAVoid {
- public:
-  void setTargetRegions(const std::vector<Zone> &regions) { regions_ = regions; }
-  void setCheckConflictAgainstRegionBoundary(bool b) { check_boundary_ = b; }
-  void setOldBodies(const std::vector<double> &pos, const std::vector<double> &radii,
-                    double mean_radius);
-  void avoidAVoid(const std::function<double()> &sampler, MersenneTwister1998 &rng);
-  const std::vector<double> &positions() const { return pos_; }
-  const std::vector<double> &radii() const { return r_; }
-  int numBodies() const { return int(r_.size()); }
-
- private:
-  bool conflictCheck(const double p[2], double radius);
-  void randomFillRegion(const Zone &z, int numTrial, const std::function<double()> &sampler,
-                        MersenneTwister1998 &rng);
-  std::vector<double> pos_, r_;
-  double mean_radius_ = 1.0, reduce_rate_ = 0.97;
-  std::vector<Zone> regions_;
-  bool check_boundary_ = false;
-  double cur_min_[2] = {0, 0}, cur_max_[2] = {0, 0};
-  std::unique_ptr<CUniformGridH2D> grid_;
-  std::vector<int> mailbox_;
-  int test_id_ = 100;
-};
-
 // ---- ResamplingTools.cpp and the state edits it makes ------------------------------
 // What the resampling needs beyond the step's state: the material points'
 // reference positions and unweighted mass and volume, the grains' reference
@@ -250,7 +202,6 @@ class ResamplingTools {
   std::default_random_engine generator_;
   std::normal_distribution<double> dist_;
   double r_mean_ = -1, r_std_ = 0;
-  CAvoidAVoid aav_;
 };
 
 // ResamplingTools::homogenizeStress (ResamplingTools.cpp:744-801): each

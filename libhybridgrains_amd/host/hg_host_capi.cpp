@@ -157,7 +157,7 @@ hg_status hgx_mt1998(uint32_t seed, uint32_t n, uint32_t *words, float *reals) {
 
 double hgx_circle_rect_area(double x0, double x1, double y0, double y1, double cx, double cy,
                             double r) {
-  return hg::circleRectangleIntersectionArea(x0, x1, y0, y1, cx, cy, r);
+  return hg::boxCircleOverlapArea(x0, x1, y0, y1, cx, cy, r);
 }
 
 hg_status hgx_zone_tools(const double gmin[2], double h, const uint32_t cells[2], double ls_h,

@@ -5,6 +5,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <functional>
+#include <memory>
 #include <queue>
 #include <sstream>
 #include <stdexcept>
@@ -53,76 +55,108 @@ float MersenneTwister1998::randomMT() {
   return float(randomMT_i()) * float(2.3283064365386963e-10);
 }
 
-// ---- CircleRectangleIntersection.cpp ------------------------------------------------
-// The area of a box and a circle at the origin from the antiderivative of the
-// circle's height above a line y = h: G(x) = (x sqrt(r^2 - x^2) + r^2
-// asin(x / r)) / 2 - h x, integrated between the box's x range clipped to the
-// chord; a box below the x axis is reflected, a box across it split.
+// ---- area of a box inside a circle ------------------------------------------------
+// With the circle at the origin, the box's area inside it is the integral over
+// u of the length of [y0, y1] inside [-c(u), c(u)], c(u) = sqrt(r^2 - u^2).
+// Between the abscissae where c(u) meets |y0| or |y1| that length is one of
+// 2c, c - y0, c + y1 or y1 - y0 (or zero), so the integral is a sum of closed
+// forms over at most five pieces, with C(u) = (u c(u) + r^2 asin(u / r)) / 2
+// the antiderivative of c.
 namespace {
-double chord_half(double h, double r) { return h < r ? std::sqrt(r * r - h * h) : 0.0; }
-double antiderivative(double x, double h, double r) {
-  return 0.5 * (std::sqrt(1.0 - x * x / (r * r)) * x * r + r * r * std::asin(x / r) - 2.0 * h * x);
-}
-// box [x0, x1] x [h, +inf)
-double area_above(double x0, double x1, double h, double r) {
-  if (x0 > x1) std::swap(x0, x1);
-  const double s = chord_half(h, r);
-  const double a = std::max(-s, std::min(s, x1)), b = std::max(-s, std::min(s, x0));
-  return antiderivative(a, h, r) - antiderivative(b, h, r);
-}
-double area_box(double x0, double x1, double y0, double y1, double r) {
-  if (y0 > y1) std::swap(y0, y1);
-  if (y0 < 0.0) {
-    if (y1 < 0.0) return area_box(x0, x1, -y0, -y1, r);
-    return area_box(x0, x1, 0.0, -y0, r) + area_box(x0, x1, 0.0, y1, r);
-  }
-  return area_above(x0, x1, y0, r) - area_above(x0, x1, y1, r);
+double semichord_integral(double u, double r) {
+  const double t = std::max(-1.0, std::min(1.0, u / r));
+  return 0.5 * (u * std::sqrt(std::max(0.0, r * r - u * u)) + r * r * std::asin(t));
 }
 }  // namespace
 
-double circleRectangleIntersectionArea(double x0, double x1, double y0, double y1, double cx,
-                                       double cy, double r) {
-  return area_box(x0 - cx, x1 - cx, y0 - cy, y1 - cy, r);
-}
-
-// ---- uniformgrid.cpp ------------------------------------------------------------------
-CUniformGridH2D::CUniformGridH2D(const double min_coords[2], const std::uint32_t res[2],
-                                 double cell_width)
-    : res_{res[0], res[1]}, min_{min_coords[0], min_coords[1]}, h_(cell_width) {
-  ids_.resize(std::size_t(res_[0]) * std::size_t(res_[1]));
-}
-
-void CUniformGridH2D::getGridID(const double x[2], std::uint32_t idx[2]) const {
-  for (int d = 0; d < 2; ++d) {
-    const double u = (x[d] - min_[d]) / h_;
-    const int c = int(std::floor(u));
-    idx[d] = std::uint32_t(std::max(0, std::min(int(res_[d]) - 1, c)));
+double boxCircleOverlapArea(double x0, double x1, double y0, double y1, double cx, double cy,
+                            double r) {
+  double ax = x0 - cx, bx = x1 - cx, ay = y0 - cy, by = y1 - cy;
+  if (ax > bx) std::swap(ax, bx);
+  if (ay > by) std::swap(ay, by);
+  const double lo = std::max(ax, -r), hi = std::min(bx, r);
+  if (!(lo < hi) || !(ay < r) || !(by > -r)) return 0.0;
+  double cut[6] = {lo, hi};
+  int n = 2;
+  for (double y : {ay, by})
+    if (std::fabs(y) < r) {
+      const double u = std::sqrt(r * r - y * y);
+      if (lo < -u && -u < hi) cut[n++] = -u;
+      if (lo < u && u < hi) cut[n++] = u;
+    }
+  std::sort(cut, cut + n);
+  double area = 0.0;
+  for (int k = 0; k + 1 < n; ++k) {
+    const double a = cut[k], b = cut[k + 1];
+    if (!(a < b)) continue;
+    const double mid = 0.5 * (a + b), c = std::sqrt(std::max(0.0, r * r - mid * mid));
+    // the piece's length is kc c(u) + k1; which bound is the circle's
+    // depends only on the piece
+    const bool circle_top = c <= by, circle_bottom = -c >= ay;
+    const double kc = circle_top ? (circle_bottom ? 2.0 : 1.0) : (circle_bottom ? 1.0 : 0.0);
+    const double k1 = circle_top ? (circle_bottom ? 0.0 : -ay) : (circle_bottom ? by : by - ay);
+    if (kc * c + k1 <= 0.0) continue;  // the box misses the circle over this piece
+    area += kc * (semichord_integral(b, r) - semichord_integral(a, r)) + k1 * (b - a);
   }
+  return area;
 }
 
-void CUniformGridH2D::getGridIDRange(const double min_c[2], const double max_c[2],
-                                     std::uint32_t lo[2], std::uint32_t hi[2]) const {
-  getGridID(min_c, lo);
-  getGridID(max_c, hi);
-}
+// ---- items binned by cell (compressed rows) -------------------------------------------
+// Items (points, or boxes over several cells) counted into the cells of a
+// uniform grid and laid out cell by cell, each cell's items in the order they
+// were given; positions outside the grid fall into its border cells.
+namespace {
+class CellBins {
+ public:
+  CellBins(const double mn[2], const std::uint32_t n[2], double h) : h_(h) {
+    for (int d = 0; d < 2; ++d) {
+      min_[d] = mn[d];
+      n_[d] = n[d];
+    }
+  }
+  std::uint32_t cellOf(double x, int d) const {
+    const double c = std::floor((x - min_[d]) / h_);
+    return std::uint32_t(std::max(0.0, std::min(double(n_[d]) - 1.0, c)));
+  }
+  // items given as (lo, hi) cell ranges, one per item
+  template <typename Range>
+  void build(std::size_t count, Range range) {
+    start_.assign(std::size_t(n_[0]) * n_[1] + 1, 0);
+    for (int pass = 0; pass < 2; ++pass) {
+      std::vector<std::uint32_t> fill;
+      if (pass == 1) {
+        for (std::size_t c = 1; c < start_.size(); ++c) start_[c] += start_[c - 1];
+        items_.resize(start_.back());
+        fill.assign(start_.begin(), start_.end() - 1);
+      }
+      for (std::size_t k = 0; k < count; ++k) {
+        std::uint32_t lo[2], hi[2];
+        range(k, lo, hi);
+        for (std::uint32_t j = lo[1]; j <= hi[1]; ++j)
+          for (std::uint32_t i = lo[0]; i <= hi[0]; ++i) {
+            const std::size_t c = std::size_t(j) * n_[0] + i;
+            if (pass == 0) ++start_[c + 1];
+            else items_[fill[c]++] = int(k);
+          }
+      }
+    }
+  }
+  // the items of cell (i, j)
+  const int *begin(std::uint32_t i, std::uint32_t j) const {
+    return items_.data() + start_[std::size_t(j) * n_[0] + i];
+  }
+  const int *end(std::uint32_t i, std::uint32_t j) const {
+    return items_.data() + start_[std::size_t(j) * n_[0] + i + 1];
+  }
+  std::uint32_t size(int d) const { return n_[d]; }
 
-void CUniformGridH2D::registerData(const double min_c[2], const double max_c[2], int id) {
-  std::uint32_t lo[2], hi[2];
-  getGridIDRange(min_c, max_c, lo, hi);
-  for (std::uint32_t j = lo[1]; j <= hi[1]; ++j)
-    for (std::uint32_t i = lo[0]; i <= hi[0]; ++i) ids_[std::size_t(j) * res_[0] + i].push_back(id);
-}
-
-void CUniformGridH2D::registerPointData(const double x[2], int id) {
-  std::uint32_t c[2];
-  getGridID(x, c);
-  ids_[std::size_t(c[1]) * res_[0] + c[0]].push_back(id);
-}
-
-const std::vector<int> &CUniformGridH2D::getIDs(std::uint32_t i, std::uint32_t j) const {
-  if (i >= res_[0] || j >= res_[1]) return empty_;
-  return ids_[std::size_t(j) * res_[0] + i];
-}
+ private:
+  double min_[2], h_;
+  std::uint32_t n_[2];
+  std::vector<std::uint32_t> start_;
+  std::vector<int> items_;
+};
+}  // namespace
 
 // ---- distgrid.cpp, levelsetutil.cpp -----------------------------------------------------
 void DistGrid::resize(const double mn[2], const std::uint32_t n[2], double cell_h) {
@@ -226,68 +260,78 @@ void ZoneTools::init(const PhysicsGrid &grid, double ls_h, bool allow_direct) {
   prev_zone_mpm_.assign(nc, ZT_DISCRETE);
 }
 
-// :149-183 and :185-270
+// :149-183 and :185-270.  Each level-set cell's packing fraction is the mean,
+// over its spc x spc sample points, of the fraction of the window box around
+// the point that the circles cover.  The windows lie on a regular lattice, so
+// each circle adds its overlap to the windows whose boxes reach its bounding
+// box, found by index arithmetic, and the cover of every window is then
+// divided by the window's area.
 void ZoneTools::updatePackingFraction(const std::vector<double> &cpos,
                                       const std::vector<double> &cr,
                                       const std::vector<double> &points, double window,
                                       int spc) {
-  std::fill(phi_.begin(), phi_.end(), 0.0);
   const double *rmin = dist_.min_coords;
-  const std::uint32_t *res = dist_.N;
-  const double h = dist_.h;
-  CUniformGridH2D grid(rmin, res, h);
-  const std::size_t nc = cr.size();
-  for (std::size_t c = 0; c < nc; ++c) {
-    const double mn[2] = {cpos[2 * c] - cr[c], cpos[2 * c + 1] - cr[c]};
-    const double mx[2] = {cpos[2 * c] + cr[c], cpos[2 * c + 1] + cr[c]};
-    grid.registerData(mn, mx, int(c));
-  }
-  std::vector<int> mailbox(nc, 0);
-  int test_id = 1;
-  auto box_fraction = [&](const double bmin[2], const double bmax[2]) {
-    double tot = 0.0;
-    const double region = (bmax[0] - bmin[0]) * (bmax[1] - bmin[1]);
-    std::uint32_t lo[2], hi[2];
-    grid.getGridIDRange(bmin, bmax, lo, hi);
-    for (std::uint32_t j = lo[1]; j <= hi[1]; ++j)
-      for (std::uint32_t i = lo[0]; i <= hi[0]; ++i)
-        for (int id : grid.getIDs(i, j)) {
-          if (mailbox[std::size_t(id)] == test_id) continue;
-          mailbox[std::size_t(id)] = test_id;
-          tot += circleRectangleIntersectionArea(bmin[0], bmax[0], bmin[1], bmax[1],
-                                                 cpos[2 * std::size_t(id)],
-                                                 cpos[2 * std::size_t(id) + 1], cr[std::size_t(id)]);
-        }
-    ++test_id;
-    return tot / region;
+  const std::uint32_t nx = dist_.N[0], ny = dist_.N[1];
+  const double h = dist_.h, half = 0.5 * window;
+  const std::uint32_t sx = nx * std::uint32_t(spc), sy = ny * std::uint32_t(spc);
+  // the sample point of lattice index k along one axis (k = cell * spc + sub)
+  auto sample = [&](int d, std::uint32_t k) {
+    const std::uint32_t cell = k / std::uint32_t(spc), sub = k % std::uint32_t(spc);
+    return rmin[d] + (double(cell) + (double(sub) + 0.5) / double(spc)) * h;
   };
-  for (std::uint32_t j = 0; j < res[1]; ++j)
-    for (std::uint32_t i = 0; i < res[0]; ++i) {
+  std::vector<double> cover(std::size_t(sx) * sy, 0.0);
+  const double pitch = h / double(spc);
+  auto lattice_range = [&](int d, double a, double b, std::uint32_t n, std::uint32_t &lo,
+                           std::uint32_t &hi) {
+    // samples whose window [s - half, s + half] reaches [a, b], with a margin
+    // of one sample (those add nothing)
+    const double kl = std::floor((a - half - rmin[d]) / pitch - 0.5) - 1.0;
+    const double kh = std::ceil((b + half - rmin[d]) / pitch - 0.5) + 1.0;
+    if (kh < 0.0 || kl > double(n) - 1.0 || n == 0) return false;
+    lo = std::uint32_t(std::max(0.0, kl));
+    hi = std::uint32_t(std::min(double(n) - 1.0, kh));
+    return lo <= hi;
+  };
+  for (std::size_t c = 0; c < cr.size(); ++c) {
+    const double x = cpos[2 * c], y = cpos[2 * c + 1], r = cr[c];
+    std::uint32_t il, ih, jl, jh;
+    if (!lattice_range(0, x - r, x + r, sx, il, ih) || !lattice_range(1, y - r, y + r, sy, jl, jh))
+      continue;
+    for (std::uint32_t j = jl; j <= jh; ++j) {
+      const double py = sample(1, j);
+      for (std::uint32_t i = il; i <= ih; ++i) {
+        const double px = sample(0, i);
+        cover[std::size_t(j) * sx + i] +=
+            boxCircleOverlapArea(px - half, px + half, py - half, py + half, x, y, r);
+      }
+    }
+  }
+  for (std::uint32_t j = 0; j < ny; ++j)
+    for (std::uint32_t i = 0; i < nx; ++i) {
       double pf = 0.0;
       for (int v = 0; v < spc; ++v)
         for (int u = 0; u < spc; ++u) {
-          const double p[2] = {rmin[0] + (double(i) + (double(u) + 0.5) / double(spc)) * h,
-                               rmin[1] + (double(j) + (double(v) + 0.5) / double(spc)) * h};
-          const double bmin[2] = {p[0] - window * 0.5, p[1] - window * 0.5};
-          const double bmax[2] = {p[0] + window * 0.5, p[1] + window * 0.5};
-          pf += box_fraction(bmin, bmax);
+          const std::uint32_t ki = i * std::uint32_t(spc) + std::uint32_t(u);
+          const std::uint32_t kj = j * std::uint32_t(spc) + std::uint32_t(v);
+          const double px = sample(0, ki), py = sample(1, kj);
+          const double area = ((px + half) - (px - half)) * ((py + half) - (py - half));
+          pf += cover[std::size_t(kj) * sx + ki] / area;
         }
-      phi_[std::size_t(j) * res[0] + i] = pf / double(spc * spc);
+      phi_[std::size_t(j) * nx + i] = pf / double(spc * spc);
     }
-  // material points mark every level-set cell their window reaches
-  // (cells on the grid's first row and column excepted, as the reference)
+  // A material point sets the fraction to 1 in every level-set cell its
+  // window reaches, except cells of the grid's first row and column.
   const std::size_t np = points.size() / 2;
   for (std::size_t p = 0; p < np; ++p) {
     int lo[2], hi[2];
+    const std::uint32_t n[2] = {nx, ny};
     for (int d = 0; d < 2; ++d) {
-      const double x = points[2 * p + std::size_t(d)] - rmin[d];
-      lo[d] = int(std::floor((x - window * 0.5) / h));
-      hi[d] = int(std::floor((x + window * 0.5) / h));
+      const double rel = points[2 * p + std::size_t(d)] - rmin[d];
+      lo[d] = std::max(1, int(std::floor((rel - half) / h)));
+      hi[d] = std::min(int(n[d]) - 1, int(std::floor((rel + half) / h)));
     }
     for (int j = lo[1]; j <= hi[1]; ++j)
-      for (int i = lo[0]; i <= hi[0]; ++i)
-        if (i > 0 && j > 0 && i < int(res[0]) && j < int(res[1]))
-          phi_[std::size_t(j) * res[0] + std::size_t(i)] = 1.0;
+      for (int i = lo[0]; i <= hi[0]; ++i) phi_[std::size_t(j) * nx + std::size_t(i)] = 1.0;
   }
 }
 
@@ -511,90 +555,103 @@ void ZoneTools::identifyInnerContinuumRegionExcludingDirectTransitionZones(
     }
 }
 
-// ---- AvoidAvoid.cpp ---------------------------------------------------------------------
-void CAvoidAVoid::setOldBodies(const std::vector<double> &pos, const std::vector<double> &radii,
-                               double mean_radius) {
-  pos_ = pos;
-  r_ = radii;
-  mean_radius_ = mean_radius;
-}
-
-// :102-151, with randomFillRegion (:153-179) and conflictCheck (:262-303)
-void CAvoidAVoid::avoidAVoid(const std::function<double()> &sampler, MersenneTwister1998 &rng) {
-  if (regions_.empty()) return;
-  double mn[2] = {regions_[0].v[0], regions_[0].v[1]}, mx[2] = {regions_[0].v[2], regions_[0].v[3]};
-  for (const Zone &z : regions_) {
-    mn[0] = std::min(mn[0], z.v[0]);
-    mn[1] = std::min(mn[1], z.v[1]);
-    mx[0] = std::max(mx[0], z.v[2]);
-    mx[1] = std::max(mx[1], z.v[3]);
+// ---- filling zones with disks (AvoidAvoid.cpp:102-303) -----------------------------
+// The rule the reference's avoid-a-void sampler follows, which fixes the
+// results: zone by zone in order, 16 trials each; a trial draws the position
+// (two randomMT() reals scaling the zone's extent) and then the radius, and
+// keeps the disk unless it comes closer to an existing disk (old or kept
+// before it) than 0.97 of their radii's sum.  The search for such a disk is
+// this file's own: disks are binned by centre in singly linked lists over a
+// grid of cells two mean radii wide, and a trial visits the cells within its
+// radius plus the largest radius so far.
+namespace {
+class DiskFiller {
+ public:
+  DiskFiller(const std::vector<double> &pos, const std::vector<double> &radii,
+             const std::vector<Zone> &zones, double mean_radius)
+      : pos_(pos), r_(radii) {
+    double lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
+    auto grow = [&](double x, double y) {
+      lo[0] = std::min(lo[0], x);
+      lo[1] = std::min(lo[1], y);
+      hi[0] = std::max(hi[0], x);
+      hi[1] = std::max(hi[1], y);
+    };
+    for (const Zone &z : zones) {
+      grow(z.v[0], z.v[1]);
+      grow(z.v[2], z.v[3]);
+    }
+    for (std::size_t k = 0; k < r_.size(); ++k) {
+      grow(pos_[2 * k], pos_[2 * k + 1]);
+      rmax_ = std::max(rmax_, r_[k]);
+    }
+    w_ = 2.0 * mean_radius;
+    if (!(w_ > 0.0) || !std::isfinite(w_)) w_ = 1.0;
+    for (int d = 0; d < 2; ++d) {
+      org_[d] = lo[d] - w_;
+      // at most about 2^24 cells: a wider spread of disks gets wider cells
+      n_[d] = std::uint32_t(std::min(4096.0, std::floor((hi[d] - lo[d]) / w_) + 3.0));
+    }
+    w_ = std::max({w_, (hi[0] - lo[0] + 2.0 * w_) / double(n_[0] - 1),
+                   (hi[1] - lo[1] + 2.0 * w_) / double(n_[1] - 1)});
+    head_.assign(std::size_t(n_[0]) * n_[1], -1);
+    next_.reserve(r_.size() + 16 * zones.size());
+    for (std::size_t k = 0; k < r_.size(); ++k) link(int(k));
   }
-  const double gmin[2] = {mn[0] - 6.0 * mean_radius_, mn[1] - 6.0 * mean_radius_};
-  const double gmax[2] = {mx[0] + 6.0 * mean_radius_, mx[1] + 6.0 * mean_radius_};
-  const double cw = 4.0 * mean_radius_;
-  const std::uint32_t res[2] = {std::uint32_t(std::ceil((gmax[0] - gmin[0]) / cw)),
-                                std::uint32_t(std::ceil((gmax[1] - gmin[1]) / cw))};
-  grid_.reset(new CUniformGridH2D(gmin, res, cw));
-  mailbox_.assign(r_.size(), 0);
-  for (std::size_t b = 0; b < r_.size(); ++b) {
-    const double a[2] = {pos_[2 * b] - r_[b], pos_[2 * b + 1] - r_[b]};
-    const double c[2] = {pos_[2 * b] + r_[b], pos_[2 * b + 1] + r_[b]};
-    grid_->registerData(a, c, int(b));
-  }
-  for (const Zone &z : regions_) randomFillRegion(z, 16, sampler, rng);
-}
 
-void CAvoidAVoid::randomFillRegion(const Zone &z, int numTrial,
-                                   const std::function<double()> &sampler,
-                                   MersenneTwister1998 &rng) {
-  cur_min_[0] = z.v[0];
-  cur_min_[1] = z.v[1];
-  cur_max_[0] = z.v[2];
-  cur_max_[1] = z.v[3];
-  const double diff[2] = {z.v[2] - z.v[0], z.v[3] - z.v[1]};
-  for (int t = 0; t < numTrial; ++t) {
-    const double ux = double(rng.randomMT());
-    const double uy = double(rng.randomMT());
-    const double p[2] = {ux * diff[0] + z.v[0], uy * diff[1] + z.v[1]};
-    const double radius = sampler();
-    if (conflictCheck(p, radius)) continue;
-    const int id = int(r_.size());
-    pos_.push_back(p[0]);
-    pos_.push_back(p[1]);
-    r_.push_back(radius);
-    const double a[2] = {p[0] - radius, p[1] - radius}, c[2] = {p[0] + radius, p[1] + radius};
-    grid_->registerData(a, c, id);
-    mailbox_.push_back(0);
-  }
-}
-
-bool CAvoidAVoid::conflictCheck(const double p[2], double radius) {
-  const int tid = test_id_;
-  auto next_id = [&]() {
-    ++test_id_;
-    if (test_id_ == 0) test_id_ = 1;
-  };
-  const double a[2] = {p[0] - radius, p[1] - radius}, c[2] = {p[0] + radius, p[1] + radius};
-  std::uint32_t lo[2], hi[2];
-  grid_->getGridIDRange(a, c, lo, hi);
-  for (std::uint32_t v = lo[1]; v <= hi[1]; ++v)
-    for (std::uint32_t u = lo[0]; u <= hi[0]; ++u)
-      for (int id : grid_->getIDs(u, v)) {
-        if (mailbox_[std::size_t(id)] == tid) continue;
-        mailbox_[std::size_t(id)] = tid;
-        const double d = std::hypot(pos_[2 * std::size_t(id)] - p[0], pos_[2 * std::size_t(id) + 1] - p[1]);
-        if (d < (r_[std::size_t(id)] + radius) * reduce_rate_) {
-          next_id();
-          return true;
-        }
+  void fill(const std::vector<Zone> &zones, const std::function<double()> &radius,
+            MersenneTwister1998 &rng) {
+    for (const Zone &z : zones) {
+      const double ex = z.v[2] - z.v[0], ey = z.v[3] - z.v[1];
+      for (int trial = 0; trial < 16; ++trial) {
+        const double ux = double(rng.randomMT());
+        const double uy = double(rng.randomMT());
+        const double x = ux * ex + z.v[0], y = uy * ey + z.v[1];
+        const double r = radius();
+        if (blocked(x, y, r)) continue;
+        pos_.push_back(x);
+        pos_.push_back(y);
+        r_.push_back(r);
+        rmax_ = std::max(rmax_, r);
+        link(int(r_.size()) - 1);
       }
-  next_id();
-  if (check_boundary_) {
-    if (p[0] < cur_min_[0] + radius || p[0] > cur_max_[0] - radius) return true;
-    if (p[1] < cur_min_[1] + radius || p[1] > cur_max_[1] - radius) return true;
+    }
   }
-  return false;
-}
+  const std::vector<double> &positions() const { return pos_; }
+  const std::vector<double> &radii() const { return r_; }
+  int count() const { return int(r_.size()); }
+
+ private:
+  int cell(double v, int d) const {
+    const double c = std::floor((v - org_[d]) / w_);
+    return int(std::max(0.0, std::min(double(n_[d]) - 1.0, c)));
+  }
+  void link(int k) {
+    const std::size_t c = std::size_t(cell(pos_[2 * std::size_t(k) + 1], 1)) * n_[0] +
+                          std::size_t(cell(pos_[2 * std::size_t(k)], 0));
+    next_.push_back(head_[c]);
+    head_[c] = k;
+  }
+  bool blocked(double x, double y, double r) const {
+    const double reach = r + rmax_;
+    const int i0 = cell(x - reach, 0), i1 = cell(x + reach, 0);
+    const int j0 = cell(y - reach, 1), j1 = cell(y + reach, 1);
+    for (int j = j0; j <= j1; ++j)
+      for (int i = i0; i <= i1; ++i)
+        for (int k = head_[std::size_t(j) * n_[0] + std::size_t(i)]; k >= 0;
+             k = next_[std::size_t(k)]) {
+          const std::size_t K = std::size_t(k);
+          if (std::hypot(pos_[2 * K] - x, pos_[2 * K + 1] - y) < (r_[K] + r) * 0.97) return true;
+        }
+    return false;
+  }
+
+  std::vector<double> pos_, r_;
+  double rmax_ = 0.0, w_ = 1.0, org_[2] = {0.0, 0.0};
+  std::uint32_t n_[2] = {1, 1};
+  std::vector<int> head_, next_;
+};
+}  // namespace
 
 // ---- bases (uGIMPLinearBasisFunctions.cpp, LinearBasisFunctions.cpp,
 // CubicBasisFunctions.cpp) ------------------------------------------------------------
@@ -756,23 +813,26 @@ double ResamplingTools::randomRadiusSampler() {
 
 namespace {
 
-// a grid over the continuum cells with every point (or grain) registered at
-// its position (ZoneTools::setupUniformGridElementsAsPoints, ZoneTools.cpp:816-839)
-std::unique_ptr<CUniformGridH2D> point_grid(const PhysicsGrid &g, const std::vector<double> &xy,
-                                            std::size_t stride) {
-  std::unique_ptr<CUniformGridH2D> u(new CUniformGridH2D(g.min, g.cell_count, g.cell_width));
-  for (std::size_t k = 0; k < xy.size() / stride; ++k) u->registerPointData(&xy[stride * k], int(k));
-  return u;
+// The points (or grains) binned by continuum cell, as the reference's
+// setupUniformGridElementsAsPoints registers them (ZoneTools.cpp:816-839): the
+// Gaussian-weighted sums below visit the cells of a box row by row and each
+// cell's items in index order, and their rounding follows that order.
+std::unique_ptr<CellBins> point_bins(const PhysicsGrid &g, const std::vector<double> &xy,
+                                     std::size_t stride) {
+  std::unique_ptr<CellBins> b(new CellBins(g.min, g.cell_count, g.cell_width));
+  b->build(xy.size() / stride, [&](std::size_t k, std::uint32_t lo[2], std::uint32_t hi[2]) {
+    for (int d = 0; d < 2; ++d) lo[d] = hi[d] = b->cellOf(xy[stride * k + std::size_t(d)], d);
+  });
+  return b;
 }
 
 template <typename F>
-void for_ids_in_box(const CUniformGridH2D &u, const double x[2], double r, F f) {
-  const double a[2] = {x[0] - r, x[1] - r}, c[2] = {x[0] + r, x[1] + r};
-  std::uint32_t lo[2], hi[2];
-  u.getGridIDRange(a, c, lo, hi);
-  for (std::uint32_t j = lo[1]; j <= hi[1]; ++j)
-    for (std::uint32_t i = lo[0]; i <= hi[0]; ++i)
-      for (int id : u.getIDs(i, j)) f(id);
+void for_ids_in_box(const CellBins &b, const double x[2], double r, F f) {
+  const std::uint32_t i0 = b.cellOf(x[0] - r, 0), i1 = b.cellOf(x[0] + r, 0);
+  const std::uint32_t j0 = b.cellOf(x[1] - r, 1), j1 = b.cellOf(x[1] + r, 1);
+  for (std::uint32_t j = j0; j <= j1; ++j)
+    for (std::uint32_t i = i0; i <= i1; ++i)
+      for (const int *id = b.begin(i, j); id != b.end(i, j); ++id) f(*id);
 }
 
 // Grid: mass-weighted nodal fields of the first n points, normalised by the
@@ -857,12 +917,10 @@ void ResamplingTools::avoidAVoidMPM(HybridResampleState &s, const std::vector<Zo
   MaterialPoints &mp = s.continuum.material_points;
   before = int(mp.npoints());
   std::vector<double> radii(mp.npoints(), r);
-  aav_.setTargetRegions(zones);
-  aav_.setCheckConflictAgainstRegionBoundary(false);
-  aav_.setOldBodies(mp.q, radii, r);
-  aav_.avoidAVoid([r]() { return r; }, mt_);
-  after = aav_.numBodies();
-  const std::vector<double> &pos = aav_.positions();
+  DiskFiller filler(mp.q, radii, zones, r);
+  filler.fill(zones, [r]() { return r; }, mt_);
+  after = filler.count();
+  const std::vector<double> &pos = filler.positions();
   for (int k = before; k < after; ++k)
     add_material_point(s, &pos[2 * std::size_t(k)], s.initial_particle_size * 0.5);
 }
@@ -879,13 +937,11 @@ void ResamplingTools::avoidAVoidDEM(HybridResampleState &s, const std::vector<Zo
   }
   if (r_mean_ < 0.0)
     throw std::runtime_error("demRMean is not set: the grain radius sampler needs it");
-  aav_.setTargetRegions(zones);
-  aav_.setCheckConflictAgainstRegionBoundary(false);
-  aav_.setOldBodies(pos, d.radius, r_mean_);
-  aav_.avoidAVoid([this]() { return randomRadiusSampler(); }, mt_);
-  after = aav_.numBodies();
-  const std::vector<double> &np = aav_.positions();
-  const std::vector<double> &nr = aav_.radii();
+  DiskFiller filler(pos, d.radius, zones, r_mean_);
+  filler.fill(zones, [this]() { return randomRadiusSampler(); }, mt_);
+  after = filler.count();
+  const std::vector<double> &np = filler.positions();
+  const std::vector<double> &nr = filler.radii();
   for (int k = before; k < after; ++k) {
     const double x = np[2 * std::size_t(k)], y = np[2 * std::size_t(k) + 1], r = nr[std::size_t(k)];
     const double m = rho * M_PI * r * r, I = 0.5 * m * r * r;
@@ -952,8 +1008,8 @@ ResampleReport ResamplingTools::resample(ZoneTools &zt, HybridResampleState &s,
   rep.dem_after_sampling = int(ds.nbodies());
   rep.mpm_after_sampling = int(mp.npoints());
 
-  std::unique_ptr<CUniformGridH2D> ug_c = point_grid(g, mp.q, 2);
-  std::unique_ptr<CUniformGridH2D> ug_d = point_grid(g, ds.q, 3);
+  std::unique_ptr<CellBins> ug_c = point_bins(g, mp.q, 2);
+  std::unique_ptr<CellBins> ug_d = point_bins(g, ds.q, 3);
   const double h = g.cell_width, cell_volume = h * h;
   const std::size_t nn = g.numGridPoints();
   const std::uint32_t W = g.cell_count[0] + 1;
@@ -973,11 +1029,10 @@ ResampleReport ResamplingTools::resample(ZoneTools &zt, HybridResampleState &s,
   auto hybrid_quantities = [&](const std::vector<CellIndex> &zidx, int before, int after) {
     const double mass_per_cell = s.material_density * cell_volume;
     for (const CellIndex &c : zidx) {
-      const std::vector<int> &ids = ug_c->getIDs(c.i, c.j);
-      const double n = double(ids.size());
-      for (int id : ids) {
-        s.mpm_m0[std::size_t(id)] = mass_per_cell / n;
-        s.mpm_volume0[std::size_t(id)] = cell_volume / n;
+      const double n = double(ug_c->end(c.i, c.j) - ug_c->begin(c.i, c.j));
+      for (const int *id = ug_c->begin(c.i, c.j); id != ug_c->end(c.i, c.j); ++id) {
+        s.mpm_m0[std::size_t(*id)] = mass_per_cell / n;
+        s.mpm_volume0[std::size_t(*id)] = cell_volume / n;
       }
     }
     const double sigma = 0.5 * s.initial_particle_size;
