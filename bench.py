@@ -20,6 +20,10 @@ Beside it:
                   plain FP64 path); batched_fp32_over_plain_fp64 is the ratio
                   of the first to the last, on the same scenes;
   dem_column_20k  the 20k-circle DEM column, FP64 and FP32;
+  hybrid_with_resampling
+                  config 2 with its zone resampling every 100 steps, through
+                  the C++ driver (HybridGrains2DSim): step time, pass time
+                  and the amortised rate;
   served          Poisson streams of full-size scene requests (10k-2M
                   particles, 500-5000 steps, 50/30/20 MPM/DEM/hybrid) through
                   the scene server at three arrival rates: p50/p99 completion
@@ -233,6 +237,48 @@ def _served(args):
         reqs = scenes.served_requests(args.served_requests, seed=1 + k, rate=rate, **size)
         out["streams"].append(_served_one(reqs, rate, args.served_timeout))
     return out
+
+
+def _resampling(steps=200, every=100):
+    """Config 2 with its zone resampling, through the reference-named C++
+    driver (hybridgrains2d_cli, HybridGrains2DSim): the hybrid column written
+    as the reference's scene files (the continuum and a grain packing over the
+    whole 0.5 m column; the resampler keeps the grains of the zones within
+    2 cm of the free sides and seeds the continuum), FP32, `steps` steps with
+    a resampling pass every `every` steps, and the same steps with none.
+    The difference per pass is the pass's cost: device state to the host,
+    the host zone tools, new device handles."""
+    import re
+
+    from libhybridgrains_amd import scenes
+
+    exe = ROOT / "libhybridgrains_amd" / "bin" / "hybridgrains2d_cli"
+    d = Path(tempfile.mkdtemp(prefix="hg_resample_"))
+    runs = {}
+    try:
+        for name, freq in (("with", every), ("without", 10 ** 9)):
+            log(f"resampling: {name} passes, {steps} steps")
+            xml = scenes.write_hybrid_column_xml(d / name, level=0.02, resample_every=freq)
+            r = subprocess.run([str(exe), "-n", str(steps), "-f", str(steps), "-p", "fp32", str(xml)],
+                               cwd=str(d / name), capture_output=True, text=True, timeout=600)
+            if r.returncode != 0:
+                return {"error": (r.stderr or r.stdout)[-500:]}
+            st = re.findall(r"grains (\d+) points (\d+) resamplings (\d+)", r.stdout)
+            done = re.search(r"Simulation complete: (\d+) steps in ([0-9.]+) s", r.stdout)
+            g, p, n = (int(v) for v in st[-1])
+            runs[name] = {"seconds": float(done.group(2)), "grains": g, "points": p, "passes": n}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    w, wo = runs["with"], runs["without"]
+    npass = w["passes"] - wo["passes"]
+    pass_s = (w["seconds"] - wo["seconds"]) / npass if npass > 0 else None
+    step_s = wo["seconds"] / steps
+    amort = step_s + (pass_s or 0.0) / every
+    return {"steps": steps, "resample_every": every, "runs": runs,
+            "pass_s": pass_s, "step_ms": 1e3 * step_s, "amortised_ms_per_step": 1e3 * amort,
+            "amortised_particle_steps_per_s": (w["grains"] + w["points"]) / amort,
+            "note": "steps and passes through hybridgrains2d_cli, FP32; one status download "
+                    "at the end of each run"}
 
 
 def _copy_gbs(_lib, nbytes=1 << 30, iters=20):
@@ -489,6 +535,8 @@ def main() -> int:
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 runs")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-resampling", action="store_true",
+                    help="skip the config-2 run with zone resampling")
     ap.add_argument("--served-requests", type=int, default=16, help="requests per rate")
     ap.add_argument("--served-rates", default="0.5,4,20")
     ap.add_argument("--served-small", action="store_true")
@@ -553,6 +601,7 @@ def main() -> int:
         batched = _time_batched(args)
     log("dem")
     dem = _time_dem(args)
+    resampling = None if args.no_resampling else _resampling()
     served = None
     if not args.no_served:
         served = _served(args)
@@ -607,6 +656,7 @@ def main() -> int:
                                            batched["batched_fp64"]["value"]) if batched else None,
         "served": served,
         "dem_column_20k": dem,
+        "hybrid_with_resampling": resampling,
         "hbm_copy_gbs_measured": copy_gbs,
         "kernels_fp32_rocprof": kernels,
         "kernels_ms_fp32": f32["kernels_ms"],

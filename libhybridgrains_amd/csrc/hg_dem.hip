@@ -24,11 +24,11 @@
 //    sorted by partner key, double-buffered so that the next step's
 //    contacts join their spring history by partner; per-plane slots likewise.
 //
-// Kernels of one step: dem_contacts (thread per body in cell order: narrow
-// phase over the 3x3 cells, history join, penalty force with the Coulomb
-// projection of the spring, plane contacts), dem_integrate (thread per body:
-// gathers its forces in the reference's summation order -- gravity, planes,
-// pairs owned by smaller partners, own pairs, kinematic pairs -- then
+// Kernels of one step: dem_contacts (a team of lanes per body in cell order:
+// narrow phase over the 3x3 cells, history join, penalty force with the
+// Coulomb projection of the spring, plane contacts), dem_integrate (thread per
+// body: gathers its forces in the reference's summation order -- gravity,
+// planes, pairs owned by smaller partners, own pairs, kinematic pairs -- then
 // symplectic Euler, then the new cell key), and the counting sort.
 #include "hg_dem_handle.h"
 #include "hg_sort.h"
@@ -139,80 +139,89 @@ struct ContactArgs {
   double *cpoint;  // optional: contact point per owned slot (circle pairs)
 };
 
-template <typename T, int CAP>
-__global__ __launch_bounds__(kThreads) void dem_contacts_kernel(ContactArgs a) {
-  __shared__ uint32_t s_own[kThreads][CAP + 1];
-  __shared__ uint32_t s_in[kThreads][CAP + 1];
-  const uint32_t blk = hg_xcd_remap(blockIdx.x, a.nblocks_padded);
-  const uint32_t s = blk * kThreads + threadIdx.x;
-  if (s >= a.nbodies) return;
-  const uint32_t i = a.perm[s];
-  const uint32_t sid = scene_of(a.b_off, a.nscenes, i);
-  const SceneDev &sc = a.scenes[sid];
-  const double xi = a.b.x[i], yi = a.b.y[i], ri = a.b.r[i];
-  const bool fi = a.b.fixed[i] != 0;
-  uint32_t *own = s_own[threadIdx.x];
-  uint32_t *inl = s_in[threadIdx.x];
-  int nown = 0, nin = 0;
-  bool overflow = false;
-  // A body whose state is no longer finite (flagged by the integration that
-  // produced it) looks for no partners: every such body sorts into one
-  // bucket, and scanning it would cost the square of their number.
-  const bool finite_i = isfinite(xi) && isfinite(yi);
+// A team of kTeam lanes per body: the lanes split the body's candidates and
+// then its contacts, so that a body's chain of dependent loads is a fraction
+// of what one lane would walk.  Contacts are collected unordered in LDS and
+// each takes the slot of its rank among the body's keys, which is the order
+// of the partner keys, as a sorted list would have it.
+constexpr int kTeam = 8;
+constexpr int kTeamThreads = 256;
+constexpr int kTeamBodies = kTeamThreads / kTeam;
 
-  int cx, cy;
-  dem_cell_of(sc, xi, yi, cx, cy);
-  const uint32_t *cs = a.cell_start + sc.cell_off;
-  const uint32_t wx = uint32_t(sc.ncx), bx0 = uint32_t(cx - 1) & (wx - 1u);
-  // the buckets of the 3x3 cells: per row one run of three consecutive
-  // buckets, or two runs where the row wraps (the second is empty otherwise)
-  const uint32_t len0 = min(bx0 + 3u, wx) - bx0;
-  for (int oy = -1; oy <= (finite_i ? 1 : -2); ++oy) {
-    const uint32_t row = (uint32_t(cy + oy) & uint32_t(sc.ncy - 1)) * wx;
-    for (int part = 0; part < 2; ++part) {
+template <typename T, int CAP>
+__global__ __launch_bounds__(kTeamThreads) void dem_contacts_kernel(ContactArgs a) {
+  __shared__ uint32_t s_own[kTeamBodies][CAP];
+  __shared__ uint32_t s_in[kTeamBodies][CAP];
+  __shared__ int s_nown[kTeamBodies], s_nin[kTeamBodies];
+  const int tb = int(threadIdx.x) / kTeam, lane = int(threadIdx.x) % kTeam;
+  const uint32_t blk = hg_xcd_remap(blockIdx.x, a.nblocks_padded);
+  const uint32_t s = blk * uint32_t(kTeamBodies) + uint32_t(tb);
+  const bool active = s < a.nbodies;
+  if (lane == 0) {
+    s_nown[tb] = 0;
+    s_nin[tb] = 0;
+  }
+  __syncthreads();
+  const uint32_t i = active ? a.perm[s] : 0u;
+  const uint32_t sid = active ? scene_of(a.b_off, a.nscenes, i) : 0u;
+  const SceneDev &sc = a.scenes[sid];
+  const double xi = active ? a.b.x[i] : 0.0, yi = active ? a.b.y[i] : 0.0;
+  const double ri = active ? a.b.r[i] : 0.0;
+  const bool fi = active ? a.b.fixed[i] != 0 : true;
+  if (active) {
+    // A body whose state is no longer finite (flagged by the integration that
+    // produced it) looks for no partners: every such body sorts into one
+    // bucket, and scanning it would cost the square of their number.
+    const bool finite_i = isfinite(xi) && isfinite(yi);
+    int cx, cy;
+    dem_cell_of(sc, xi, yi, cx, cy);
+    const uint32_t *cs = a.cell_start + sc.cell_off;
+    const uint32_t wx = uint32_t(sc.ncx), bx0 = uint32_t(cx - 1) & (wx - 1u);
+    // the buckets of the 3x3 cells: per row one run of three consecutive
+    // buckets, or two runs where the row wraps (the second is empty otherwise)
+    const uint32_t len0 = min(bx0 + 3u, wx) - bx0;
+    uint32_t rb[6], rn[6], tot = 0;
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+      const int oy = r / 2 - 1, part = r % 2;
+      const uint32_t row = (uint32_t(cy + oy) & uint32_t(sc.ncy - 1)) * wx;
       const uint32_t k0 = part ? row : row + bx0;
       const uint32_t k1 = part ? row + (3u - len0) : row + bx0 + len0;
-      for (uint32_t t = cs[k0]; t < cs[k1]; ++t) {
-        const uint32_t j = a.perm[t];
-        if (j == i) continue;
-        const bool fj = a.b.fixed[j] != 0;
-        if (fi && fj) continue;
-        const double dx = xi - a.b.x[j], dy = yi - a.b.y[j];
-        // a partner that left the finite range is no candidate
-        if (!isfinite(dx) || !isfinite(dy)) continue;
-        const double rs = ri + a.b.r[j];
-        const double c0 = (dx * dx + dy * dy) - rs * rs;
-        if (!(c0 <= 0.0)) continue;
-        if (!fi && (fj || i < j)) {
-          const uint32_t key = (fj ? kKinematic : 0u) | (j - sc.b_off);
-          if (nown < CAP) {
-            // insertion into the sorted list
-            int q = nown++;
-            while (q > 0 && own[q - 1] > key) {
-              own[q] = own[q - 1];
-              --q;
-            }
-            own[q] = key;
-          } else {
-            overflow = true;
-          }
-        } else if (!fi && !fj) {
-          const uint32_t key = j - sc.b_off;
-          if (nin < CAP) {
-            int q = nin++;
-            while (q > 0 && inl[q - 1] > key) {
-              inl[q] = inl[q - 1];
-              --q;
-            }
-            inl[q] = key;
-          } else {
-            overflow = true;
-          }
-        }
+      rb[r] = finite_i ? cs[k0] : 0u;
+      rn[r] = finite_i ? cs[k1] - rb[r] : 0u;
+      tot += rn[r];
+    }
+    for (uint32_t t = uint32_t(lane); t < tot; t += uint32_t(kTeam)) {
+      uint32_t off = t;
+      int r = 0;
+      while (off >= rn[r]) {
+        off -= rn[r];
+        ++r;
+      }
+      const uint32_t j = a.perm[rb[r] + off];
+      if (j == i) continue;
+      const bool fj = a.b.fixed[j] != 0;
+      if (fi && fj) continue;
+      const double dx = xi - a.b.x[j], dy = yi - a.b.y[j];
+      // a partner that left the finite range is no candidate
+      if (!isfinite(dx) || !isfinite(dy)) continue;
+      const double rs = ri + a.b.r[j];
+      const double c0 = (dx * dx + dy * dy) - rs * rs;
+      if (!(c0 <= 0.0)) continue;
+      if (!fi && (fj || i < j)) {
+        const int q = atomicAdd(&s_nown[tb], 1);
+        if (q < CAP) s_own[tb][q] = (fj ? kKinematic : 0u) | (j - sc.b_off);
+      } else if (!fi && !fj) {
+        const int q = atomicAdd(&s_nin[tb], 1);
+        if (q < CAP) s_in[tb][q] = j - sc.b_off;
       }
     }
   }
-  if (overflow) atomicOr(&a.err[sid], 4);
+  __syncthreads();
+  if (!active) return;
+  const int cnt_own = s_nown[tb], cnt_in = s_nin[tb];
+  if (lane == 0 && (cnt_own > CAP || cnt_in > CAP)) atomicOr(&a.err[sid], 4);
+  const int nown = min(cnt_own, CAP), nin = min(cnt_in, CAP);
 
   const double dt = sc.dt;
   const double vxi = double(ld<T>(a.b.vx, i)), vyi = double(ld<T>(a.b.vy, i));
@@ -220,8 +229,10 @@ __global__ __launch_bounds__(kThreads) void dem_contacts_kernel(ContactArgs a) {
   const double wi = sc.weighted ? double(ld<T>(a.b.w, i)) : 1.0;
   const uint32_t obase = i * uint32_t(CAP);
   const int old_n = a.old_s.ocnt[i];
-  for (int k = 0; k < nown; ++k) {
-    const uint32_t key = own[k];
+  for (int k = lane; k < nown; k += kTeam) {
+    const uint32_t key = s_own[tb][k];
+    int rank = 0;
+    for (int q = 0; q < nown; ++q) rank += (s_own[tb][q] < key) ? 1 : 0;
     const bool kin = (key & kKinematic) != 0;
     const uint32_t j = (key & ~kKinematic) + sc.b_off;
     const double xj = a.b.x[j], yj = a.b.y[j], rj = a.b.r[j];
@@ -238,6 +249,7 @@ __global__ __launch_bounds__(kThreads) void dem_contacts_kernel(ContactArgs a) {
     const double depth = gap < 0.0 ? gap : 0.0;
     const double vxj = double(ld<T>(a.b.vx, j)), vyj = double(ld<T>(a.b.vy, j));
     const double omj = double(ld<T>(a.b.om, j));
+    const uint32_t sl = obase + uint32_t(rank);
     double vr0, vr1, r00, r01, r10 = 0.0, r11 = 0.0, w;
     if (!kin) {
       // CircleCircleConstraint (CircleCircleConstraint.cpp:171-193, :224-233)
@@ -251,8 +263,8 @@ __global__ __launch_bounds__(kThreads) void dem_contacts_kernel(ContactArgs a) {
       vr1 = vyi + omi * r00 - vyj - omj * r10;
       w = sc.weighted ? (wi + double(ld<T>(a.b.w, j))) * 0.5 : 1.0;
       if (a.cpoint) {
-        a.cpoint[2 * (obase + uint32_t(k))] = p0;
-        a.cpoint[2 * (obase + uint32_t(k)) + 1] = p1;
+        a.cpoint[2 * sl] = p0;
+        a.cpoint[2 * sl + 1] = p1;
       }
     } else {
       // KinematicCircleCircleConstraint (KinematicObjectCircleConstraint.cpp:121-199)
@@ -278,7 +290,6 @@ __global__ __launch_bounds__(kThreads) void dem_contacts_kernel(ContactArgs a) {
     double ds0, ds1, F0, F1;
     spring(had, o0, o1, dt, vr0, vr1, n0, n1, ds0, ds1);
     penalty_force(sc, w, depth, n0, n1, vr0, vr1, ds0, ds1, F0, F1);
-    const uint32_t sl = obase + uint32_t(k);
     a.new_s.okey[sl] = key;
     st<T>(a.new_s.ods, 2 * sl, T(ds0));
     st<T>(a.new_s.ods, 2 * sl + 1, T(ds1));
@@ -287,12 +298,19 @@ __global__ __launch_bounds__(kThreads) void dem_contacts_kernel(ContactArgs a) {
     st<T>(a.f.otau0, sl, T(r00 * F1 - r01 * F0));
     st<T>(a.f.otau1, sl, T(r10 * F1 - r11 * F0));
   }
-  a.new_s.ocnt[i] = uint8_t(nown);
-  for (int k = 0; k < nin; ++k) a.f.ikey[i * uint32_t(CAP) + uint32_t(k)] = inl[k];
-  a.f.icnt[i] = uint8_t(nin);
+  for (int k = lane; k < nin; k += kTeam) {
+    const uint32_t key = s_in[tb][k];
+    int rank = 0;
+    for (int q = 0; q < nin; ++q) rank += (s_in[tb][q] < key) ? 1 : 0;
+    a.f.ikey[i * uint32_t(CAP) + uint32_t(rank)] = key;
+  }
+  if (lane == 0) {
+    a.new_s.ocnt[i] = uint8_t(nown);
+    a.f.icnt[i] = uint8_t(nin);
+  }
 
   // static planes (RigidBody2DSim.cpp:777-842; StaticPlaneCircleConstraint.cpp)
-  for (int k = 0; k < sc.nplanes; ++k) {
+  for (int k = lane; k < sc.nplanes; k += kTeam) {
     const uint32_t ps = i * kMaxPlanes + uint32_t(k);
     const double pn0 = sc.plane_n[2 * k], pn1 = sc.plane_n[2 * k + 1];
     const double rel0 = xi - sc.plane_x[2 * k], rel1 = yi - sc.plane_x[2 * k + 1];
@@ -332,18 +350,25 @@ struct IntegrateArgs {
   int32_t *err;
   double *F_out;  // optional: total force per body (3B)
   int32_t rekey;  // produce the next step's cell keys
-  uint32_t cap;   // contact slots per body
 };
 
 // Gathers a body's forces in the reference's summation order, then
 // v1 = v0 + dt M^-1 F, q1 = q0 + dt v1 (PenaltyImpactFrictionMap.cpp:1107-1114).
-template <typename T>
+// One thread per body.  The body's own state is loaded first, so that its
+// loads overlap the force gather, and the search of a partner's sorted slots
+// for this body loads all of the partner's keys at once.
+template <typename T, int CAP>
 __global__ __launch_bounds__(kThreads) void dem_integrate_kernel(IntegrateArgs a) {
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= a.nbodies) return;
   const uint32_t sid = scene_of(a.b_off, a.nscenes, i);
   const SceneDev &sc = a.scenes[sid];
   const double m = double(ld<T>(a.b.m, i));
+  const double im = double(ld<T>(a.b.im, i)), iI = double(ld<T>(a.b.iI, i));
+  const double vx0 = double(ld<T>(a.b.vx, i)), vy0 = double(ld<T>(a.b.vy, i));
+  const double om0 = double(ld<T>(a.b.om, i));
+  const double x0 = a.b.x[i], y0 = a.b.y[i], th0 = a.b.th[i];
+  const bool fixed = a.b.fixed[i] != 0;
   double F0 = 0.0, F1 = 0.0, F2 = 0.0;
   F0 += m * sc.g0;
   F1 += m * sc.g1;
@@ -358,46 +383,52 @@ __global__ __launch_bounds__(kThreads) void dem_integrate_kernel(IntegrateArgs a
   const uint32_t lid = i - sc.b_off;
   const int nin = a.f.icnt[i];
   for (int k = 0; k < nin; ++k) {
-    const uint32_t j = a.f.ikey[i * a.cap + uint32_t(k)] + sc.b_off;
+    const uint32_t j = a.f.ikey[i * uint32_t(CAP) + uint32_t(k)] + sc.b_off;
     const int nj = a.s.ocnt[j];
-    for (int q = 0; q < nj; ++q) {
-      const uint32_t sl = j * a.cap + uint32_t(q);
-      if (a.s.okey[sl] == lid) {
-        F0 -= double(ld<T>(a.f.oF, 2 * sl));
-        F1 -= double(ld<T>(a.f.oF, 2 * sl + 1));
-        F2 -= double(ld<T>(a.f.otau1, sl));
-        break;
-      }
+    const uint4 *kv = reinterpret_cast<const uint4 *>(a.s.okey + j * uint32_t(CAP));
+    int hit = -1;
+#pragma unroll
+    for (int q4 = 0; q4 < CAP / 4; ++q4) {
+      const uint4 v = kv[q4];
+      const uint32_t kk[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (hit < 0 && 4 * q4 + e < nj && kk[e] == lid) hit = 4 * q4 + e;
+    }
+    if (hit >= 0) {
+      const uint32_t sl = j * uint32_t(CAP) + uint32_t(hit);
+      F0 -= double(ld<T>(a.f.oF, 2 * sl));
+      F1 -= double(ld<T>(a.f.oF, 2 * sl + 1));
+      F2 -= double(ld<T>(a.f.otau1, sl));
     }
   }
   // own pairs, then kinematic pairs (the key order)
   const int nown = a.s.ocnt[i];
   for (int q = 0; q < nown; ++q) {
-    const uint32_t sl = i * a.cap + uint32_t(q);
+    const uint32_t sl = i * uint32_t(CAP) + uint32_t(q);
     F0 += double(ld<T>(a.f.oF, 2 * sl));
     F1 += double(ld<T>(a.f.oF, 2 * sl + 1));
     F2 += double(ld<T>(a.f.otau0, sl));
   }
-  if (a.b.fixed[i]) F0 = F1 = F2 = 0.0;
+  if (fixed) F0 = F1 = F2 = 0.0;
   if (a.F_out) {
     a.F_out[3 * i] = F0;
     a.F_out[3 * i + 1] = F1;
     a.F_out[3 * i + 2] = F2;
   }
   const double dt = sc.dt;
-  const double im = double(ld<T>(a.b.im, i)), iI = double(ld<T>(a.b.iI, i));
-  const double v0 = double(ld<T>(a.b.vx, i)) + (dt * im) * F0;
-  const double v1 = double(ld<T>(a.b.vy, i)) + (dt * im) * F1;
-  const double v2 = double(ld<T>(a.b.om, i)) + (dt * iI) * F2;
+  const double v0 = vx0 + (dt * im) * F0;
+  const double v1 = vy0 + (dt * im) * F1;
+  const double v2 = om0 + (dt * iI) * F2;
   const T tv0 = T(v0), tv1 = T(v1), tv2 = T(v2);
   st<T>(a.b.vx, i, tv0);
   st<T>(a.b.vy, i, tv1);
   st<T>(a.b.om, i, tv2);
-  const double x = a.b.x[i] + dt * double(tv0);
-  const double y = a.b.y[i] + dt * double(tv1);
+  const double x = x0 + dt * double(tv0);
+  const double y = y0 + dt * double(tv1);
   a.b.x[i] = x;
   a.b.y[i] = y;
-  a.b.th[i] = a.b.th[i] + dt * double(tv2);
+  a.b.th[i] = th0 + dt * double(tv2);
   if (!(isfinite(x) && isfinite(y) && isfinite(v2))) atomicOr(&a.err[sid], 2);
   if (!a.rekey) return;
   const uint32_t k = dem_bucket(sc, x, y);
@@ -458,7 +489,7 @@ static void launch_contacts_t(hg_dem *h) {
   a.b_off = h->d_b_off;
   a.nscenes = h->nscenes;
   a.nbodies = h->nbodies;
-  a.nblocks_padded = hg_pad8((h->nbodies + kThreads - 1) / kThreads);
+  a.nblocks_padded = hg_pad8((h->nbodies + kTeamBodies - 1) / kTeamBodies);
   a.b = h->b;
   a.old_s = h->slots[h->cur];
   a.new_s = h->slots[1 - h->cur];
@@ -468,7 +499,7 @@ static void launch_contacts_t(hg_dem *h) {
   a.err = h->err;
   a.cpoint = h->cpoint;
   ProfScope ps(h->prof, "dem_contacts", h->stream);
-  const dim3 g(a.nblocks_padded), b(kThreads);
+  const dim3 g(a.nblocks_padded), b(kTeamThreads);
   if (h->cap <= 8) hipLaunchKernelGGL((dem_contacts_kernel<T, 8>), g, b, 0, h->stream, a);
   else if (h->cap <= 16) hipLaunchKernelGGL((dem_contacts_kernel<T, 16>), g, b, 0, h->stream, a);
   else hipLaunchKernelGGL((dem_contacts_kernel<T, 32>), g, b, 0, h->stream, a);
@@ -492,10 +523,11 @@ static void launch_integrate_t(hg_dem *h, bool rekey) {
   a.err = h->err;
   a.F_out = h->keep_force ? h->F_out : nullptr;
   a.rekey = rekey ? 1 : 0;
-  a.cap = uint32_t(h->cap);
   ProfScope ps(h->prof, "dem_integrate", h->stream);
-  hipLaunchKernelGGL(dem_integrate_kernel<T>, dim3((h->nbodies + kThreads - 1) / kThreads),
-                     dim3(kThreads), 0, h->stream, a);
+  const dim3 g((h->nbodies + kThreads - 1) / kThreads), b(kThreads);
+  if (h->cap <= 8) hipLaunchKernelGGL((dem_integrate_kernel<T, 8>), g, b, 0, h->stream, a);
+  else if (h->cap <= 16) hipLaunchKernelGGL((dem_integrate_kernel<T, 16>), g, b, 0, h->stream, a);
+  else hipLaunchKernelGGL((dem_integrate_kernel<T, 32>), g, b, 0, h->stream, a);
 }
 
 void hg_dem_launch_step(hg_dem *h, bool rekey) {

@@ -986,14 +986,20 @@ struct FusedArgs {
 
 // A term already scaled to quanta, rounded down to a whole number of them, as
 // a two's-complement 64-bit integer (the caller bounds it below 2^50).  In
-// FP64 (full rate on this GPU): with t = scaled 2^-32, H = floor(t) is the
-// high word and (t - H) 2^32, in [0, 2^32), the low word; every step is exact.
+// FP32 and integer operations, all exact: f = floor(scaled) is a float; its
+// magnitude a splits into hi = floor(a 2^-32) and lo = a - hi 2^32, which is
+// exact because a >= 2^32 makes a and hi 2^32 multiples of 2^9, so lo < 2^32
+// needs at most 23 significant bits; both halves convert exactly to 32-bit
+// integers, and a negative f negates the 64-bit word.
 __device__ __forceinline__ unsigned long long fx_of(float scaled) {
-  const double t = ldexp(double(scaled), -32);
-  const double H = floor(t);
-  const uint32_t lo = uint32_t(ldexp(t - H, 32));
-  const int32_t hi = int32_t(H);
-  return (static_cast<unsigned long long>(static_cast<uint32_t>(hi)) << 32) | lo;
+  const float f = floorf(scaled);
+  const float a = fabsf(f);
+  const float hi = floorf(a * 0x1p-32f);
+  const float lo = __builtin_fmaf(-hi, 0x1p32f, a);
+  const unsigned long long u =
+      (static_cast<unsigned long long>(static_cast<uint32_t>(hi)) << 32) |
+      static_cast<unsigned long long>(static_cast<uint32_t>(lo));
+  return f < 0.0f ? 0ull - u : u;
 }
 
 // A point's rasterisation terms in FP32: per axis the S weights and masked

@@ -361,3 +361,84 @@ def served_requests(n: int, seed: int = 0, rate: float = 8.0, n_min: float = 1e4
                                     r_mean=2.5e-3 * h / 0.005 * 1.0, seed=rs)
             out.append((t, kind, (m, d, w), 1e-5, nsteps))
     return out
+
+
+def write_hybrid_column_xml(folder, *, domain=(2.0, 1.0), h: float = 0.001, column=(0.5, 0.5),
+                            r_mean: float = 0.5e-3, r_spread: float = 0.1, rho: float = 2500.0,
+                            E: float = 1.0e6, nu: float = 0.3, phi_deg: float = 30.0,
+                            kn: float = 1.0e5, restitution: float = 0.3, mu: float = 0.3,
+                            level: float = 0.1, half_thickness_cells: int = 2,
+                            dt_den: int = 200000, resample_every: int = 100, seed: int = 0):
+    """The hybrid column collapse as the reference's three scene files
+    (hybrid.xml, mpm.xml, dem.xml) in folder, for hybridgrains2d_cli: the
+    continuum rectangle over the whole column at 2 x 2 points per cell, and
+    circles of radius r_mean +- r_spread on a hexagonal packing over the whole
+    column, from which the zone resampling keeps those of the discrete and
+    hybrid zones (the zone boundary `level` below the free surface, a hybrid
+    band of half_thickness_cells cells).  The resampling runs every
+    resample_every steps (avoidAVoidFreq = resample_every - 1); dt = 1 /
+    dt_den.  At the defaults: 1,000,000 points and about 180k circles before
+    the first resampling.  Returns the path of hybrid.xml."""
+    from pathlib import Path
+
+    folder = Path(folder)
+    folder.mkdir(parents=True, exist_ok=True)
+    rng = np.random.default_rng(seed)
+    x0 = y0 = 2 * h
+    cw, chh = column
+    mu_s, kappa = lame_from_young(E, nu)
+    alpha = dp_alpha_from_friction_angle(phi_deg)
+    dt = f"1 / {int(dt_den)}"
+    (folder / "mpm.xml").write_text(f"""<mpmgrains2d>
+  <end_time t="1.0"/>
+  <grid min="0.0 0.0" max="{domain[0]!r} {domain[1]!r}" cell_width="{h!r}"/>
+  <integrator dt="{dt}" alpha="0.999" shape_function_order="1" shape_function_category="uGIMP"/>
+  <material shear_modulus="{mu_s!r}" bulk_modulus="{kappa!r}" alpha="{alpha!r}"/>
+  <rectangle min="{x0!r} {y0!r}" max="{x0 + cw!r} {y0 + chh!r}" density="{rho!r}" cell_samples_per_dim="2" vel="0.0 0.0" omega="0.0 0.0"/>
+  <near_earth_gravity f="0.0 -9.81"/>
+  <static_plane x="{x0!r} 0.0" n="1.0 0.0" boundary_behavior="sliding"/>
+  <static_plane x="0.0 {y0!r}" n="0.0 1.0" boundary_behavior="sliding"/>
+</mpmgrains2d>
+""")
+    spacing = 2.0 * r_mean * (1.0 + r_spread) + 1e-5
+    dy = spacing * math.sqrt(3.0) / 2.0
+    xs, ys = [], []
+    j = 0
+    y = y0 + r_mean * (1 + r_spread)
+    while y < y0 + chh - r_mean:
+        x = x0 + r_mean * (1 + r_spread) + (0.5 * spacing if j % 2 else 0.0)
+        row = np.arange(x, x0 + cw - r_mean, spacing)
+        xs.append(row)
+        ys.append(np.full(row.size, y))
+        y += dy
+        j += 1
+    px, py = np.concatenate(xs), np.concatenate(ys)
+    radius = rng.uniform(r_mean * (1 - r_spread), r_mean * (1 + r_spread), px.size)
+    m_pair = 0.5 * rho * math.pi * r_mean * r_mean
+    gn = damping_from_restitution(restitution, kn, m_pair)
+    lines = ["<rigidbody2d_scene>",
+             '  <collision_detection broadphase="spatial_grid" cell_scale="1.0" bandwidth_reduction="none"/>',
+             f'  <penalty_impact_map dt="{dt}" kn="{kn!r}" gamman="{gn!r}" kt="{kn!r}" '
+             f'gammat="{gn!r}" mu="{mu!r}"/>',
+             '  <end_time t="1.0"/>',
+             '  <near_earth_gravity f="0.0 -9.81"/>',
+             f'  <static_plane x="0.0 {y0!r}" n="0 1"/>',
+             f'  <static_plane x="{x0!r} 0.0" n="1 0"/>']
+    lines += [f'  <geometry type="circle" r="{r!r}"/>' for r in radius.tolist()]
+    lines += [f'  <rigid_body x="{a!r} {b!r}" theta="0" v="0 0" omega="0" rho="{rho!r}" '
+              f'geo_idx="{k}" fixed="0"/>' for k, (a, b) in enumerate(zip(px.tolist(), py.tolist()))]
+    lines.append("</rigidbody2d_scene>")
+    (folder / "dem.xml").write_text("\n".join(lines) + "\n")
+    (folder / "hybrid.xml").write_text(f"""<hybridgrains2d_scene>
+  <discrete_sim file_name="dem.xml"/>
+  <continuum_sim file_name="mpm.xml"/>
+  <time_step dt="{dt}"/>
+  <end_time t="1.0"/>
+  <integrator_type type="coupled"/>
+  <integrator_style style="prediction_correction_advection_node_node"/>
+  <post_stabilization_settings stabilization_order="after_advection" stabilization_type="fix_penetration" velocity_blending="0.0" spring_constant="0.0" tolerance="1.0e-5" max_iter="1000"/>
+  <hybrid_kinematic_script type="none"/>
+  <coupling_force k="0.0" hw="1.0" order="1" test_number="6" enrichment_flags="00000" fixed_rzone="0 0 0 0" lmcoupling="1" LMEnableFixedRZone="0" rzoneHalfThickness="{half_thickness_cells * h!r}" demRMean="{r_mean!r}" demRStd="{r_spread * r_mean / math.sqrt(3.0)!r}" avoidAVoidFreq="{int(resample_every) - 1}" phiThreshold="0.3" rzoneLevelSet="{level!r}" phi_window_size="{1.25 * h!r}" phi_samples_per_cell_side="1" level_set_cell_width="{h!r}" rho_dem="{rho!r}" homogenize_stress="0" grid_smoothing_homogenized_stress="0" homogenize_velocity="0" grid_smoothing_homogenized_velocity="0"/>
+</hybridgrains2d_scene>
+""")
+    return folder / "hybrid.xml"
