@@ -42,7 +42,9 @@ HOST_SOURCES = ["hg_xml.cpp", "hg_scene.cpp", "hg_sim.cpp", "hg_resample.cpp", "
 CLIS = ["mpmgrains2d_cli", "rigidbody2d_cli", "hybridgrains2d_cli"]
 CXX = os.environ.get("CXX", "g++")
 ROCM_LIB = "/opt/rocm/lib"
-CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", f"-I{ROOT / 'include'}"]
+# the host tools run the zone resampling's independent per-trial and
+# per-point loops on OpenMP threads
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-fopenmp", f"-I{ROOT / 'include'}"]
 
 
 def _run(cmd: list[str], cwd: Path | None = None) -> None:

@@ -563,7 +563,10 @@ void ZoneTools::identifyInnerContinuumRegionExcludingDirectTransitionZones(
 // before it) than 0.97 of their radii's sum.  The search for such a disk is
 // this file's own: disks are binned by centre in singly linked lists over a
 // grid of cells two mean radii wide, and a trial visits the cells within its
-// radius plus the largest radius so far.
+// radius plus the largest radius of the set searched.  A trial blocked by an
+// old disk is blocked in any order, so all trials are first checked against
+// the old disks in parallel; the few that pass are then checked in draw order
+// against the disks kept before them, which gives the sequential rule's result.
 namespace {
 class DiskFiller {
  public:
@@ -583,7 +586,7 @@ class DiskFiller {
     }
     for (std::size_t k = 0; k < r_.size(); ++k) {
       grow(pos_[2 * k], pos_[2 * k + 1]);
-      rmax_ = std::max(rmax_, r_[k]);
+      rmax_old_ = std::max(rmax_old_, r_[k]);
     }
     w_ = 2.0 * mean_radius;
     if (!(w_ > 0.0) || !std::isfinite(w_)) w_ = 1.0;
@@ -594,27 +597,41 @@ class DiskFiller {
     }
     w_ = std::max({w_, (hi[0] - lo[0] + 2.0 * w_) / double(n_[0] - 1),
                    (hi[1] - lo[1] + 2.0 * w_) / double(n_[1] - 1)});
-    head_.assign(std::size_t(n_[0]) * n_[1], -1);
-    next_.reserve(r_.size() + 16 * zones.size());
-    for (std::size_t k = 0; k < r_.size(); ++k) link(int(k));
+    head_old_.assign(std::size_t(n_[0]) * n_[1], -1);
+    head_new_.assign(std::size_t(n_[0]) * n_[1], -1);
+    next_.assign(r_.size(), -1);
+    for (std::size_t k = 0; k < r_.size(); ++k) link(head_old_, int(k));
   }
 
   void fill(const std::vector<Zone> &zones, const std::function<double()> &radius,
             MersenneTwister1998 &rng) {
-    for (const Zone &z : zones) {
-      const double ex = z.v[2] - z.v[0], ey = z.v[3] - z.v[1];
+    const std::size_t nt = 16 * zones.size();
+    std::vector<double> tx(nt), ty(nt), tr(nt);
+    for (std::size_t z = 0; z < zones.size(); ++z) {
+      const Zone &zn = zones[z];
+      const double ex = zn.v[2] - zn.v[0], ey = zn.v[3] - zn.v[1];
       for (int trial = 0; trial < 16; ++trial) {
+        const std::size_t t = 16 * z + std::size_t(trial);
         const double ux = double(rng.randomMT());
         const double uy = double(rng.randomMT());
-        const double x = ux * ex + z.v[0], y = uy * ey + z.v[1];
-        const double r = radius();
-        if (blocked(x, y, r)) continue;
-        pos_.push_back(x);
-        pos_.push_back(y);
-        r_.push_back(r);
-        rmax_ = std::max(rmax_, r);
-        link(int(r_.size()) - 1);
+        tx[t] = ux * ex + zn.v[0];
+        ty[t] = uy * ey + zn.v[1];
+        tr[t] = radius();
       }
+    }
+    std::vector<std::uint8_t> open(nt);
+#pragma omp parallel for schedule(static, 1024)
+    for (long t = 0; t < long(nt); ++t)
+      open[std::size_t(t)] = blocked(head_old_, rmax_old_, tx[std::size_t(t)], ty[std::size_t(t)],
+                                     tr[std::size_t(t)]) ? 0 : 1;
+    for (std::size_t t = 0; t < nt; ++t) {
+      if (!open[t] || blocked(head_new_, rmax_new_, tx[t], ty[t], tr[t])) continue;
+      pos_.push_back(tx[t]);
+      pos_.push_back(ty[t]);
+      r_.push_back(tr[t]);
+      next_.push_back(-1);
+      rmax_new_ = std::max(rmax_new_, tr[t]);
+      link(head_new_, int(r_.size()) - 1);
     }
   }
   const std::vector<double> &positions() const { return pos_; }
@@ -626,19 +643,19 @@ class DiskFiller {
     const double c = std::floor((v - org_[d]) / w_);
     return int(std::max(0.0, std::min(double(n_[d]) - 1.0, c)));
   }
-  void link(int k) {
+  void link(std::vector<int> &head, int k) {
     const std::size_t c = std::size_t(cell(pos_[2 * std::size_t(k) + 1], 1)) * n_[0] +
                           std::size_t(cell(pos_[2 * std::size_t(k)], 0));
-    next_.push_back(head_[c]);
-    head_[c] = k;
+    next_[std::size_t(k)] = head[c];
+    head[c] = k;
   }
-  bool blocked(double x, double y, double r) const {
-    const double reach = r + rmax_;
+  bool blocked(const std::vector<int> &head, double rmax, double x, double y, double r) const {
+    const double reach = r + rmax;
     const int i0 = cell(x - reach, 0), i1 = cell(x + reach, 0);
     const int j0 = cell(y - reach, 1), j1 = cell(y + reach, 1);
     for (int j = j0; j <= j1; ++j)
       for (int i = i0; i <= i1; ++i)
-        for (int k = head_[std::size_t(j) * n_[0] + std::size_t(i)]; k >= 0;
+        for (int k = head[std::size_t(j) * n_[0] + std::size_t(i)]; k >= 0;
              k = next_[std::size_t(k)]) {
           const std::size_t K = std::size_t(k);
           if (std::hypot(pos_[2 * K] - x, pos_[2 * K + 1] - y) < (r_[K] + r) * 0.97) return true;
@@ -647,9 +664,9 @@ class DiskFiller {
   }
 
   std::vector<double> pos_, r_;
-  double rmax_ = 0.0, w_ = 1.0, org_[2] = {0.0, 0.0};
+  double rmax_old_ = 0.0, rmax_new_ = 0.0, w_ = 1.0, org_[2] = {0.0, 0.0};
   std::uint32_t n_[2] = {1, 1};
-  std::vector<int> head_, next_;
+  std::vector<int> head_old_, head_new_, next_;
 };
 }  // namespace
 
@@ -838,57 +855,89 @@ void for_ids_in_box(const CellBins &b, const double x[2], double r, F f) {
 // Grid: mass-weighted nodal fields of the first n points, normalised by the
 // nodal mass, with a default where the mass is zero (Grid.cpp:149-353).
 // Stencil nodes past the grid (which the reference only asserts against) are
-// skipped here and in the interpolations below.
-struct NodalField {
-  std::vector<double> m, val;
+// skipped here and in the interpolations below.  Only the nodes that the
+// points to be interpolated reach are computed: each of those sums its points
+// in index order, as a pass over the whole grid would, and the other nodes
+// are never read.  The fields (J, v and b_bar) share one pass.
+struct NodalFields {
+  std::vector<int> slot;  // node -> compact index, or -1
+  std::vector<double> m, val;  // per compact node: mass, then `len` values
   int len = 0;
 };
-NodalField nodal_field(const SimulationState &st, std::size_t n, int len,
-                       const std::vector<double> &values, const double *defaults) {
+
+void node_range(const SimulationState &st, const double x[2], double hl, long lo[2], long hi[2]) {
   const PhysicsGrid &g = st.physics_grid;
-  const std::size_t nn = g.numGridPoints();
+  basisStencil(st.basis, x, hl, g.min, g.cell_width, lo, hi);
+  lo[0] = std::max(lo[0], 0L);
+  lo[1] = std::max(lo[1], 0L);
+  hi[0] = std::min(hi[0], long(g.cell_count[0]));
+  hi[1] = std::min(hi[1], long(g.cell_count[1]));
+}
+
+NodalFields nodal_fields(const SimulationState &st, std::size_t n, std::size_t q_begin,
+                         std::size_t q_end, const std::vector<const std::vector<double> *> &values,
+                         const std::vector<int> &widths, const std::vector<double> &defaults) {
+  const PhysicsGrid &g = st.physics_grid;
   const std::uint32_t W = g.cell_count[0] + 1;
-  NodalField f;
-  f.len = len;
-  f.m.assign(nn, 0.0);
-  f.val.assign(nn * std::size_t(len), 0.0);
   const MaterialPoints &mp = st.material_points;
   const double h1 = mp.hl.empty() ? 0.0 : mp.hl[0];
-  for (std::size_t p = 0; p < n; ++p) {
+  NodalFields f;
+  for (int w : widths) f.len += w;
+  f.slot.assign(g.numGridPoints(), -1);
+  int nslot = 0;
+  for (std::size_t p = q_begin; p < q_end; ++p) {
     long lo[2], hi[2];
-    basisStencil(st.basis, &mp.q[2 * p], h1, g.min, g.cell_width, lo, hi);
-    for (long y = std::max(lo[1], 0L); y <= std::min(hi[1], long(g.cell_count[1])); ++y)
-      for (long x = std::max(lo[0], 0L); x <= std::min(hi[0], long(g.cell_count[0])); ++x) {
-        const double w = basisWeight(st.basis, &mp.q[2 * p], h1, std::uint32_t(x), std::uint32_t(y),
-                                     g.min, g.cell_width);
-        const std::size_t node = std::size_t(y) * W + std::size_t(x);
-        f.m[node] += w * mp.m[p];
-        for (int k = 0; k < len; ++k)
-          f.val[node * std::size_t(len) + std::size_t(k)] +=
-              w * mp.m[p] * values[p * std::size_t(len) + std::size_t(k)];
+    node_range(st, &mp.q[2 * p], h1, lo, hi);
+    for (long y = lo[1]; y <= hi[1]; ++y)
+      for (long x = lo[0]; x <= hi[0]; ++x) {
+        int &sl = f.slot[std::size_t(y) * W + std::size_t(x)];
+        if (sl < 0) sl = nslot++;
       }
   }
-  for (std::size_t node = 0; node < nn; ++node)
-    for (int k = 0; k < len; ++k) {
-      double &v = f.val[node * std::size_t(len) + std::size_t(k)];
-      v = f.m[node] > 0.0 ? v / f.m[node] : defaults[k];
+  f.m.assign(std::size_t(nslot), 0.0);
+  f.val.assign(std::size_t(nslot) * std::size_t(f.len), 0.0);
+  for (std::size_t p = 0; p < n; ++p) {
+    long lo[2], hi[2];
+    node_range(st, &mp.q[2 * p], h1, lo, hi);
+    for (long y = lo[1]; y <= hi[1]; ++y)
+      for (long x = lo[0]; x <= hi[0]; ++x) {
+        const int sl = f.slot[std::size_t(y) * W + std::size_t(x)];
+        if (sl < 0) continue;
+        const double w = basisWeight(st.basis, &mp.q[2 * p], h1, std::uint32_t(x), std::uint32_t(y),
+                                     g.min, g.cell_width);
+        f.m[std::size_t(sl)] += w * mp.m[p];
+        double *out = &f.val[std::size_t(sl) * std::size_t(f.len)];
+        for (std::size_t v = 0; v < values.size(); ++v) {
+          const int len = widths[v];
+          for (int k = 0; k < len; ++k)
+            out[k] += w * mp.m[p] * (*values[v])[p * std::size_t(len) + std::size_t(k)];
+          out += len;
+        }
+      }
+  }
+  for (std::size_t sl = 0; sl < f.m.size(); ++sl)
+    for (int k = 0; k < f.len; ++k) {
+      double &v = f.val[sl * std::size_t(f.len) + std::size_t(k)];
+      v = f.m[sl] > 0.0 ? v / f.m[sl] : defaults[std::size_t(k)];
     }
   return f;
 }
 
-void interpolate(const SimulationState &st, const NodalField &f, const double x[2], double *out) {
+// The fields at x: out gets f.len values.
+void interpolate(const SimulationState &st, const NodalFields &f, const double x[2], double *out) {
   const PhysicsGrid &g = st.physics_grid;
   const std::uint32_t W = g.cell_count[0] + 1;
   const double h1 = st.material_points.hl[0];
   long lo[2], hi[2];
-  basisStencil(st.basis, x, h1, g.min, g.cell_width, lo, hi);
+  node_range(st, x, h1, lo, hi);
   for (int k = 0; k < f.len; ++k) out[k] = 0.0;
-  for (long y = std::max(lo[1], 0L); y <= std::min(hi[1], long(g.cell_count[1])); ++y)
-    for (long xx = std::max(lo[0], 0L); xx <= std::min(hi[0], long(g.cell_count[0])); ++xx) {
+  for (long y = lo[1]; y <= hi[1]; ++y)
+    for (long xx = lo[0]; xx <= hi[0]; ++xx) {
       const double w = basisWeight(st.basis, x, h1, std::uint32_t(xx), std::uint32_t(y), g.min,
                                    g.cell_width);
-      const std::size_t node = std::size_t(y) * W + std::size_t(xx);
-      for (int k = 0; k < f.len; ++k) out[k] += w * f.val[node * std::size_t(f.len) + std::size_t(k)];
+      const int sl = f.slot[std::size_t(y) * W + std::size_t(xx)];
+      for (int k = 0; k < f.len; ++k)
+        out[k] += w * f.val[std::size_t(sl) * std::size_t(f.len) + std::size_t(k)];
     }
 }
 
@@ -1037,6 +1086,9 @@ ResampleReport ResamplingTools::resample(ZoneTools &zt, HybridResampleState &s,
     }
     const double sigma = 0.5 * s.initial_particle_size;
     const double max_dist = 3.0 * s.initial_particle_size;
+    // each new point reads only the points and grains that were there before
+    // and writes only its own fields
+#pragma omp parallel for schedule(dynamic, 256)
     for (int pi = before; pi < after; ++pi) {
       const std::size_t P = std::size_t(pi);
       const double x[2] = {mp.q[2 * P], mp.q[2 * P + 1]};
@@ -1122,6 +1174,7 @@ ResampleReport ResamplingTools::resample(ZoneTools &zt, HybridResampleState &s,
   // determineDEMQuantitiesForHybridZones (:647-734)
   {
     const double sigma = r_mean_, max_dist = 6.0 * r_mean_;
+#pragma omp parallel for schedule(dynamic, 256)
     for (int b = g_before; b < g_after; ++b) {
       const std::size_t B = std::size_t(b);
       const double x[2] = {ds.q[3 * B], ds.q[3 * B + 1]};
@@ -1161,11 +1214,10 @@ ResampleReport ResamplingTools::resample(ZoneTools &zt, HybridResampleState &s,
   // determineMPMQuantitiesForInnerContinuumZones (:381-420)
   {
     const std::size_t n = std::size_t(p2_before);
-    std::vector<double> Jv(mp.J.begin(), mp.J.end());
-    const double dJ[1] = {1.0}, dV[2] = {0.0, 0.0}, dB[4] = {1.0, 0.0, 0.0, 1.0};
-    const NodalField fJ = nodal_field(cs, n, 1, Jv, dJ);
-    const NodalField fV = nodal_field(cs, n, 2, mp.v, dV);
-    const NodalField fB = nodal_field(cs, n, 4, mp.be_bar, dB);
+    // J, v and b_bar of the first n points, at the nodes the new points reach
+    const NodalFields fld = nodal_fields(cs, n, std::size_t(p2_before), std::size_t(p2_after),
+                                         {&mp.J, &mp.v, &mp.be_bar}, {1, 2, 4},
+                                         {1.0, 0.0, 0.0, 1.0, 0.0, 0.0, 1.0});
     // gatherMassAndVolume (:198-280)
     const double sigma = 0.5 * s.initial_particle_size, r = 2.0 * s.initial_particle_size;
     for (int pi = p2_before; pi < p2_after; ++pi) {
@@ -1199,12 +1251,14 @@ ResampleReport ResamplingTools::resample(ZoneTools &zt, HybridResampleState &s,
         s.mpm_volume0[k] *= f;
       }
     }
+#pragma omp parallel for schedule(dynamic, 256)
     for (int pi = p2_before; pi < p2_after; ++pi) {
       const std::size_t P = std::size_t(pi);
-      double J, be[4], v[2];
-      interpolate(cs, fJ, &mp.q[2 * P], &J);
-      interpolate(cs, fB, &mp.q[2 * P], be);
-      interpolate(cs, fV, &mp.q[2 * P], v);
+      double f7[7];
+      interpolate(cs, fld, &mp.q[2 * P], f7);
+      const double J = f7[0];
+      const double v[2] = {f7[1], f7[2]};
+      double be[4] = {f7[3], f7[4], f7[5], f7[6]};
       const double sd = std::sqrt(be[0] * be[3] - be[2] * be[1]);
       for (double &b : be) b /= sd;
       mp.v[2 * P] = v[0];
