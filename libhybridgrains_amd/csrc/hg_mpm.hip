@@ -984,22 +984,15 @@ struct FusedArgs {
   int32_t *err;
 };
 
-// A term already scaled to quanta, rounded down to a whole number of them, as
-// a two's-complement 64-bit integer (the caller bounds it below 2^50).  In
-// FP32 and integer operations, all exact: f = floor(scaled) is a float; its
-// magnitude a splits into hi = floor(a 2^-32) and lo = a - hi 2^32, which is
-// exact because a >= 2^32 makes a and hi 2^32 multiples of 2^9, so lo < 2^32
-// needs at most 23 significant bits; both halves convert exactly to 32-bit
-// integers, and a negative f negates the 64-bit word.
+// A term already scaled to quanta, rounded to the nearest whole number of
+// them (ties to even), as a two's-complement 64-bit integer (the caller
+// bounds it below 2^50).  The term is exact in FP64; adding 1.5 2^52 brings
+// it into [2^52, 2^53), where the spacing of doubles is 1, so the addition
+// does the rounding, and the sum's bit pattern is 0x4338000000000000 plus the
+// rounded term.
 __device__ __forceinline__ unsigned long long fx_of(float scaled) {
-  const float f = floorf(scaled);
-  const float a = fabsf(f);
-  const float hi = floorf(a * 0x1p-32f);
-  const float lo = __builtin_fmaf(-hi, 0x1p32f, a);
-  const unsigned long long u =
-      (static_cast<unsigned long long>(static_cast<uint32_t>(hi)) << 32) |
-      static_cast<unsigned long long>(static_cast<uint32_t>(lo));
-  return f < 0.0f ? 0ull - u : u;
+  const double b = double(scaled) + 0x1.8p52;
+  return static_cast<unsigned long long>(__double_as_longlong(b)) - 0x4338000000000000ull;
 }
 
 // A point's rasterisation terms in FP32: per axis the S weights and masked
@@ -1144,6 +1137,14 @@ constexpr int kFarCap = 256;
 
 template <int TW, int TH> constexpr int fused_threads() { return TW * TH; }
 
+// A workgroup barrier that orders LDS only: the global stores before it may
+// still be in flight (no thread reads them before a full barrier).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <int BASIS, int TW, int TH, bool PRIME>
 __global__ __launch_bounds__((fused_threads<TW, TH>())) void mpm_fused_kernel(FusedArgs a) {
   constexpr int S = BasisTraits<BASIS>::S;
@@ -1287,7 +1288,7 @@ __global__ __launch_bounds__((fused_threads<TW, TH>())) void mpm_fused_kernel(Fu
         tag = PRIME ? p : sp;
       }
       s_pi[t] = tag;
-      __syncthreads();
+      lds_barrier();
       // the moved points' terms for the next step, at their new stencils
       const uint32_t ptag = s_pi[lane_pt];
       if (ptag != 0xffffffffu) {
@@ -1323,7 +1324,7 @@ __global__ __launch_bounds__((fused_threads<TW, TH>())) void mpm_fused_kernel(Fu
           else atomicOr(&a.err[sid], kMpmFixedRangeBit);
         }
       }
-      __syncthreads();  // the stash is refilled by the next pass
+      lds_barrier();  // the stash is refilled by the next pass
     }
     __syncthreads();
     // the window's sums into the node sums (nodes off the grid get nothing)

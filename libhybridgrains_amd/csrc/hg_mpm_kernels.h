@@ -40,9 +40,9 @@ enum : uint32_t {
 };
 
 // The FP32 step's node sums are exact: each point-node contribution, computed
-// in FP32 and scaled by 2^shift (a per-scene power of two), is rounded down
-// to a whole number of quanta and summed as a 64-bit integer, so the sum does
-// not depend on the order in which the contributions arrive.  Quantities:
+// in FP32 and scaled by 2^shift (a per-scene power of two), is rounded to the
+// nearest whole number of quanta and summed as a 64-bit integer, so the sum
+// does not depend on the order in which the contributions arrive.  Quantities:
 // node mass, the two momentum components, the two internal-force
 // components.
 enum : int { kFxMass = 0, kFxMom0 = 1, kFxMom1 = 2, kFxF0 = 3, kFxF1 = 4, kFxQuantities = 5 };
