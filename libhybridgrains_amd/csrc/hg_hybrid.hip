@@ -58,6 +58,10 @@ struct HybArgs {
   uint32_t *count, *next_count;  // this step's list length; the next step's
   int32_t *err;
   uint8_t *hfac;  // coupling weights per node in halves (0, 1, 2), or null
+  // FP32 fused continuum: the node mass sums in quanta, and the quantum
+  // (null when grain_mass reads the grid update's mass)
+  const unsigned long long *acc_mass;
+  double mass_quantum;
 };
 
 template <typename T>
@@ -173,7 +177,11 @@ __global__ __launch_bounds__(kThreads) void hyb_grain_mass(HybArgs a) {
         if (!((g.on >> q) & 1u)) continue;
         const int ix = g.lx + q % S, iy = g.ly + q / S;
         const uint32_t node = sc.node_off + uint32_t(iy) * uint32_t(sc.snx) + uint32_t(ix);
-        const double m = node_active(a, ix, iy) ? a.gm[node] : 0.0;
+        // the continuum mass of this step's grid: from the node sums when
+        // they are what the grid update will read (zero off the points'
+        // stencils), else from the grid update's output on active tiles
+        const double m = a.acc_mass ? double((long long)a.acc_mass[node]) * a.mass_quantum
+                                    : (node_active(a, ix, iy) ? a.gm[node] : 0.0);
         if (m <= 1.0e-6) all_pos = false;
       }
       if (all_pos) {
@@ -225,35 +233,12 @@ __global__ __launch_bounds__(kThreads) void hyb_grain_mass(HybArgs a) {
 // P2G wrote.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void hyb_exchange(HybArgs a) {
-  using T4 = typename hgmpm::Vec4T<T>::type;
   const uint32_t cnt = min(*a.count, a.g.max_list);
   for (uint32_t k = blockIdx.x * kThreads + threadIdx.x; k < cnt; k += gridDim.x * kThreads) {
     const uint32_t n = a.g.list[k];
-    const double m_dem = a.g.dmass[n];
-    if (!(m_dem > 0.0)) continue;
-    const uint32_t node = a.msc->node_off + n;
-    const double m_mpm = a.gm[node];
-    const double msum = m_mpm + m_dem;
-    const double l0 = (m_dem * a.gpn[2 * node] - m_mpm * a.g.dmom[2 * n]) / msum;
-    const double l1 = (m_dem * a.gpn[2 * node + 1] - m_mpm * a.g.dmom[2 * n + 1]) / msum;
-    const double pn0 = a.gpn[2 * node] - l0, pn1 = a.gpn[2 * node + 1] - l1;
-    a.gpn[2 * node] = pn0;
-    a.gpn[2 * node + 1] = pn1;
-    a.g.vafter[2 * n] = (a.g.dmom[2 * n] + l0) / m_dem;
-    a.g.vafter[2 * n + 1] = (a.g.dmom[2 * n + 1] + l1) / m_dem;
-    a.g.aafter[2 * n] = l0 / m_dem;
-    a.g.aafter[2 * n + 1] = l1 / m_dem;
-    T4 va;
-    if (m_mpm > 0.0) {
-      const double dm = a.msc->dt * m_mpm;
-      va.x = T(pn0 / m_mpm);
-      va.y = T(pn1 / m_mpm);
-      va.z = T((pn0 - a.gp[2 * node]) / dm);
-      va.w = T((pn1 - a.gp[2 * node + 1]) / dm);
-    } else {
-      va.x = va.y = va.z = va.w = T(0);
-    }
-    static_cast<T4 *>(a.gva)[node] = va;
+    if (!(a.g.dmass[n] > 0.0)) continue;
+    const hgmpm::GrainNodes gn = {a.g.dmass, a.g.dmom, a.g.vafter, a.g.aafter};
+    hgmpm::exchange_node<T>(a.msc->dt, a.msc->node_off + n, n, gn, a.gm, a.gp, a.gpn, a.gva);
   }
 }
 
@@ -510,6 +495,8 @@ static HybArgs make_args(hg_hybrid *h) {
   a.next_count = h->g.count + (1 - h->par);
   a.err = h->err;
   a.hfac = h->hfac_on ? h->hfac : nullptr;
+  a.acc_mass = nullptr;
+  a.mass_quantum = 0.0;
   return a;
 }
 
@@ -538,6 +525,87 @@ static void launch_hybrid_steps(hg_hybrid *h, double dt, uint32_t n, bool concur
     hg_dem_launch_step(h->dem, false);
     h->dem->stream = s;
   };
+  auto homog = [&](const HybArgs &a) {
+    if (!(h->homog_stress || h->homog_vel)) return;
+    HomogArgs g;
+    g.hg = h->hg;
+    g.s = h->dem->slots[h->dem->cur];
+    g.oF = h->dem->f.oF;
+    g.cpoint = h->dem->cpoint;
+    g.count = h->hg.count + h->par;
+    g.next_count = h->hg.count + (1 - h->par);
+    g.stress = h->homog_stress ? 1 : 0;
+    g.vel = h->homog_vel ? 1 : 0;
+    g.cap = uint32_t(h->dem->cap);
+    {
+      ProfScope ps(prof, "hyb_homog_grains", side);
+      hipLaunchKernelGGL((hyb_homog_grains<T, BASIS>), gb, tb, 0, side, a, g);
+    }
+    if (h->homog_vel) {
+      ProfScope ps(prof, "hyb_homog_nodes", side);
+      hipLaunchKernelGGL(hyb_homog_nodes, gb, tb, 0, side, g);
+    }
+  };
+  auto homog_factor = [&](const HybArgs &a) {
+    if (!a.hfac) return;
+    // on the continuum stream, before the fork: it reads the tile flags,
+    // which the sort after G2P rewrites on this stream for the next step
+    const hgmpm::SceneDev &msc = h->mpm->scenes[0];
+    const uint32_t per_row = (uint32_t(msc.nnx) + 4 * kThreads - 1) / (4 * kThreads);
+    ProfScope ps(prof, "hyb_homog_factor", s);
+    hipLaunchKernelGGL(hyb_homog_factor, dim3(per_row, uint32_t(msc.nny)), tb, 0, s, a);
+  };
+  auto grain_end = [&](const HybArgs &a) {
+    {
+      ProfScope ps(prof, "hyb_transfer", side);
+      hipLaunchKernelGGL((hyb_transfer<T, BASIS>), gb, tb, 0, side, a);
+    }
+    {
+      ProfScope ps(prof, "hyb_finish", side);
+      hipLaunchKernelGGL(hyb_finish<T>, gb, tb, 0, side, a, dt);
+    }
+    h->dem->stream = side;
+    hg_dem_launch_sort(h->dem);
+    h->dem->stream = s;
+    homog(a);
+  };
+  if (hg_mpm_is_fused(h->mpm)) {
+    // FP32 continuum.  The node sums of the moved points are complete when
+    // the fused G2P ends, so the next step's grain mass (which reads the
+    // continuum mass from them) runs on the grain stream during the
+    // continuum's sort, and the grid update applies the exchange at the
+    // nodes holding grain mass: neither is on the continuum's path.
+    hg_mpm_prepare_fused(h->mpm, dt);
+    const hgmpm::GrainNodes gn = {h->g.dmass, h->g.dmom, h->g.vafter, h->g.aafter};
+    auto grain_mass = [&]() {
+      HybArgs a = make_args(h);
+      a.acc_mass = hg_mpm_mass_sums(h->mpm);
+      a.mass_quantum = hg_mpm_mass_quantum(h->mpm);
+      ProfScope ps(prof, "hyb_grain_mass", side);
+      hipLaunchKernelGGL((hyb_grain_mass<T, BASIS>), gb, tb, 0, side, a);
+    };
+    order(h->fork_ev, s, side);
+    dem_step();
+    grain_mass();
+    for (uint32_t k = 0; k < n; ++k) {
+      const HybArgs a = make_args(h);
+      order(h->join_ev, side, s);
+      hg_mpm_launch_coupled_grid(h->mpm, dt, &gn);
+      homog_factor(a);
+      order(h->fork_ev, s, side);
+      grain_end(a);
+      hg_mpm_launch_fused_g2p(h->mpm);
+      h->par = 1 - h->par;
+      if (k + 1 < n) {
+        dem_step();  // the next step's DEM step, ahead
+        order(h->fork_ev, s, side);
+        grain_mass();
+      }
+      hg_mpm_launch_fused_sort(h->mpm);
+    }
+    order(h->join_ev, side, s);
+    return;
+  }
   order(h->fork_ev, s, side);
   dem_step();
   for (uint32_t k = 0; k < n; ++k) {
@@ -560,46 +628,9 @@ static void launch_hybrid_steps(hg_hybrid *h, double dt, uint32_t n, bool concur
       ProfScope ps(prof, "hyb_exchange", s);
       hipLaunchKernelGGL(hyb_exchange<T>, gb, tb, 0, s, a);
     }
-    if (a.hfac) {
-      // on the continuum stream, before the fork: it reads the tile flags,
-      // which the sort after G2P rewrites on this stream for the next step
-      const hgmpm::SceneDev &msc = h->mpm->scenes[0];
-      const uint32_t per_row = (uint32_t(msc.nnx) + 4 * kThreads - 1) / (4 * kThreads);
-      ProfScope ps(prof, "hyb_homog_factor", s);
-      hipLaunchKernelGGL(hyb_homog_factor, dim3(per_row, uint32_t(msc.nny)), tb, 0, s, a);
-    }
+    homog_factor(a);
     order(h->fork_ev, s, side);
-    {
-      ProfScope ps(prof, "hyb_transfer", side);
-      hipLaunchKernelGGL((hyb_transfer<T, BASIS>), gb, tb, 0, side, a);
-    }
-    {
-      ProfScope ps(prof, "hyb_finish", side);
-      hipLaunchKernelGGL(hyb_finish<T>, gb, tb, 0, side, a, dt);
-    }
-    h->dem->stream = side;
-    hg_dem_launch_sort(h->dem);
-    h->dem->stream = s;
-    if (h->homog_stress || h->homog_vel) {
-      HomogArgs g;
-      g.hg = h->hg;
-      g.s = h->dem->slots[h->dem->cur];
-      g.oF = h->dem->f.oF;
-      g.cpoint = h->dem->cpoint;
-      g.count = h->hg.count + h->par;
-      g.next_count = h->hg.count + (1 - h->par);
-      g.stress = h->homog_stress ? 1 : 0;
-      g.vel = h->homog_vel ? 1 : 0;
-      g.cap = uint32_t(h->dem->cap);
-      {
-        ProfScope ps(prof, "hyb_homog_grains", side);
-        hipLaunchKernelGGL((hyb_homog_grains<T, BASIS>), gb, tb, 0, side, a, g);
-      }
-      if (h->homog_vel) {
-        ProfScope ps(prof, "hyb_homog_nodes", side);
-        hipLaunchKernelGGL(hyb_homog_nodes, gb, tb, 0, side, g);
-      }
-    }
+    grain_end(a);
     if (k + 1 < n) dem_step();  // the next step's DEM step, ahead
     hg_mpm_launch_g2p(h->mpm);
     h->par = 1 - h->par;

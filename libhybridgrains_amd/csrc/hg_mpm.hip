@@ -1369,10 +1369,14 @@ struct GridArgs {
   uint32_t store;
   int32_t *err;
   int32_t tw, th;
+  GrainNodes ex;     // the hybrid's grain side of each node
+  int32_t exchange;  // apply the grain exchange where grain mass lies
 };
 
 // Node sums of 2^62 quanta or more are flagged: further additions could wrap
-// around.
+// around.  With exchange set (the hybrid's continuum, one scene, node mass
+// and both momenta stored) a node holding grain mass then gets the
+// grain-continuum exchange.
 __global__ __launch_bounds__(256) void mpm_grid_kernel(GridArgs a) {
   const uint32_t ntile_list = *a.tile_count;
   const uint32_t per_xcd = (ntile_list + 7u) >> 3, xcd = blockIdx.x & 7u;
@@ -1403,6 +1407,10 @@ __global__ __launch_bounds__(256) void mpm_grid_kernel(GridArgs a) {
       if (big) atomicOr(&a.err[sid], kMpmFixedRangeBit);
       grid_node_update<float>(sc, nx, ny, v[kFxMass], v[kFxMom0], v[kFxMom1], v[kFxF0], v[kFxF1],
                               a.store, a.gva, a.gm, a.gp, a.gpn, a.gf);
+      if (a.exchange) {
+        const uint32_t n = node - sc.node_off;
+        if (a.ex.dmass[n] > 0.0) exchange_node<float>(sc.dt, node, n, a.ex, a.gm, a.gp, a.gpn, a.gva);
+      }
     }
   }
 }
@@ -1666,7 +1674,7 @@ static void launch_fused(hg_mpm *h, bool prime) {
 
 // The grid kernel over the current active tiles: node sums -> grid update,
 // sums cleared.
-static void launch_grid(hg_mpm *h, uint32_t store) {
+static void launch_grid(hg_mpm *h, uint32_t store, const GrainNodes *ex = nullptr) {
   GridArgs a;
   a.scenes = h->d_scenes;
   a.tile_scene = h->d_tile_scene;
@@ -1683,6 +1691,9 @@ static void launch_grid(hg_mpm *h, uint32_t store) {
   a.err = h->err;
   a.tw = h->tw();
   a.th = h->th();
+  a.exchange = ex ? 1 : 0;
+  if (ex) a.ex = *ex;
+  else a.ex = GrainNodes{nullptr, nullptr, nullptr, nullptr};
   ProfScope ps(h->prof, "mpm_grid", h->stream);
   hipLaunchKernelGGL(mpm_grid_kernel, dim3(p2g_grid(h)), dim3(256), 0, h->stream, a);
   h->acc_ready = false;
@@ -1765,6 +1776,36 @@ hg_status hg_mpm_launch_coupled_p2g(hg_mpm *h, double dt) {
   launch_p2g(h, store);
   return HG_OK;
 }
+
+bool hg_mpm_is_fused(const hg_mpm *h) { return fused_mode(h); }
+
+hg_status hg_mpm_prepare_fused(hg_mpm *h, double dt) {
+  HG_TRY(upload_dt(h, dt));
+  hg_mpm_prime(h);
+  return HG_OK;
+}
+
+hg_status hg_mpm_launch_coupled_grid(hg_mpm *h, double dt, const GrainNodes *g) {
+  HG_TRY(upload_dt(h, dt));
+  const uint32_t store = kStoreVelAcc | kStoreMass | kStoreMom | kStoreMomNew;
+  if (!h->acc_ready) launch_fused(h, true);
+  launch_grid(h, store, g);
+  return HG_OK;
+}
+
+void hg_mpm_launch_fused_g2p(hg_mpm *h) {
+  launch_fused(h, false);
+  h->cur = 1 - h->cur;
+  h->acc_ready = true;
+  h->grid_ready = false;
+}
+
+void hg_mpm_launch_fused_sort(hg_mpm *h) { launch_sort(h); }
+
+const unsigned long long *hg_mpm_mass_sums(const hg_mpm *h) {
+  return h->acc + size_t(kFxMass) * h->nnode;
+}
+double hg_mpm_mass_quantum(const hg_mpm *h) { return h->scenes[0].fx_quantum[kFxMass]; }
 
 void hg_mpm_launch_g2p(hg_mpm *h) {
   if (fused_mode(h)) {

@@ -67,6 +67,22 @@ void hg_mpm_launch_g2p(hg_mpm *h);
 // Before capturing steps into a graph: FP32 handles rasterise the current
 // points now if no step has (so every captured step finds the sums ready).
 void hg_mpm_prime(hg_mpm *h);
+// FP32 handles that run the fused step (the hybrid's continuum reorders the
+// halves around the grain work):
+bool hg_mpm_is_fused(const hg_mpm *h);
+// dt on the device and the node sums of the current points ready (a
+// rasterisation if no step has left them)
+hg_status hg_mpm_prepare_fused(hg_mpm *h, double dt);
+// the grid update of the current points from the node sums (rasterised
+// first if no step has), storing node mass, both momenta and (v, a), with the
+// grain exchange at every node that holds grain mass (g may be null)
+hg_status hg_mpm_launch_coupled_grid(hg_mpm *h, double dt, const hgmpm::GrainNodes *g);
+// the fused G2P and rasterisation alone, then the sort of the moved points
+void hg_mpm_launch_fused_g2p(hg_mpm *h);
+void hg_mpm_launch_fused_sort(hg_mpm *h);
+// the node mass sums (whole quanta) and the quantum of scene 0
+const unsigned long long *hg_mpm_mass_sums(const hg_mpm *h);
+double hg_mpm_mass_quantum(const hg_mpm *h);
 // Re-keys and re-sorts the points of the current buffer, as at creation,
 // and waits for it (used after the buffer was rewritten).
 hg_status hg_mpm_rebin(hg_mpm *h);

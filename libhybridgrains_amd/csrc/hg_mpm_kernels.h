@@ -279,4 +279,46 @@ __device__ __forceinline__ void basis_factor(double x, T hl, double gmin, double
   }
 }
 
+// The grain-continuum exchange at one node holding grain mass
+// (HybridCoupling2D::updateVelocitiesOnGrid, Coupling.cpp:533-579): the
+// multiplier that equalises the two velocities while conserving the node's
+// momentum; the continuum's new momentum and its (v, a) (as the grid update
+// computes them, PhysicsGrid.cpp:588-609), and the grain side's velocity and
+// acceleration after the exchange.  node indexes the continuum's arrays, n
+// the grain-side arrays (the node's index within its scene).
+struct GrainNodes {
+  const double *dmass, *dmom;
+  double *vafter, *aafter;
+};
+
+template <typename T>
+__device__ __forceinline__ void exchange_node(double dt, uint32_t node, uint32_t n,
+                                              const GrainNodes &g, const double *gm,
+                                              const double *gp, double *gpn, void *gva) {
+  using T4 = typename Vec4T<T>::type;
+  const double m_dem = g.dmass[n];
+  const double m_mpm = gm[node];
+  const double msum = m_mpm + m_dem;
+  const double l0 = (m_dem * gpn[2 * node] - m_mpm * g.dmom[2 * n]) / msum;
+  const double l1 = (m_dem * gpn[2 * node + 1] - m_mpm * g.dmom[2 * n + 1]) / msum;
+  const double pn0 = gpn[2 * node] - l0, pn1 = gpn[2 * node + 1] - l1;
+  gpn[2 * node] = pn0;
+  gpn[2 * node + 1] = pn1;
+  g.vafter[2 * n] = (g.dmom[2 * n] + l0) / m_dem;
+  g.vafter[2 * n + 1] = (g.dmom[2 * n + 1] + l1) / m_dem;
+  g.aafter[2 * n] = l0 / m_dem;
+  g.aafter[2 * n + 1] = l1 / m_dem;
+  T4 va;
+  if (m_mpm > 0.0) {
+    const double dm = dt * m_mpm;
+    va.x = T(pn0 / m_mpm);
+    va.y = T(pn1 / m_mpm);
+    va.z = T((pn0 - gp[2 * node]) / dm);
+    va.w = T((pn1 - gp[2 * node + 1]) / dm);
+  } else {
+    va.x = va.y = va.z = va.w = T(0);
+  }
+  static_cast<T4 *>(gva)[node] = va;
+}
+
 }  // namespace hgmpm
