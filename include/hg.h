@@ -159,9 +159,19 @@ hg_status hg_mpm_set_tile(hg_mpm *h, int32_t aw, int32_t ah);
 /* Autotuner (no reference counterpart): times the P2G pass for each tile
  * shape on the handle's own scenes (reps launches each; the point state is
  * not changed), keeps the fastest and returns it; us (4 entries, or NULL)
- * receives the microseconds per launch of 16x16, 16x8, 32x8 and 64x4. */
+ * receives the microseconds per launch of 16x16, 16x8, 32x8 and 64x4.  FP32
+ * handles time whole fused steps instead (the point state is saved first and
+ * restored after), each shape with one and with two threads per cell, and us
+ * gets the faster of the two. */
 hg_status hg_mpm_autotune(hg_mpm *h, double dt, int32_t reps, int32_t *aw, int32_t *ah,
                           double *us);
+/* The tuning in use (no reference counterpart): the tile shape and, for FP32
+ * handles, the points each tile takes per pass in threads per cell (1 or 2;
+ * the autotuner times both and keeps the faster; 1 otherwise). */
+hg_status hg_mpm_tuning(const hg_mpm *h, int32_t *aw, int32_t *ah, int32_t *points_per_pass);
+/* Sets the FP32 step's points per tile pass (1 or 2 threads per cell; no
+ * reference counterpart; ignored in FP64).  Results do not depend on it. */
+hg_status hg_mpm_set_points_per_pass(hg_mpm *h, int32_t points_per_pass);
 
 /* ---- timing (no reference counterpart) --------------------------------------
  * With profiling on, every kernel launch is bracketed by HIP events on the

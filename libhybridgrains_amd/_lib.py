@@ -150,6 +150,8 @@ def lib() -> C.CDLL:
         ),
         "hg_mpm_set_tile": ([vp, C.c_int32, C.c_int32], C.c_int32),
         "hg_mpm_autotune": ([vp, C.c_double, C.c_int32, _ip, _ip, _dp], C.c_int32),
+        "hg_mpm_tuning": ([vp, _ip, _ip, _ip], C.c_int32),
+        "hg_mpm_set_points_per_pass": ([vp, C.c_int32], C.c_int32),
         "hg_bench_copy": ([vp, C.c_uint64, C.c_int32, _dp], C.c_int32),
         "hg_dem_create": ([vp, C.POINTER(DemDesc), C.POINTER(vp)], C.c_int32),
         "hg_dem_create_batch": ([vp, C.POINTER(DemDesc), C.c_uint32, C.POINTER(vp)], C.c_int32),

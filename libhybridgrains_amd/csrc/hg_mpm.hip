@@ -1135,7 +1135,9 @@ __device__ __forceinline__ FusedPoint fused_load(const PtsRaw &p, uint32_t i, co
 // are added after the tile's pass straight into the node sums.
 constexpr int kFarCap = 256;
 
-template <int TW, int TH> constexpr int fused_threads() { return TW * TH; }
+// Threads of a fused tile: MULT per owned cell (MULT points per cell per
+// pass), a knob the autotuner sets.
+template <int TW, int TH, int MULT> constexpr int fused_threads() { return MULT * TW * TH; }
 
 // A workgroup barrier that orders LDS only: the global stores before it may
 // still be in flight (no thread reads them before a full barrier).
@@ -1145,10 +1147,10 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-template <int BASIS, int TW, int TH, bool PRIME>
-__global__ __launch_bounds__((fused_threads<TW, TH>())) void mpm_fused_kernel(FusedArgs a) {
+template <int BASIS, int TW, int TH, bool PRIME, int MULT>
+__global__ __launch_bounds__((fused_threads<TW, TH, MULT>())) void mpm_fused_kernel(FusedArgs a) {
   constexpr int S = BasisTraits<BASIS>::S;
-  constexpr int NT = fused_threads<TW, TH>();
+  constexpr int NT = fused_threads<TW, TH, MULT>();
   constexpr int GW = TW + S - 1, GH = TH + S - 1;  // G2P node window
   constexpr int RW = TW + S + 1, RH = TH + S + 1;  // rasterisation window
   constexpr int RN = RW * RH;
@@ -1643,14 +1645,16 @@ static void launch_fused_t(hg_mpm *h, bool prime) {
   a.totals = h->scan_totals + size_t(1 - h->cur) * hg_scan_totals_words(h->nscan);  // the keys of buf[1 - cur]
   a.err = h->err;
   ProfScope ps(h->prof, prime ? "mpm_raster" : "mpm_fused", h->stream);
-  if (prime)
-    hipLaunchKernelGGL((mpm_fused_kernel<BASIS, TW, TH, true>), dim3(p2g_grid(h)),
-                       dim3(fused_threads<TW, TH>()),
-                       0, h->stream, a);
-  else
-    hipLaunchKernelGGL((mpm_fused_kernel<BASIS, TW, TH, false>), dim3(p2g_grid(h)),
-                       dim3(fused_threads<TW, TH>()),
-                       0, h->stream, a);
+  const dim3 g(p2g_grid(h));
+  if (h->ppass == 2) {
+    const dim3 b(fused_threads<TW, TH, 2>());
+    if (prime) hipLaunchKernelGGL((mpm_fused_kernel<BASIS, TW, TH, true, 2>), g, b, 0, h->stream, a);
+    else hipLaunchKernelGGL((mpm_fused_kernel<BASIS, TW, TH, false, 2>), g, b, 0, h->stream, a);
+  } else {
+    const dim3 b(fused_threads<TW, TH, 1>());
+    if (prime) hipLaunchKernelGGL((mpm_fused_kernel<BASIS, TW, TH, true, 1>), g, b, 0, h->stream, a);
+    else hipLaunchKernelGGL((mpm_fused_kernel<BASIS, TW, TH, false, 1>), g, b, 0, h->stream, a);
+  }
 }
 
 template <int BASIS>
@@ -2345,6 +2349,29 @@ extern "C" hg_status hg_mpm_kernel_times(hg_mpm *h, int32_t max_kernels, char *n
   return hg_profiler_report(h->prof, h->ctx, max_kernels, names, ms, launches, nkernels);
 }
 
+extern "C" hg_status hg_mpm_set_points_per_pass(hg_mpm *h, int32_t points_per_pass) {
+  if (!h) return HG_ERR_INVALID;
+  if (points_per_pass != 1 && points_per_pass != 2)
+    return hg_fail(h->ctx, HG_ERR_UNSUPPORTED, "points per pass must be 1 or 2 per cell");
+  if (h->precision != HG_FP32) return HG_OK;  // the FP64 step has no such knob
+  if (points_per_pass == h->ppass) return HG_OK;
+  h->ppass = points_per_pass;
+  if (h->graph) {
+    hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  return HG_OK;
+}
+
+extern "C" hg_status hg_mpm_tuning(const hg_mpm *h, int32_t *aw, int32_t *ah,
+                                   int32_t *points_per_pass) {
+  if (!h) return HG_ERR_INVALID;
+  if (aw) *aw = h->aw;
+  if (ah) *ah = h->ah;
+  if (points_per_pass) *points_per_pass = h->precision == HG_FP32 ? h->ppass : 1;
+  return HG_OK;
+}
+
 extern "C" hg_status hg_mpm_set_tile(hg_mpm *h, int32_t aw, int32_t ah) {
   if (!h) return HG_ERR_INVALID;
   if (!((aw == 16 && ah == 16) || (aw == 16 && ah == 8) || (aw == 32 && ah == 8) ||
@@ -2456,8 +2483,8 @@ extern "C" hg_status hg_mpm_device_bytes(hg_mpm *h, uint64_t *bytes) {
 
 // Tile-shape autotuner: times the P2G pass (which reads the point state and
 // writes only grid fields that every step recomputes), or in FP32 whole fused
-// steps, for each supported tile shape on the handle's own scenes and keeps
-// the fastest.  The choice changes no result: a node's sum order does not
+// steps with one and with two points per cell per tile pass, for each
+// supported tile shape on the handle's own scenes and keeps the fastest.  The choice changes no result: a node's sum order does not
 // depend on the tile shape, and FP32 node sums are exact.
 extern "C" hg_status hg_mpm_autotune(hg_mpm *h, double dt, int32_t reps, int32_t *aw_out,
                                      int32_t *ah_out, double *us /* [4] or NULL */) {
@@ -2493,26 +2520,36 @@ extern "C" hg_status hg_mpm_autotune(hg_mpm *h, double dt, int32_t reps, int32_t
     if (fused) launch_step(h);
     else launch_p2g(h, kStoreVelAcc);
   };
+  // fused handles also try two points per cell per tile pass
+  int bp = 1;
+  const int npp = fused ? 2 : 1;
   for (int s = 0; s < 4 && st == HG_OK; ++s) {
     st = hg_mpm_set_tile(h, shapes[s][0], shapes[s][1]);
     if (st != HG_OK) break;
-    pass();  // warm-up
-    hipEventRecord(e0, h->stream);
-    for (int r = 0; r < reps; ++r) pass();
-    hipEventRecord(e1, h->stream);
-    if (hipEventSynchronize(e1) != hipSuccess) {
-      st = hg_fail(h->ctx, HG_ERR_HIP, "autotune timing failed");
-      break;
+    double shape_best = 1e300;
+    for (int pp = 1; pp <= npp && st == HG_OK; ++pp) {
+      h->ppass = pp;
+      pass();  // warm-up
+      hipEventRecord(e0, h->stream);
+      for (int r = 0; r < reps; ++r) pass();
+      hipEventRecord(e1, h->stream);
+      if (hipEventSynchronize(e1) != hipSuccess) {
+        st = hg_fail(h->ctx, HG_ERR_HIP, "autotune timing failed");
+        break;
+      }
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, e0, e1);
+      const double t = 1e3 * double(ms) / reps;
+      shape_best = std::min(shape_best, t);
+      if (t < best) {
+        best = t;
+        bi = s;
+        bp = pp;
+      }
     }
-    float ms = 0.f;
-    hipEventElapsedTime(&ms, e0, e1);
-    const double t = 1e3 * double(ms) / reps;
-    if (us) us[s] = t;
-    if (t < best) {
-      best = t;
-      bi = s;
-    }
+    if (us) us[s] = shape_best;
   }
+  h->ppass = bp;
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   h->prof.on = was_on;

@@ -13,6 +13,7 @@ struct hg_mpm {
   int precision = HG_FP64;
   int basis = HG_BASIS_UGIMP_LINEAR;
   int aw = 16, ah = 16;  // P2G tile of owned nodes (threads per workgroup = aw*ah)
+  int ppass = 1;         // FP32 fused step: points per tile pass, in threads per cell (1 or 2)
   std::vector<SceneDev> scenes;
   uint32_t nscenes = 0;
   uint32_t npoints = 0;

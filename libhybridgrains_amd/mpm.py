@@ -148,14 +148,25 @@ class GpuMpm:
         self.ctx.check(self.L.hg_mpm_sync(self._h))
 
     def autotune(self, dt: float, reps: int = 5) -> dict:
-        """Picks the fastest P2G tile shape for these scenes (results do not
-        depend on it); returns the choice and the timings."""
+        """Picks the fastest tile shape (and, in FP32, points per tile pass)
+        for these scenes (results do not depend on them); returns the choice
+        and the timings."""
         aw, ah = C.c_int32(), C.c_int32()
         us = np.zeros(4)
         self.ctx.check(self.L.hg_mpm_autotune(self._h, float(dt), int(reps), C.byref(aw),
                                               C.byref(ah), ptr(us)))
-        return {"tile": (aw.value, ah.value),
+        return {"tile": (aw.value, ah.value), "points_per_pass": self.tuning()["points_per_pass"],
                 "us": {"16x16": us[0], "16x8": us[1], "32x8": us[2], "64x4": us[3]}}
+
+    def set_points_per_pass(self, n: int) -> None:
+        """FP32: points per cell per tile pass of the fused step (1 or 2)."""
+        self.ctx.check(self.L.hg_mpm_set_points_per_pass(self._h, int(n)))
+
+    def tuning(self) -> dict:
+        """The tile shape and points per tile pass (threads per cell) in use."""
+        aw, ah, pp = C.c_int32(), C.c_int32(), C.c_int32()
+        self.ctx.check(self.L.hg_mpm_tuning(self._h, C.byref(aw), C.byref(ah), C.byref(pp)))
+        return {"tile": (aw.value, ah.value), "points_per_pass": pp.value}
 
     def scene_error(self, scene: int) -> int:
         f = C.c_int32()

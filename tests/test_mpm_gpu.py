@@ -200,6 +200,23 @@ def test_hold_occupies_the_stream(hg_ctx):
         g.hold(-1.0)
 
 
+@pytest.mark.parametrize("tile", [(16, 16), (32, 8), (64, 4)])
+def test_fp32_points_per_pass_change_no_result(hg_ctx, tile):
+    # one or two threads per cell take a tile's points in different passes and
+    # lanes; the exact node sums make the steps bit-identical
+    s = _small_scene(seed=11)
+    runs = []
+    for pp in (1, 2):
+        g = GpuMpm(s, HG_FP32, ctx=hg_ctx, tile=tile)
+        g.set_points_per_pass(pp)
+        assert g.tuning() == {"tile": tile, "points_per_pass": pp}
+        g.flow(DT, 4)
+        g.flow(DT, 4, graph=True)
+        runs.append(g.points())
+    for k in runs[0]:
+        assert np.array_equal(runs[0][k], runs[1][k]), k
+
+
 @pytest.mark.parametrize("prec", [HG_FP64, HG_FP32])
 def test_autotune_picks_a_shape_and_changes_no_result(hg_ctx, prec):
     # FP32 times whole fused steps and restores the saved state afterwards
