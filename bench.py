@@ -505,7 +505,7 @@ def _profile_mode(args):
         m, d, w = scenes.hybrid_1m()
         g = GpuHybrid(m, d, w, prec)
         cont = g.continuum
-        step = lambda n: g.step(dt, n, graph=True)  # noqa: E731
+        step = lambda n: g.step(dt, n, graph=not args.eager)  # noqa: E731
     if cont is not None:
         if args.keep_grid:
             cont.set_keep_grid(True)  # FP32: the separate P2G and G2P kernels
@@ -543,6 +543,9 @@ def main() -> int:
     ap.add_argument("--served-timeout", type=float, default=240.0)
     ap.add_argument("--tile", default="", help="P2G tile WxH for the profiling modes "
                                                "(default: autotune)")
+    ap.add_argument("--eager", action="store_true",
+                    help="hybrid profiling modes: eager steps (the grain branch on its own "
+                         "stream) instead of hipGraph replays")
     ap.add_argument("--keep-grid", action="store_true",
                     help="profiling modes: keep every grid field (FP32 then runs the separate "
                          "P2G and G2P kernels instead of the fused step)")

@@ -137,19 +137,34 @@ struct ContactArgs {
   const uint32_t *perm, *cell_start;
   int32_t *err;
   double *cpoint;  // optional: contact point per owned slot (circle pairs)
+  TableRaw tcur, tnext;  // bucket tables: read, and cleared for the integration
+  int32_t kslots;
 };
+
+// Files body i in bucket k of table t (and records where it went).
+__device__ __forceinline__ void table_insert(const TableRaw &t, int kslots, uint32_t k,
+                                             uint32_t i) {
+  t.bkey[i] = k;
+  const uint32_t s = atomicAdd(&t.count[k], 1u);
+  if (s < uint32_t(kslots)) t.items[size_t(k) * uint32_t(kslots) + s] = i;
+  else t.ovf[atomicAdd(t.novf, 1u)] = i;
+}
 
 // A team of kTeam lanes per body: the lanes split the body's candidates and
 // then its contacts, so that a body's chain of dependent loads is a fraction
 // of what one lane would walk.  Contacts are collected unordered in LDS and
 // each takes the slot of its rank among the body's keys, which is the order
 // of the partner keys, as a sorted list would have it.
-constexpr int kTeam = 8;
+// Lanes per body: 8 for the DEM-only tables, 4 for the sorted order of the
+// hybrid's grains (timed on the 20k column and the 1M hybrid scene).
+template <bool TABLE> constexpr int team_lanes() { return TABLE ? 8 : 4; }
 constexpr int kTeamThreads = 256;
-constexpr int kTeamBodies = kTeamThreads / kTeam;
+template <bool TABLE> constexpr int team_bodies() { return kTeamThreads / team_lanes<TABLE>(); }
 
-template <typename T, int CAP>
+template <typename T, int CAP, bool TABLE>
 __global__ __launch_bounds__(kTeamThreads) void dem_contacts_kernel(ContactArgs a) {
+  constexpr int kTeam = team_lanes<TABLE>();
+  constexpr int kTeamBodies = team_bodies<TABLE>();
   __shared__ uint32_t s_own[kTeamBodies][CAP];
   __shared__ uint32_t s_in[kTeamBodies][CAP];
   __shared__ int s_nown[kTeamBodies], s_nin[kTeamBodies];
@@ -162,13 +177,69 @@ __global__ __launch_bounds__(kTeamThreads) void dem_contacts_kernel(ContactArgs 
     s_nin[tb] = 0;
   }
   __syncthreads();
-  const uint32_t i = active ? a.perm[s] : 0u;
+  // teams take the bodies in cell order, or in creation order with the
+  // tables (no sort keeps an order then)
+  const uint32_t i = active ? (TABLE ? s : a.perm[s]) : 0u;
   const uint32_t sid = active ? scene_of(a.b_off, a.nscenes, i) : 0u;
   const SceneDev &sc = a.scenes[sid];
   const double xi = active ? a.b.x[i] : 0.0, yi = active ? a.b.y[i] : 0.0;
   const double ri = active ? a.b.r[i] : 0.0;
   const bool fi = active ? a.b.fixed[i] != 0 : true;
-  if (active) {
+  if constexpr (TABLE) {
+    // the table the integration files into next: every bucket it holds
+    // bodies in is some body's recorded bucket
+    if (active && lane == 0) a.tnext.count[a.tnext.bkey[i]] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.tnext.novf = 0u;
+  }
+  if (active && TABLE) {
+    const bool finite_i = isfinite(xi) && isfinite(yi);
+    int cx, cy;
+    dem_cell_of(sc, xi, yi, cx, cy);
+    const uint32_t wx = uint32_t(sc.ncx), wy = uint32_t(sc.ncy);
+    const uint32_t ks = uint32_t(a.kslots);
+    // the buckets of the 3x3 cells, then the overflow list
+    uint32_t rb[9], rn[9], tot = 0;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+      const uint32_t bk = sc.cell_off + (uint32_t(cx + r % 3 - 1) & (wx - 1u)) +
+                          wx * (uint32_t(cy + r / 3 - 1) & (wy - 1u));
+      rb[r] = bk * ks;
+      rn[r] = finite_i ? min(a.tcur.count[bk], ks) : 0u;
+      tot += rn[r];
+    }
+    const uint32_t nov = finite_i ? *a.tcur.novf : 0u;
+    for (uint32_t t = uint32_t(lane); t < tot + nov; t += uint32_t(kTeam)) {
+      uint32_t j;
+      if (t < tot) {
+        uint32_t off = t;
+        int r = 0;
+        while (off >= rn[r]) {
+          off -= rn[r];
+          ++r;
+        }
+        j = a.tcur.items[rb[r] + off];
+      } else {
+        j = a.tcur.ovf[t - tot];
+        if (j < sc.b_off || j >= sc.b_off + sc.b_cnt) continue;  // another scene's body
+      }
+      if (j == i) continue;
+      const bool fj = a.b.fixed[j] != 0;
+      if (fi && fj) continue;
+      const double dx = xi - a.b.x[j], dy = yi - a.b.y[j];
+      if (!isfinite(dx) || !isfinite(dy)) continue;
+      const double rs = ri + a.b.r[j];
+      const double c0 = (dx * dx + dy * dy) - rs * rs;
+      if (!(c0 <= 0.0)) continue;
+      if (!fi && (fj || i < j)) {
+        const int q = atomicAdd(&s_nown[tb], 1);
+        if (q < CAP) s_own[tb][q] = (fj ? kKinematic : 0u) | (j - sc.b_off);
+      } else if (!fi && !fj) {
+        const int q = atomicAdd(&s_nin[tb], 1);
+        if (q < CAP) s_in[tb][q] = j - sc.b_off;
+      }
+    }
+  }
+  if (active && !TABLE) {
     // A body whose state is no longer finite (flagged by the integration that
     // produced it) looks for no partners: every such body sorts into one
     // bucket, and scanning it would cost the square of their number.
@@ -277,16 +348,27 @@ __global__ __launch_bounds__(kTeamThreads) void dem_contacts_kernel(ContactArgs 
       vr1 = vyi + omi * r00 - kv1;
       w = wi;
     }
-    // spring history of this pair
+    // spring history of this pair: the body's previous keys in a few vector
+    // loads, then the matching slot's spring
     bool had = false;
     double o0 = 0.0, o1 = 0.0;
-    for (int q = 0; q < old_n; ++q)
-      if (a.old_s.okey[obase + q] == key) {
-        had = true;
-        o0 = double(ld<T>(a.old_s.ods, 2 * (obase + q)));
-        o1 = double(ld<T>(a.old_s.ods, 2 * (obase + q) + 1));
-        break;
+    {
+      const uint4 *kv = reinterpret_cast<const uint4 *>(a.old_s.okey + obase);
+      int hit = -1;
+#pragma unroll
+      for (int q4 = 0; q4 < CAP / 4; ++q4) {
+        const uint4 v = kv[q4];
+        const uint32_t kk[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (hit < 0 && 4 * q4 + e < old_n && kk[e] == key) hit = 4 * q4 + e;
       }
+      if (hit >= 0) {
+        had = true;
+        o0 = double(ld<T>(a.old_s.ods, 2 * (obase + uint32_t(hit))));
+        o1 = double(ld<T>(a.old_s.ods, 2 * (obase + uint32_t(hit)) + 1));
+      }
+    }
     double ds0, ds1, F0, F1;
     spring(had, o0, o1, dt, vr0, vr1, n0, n1, ds0, ds1);
     penalty_force(sc, w, depth, n0, n1, vr0, vr1, ds0, ds1, F0, F1);
@@ -350,6 +432,9 @@ struct IntegrateArgs {
   int32_t *err;
   double *F_out;  // optional: total force per body (3B)
   int32_t rekey;  // produce the next step's cell keys
+  int32_t table;  // ... by filing the bodies in tnext (else keys and slots for the sort)
+  TableRaw tnext;
+  int32_t kslots;
 };
 
 // Gathers a body's forces in the reference's summation order, then
@@ -432,6 +517,10 @@ __global__ __launch_bounds__(kThreads) void dem_integrate_kernel(IntegrateArgs a
   if (!(isfinite(x) && isfinite(y) && isfinite(v2))) atomicOr(&a.err[sid], 2);
   if (!a.rekey) return;
   const uint32_t k = dem_bucket(sc, x, y);
+  if (a.table) {
+    table_insert(a.tnext, a.kslots, k, i);
+    return;
+  }
   a.key[i] = k;
   a.slot[i] = atomicAdd(&a.counts[k], 1u);
   hg_scan_block_count(a.totals, k, blockIdx.x);
@@ -443,6 +532,9 @@ struct BinArgs {
   uint32_t nscenes, nbodies;
   const double *x, *y;
   uint32_t *key, *slot, *counts, *totals;
+  int32_t table;  // file the bodies in t instead
+  TableRaw t;
+  int32_t kslots;
 };
 
 __global__ __launch_bounds__(kThreads) void dem_bin_kernel(BinArgs a) {
@@ -450,6 +542,10 @@ __global__ __launch_bounds__(kThreads) void dem_bin_kernel(BinArgs a) {
   if (i >= a.nbodies) return;
   const SceneDev &sc = a.scenes[scene_of(a.b_off, a.nscenes, i)];
   const uint32_t k = dem_bucket(sc, a.x[i], a.y[i]);
+  if (a.table) {
+    table_insert(a.t, a.kslots, k, i);
+    return;
+  }
   a.key[i] = k;
   a.slot[i] = atomicAdd(&a.counts[k], 1u);
   hg_scan_block_count(a.totals, k, blockIdx.x);
@@ -489,7 +585,8 @@ static void launch_contacts_t(hg_dem *h) {
   a.b_off = h->d_b_off;
   a.nscenes = h->nscenes;
   a.nbodies = h->nbodies;
-  a.nblocks_padded = hg_pad8((h->nbodies + kTeamBodies - 1) / kTeamBodies);
+  const uint32_t per_block = h->table ? team_bodies<true>() : team_bodies<false>();
+  a.nblocks_padded = hg_pad8((h->nbodies + per_block - 1) / per_block);
   a.b = h->b;
   a.old_s = h->slots[h->cur];
   a.new_s = h->slots[1 - h->cur];
@@ -498,11 +595,20 @@ static void launch_contacts_t(hg_dem *h) {
   a.cell_start = h->cell_start;
   a.err = h->err;
   a.cpoint = h->cpoint;
+  a.tcur = h->tables[h->cur];
+  a.tnext = h->tables[1 - h->cur];
+  a.kslots = h->kslots;
   ProfScope ps(h->prof, "dem_contacts", h->stream);
   const dim3 g(a.nblocks_padded), b(kTeamThreads);
-  if (h->cap <= 8) hipLaunchKernelGGL((dem_contacts_kernel<T, 8>), g, b, 0, h->stream, a);
-  else if (h->cap <= 16) hipLaunchKernelGGL((dem_contacts_kernel<T, 16>), g, b, 0, h->stream, a);
-  else hipLaunchKernelGGL((dem_contacts_kernel<T, 32>), g, b, 0, h->stream, a);
+  if (h->table) {
+    if (h->cap <= 8) hipLaunchKernelGGL((dem_contacts_kernel<T, 8, true>), g, b, 0, h->stream, a);
+    else if (h->cap <= 16) hipLaunchKernelGGL((dem_contacts_kernel<T, 16, true>), g, b, 0, h->stream, a);
+    else hipLaunchKernelGGL((dem_contacts_kernel<T, 32, true>), g, b, 0, h->stream, a);
+  } else {
+    if (h->cap <= 8) hipLaunchKernelGGL((dem_contacts_kernel<T, 8, false>), g, b, 0, h->stream, a);
+    else if (h->cap <= 16) hipLaunchKernelGGL((dem_contacts_kernel<T, 16, false>), g, b, 0, h->stream, a);
+    else hipLaunchKernelGGL((dem_contacts_kernel<T, 32, false>), g, b, 0, h->stream, a);
+  }
 }
 
 template <typename T>
@@ -523,6 +629,9 @@ static void launch_integrate_t(hg_dem *h, bool rekey) {
   a.err = h->err;
   a.F_out = h->keep_force ? h->F_out : nullptr;
   a.rekey = rekey ? 1 : 0;
+  a.table = h->table ? 1 : 0;
+  a.tnext = h->tables[1 - h->cur];
+  a.kslots = h->kslots;
   ProfScope ps(h->prof, "dem_integrate", h->stream);
   const dim3 g((h->nbodies + kThreads - 1) / kThreads), b(kThreads);
   if (h->cap <= 8) hipLaunchKernelGGL((dem_integrate_kernel<T, 8>), g, b, 0, h->stream, a);
@@ -541,10 +650,33 @@ void hg_dem_launch_step(hg_dem *h, bool rekey) {
     }
   }
   h->cur = 1 - h->cur;
-  if (rekey) hg_dem_launch_sort(h);
+  if (rekey && !h->table) hg_dem_launch_sort(h);
 }
 
 void hg_dem_launch_rebin(hg_dem *h) {
+  if (h->table) {
+    // both tables emptied, the current one filled from the positions
+    for (int c = 0; c < 2; ++c) {
+      hipMemsetAsync(h->tables[c].count, 0, size_t(h->ncell) * 4, h->stream);
+      hipMemsetAsync(h->tables[c].novf, 0, 4, h->stream);
+    }
+    if (h->nbodies) {
+      BinArgs a;
+      a.scenes = h->d_scenes;
+      a.b_off = h->d_b_off;
+      a.nscenes = h->nscenes;
+      a.nbodies = h->nbodies;
+      a.x = h->b.x;
+      a.y = h->b.y;
+      a.table = 1;
+      a.t = h->tables[h->cur];
+      a.kslots = h->kslots;
+      ProfScope ps(h->prof, "dem_bin", h->stream);
+      hipLaunchKernelGGL(dem_bin_kernel, dim3((h->nbodies + kThreads - 1) / kThreads),
+                         dim3(kThreads), 0, h->stream, a);
+    }
+    return;
+  }
   uint32_t *totals = h->scan_totals + size_t(h->cur) * hg_scan_totals_words(h->nscan);
   hg_zero_words(h->stream, totals, hg_scan_totals_words(h->nscan));
   if (h->nbodies) {
@@ -559,6 +691,7 @@ void hg_dem_launch_rebin(hg_dem *h) {
     a.slot = h->slot;
     a.counts = h->counts;
     a.totals = totals;
+    a.table = 0;
     ProfScope ps(h->prof, "dem_bin", h->stream);
     hipLaunchKernelGGL(dem_bin_kernel, dim3((h->nbodies + kThreads - 1) / kThreads), dim3(kThreads),
                        0, h->stream, a);
@@ -574,6 +707,19 @@ void hg_dem_launch_sort(hg_dem *h) {
 }
 
 static void launch_step(hg_dem *h) { hg_dem_launch_step(h, true); }
+
+hg_status hg_dem_use_sorted_order(hg_dem *h) {
+  if (!h->table) return HG_OK;
+  h->table = false;
+  if (h->graph) {
+    hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  hg_dem_launch_rebin(h);
+  HG_HIP(h->ctx, hipGetLastError());
+  HG_HIP(h->ctx, hipStreamSynchronize(h->stream));
+  return HG_OK;
+}
 
 static hg_status dem_upload_dt(hg_dem *h, double dt) {
   if (dt == h->dt_dev) return HG_OK;
@@ -727,6 +873,30 @@ static void dem_plan_grid(const hg_dem_desc &d, SceneDev &sd) {
   sd.ncy = int32_t(wy);
 }
 
+// Item slots per bucket of the DEM-only tables: a cell at least one largest
+// diameter wide holds at most about (cw / (2 r_min) + 1)^2 centres of
+// non-overlapping circles; half as many again for overlaps, rounded up to a
+// power of two in [8, 64].  A body that finds its bucket full goes to the
+// overflow list, so the figure sets only speed, never results.
+static int dem_table_slots(const hg_dem_desc *descs, uint32_t nscenes) {
+  double worst = 1.0;
+  for (uint32_t s = 0; s < nscenes; ++s) {
+    const hg_dem_desc &d = descs[s];
+    if (d.nbodies == 0) continue;
+    SceneDev sd;
+    std::memset(&sd, 0, sizeof(sd));
+    dem_plan_grid(d, sd);
+    double rmin = INFINITY;
+    for (uint32_t k = 0; k < d.nbodies; ++k) rmin = std::min(rmin, d.radius[k]);
+    if (!(rmin > 0.0)) continue;
+    const double n = sd.cw / (2.0 * rmin) + 1.0;
+    worst = std::max(worst, 1.5 * n * n);
+  }
+  int k = 8;
+  while (k < 64 && double(k) < worst) k *= 2;
+  return k;
+}
+
 static hg_status dem_create_impl(hg_ctx *ctx, const hg_dem_desc *descs, uint32_t nscenes,
                                  hg_dem **out) {
   if (!ctx || !descs || !out || nscenes == 0) return hg_fail(ctx, HG_ERR_INVALID, "null argument");
@@ -803,6 +973,18 @@ static hg_status dem_create_impl(hg_ctx *ctx, const hg_dem_desc *descs, uint32_t
   HG_TRY(dalloc(hp, &h->scan_totals, size_t(hg_scan_totals_words(h->nscan)) * 8));
   HG_TRY(dalloc(hp, &h->err, nscenes * 4));
   HG_TRY(dalloc(hp, &h->F_out, size_t(h->nbodies) * 24));
+  h->kslots = dem_table_slots(descs, nscenes);
+  for (int c = 0; c < 2; ++c) {
+    TableRaw &t = h->tables[c];
+    HG_TRY(dalloc(hp, &t.count, size_t(h->ncell) * 4));
+    HG_TRY(dalloc(hp, &t.items, size_t(h->ncell) * size_t(h->kslots) * 4));
+    HG_TRY(dalloc(hp, &t.bkey, size_t(h->nbodies) * 4));
+    HG_TRY(dalloc(hp, &t.novf, 4));
+    HG_TRY(dalloc(hp, &t.ovf, size_t(h->nbodies) * 4));
+    HG_HIP(ctx, hipMemset(t.count, 0, size_t(h->ncell) * 4));
+    HG_HIP(ctx, hipMemset(t.bkey, 0, size_t(h->nbodies ? h->nbodies : 1) * 4));
+    HG_HIP(ctx, hipMemset(t.novf, 0, 4));
+  }
   HG_HIP(ctx, hipMemcpy(h->d_scenes, h->scenes.data(), nscenes * sizeof(SceneDev), hipMemcpyHostToDevice));
   HG_HIP(ctx, hipMemcpy(h->d_b_off, h_b_off.data(), (nscenes + 1) * 4, hipMemcpyHostToDevice));
   HG_HIP(ctx, hipMemset(h->counts, 0, size_t(h->ncell) * 4));
@@ -1199,8 +1381,11 @@ extern "C" hg_status hg_dem_estimate_bytes(const hg_dem_desc *descs, uint32_t ns
   }
   const size_t T = descs[0].precision == HG_FP64 ? 8 : 4;
   const uint64_t nscan = (cells + kScanChunk - 1) / kScanChunk;
+  // the two bucket tables: counts and item slots per bucket, bucket and
+  // overflow entry per body
+  const uint64_t tables = 2 * (cells * (uint64_t(dem_table_slots(descs, nscenes)) + 1) * 4 + B * 8 + 4);
   *bytes = B * dem_bytes_per_body(T, hg_dem_contact_capacity(descs, nscenes)) + cells * 8 + 4 + nscan * 8 * kScanSubTotals +
-           uint64_t(nscenes) * (sizeof(SceneDev) + 16) + (1u << 20);
+           tables + uint64_t(nscenes) * (sizeof(SceneDev) + 16) + (1u << 20);
   return HG_OK;
 }
 

@@ -58,6 +58,20 @@ struct ForceRaw {
   uint8_t *icnt;
 };
 
+// The broad-phase table of DEM-only steps: each bucket keeps up to kslots
+// body indices in the order the integration's atomics file them; a body that
+// finds its bucket full goes to an overflow list that every body also scans
+// (empty in practice), so no candidate is ever lost.  Two tables alternate
+// with the contact slots: the step that starts with slots[c] current reads
+// table c and files the moved bodies into the other, which it found cleared.
+struct TableRaw {
+  uint32_t *count;  // [ncell] bodies filed in each bucket (may exceed kslots)
+  uint32_t *items;  // [ncell][kslots]
+  uint32_t *bkey;   // [B] the bucket each body was filed in
+  uint32_t *novf;   // bodies that did not fit
+  uint32_t *ovf;    // [B] their indices
+};
+
 // Broad-phase cell of a position (unbounded; clamped to +-2^30 so that any
 // finite or non-finite coordinate gives a valid integer).
 __device__ __forceinline__ void dem_cell_of(const SceneDev &sc, double x, double y, int &cx,
@@ -99,6 +113,11 @@ struct hg_dem {
   ForceRaw f = {};
   uint32_t *perm = nullptr, *key = nullptr, *slot = nullptr, *counts = nullptr,
            *cell_start = nullptr;
+  // DEM-only handles find candidates in bucket tables that the integration
+  // fills (no sort per step); the hybrid's grains use the sorted order
+  bool table = true;
+  int kslots = 8;
+  hgdem::TableRaw tables[2] = {};
   // per-scan-block totals of the bucket keys that the sort after a step
   // with slots[c] current reads, at scan_totals + c * hg_scan_totals_words(nscan)
   uint32_t *scan_totals = nullptr;
@@ -125,5 +144,8 @@ int hg_dem_contact_capacity(const hg_dem_desc *descs, uint32_t nscenes);
 // hg_dem_launch_rebin after editing positions.
 void hg_dem_launch_step(hg_dem *h, bool rekey);
 void hg_dem_launch_rebin(hg_dem *h);
+// The hybrid's grains: the counting sort of every step instead of the
+// bucket tables (re-bins the current positions).
+hg_status hg_dem_use_sorted_order(hg_dem *h);
 // The counting sort alone, for a caller that wrote the cell keys and slots.
 void hg_dem_launch_sort(hg_dem *h);

@@ -673,6 +673,12 @@ extern "C" hg_status hg_hybrid_create(hg_ctx *ctx, const hg_mpm_desc *mpm_desc,
     hg_mpm_destroy(h->mpm);
     return st;
   }
+  // the grains' keys come from the finish kernel, into the sorted order
+  if ((st = hg_dem_use_sorted_order(h->dem)) != HG_OK) {
+    hg_dem_destroy(h->dem);
+    hg_mpm_destroy(h->mpm);
+    return st;
+  }
   // one stream for both: the phases interleave
   HG_HIP(ctx, hipStreamSynchronize(h->dem->stream));
   HG_HIP(ctx, hipStreamDestroy(h->dem->stream));
