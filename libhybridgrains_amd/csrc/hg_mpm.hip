@@ -696,6 +696,29 @@ struct G2PArgs {
   int32_t *err;
 };
 
+// FP64 reciprocal square root, square root and reciprocal for the FP32 mode's
+// strain update, from the hardware estimates (about 23 correct bits) and two
+// Newton-Raphson steps each, which take them to within a few ulps of the
+// correctly rounded results: far below what the FP32 strain storage keeps,
+// and without the range-reduction and rounding-fixup sequences of the
+// correctly rounded forms.  Inputs are positive and normal in every use;
+// anything else yields a non-finite result that the point checks flag.
+__device__ __forceinline__ double rsq_nr(double a) {
+  double y = __builtin_amdgcn_rsq(a);
+  y = fma(0.5 * y, fma(-(a * y), y, 1.0), y);
+  return fma(0.5 * y, fma(-(a * y), y, 1.0), y);
+}
+// sqrt(a) given y = rsq_nr(a): a y, corrected once by its residual.
+__device__ __forceinline__ double sqrt_rsq(double a, double y) {
+  const double s = a * y;
+  return fma(fma(-s, s, a), 0.5 * y, s);
+}
+__device__ __forceinline__ double rcp_nr(double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = fma(fma(-b, r, 1.0), r, r);
+  return fma(fma(-b, r, 1.0), r, r);
+}
+
 // The particle half of the step for one point: gathers the node (v, a) of
 // its stencil (fetch(j, i) gives node (bx + i, by + j)), then
 // MaterialPoints::computeVelGrad (MaterialPoints.cpp:296-322),
@@ -780,19 +803,52 @@ __device__ __forceinline__ void g2p_point(const SceneDev &sc, const Fetch &fetch
   const double e00 = n00 + dt * (Lb00 + bL00);
   const double e01 = n01 + dt * (Lb10 + bL10);
   const double e11 = n11 + dt * (Lb11 + bL11);
-  // FP32 mode stores J and b_bar in FP32, so it divides by a shared
-  // reciprocal (a last-bit FP64 difference); FP64 mode divides as written.
   constexpr bool f32 = std::is_same<T, float>::value;
-  double detE = e00 * e11 - e01 * e01;
-  J1 = sqrt(detE);
-  double nrm = sqrt(detE);
-  const double inrm = f32 ? 1.0 / nrm : 0.0;
-  c00 = f32 ? e00 * inrm : e00 / nrm;
-  c01 = f32 ? e01 * inrm : e01 / nrm;
-  c11 = f32 ? e11 * inrm : e11 / nrm;
+  if constexpr (f32) {
+    // FP32 mode (J and b_bar stored in FP32): the same return in FP64
+    // arithmetic, on b* itself.  The FP64 mode's round trip through b_bar
+    // and back (s = b_bar J, whose determinant is det b* again) is skipped,
+    // the yield test is taken on squares (mu |q| > ys with q = dev b* / J
+    // becomes mu^2 |dev b*|^2 > ys^2 det b*, both sides non-negative), and
+    // square roots and quotients come from rsq_nr / sqrt_rsq / rcp_nr.
+    const double detE = e00 * e11 - e01 * e01;
+    const double yE = rsq_nr(detE);
+    const double JE = sqrt_rsq(detE, yE);
+    double ys = 0.5 * sc.dp_alpha * sc.kappa * (1.0 + JE) * (1.0 - JE);
+    if (ys < 0.0) ys = 0.0;
+    const double half_tr = 0.5 * (e00 + e11);
+    const double d00 = e00 - half_tr, d01 = e01, d11 = e11 - half_tr;
+    const double dd = d00 * d00 + d01 * d01 + d01 * d01 + d11 * d11;
+    if (sc.mu * sc.mu * dd > ys * ys * detE) {
+      const double fn = sqrt_rsq(dd, rsq_nr(dd)) * yE;
+      const double lambda2 = ys * rcp_nr(sc.mu * fn);
+      const double detD = d00 * d11 - d01 * d01;
+      const double l1sq = detE - lambda2 * lambda2 * detD;
+      const double lambda1 = sqrt_rsq(l1sq, rsq_nr(l1sq));
+      const double r00 = lambda1 + lambda2 * d00;
+      const double r01 = lambda2 * d01;
+      const double r11 = lambda1 + lambda2 * d11;
+      const double detR = r00 * r11 - r01 * r01;
+      const double yR = rsq_nr(detR);
+      J1 = sqrt_rsq(detR, yR);
+      c00 = r00 * yR;
+      c01 = r01 * yR;
+      c11 = r11 * yR;
+    } else {
+      J1 = JE;
+      c00 = e00 * yE;
+      c01 = e01 * yE;
+      c11 = e11 * yE;
+    }
+  } else {
+    double detE = e00 * e11 - e01 * e01;
+    J1 = sqrt(detE);
+    double nrm = sqrt(detE);
+    c00 = e00 / nrm;
+    c01 = e01 / nrm;
+    c11 = e11 / nrm;
 
-  // Drucker-Prager return on b* = J b_bar
-  {
+    // Drucker-Prager return on b* = J b_bar
     const double s00 = c00 * J1, s01 = c01 * J1, s11 = c11 * J1;
     const double detS = s00 * s11 - s01 * s01;
     const double Jr = sqrt(detS);
@@ -800,10 +856,9 @@ __device__ __forceinline__ void g2p_point(const SceneDev &sc, const Fetch &fetch
     if (ys < 0.0) ys = 0.0;
     const double half_tr = 0.5 * (s00 + s11);
     const double d00 = s00 - half_tr, d01 = s01 - 0.0, d11 = s11 - half_tr;
-    const double iJr = f32 ? 1.0 / Jr : 0.0;
-    const double q00 = f32 ? d00 * iJr : d00 / Jr;
-    const double q01 = f32 ? d01 * iJr : d01 / Jr;
-    const double q11 = f32 ? d11 * iJr : d11 / Jr;
+    const double q00 = d00 / Jr;
+    const double q01 = d01 / Jr;
+    const double q11 = d11 / Jr;
     const double fn = sqrt(q00 * q00 + q01 * q01 + q01 * q01 + q11 * q11);
     double r00 = s00, r01 = s01, r11 = s11;
     if (sc.mu * fn > ys) {
@@ -818,10 +873,9 @@ __device__ __forceinline__ void g2p_point(const SceneDev &sc, const Fetch &fetch
     const double detR = r00 * r11 - r01 * r01;
     J1 = sqrt(detR);
     const double rn = sqrt(detR);
-    const double irn = f32 ? 1.0 / rn : 0.0;
-    c00 = f32 ? r00 * irn : r00 / rn;
-    c01 = f32 ? r01 * irn : r01 / rn;
-    c11 = f32 ? r11 * irn : r11 / rn;
+    c00 = r00 / rn;
+    c01 = r01 / rn;
+    c11 = r11 / rn;
   }
 
   // PIC/FLIP velocity and advection
@@ -998,7 +1052,7 @@ __device__ __forceinline__ unsigned long long fx_of(float scaled) {
 // A point's rasterisation terms in FP32: per axis the S weights and masked
 // weight-derivative factors of its stencil (base node bx, by; cntx, cnty
 // stencil nodes per axis), and its mass, momentum and -V J sigma.  The stress
-// is formed in FP64, as in P2G.
+// is formed in FP64 (roots from rsq_nr / sqrt_rsq).
 template <int BASIS>
 struct P2GTerms {
   static constexpr int S = BasisTraits<BASIS>::S;
@@ -1018,8 +1072,8 @@ __device__ __forceinline__ void p2g_terms(const SceneDev &sc, double x, double y
   if (J <= 1.0) {
     const double e00 = b00 * J, e01 = b01 * J, e11 = b11 * J;
     const double det = e00 * e11 - e01 * e01;
-    const double Jt = sqrt(det);
-    const double inrm = 1.0 / Jt;
+    const double inrm = rsq_nr(det);
+    const double Jt = sqrt_rsq(det, inrm);
     const double bb00 = e00 * inrm, bb01 = e01 * inrm, bb11 = e11 * inrm;
     const double half_tr = (bb00 + bb11) * 0.5;
     const double c = sc.kappa * 0.5 * (Jt + 1.0) * (Jt - 1.0);
@@ -1027,23 +1081,32 @@ __device__ __forceinline__ void p2g_terms(const SceneDev &sc, double x, double y
     t01 = sc.mu * bb01;
     t11 = c + sc.mu * (bb11 - half_tr);
   }
-  const double iJ = 1.0 / J;
-  const float c = float(-double(V) * J);
+  // -V J sigma with sigma = tau / J is -V tau
+  const double nV = -double(V);
   const int sf = sc.fx_shift[kFxF0], sp = sc.fx_shift[kFxMom0];
-  o.f00 = ldexpf(c * float(t00 * iJ), sf);
-  o.f01 = ldexpf(c * float(t01 * iJ), sf);
-  o.f11 = ldexpf(c * float(t11 * iJ), sf);
+  o.f00 = ldexpf(float(nV * t00), sf);
+  o.f01 = ldexpf(float(nV * t01), sf);
+  o.f11 = ldexpf(float(nV * t11), sf);
   o.m = ldexpf(m, sc.fx_shift[kFxMass]);
   o.mv0 = ldexpf(m * v0, sp);
   o.mv1 = ldexpf(m * v1, sp);
   const float inv_h = float(sc.inv_h);
   float inv_vp = 1.0f;
   if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) inv_vp = 1.0f / (4.0f * hl * hl);
+  float fax[S], fdx[S], fay[S], fdy[S];
+  if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) {
+    ugimp_1d_f32(x, hl, sc.min0, sc.h, inv_h, bx, cntx, fax, fdx);
+    ugimp_1d_f32(y, hl, sc.min1, sc.h, inv_h, by, cnty, fay, fdy);
+  } else {
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      basis_factor<float, BASIS>(x, hl, sc.min0, sc.h, inv_h, bx + k, k, cntx, fax[k], fdx[k]);
+      basis_factor<float, BASIS>(y, hl, sc.min1, sc.h, inv_h, by + k, k, cnty, fay[k], fdy[k]);
+    }
+  }
 #pragma unroll
   for (int k = 0; k < S; ++k) {
-    float ax, dx, ay, dy;
-    basis_factor<float, BASIS>(x, hl, sc.min0, sc.h, inv_h, bx + k, k, cntx, ax, dx);
-    basis_factor<float, BASIS>(y, hl, sc.min1, sc.h, inv_h, by + k, k, cnty, ay, dy);
+    const float ax = fax[k], dx = fdx[k], ay = fay[k], dy = fdy[k];
     float wxk, gxk, gyk;
     if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) {
       wxk = ax * inv_vp;
@@ -1114,7 +1177,10 @@ __device__ __forceinline__ void fused_strain(FusedPoint &q) {
   q.c11 = strain_load<float>(q.s11);
 }
 
-__device__ __forceinline__ FusedPoint fused_load(const PtsRaw &p, uint32_t i, const SceneDev &sc) {
+// The stored fields only (the strain as stored), so that a load can be issued
+// a pass ahead of its use without holding the FP64 strain in registers.
+__device__ __forceinline__ FusedPoint fused_load_raw(const PtsRaw &p, uint32_t i,
+                                                     const SceneDev &sc) {
   FusedPoint q;
   q.x = p.x[i];
   q.y = p.y[i];
@@ -1124,11 +1190,26 @@ __device__ __forceinline__ FusedPoint fused_load(const PtsRaw &p, uint32_t i, co
   q.s00 = ld<float>(p.b00, i);
   q.s01 = ld<float>(p.b01, i);
   q.s11 = ld<float>(p.b11, i);
-  fused_strain(q);
   q.m = ld<float>(p.m, i);
   q.V = ld<float>(p.V, i);
   q.hl = p.hl ? ld<float>(p.hl, i) : float(sc.hl_uniform);
   return q;
+}
+
+__device__ __forceinline__ FusedPoint fused_load(const PtsRaw &p, uint32_t i, const SceneDev &sc) {
+  FusedPoint q = fused_load_raw(p, i, sc);
+  fused_strain(q);
+  return q;
+}
+
+// A rasterised point's stencil in the tile's window, packed into the stash:
+// base node offsets from the window origin (8 bits each; the window is at most
+// 64 + 5 nodes wide) and the stencil's node counts less one (2 bits each).
+// kStashNone marks a lane with no point, kStashFar a point whose stencil
+// leaves the window.
+constexpr uint32_t kStashNone = 0xffffffffu, kStashFar = 0xfffffffeu;
+__device__ __forceinline__ uint32_t stash_key(int dx, int dy, int cx, int cy) {
+  return uint32_t(dx) | (uint32_t(dy) << 8) | (uint32_t(cx - 1) << 16) | (uint32_t(cy - 1) << 18);
 }
 
 // Points of a tile that moved more than one cell since the sort, whose terms
@@ -1148,7 +1229,7 @@ __device__ __forceinline__ void lds_barrier() {
 }
 
 template <int BASIS, int TW, int TH, bool PRIME, int MULT>
-__global__ __launch_bounds__((fused_threads<TW, TH, MULT>())) void mpm_fused_kernel(FusedArgs a) {
+__global__ __launch_bounds__((fused_threads<TW, TH, MULT>())) __attribute__((amdgpu_waves_per_eu(4))) void mpm_fused_kernel(FusedArgs a) {
   constexpr int S = BasisTraits<BASIS>::S;
   constexpr int NT = fused_threads<TW, TH, MULT>();
   constexpr int GW = TW + S - 1, GH = TH + S - 1;  // G2P node window
@@ -1162,7 +1243,8 @@ __global__ __launch_bounds__((fused_threads<TW, TH, MULT>())) void mpm_fused_ker
   // one pass's points after G2P (or as loaded, priming), for the terms
   __shared__ double s_px[NT], s_py[NT];
   __shared__ float s_pf[9][NT];
-  __shared__ uint32_t s_pi[NT];
+  __shared__ uint32_t s_pi[NT], s_pk[NT];
+  static_assert(TW + S + 1 <= 256 && TH + S + 1 <= 256 && S <= 4, "stash key fields");
 
   const uint32_t ntile_list = *a.tile_count;
   const int t = threadIdx.x;
@@ -1225,28 +1307,61 @@ __global__ __launch_bounds__((fused_threads<TW, TH, MULT>())) void mpm_fused_ker
     constexpr int NW = NT / 64;
     static_assert(NT % 64 == 0, "whole waves per tile");
     const uint32_t lane_pt = uint32_t((t & 63) * NW + (t >> 6));
+    // position in the current order of the tile's k-th point
+    auto order_pos = [&](uint32_t k) {
+      int r = 0;
+#pragma unroll
+      for (int q = 1; q < TH; ++q) r += (s_ro[q] <= k) ? 1 : 0;
+      return s_rb[r] + (k - s_ro[r]);
+    };
+    // Loads run ahead of their use: a pass's point fields are requested
+    // before the raster half of the pass before it, and their buffer indices
+    // (perm) one pass earlier still.
+    uint32_t sp = 0, p = 0, sp_n = 0, p_n = 0, id = 0;
+    bool scripted = false;
+    FusedPoint q;
+    if (uint32_t(t) < total) {
+      sp = order_pos(uint32_t(t));
+      p = a.perm[sp];
+    }
+    if (uint32_t(NT + t) < total) {
+      sp_n = order_pos(uint32_t(NT + t));
+      p_n = a.perm[sp_n];
+    }
+    if (uint32_t(t) < total) {
+      q = fused_load_raw(a.src, p, sc);
+      if constexpr (!PRIME) {
+        id = a.src.id[p];
+        scripted = a.src.flags ? (a.src.flags[p] != 0) : false;
+      }
+    }
     for (uint32_t k0 = 0; k0 < total; k0 += uint32_t(NT)) {
       const uint32_t k = k0 + uint32_t(t);
-      uint32_t tag = 0xffffffffu;  // no point
+      uint32_t tag = kStashNone, pk = kStashNone;
+      uint32_t slot_v = 0, slot_at = 0xffffffffu;
       if (k < total) {
-        int r = 0;
-#pragma unroll
-        for (int q = 1; q < TH; ++q) r += (s_ro[q] <= k) ? 1 : 0;
-        const uint32_t sp = s_rb[r] + (k - s_ro[r]);  // position in the current order
-        const uint32_t p = a.perm[sp];
-        FusedPoint q = fused_load(a.src, p, sc);
+        fused_strain(q);
+        int bx, by, cx, cy;
         if constexpr (!PRIME) {
-          int bx, by, cx, cy;
           point_key<BASIS>(sc, q.x, q.y, double(q.hl), bx, by, cx, cy);
-          const bool scripted = a.src.flags ? (a.src.flags[p] != 0) : false;
           const bool inwin = bx >= x0 && bx < x0 + TW && by >= y0 && by < y0 + TH;
-          const float4 *g = a.gva + sc.node_off;
-          auto fetch = [&](int j, int ii) {
-            return inwin ? s_va[(by - y0 + j) * GW + (bx - x0 + ii)]
-                         : g[uint32_t(by + j) * uint32_t(sc.snx) + uint32_t(bx + ii)];
-          };
-          g2p_point<float, BASIS>(sc, fetch, bx, by, cx, cy, q.hl, scripted, q.x, q.y, q.v0, q.v1,
-                                  q.J, q.c00, q.c01, q.c11);
+          // the points of the tile's cells gather from the LDS window; a
+          // wave with a point whose stencil is elsewhere (moved since its
+          // sort) takes the general form
+          if (__all(inwin)) {
+            const float4 *w = s_va + ((by - y0) * GW + (bx - x0));
+            auto fetch = [&](int j, int ii) { return w[j * GW + ii]; };
+            g2p_point<float, BASIS>(sc, fetch, bx, by, cx, cy, q.hl, scripted, q.x, q.y, q.v0,
+                                    q.v1, q.J, q.c00, q.c01, q.c11);
+          } else {
+            const float4 *g = a.gva + sc.node_off;
+            auto fetch = [&](int j, int ii) {
+              return inwin ? s_va[(by - y0 + j) * GW + (bx - x0 + ii)]
+                           : g[uint32_t(by + j) * uint32_t(sc.snx) + uint32_t(bx + ii)];
+            };
+            g2p_point<float, BASIS>(sc, fetch, bx, by, cx, cy, q.hl, scripted, q.x, q.y, q.v0,
+                                    q.v1, q.J, q.c00, q.c01, q.c11);
+          }
           // the point's one write, at its position in the current order; the
           // next step's terms use the stored (rounded) strain, as a
           // rasterisation of the stored state (after a restore) does
@@ -1264,7 +1379,7 @@ __global__ __launch_bounds__((fused_threads<TW, TH, MULT>())) void mpm_fused_ker
           st<float>(a.dst.J, sp, q.sJ);
           st<float>(a.dst.m, sp, q.m);
           st<float>(a.dst.V, sp, q.V);
-          a.dst.id[sp] = a.src.id[p];
+          a.dst.id[sp] = id;
           if (a.src.hl) st<float>(a.dst.hl, sp, q.hl);
           if (a.src.flags) a.dst.flags[sp] = scripted ? 1 : 0;
           const bool ok = point_key<BASIS>(sc, q.x, q.y, double(q.hl), bx, by, cx, cy);
@@ -1273,8 +1388,13 @@ __global__ __launch_bounds__((fused_threads<TW, TH, MULT>())) void mpm_fused_ker
           if (!ok || !finite) atomicOr(&a.err[sid], (ok ? 0 : 1) | (finite ? 0 : 2));
           const uint32_t key = sc.cell_off + uint32_t(by) * uint32_t(sc.snx) + uint32_t(bx);
           a.key[sp] = key;
-          a.slot[sp] = hg_run_slot(a.counts, key);
+          // the slot's atomic returns while the raster half runs; it is
+          // stored after it
+          slot_v = hg_run_slot(a.counts, key);
+          slot_at = sp;
           hg_scan_block_count(a.totals, key, blockIdx.x);
+        } else {
+          point_key<BASIS>(sc, q.x, q.y, double(q.hl), bx, by, cx, cy);
         }
         s_px[t] = q.x;
         s_py[t] = q.y;
@@ -1288,43 +1408,65 @@ __global__ __launch_bounds__((fused_threads<TW, TH, MULT>())) void mpm_fused_ker
         s_pf[7][t] = q.s01;
         s_pf[8][t] = q.s11;
         tag = PRIME ? p : sp;
+        pk = (bx >= wx0 && bx + S <= wx0 + RW && by >= wy0 && by + S <= wy0 + RH)
+                 ? stash_key(bx - wx0, by - wy0, cx, cy)
+                 : kStashFar;
       }
       s_pi[t] = tag;
+      s_pk[t] = pk;
+      // the next pass's fields and the pass after it's buffer indices
+      {
+        const uint32_t k1 = k0 + uint32_t(NT + t), k2 = k0 + uint32_t(2 * NT + t);
+        sp = sp_n;
+        p = p_n;
+        if (k1 < total) {
+          q = fused_load_raw(a.src, p, sc);
+          if constexpr (!PRIME) {
+            id = a.src.id[p];
+            scripted = a.src.flags ? (a.src.flags[p] != 0) : false;
+          }
+        }
+        if (k2 < total) {
+          sp_n = order_pos(k2);
+          p_n = a.perm[sp_n];
+        }
+      }
       lds_barrier();
       // the moved points' terms for the next step, at their new stencils
-      const uint32_t ptag = s_pi[lane_pt];
-      if (ptag != 0xffffffffu) {
-        FusedPoint q;
-        q.x = s_px[lane_pt];
-        q.y = s_py[lane_pt];
-        q.v0 = s_pf[0][lane_pt];
-        q.v1 = s_pf[1][lane_pt];
-        q.m = s_pf[2][lane_pt];
-        q.V = s_pf[3][lane_pt];
-        q.hl = s_pf[4][lane_pt];
-        q.sJ = s_pf[5][lane_pt];
-        q.s00 = s_pf[6][lane_pt];
-        q.s01 = s_pf[7][lane_pt];
-        q.s11 = s_pf[8][lane_pt];
-        fused_strain(q);
-        int bx, by, cx, cy;
-        point_key<BASIS>(sc, q.x, q.y, double(q.hl), bx, by, cx, cy);
-        if (bx >= wx0 && bx + S <= wx0 + RW && by >= wy0 && by + S <= wy0 + RH) {
-          P2GTerms<BASIS> o;
-          p2g_terms<BASIS>(sc, q.x, q.y, q.v0, q.v1, q.m, q.V, q.hl, q.J, q.c00, q.c01, q.c11, bx,
-                           by, cx, cy, o);
-          if (!p2g_terms_in_range<BASIS>(o)) atomicOr(&a.err[sid], kMpmFixedRangeBit);
-          unsigned long long *base = &s_acc[0][0] + (by - wy0) * RW + (bx - wx0);
-          p2g_scatter<BASIS>(o, [&](int f, int i, int j, unsigned long long v) {
-            atomicAdd(base + f * RN + j * RW + i, v);
-          });
-        } else {
-          // moved more than a cell since the sort (or left the grid): its
-          // terms go straight into the node sums after the tile
-          const uint32_t slot = atomicAdd(&s_nfar, 1u);
-          if (slot < uint32_t(kFarCap)) s_far[slot] = ptag;
-          else atomicOr(&a.err[sid], kMpmFixedRangeBit);
-        }
+      const uint32_t rk = s_pk[lane_pt];
+      if (rk == kStashFar) {
+        // moved more than a cell since the sort (or left the grid): its
+        // terms go straight into the node sums after the tile
+        const uint32_t slot = atomicAdd(&s_nfar, 1u);
+        if (slot < uint32_t(kFarCap)) s_far[slot] = s_pi[lane_pt];
+        else atomicOr(&a.err[sid], kMpmFixedRangeBit);
+      } else if (rk != kStashNone) {
+        FusedPoint r;
+        r.x = s_px[lane_pt];
+        r.y = s_py[lane_pt];
+        r.v0 = s_pf[0][lane_pt];
+        r.v1 = s_pf[1][lane_pt];
+        r.m = s_pf[2][lane_pt];
+        r.V = s_pf[3][lane_pt];
+        r.hl = s_pf[4][lane_pt];
+        r.sJ = s_pf[5][lane_pt];
+        r.s00 = s_pf[6][lane_pt];
+        r.s01 = s_pf[7][lane_pt];
+        r.s11 = s_pf[8][lane_pt];
+        fused_strain(r);
+        const int dx = int(rk & 0xffu), dy = int((rk >> 8) & 0xffu);
+        const int cx = int((rk >> 16) & 3u) + 1, cy = int((rk >> 18) & 3u) + 1;
+        P2GTerms<BASIS> o;
+        p2g_terms<BASIS>(sc, r.x, r.y, r.v0, r.v1, r.m, r.V, r.hl, r.J, r.c00, r.c01, r.c11,
+                         wx0 + dx, wy0 + dy, cx, cy, o);
+        if (!p2g_terms_in_range<BASIS>(o)) atomicOr(&a.err[sid], kMpmFixedRangeBit);
+        unsigned long long *base = &s_acc[0][0] + dy * RW + dx;
+        p2g_scatter<BASIS>(o, [&](int f, int i, int j, unsigned long long v) {
+          atomicAdd(base + f * RN + j * RW + i, v);
+        });
+      }
+      if constexpr (!PRIME) {
+        if (slot_at != 0xffffffffu) a.slot[slot_at] = slot_v;
       }
       lds_barrier();  // the stash is refilled by the next pass
     }

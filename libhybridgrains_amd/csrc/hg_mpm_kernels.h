@@ -201,6 +201,50 @@ __device__ __forceinline__ void basis_1d(double x, T hl, double gmin, double h, 
   }
 }
 
+// FP32 uGIMP factors of the three nodes lo, lo + 1, lo + 2 along one axis,
+// for a point whose support [x - hl, x + hl] starts in cell lo
+// (lo = floor((x - hl - gmin) / h), so r0 = x - x_lo lies in [hl, h + hl))
+// and is shorter than a cell (hl < h / 2, which scene creation requires).
+// Each node then has two pieces of basis_1d's four to choose from:
+//   node 0, r0 in [hl, h + hl):        tail (|r| >= h - hl) or linear;
+//   node 1, r1 = r0 - h in [hl - h, hl): inner (|r| < hl) or linear, r1 < 0;
+//   node 2, r2 = r0 - 2h in [hl - 2h, hl - h): tail, or nothing past h + hl
+//                                        or past the stencil's count.
+// The pieces are basis_1d's expressions.  The factors are continuous across
+// the piece boundaries, so where rounding puts a distance just across a
+// boundary the other piece's value differs only at rounding level.
+__device__ __forceinline__ void ugimp_1d_f32(double x, float hl, double gmin, double h,
+                                             float inv_w, int lo, int cnt, float (&a)[3],
+                                             float (&d)[3]) {
+  const float w = float(h);
+  const float r0 = float(x - (h * double(lo) + gmin));
+  const float r1 = float(x - (h * double(lo + 1) + gmin));
+  const float r2 = float(x - (h * double(lo + 2) + gmin));
+  const float c05 = 0.5f * inv_w;
+  const float hl2 = 2.0f * hl;
+  const float lin = hl2 * inv_w;  // |d| on the linear piece
+  const float wph = w + hl, wmh = w - hl;
+  {
+    const float u = wph - r0;
+    const bool tail = r0 >= wmh;
+    a[0] = tail ? u * u * c05 : hl2 * (1.0f - r0 * inv_w);
+    d[0] = tail ? -u * inv_w : -lin;
+  }
+  {
+    const float diff = fabsf(r1);
+    const bool inner = diff < hl;
+    a[1] = inner ? hl2 - (hl * hl + diff * diff) * inv_w : hl2 * (1.0f - diff * inv_w);
+    d[1] = inner ? 2.0f * (-r1) * inv_w : lin;
+  }
+  {
+    const float diff = -r2;
+    const float u = wph - diff;
+    const bool none = diff >= wph || cnt < 3;
+    a[2] = none ? 0.0f : u * u * c05;
+    d[2] = none ? 0.0f : u * inv_w;
+  }
+}
+
 // Both axes' factors, pre-scaled so that a node's weight is ax[i]*ay[j] and
 // its gradient (dx[i]*ay[j], ax[i]*dy[j]): uGIMP divides the product of the
 // overlap integrals by the point volume 4 hl^2 (uGIMP :108-123/:141-156);
@@ -214,8 +258,13 @@ __device__ __forceinline__ void basis_2d(double x, double y, T hl, double min0, 
                                          T (&ay)[BasisTraits<BASIS>::S],
                                          T (&dy)[BasisTraits<BASIS>::S]) {
   constexpr int S = BasisTraits<BASIS>::S;
-  basis_1d<T, BASIS>(x, hl, min0, h, inv_h, bx, cntx, ax, dx);
-  basis_1d<T, BASIS>(y, hl, min1, h, inv_h, by, cnty, ay, dy);
+  if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR && std::is_same<T, float>::value) {
+    ugimp_1d_f32(x, hl, min0, h, inv_h, bx, cntx, ax, dx);
+    ugimp_1d_f32(y, hl, min1, h, inv_h, by, cnty, ay, dy);
+  } else {
+    basis_1d<T, BASIS>(x, hl, min0, h, inv_h, bx, cntx, ax, dx);
+    basis_1d<T, BASIS>(y, hl, min1, h, inv_h, by, cnty, ay, dy);
+  }
   if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) {
     const T inv_vp = T(1) / (T(4) * hl * hl);
 #pragma unroll
