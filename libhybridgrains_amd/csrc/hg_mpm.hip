@@ -1010,8 +1010,9 @@ __global__ __launch_bounds__(256) void mpm_bin_kernel(BinArgs a) {
 //    each node of its new stencil (PhysicsGrid::rasterizePointMasses
 //    :168-207, rasterizePointMomentum :274-311, computeForces :413-473, with
 //    MaterialPoints::computeHyperelasticCauchyStress, MaterialPoints.cpp:
-//    183-202) -- in FP32, rounds each down to a whole number of the scene's
-//    quantum (fx_shift) and adds it into LDS accumulators covering the
+//    183-202) -- from FP32 weight factors with exact FP64 products, rounds
+//    each to the nearest whole number of the scene's quantum (fx_shift) in
+//    the same fused multiply-add and adds it into LDS accumulators covering the
 //    tile's nodes one cell beyond its points' reach; a point that moved
 //    further adds straight into the node sums in HBM after the pass;
 //  * the accumulators are then added into the node sums in HBM (atomics, at
@@ -1038,58 +1039,79 @@ struct FusedArgs {
   int32_t *err;
 };
 
-// A term already scaled to quanta, rounded to the nearest whole number of
-// them (ties to even), as a two's-complement 64-bit integer (the caller
-// bounds it below 2^50).  The term is exact in FP64; adding 1.5 2^52 brings
-// it into [2^52, 2^53), where the spacing of doubles is 1, so the addition
-// does the rounding, and the sum's bit pattern is 0x4338000000000000 plus the
-// rounded term.
-__device__ __forceinline__ unsigned long long fx_of(float scaled) {
-  const double b = double(scaled) + 0x1.8p52;
-  return static_cast<unsigned long long>(__double_as_longlong(b)) - 0x4338000000000000ull;
+// Node-sum terms in whole quanta.  A term t scaled to quanta with |t| < 2^51,
+// added to 1.5 2^52, lands in [2^52, 2^53), where the spacing of doubles is 1:
+// the sum is t rounded to the nearest whole number (ties to even), and its bit
+// pattern is kFxBias plus that number as a two's-complement integer.  A fused
+// multiply-add with the magic number as its addend therefore forms a product
+// and rounds it to quanta in one instruction (the caller bounds the terms
+// below 2^50).
+constexpr double kFxMagic = 0x1.8p52;
+constexpr unsigned long long kFxBias = 0x4338000000000000ull;
+__device__ __forceinline__ unsigned long long fx_bits(double biased) {
+  return static_cast<unsigned long long>(__double_as_longlong(biased));
 }
 
-// A point's rasterisation terms in FP32: per axis the S weights and masked
+// A point's rasterisation terms: per axis the S weights and masked
 // weight-derivative factors of its stencil (base node bx, by; cntx, cnty
-// stencil nodes per axis), and its mass, momentum and -V J sigma.  The stress
-// is formed in FP64 (roots from rsq_nr / sqrt_rsq).
+// stencil nodes per axis), formed in FP32 and held in FP64 (exactly), so that
+// every node's weight and gradient products are exact; and its mass, momentum
+// and -V J sigma.  The stress comes from the strain as stored: the deviations
+// dJ = J - 1 and d = b_bar - I (off-diagonal b_bar01).
 template <int BASIS>
 struct P2GTerms {
   static constexpr int S = BasisTraits<BASIS>::S;
-  float wx[S], gx[S], wy[S], gy[S];
+  double wx[S], gx[S], wy[S], gy[S];
   // mass, momentum and -V J sigma, each times its quantity's 2^fx_shift
   // (an exact power-of-two scaling)
-  float m, mv0, mv1, f00, f01, f11;
+  double m, mv0, mv1, f00, f01, f11;
 };
+
+// MaterialPoints::computeHyperelasticCauchyStress (MaterialPoints.cpp:183-202)
+// gives tau = kappa/2 (Jt^2 - 1) I + mu dev(e / Jt) for e = J b_bar and
+// Jt = sqrt(det e), with no stress in tension (J > 1).  Written in the
+// deviations a = e00 - 1, c = e11 - 1 and e01, all small, it needs no FP64:
+// Jt^2 - 1 = det e - 1 = a + c + a c - e01^2, and dev(e / Jt) has the entries
+// +-(a - c) / (2 Jt) and e01 / Jt, so every quantity keeps FP32's relative
+// precision (none is formed as a difference of numbers near 1).
+__device__ __forceinline__ void stress_from_deviations(const SceneDev &sc, float dJ, float d00,
+                                                       float d01, float d11, float &t00,
+                                                       float &t01, float &t11) {
+  t00 = t01 = t11 = 0.0f;
+  if (dJ <= 0.0f) {
+    const float a = __builtin_fmaf(d00, dJ, d00 + dJ);
+    const float c = __builtin_fmaf(d11, dJ, d11 + dJ);
+    const float e01 = __builtin_fmaf(d01, dJ, d01);
+    const float del = __builtin_fmaf(-e01, e01, __builtin_fmaf(a, c, a + c));
+    const float rj = __builtin_amdgcn_rsqf(1.0f + del);
+    const float p = 0.5f * float(sc.kappa) * del;
+    const float h = 0.5f * float(sc.mu) * (a - c) * rj;
+    t00 = p + h;
+    t11 = p - h;
+    t01 = float(sc.mu) * e01 * rj;
+  }
+}
 
 template <int BASIS>
 __device__ __forceinline__ void p2g_terms(const SceneDev &sc, double x, double y, float v0,
-                                          float v1, float m, float V, float hl, double J,
-                                          double b00, double b01, double b11, int bx, int by,
+                                          float v1, float m, float V, float hl, float dJ,
+                                          float d00, float d01, float d11, int bx, int by,
                                           int cntx, int cnty, P2GTerms<BASIS> &o) {
   constexpr int S = BasisTraits<BASIS>::S;
-  double t00 = 0.0, t01 = 0.0, t11 = 0.0;
-  if (J <= 1.0) {
-    const double e00 = b00 * J, e01 = b01 * J, e11 = b11 * J;
-    const double det = e00 * e11 - e01 * e01;
-    const double inrm = rsq_nr(det);
-    const double Jt = sqrt_rsq(det, inrm);
-    const double bb00 = e00 * inrm, bb01 = e01 * inrm, bb11 = e11 * inrm;
-    const double half_tr = (bb00 + bb11) * 0.5;
-    const double c = sc.kappa * 0.5 * (Jt + 1.0) * (Jt - 1.0);
-    t00 = c + sc.mu * (bb00 - half_tr);
-    t01 = sc.mu * bb01;
-    t11 = c + sc.mu * (bb11 - half_tr);
-  }
-  // -V J sigma with sigma = tau / J is -V tau
+  float t00, t01, t11;
+  stress_from_deviations(sc, dJ, d00, d01, d11, t00, t01, t11);
+  // -V J sigma with sigma = tau / J is -V tau (products of two FP32 values,
+  // exact in FP64)
   const double nV = -double(V);
   const int sf = sc.fx_shift[kFxF0], sp = sc.fx_shift[kFxMom0];
-  o.f00 = ldexpf(float(nV * t00), sf);
-  o.f01 = ldexpf(float(nV * t01), sf);
-  o.f11 = ldexpf(float(nV * t11), sf);
-  o.m = ldexpf(m, sc.fx_shift[kFxMass]);
-  o.mv0 = ldexpf(m * v0, sp);
-  o.mv1 = ldexpf(m * v1, sp);
+  o.f00 = ldexp(nV * double(t00), sf);
+  o.f01 = ldexp(nV * double(t01), sf);
+  o.f11 = ldexp(nV * double(t11), sf);
+  const double md = double(m);
+  o.m = ldexp(md, sc.fx_shift[kFxMass]);
+  // products of two FP32 values are exact in FP64
+  o.mv0 = ldexp(md * double(v0), sp);
+  o.mv1 = ldexp(md * double(v1), sp);
   const float inv_h = float(sc.inv_h);
   float inv_vp = 1.0f;
   if constexpr (BASIS == HG_BASIS_UGIMP_LINEAR) inv_vp = 1.0f / (4.0f * hl * hl);
@@ -1120,10 +1142,10 @@ __device__ __forceinline__ void p2g_terms(const SceneDev &sc, double x, double y
     // a node's terms count only where its weight's factors are nonzero
     // (PhysicsGrid.cpp:413-473 skips zero-weight nodes); with a zero factor
     // every term of the node is then exactly zero
-    o.wx[k] = wxk;
-    o.gx[k] = (wxk != 0.0f) ? gxk : 0.0f;
-    o.wy[k] = ay;
-    o.gy[k] = (ay != 0.0f) ? gyk : 0.0f;
+    o.wx[k] = double(wxk);
+    o.gx[k] = double((wxk != 0.0f) ? gxk : 0.0f);
+    o.wy[k] = double(ay);
+    o.gy[k] = double((ay != 0.0f) ? gyk : 0.0f);
   }
 }
 
@@ -1133,16 +1155,19 @@ __device__ __forceinline__ void p2g_terms(const SceneDev &sc, double x, double y
 template <int BASIS>
 __device__ __forceinline__ bool p2g_terms_in_range(const P2GTerms<BASIS> &o) {
   constexpr int S = BasisTraits<BASIS>::S;
-  float g = 0.0f;
+  double g = 0.0;
 #pragma unroll
-  for (int k = 0; k < S; ++k) g = fmaxf(g, fmaxf(fabsf(o.gx[k]), fabsf(o.gy[k])));
-  const float fb = (fabsf(o.f00) + 2.0f * fabsf(o.f01) + fabsf(o.f11)) * g;
-  const float lim = 1.0e15f;  // below 2^50
-  return fabsf(o.m) < lim && fmaxf(fabsf(o.mv0), fabsf(o.mv1)) < lim && fb < lim;
+  for (int k = 0; k < S; ++k) g = fmax(g, fmax(fabs(o.gx[k]), fabs(o.gy[k])));
+  const double fb = (fabs(o.f00) + 2.0 * fabs(o.f01) + fabs(o.f11)) * g;
+  const double lim = 1.0e15;  // below 2^50
+  return fabs(o.m) < lim && fmax(fabs(o.mv0), fabs(o.mv1)) < lim && fb < lim;
 }
 
-// Adds a point's terms at the nodes of its stencil: add(q, i, j, value) for
-// quantity q at node (bx + i, by + j).
+// Adds a point's terms at the nodes of its stencil: add(q, i, j, bits) for
+// quantity q at node (bx + i, by + j), where bits is kFxBias plus the term in
+// whole quanta (mod 2^64).  The weight and gradient products are exact; each
+// mass and momentum term is rounded once, each force term (the sum of two
+// products) once after each product.
 template <int BASIS, typename Add>
 __device__ __forceinline__ void p2g_scatter(const P2GTerms<BASIS> &o, const Add &add) {
   constexpr int S = BasisTraits<BASIS>::S;
@@ -1150,14 +1175,14 @@ __device__ __forceinline__ void p2g_scatter(const P2GTerms<BASIS> &o, const Add 
   for (int j = 0; j < S; ++j) {
 #pragma unroll
     for (int i = 0; i < S; ++i) {
-      const float w = o.wx[i] * o.wy[j];
-      const float gx = o.gx[i] * o.wy[j];
-      const float gy = o.wx[i] * o.gy[j];
-      add(kFxMass, i, j, fx_of(w * o.m));
-      add(kFxMom0, i, j, fx_of(w * o.mv0));
-      add(kFxMom1, i, j, fx_of(w * o.mv1));
-      add(kFxF0, i, j, fx_of(__builtin_fmaf(o.f01, gy, o.f00 * gx)));
-      add(kFxF1, i, j, fx_of(__builtin_fmaf(o.f11, gy, o.f01 * gx)));
+      const double w = o.wx[i] * o.wy[j];
+      const double gx = o.gx[i] * o.wy[j];
+      const double gy = o.wx[i] * o.gy[j];
+      add(kFxMass, i, j, fx_bits(__builtin_fma(w, o.m, kFxMagic)));
+      add(kFxMom0, i, j, fx_bits(__builtin_fma(w, o.mv0, kFxMagic)));
+      add(kFxMom1, i, j, fx_bits(__builtin_fma(w, o.mv1, kFxMagic)));
+      add(kFxF0, i, j, fx_bits(__builtin_fma(o.f01, gy, __builtin_fma(o.f00, gx, kFxMagic))));
+      add(kFxF1, i, j, fx_bits(__builtin_fma(o.f11, gy, __builtin_fma(o.f01, gx, kFxMagic))));
     }
   }
 }
@@ -1193,12 +1218,6 @@ __device__ __forceinline__ FusedPoint fused_load_raw(const PtsRaw &p, uint32_t i
   q.m = ld<float>(p.m, i);
   q.V = ld<float>(p.V, i);
   q.hl = p.hl ? ld<float>(p.hl, i) : float(sc.hl_uniform);
-  return q;
-}
-
-__device__ __forceinline__ FusedPoint fused_load(const PtsRaw &p, uint32_t i, const SceneDev &sc) {
-  FusedPoint q = fused_load_raw(p, i, sc);
-  fused_strain(q);
   return q;
 }
 
@@ -1453,16 +1472,15 @@ __global__ __launch_bounds__((fused_threads<TW, TH, MULT>())) __attribute__((amd
         r.s00 = s_pf[6][lane_pt];
         r.s01 = s_pf[7][lane_pt];
         r.s11 = s_pf[8][lane_pt];
-        fused_strain(r);
         const int dx = int(rk & 0xffu), dy = int((rk >> 8) & 0xffu);
         const int cx = int((rk >> 16) & 3u) + 1, cy = int((rk >> 18) & 3u) + 1;
         P2GTerms<BASIS> o;
-        p2g_terms<BASIS>(sc, r.x, r.y, r.v0, r.v1, r.m, r.V, r.hl, r.J, r.c00, r.c01, r.c11,
+        p2g_terms<BASIS>(sc, r.x, r.y, r.v0, r.v1, r.m, r.V, r.hl, r.sJ, r.s00, r.s01, r.s11,
                          wx0 + dx, wy0 + dy, cx, cy, o);
         if (!p2g_terms_in_range<BASIS>(o)) atomicOr(&a.err[sid], kMpmFixedRangeBit);
         unsigned long long *base = &s_acc[0][0] + dy * RW + dx;
         p2g_scatter<BASIS>(o, [&](int f, int i, int j, unsigned long long v) {
-          atomicAdd(base + f * RN + j * RW + i, v);
+          atomicAdd(base + f * RN + j * RW + i, v - kFxBias);
         });
       }
       if constexpr (!PRIME) {
@@ -1486,17 +1504,17 @@ __global__ __launch_bounds__((fused_threads<TW, TH, MULT>())) __attribute__((amd
     const uint32_t nfar = min(s_nfar, uint32_t(kFarCap));
     for (uint32_t f = uint32_t(t); f < nfar; f += uint32_t(NT)) {
       const uint32_t i = s_far[f];
-      const FusedPoint q = fused_load(PRIME ? a.src : a.dst, i, sc);
+      const FusedPoint q = fused_load_raw(PRIME ? a.src : a.dst, i, sc);
       int bx, by, cx, cy;
       point_key<BASIS>(sc, q.x, q.y, double(q.hl), bx, by, cx, cy);
       P2GTerms<BASIS> o;
-      p2g_terms<BASIS>(sc, q.x, q.y, q.v0, q.v1, q.m, q.V, q.hl, q.J, q.c00, q.c01, q.c11, bx, by,
-                       cx, cy, o);
+      p2g_terms<BASIS>(sc, q.x, q.y, q.v0, q.v1, q.m, q.V, q.hl, q.sJ, q.s00, q.s01, q.s11, bx,
+                       by, cx, cy, o);
       if (!p2g_terms_in_range<BASIS>(o)) atomicOr(&a.err[sid], kMpmFixedRangeBit);
       const uint32_t nb = sc.node_off + uint32_t(by) * uint32_t(sc.snx) + uint32_t(bx);
       p2g_scatter<BASIS>(o, [&](int fq, int i2, int j2, unsigned long long v) {
         atomicAdd(&a.acc[size_t(fq) * a.nnode + nb + uint32_t(j2) * uint32_t(sc.snx) + uint32_t(i2)],
-                  v);
+                  v - kFxBias);
       });
     }
     __syncthreads();  // LDS is reused by the next tile
