@@ -25,7 +25,10 @@
 //                          | pre-step positions, list clean-up), DEM sort,
 //                          | homogenisation when on, then the next step's
 //                          | DEM contacts and integration
-// with events where one stream needs the other's results.  The pre-step
+// with events where one stream needs the other's results (FP32, with the
+// fused continuum step: grid update with the exchange, then the fused G2P and
+// the MPM sort beside the grain branch, then the next step's grain mass on
+// the continuum stream).  The pre-step
 // positions the coupling uses are saved by the previous step's finish (and
 // at creation), so the DEM integration may move the grains while grain_mass
 // still reads them.  A graph capture queues the same kernels on one stream.
@@ -570,40 +573,40 @@ static void launch_hybrid_steps(hg_hybrid *h, double dt, uint32_t n, bool concur
     homog(a);
   };
   if (hg_mpm_is_fused(h->mpm)) {
-    // FP32 continuum.  The node sums of the moved points are complete when
-    // the fused G2P ends, so the next step's grain mass (which reads the
-    // continuum mass from them) runs on the grain stream during the
-    // continuum's sort, and the grid update applies the exchange at the
-    // nodes holding grain mass: neither is on the continuum's path.
+    // FP32 continuum.  The grid update applies the exchange at the nodes
+    // holding grain mass.  The grain branch (end of the coupling, the DEM
+    // sort, the next step's DEM step) runs beside the fused G2P; the next
+    // step's grain mass, which reads the continuum mass from the node sums
+    // the fused G2P completes, follows the continuum's sort on the continuum
+    // stream, so that one wait per step joins the branches and it comes
+    // when the grain branch has long finished (a wait on an event recorded
+    // just before costs about as much as the grain mass itself).
     hg_mpm_prepare_fused(h->mpm, dt);
     const hgmpm::GrainNodes gn = {h->g.dmass, h->g.dmom, h->g.vafter, h->g.aafter};
-    auto grain_mass = [&]() {
+    auto grain_mass_on = [&](hipStream_t st) {
       HybArgs a = make_args(h);
       a.acc_mass = hg_mpm_mass_sums(h->mpm);
       a.mass_quantum = hg_mpm_mass_quantum(h->mpm);
-      ProfScope ps(prof, "hyb_grain_mass", side);
-      hipLaunchKernelGGL((hyb_grain_mass<T, BASIS>), gb, tb, 0, side, a);
+      ProfScope ps(prof, "hyb_grain_mass", st);
+      hipLaunchKernelGGL((hyb_grain_mass<T, BASIS>), gb, tb, 0, st, a);
     };
     order(h->fork_ev, s, side);
     dem_step();
-    grain_mass();
+    order(h->join_ev, side, s);
+    grain_mass_on(s);
     for (uint32_t k = 0; k < n; ++k) {
       const HybArgs a = make_args(h);
-      order(h->join_ev, side, s);
       hg_mpm_launch_coupled_grid(h->mpm, dt, &gn);
       homog_factor(a);
       order(h->fork_ev, s, side);
       grain_end(a);
+      if (k + 1 < n) dem_step();  // the next step's DEM step, ahead
       hg_mpm_launch_fused_g2p(h->mpm);
       h->par = 1 - h->par;
-      if (k + 1 < n) {
-        dem_step();  // the next step's DEM step, ahead
-        order(h->fork_ev, s, side);
-        grain_mass();
-      }
       hg_mpm_launch_fused_sort(h->mpm);
+      order(h->join_ev, side, s);
+      if (k + 1 < n) grain_mass_on(s);
     }
-    order(h->join_ev, side, s);
     return;
   }
   order(h->fork_ev, s, side);
