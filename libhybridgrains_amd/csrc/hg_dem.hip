@@ -378,13 +378,66 @@ __global__ __launch_bounds__(kTeamThreads) void dem_contacts_kernel(ContactArgs 
     st<T>(a.f.oF, 2 * sl, T(F0));
     st<T>(a.f.oF, 2 * sl + 1, T(F1));
     st<T>(a.f.otau0, sl, T(r00 * F1 - r01 * F0));
-    st<T>(a.f.otau1, sl, T(r10 * F1 - r11 * F0));
   }
+  // The contacts owned by a smaller partner j: the owner's computation above
+  // (j as the body, this body as its partner: geometry, spring history from
+  // j's previous slots, force), so that the integration finds the force this
+  // body receives among its own records.  Every operation is the owner's, in
+  // its order, so the values equal the owner's bit for bit.
+  const uint32_t lid = i - sc.b_off;
   for (int k = lane; k < nin; k += kTeam) {
     const uint32_t key = s_in[tb][k];
     int rank = 0;
     for (int q = 0; q < nin; ++q) rank += (s_in[tb][q] < key) ? 1 : 0;
-    a.f.ikey[i * uint32_t(CAP) + uint32_t(rank)] = key;
+    const uint32_t j = key + sc.b_off;
+    const double xo = a.b.x[j], yo = a.b.y[j], ro = a.b.r[j];
+    const double vxo = double(ld<T>(a.b.vx, j)), vyo = double(ld<T>(a.b.vy, j));
+    const double omo = double(ld<T>(a.b.om, j));
+    const double d0 = xo - xi, d1 = yo - yi;
+    const double dz = d0 * d0 + d1 * d1;
+    double n0 = d0, n1 = d1;
+    if (dz > 0.0) {
+      const double nz = sqrt(dz);
+      n0 = d0 / nz;
+      n1 = d1 / nz;
+    }
+    const double gap = sqrt(d0 * d0 + d1 * d1) - ro - ri;
+    const double depth = gap < 0.0 ? gap : 0.0;
+    const double f = ro / (ro + ri);
+    const double p0 = xo + f * (xi - xo), p1 = yo + f * (yi - yo);
+    const double r00 = p0 - xo, r01 = p1 - yo;
+    const double r10 = p0 - xi, r11 = p1 - yi;
+    const double vr0 = vxo + omo * (-r01) - vxi - omi * (-r11);
+    const double vr1 = vyo + omo * r00 - vyi - omi * r10;
+    const double w = sc.weighted ? (double(ld<T>(a.b.w, j)) + wi) * 0.5 : 1.0;
+    bool had = false;
+    double o0 = 0.0, o1 = 0.0;
+    {
+      const uint32_t jbase = j * uint32_t(CAP);
+      const int jold = a.old_s.ocnt[j];
+      const uint4 *kv = reinterpret_cast<const uint4 *>(a.old_s.okey + jbase);
+      int hit = -1;
+#pragma unroll
+      for (int q4 = 0; q4 < CAP / 4; ++q4) {
+        const uint4 v = kv[q4];
+        const uint32_t kk[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (hit < 0 && 4 * q4 + e < jold && kk[e] == lid) hit = 4 * q4 + e;
+      }
+      if (hit >= 0) {
+        had = true;
+        o0 = double(ld<T>(a.old_s.ods, 2 * (jbase + uint32_t(hit))));
+        o1 = double(ld<T>(a.old_s.ods, 2 * (jbase + uint32_t(hit)) + 1));
+      }
+    }
+    double ds0, ds1, F0, F1;
+    spring(had, o0, o1, dt, vr0, vr1, n0, n1, ds0, ds1);
+    penalty_force(sc, w, depth, n0, n1, vr0, vr1, ds0, ds1, F0, F1);
+    const uint32_t sl = i * uint32_t(CAP) + uint32_t(rank);
+    st<T>(a.f.iF, 2 * sl, T(F0));
+    st<T>(a.f.iF, 2 * sl + 1, T(F1));
+    st<T>(a.f.itau, sl, T(r10 * F1 - r11 * F0));
   }
   if (lane == 0) {
     a.new_s.ocnt[i] = uint8_t(nown);
@@ -464,28 +517,15 @@ __global__ __launch_bounds__(kThreads) void dem_integrate_kernel(IntegrateArgs a
     F1 += double(ld<T>(a.f.pF, 2 * ps + 1));
     F2 += double(ld<T>(a.f.ptau, ps));
   }
-  // pairs owned by a smaller partner, in partner order
-  const uint32_t lid = i - sc.b_off;
+  // pairs owned by a smaller partner, in partner order (the owner's force,
+  // received negated, and the torque of its negation, as the contacts kernel
+  // recorded them among this body's records)
   const int nin = a.f.icnt[i];
   for (int k = 0; k < nin; ++k) {
-    const uint32_t j = a.f.ikey[i * uint32_t(CAP) + uint32_t(k)] + sc.b_off;
-    const int nj = a.s.ocnt[j];
-    const uint4 *kv = reinterpret_cast<const uint4 *>(a.s.okey + j * uint32_t(CAP));
-    int hit = -1;
-#pragma unroll
-    for (int q4 = 0; q4 < CAP / 4; ++q4) {
-      const uint4 v = kv[q4];
-      const uint32_t kk[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        if (hit < 0 && 4 * q4 + e < nj && kk[e] == lid) hit = 4 * q4 + e;
-    }
-    if (hit >= 0) {
-      const uint32_t sl = j * uint32_t(CAP) + uint32_t(hit);
-      F0 -= double(ld<T>(a.f.oF, 2 * sl));
-      F1 -= double(ld<T>(a.f.oF, 2 * sl + 1));
-      F2 -= double(ld<T>(a.f.otau1, sl));
-    }
+    const uint32_t sl = i * uint32_t(CAP) + uint32_t(k);
+    F0 -= double(ld<T>(a.f.iF, 2 * sl));
+    F1 -= double(ld<T>(a.f.iF, 2 * sl + 1));
+    F2 -= double(ld<T>(a.f.itau, sl));
   }
   // own pairs, then kinematic pairs (the key order)
   const int nown = a.s.ocnt[i];
@@ -798,10 +838,10 @@ static hg_status upload_bodies(hg_dem *h, const hg_dem_desc *descs) {
   ForceRaw &f = h->f;
   HG_TRY(dalloc(h, &f.oF, size_t(B) * h->cap * 2 * sizeof(T)));
   HG_TRY(dalloc(h, &f.otau0, size_t(B) * h->cap * sizeof(T)));
-  HG_TRY(dalloc(h, &f.otau1, size_t(B) * h->cap * sizeof(T)));
   HG_TRY(dalloc(h, &f.pF, size_t(B) * kMaxPlanes * 2 * sizeof(T)));
   HG_TRY(dalloc(h, &f.ptau, size_t(B) * kMaxPlanes * sizeof(T)));
-  HG_TRY(dalloc(h, &f.ikey, size_t(B) * h->cap * 4));
+  HG_TRY(dalloc(h, &f.iF, size_t(B) * h->cap * 2 * sizeof(T)));
+  HG_TRY(dalloc(h, &f.itau, size_t(B) * h->cap * sizeof(T)));
   HG_TRY(dalloc(h, &f.icnt, B));
   return HG_OK;
 }
@@ -1364,7 +1404,7 @@ static uint64_t dem_bytes_per_body(size_t T, int cap) {
   const uint64_t C = uint64_t(cap);
   const uint64_t bodies = 4 * 8 + 7 * T + 1;
   const uint64_t slots = C * 4 + C * 2 * T + 1 + uint64_t(kMaxPlanes) + uint64_t(kMaxPlanes) * 2 * T;
-  const uint64_t forces = C * 4 * T + uint64_t(kMaxPlanes) * 3 * T + C * 4 + 1;
+  const uint64_t forces = C * 6 * T + uint64_t(kMaxPlanes) * 3 * T + 1;
   return bodies + 2 * slots + forces + 3 * 4 + 24;
 }
 

@@ -49,12 +49,15 @@ struct SlotsRaw {
 };
 
 struct ForceRaw {
-  void *oF;     // [B][cap][2]
-  void *otau0;  // [B][cap]
-  void *otau1;  // [B][cap]
+  void *oF;     // [B][cap][2] force of each owned contact on its owner
+  void *otau0;  // [B][cap]    its torque on the owner
   void *pF;     // [B][kMaxPlanes][2]
   void *ptau;   // [B][kMaxPlanes]
-  uint32_t *ikey;  // [B][cap] owners of contacts with this body
+  // contacts owned by a smaller partner, in partner order: the force on the
+  // owner (which this body receives negated) and the torque of its negation
+  // about this body, as the owner computes them
+  void *iF;     // [B][cap][2]
+  void *itau;   // [B][cap]
   uint8_t *icnt;
 };
 
