@@ -15,7 +15,8 @@
 //    with room to spread, so the buckets of a compact scene are its cells;
 //    cells a table width apart share a bucket, which only adds candidates
 //    the distance test rejects (the 3x3 neighbour buckets stay distinct).
-//    Bodies are counting-sorted by bucket every step (hg_sort).
+//    DEM-only steps file the moved bodies in bucket tables (three rotating);
+//    the hybrid's grains are counting-sorted by bucket every step (hg_sort).
 //  * contacts: each contact is owned by one body -- the smaller index of a
 //    circle pair, or the simulated body of a pair with a fixed body, as the
 //    reference keys its CollisionTypeCache.  A body keeps up to cap owned
@@ -24,12 +25,15 @@
 //    sorted by partner key, double-buffered so that the next step's
 //    contacts join their spring history by partner; per-plane slots likewise.
 //
-// Kernels of one step: dem_contacts (a team of lanes per body in cell order:
-// narrow phase over the 3x3 cells, history join, penalty force with the
-// Coulomb projection of the spring, plane contacts), dem_integrate (thread per
-// body: gathers its forces in the reference's summation order -- gravity,
-// planes, pairs owned by smaller partners, own pairs, kinematic pairs -- then
-// symplectic Euler, then the new cell key), and the counting sort.
+// Kernels of one step: dem_contacts (a team of lanes per body: narrow phase
+// over the 3x3 cells, history join, penalty force with the Coulomb projection
+// of the spring for the body's own contacts and for those a smaller partner
+// owns, plane contacts), then symplectic Euler with the forces gathered in the
+// reference's summation order -- gravity, planes, pairs owned by smaller
+// partners, own pairs, kinematic pairs.  DEM-only steps do it all in one
+// launch (the team's first lane integrates into the other state buffer and
+// files the body in the next table); the hybrid's grains run dem_integrate
+// (thread per body, new cell key) and the counting sort after dem_contacts.
 #include "hg_dem_handle.h"
 #include "hg_sort.h"
 
@@ -137,8 +141,15 @@ struct ContactArgs {
   const uint32_t *perm, *cell_start;
   int32_t *err;
   double *cpoint;  // optional: contact point per owned slot (circle pairs)
-  TableRaw tcur, tnext;  // bucket tables: read, and cleared for the integration
+  // DEM-only steps (the tables): the table read, the one the moved bodies
+  // are filed in, and the one cleared for the step after (read last step)
+  TableRaw tcur, tnext, tclr;
   int32_t kslots;
+  // DEM-only steps integrate in the same launch: the state after the step
+  // (position, angle and velocities; the other fields are b's) and, when
+  // kept, the total force per body
+  BodiesRaw bn;
+  double *F_out;
 };
 
 // Files body i in bucket k of table t (and records where it went).
@@ -161,6 +172,12 @@ template <bool TABLE> constexpr int team_lanes() { return TABLE ? 8 : 4; }
 constexpr int kTeamThreads = 256;
 template <bool TABLE> constexpr int team_bodies() { return kTeamThreads / team_lanes<TABLE>(); }
 
+// With the tables (DEM-only handles) the same launch also ends the step: the
+// team's force terms (rounded to T as stored) go to LDS, and its first lane
+// sums them in the reference's order and integrates the body into bn (the
+// state this launch does not read), then files it in the next table.  Each
+// body's team forms every contact term the body receives, so no body waits
+// for another's.
 template <typename T, int CAP, bool TABLE>
 __global__ __launch_bounds__(kTeamThreads) void dem_contacts_kernel(ContactArgs a) {
   constexpr int kTeam = team_lanes<TABLE>();
@@ -168,6 +185,11 @@ __global__ __launch_bounds__(kTeamThreads) void dem_contacts_kernel(ContactArgs 
   __shared__ uint32_t s_own[kTeamBodies][CAP];
   __shared__ uint32_t s_in[kTeamBodies][CAP];
   __shared__ int s_nown[kTeamBodies], s_nin[kTeamBodies];
+  // the terms the integration sums (tables only): own and incoming pairs by
+  // slot, planes by index (with their activity)
+  constexpr int TB = TABLE ? kTeamBodies : 1, TC = TABLE ? CAP : 1, TP = TABLE ? kMaxPlanes : 1;
+  __shared__ double s_of[TB][TC][3], s_if[TB][TC][3], s_pf[TB][TP][3];
+  __shared__ uint8_t s_pon[TB][TP];
   const int tb = int(threadIdx.x) / kTeam, lane = int(threadIdx.x) % kTeam;
   const uint32_t blk = hg_xcd_remap(blockIdx.x, a.nblocks_padded);
   const uint32_t s = blk * uint32_t(kTeamBodies) + uint32_t(tb);
@@ -186,10 +208,10 @@ __global__ __launch_bounds__(kTeamThreads) void dem_contacts_kernel(ContactArgs 
   const double ri = active ? a.b.r[i] : 0.0;
   const bool fi = active ? a.b.fixed[i] != 0 : true;
   if constexpr (TABLE) {
-    // the table the integration files into next: every bucket it holds
-    // bodies in is some body's recorded bucket
-    if (active && lane == 0) a.tnext.count[a.tnext.bkey[i]] = 0u;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *a.tnext.novf = 0u;
+    // the table the next step files into: every bucket it holds bodies in is
+    // some body's recorded bucket
+    if (active && lane == 0) a.tclr.count[a.tclr.bkey[i]] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.tclr.novf = 0u;
   }
   if (active && TABLE) {
     const bool finite_i = isfinite(xi) && isfinite(yi);
@@ -289,188 +311,265 @@ __global__ __launch_bounds__(kTeamThreads) void dem_contacts_kernel(ContactArgs 
     }
   }
   __syncthreads();
-  if (!active) return;
-  const int cnt_own = s_nown[tb], cnt_in = s_nin[tb];
+  if (!TABLE && !active) return;
+  const int cnt_own = active ? s_nown[tb] : 0, cnt_in = active ? s_nin[tb] : 0;
   if (lane == 0 && (cnt_own > CAP || cnt_in > CAP)) atomicOr(&a.err[sid], 4);
   const int nown = min(cnt_own, CAP), nin = min(cnt_in, CAP);
 
   const double dt = sc.dt;
+  // (a lane without a body reads body 0 and writes nothing)
   const double vxi = double(ld<T>(a.b.vx, i)), vyi = double(ld<T>(a.b.vy, i));
   const double omi = double(ld<T>(a.b.om, i));
   const double wi = sc.weighted ? double(ld<T>(a.b.w, i)) : 1.0;
   const uint32_t obase = i * uint32_t(CAP);
   const int old_n = a.old_s.ocnt[i];
-  for (int k = lane; k < nown; k += kTeam) {
-    const uint32_t key = s_own[tb][k];
-    int rank = 0;
-    for (int q = 0; q < nown; ++q) rank += (s_own[tb][q] < key) ? 1 : 0;
-    const bool kin = (key & kKinematic) != 0;
-    const uint32_t j = (key & ~kKinematic) + sc.b_off;
-    const double xj = a.b.x[j], yj = a.b.y[j], rj = a.b.r[j];
-    // n = normalized(q_i - q_j) (RigidBody2DSim.cpp:320)
-    const double d0 = xi - xj, d1 = yi - yj;
-    const double dz = d0 * d0 + d1 * d1;
-    double n0 = d0, n1 = d1;
-    if (dz > 0.0) {
-      const double nz = sqrt(dz);
-      n0 = d0 / nz;
-      n1 = d1 / nz;
-    }
-    const double gap = sqrt(d0 * d0 + d1 * d1) - ri - rj;
-    const double depth = gap < 0.0 ? gap : 0.0;
-    const double vxj = double(ld<T>(a.b.vx, j)), vyj = double(ld<T>(a.b.vy, j));
-    const double omj = double(ld<T>(a.b.om, j));
-    const uint32_t sl = obase + uint32_t(rank);
-    double vr0, vr1, r00, r01, r10 = 0.0, r11 = 0.0, w;
-    if (!kin) {
-      // CircleCircleConstraint (CircleCircleConstraint.cpp:171-193, :224-233)
-      const double f = ri / (ri + rj);
-      const double p0 = xi + f * (xj - xi), p1 = yi + f * (yj - yi);
-      r00 = p0 - xi;
-      r01 = p1 - yi;
-      r10 = p0 - xj;
-      r11 = p1 - yj;
-      vr0 = vxi + omi * (-r01) - vxj - omj * (-r11);
-      vr1 = vyi + omi * r00 - vyj - omj * r10;
-      w = sc.weighted ? (wi + double(ld<T>(a.b.w, j))) * 0.5 : 1.0;
-      if (a.cpoint) {
-        a.cpoint[2 * sl] = p0;
-        a.cpoint[2 * sl + 1] = p1;
+  if (active) {
+    for (int k = lane; k < nown; k += kTeam) {
+      const uint32_t key = s_own[tb][k];
+      int rank = 0;
+      for (int q = 0; q < nown; ++q) rank += (s_own[tb][q] < key) ? 1 : 0;
+      const bool kin = (key & kKinematic) != 0;
+      const uint32_t j = (key & ~kKinematic) + sc.b_off;
+      const double xj = a.b.x[j], yj = a.b.y[j], rj = a.b.r[j];
+      // n = normalized(q_i - q_j) (RigidBody2DSim.cpp:320)
+      const double d0 = xi - xj, d1 = yi - yj;
+      const double dz = d0 * d0 + d1 * d1;
+      double n0 = d0, n1 = d1;
+      if (dz > 0.0) {
+        const double nz = sqrt(dz);
+        n0 = d0 / nz;
+        n1 = d1 / nz;
       }
-    } else {
-      // KinematicCircleCircleConstraint (KinematicObjectCircleConstraint.cpp:121-199)
-      r00 = -ri * n0;
-      r01 = -ri * n1;
-      const double cp0 = xi - ri * n0, cp1 = yi - ri * n1;
-      const double arm0 = cp0 - xj, arm1 = cp1 - yj;
-      const double kv0 = vxj + omj * (-arm1), kv1 = vyj + omj * arm0;
-      vr0 = vxi + omi * (-r01) - kv0;
-      vr1 = vyi + omi * r00 - kv1;
-      w = wi;
-    }
-    // spring history of this pair: the body's previous keys in a few vector
-    // loads, then the matching slot's spring
-    bool had = false;
-    double o0 = 0.0, o1 = 0.0;
-    {
-      const uint4 *kv = reinterpret_cast<const uint4 *>(a.old_s.okey + obase);
-      int hit = -1;
-#pragma unroll
-      for (int q4 = 0; q4 < CAP / 4; ++q4) {
-        const uint4 v = kv[q4];
-        const uint32_t kk[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-        for (int e = 0; e < 4; ++e)
-          if (hit < 0 && 4 * q4 + e < old_n && kk[e] == key) hit = 4 * q4 + e;
+      const double gap = sqrt(d0 * d0 + d1 * d1) - ri - rj;
+      const double depth = gap < 0.0 ? gap : 0.0;
+      const double vxj = double(ld<T>(a.b.vx, j)), vyj = double(ld<T>(a.b.vy, j));
+      const double omj = double(ld<T>(a.b.om, j));
+      const uint32_t sl = obase + uint32_t(rank);
+      double vr0, vr1, r00, r01, r10 = 0.0, r11 = 0.0, w;
+      if (!kin) {
+        // CircleCircleConstraint (CircleCircleConstraint.cpp:171-193, :224-233)
+        const double f = ri / (ri + rj);
+        const double p0 = xi + f * (xj - xi), p1 = yi + f * (yj - yi);
+        r00 = p0 - xi;
+        r01 = p1 - yi;
+        r10 = p0 - xj;
+        r11 = p1 - yj;
+        vr0 = vxi + omi * (-r01) - vxj - omj * (-r11);
+        vr1 = vyi + omi * r00 - vyj - omj * r10;
+        w = sc.weighted ? (wi + double(ld<T>(a.b.w, j))) * 0.5 : 1.0;
+        if (a.cpoint) {
+          a.cpoint[2 * sl] = p0;
+          a.cpoint[2 * sl + 1] = p1;
+        }
+      } else {
+        // KinematicCircleCircleConstraint (KinematicObjectCircleConstraint.cpp:121-199)
+        r00 = -ri * n0;
+        r01 = -ri * n1;
+        const double cp0 = xi - ri * n0, cp1 = yi - ri * n1;
+        const double arm0 = cp0 - xj, arm1 = cp1 - yj;
+        const double kv0 = vxj + omj * (-arm1), kv1 = vyj + omj * arm0;
+        vr0 = vxi + omi * (-r01) - kv0;
+        vr1 = vyi + omi * r00 - kv1;
+        w = wi;
       }
-      if (hit >= 0) {
-        had = true;
-        o0 = double(ld<T>(a.old_s.ods, 2 * (obase + uint32_t(hit))));
-        o1 = double(ld<T>(a.old_s.ods, 2 * (obase + uint32_t(hit)) + 1));
+      // spring history of this pair: the body's previous keys in a few vector
+      // loads, then the matching slot's spring
+      bool had = false;
+      double o0 = 0.0, o1 = 0.0;
+      {
+        const uint4 *kv = reinterpret_cast<const uint4 *>(a.old_s.okey + obase);
+        int hit = -1;
+  #pragma unroll
+        for (int q4 = 0; q4 < CAP / 4; ++q4) {
+          const uint4 v = kv[q4];
+          const uint32_t kk[4] = {v.x, v.y, v.z, v.w};
+  #pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if (hit < 0 && 4 * q4 + e < old_n && kk[e] == key) hit = 4 * q4 + e;
+        }
+        if (hit >= 0) {
+          had = true;
+          o0 = double(ld<T>(a.old_s.ods, 2 * (obase + uint32_t(hit))));
+          o1 = double(ld<T>(a.old_s.ods, 2 * (obase + uint32_t(hit)) + 1));
+        }
       }
-    }
-    double ds0, ds1, F0, F1;
-    spring(had, o0, o1, dt, vr0, vr1, n0, n1, ds0, ds1);
-    penalty_force(sc, w, depth, n0, n1, vr0, vr1, ds0, ds1, F0, F1);
-    a.new_s.okey[sl] = key;
-    st<T>(a.new_s.ods, 2 * sl, T(ds0));
-    st<T>(a.new_s.ods, 2 * sl + 1, T(ds1));
-    st<T>(a.f.oF, 2 * sl, T(F0));
-    st<T>(a.f.oF, 2 * sl + 1, T(F1));
-    st<T>(a.f.otau0, sl, T(r00 * F1 - r01 * F0));
-  }
-  // The contacts owned by a smaller partner j: the owner's computation above
-  // (j as the body, this body as its partner: geometry, spring history from
-  // j's previous slots, force), so that the integration finds the force this
-  // body receives among its own records.  Every operation is the owner's, in
-  // its order, so the values equal the owner's bit for bit.
-  const uint32_t lid = i - sc.b_off;
-  for (int k = lane; k < nin; k += kTeam) {
-    const uint32_t key = s_in[tb][k];
-    int rank = 0;
-    for (int q = 0; q < nin; ++q) rank += (s_in[tb][q] < key) ? 1 : 0;
-    const uint32_t j = key + sc.b_off;
-    const double xo = a.b.x[j], yo = a.b.y[j], ro = a.b.r[j];
-    const double vxo = double(ld<T>(a.b.vx, j)), vyo = double(ld<T>(a.b.vy, j));
-    const double omo = double(ld<T>(a.b.om, j));
-    const double d0 = xo - xi, d1 = yo - yi;
-    const double dz = d0 * d0 + d1 * d1;
-    double n0 = d0, n1 = d1;
-    if (dz > 0.0) {
-      const double nz = sqrt(dz);
-      n0 = d0 / nz;
-      n1 = d1 / nz;
-    }
-    const double gap = sqrt(d0 * d0 + d1 * d1) - ro - ri;
-    const double depth = gap < 0.0 ? gap : 0.0;
-    const double f = ro / (ro + ri);
-    const double p0 = xo + f * (xi - xo), p1 = yo + f * (yi - yo);
-    const double r00 = p0 - xo, r01 = p1 - yo;
-    const double r10 = p0 - xi, r11 = p1 - yi;
-    const double vr0 = vxo + omo * (-r01) - vxi - omi * (-r11);
-    const double vr1 = vyo + omo * r00 - vyi - omi * r10;
-    const double w = sc.weighted ? (double(ld<T>(a.b.w, j)) + wi) * 0.5 : 1.0;
-    bool had = false;
-    double o0 = 0.0, o1 = 0.0;
-    {
-      const uint32_t jbase = j * uint32_t(CAP);
-      const int jold = a.old_s.ocnt[j];
-      const uint4 *kv = reinterpret_cast<const uint4 *>(a.old_s.okey + jbase);
-      int hit = -1;
-#pragma unroll
-      for (int q4 = 0; q4 < CAP / 4; ++q4) {
-        const uint4 v = kv[q4];
-        const uint32_t kk[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-        for (int e = 0; e < 4; ++e)
-          if (hit < 0 && 4 * q4 + e < jold && kk[e] == lid) hit = 4 * q4 + e;
-      }
-      if (hit >= 0) {
-        had = true;
-        o0 = double(ld<T>(a.old_s.ods, 2 * (jbase + uint32_t(hit))));
-        o1 = double(ld<T>(a.old_s.ods, 2 * (jbase + uint32_t(hit)) + 1));
+      double ds0, ds1, F0, F1;
+      spring(had, o0, o1, dt, vr0, vr1, n0, n1, ds0, ds1);
+      penalty_force(sc, w, depth, n0, n1, vr0, vr1, ds0, ds1, F0, F1);
+      a.new_s.okey[sl] = key;
+      st<T>(a.new_s.ods, 2 * sl, T(ds0));
+      st<T>(a.new_s.ods, 2 * sl + 1, T(ds1));
+      st<T>(a.f.oF, 2 * sl, T(F0));
+      st<T>(a.f.oF, 2 * sl + 1, T(F1));
+      const T tau = T(r00 * F1 - r01 * F0);
+      if constexpr (TABLE) {
+        s_of[tb][rank][0] = double(T(F0));
+        s_of[tb][rank][1] = double(T(F1));
+        s_of[tb][rank][2] = double(tau);
+      } else {
+        st<T>(a.f.otau0, sl, tau);
       }
     }
-    double ds0, ds1, F0, F1;
-    spring(had, o0, o1, dt, vr0, vr1, n0, n1, ds0, ds1);
-    penalty_force(sc, w, depth, n0, n1, vr0, vr1, ds0, ds1, F0, F1);
-    const uint32_t sl = i * uint32_t(CAP) + uint32_t(rank);
-    st<T>(a.f.iF, 2 * sl, T(F0));
-    st<T>(a.f.iF, 2 * sl + 1, T(F1));
-    st<T>(a.f.itau, sl, T(r10 * F1 - r11 * F0));
-  }
-  if (lane == 0) {
-    a.new_s.ocnt[i] = uint8_t(nown);
-    a.f.icnt[i] = uint8_t(nin);
+    // The contacts owned by a smaller partner j: the owner's computation above
+    // (j as the body, this body as its partner: geometry, spring history from
+    // j's previous slots, force), so that the integration finds the force this
+    // body receives among its own records.  Every operation is the owner's, in
+    // its order, so the values equal the owner's bit for bit.
+    const uint32_t lid = i - sc.b_off;
+    for (int k = lane; k < nin; k += kTeam) {
+      const uint32_t key = s_in[tb][k];
+      int rank = 0;
+      for (int q = 0; q < nin; ++q) rank += (s_in[tb][q] < key) ? 1 : 0;
+      const uint32_t j = key + sc.b_off;
+      const double xo = a.b.x[j], yo = a.b.y[j], ro = a.b.r[j];
+      const double vxo = double(ld<T>(a.b.vx, j)), vyo = double(ld<T>(a.b.vy, j));
+      const double omo = double(ld<T>(a.b.om, j));
+      const double d0 = xo - xi, d1 = yo - yi;
+      const double dz = d0 * d0 + d1 * d1;
+      double n0 = d0, n1 = d1;
+      if (dz > 0.0) {
+        const double nz = sqrt(dz);
+        n0 = d0 / nz;
+        n1 = d1 / nz;
+      }
+      const double gap = sqrt(d0 * d0 + d1 * d1) - ro - ri;
+      const double depth = gap < 0.0 ? gap : 0.0;
+      const double f = ro / (ro + ri);
+      const double p0 = xo + f * (xi - xo), p1 = yo + f * (yi - yo);
+      const double r00 = p0 - xo, r01 = p1 - yo;
+      const double r10 = p0 - xi, r11 = p1 - yi;
+      const double vr0 = vxo + omo * (-r01) - vxi - omi * (-r11);
+      const double vr1 = vyo + omo * r00 - vyi - omi * r10;
+      const double w = sc.weighted ? (double(ld<T>(a.b.w, j)) + wi) * 0.5 : 1.0;
+      bool had = false;
+      double o0 = 0.0, o1 = 0.0;
+      {
+        const uint32_t jbase = j * uint32_t(CAP);
+        const int jold = a.old_s.ocnt[j];
+        const uint4 *kv = reinterpret_cast<const uint4 *>(a.old_s.okey + jbase);
+        int hit = -1;
+  #pragma unroll
+        for (int q4 = 0; q4 < CAP / 4; ++q4) {
+          const uint4 v = kv[q4];
+          const uint32_t kk[4] = {v.x, v.y, v.z, v.w};
+  #pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if (hit < 0 && 4 * q4 + e < jold && kk[e] == lid) hit = 4 * q4 + e;
+        }
+        if (hit >= 0) {
+          had = true;
+          o0 = double(ld<T>(a.old_s.ods, 2 * (jbase + uint32_t(hit))));
+          o1 = double(ld<T>(a.old_s.ods, 2 * (jbase + uint32_t(hit)) + 1));
+        }
+      }
+      double ds0, ds1, F0, F1;
+      spring(had, o0, o1, dt, vr0, vr1, n0, n1, ds0, ds1);
+      penalty_force(sc, w, depth, n0, n1, vr0, vr1, ds0, ds1, F0, F1);
+      const T tau = T(r10 * F1 - r11 * F0);
+      if constexpr (TABLE) {
+        s_if[tb][rank][0] = double(T(F0));
+        s_if[tb][rank][1] = double(T(F1));
+        s_if[tb][rank][2] = double(tau);
+      } else {
+        const uint32_t sl = i * uint32_t(CAP) + uint32_t(rank);
+        st<T>(a.f.iF, 2 * sl, T(F0));
+        st<T>(a.f.iF, 2 * sl + 1, T(F1));
+        st<T>(a.f.itau, sl, tau);
+      }
+    }
+    if (lane == 0) {
+      a.new_s.ocnt[i] = uint8_t(nown);
+      a.f.icnt[i] = uint8_t(nin);
+    }
+
+    // static planes (RigidBody2DSim.cpp:777-842; StaticPlaneCircleConstraint.cpp)
+    for (int k = lane; k < sc.nplanes; k += kTeam) {
+      const uint32_t ps = i * kMaxPlanes + uint32_t(k);
+      const double pn0 = sc.plane_n[2 * k], pn1 = sc.plane_n[2 * k + 1];
+      const double rel0 = xi - sc.plane_x[2 * k], rel1 = yi - sc.plane_x[2 * k + 1];
+      const double dist = pn0 * rel0 + pn1 * rel1;
+      if (fi || !(dist <= ri)) {
+        a.new_s.pact[ps] = 0;
+        if constexpr (TABLE) s_pon[tb][k] = 0;
+        continue;
+      }
+      const double gap = dist - ri;
+      const double depth = gap < 0.0 ? gap : 0.0;
+      const double r00 = -ri * pn0, r01 = -ri * pn1;
+      const double vr0 = vxi + omi * (-r01) - 0.0;
+      const double vr1 = vyi + omi * r00 - 0.0;
+      const bool had = a.old_s.pact[ps] != 0;
+      const double o0 = had ? double(ld<T>(a.old_s.pds, 2 * ps)) : 0.0;
+      const double o1 = had ? double(ld<T>(a.old_s.pds, 2 * ps + 1)) : 0.0;
+      double ds0, ds1, F0, F1;
+      spring(had, o0, o1, dt, vr0, vr1, pn0, pn1, ds0, ds1);
+      penalty_force(sc, wi, depth, pn0, pn1, vr0, vr1, ds0, ds1, F0, F1);
+      a.new_s.pact[ps] = 1;
+      st<T>(a.new_s.pds, 2 * ps, T(ds0));
+      st<T>(a.new_s.pds, 2 * ps + 1, T(ds1));
+      const T tau = T(r00 * F1 - r01 * F0);
+      if constexpr (TABLE) {
+        s_pon[tb][k] = 1;
+        s_pf[tb][k][0] = double(T(F0));
+        s_pf[tb][k][1] = double(T(F1));
+        s_pf[tb][k][2] = double(tau);
+      } else {
+        st<T>(a.f.pF, 2 * ps, T(F0));
+        st<T>(a.f.pF, 2 * ps + 1, T(F1));
+        st<T>(a.f.ptau, ps, tau);
+      }
+    }
   }
 
-  // static planes (RigidBody2DSim.cpp:777-842; StaticPlaneCircleConstraint.cpp)
-  for (int k = lane; k < sc.nplanes; k += kTeam) {
-    const uint32_t ps = i * kMaxPlanes + uint32_t(k);
-    const double pn0 = sc.plane_n[2 * k], pn1 = sc.plane_n[2 * k + 1];
-    const double rel0 = xi - sc.plane_x[2 * k], rel1 = yi - sc.plane_x[2 * k + 1];
-    const double dist = pn0 * rel0 + pn1 * rel1;
-    if (fi || !(dist <= ri)) {
-      a.new_s.pact[ps] = 0;
-      continue;
+  if constexpr (TABLE) {
+    __syncthreads();  // the team's terms are in LDS
+    if (!active || lane != 0) return;
+    // PenaltyImpactFrictionMap::flow's update (PenaltyImpactFrictionMap.cpp:
+    // 1107-1114) with the forces gathered in the reference's summation order:
+    // gravity, planes, pairs owned by smaller partners, own pairs, kinematic
+    // pairs (the last two in key order) -- dem_integrate_kernel's arithmetic
+    const double m = double(ld<T>(a.b.m, i));
+    const double im = double(ld<T>(a.b.im, i)), iI = double(ld<T>(a.b.iI, i));
+    const double th0 = a.b.th[i];
+    double F0 = 0.0, F1 = 0.0, F2 = 0.0;
+    F0 += m * sc.g0;
+    F1 += m * sc.g1;
+    for (int k = 0; k < sc.nplanes; ++k) {
+      if (!s_pon[tb][k]) continue;
+      F0 += s_pf[tb][k][0];
+      F1 += s_pf[tb][k][1];
+      F2 += s_pf[tb][k][2];
     }
-    const double gap = dist - ri;
-    const double depth = gap < 0.0 ? gap : 0.0;
-    const double r00 = -ri * pn0, r01 = -ri * pn1;
-    const double vr0 = vxi + omi * (-r01) - 0.0;
-    const double vr1 = vyi + omi * r00 - 0.0;
-    const bool had = a.old_s.pact[ps] != 0;
-    const double o0 = had ? double(ld<T>(a.old_s.pds, 2 * ps)) : 0.0;
-    const double o1 = had ? double(ld<T>(a.old_s.pds, 2 * ps + 1)) : 0.0;
-    double ds0, ds1, F0, F1;
-    spring(had, o0, o1, dt, vr0, vr1, pn0, pn1, ds0, ds1);
-    penalty_force(sc, wi, depth, pn0, pn1, vr0, vr1, ds0, ds1, F0, F1);
-    a.new_s.pact[ps] = 1;
-    st<T>(a.new_s.pds, 2 * ps, T(ds0));
-    st<T>(a.new_s.pds, 2 * ps + 1, T(ds1));
-    st<T>(a.f.pF, 2 * ps, T(F0));
-    st<T>(a.f.pF, 2 * ps + 1, T(F1));
-    st<T>(a.f.ptau, ps, T(r00 * F1 - r01 * F0));
+    for (int k = 0; k < nin; ++k) {
+      F0 -= s_if[tb][k][0];
+      F1 -= s_if[tb][k][1];
+      F2 -= s_if[tb][k][2];
+    }
+    for (int q = 0; q < nown; ++q) {
+      F0 += s_of[tb][q][0];
+      F1 += s_of[tb][q][1];
+      F2 += s_of[tb][q][2];
+    }
+    if (fi) F0 = F1 = F2 = 0.0;
+    if (a.F_out) {
+      a.F_out[3 * i] = F0;
+      a.F_out[3 * i + 1] = F1;
+      a.F_out[3 * i + 2] = F2;
+    }
+    const double v0 = vxi + (dt * im) * F0;
+    const double v1 = vyi + (dt * im) * F1;
+    const double v2 = omi + (dt * iI) * F2;
+    const T tv0 = T(v0), tv1 = T(v1), tv2 = T(v2);
+    st<T>(a.bn.vx, i, tv0);
+    st<T>(a.bn.vy, i, tv1);
+    st<T>(a.bn.om, i, tv2);
+    const double x = xi + dt * double(tv0);
+    const double y = yi + dt * double(tv1);
+    a.bn.x[i] = x;
+    a.bn.y[i] = y;
+    a.bn.th[i] = th0 + dt * double(tv2);
+    if (!(isfinite(x) && isfinite(y) && isfinite(v2))) atomicOr(&a.err[sid], 2);
+    table_insert(a.tnext, a.kslots, dem_bucket(sc, x, y), i);
   }
 }
 
@@ -484,10 +583,7 @@ struct IntegrateArgs {
   uint32_t *key, *slot, *counts, *totals;
   int32_t *err;
   double *F_out;  // optional: total force per body (3B)
-  int32_t rekey;  // produce the next step's cell keys
-  int32_t table;  // ... by filing the bodies in tnext (else keys and slots for the sort)
-  TableRaw tnext;
-  int32_t kslots;
+  int32_t rekey;  // produce the next step's cell keys and slots for the sort
 };
 
 // Gathers a body's forces in the reference's summation order, then
@@ -557,10 +653,6 @@ __global__ __launch_bounds__(kThreads) void dem_integrate_kernel(IntegrateArgs a
   if (!(isfinite(x) && isfinite(y) && isfinite(v2))) atomicOr(&a.err[sid], 2);
   if (!a.rekey) return;
   const uint32_t k = dem_bucket(sc, x, y);
-  if (a.table) {
-    table_insert(a.tnext, a.kslots, k, i);
-    return;
-  }
   a.key[i] = k;
   a.slot[i] = atomicAdd(&a.counts[k], 1u);
   hg_scan_block_count(a.totals, k, blockIdx.x);
@@ -635,10 +727,13 @@ static void launch_contacts_t(hg_dem *h) {
   a.cell_start = h->cell_start;
   a.err = h->err;
   a.cpoint = h->cpoint;
-  a.tcur = h->tables[h->cur];
-  a.tnext = h->tables[1 - h->cur];
+  a.tcur = h->tables[h->tph];
+  a.tnext = h->tables[(h->tph + 1) % 3];
+  a.tclr = h->tables[(h->tph + 2) % 3];
   a.kslots = h->kslots;
-  ProfScope ps(h->prof, "dem_contacts", h->stream);
+  a.bn = h->b_alt;
+  a.F_out = h->keep_force ? h->F_out : nullptr;
+  ProfScope ps(h->prof, h->table ? "dem_step" : "dem_contacts", h->stream);
   const dim3 g(a.nblocks_padded), b(kTeamThreads);
   if (h->table) {
     if (h->cap <= 8) hipLaunchKernelGGL((dem_contacts_kernel<T, 8, true>), g, b, 0, h->stream, a);
@@ -669,9 +764,6 @@ static void launch_integrate_t(hg_dem *h, bool rekey) {
   a.err = h->err;
   a.F_out = h->keep_force ? h->F_out : nullptr;
   a.rekey = rekey ? 1 : 0;
-  a.table = h->table ? 1 : 0;
-  a.tnext = h->tables[1 - h->cur];
-  a.kslots = h->kslots;
   ProfScope ps(h->prof, "dem_integrate", h->stream);
   const dim3 g((h->nbodies + kThreads - 1) / kThreads), b(kThreads);
   if (h->cap <= 8) hipLaunchKernelGGL((dem_integrate_kernel<T, 8>), g, b, 0, h->stream, a);
@@ -680,6 +772,22 @@ static void launch_integrate_t(hg_dem *h, bool rekey) {
 }
 
 void hg_dem_launch_step(hg_dem *h, bool rekey) {
+  if (h->table) {
+    // one launch: contacts, integration into b_alt, filing in the next table
+    if (h->nbodies) {
+      if (h->precision == HG_FP64) launch_contacts_t<double>(h);
+      else launch_contacts_t<float>(h);
+    }
+    std::swap(h->b.x, h->b_alt.x);
+    std::swap(h->b.y, h->b_alt.y);
+    std::swap(h->b.th, h->b_alt.th);
+    std::swap(h->b.vx, h->b_alt.vx);
+    std::swap(h->b.vy, h->b_alt.vy);
+    std::swap(h->b.om, h->b_alt.om);
+    h->tph = (h->tph + 1) % 3;
+    h->cur = 1 - h->cur;
+    return;
+  }
   if (h->nbodies) {
     if (h->precision == HG_FP64) {
       launch_contacts_t<double>(h);
@@ -690,13 +798,14 @@ void hg_dem_launch_step(hg_dem *h, bool rekey) {
     }
   }
   h->cur = 1 - h->cur;
-  if (rekey && !h->table) hg_dem_launch_sort(h);
+  if (rekey) hg_dem_launch_sort(h);
 }
 
 void hg_dem_launch_rebin(hg_dem *h) {
   if (h->table) {
-    // both tables emptied, the current one filled from the positions
-    for (int c = 0; c < 2; ++c) {
+    // every table emptied, the one the next step reads filled from the
+    // positions
+    for (int c = 0; c < 3; ++c) {
       hipMemsetAsync(h->tables[c].count, 0, size_t(h->ncell) * 4, h->stream);
       hipMemsetAsync(h->tables[c].novf, 0, 4, h->stream);
     }
@@ -709,7 +818,7 @@ void hg_dem_launch_rebin(hg_dem *h) {
       a.x = h->b.x;
       a.y = h->b.y;
       a.table = 1;
-      a.t = h->tables[h->cur];
+      a.t = h->tables[h->tph];
       a.kslots = h->kslots;
       ProfScope ps(h->prof, "dem_bin", h->stream);
       hipLaunchKernelGGL(dem_bin_kernel, dim3((h->nbodies + kThreads - 1) / kThreads),
@@ -805,6 +914,13 @@ static hg_status upload_bodies(hg_dem *h, const hg_dem_desc *descs) {
   HG_TRY(dalloc(h, &b.vx, B * sizeof(T)));
   HG_TRY(dalloc(h, &b.vy, B * sizeof(T)));
   HG_TRY(dalloc(h, &b.om, B * sizeof(T)));
+  // the other state buffer of the DEM-only steps
+  HG_TRY(dalloc(h, &h->b_alt.x, B * 8));
+  HG_TRY(dalloc(h, &h->b_alt.y, B * 8));
+  HG_TRY(dalloc(h, &h->b_alt.th, B * 8));
+  HG_TRY(dalloc(h, &h->b_alt.vx, B * sizeof(T)));
+  HG_TRY(dalloc(h, &h->b_alt.vy, B * sizeof(T)));
+  HG_TRY(dalloc(h, &h->b_alt.om, B * sizeof(T)));
   HG_TRY(dalloc(h, &b.im, B * sizeof(T)));
   HG_TRY(dalloc(h, &b.iI, B * sizeof(T)));
   HG_TRY(dalloc(h, &b.m, B * sizeof(T)));
@@ -825,6 +941,13 @@ static hg_status upload_bodies(hg_dem *h, const hg_dem_desc *descs) {
     std::vector<T> ones(B, T(1));
     HG_HIP(h->ctx, hipMemcpy(b.w, ones.data(), B * sizeof(T), hipMemcpyHostToDevice));
   }
+  // the fields the steps do not rewrite are shared by both buffers
+  h->b_alt.r = b.r;
+  h->b_alt.im = b.im;
+  h->b_alt.iI = b.iI;
+  h->b_alt.m = b.m;
+  h->b_alt.fixed = b.fixed;
+  h->b_alt.w = b.w;
   for (int k = 0; k < 2; ++k) {
     SlotsRaw &s = h->slots[k];
     HG_TRY(dalloc(h, &s.okey, size_t(B) * h->cap * 4));
@@ -1014,7 +1137,7 @@ static hg_status dem_create_impl(hg_ctx *ctx, const hg_dem_desc *descs, uint32_t
   HG_TRY(dalloc(hp, &h->err, nscenes * 4));
   HG_TRY(dalloc(hp, &h->F_out, size_t(h->nbodies) * 24));
   h->kslots = dem_table_slots(descs, nscenes);
-  for (int c = 0; c < 2; ++c) {
+  for (int c = 0; c < 3; ++c) {
     TableRaw &t = h->tables[c];
     HG_TRY(dalloc(hp, &t.count, size_t(h->ncell) * 4));
     HG_TRY(dalloc(hp, &t.items, size_t(h->ncell) * size_t(h->kslots) * 4));
@@ -1091,24 +1214,29 @@ extern "C" hg_status hg_dem_step_graph(hg_dem *h, double dt, uint32_t nsteps) {
   if (!(dt > 0.0)) return hg_fail(h->ctx, HG_ERR_INVALID, "dt must be > 0");
   if (h->prof.on) return hg_dem_step(h, dt, nsteps);
   HG_TRY(dem_upload_dt(h, dt));
-  if (h->cur != 0 && nsteps) {
+  // The buffers a step uses repeat with a period: the contact slots (and, in
+  // the DEM-only steps, the state buffers) alternate, the tables rotate by
+  // three.  A graph of one period, captured from phase 0, replays from any
+  // later phase 0; steps before and after run without it.
+  const uint32_t period = h->table ? 6u : 2u;
+  auto phase0 = [&]() { return h->cur == 0 && (!h->table || h->tph == 0); };
+  while (!phase0() && nsteps) {
     launch_step(h);
     --nsteps;
   }
-  const uint32_t pairs = nsteps / 2;
-  if (pairs) {
+  const uint32_t reps = nsteps / period;
+  if (reps) {
     if (!h->graph) {
       hipGraph_t g = nullptr;
       HG_HIP(h->ctx, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-      launch_step(h);
-      launch_step(h);
+      for (uint32_t k = 0; k < period; ++k) launch_step(h);
       HG_HIP(h->ctx, hipStreamEndCapture(h->stream, &g));
       HG_HIP(h->ctx, hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0));
       HG_HIP(h->ctx, hipGraphDestroy(g));
     }
-    for (uint32_t k = 0; k < pairs; ++k) HG_HIP(h->ctx, hipGraphLaunch(h->graph, h->stream));
+    for (uint32_t k = 0; k < reps; ++k) HG_HIP(h->ctx, hipGraphLaunch(h->graph, h->stream));
   }
-  if (nsteps & 1u) launch_step(h);
+  for (uint32_t k = 0; k < nsteps % period; ++k) launch_step(h);
   HG_HIP(h->ctx, hipGetLastError());
   return HG_OK;
 }
@@ -1402,7 +1530,7 @@ extern "C" hg_status hg_dem_kernel_times(hg_dem *h, int32_t max_kernels, char *n
 // contact-slot sets, force records, sort keys), for the estimate below.
 static uint64_t dem_bytes_per_body(size_t T, int cap) {
   const uint64_t C = uint64_t(cap);
-  const uint64_t bodies = 4 * 8 + 7 * T + 1;
+  const uint64_t bodies = 4 * 8 + 7 * T + 1 + 3 * 8 + 3 * T;
   const uint64_t slots = C * 4 + C * 2 * T + 1 + uint64_t(kMaxPlanes) + uint64_t(kMaxPlanes) * 2 * T;
   const uint64_t forces = C * 6 * T + uint64_t(kMaxPlanes) * 3 * T + 1;
   return bodies + 2 * slots + forces + 3 * 4 + 24;
@@ -1421,9 +1549,9 @@ extern "C" hg_status hg_dem_estimate_bytes(const hg_dem_desc *descs, uint32_t ns
   }
   const size_t T = descs[0].precision == HG_FP64 ? 8 : 4;
   const uint64_t nscan = (cells + kScanChunk - 1) / kScanChunk;
-  // the two bucket tables: counts and item slots per bucket, bucket and
+  // the three bucket tables: counts and item slots per bucket, bucket and
   // overflow entry per body
-  const uint64_t tables = 2 * (cells * (uint64_t(dem_table_slots(descs, nscenes)) + 1) * 4 + B * 8 + 4);
+  const uint64_t tables = 3 * (cells * (uint64_t(dem_table_slots(descs, nscenes)) + 1) * 4 + B * 8 + 4);
   *bytes = B * dem_bytes_per_body(T, hg_dem_contact_capacity(descs, nscenes)) + cells * 8 + 4 + nscan * 8 * kScanSubTotals +
            tables + uint64_t(nscenes) * (sizeof(SceneDev) + 16) + (1u << 20);
   return HG_OK;

@@ -64,9 +64,10 @@ struct ForceRaw {
 // The broad-phase table of DEM-only steps: each bucket keeps up to kslots
 // body indices in the order the integration's atomics file them; a body that
 // finds its bucket full goes to an overflow list that every body also scans
-// (empty in practice), so no candidate is ever lost.  Two tables alternate
-// with the contact slots: the step that starts with slots[c] current reads
-// table c and files the moved bodies into the other, which it found cleared.
+// (empty in practice), so no candidate is ever lost.  Three tables rotate: a
+// step reads table t (filled by the step before), files the moved bodies in
+// table t + 1 (cleared by the step before) and clears table t + 2 (read by
+// the step before), all in its one launch.
 struct TableRaw {
   uint32_t *count;  // [ncell] bodies filed in each bucket (may exceed kslots)
   uint32_t *items;  // [ncell][kslots]
@@ -111,6 +112,9 @@ struct hg_dem {
   SceneDev *d_scenes = nullptr;
   uint32_t *d_b_off = nullptr;
   BodiesRaw b = {};
+  // DEM-only steps write the new position, angle and velocities here, and the
+  // two swap after the step (b always holds the current state)
+  BodiesRaw b_alt = {};
   SlotsRaw slots[2] = {};
   int cur = 0;  // slots[cur] holds the last step's contacts
   ForceRaw f = {};
@@ -120,7 +124,8 @@ struct hg_dem {
   // fills (no sort per step); the hybrid's grains use the sorted order
   bool table = true;
   int kslots = 8;
-  hgdem::TableRaw tables[2] = {};
+  hgdem::TableRaw tables[3] = {};
+  int tph = 0;  // the table the next step reads
   // per-scan-block totals of the bucket keys that the sort after a step
   // with slots[c] current reads, at scan_totals + c * hg_scan_totals_words(nscan)
   uint32_t *scan_totals = nullptr;
