@@ -150,16 +150,8 @@ struct ContactArgs {
   // kept, the total force per body
   BodiesRaw bn;
   double *F_out;
+  int32_t file;  // file the moved bodies in tnext (else the caller does)
 };
-
-// Files body i in bucket k of table t (and records where it went).
-__device__ __forceinline__ void table_insert(const TableRaw &t, int kslots, uint32_t k,
-                                             uint32_t i) {
-  t.bkey[i] = k;
-  const uint32_t s = atomicAdd(&t.count[k], 1u);
-  if (s < uint32_t(kslots)) t.items[size_t(k) * uint32_t(kslots) + s] = i;
-  else t.ovf[atomicAdd(t.novf, 1u)] = i;
-}
 
 // A team of kTeam lanes per body: the lanes split the body's candidates and
 // then its contacts, so that a body's chain of dependent loads is a fraction
@@ -569,7 +561,7 @@ __global__ __launch_bounds__(kTeamThreads) void dem_contacts_kernel(ContactArgs 
     a.bn.y[i] = y;
     a.bn.th[i] = th0 + dt * double(tv2);
     if (!(isfinite(x) && isfinite(y) && isfinite(v2))) atomicOr(&a.err[sid], 2);
-    table_insert(a.tnext, a.kslots, dem_bucket(sc, x, y), i);
+    if (a.file) table_insert(a.tnext, a.kslots, dem_bucket(sc, x, y), i);
   }
 }
 
@@ -733,6 +725,7 @@ static void launch_contacts_t(hg_dem *h) {
   a.kslots = h->kslots;
   a.bn = h->b_alt;
   a.F_out = h->keep_force ? h->F_out : nullptr;
+  a.file = h->caller_files ? 0 : 1;
   ProfScope ps(h->prof, h->table ? "dem_step" : "dem_contacts", h->stream);
   const dim3 g(a.nblocks_padded), b(kTeamThreads);
   if (h->table) {
@@ -856,6 +849,14 @@ void hg_dem_launch_sort(hg_dem *h) {
 }
 
 static void launch_step(hg_dem *h) { hg_dem_launch_step(h, true); }
+
+void hg_dem_caller_files(hg_dem *h) {
+  h->caller_files = true;
+  if (h->graph) {
+    hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+}
 
 hg_status hg_dem_use_sorted_order(hg_dem *h) {
   if (!h->table) return HG_OK;

@@ -76,6 +76,15 @@ struct TableRaw {
   uint32_t *ovf;    // [B] their indices
 };
 
+// Files body i in bucket k of table t (and records where it went).
+__device__ __forceinline__ void table_insert(const TableRaw &t, int kslots, uint32_t k,
+                                             uint32_t i) {
+  t.bkey[i] = k;
+  const uint32_t s = atomicAdd(&t.count[k], 1u);
+  if (s < uint32_t(kslots)) t.items[size_t(k) * uint32_t(kslots) + s] = i;
+  else t.ovf[atomicAdd(t.novf, 1u)] = i;
+}
+
 // Broad-phase cell of a position (unbounded; clamped to +-2^30 so that any
 // finite or non-finite coordinate gives a valid integer).
 __device__ __forceinline__ void dem_cell_of(const SceneDev &sc, double x, double y, int &cx,
@@ -123,6 +132,9 @@ struct hg_dem {
   // DEM-only handles find candidates in bucket tables that the integration
   // fills (no sort per step); the hybrid's grains use the sorted order
   bool table = true;
+  // the hybrid's grains: the step does not file the bodies, the coupling's
+  // last kernel files them at their final positions (in tables[tph])
+  bool caller_files = false;
   int kslots = 8;
   hgdem::TableRaw tables[3] = {};
   int tph = 0;  // the table the next step reads
@@ -155,5 +167,8 @@ void hg_dem_launch_rebin(hg_dem *h);
 // The hybrid's grains: the counting sort of every step instead of the
 // bucket tables (re-bins the current positions).
 hg_status hg_dem_use_sorted_order(hg_dem *h);
+// The hybrid's grains with the tables: steps leave the filing of the moved
+// bodies to the caller, which files each body in tables[tph] after the step.
+void hg_dem_caller_files(hg_dem *h);
 // The counting sort alone, for a caller that wrote the cell keys and slots.
 void hg_dem_launch_sort(hg_dem *h);

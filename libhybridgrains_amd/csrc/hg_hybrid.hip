@@ -16,15 +16,15 @@
 //
 // Kernels of one step:
 //   continuum stream       | grain stream
-//   P2G + grid update      | DEM contacts, DEM integration
+//   P2G + grid update      | DEM step (contacts and integration, one launch)
 //   grain_mass (coupling test, grain mass and momentum spread)
 //   exchange (also refreshes the node velocity and acceleration P2G wrote
 //   at the nodes it changes)
-//   G2P, MPM sort, tiles   | transfer, finish (coupled grain positions, the
-//                          | grains' broad-phase keys, the next step's
-//                          | pre-step positions, list clean-up), DEM sort,
+//   G2P, MPM sort, tiles   | transfer, finish (coupled grain positions, every
+//                          | grain filed in the DEM's next bucket table, the
+//                          | next step's pre-step positions, list clean-up),
 //                          | homogenisation when on, then the next step's
-//                          | DEM contacts and integration
+//                          | DEM step
 // with events where one stream needs the other's results (FP32, with the
 // fused continuum step: grid update with the exchange, then the fused G2P and
 // the MPM sort beside the grain branch, then the next step's grain mass on
@@ -53,8 +53,9 @@ struct HybArgs {
   double hl;  // grains use the first point's half width (Coupling.cpp:31)
   hgdem::BodiesRaw b;
   uint32_t nbodies;
-  const hgdem::SceneDev *dsc;  // DEM scene 0 (broad-phase keys)
-  uint32_t *dkey, *dslot, *dcounts, *dtotals;
+  const hgdem::SceneDev *dsc;  // DEM scene 0 (broad-phase buckets)
+  hgdem::TableRaw dtab;        // the bucket table the next DEM step reads
+  int32_t kslots;
   double *prex, *prey;
   uint8_t *coupled;
   GrainGrid g;
@@ -305,10 +306,10 @@ __global__ __launch_bounds__(kThreads) void hyb_transfer(HybArgs a) {
 }
 
 // End of the step: HybridCoupling2D::updatePositions_node_node
-// (Coupling.cpp:581-591) for the coupled grains, every grain's broad-phase
-// key and slot for the DEM sort and its position for the next step's
-// coupling, and the clean-up of the touched nodes and of the next step's
-// list counter.
+// (Coupling.cpp:581-591) for the coupled grains, every grain filed in the
+// bucket table of the next DEM step at its final position, its position for
+// the next step's coupling, and the clean-up of the touched nodes and of the
+// next step's list counter.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void hyb_finish(HybArgs a, double dt) {
   const uint32_t tid = blockIdx.x * kThreads + threadIdx.x;
@@ -324,10 +325,7 @@ __global__ __launch_bounds__(kThreads) void hyb_finish(HybArgs a, double dt) {
     }
     a.prex[b] = x;
     a.prey[b] = y;
-    const uint32_t k = hgdem::dem_bucket(*a.dsc, x, y);
-    a.dkey[b] = k;
-    a.dslot[b] = atomicAdd(&a.dcounts[k], 1u);
-    hg_scan_block_count(a.dtotals, k, blockIdx.x);
+    hgdem::table_insert(a.dtab, a.kslots, hgdem::dem_bucket(*a.dsc, x, y), b);
   }
   const uint32_t cnt = min(*a.count, a.g.max_list);
   for (uint32_t k = tid; k < cnt; k += gridDim.x * kThreads) {
@@ -484,12 +482,10 @@ static HybArgs make_args(hg_hybrid *h) {
   a.b = h->dem->b;
   a.nbodies = h->dem->nbodies;
   a.dsc = h->dem->d_scenes;
-  a.dkey = h->dem->key;
-  a.dslot = h->dem->slot;
-  a.dcounts = h->dem->counts;
-  // read by the DEM sort after this step's finish (the DEM step has flipped
-  // its slots already)
-  a.dtotals = h->dem->scan_totals + size_t(h->dem->cur) * hg_scan_totals_words(h->dem->nscan);
+  // the table the DEM step after this one reads (this step's DEM step has
+  // advanced the rotation already)
+  a.dtab = h->dem->tables[h->dem->tph];
+  a.kslots = h->dem->kslots;
   a.prex = h->prex;
   a.prey = h->prey;
   a.coupled = h->coupled;
@@ -567,9 +563,6 @@ static void launch_hybrid_steps(hg_hybrid *h, double dt, uint32_t n, bool concur
       ProfScope ps(prof, "hyb_finish", side);
       hipLaunchKernelGGL(hyb_finish<T>, gb, tb, 0, side, a, dt);
     }
-    h->dem->stream = side;
-    hg_dem_launch_sort(h->dem);
-    h->dem->stream = s;
     homog(a);
   };
   if (hg_mpm_is_fused(h->mpm)) {
@@ -676,12 +669,9 @@ extern "C" hg_status hg_hybrid_create(hg_ctx *ctx, const hg_mpm_desc *mpm_desc,
     hg_mpm_destroy(h->mpm);
     return st;
   }
-  // the grains' keys come from the finish kernel, into the sorted order
-  if ((st = hg_dem_use_sorted_order(h->dem)) != HG_OK) {
-    hg_dem_destroy(h->dem);
-    hg_mpm_destroy(h->mpm);
-    return st;
-  }
+  // the grains are filed in the DEM's bucket tables by the finish kernel, at
+  // their final positions
+  hg_dem_caller_files(h->dem);
   // one stream for both: the phases interleave
   HG_HIP(ctx, hipStreamSynchronize(h->dem->stream));
   HG_HIP(ctx, hipStreamDestroy(h->dem->stream));
@@ -783,22 +773,26 @@ extern "C" hg_status hg_hybrid_step_graph(hg_hybrid *h, double dt, uint32_t nste
     hipGraphExecDestroy(h->graph);
     h->graph = nullptr;
   }
-  // The solvers and the list counters alternate every step, so a graph of two
-  // steps captured with the MPM points in buffer 0 and the list counter 0
-  // replays from any later state with both back at 0.  (Stepping the MPM
-  // handle alone in between breaks that alignment; the steps then run
-  // without the graph.)
-  if ((h->mpm->cur != 0 || h->par != 0) && nsteps) {
+  // The solvers' buffers and the list counters alternate every step and the
+  // DEM's bucket tables rotate by three, so a graph of six steps captured
+  // with all of them at phase 0 replays from any later phase 0.  (Stepping
+  // the MPM handle alone in between breaks that alignment; the steps then
+  // run without the graph.)
+  constexpr uint32_t kPeriod = 6;
+  auto phase0 = [&]() {
+    return h->mpm->cur == 0 && h->par == 0 && h->dem->cur == 0 && h->dem->tph == 0;
+  };
+  for (uint32_t k = 0; k < kPeriod - 1 && !phase0() && nsteps; ++k) {
     hybrid_steps(h, dt, 1, true);
     --nsteps;
   }
-  const bool aligned = h->mpm->cur == 0 && h->par == 0;
-  const uint32_t pairs = aligned ? nsteps / 2 : 0;
+  const bool aligned = phase0();
+  const uint32_t reps = aligned ? nsteps / kPeriod : 0;
   if (!aligned) {
     hybrid_steps(h, dt, nsteps, true);
     nsteps = 0;
   }
-  if (pairs) {
+  if (reps) {
     if (!h->graph) {
       // the MPM dt upload synchronises; do it before capturing
       HG_TRY(hg_mpm_step(h->mpm, dt, 0));
@@ -806,8 +800,7 @@ extern "C" hg_status hg_hybrid_step_graph(hg_hybrid *h, double dt, uint32_t nste
       hg_mpm_prime(h->mpm);
       hipGraph_t g = nullptr;
       HG_HIP(h->ctx, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-      hybrid_steps(h, dt, 1, false);
-      hybrid_steps(h, dt, 1, false);
+      for (uint32_t k = 0; k < kPeriod; ++k) hybrid_steps(h, dt, 1, false);
       HG_HIP(h->ctx, hipStreamEndCapture(h->stream, &g));
       HG_HIP(h->ctx, hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0));
       HG_HIP(h->ctx, hipGraphDestroy(g));
@@ -818,9 +811,9 @@ extern "C" hg_status hg_hybrid_step_graph(hg_hybrid *h, double dt, uint32_t nste
       h->graph_homog_vel = h->homog_vel;
       h->graph_hfac_on = h->hfac_on;
     }
-    for (uint32_t k = 0; k < pairs; ++k) HG_HIP(h->ctx, hipGraphLaunch(h->graph, h->stream));
+    for (uint32_t k = 0; k < reps; ++k) HG_HIP(h->ctx, hipGraphLaunch(h->graph, h->stream));
   }
-  if (nsteps & 1u) hybrid_steps(h, dt, 1, true);
+  if (nsteps % kPeriod) hybrid_steps(h, dt, nsteps % kPeriod, true);
   HG_HIP(h->ctx, hipGetLastError());
   return HG_OK;
 }
