@@ -15,8 +15,8 @@
 //    with room to spread, so the buckets of a compact scene are its cells;
 //    cells a table width apart share a bucket, which only adds candidates
 //    the distance test rejects (the 3x3 neighbour buckets stay distinct).
-//    DEM-only steps file the moved bodies in bucket tables (three rotating);
-//    the hybrid's grains are counting-sorted by bucket every step (hg_sort).
+//    Each step files the moved bodies in the next of three rotating bucket
+//    tables (up to kslots per bucket, an overflow list beyond).
 //  * contacts: each contact is owned by one body -- the smaller index of a
 //    circle pair, or the simulated body of a pair with a fixed body, as the
 //    reference keys its CollisionTypeCache.  A body keeps up to cap owned
@@ -25,17 +25,16 @@
 //    sorted by partner key, double-buffered so that the next step's
 //    contacts join their spring history by partner; per-plane slots likewise.
 //
-// Kernels of one step: dem_contacts (a team of lanes per body: narrow phase
-// over the 3x3 cells, history join, penalty force with the Coulomb projection
-// of the spring for the body's own contacts and for those a smaller partner
-// owns, plane contacts), then symplectic Euler with the forces gathered in the
+// A step is one launch, dem_step_kernel: a team of lanes per body does the
+// narrow phase over the 3x3 cells, the history join and the penalty force
+// with the Coulomb projection of the spring for the body's own contacts and
+// for those a smaller partner owns, and the plane contacts; its first lane
+// then integrates (symplectic Euler) with the forces gathered in the
 // reference's summation order -- gravity, planes, pairs owned by smaller
-// partners, own pairs, kinematic pairs.  DEM-only steps do it all in one
-// launch (the team's first lane integrates into the other state buffer and
-// files the body in the next table); the hybrid's grains run dem_integrate
-// (thread per body, new cell key) and the counting sort after dem_contacts.
+// partners, own pairs, kinematic pairs -- into the other state buffer and
+// files the body in the next table (the hybrid's coupling files its grains
+// itself, at their coupled positions).
 #include "hg_dem_handle.h"
-#include "hg_sort.h"
 
 #include <algorithm>
 #include <cmath>
@@ -138,16 +137,14 @@ struct ContactArgs {
   BodiesRaw b;
   SlotsRaw old_s, new_s;
   ForceRaw f;
-  const uint32_t *perm, *cell_start;
   int32_t *err;
   double *cpoint;  // optional: contact point per owned slot (circle pairs)
-  // DEM-only steps (the tables): the table read, the one the moved bodies
-  // are filed in, and the one cleared for the step after (read last step)
+  // the table read, the one the moved bodies are filed in, and the one
+  // cleared for the step after (read by the step before)
   TableRaw tcur, tnext, tclr;
   int32_t kslots;
-  // DEM-only steps integrate in the same launch: the state after the step
-  // (position, angle and velocities; the other fields are b's) and, when
-  // kept, the total force per body
+  // the state after the step (position, angle and velocities; the other
+  // fields are b's) and, when kept, the total force per body
   BodiesRaw bn;
   double *F_out;
   int32_t file;  // file the moved bodies in tnext (else the caller does)
@@ -157,31 +154,28 @@ struct ContactArgs {
 // then its contacts, so that a body's chain of dependent loads is a fraction
 // of what one lane would walk.  Contacts are collected unordered in LDS and
 // each takes the slot of its rank among the body's keys, which is the order
-// of the partner keys, as a sorted list would have it.
-// Lanes per body: 8 for the DEM-only tables, 4 for the sorted order of the
-// hybrid's grains (timed on the 20k column and the 1M hybrid scene).
-template <bool TABLE> constexpr int team_lanes() { return TABLE ? 8 : 4; }
+// of the partner keys, as a sorted list would have it.  The team then sums
+// the body's terms (rounded to T as stored) in the reference's order in its
+// first lane and integrates the body into bn (the state this launch does not
+// read), filing it in the next table unless the caller does (the hybrid's
+// grains, filed at their coupled positions).  Each body's team forms every
+// contact term the body receives, so no body waits for another's: the step
+// is one launch.  Eight lanes per body (timed against four on the 20k column
+// and the 1M hybrid scene's grains).
+constexpr int kTeam = 8;
 constexpr int kTeamThreads = 256;
-template <bool TABLE> constexpr int team_bodies() { return kTeamThreads / team_lanes<TABLE>(); }
+constexpr int kTeamBodies = kTeamThreads / kTeam;
 
-// With the tables (DEM-only handles) the same launch also ends the step: the
-// team's force terms (rounded to T as stored) go to LDS, and its first lane
-// sums them in the reference's order and integrates the body into bn (the
-// state this launch does not read), then files it in the next table.  Each
-// body's team forms every contact term the body receives, so no body waits
-// for another's.
-template <typename T, int CAP, bool TABLE>
-__global__ __launch_bounds__(kTeamThreads) void dem_contacts_kernel(ContactArgs a) {
-  constexpr int kTeam = team_lanes<TABLE>();
-  constexpr int kTeamBodies = team_bodies<TABLE>();
+template <typename T, int CAP>
+__global__ __launch_bounds__(kTeamThreads) void dem_step_kernel(ContactArgs a) {
   __shared__ uint32_t s_own[kTeamBodies][CAP];
   __shared__ uint32_t s_in[kTeamBodies][CAP];
   __shared__ int s_nown[kTeamBodies], s_nin[kTeamBodies];
-  // the terms the integration sums (tables only): own and incoming pairs by
-  // slot, planes by index (with their activity)
-  constexpr int TB = TABLE ? kTeamBodies : 1, TC = TABLE ? CAP : 1, TP = TABLE ? kMaxPlanes : 1;
-  __shared__ double s_of[TB][TC][3], s_if[TB][TC][3], s_pf[TB][TP][3];
-  __shared__ uint8_t s_pon[TB][TP];
+  // the terms the integration sums: own and incoming pairs by slot, planes
+  // by index (with their activity)
+  __shared__ double s_of[kTeamBodies][CAP][3], s_if[kTeamBodies][CAP][3];
+  __shared__ double s_pf[kTeamBodies][kMaxPlanes][3];
+  __shared__ uint8_t s_pon[kTeamBodies][kMaxPlanes];
   const int tb = int(threadIdx.x) / kTeam, lane = int(threadIdx.x) % kTeam;
   const uint32_t blk = hg_xcd_remap(blockIdx.x, a.nblocks_padded);
   const uint32_t s = blk * uint32_t(kTeamBodies) + uint32_t(tb);
@@ -191,21 +185,18 @@ __global__ __launch_bounds__(kTeamThreads) void dem_contacts_kernel(ContactArgs 
     s_nin[tb] = 0;
   }
   __syncthreads();
-  // teams take the bodies in cell order, or in creation order with the
-  // tables (no sort keeps an order then)
-  const uint32_t i = active ? (TABLE ? s : a.perm[s]) : 0u;
+  // teams take the bodies in creation order
+  const uint32_t i = active ? s : 0u;
   const uint32_t sid = active ? scene_of(a.b_off, a.nscenes, i) : 0u;
   const SceneDev &sc = a.scenes[sid];
   const double xi = active ? a.b.x[i] : 0.0, yi = active ? a.b.y[i] : 0.0;
   const double ri = active ? a.b.r[i] : 0.0;
   const bool fi = active ? a.b.fixed[i] != 0 : true;
-  if constexpr (TABLE) {
-    // the table the next step files into: every bucket it holds bodies in is
-    // some body's recorded bucket
-    if (active && lane == 0) a.tclr.count[a.tclr.bkey[i]] = 0u;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *a.tclr.novf = 0u;
-  }
-  if (active && TABLE) {
+  // the table the next step files into: every bucket it holds bodies in is
+  // some body's recorded bucket
+  if (active && lane == 0) a.tclr.count[a.tclr.bkey[i]] = 0u;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *a.tclr.novf = 0u;
+  if (active) {
     const bool finite_i = isfinite(xi) && isfinite(yi);
     int cx, cy;
     dem_cell_of(sc, xi, yi, cx, cy);
@@ -253,57 +244,7 @@ __global__ __launch_bounds__(kTeamThreads) void dem_contacts_kernel(ContactArgs 
       }
     }
   }
-  if (active && !TABLE) {
-    // A body whose state is no longer finite (flagged by the integration that
-    // produced it) looks for no partners: every such body sorts into one
-    // bucket, and scanning it would cost the square of their number.
-    const bool finite_i = isfinite(xi) && isfinite(yi);
-    int cx, cy;
-    dem_cell_of(sc, xi, yi, cx, cy);
-    const uint32_t *cs = a.cell_start + sc.cell_off;
-    const uint32_t wx = uint32_t(sc.ncx), bx0 = uint32_t(cx - 1) & (wx - 1u);
-    // the buckets of the 3x3 cells: per row one run of three consecutive
-    // buckets, or two runs where the row wraps (the second is empty otherwise)
-    const uint32_t len0 = min(bx0 + 3u, wx) - bx0;
-    uint32_t rb[6], rn[6], tot = 0;
-#pragma unroll
-    for (int r = 0; r < 6; ++r) {
-      const int oy = r / 2 - 1, part = r % 2;
-      const uint32_t row = (uint32_t(cy + oy) & uint32_t(sc.ncy - 1)) * wx;
-      const uint32_t k0 = part ? row : row + bx0;
-      const uint32_t k1 = part ? row + (3u - len0) : row + bx0 + len0;
-      rb[r] = finite_i ? cs[k0] : 0u;
-      rn[r] = finite_i ? cs[k1] - rb[r] : 0u;
-      tot += rn[r];
-    }
-    for (uint32_t t = uint32_t(lane); t < tot; t += uint32_t(kTeam)) {
-      uint32_t off = t;
-      int r = 0;
-      while (off >= rn[r]) {
-        off -= rn[r];
-        ++r;
-      }
-      const uint32_t j = a.perm[rb[r] + off];
-      if (j == i) continue;
-      const bool fj = a.b.fixed[j] != 0;
-      if (fi && fj) continue;
-      const double dx = xi - a.b.x[j], dy = yi - a.b.y[j];
-      // a partner that left the finite range is no candidate
-      if (!isfinite(dx) || !isfinite(dy)) continue;
-      const double rs = ri + a.b.r[j];
-      const double c0 = (dx * dx + dy * dy) - rs * rs;
-      if (!(c0 <= 0.0)) continue;
-      if (!fi && (fj || i < j)) {
-        const int q = atomicAdd(&s_nown[tb], 1);
-        if (q < CAP) s_own[tb][q] = (fj ? kKinematic : 0u) | (j - sc.b_off);
-      } else if (!fi && !fj) {
-        const int q = atomicAdd(&s_nin[tb], 1);
-        if (q < CAP) s_in[tb][q] = j - sc.b_off;
-      }
-    }
-  }
   __syncthreads();
-  if (!TABLE && !active) return;
   const int cnt_own = active ? s_nown[tb] : 0, cnt_in = active ? s_nin[tb] : 0;
   if (lane == 0 && (cnt_own > CAP || cnt_in > CAP)) atomicOr(&a.err[sid], 4);
   const int nown = min(cnt_own, CAP), nin = min(cnt_in, CAP);
@@ -393,14 +334,9 @@ __global__ __launch_bounds__(kTeamThreads) void dem_contacts_kernel(ContactArgs 
       st<T>(a.new_s.ods, 2 * sl + 1, T(ds1));
       st<T>(a.f.oF, 2 * sl, T(F0));
       st<T>(a.f.oF, 2 * sl + 1, T(F1));
-      const T tau = T(r00 * F1 - r01 * F0);
-      if constexpr (TABLE) {
-        s_of[tb][rank][0] = double(T(F0));
-        s_of[tb][rank][1] = double(T(F1));
-        s_of[tb][rank][2] = double(tau);
-      } else {
-        st<T>(a.f.otau0, sl, tau);
-      }
+      s_of[tb][rank][0] = double(T(F0));
+      s_of[tb][rank][1] = double(T(F1));
+      s_of[tb][rank][2] = double(T(r00 * F1 - r01 * F0));
     }
     // The contacts owned by a smaller partner j: the owner's computation above
     // (j as the body, this body as its partner: geometry, spring history from
@@ -457,22 +393,11 @@ __global__ __launch_bounds__(kTeamThreads) void dem_contacts_kernel(ContactArgs 
       double ds0, ds1, F0, F1;
       spring(had, o0, o1, dt, vr0, vr1, n0, n1, ds0, ds1);
       penalty_force(sc, w, depth, n0, n1, vr0, vr1, ds0, ds1, F0, F1);
-      const T tau = T(r10 * F1 - r11 * F0);
-      if constexpr (TABLE) {
-        s_if[tb][rank][0] = double(T(F0));
-        s_if[tb][rank][1] = double(T(F1));
-        s_if[tb][rank][2] = double(tau);
-      } else {
-        const uint32_t sl = i * uint32_t(CAP) + uint32_t(rank);
-        st<T>(a.f.iF, 2 * sl, T(F0));
-        st<T>(a.f.iF, 2 * sl + 1, T(F1));
-        st<T>(a.f.itau, sl, tau);
-      }
+      s_if[tb][rank][0] = double(T(F0));
+      s_if[tb][rank][1] = double(T(F1));
+      s_if[tb][rank][2] = double(T(r10 * F1 - r11 * F0));
     }
-    if (lane == 0) {
-      a.new_s.ocnt[i] = uint8_t(nown);
-      a.f.icnt[i] = uint8_t(nin);
-    }
+    if (lane == 0) a.new_s.ocnt[i] = uint8_t(nown);
 
     // static planes (RigidBody2DSim.cpp:777-842; StaticPlaneCircleConstraint.cpp)
     for (int k = lane; k < sc.nplanes; k += kTeam) {
@@ -482,7 +407,7 @@ __global__ __launch_bounds__(kTeamThreads) void dem_contacts_kernel(ContactArgs 
       const double dist = pn0 * rel0 + pn1 * rel1;
       if (fi || !(dist <= ri)) {
         a.new_s.pact[ps] = 0;
-        if constexpr (TABLE) s_pon[tb][k] = 0;
+        s_pon[tb][k] = 0;
         continue;
       }
       const double gap = dist - ri;
@@ -499,155 +424,61 @@ __global__ __launch_bounds__(kTeamThreads) void dem_contacts_kernel(ContactArgs 
       a.new_s.pact[ps] = 1;
       st<T>(a.new_s.pds, 2 * ps, T(ds0));
       st<T>(a.new_s.pds, 2 * ps + 1, T(ds1));
-      const T tau = T(r00 * F1 - r01 * F0);
-      if constexpr (TABLE) {
-        s_pon[tb][k] = 1;
-        s_pf[tb][k][0] = double(T(F0));
-        s_pf[tb][k][1] = double(T(F1));
-        s_pf[tb][k][2] = double(tau);
-      } else {
-        st<T>(a.f.pF, 2 * ps, T(F0));
-        st<T>(a.f.pF, 2 * ps + 1, T(F1));
-        st<T>(a.f.ptau, ps, tau);
-      }
+      s_pon[tb][k] = 1;
+      s_pf[tb][k][0] = double(T(F0));
+      s_pf[tb][k][1] = double(T(F1));
+      s_pf[tb][k][2] = double(T(r00 * F1 - r01 * F0));
     }
   }
 
-  if constexpr (TABLE) {
-    __syncthreads();  // the team's terms are in LDS
-    if (!active || lane != 0) return;
-    // PenaltyImpactFrictionMap::flow's update (PenaltyImpactFrictionMap.cpp:
-    // 1107-1114) with the forces gathered in the reference's summation order:
-    // gravity, planes, pairs owned by smaller partners, own pairs, kinematic
-    // pairs (the last two in key order) -- dem_integrate_kernel's arithmetic
-    const double m = double(ld<T>(a.b.m, i));
-    const double im = double(ld<T>(a.b.im, i)), iI = double(ld<T>(a.b.iI, i));
-    const double th0 = a.b.th[i];
-    double F0 = 0.0, F1 = 0.0, F2 = 0.0;
-    F0 += m * sc.g0;
-    F1 += m * sc.g1;
-    for (int k = 0; k < sc.nplanes; ++k) {
-      if (!s_pon[tb][k]) continue;
-      F0 += s_pf[tb][k][0];
-      F1 += s_pf[tb][k][1];
-      F2 += s_pf[tb][k][2];
-    }
-    for (int k = 0; k < nin; ++k) {
-      F0 -= s_if[tb][k][0];
-      F1 -= s_if[tb][k][1];
-      F2 -= s_if[tb][k][2];
-    }
-    for (int q = 0; q < nown; ++q) {
-      F0 += s_of[tb][q][0];
-      F1 += s_of[tb][q][1];
-      F2 += s_of[tb][q][2];
-    }
-    if (fi) F0 = F1 = F2 = 0.0;
-    if (a.F_out) {
-      a.F_out[3 * i] = F0;
-      a.F_out[3 * i + 1] = F1;
-      a.F_out[3 * i + 2] = F2;
-    }
-    const double v0 = vxi + (dt * im) * F0;
-    const double v1 = vyi + (dt * im) * F1;
-    const double v2 = omi + (dt * iI) * F2;
-    const T tv0 = T(v0), tv1 = T(v1), tv2 = T(v2);
-    st<T>(a.bn.vx, i, tv0);
-    st<T>(a.bn.vy, i, tv1);
-    st<T>(a.bn.om, i, tv2);
-    const double x = xi + dt * double(tv0);
-    const double y = yi + dt * double(tv1);
-    a.bn.x[i] = x;
-    a.bn.y[i] = y;
-    a.bn.th[i] = th0 + dt * double(tv2);
-    if (!(isfinite(x) && isfinite(y) && isfinite(v2))) atomicOr(&a.err[sid], 2);
-    if (a.file) table_insert(a.tnext, a.kslots, dem_bucket(sc, x, y), i);
-  }
-}
-
-struct IntegrateArgs {
-  const SceneDev *scenes;
-  const uint32_t *b_off;
-  uint32_t nscenes, nbodies;
-  BodiesRaw b;
-  SlotsRaw s;  // the slots this step wrote
-  ForceRaw f;
-  uint32_t *key, *slot, *counts, *totals;
-  int32_t *err;
-  double *F_out;  // optional: total force per body (3B)
-  int32_t rekey;  // produce the next step's cell keys and slots for the sort
-};
-
-// Gathers a body's forces in the reference's summation order, then
-// v1 = v0 + dt M^-1 F, q1 = q0 + dt v1 (PenaltyImpactFrictionMap.cpp:1107-1114).
-// One thread per body.  The body's own state is loaded first, so that its
-// loads overlap the force gather, and the search of a partner's sorted slots
-// for this body loads all of the partner's keys at once.
-template <typename T, int CAP>
-__global__ __launch_bounds__(kThreads) void dem_integrate_kernel(IntegrateArgs a) {
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  if (i >= a.nbodies) return;
-  const uint32_t sid = scene_of(a.b_off, a.nscenes, i);
-  const SceneDev &sc = a.scenes[sid];
+  __syncthreads();  // the team's terms are in LDS
+  if (!active || lane != 0) return;
+  // PenaltyImpactFrictionMap::flow's update (PenaltyImpactFrictionMap.cpp:
+  // 1107-1114) with the forces gathered in the reference's summation order:
+  // gravity, planes, pairs owned by smaller partners, own pairs, kinematic
+  // pairs (the last two in key order)
   const double m = double(ld<T>(a.b.m, i));
   const double im = double(ld<T>(a.b.im, i)), iI = double(ld<T>(a.b.iI, i));
-  const double vx0 = double(ld<T>(a.b.vx, i)), vy0 = double(ld<T>(a.b.vy, i));
-  const double om0 = double(ld<T>(a.b.om, i));
-  const double x0 = a.b.x[i], y0 = a.b.y[i], th0 = a.b.th[i];
-  const bool fixed = a.b.fixed[i] != 0;
+  const double th0 = a.b.th[i];
   double F0 = 0.0, F1 = 0.0, F2 = 0.0;
   F0 += m * sc.g0;
   F1 += m * sc.g1;
   for (int k = 0; k < sc.nplanes; ++k) {
-    const uint32_t ps = i * kMaxPlanes + uint32_t(k);
-    if (!a.s.pact[ps]) continue;
-    F0 += double(ld<T>(a.f.pF, 2 * ps));
-    F1 += double(ld<T>(a.f.pF, 2 * ps + 1));
-    F2 += double(ld<T>(a.f.ptau, ps));
+    if (!s_pon[tb][k]) continue;
+    F0 += s_pf[tb][k][0];
+    F1 += s_pf[tb][k][1];
+    F2 += s_pf[tb][k][2];
   }
-  // pairs owned by a smaller partner, in partner order (the owner's force,
-  // received negated, and the torque of its negation, as the contacts kernel
-  // recorded them among this body's records)
-  const int nin = a.f.icnt[i];
   for (int k = 0; k < nin; ++k) {
-    const uint32_t sl = i * uint32_t(CAP) + uint32_t(k);
-    F0 -= double(ld<T>(a.f.iF, 2 * sl));
-    F1 -= double(ld<T>(a.f.iF, 2 * sl + 1));
-    F2 -= double(ld<T>(a.f.itau, sl));
+    F0 -= s_if[tb][k][0];
+    F1 -= s_if[tb][k][1];
+    F2 -= s_if[tb][k][2];
   }
-  // own pairs, then kinematic pairs (the key order)
-  const int nown = a.s.ocnt[i];
   for (int q = 0; q < nown; ++q) {
-    const uint32_t sl = i * uint32_t(CAP) + uint32_t(q);
-    F0 += double(ld<T>(a.f.oF, 2 * sl));
-    F1 += double(ld<T>(a.f.oF, 2 * sl + 1));
-    F2 += double(ld<T>(a.f.otau0, sl));
+    F0 += s_of[tb][q][0];
+    F1 += s_of[tb][q][1];
+    F2 += s_of[tb][q][2];
   }
-  if (fixed) F0 = F1 = F2 = 0.0;
+  if (fi) F0 = F1 = F2 = 0.0;
   if (a.F_out) {
     a.F_out[3 * i] = F0;
     a.F_out[3 * i + 1] = F1;
     a.F_out[3 * i + 2] = F2;
   }
-  const double dt = sc.dt;
-  const double v0 = vx0 + (dt * im) * F0;
-  const double v1 = vy0 + (dt * im) * F1;
-  const double v2 = om0 + (dt * iI) * F2;
+  const double v0 = vxi + (dt * im) * F0;
+  const double v1 = vyi + (dt * im) * F1;
+  const double v2 = omi + (dt * iI) * F2;
   const T tv0 = T(v0), tv1 = T(v1), tv2 = T(v2);
-  st<T>(a.b.vx, i, tv0);
-  st<T>(a.b.vy, i, tv1);
-  st<T>(a.b.om, i, tv2);
-  const double x = x0 + dt * double(tv0);
-  const double y = y0 + dt * double(tv1);
-  a.b.x[i] = x;
-  a.b.y[i] = y;
-  a.b.th[i] = th0 + dt * double(tv2);
+  st<T>(a.bn.vx, i, tv0);
+  st<T>(a.bn.vy, i, tv1);
+  st<T>(a.bn.om, i, tv2);
+  const double x = xi + dt * double(tv0);
+  const double y = yi + dt * double(tv1);
+  a.bn.x[i] = x;
+  a.bn.y[i] = y;
+  a.bn.th[i] = th0 + dt * double(tv2);
   if (!(isfinite(x) && isfinite(y) && isfinite(v2))) atomicOr(&a.err[sid], 2);
-  if (!a.rekey) return;
-  const uint32_t k = dem_bucket(sc, x, y);
-  a.key[i] = k;
-  a.slot[i] = atomicAdd(&a.counts[k], 1u);
-  hg_scan_block_count(a.totals, k, blockIdx.x);
+  if (a.file) table_insert(a.tnext, a.kslots, dem_bucket(sc, x, y), i);
 }
 
 struct BinArgs {
@@ -655,24 +486,16 @@ struct BinArgs {
   const uint32_t *b_off;
   uint32_t nscenes, nbodies;
   const double *x, *y;
-  uint32_t *key, *slot, *counts, *totals;
-  int32_t table;  // file the bodies in t instead
   TableRaw t;
   int32_t kslots;
 };
 
+// Files every body at its position in table t.
 __global__ __launch_bounds__(kThreads) void dem_bin_kernel(BinArgs a) {
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= a.nbodies) return;
   const SceneDev &sc = a.scenes[scene_of(a.b_off, a.nscenes, i)];
-  const uint32_t k = dem_bucket(sc, a.x[i], a.y[i]);
-  if (a.table) {
-    table_insert(a.t, a.kslots, k, i);
-    return;
-  }
-  a.key[i] = k;
-  a.slot[i] = atomicAdd(&a.counts[k], 1u);
-  hg_scan_block_count(a.totals, k, blockIdx.x);
+  table_insert(a.t, a.kslots, dem_bucket(sc, a.x[i], a.y[i]), i);
 }
 
 }  // namespace hgdem
@@ -703,20 +526,17 @@ static hg_status dalloc(hg_dem *h, P **out, size_t bytes) {
   } while (0)
 
 template <typename T>
-static void launch_contacts_t(hg_dem *h) {
+static void launch_step_t(hg_dem *h) {
   ContactArgs a;
   a.scenes = h->d_scenes;
   a.b_off = h->d_b_off;
   a.nscenes = h->nscenes;
   a.nbodies = h->nbodies;
-  const uint32_t per_block = h->table ? team_bodies<true>() : team_bodies<false>();
-  a.nblocks_padded = hg_pad8((h->nbodies + per_block - 1) / per_block);
+  a.nblocks_padded = hg_pad8((h->nbodies + kTeamBodies - 1) / kTeamBodies);
   a.b = h->b;
   a.old_s = h->slots[h->cur];
   a.new_s = h->slots[1 - h->cur];
   a.f = h->f;
-  a.perm = h->perm;
-  a.cell_start = h->cell_start;
   a.err = h->err;
   a.cpoint = h->cpoint;
   a.tcur = h->tables[h->tph];
@@ -726,101 +546,36 @@ static void launch_contacts_t(hg_dem *h) {
   a.bn = h->b_alt;
   a.F_out = h->keep_force ? h->F_out : nullptr;
   a.file = h->caller_files ? 0 : 1;
-  ProfScope ps(h->prof, h->table ? "dem_step" : "dem_contacts", h->stream);
+  ProfScope ps(h->prof, "dem_step", h->stream);
   const dim3 g(a.nblocks_padded), b(kTeamThreads);
-  if (h->table) {
-    if (h->cap <= 8) hipLaunchKernelGGL((dem_contacts_kernel<T, 8, true>), g, b, 0, h->stream, a);
-    else if (h->cap <= 16) hipLaunchKernelGGL((dem_contacts_kernel<T, 16, true>), g, b, 0, h->stream, a);
-    else hipLaunchKernelGGL((dem_contacts_kernel<T, 32, true>), g, b, 0, h->stream, a);
-  } else {
-    if (h->cap <= 8) hipLaunchKernelGGL((dem_contacts_kernel<T, 8, false>), g, b, 0, h->stream, a);
-    else if (h->cap <= 16) hipLaunchKernelGGL((dem_contacts_kernel<T, 16, false>), g, b, 0, h->stream, a);
-    else hipLaunchKernelGGL((dem_contacts_kernel<T, 32, false>), g, b, 0, h->stream, a);
-  }
+  if (h->cap <= 8) hipLaunchKernelGGL((dem_step_kernel<T, 8>), g, b, 0, h->stream, a);
+  else if (h->cap <= 16) hipLaunchKernelGGL((dem_step_kernel<T, 16>), g, b, 0, h->stream, a);
+  else hipLaunchKernelGGL((dem_step_kernel<T, 32>), g, b, 0, h->stream, a);
 }
 
-template <typename T>
-static void launch_integrate_t(hg_dem *h, bool rekey) {
-  IntegrateArgs a;
-  a.scenes = h->d_scenes;
-  a.b_off = h->d_b_off;
-  a.nscenes = h->nscenes;
-  a.nbodies = h->nbodies;
-  a.b = h->b;
-  a.s = h->slots[1 - h->cur];
-  a.f = h->f;
-  a.key = h->key;
-  a.slot = h->slot;
-  a.counts = h->counts;
-  // read by the sort that follows this step, after slots flip
-  a.totals = h->scan_totals + size_t(1 - h->cur) * hg_scan_totals_words(h->nscan);
-  a.err = h->err;
-  a.F_out = h->keep_force ? h->F_out : nullptr;
-  a.rekey = rekey ? 1 : 0;
-  ProfScope ps(h->prof, "dem_integrate", h->stream);
-  const dim3 g((h->nbodies + kThreads - 1) / kThreads), b(kThreads);
-  if (h->cap <= 8) hipLaunchKernelGGL((dem_integrate_kernel<T, 8>), g, b, 0, h->stream, a);
-  else if (h->cap <= 16) hipLaunchKernelGGL((dem_integrate_kernel<T, 16>), g, b, 0, h->stream, a);
-  else hipLaunchKernelGGL((dem_integrate_kernel<T, 32>), g, b, 0, h->stream, a);
-}
-
-void hg_dem_launch_step(hg_dem *h, bool rekey) {
-  if (h->table) {
-    // one launch: contacts, integration into b_alt, filing in the next table
-    if (h->nbodies) {
-      if (h->precision == HG_FP64) launch_contacts_t<double>(h);
-      else launch_contacts_t<float>(h);
-    }
-    std::swap(h->b.x, h->b_alt.x);
-    std::swap(h->b.y, h->b_alt.y);
-    std::swap(h->b.th, h->b_alt.th);
-    std::swap(h->b.vx, h->b_alt.vx);
-    std::swap(h->b.vy, h->b_alt.vy);
-    std::swap(h->b.om, h->b_alt.om);
-    h->tph = (h->tph + 1) % 3;
-    h->cur = 1 - h->cur;
-    return;
-  }
+void hg_dem_launch_step(hg_dem *h) {
+  // one launch: contacts, integration into b_alt, filing in the next table
   if (h->nbodies) {
-    if (h->precision == HG_FP64) {
-      launch_contacts_t<double>(h);
-      launch_integrate_t<double>(h, rekey);
-    } else {
-      launch_contacts_t<float>(h);
-      launch_integrate_t<float>(h, rekey);
-    }
+    if (h->precision == HG_FP64) launch_step_t<double>(h);
+    else launch_step_t<float>(h);
   }
+  std::swap(h->b.x, h->b_alt.x);
+  std::swap(h->b.y, h->b_alt.y);
+  std::swap(h->b.th, h->b_alt.th);
+  std::swap(h->b.vx, h->b_alt.vx);
+  std::swap(h->b.vy, h->b_alt.vy);
+  std::swap(h->b.om, h->b_alt.om);
+  h->tph = (h->tph + 1) % 3;
   h->cur = 1 - h->cur;
-  if (rekey) hg_dem_launch_sort(h);
 }
 
 void hg_dem_launch_rebin(hg_dem *h) {
-  if (h->table) {
-    // every table emptied, the one the next step reads filled from the
-    // positions
-    for (int c = 0; c < 3; ++c) {
-      hipMemsetAsync(h->tables[c].count, 0, size_t(h->ncell) * 4, h->stream);
-      hipMemsetAsync(h->tables[c].novf, 0, 4, h->stream);
-    }
-    if (h->nbodies) {
-      BinArgs a;
-      a.scenes = h->d_scenes;
-      a.b_off = h->d_b_off;
-      a.nscenes = h->nscenes;
-      a.nbodies = h->nbodies;
-      a.x = h->b.x;
-      a.y = h->b.y;
-      a.table = 1;
-      a.t = h->tables[h->tph];
-      a.kslots = h->kslots;
-      ProfScope ps(h->prof, "dem_bin", h->stream);
-      hipLaunchKernelGGL(dem_bin_kernel, dim3((h->nbodies + kThreads - 1) / kThreads),
-                         dim3(kThreads), 0, h->stream, a);
-    }
-    return;
+  // every table emptied, the one the next step reads filled from the
+  // positions
+  for (int c = 0; c < 3; ++c) {
+    hipMemsetAsync(h->tables[c].count, 0, size_t(h->ncell) * 4, h->stream);
+    hipMemsetAsync(h->tables[c].novf, 0, 4, h->stream);
   }
-  uint32_t *totals = h->scan_totals + size_t(h->cur) * hg_scan_totals_words(h->nscan);
-  hg_zero_words(h->stream, totals, hg_scan_totals_words(h->nscan));
   if (h->nbodies) {
     BinArgs a;
     a.scenes = h->d_scenes;
@@ -829,26 +584,15 @@ void hg_dem_launch_rebin(hg_dem *h) {
     a.nbodies = h->nbodies;
     a.x = h->b.x;
     a.y = h->b.y;
-    a.key = h->key;
-    a.slot = h->slot;
-    a.counts = h->counts;
-    a.totals = totals;
-    a.table = 0;
+    a.t = h->tables[h->tph];
+    a.kslots = h->kslots;
     ProfScope ps(h->prof, "dem_bin", h->stream);
-    hipLaunchKernelGGL(dem_bin_kernel, dim3((h->nbodies + kThreads - 1) / kThreads), dim3(kThreads),
-                       0, h->stream, a);
+    hipLaunchKernelGGL(dem_bin_kernel, dim3((h->nbodies + kThreads - 1) / kThreads),
+                       dim3(kThreads), 0, h->stream, a);
   }
-  hg_dem_launch_sort(h);
 }
 
-void hg_dem_launch_sort(hg_dem *h) {
-  hg_counting_sort(h->prof, h->stream, h->counts, h->ncell,
-                   h->scan_totals + size_t(h->cur) * hg_scan_totals_words(h->nscan),
-                   h->scan_totals + size_t(1 - h->cur) * hg_scan_totals_words(h->nscan), h->nscan, h->cell_start,
-                   h->key, h->slot, h->nbodies, h->perm, h->err);
-}
-
-static void launch_step(hg_dem *h) { hg_dem_launch_step(h, true); }
+static void launch_step(hg_dem *h) { hg_dem_launch_step(h); }
 
 void hg_dem_caller_files(hg_dem *h) {
   h->caller_files = true;
@@ -856,19 +600,6 @@ void hg_dem_caller_files(hg_dem *h) {
     hipGraphExecDestroy(h->graph);
     h->graph = nullptr;
   }
-}
-
-hg_status hg_dem_use_sorted_order(hg_dem *h) {
-  if (!h->table) return HG_OK;
-  h->table = false;
-  if (h->graph) {
-    hipGraphExecDestroy(h->graph);
-    h->graph = nullptr;
-  }
-  hg_dem_launch_rebin(h);
-  HG_HIP(h->ctx, hipGetLastError());
-  HG_HIP(h->ctx, hipStreamSynchronize(h->stream));
-  return HG_OK;
 }
 
 static hg_status dem_upload_dt(hg_dem *h, double dt) {
@@ -959,14 +690,7 @@ static hg_status upload_bodies(hg_dem *h, const hg_dem_desc *descs) {
     HG_HIP(h->ctx, hipMemset(s.ocnt, 0, B ? B : 1));
     HG_HIP(h->ctx, hipMemset(s.pact, 0, size_t(B ? B : 1) * kMaxPlanes));
   }
-  ForceRaw &f = h->f;
-  HG_TRY(dalloc(h, &f.oF, size_t(B) * h->cap * 2 * sizeof(T)));
-  HG_TRY(dalloc(h, &f.otau0, size_t(B) * h->cap * sizeof(T)));
-  HG_TRY(dalloc(h, &f.pF, size_t(B) * kMaxPlanes * 2 * sizeof(T)));
-  HG_TRY(dalloc(h, &f.ptau, size_t(B) * kMaxPlanes * sizeof(T)));
-  HG_TRY(dalloc(h, &f.iF, size_t(B) * h->cap * 2 * sizeof(T)));
-  HG_TRY(dalloc(h, &f.itau, size_t(B) * h->cap * sizeof(T)));
-  HG_TRY(dalloc(h, &f.icnt, B));
+  HG_TRY(dalloc(h, &h->f.oF, size_t(B) * h->cap * 2 * sizeof(T)));
   return HG_OK;
 }
 
@@ -1124,17 +848,9 @@ static hg_status dem_create_impl(hg_ctx *ctx, const hg_dem_desc *descs, uint32_t
   h_b_off.push_back(uint32_t(b_off));
   h->nbodies = uint32_t(b_off);
   h->ncell = uint32_t(cell_off);
-  h->nscan = hg_scan_blocks(h->ncell);
-  if (h->nscan > uint32_t(kScanChunk)) return hg_fail(ctx, HG_ERR_UNSUPPORTED, "too many cells");
   hg_dem *hp = h.get();
   HG_TRY(dalloc(hp, &h->d_scenes, nscenes * sizeof(SceneDev)));
   HG_TRY(dalloc(hp, &h->d_b_off, (nscenes + 1) * 4));
-  HG_TRY(dalloc(hp, &h->perm, size_t(h->nbodies) * 4));
-  HG_TRY(dalloc(hp, &h->key, size_t(h->nbodies) * 4));
-  HG_TRY(dalloc(hp, &h->slot, size_t(h->nbodies) * 4));
-  HG_TRY(dalloc(hp, &h->counts, size_t(h->ncell) * 4));
-  HG_TRY(dalloc(hp, &h->cell_start, size_t(h->ncell + 1) * 4));
-  HG_TRY(dalloc(hp, &h->scan_totals, size_t(hg_scan_totals_words(h->nscan)) * 8));
   HG_TRY(dalloc(hp, &h->err, nscenes * 4));
   HG_TRY(dalloc(hp, &h->F_out, size_t(h->nbodies) * 24));
   h->kslots = dem_table_slots(descs, nscenes);
@@ -1151,9 +867,6 @@ static hg_status dem_create_impl(hg_ctx *ctx, const hg_dem_desc *descs, uint32_t
   }
   HG_HIP(ctx, hipMemcpy(h->d_scenes, h->scenes.data(), nscenes * sizeof(SceneDev), hipMemcpyHostToDevice));
   HG_HIP(ctx, hipMemcpy(h->d_b_off, h_b_off.data(), (nscenes + 1) * 4, hipMemcpyHostToDevice));
-  HG_HIP(ctx, hipMemset(h->counts, 0, size_t(h->ncell) * 4));
-  HG_HIP(ctx, hipMemset(h->scan_totals, 0, size_t(hg_scan_totals_words(h->nscan)) * 8));
-  HG_HIP(ctx, hipMemset(h->perm, 0, size_t(h->nbodies) * 4));
   HG_HIP(ctx, hipMemset(h->err, 0, nscenes * 4));
   HG_HIP(ctx, hipMemset(h->F_out, 0, size_t(h->nbodies ? h->nbodies : 1) * 24));
   if (h->precision == HG_FP64) HG_TRY(upload_bodies<double>(hp, descs));
@@ -1173,7 +886,6 @@ static hg_status dem_check(hg_dem *h) {
   std::vector<int32_t> e(h->nscenes);
   HG_HIP(h->ctx, hipMemcpy(e.data(), h->err, h->nscenes * 4, hipMemcpyDeviceToHost));
   for (uint32_t s = 0; s < h->nscenes; ++s) {
-    if (e[s] & kSortTotalsBit) return hg_fail(h->ctx, HG_ERR_STATE, "bucket sort: block totals do not match the counts");
     if (e[s] & 4)
       return hg_fail(h->ctx, HG_ERR_UNSUPPORTED, "scene " + std::to_string(s) + ": more than " +
                                                      std::to_string(h->cap) + " contacts on one body");
@@ -1215,12 +927,12 @@ extern "C" hg_status hg_dem_step_graph(hg_dem *h, double dt, uint32_t nsteps) {
   if (!(dt > 0.0)) return hg_fail(h->ctx, HG_ERR_INVALID, "dt must be > 0");
   if (h->prof.on) return hg_dem_step(h, dt, nsteps);
   HG_TRY(dem_upload_dt(h, dt));
-  // The buffers a step uses repeat with a period: the contact slots (and, in
-  // the DEM-only steps, the state buffers) alternate, the tables rotate by
-  // three.  A graph of one period, captured from phase 0, replays from any
-  // later phase 0; steps before and after run without it.
-  const uint32_t period = h->table ? 6u : 2u;
-  auto phase0 = [&]() { return h->cur == 0 && (!h->table || h->tph == 0); };
+  // The buffers a step uses repeat every six steps: the contact slots and
+  // the state buffers alternate, the tables rotate by three.  A graph of one
+  // period, captured from phase 0, replays from any later phase 0; steps
+  // before and after run without it.
+  const uint32_t period = 6u;
+  auto phase0 = [&]() { return h->cur == 0 && h->tph == 0; };
   while (!phase0() && nsteps) {
     launch_step(h);
     --nsteps;
@@ -1527,14 +1239,15 @@ extern "C" hg_status hg_dem_kernel_times(hg_dem *h, int32_t max_kernels, char *n
   return hg_profiler_report(h->prof, h->ctx, max_kernels, names, ms, launches, nkernels);
 }
 
-// Device bytes per body of the arrays dalloc gives a handle (bodies, two
-// contact-slot sets, force records, sort keys), for the estimate below.
+// Device bytes per body of the arrays dalloc gives a handle (bodies and the
+// second state buffer, two contact-slot sets, the owned contacts' forces, the
+// kept total force), for the estimate below.
 static uint64_t dem_bytes_per_body(size_t T, int cap) {
   const uint64_t C = uint64_t(cap);
   const uint64_t bodies = 4 * 8 + 7 * T + 1 + 3 * 8 + 3 * T;
   const uint64_t slots = C * 4 + C * 2 * T + 1 + uint64_t(kMaxPlanes) + uint64_t(kMaxPlanes) * 2 * T;
-  const uint64_t forces = C * 6 * T + uint64_t(kMaxPlanes) * 3 * T + 1;
-  return bodies + 2 * slots + forces + 3 * 4 + 24;
+  const uint64_t forces = C * 2 * T;
+  return bodies + 2 * slots + forces + 24;
 }
 
 extern "C" hg_status hg_dem_estimate_bytes(const hg_dem_desc *descs, uint32_t nscenes,
@@ -1549,12 +1262,11 @@ extern "C" hg_status hg_dem_estimate_bytes(const hg_dem_desc *descs, uint32_t ns
     cells += uint64_t(sd.ncx) * uint64_t(sd.ncy);
   }
   const size_t T = descs[0].precision == HG_FP64 ? 8 : 4;
-  const uint64_t nscan = (cells + kScanChunk - 1) / kScanChunk;
   // the three bucket tables: counts and item slots per bucket, bucket and
   // overflow entry per body
   const uint64_t tables = 3 * (cells * (uint64_t(dem_table_slots(descs, nscenes)) + 1) * 4 + B * 8 + 4);
-  *bytes = B * dem_bytes_per_body(T, hg_dem_contact_capacity(descs, nscenes)) + cells * 8 + 4 + nscan * 8 * kScanSubTotals +
-           tables + uint64_t(nscenes) * (sizeof(SceneDev) + 16) + (1u << 20);
+  *bytes = B * dem_bytes_per_body(T, hg_dem_contact_capacity(descs, nscenes)) + tables +
+           uint64_t(nscenes) * (sizeof(SceneDev) + 16) + (1u << 20);
   return HG_OK;
 }
 

@@ -48,17 +48,10 @@ struct SlotsRaw {
   void *pds;       // [B][kMaxPlanes][2]
 };
 
+// The step's records read after it (contact-force downloads, the hybrid's
+// homogenised stress): the force of each owned contact on its owner.
 struct ForceRaw {
-  void *oF;     // [B][cap][2] force of each owned contact on its owner
-  void *otau0;  // [B][cap]    its torque on the owner
-  void *pF;     // [B][kMaxPlanes][2]
-  void *ptau;   // [B][kMaxPlanes]
-  // contacts owned by a smaller partner, in partner order: the force on the
-  // owner (which this body receives negated) and the torque of its negation
-  // about this body, as the owner computes them
-  void *iF;     // [B][cap][2]
-  void *itau;   // [B][cap]
-  uint8_t *icnt;
+  void *oF;  // [B][cap][2]
 };
 
 // The broad-phase table of DEM-only steps: each bucket keeps up to kslots
@@ -117,30 +110,22 @@ struct hg_dem {
   int precision = HG_FP64;
   int cap = hgdem::kCapMin;  // contact slots per body (owned, and incoming)
   std::vector<SceneDev> scenes;
-  uint32_t nscenes = 0, nbodies = 0, ncell = 0, nscan = 0;
+  uint32_t nscenes = 0, nbodies = 0, ncell = 0;
   SceneDev *d_scenes = nullptr;
   uint32_t *d_b_off = nullptr;
   BodiesRaw b = {};
-  // DEM-only steps write the new position, angle and velocities here, and the
-  // two swap after the step (b always holds the current state)
+  // a step writes the new position, angle and velocities here, and the two
+  // swap after it (b always holds the current state)
   BodiesRaw b_alt = {};
   SlotsRaw slots[2] = {};
   int cur = 0;  // slots[cur] holds the last step's contacts
   ForceRaw f = {};
-  uint32_t *perm = nullptr, *key = nullptr, *slot = nullptr, *counts = nullptr,
-           *cell_start = nullptr;
-  // DEM-only handles find candidates in bucket tables that the integration
-  // fills (no sort per step); the hybrid's grains use the sorted order
-  bool table = true;
   // the hybrid's grains: the step does not file the bodies, the coupling's
   // last kernel files them at their final positions (in tables[tph])
   bool caller_files = false;
   int kslots = 8;
   hgdem::TableRaw tables[3] = {};
   int tph = 0;  // the table the next step reads
-  // per-scan-block totals of the bucket keys that the sort after a step
-  // with slots[c] current reads, at scan_totals + c * hg_scan_totals_words(nscan)
-  uint32_t *scan_totals = nullptr;
   int32_t *err = nullptr;
   double *F_out = nullptr;
   double *cpoint = nullptr;  // contact points of the owned circle pairs (kept on request)
@@ -159,16 +144,11 @@ struct hg_dem {
 // Contact slots per body for a batch of scenes: 8, 16 or 32 (see hg_dem.hip).
 int hg_dem_contact_capacity(const hg_dem_desc *descs, uint32_t nscenes);
 
-// Launches one DEM step (contacts, integrate); with rekey false the
-// integrate kernel does not produce cell keys and the caller re-bins with
-// hg_dem_launch_rebin after editing positions.
-void hg_dem_launch_step(hg_dem *h, bool rekey);
+// Launches one DEM step (one launch: contacts, integration, filing in the
+// next bucket table), and re-bins the current positions into the table the
+// next step reads (after a caller edited them).
+void hg_dem_launch_step(hg_dem *h);
 void hg_dem_launch_rebin(hg_dem *h);
-// The hybrid's grains: the counting sort of every step instead of the
-// bucket tables (re-bins the current positions).
-hg_status hg_dem_use_sorted_order(hg_dem *h);
-// The hybrid's grains with the tables: steps leave the filing of the moved
-// bodies to the caller, which files each body in tables[tph] after the step.
+// The hybrid's grains: steps leave the filing of the moved bodies to the
+// caller, which files each body in tables[tph] after the step.
 void hg_dem_caller_files(hg_dem *h);
-// The counting sort alone, for a caller that wrote the cell keys and slots.
-void hg_dem_launch_sort(hg_dem *h);

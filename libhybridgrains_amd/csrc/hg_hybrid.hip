@@ -521,7 +521,7 @@ static void launch_hybrid_steps(hg_hybrid *h, double dt, uint32_t n, bool concur
   };
   auto dem_step = [&]() {
     h->dem->stream = side;
-    hg_dem_launch_step(h->dem, false);
+    hg_dem_launch_step(h->dem);
     h->dem->stream = s;
   };
   auto homog = [&](const HybArgs &a) {
