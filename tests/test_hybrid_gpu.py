@@ -56,19 +56,27 @@ def test_twenty_steps_fp64_matches_oracle(hg_ctx):
     _compare(g, o, 1e-11, 1e-10)
 
 
-def test_graph_matches_stream(hg_ctx):
+@pytest.mark.parametrize("prec", [HG_FP64, HG_FP32])
+def test_graph_matches_stream(hg_ctx, prec):
+    """Graphs span six steps (the solvers' buffers alternate, the DEM's bucket
+    tables rotate by three): the first call captures and replays one period
+    and steps the rest eagerly; the second aligns to the period eagerly,
+    replays it and ends eagerly."""
     m, d, w = scenes.hybrid_column(seed=3)
-    a = GpuHybrid(m, d, w, HG_FP64, ctx=hg_ctx)
-    b = GpuHybrid(m, d, w, HG_FP64, ctx=hg_ctx)
-    a.step(DT, 9)
-    b.step(DT, 4, graph=True)  # even count from buffer 0
-    b.step(DT, 5, graph=True)  # odd count, replays the captured pair
+    a = GpuHybrid(m, d, w, prec, ctx=hg_ctx)
+    b = GpuHybrid(m, d, w, prec, ctx=hg_ctx)
+    a.step(DT, 20)
+    b.step(DT, 7, graph=True)
+    b.step(DT, 13, graph=True)
+    # the grain side's node sums are FP64 atomics (reproducible to a few
+    # ulps), which FP32 storage can round to a neighbouring value
+    tol = 1e-12 if prec == HG_FP64 else 1e-5
     pa, pb = a.points(), b.points()
     for k in pa:
-        assert_close(pb[k], pa[k], 1e-12, k)
+        assert_close(pb[k], pa[k], tol, k)
     qa, qb = a.bodies(), b.bodies()
-    assert_close(qb["q"], qa["q"], 1e-12, "q")
-    assert_close(qb["v"], qa["v"], 1e-12, "v")
+    assert_close(qb["q"], qa["q"], tol, "q")
+    assert_close(qb["v"], qa["v"], tol, "v")
     assert np.array_equal(a.coupled(), b.coupled())
 
 
