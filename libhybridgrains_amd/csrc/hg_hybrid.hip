@@ -589,7 +589,8 @@ static void launch_hybrid_steps(hg_hybrid *h, double dt, uint32_t n, bool concur
     grain_mass_on(s);
     for (uint32_t k = 0; k < n; ++k) {
       const HybArgs a = make_args(h);
-      hg_mpm_launch_coupled_grid(h->mpm, dt, &gn);
+      // the node mass is kept for the coupling weights when they are recorded
+      hg_mpm_launch_coupled_grid(h->mpm, dt, &gn, h->hfac_on);
       homog_factor(a);
       order(h->fork_ev, s, side);
       grain_end(a);

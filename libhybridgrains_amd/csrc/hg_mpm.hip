@@ -124,16 +124,14 @@ struct P2GArgs {
 // lumpedMassVelocityAndAccelerationUpdate (:588-609), writing what store
 // asks for.
 template <typename T>
-__device__ __forceinline__ void grid_node_update(const SceneDev &sc, int nx, int ny, double m,
-                                                 double p0, double p1, double f0, double f1,
-                                                 uint32_t store, void *gva, double *gm,
-                                                 double *gp, double *gpn, double *gf) {
-  using T4 = typename Vec4T<T>::type;
+__device__ __forceinline__ void grid_node_momentum(const SceneDev &sc, int nx, int ny, double m,
+                                                   double p0, double p1, double &f0, double &f1,
+                                                   double &pn0, double &pn1) {
   const double dt = sc.dt;
-  const uint32_t node = sc.node_off + uint32_t(ny) * uint32_t(sc.snx) + uint32_t(nx);
   f0 += m * sc.g0;
   f1 += m * sc.g1;
-  double pn0 = 0.0, pn1 = 0.0;
+  pn0 = 0.0;
+  pn1 = 0.0;
   if (m > 0.0) {
     pn0 = p0 + dt * f0;
     pn1 = p1 + dt * f1;
@@ -159,10 +157,20 @@ __device__ __forceinline__ void grid_node_update(const SceneDev &sc, int nx, int
       pn1 = 0.0;
     }
   }
+}
+
+// The node's stored fields from its mass m, momentum p, new momentum pn and
+// force f: the (v, a) vector and what store asks for.
+template <typename T>
+__device__ __forceinline__ void grid_node_store(const SceneDev &sc, uint32_t node, double m,
+                                                double p0, double p1, double pn0, double pn1,
+                                                double f0, double f1, uint32_t store, void *gva,
+                                                double *gm, double *gp, double *gpn, double *gf) {
+  using T4 = typename Vec4T<T>::type;
   if (store & kStoreVelAcc) {
     T4 va;
     if (m > 0.0) {
-      const double dm = dt * m;
+      const double dm = sc.dt * m;
       va.x = T(pn0 / m);
       va.y = T(pn1 / m);
       va.z = T((pn0 - p0) / dm);
@@ -185,6 +193,17 @@ __device__ __forceinline__ void grid_node_update(const SceneDev &sc, int nx, int
     gf[2 * node] = f0;
     gf[2 * node + 1] = f1;
   }
+}
+
+template <typename T>
+__device__ __forceinline__ void grid_node_update(const SceneDev &sc, int nx, int ny, double m,
+                                                 double p0, double p1, double f0, double f1,
+                                                 uint32_t store, void *gva, double *gm,
+                                                 double *gp, double *gpn, double *gf) {
+  const uint32_t node = sc.node_off + uint32_t(ny) * uint32_t(sc.snx) + uint32_t(nx);
+  double pn0, pn1;
+  grid_node_momentum<T>(sc, nx, ny, m, p0, p1, f0, f1, pn0, pn1);
+  grid_node_store<T>(sc, node, m, p0, p1, pn0, pn1, f0, f1, store, gva, gm, gp, gpn, gf);
 }
 
 // What P2G needs of one point besides its weights.
@@ -1567,12 +1586,17 @@ __global__ __launch_bounds__(256) void mpm_grid_kernel(GridArgs a) {
         v[f] = double(iv) * sc.fx_quantum[f];
       }
       if (big) atomicOr(&a.err[sid], kMpmFixedRangeBit);
-      grid_node_update<float>(sc, nx, ny, v[kFxMass], v[kFxMom0], v[kFxMom1], v[kFxF0], v[kFxF1],
-                              a.store, a.gva, a.gm, a.gp, a.gpn, a.gf);
+      const double m = v[kFxMass], p0 = v[kFxMom0], p1 = v[kFxMom1];
+      double f0 = v[kFxF0], f1 = v[kFxF1], pn0, pn1;
+      grid_node_momentum<float>(sc, nx, ny, m, p0, p1, f0, f1, pn0, pn1);
       if (a.exchange) {
+        // the grain exchange where grain mass lies, on the values in hand
         const uint32_t n = node - sc.node_off;
-        if (a.ex.dmass[n] > 0.0) exchange_node<float>(sc.dt, node, n, a.ex, a.gm, a.gp, a.gpn, a.gva);
+        const double m_dem = a.ex.dmass[n];
+        if (m_dem > 0.0) exchange_in_place(n, a.ex, m_dem, m, pn0, pn1);
       }
+      grid_node_store<float>(sc, node, m, p0, p1, pn0, pn1, f0, f1, a.store, a.gva, a.gm, a.gp,
+                             a.gpn, a.gf);
     }
   }
 }
@@ -1949,9 +1973,11 @@ hg_status hg_mpm_prepare_fused(hg_mpm *h, double dt) {
   return HG_OK;
 }
 
-hg_status hg_mpm_launch_coupled_grid(hg_mpm *h, double dt, const GrainNodes *g) {
+hg_status hg_mpm_launch_coupled_grid(hg_mpm *h, double dt, const GrainNodes *g, bool keep_mass) {
   HG_TRY(upload_dt(h, dt));
-  const uint32_t store = kStoreVelAcc | kStoreMass | kStoreMom | kStoreMomNew;
+  // the exchange works on the values in hand; the node mass is kept for a
+  // reader after the step's grid update
+  const uint32_t store = kStoreVelAcc | (keep_mass ? kStoreMass : 0u);
   if (!h->acc_ready) launch_fused(h, true);
   launch_grid(h, store, g);
   return HG_OK;

@@ -340,23 +340,34 @@ struct GrainNodes {
   double *vafter, *aafter;
 };
 
+// The exchange on values in hand: m_dem is the node's grain mass, m_mpm its
+// continuum mass and (pn0, pn1) the continuum's new momentum, which becomes
+// pn - l.
+__device__ __forceinline__ void exchange_in_place(uint32_t n, const GrainNodes &g, double m_dem,
+                                                  double m_mpm, double &pn0, double &pn1) {
+  const double msum = m_mpm + m_dem;
+  const double l0 = (m_dem * pn0 - m_mpm * g.dmom[2 * n]) / msum;
+  const double l1 = (m_dem * pn1 - m_mpm * g.dmom[2 * n + 1]) / msum;
+  pn0 = pn0 - l0;
+  pn1 = pn1 - l1;
+  g.vafter[2 * n] = (g.dmom[2 * n] + l0) / m_dem;
+  g.vafter[2 * n + 1] = (g.dmom[2 * n + 1] + l1) / m_dem;
+  g.aafter[2 * n] = l0 / m_dem;
+  g.aafter[2 * n + 1] = l1 / m_dem;
+}
+
+// The same at a node whose grid update is stored (mass, momentum, new
+// momentum): the new momentum and the (v, a) vector are rewritten.
 template <typename T>
 __device__ __forceinline__ void exchange_node(double dt, uint32_t node, uint32_t n,
                                               const GrainNodes &g, const double *gm,
                                               const double *gp, double *gpn, void *gva) {
   using T4 = typename Vec4T<T>::type;
-  const double m_dem = g.dmass[n];
   const double m_mpm = gm[node];
-  const double msum = m_mpm + m_dem;
-  const double l0 = (m_dem * gpn[2 * node] - m_mpm * g.dmom[2 * n]) / msum;
-  const double l1 = (m_dem * gpn[2 * node + 1] - m_mpm * g.dmom[2 * n + 1]) / msum;
-  const double pn0 = gpn[2 * node] - l0, pn1 = gpn[2 * node + 1] - l1;
+  double pn0 = gpn[2 * node], pn1 = gpn[2 * node + 1];
+  exchange_in_place(n, g, g.dmass[n], m_mpm, pn0, pn1);
   gpn[2 * node] = pn0;
   gpn[2 * node + 1] = pn1;
-  g.vafter[2 * n] = (g.dmom[2 * n] + l0) / m_dem;
-  g.vafter[2 * n + 1] = (g.dmom[2 * n + 1] + l1) / m_dem;
-  g.aafter[2 * n] = l0 / m_dem;
-  g.aafter[2 * n + 1] = l1 / m_dem;
   T4 va;
   if (m_mpm > 0.0) {
     const double dm = dt * m_mpm;
