@@ -115,16 +115,35 @@ struct GrainStencil {
   double wx[S], wy[S];
 };
 
+// The stencil's nodes alone (lx, ly and on; the weights are not formed).
 template <int BASIS>
-__device__ __forceinline__ bool grain_stencil_weights(const hgmpm::SceneDev &sc, double x,
-                                                      double y, double hl,
-                                                      GrainStencil<BASIS> &g) {
+__device__ __forceinline__ bool grain_stencil_nodes(const hgmpm::SceneDev &sc, double x, double y,
+                                                    double hl, GrainStencil<BASIS> &g, int &hx,
+                                                    int &hy) {
   constexpr int S = GrainStencil<BASIS>::S;
-  int lx, hx, ly, hy;
+  int lx, ly;
   g.on = 0u;
   if (!grain_stencil<BASIS>(sc, x, y, hl, lx, hx, ly, hy)) return false;
   g.lx = lx;
   g.ly = ly;
+#pragma unroll
+  for (int j = 0; j < S; ++j)
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      const int ix = lx + i, iy = ly + j;
+      if (ix <= hx && iy <= hy && ix >= 0 && iy >= 0 && ix < sc.nnx && iy < sc.nny)
+        g.on |= 1u << (j * S + i);
+    }
+  return true;
+}
+
+// The 1-D weights of a stencil whose nodes grain_stencil_nodes found.
+template <int BASIS>
+__device__ __forceinline__ void grain_stencil_fill(const hgmpm::SceneDev &sc, double x, double y,
+                                                   double hl, int hx, int hy,
+                                                   GrainStencil<BASIS> &g) {
+  constexpr int S = GrainStencil<BASIS>::S;
+  const int lx = g.lx, ly = g.ly;
   const double vnorm = (BASIS == HG_BASIS_UGIMP_LINEAR) ? 1.0 / (4.0 * hl * hl) : 1.0;
 #pragma unroll
   for (int i = 0; i < S; ++i) {
@@ -138,14 +157,15 @@ __device__ __forceinline__ bool grain_stencil_weights(const hgmpm::SceneDev &sc,
     g.wx[i] = (BASIS == HG_BASIS_UGIMP_LINEAR) ? ax * vnorm : ax;
     g.wy[i] = ay;
   }
-#pragma unroll
-  for (int j = 0; j < S; ++j)
-#pragma unroll
-    for (int i = 0; i < S; ++i) {
-      const int ix = lx + i, iy = ly + j;
-      if (ix <= hx && iy <= hy && ix >= 0 && iy >= 0 && ix < sc.nnx && iy < sc.nny)
-        g.on |= 1u << (j * S + i);
-    }
+}
+
+template <int BASIS>
+__device__ __forceinline__ bool grain_stencil_weights(const hgmpm::SceneDev &sc, double x,
+                                                      double y, double hl,
+                                                      GrainStencil<BASIS> &g) {
+  int hx, hy;
+  if (!grain_stencil_nodes<BASIS>(sc, x, y, hl, g, hx, hy)) return false;
+  grain_stencil_fill<BASIS>(sc, x, y, hl, hx, hy, g);
   return true;
 }
 
@@ -174,7 +194,9 @@ __global__ __launch_bounds__(kThreads) void hyb_grain_mass(HybArgs a) {
     const double x = a.prex[b], y = a.prey[b];
     a.coupled[b] = 0;
     const hgmpm::SceneDev &sc = *a.msc;
-    if (!a.b.fixed[b] && grain_stencil_weights<BASIS>(sc, x, y, a.hl, g)) {
+    int hx, hy;
+    // the coupling test first (the weights only for a grain that passes)
+    if (!a.b.fixed[b] && grain_stencil_nodes<BASIS>(sc, x, y, a.hl, g, hx, hy)) {
       bool all_pos = true;
 #pragma unroll
       for (int q = 0; q < S * S; ++q) {
@@ -189,6 +211,7 @@ __global__ __launch_bounds__(kThreads) void hyb_grain_mass(HybArgs a) {
         if (m <= 1.0e-6) all_pos = false;
       }
       if (all_pos) {
+        grain_stencil_fill<BASIS>(sc, x, y, a.hl, hx, hy, g);
         a.coupled[b] = 1;
         const double mb = double(ld<T>(a.b.m, b));
         const double pm0 = mb * double(ld<T>(a.b.vx, b)), pm1 = mb * double(ld<T>(a.b.vy, b));
