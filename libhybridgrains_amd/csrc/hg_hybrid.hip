@@ -522,6 +522,15 @@ static HybArgs make_args(hg_hybrid *h) {
   return a;
 }
 
+// The events that order the two streams of one device: no timing, and no
+// system-scope fence at the record (each kernel's own release makes its
+// writes visible on the device, and the host reads results after the
+// continuum stream's copies).  Measured with in-kernel clocks, the kernel
+// after a record then starts about 4.7 us after its predecessor instead of
+// 7.5 us, and a waiting stream's kernel 9.3 us instead of 11 us after the
+// event's kernel.
+constexpr unsigned kOrderEventFlags = hipEventDisableTiming | hipEventDisableSystemFence;
+
 // n steps.  concurrent: the grain branch runs on the second stream (eager
 // steps), and each step's DEM contacts and integration are queued on it as
 // soon as the previous step's grain work is queued, so they overlap the
@@ -745,8 +754,8 @@ extern "C" hg_status hg_hybrid_create(hg_ctx *ctx, const hg_mpm_desc *mpm_desc,
     hipMemcpy(h->prey, h->dem->b.y, size_t(h->dem->nbodies) * 8, hipMemcpyDeviceToDevice);
   }
   if (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&h->join_ev, hipEventDisableTiming) != hipSuccess)
+      hipEventCreateWithFlags(&h->fork_ev, kOrderEventFlags) != hipSuccess ||
+      hipEventCreateWithFlags(&h->join_ev, kOrderEventFlags) != hipSuccess)
     return fail(hg_fail(ctx, HG_ERR_HIP, "hybrid streams"));
   // the memsets and copies above ran on the null stream (other handles' work
   // is not waited for)
