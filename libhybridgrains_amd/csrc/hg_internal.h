@@ -115,7 +115,16 @@ __device__ __forceinline__ uint32_t hg_wave_slot(uint32_t *counts, uint32_t key)
 // distinct key.  Lanes of a run get consecutive slots in lane order; a key
 // that appears in several runs takes one range per run.  Inactive lanes take
 // no part and end runs.
-__device__ __forceinline__ uint32_t hg_run_slot(uint32_t *counts, uint32_t key) {
+//
+// In two halves, so that a caller can leave the atomic's return in flight
+// while it works on: hg_run_slot_issue files the run's count and returns the
+// pieces, hg_run_slot_take (called later by the same lanes, nothing else
+// active) gives the slot.
+struct HgRunSlot {
+  uint32_t base;  // the run's first slot (valid in the run's first lane)
+  uint32_t fr;    // the run's first lane | this lane's rank in the run << 8
+};
+__device__ __forceinline__ HgRunSlot hg_run_slot_issue(uint32_t *counts, uint32_t key) {
   const int lane = int(threadIdx.x & 63u);
   const uint64_t active = __ballot(1);
   const uint32_t prev = uint32_t(__shfl_up(int(key), 1, 64));
@@ -130,11 +139,18 @@ __device__ __forceinline__ uint32_t hg_run_slot(uint32_t *counts, uint32_t key) 
   const int next = after ? (__ffsll((unsigned long long)after) - 1) : 64;
   const uint64_t below_first = (first == 0) ? 0ull : ((1ull << first) - 1ull);
   const uint64_t run = (next == 64 ? ~0ull : ((1ull << next) - 1ull)) & ~below_first;
-  uint32_t base = 0;
-  if (lane == first) base = atomicAdd(&counts[key], uint32_t(__popcll(run & active)));
-  base = uint32_t(__shfl(int(base), first, 64));
+  HgRunSlot r;
+  r.base = 0;
+  if (lane == first) r.base = atomicAdd(&counts[key], uint32_t(__popcll(run & active)));
   const uint64_t before = ((lane == 0) ? 0ull : ((1ull << lane) - 1ull)) & ~below_first;
-  return base + uint32_t(__popcll(before & active));
+  r.fr = uint32_t(first) | (uint32_t(__popcll(before & active)) << 8);
+  return r;
+}
+__device__ __forceinline__ uint32_t hg_run_slot_take(const HgRunSlot &r) {
+  return uint32_t(__shfl(int(r.base), int(r.fr & 0xffu), 64)) + (r.fr >> 8);
+}
+__device__ __forceinline__ uint32_t hg_run_slot(uint32_t *counts, uint32_t key) {
+  return hg_run_slot_take(hg_run_slot_issue(counts, key));
 }
 
 // Wavefront (64-lane) inclusive prefix sum of 32-bit values.

@@ -245,10 +245,10 @@ __device__ __forceinline__ P2GRaw<T> p2g_load(const PtsRaw &pts, uint32_t p, con
   P2GRaw<T> r;
   r.x = pts.x[p];
   r.y = pts.y[p];
-  r.m = ld<T>(pts.m, p);
+  r.m = pts.m ? ld<T>(pts.m, p) : T(sc.m_uniform);
   r.v0 = ld<T>(pts.vx, p);
   r.v1 = ld<T>(pts.vy, p);
-  r.V = ld<T>(pts.V, p);
+  r.V = pts.V ? ld<T>(pts.V, p) : T(sc.V_uniform);
   r.J = ld<T>(pts.J, p);
   r.b00 = ld<T>(pts.b00, p);
   r.b01 = ld<T>(pts.b01, p);
@@ -661,6 +661,16 @@ struct VelAccArgs {
   int32_t tw, th;
 };
 
+// pinv as the inverse of perm (what the scatter writes when asked for it).
+__global__ __launch_bounds__(256) void pinv_from_perm_kernel(const uint32_t *perm, uint32_t n,
+                                                             uint32_t *pinv) {
+  const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+  if (j < n) {
+    const uint32_t i = perm[j];
+    if (i < n) pinv[i] = j;
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void mpm_velacc_kernel(VelAccArgs a) {
   const uint32_t n = *a.tile_count;
@@ -968,8 +978,10 @@ __global__ __launch_bounds__(kG2PThreads) void mpm_g2p_kernel(G2PArgs a) {
   st<T>(a.dst.b01, o, T(c01));
   st<T>(a.dst.b11, o, strain_store<T>(c11));
   st<T>(a.dst.J, o, strain_store<T>(J1));
-  st<T>(a.dst.m, o, ld<T>(a.src.m, p));
-  st<T>(a.dst.V, o, ld<T>(a.src.V, p));
+  if (a.src.m) {
+    st<T>(a.dst.m, o, ld<T>(a.src.m, p));
+    st<T>(a.dst.V, o, ld<T>(a.src.V, p));
+  }
   a.dst.id[o] = a.src.id[p];
   if (a.src.hl) st<T>(a.dst.hl, o, hl);
   if (a.src.flags) a.dst.flags[o] = a.src.flags[p];
@@ -1234,8 +1246,8 @@ __device__ __forceinline__ FusedPoint fused_load_raw(const PtsRaw &p, uint32_t i
   q.s00 = ld<float>(p.b00, i);
   q.s01 = ld<float>(p.b01, i);
   q.s11 = ld<float>(p.b11, i);
-  q.m = ld<float>(p.m, i);
-  q.V = ld<float>(p.V, i);
+  q.m = p.m ? ld<float>(p.m, i) : float(sc.m_uniform);
+  q.V = p.V ? ld<float>(p.V, i) : float(sc.V_uniform);
   q.hl = p.hl ? ld<float>(p.hl, i) : float(sc.hl_uniform);
   return q;
 }
@@ -1376,7 +1388,8 @@ __global__ __launch_bounds__((fused_threads<TW, TH, MULT>())) __attribute__((amd
     for (uint32_t k0 = 0; k0 < total; k0 += uint32_t(NT)) {
       const uint32_t k = k0 + uint32_t(t);
       uint32_t tag = kStashNone, pk = kStashNone;
-      uint32_t slot_v = 0, slot_at = 0xffffffffu;
+      HgRunSlot slot_v{0u, 0u};
+      uint32_t slot_at = 0xffffffffu;
       if (k < total) {
         fused_strain(q);
         int bx, by, cx, cy;
@@ -1415,8 +1428,10 @@ __global__ __launch_bounds__((fused_threads<TW, TH, MULT>())) __attribute__((amd
           st<float>(a.dst.b01, sp, q.s01);
           st<float>(a.dst.b11, sp, q.s11);
           st<float>(a.dst.J, sp, q.sJ);
-          st<float>(a.dst.m, sp, q.m);
-          st<float>(a.dst.V, sp, q.V);
+          if (a.src.m) {
+            st<float>(a.dst.m, sp, q.m);
+            st<float>(a.dst.V, sp, q.V);
+          }
           a.dst.id[sp] = id;
           if (a.src.hl) st<float>(a.dst.hl, sp, q.hl);
           if (a.src.flags) a.dst.flags[sp] = scripted ? 1 : 0;
@@ -1426,9 +1441,10 @@ __global__ __launch_bounds__((fused_threads<TW, TH, MULT>())) __attribute__((amd
           if (!ok || !finite) atomicOr(&a.err[sid], (ok ? 0 : 1) | (finite ? 0 : 2));
           const uint32_t key = sc.cell_off + uint32_t(by) * uint32_t(sc.snx) + uint32_t(bx);
           a.key[sp] = key;
-          // the slot's atomic returns while the raster half runs; it is
-          // stored after it
-          slot_v = hg_run_slot(a.counts, key);
+          // the slot's atomic returns while the raster half runs; the slot
+          // is taken and stored after it (waiting for the return here would
+          // also wait for every store above)
+          slot_v = hg_run_slot_issue(a.counts, key);
           slot_at = sp;
           hg_scan_block_count(a.totals, key, blockIdx.x);
         } else {
@@ -1503,7 +1519,7 @@ __global__ __launch_bounds__((fused_threads<TW, TH, MULT>())) __attribute__((amd
         });
       }
       if constexpr (!PRIME) {
-        if (slot_at != 0xffffffffu) a.slot[slot_at] = slot_v;
+        if (slot_at != 0xffffffffu) a.slot[slot_at] = hg_run_slot_take(slot_v);
       }
       lds_barrier();  // the stash is refilled by the next pass
     }
@@ -1688,6 +1704,13 @@ static void launch_p2g(hg_mpm *h, uint32_t store) {
 
 template <typename T, int BASIS>
 static void launch_g2p_t(hg_mpm *h) {
+  // the separate G2P writes each point at its sorted position (pinv), which
+  // sorts for the fused step leave unwritten
+  if (!h->pinv_ok && h->npoints) {
+    hipLaunchKernelGGL(pinv_from_perm_kernel, dim3((h->npoints + 255) / 256), dim3(256), 0,
+                       h->stream, h->perm, h->npoints, h->pinv);
+    h->pinv_ok = true;
+  }
   G2PArgs a;
   a.scenes = h->d_scenes;
   a.p_off = h->d_p_off;
@@ -1744,6 +1767,8 @@ static void launch_bin_p(hg_mpm *h) {
   }
 }
 
+static bool fused_mode(const hg_mpm *h);
+
 static void launch_tile_list(hg_mpm *h, bool scatter) {
   TileListArgs a;
   a.scenes = h->d_scenes;
@@ -1760,8 +1785,10 @@ static void launch_tile_list(hg_mpm *h, bool scatter) {
   a.key = h->key;
   a.slot = h->slot;
   a.perm = h->perm;
-  a.pinv = h->pinv;
+  // pinv only for the separate G2P; the fused step reads through perm
+  a.pinv = fused_mode(h) ? nullptr : h->pinv;
   a.npoints = scatter ? h->npoints : 0u;
+  if (scatter) h->pinv_ok = a.pinv != nullptr;
   const uint32_t blocks = (h->ntiles + 255) / 256 + (a.npoints + 255) / 256;
   ProfScope ps(h->prof, "mpm_tile_list", h->stream);
   hipLaunchKernelGGL(mpm_tile_list_kernel, dim3(blocks), dim3(256), 0, h->stream, a);
@@ -2105,7 +2132,7 @@ static hg_status upload_points(hg_mpm *h, const hg_mpm_desc *descs) {
     HG_TRY(dalloc(h, &p.y, N * 8));
     void **fs[9] = {&p.vx, &p.vy, &p.b00, &p.b01, &p.b11, &p.J, &p.m, &p.V, &p.hl};
     for (int k = 0; k < 9; ++k) {
-      if (k == 8 && !h->has_hl) {
+      if ((k == 8 && !h->has_hl) || ((k == 6 || k == 7) && !h->has_mv)) {
         *fs[k] = nullptr;
         continue;
       }
@@ -2201,8 +2228,11 @@ static hg_status mpm_create_impl(hg_ctx *ctx, const hg_mpm_desc *descs, uint32_t
     sd.g1 = d.gravity[1];
     sd.dt = 0.0;
     sd.hl_uniform = d.npoints ? d.hl[0] : 0.25 * d.cell_width;
+    sd.m_uniform = d.npoints ? d.m[0] : 0.0;
+    sd.V_uniform = d.npoints ? d.volume[0] : 0.0;
     for (uint32_t i = 0; i < d.npoints; ++i) {
       if (d.hl[i] != d.hl[0]) h->has_hl = true;
+      if (d.m[i] != d.m[0] || d.volume[i] != d.volume[0]) h->has_mv = true;
       if (d.kinematically_scripted && d.kinematically_scripted[i]) h->has_flags = true;
     }
     sd.nnx = int32_t(d.cell_count[0] + 1);

@@ -19,6 +19,9 @@ struct hg_mpm {
   uint32_t npoints = 0;
   uint32_t ncell = 0, nnode = 0, ntiles = 0;
   bool has_hl = false, has_flags = false;
+  // points store their mass and volume (else each scene's are all equal and
+  // live in its SceneDev)
+  bool has_mv = false;
   PtsRaw buf[2] = {};
   int cur = 0;
   uint32_t *perm = nullptr, *pinv = nullptr, *key = nullptr, *slot = nullptr, *counts = nullptr,
@@ -41,6 +44,7 @@ struct hg_mpm {
   // other fields) belong to the current points.
   unsigned long long *acc = nullptr;
   bool acc_ready = false, grid_ready = false;
+  bool pinv_ok = false;  // pinv holds the inverse of perm
   uint32_t grid_store = 0;
   int32_t *err = nullptr;
   bool keep_all = false;

@@ -55,6 +55,8 @@ struct SceneDev {
   double mu, kappa, dp_alpha, pic_alpha, g0, g1;
   double dt;
   double hl_uniform;
+  // every point's mass and volume where the points store none (all equal)
+  double m_uniform, V_uniform;
   double inv_h;            // 1 / h
   int32_t nnx, nny;        // reference node counts (cell_count + 1)
   int32_t snx, sny;        // padded storage pitch

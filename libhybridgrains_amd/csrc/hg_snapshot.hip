@@ -104,7 +104,11 @@ std::vector<Sec> mpm_secs(hg_mpm *h) {
   const size_t n = h->npoints, T = h->tsize();
   std::vector<Sec> s = {{p.x, n * 8},  {p.y, n * 8},   {p.vx, n * T}, {p.vy, n * T},
                         {p.b00, n * T}, {p.b01, n * T}, {p.b11, n * T}, {p.J, n * T},
-                        {p.m, n * T},  {p.V, n * T},   {p.id, n * 4}};
+                        {p.id, n * 4}};
+  if (h->has_mv) {
+    s.push_back({p.m, n * T});
+    s.push_back({p.V, n * T});
+  }
   if (h->has_hl) s.push_back({p.hl, n * T});
   if (h->has_flags) s.push_back({p.flags, n});
   return s;
