@@ -28,12 +28,17 @@
 // with events where one stream needs the other's results (FP32, with the
 // fused continuum step: grid update with the exchange, then the fused G2P and
 // the MPM sort beside the grain branch, then the next step's grain mass on
-// the continuum stream).  The pre-step
+// the continuum stream).  The fork is the completion of the continuum
+// stream's last kernel before it (launched with its completion event), not an
+// event record of its own: a record between two kernels of a stream delays
+// the second by a few microseconds.  The pre-step
 // positions the coupling uses are saved by the previous step's finish (and
 // at creation), so the DEM integration may move the grains while grain_mass
 // still reads them.  A graph capture queues the same kernels on one stream.
 #include "hg_hybrid_handle.h"
 #include "hg_sort.h"
+
+#include <hip/hip_ext.h>
 
 #include <memory>
 
@@ -621,10 +626,14 @@ static void launch_hybrid_steps(hg_hybrid *h, double dt, uint32_t n, bool concur
     grain_mass_on(s);
     for (uint32_t k = 0; k < n; ++k) {
       const HybArgs a = make_args(h);
-      // the node mass is kept for the coupling weights when they are recorded
-      hg_mpm_launch_coupled_grid(h->mpm, dt, &gn, h->hfac_on);
+      // the node mass is kept for the coupling weights when they are recorded;
+      // the fork is the grid update's own completion when nothing follows it
+      // before the fork (no event record between it and the fused G2P)
+      const bool fork_at_grid = s != side && !a.hfac;
+      hg_mpm_launch_coupled_grid(h->mpm, dt, &gn, h->hfac_on, fork_at_grid ? h->fork_ev : nullptr);
       homog_factor(a);
-      order(h->fork_ev, s, side);
+      if (fork_at_grid) hipStreamWaitEvent(side, h->fork_ev, 0);
+      else order(h->fork_ev, s, side);
       grain_end(a);
       if (k + 1 < n) dem_step();  // the next step's DEM step, ahead
       hg_mpm_launch_fused_g2p(h->mpm);
@@ -651,14 +660,21 @@ static void launch_hybrid_steps(hg_hybrid *h, double dt, uint32_t n, bool concur
       ProfScope ps(prof, "hyb_grain_mass", s);
       hipLaunchKernelGGL((hyb_grain_mass<T, BASIS>), gb, tb, 0, s, a);
     }
+    // the fork is the exchange's own completion when nothing follows it
+    // before the fork (no event record between it and the G2P)
+    const bool fork_at_exchange = s != side && !a.hfac;
     {
       // the list holds at most S*S nodes per grain: a grid over the grains
       // walks it in strides
       ProfScope ps(prof, "hyb_exchange", s);
-      hipLaunchKernelGGL(hyb_exchange<T>, gb, tb, 0, s, a);
+      if (fork_at_exchange)
+        hipExtLaunchKernelGGL(hyb_exchange<T>, gb, tb, 0, s, nullptr, h->fork_ev, 0, a);
+      else
+        hipLaunchKernelGGL(hyb_exchange<T>, gb, tb, 0, s, a);
     }
     homog_factor(a);
-    order(h->fork_ev, s, side);
+    if (fork_at_exchange) hipStreamWaitEvent(side, h->fork_ev, 0);
+    else order(h->fork_ev, s, side);
     grain_end(a);
     if (k + 1 < n) dem_step();  // the next step's DEM step, ahead
     hg_mpm_launch_g2p(h->mpm);

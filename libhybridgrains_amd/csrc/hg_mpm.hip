@@ -6,6 +6,8 @@
 #include "hg_mpm_handle.h"
 #include "hg_sort.h"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -1889,7 +1891,8 @@ static void launch_fused(hg_mpm *h, bool prime) {
 
 // The grid kernel over the current active tiles: node sums -> grid update,
 // sums cleared.
-static void launch_grid(hg_mpm *h, uint32_t store, const GrainNodes *ex = nullptr) {
+static void launch_grid(hg_mpm *h, uint32_t store, const GrainNodes *ex = nullptr,
+                        hipEvent_t done = nullptr) {
   GridArgs a;
   a.scenes = h->d_scenes;
   a.tile_scene = h->d_tile_scene;
@@ -1910,7 +1913,11 @@ static void launch_grid(hg_mpm *h, uint32_t store, const GrainNodes *ex = nullpt
   if (ex) a.ex = *ex;
   else a.ex = GrainNodes{nullptr, nullptr, nullptr, nullptr};
   ProfScope ps(h->prof, "mpm_grid", h->stream);
-  hipLaunchKernelGGL(mpm_grid_kernel, dim3(p2g_grid(h)), dim3(256), 0, h->stream, a);
+  if (done)
+    hipExtLaunchKernelGGL(mpm_grid_kernel, dim3(p2g_grid(h)), dim3(256), 0, h->stream, nullptr,
+                          done, 0, a);
+  else
+    hipLaunchKernelGGL(mpm_grid_kernel, dim3(p2g_grid(h)), dim3(256), 0, h->stream, a);
   h->acc_ready = false;
   h->grid_ready = true;
   h->grid_store = store;
@@ -2000,13 +2007,14 @@ hg_status hg_mpm_prepare_fused(hg_mpm *h, double dt) {
   return HG_OK;
 }
 
-hg_status hg_mpm_launch_coupled_grid(hg_mpm *h, double dt, const GrainNodes *g, bool keep_mass) {
+hg_status hg_mpm_launch_coupled_grid(hg_mpm *h, double dt, const GrainNodes *g, bool keep_mass,
+                                     hipEvent_t done) {
   HG_TRY(upload_dt(h, dt));
   // the exchange works on the values in hand; the node mass is kept for a
   // reader after the step's grid update
   const uint32_t store = kStoreVelAcc | (keep_mass ? kStoreMass : 0u);
   if (!h->acc_ready) launch_fused(h, true);
-  launch_grid(h, store, g);
+  launch_grid(h, store, g, done);
   return HG_OK;
 }
 

@@ -80,9 +80,10 @@ bool hg_mpm_is_fused(const hg_mpm *h);
 hg_status hg_mpm_prepare_fused(hg_mpm *h, double dt);
 // the grid update of the current points from the node sums (rasterised
 // first if no step has), storing (v, a) (and the node mass with keep_mass),
-// with the grain exchange at every node that holds grain mass (g may be null)
+// with the grain exchange at every node that holds grain mass (g may be null);
+// done (if not null) is recorded by the kernel's own completion
 hg_status hg_mpm_launch_coupled_grid(hg_mpm *h, double dt, const hgmpm::GrainNodes *g,
-                                     bool keep_mass);
+                                     bool keep_mass, hipEvent_t done = nullptr);
 // the fused G2P and rasterisation alone, then the sort of the moved points
 void hg_mpm_launch_fused_g2p(hg_mpm *h);
 void hg_mpm_launch_fused_sort(hg_mpm *h);

@@ -93,14 +93,16 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(uint32_t *counts, ui
     q.z = min(run + v[0] + v[1], nitems);
     q.w = min(run + v[0] + v[1] + v[2], nitems);
     *reinterpret_cast<uint4 *>(cell_start + base) = q;
-    *reinterpret_cast<uint4 *>(counts + base) = make_uint4(0u, 0u, 0u, 0u);
+    // counts that are already zero (most cells are empty) are not rewritten
+    if ((v[0] | v[1] | v[2] | v[3]) != 0u)
+      *reinterpret_cast<uint4 *>(counts + base) = make_uint4(0u, 0u, 0u, 0u);
     if (base + uint32_t(kScanItems) == n) cell_start[n] = min(run + s, nitems);
   } else {
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
       if (base + k < n) {
         cell_start[base + k] = min(run, nitems);
-        counts[base + k] = 0u;
+        if (v[k]) counts[base + k] = 0u;
         if (base + k == n - 1) cell_start[n] = min(run + v[k], nitems);
       }
       run += v[k];
