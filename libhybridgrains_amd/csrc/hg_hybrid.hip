@@ -71,6 +71,10 @@ struct HybArgs {
   // (null when grain_mass reads the grid update's mass)
   const unsigned long long *acc_mass;
   double mass_quantum;
+  // the grid update clears the grain mass and momentum where it exchanges
+  // (FP32, no coupling weights): the grain mass neither stamps nor lists
+  // the nodes it touches, and the finish has none to clear
+  int32_t listless;
 };
 
 template <typename T>
@@ -229,11 +233,12 @@ __global__ __launch_bounds__(kThreads) void hyb_grain_mass(HybArgs a) {
           atomicAdd(&a.g.dmass[n], w * mb);
           atomicAdd(&a.g.dmom[2 * n], w * pm0);
           atomicAdd(&a.g.dmom[2 * n + 1], w * pm1);
-          if (atomicExch(&a.g.stamp[n], 1u) == 0u) fresh |= 1u << q;
+          if (!a.listless && atomicExch(&a.g.stamp[n], 1u) == 0u) fresh |= 1u << q;
         }
       }
     }
   }
+  if (a.listless) return;
   // the first grain to stamp a node appends it to the list: one counter
   // update per wave, consecutive entries per lane
   const uint32_t mine = uint32_t(__popc(fresh));
@@ -522,6 +527,7 @@ static HybArgs make_args(hg_hybrid *h) {
   a.next_count = h->g.count + (1 - h->par);
   a.err = h->err;
   a.hfac = h->hfac_on ? h->hfac : nullptr;
+  a.listless = hg_mpm_is_fused(h->mpm) && !h->hfac_on ? 1 : 0;
   a.acc_mass = nullptr;
   a.mass_quantum = 0.0;
   return a;
@@ -612,7 +618,10 @@ static void launch_hybrid_steps(hg_hybrid *h, double dt, uint32_t n, bool concur
     // when the grain branch has long finished (a wait on an event recorded
     // just before costs about as much as the grain mass itself).
     hg_mpm_prepare_fused(h->mpm, dt);
-    const hgmpm::GrainNodes gn = {h->g.dmass, h->g.dmom, h->g.vafter, h->g.aafter};
+    const bool listless = !h->hfac_on;
+    const hgmpm::GrainNodes gn = {h->g.dmass, h->g.dmom, h->g.vafter, h->g.aafter,
+                                  listless ? h->g.dmass : nullptr,
+                                  listless ? h->g.dmom : nullptr};
     auto grain_mass_on = [&](hipStream_t st) {
       HybArgs a = make_args(h);
       a.acc_mass = hg_mpm_mass_sums(h->mpm);

@@ -1611,7 +1611,14 @@ __global__ __launch_bounds__(256) void mpm_grid_kernel(GridArgs a) {
         // the grain exchange where grain mass lies, on the values in hand
         const uint32_t n = node - sc.node_off;
         const double m_dem = a.ex.dmass[n];
-        if (m_dem > 0.0) exchange_in_place(n, a.ex, m_dem, m, pn0, pn1);
+        if (m_dem > 0.0) {
+          exchange_in_place(n, a.ex, m_dem, m, pn0, pn1);
+          if (a.ex.clear_mass) {
+            a.ex.clear_mass[n] = 0.0;
+            a.ex.clear_mom[2 * n] = 0.0;
+            a.ex.clear_mom[2 * n + 1] = 0.0;
+          }
+        }
       }
       grid_node_store<float>(sc, node, m, p0, p1, pn0, pn1, f0, f1, a.store, a.gva, a.gm, a.gp,
                              a.gpn, a.gf);

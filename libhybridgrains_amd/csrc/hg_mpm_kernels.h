@@ -340,6 +340,9 @@ __device__ __forceinline__ void basis_factor(double x, T hl, double gmin, double
 struct GrainNodes {
   const double *dmass, *dmom;
   double *vafter, *aafter;
+  // where not null, the grid update clears the grain mass and momentum of a
+  // node once it has exchanged there (the same arrays as dmass and dmom)
+  double *clear_mass = nullptr, *clear_mom = nullptr;
 };
 
 // The exchange on values in hand: m_dem is the node's grain mass, m_mpm its
