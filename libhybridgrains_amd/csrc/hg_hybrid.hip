@@ -12,7 +12,10 @@
 // cleared again at the end of the step, so nothing is swept over the whole
 // grid and the step replays correctly from a hipGraph.  The list length
 // alternates between two counters: each step's last kernel clears the one
-// the next step uses.
+// the next step uses.  In FP32 without the coupling weights (which read the
+// grain mass after the grid update) there is no list: the grid update, which
+// visits every node that can hold grain mass, clears a node's grain mass and
+// momentum once it has exchanged there.
 //
 // Kernels of one step:
 //   continuum stream       | grain stream

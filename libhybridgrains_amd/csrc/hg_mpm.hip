@@ -2498,6 +2498,13 @@ extern "C" hg_status hg_mpm_set_keep_grid(hg_mpm *h, int32_t keep_all) {
   if (!h) return HG_ERR_INVALID;
   if ((keep_all != 0) != h->keep_all) fused_drain(h);
   h->keep_all = keep_all != 0;
+  // the separate G2P reads pinv, which sorts for the fused step leave
+  // unwritten: rebuilt here rather than inside a graph the next step captures
+  if (h->keep_all && !h->pinv_ok && h->npoints) {
+    hipLaunchKernelGGL(pinv_from_perm_kernel, dim3((h->npoints + 255) / 256), dim3(256), 0,
+                       h->stream, h->perm, h->npoints, h->pinv);
+    h->pinv_ok = true;
+  }
   if (h->graph) {
     hipGraphExecDestroy(h->graph);
     h->graph = nullptr;
