@@ -30,8 +30,8 @@
 //                          | DEM step
 // with events where one stream needs the other's results (FP32, with the
 // fused continuum step: grid update with the exchange, then the fused G2P and
-// the MPM sort beside the grain branch, then the next step's grain mass on
-// the continuum stream).  The fork is the completion of the continuum
+// the MPM sort beside the grain branch, whose next step's grain mass waits
+// for the fused G2P's completion).  The fork is the completion of the continuum
 // stream's last kernel before it (launched with its completion event), not an
 // event record of its own: a record between two kernels of a stream delays
 // the second by a few microseconds.  The pre-step
@@ -613,13 +613,13 @@ static void launch_hybrid_steps(hg_hybrid *h, double dt, uint32_t n, bool concur
   };
   if (hg_mpm_is_fused(h->mpm)) {
     // FP32 continuum.  The grid update applies the exchange at the nodes
-    // holding grain mass.  The grain branch (end of the coupling, the DEM
-    // sort, the next step's DEM step) runs beside the fused G2P; the next
-    // step's grain mass, which reads the continuum mass from the node sums
-    // the fused G2P completes, follows the continuum's sort on the continuum
-    // stream, so that one wait per step joins the branches and it comes
-    // when the grain branch has long finished (a wait on an event recorded
-    // just before costs about as much as the grain mass itself).
+    // holding grain mass.  The grain branch (end of the coupling, the next
+    // step's DEM step) runs beside the fused G2P; the next step's grain mass,
+    // which reads the continuum mass from the node sums the fused G2P
+    // completes, follows on the grain stream once the fused G2P's completion
+    // event has fired, beside the continuum's sort; the continuum stream
+    // waits for it before the next grid update.  (On one stream, as in a
+    // graph capture, the grain mass follows the sort.)
     hg_mpm_prepare_fused(h->mpm, dt);
     const bool listless = !h->hfac_on;
     const hgmpm::GrainNodes gn = {h->g.dmass, h->g.dmom, h->g.vafter, h->g.aafter,
@@ -648,11 +648,18 @@ static void launch_hybrid_steps(hg_hybrid *h, double dt, uint32_t n, bool concur
       else order(h->fork_ev, s, side);
       grain_end(a);
       if (k + 1 < n) dem_step();  // the next step's DEM step, ahead
-      hg_mpm_launch_fused_g2p(h->mpm);
+      // the next step's grain mass on the grain stream, beside the MPM sort:
+      // it waits for the fused G2P's completion (its node sums)
+      const bool gm_side = s != side && k + 1 < n;
+      hg_mpm_launch_fused_g2p(h->mpm, gm_side ? h->g2p_ev : nullptr);
       h->par = 1 - h->par;
       hg_mpm_launch_fused_sort(h->mpm);
+      if (gm_side) {
+        hipStreamWaitEvent(side, h->g2p_ev, 0);
+        grain_mass_on(side);
+      }
       order(h->join_ev, side, s);
-      if (k + 1 < n) grain_mass_on(s);
+      if (k + 1 < n && !gm_side) grain_mass_on(s);
     }
     return;
   }
@@ -783,7 +790,8 @@ extern "C" hg_status hg_hybrid_create(hg_ctx *ctx, const hg_mpm_desc *mpm_desc,
   }
   if (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&h->fork_ev, kOrderEventFlags) != hipSuccess ||
-      hipEventCreateWithFlags(&h->join_ev, kOrderEventFlags) != hipSuccess)
+      hipEventCreateWithFlags(&h->join_ev, kOrderEventFlags) != hipSuccess ||
+      hipEventCreateWithFlags(&h->g2p_ev, kOrderEventFlags) != hipSuccess)
     return fail(hg_fail(ctx, HG_ERR_HIP, "hybrid streams"));
   // the memsets and copies above ran on the null stream (other handles' work
   // is not waited for)
@@ -801,6 +809,7 @@ extern "C" hg_status hg_hybrid_destroy(hg_hybrid *h) {
   }
   if (h->fork_ev) hipEventDestroy(h->fork_ev);
   if (h->join_ev) hipEventDestroy(h->join_ev);
+  if (h->g2p_ev) hipEventDestroy(h->g2p_ev);
   if (h->graph) hipGraphExecDestroy(h->graph);
   for (void *p : h->allocs) hipFree(p);
   if (h->dem) hg_dem_destroy(h->dem);

@@ -1869,10 +1869,16 @@ static void launch_fused_t(hg_mpm *h, bool prime) {
   if (h->ppass == 2) {
     const dim3 b(fused_threads<TW, TH, 2>());
     if (prime) hipLaunchKernelGGL((mpm_fused_kernel<BASIS, TW, TH, true, 2>), g, b, 0, h->stream, a);
+    else if (h->fused_done)
+      hipExtLaunchKernelGGL((mpm_fused_kernel<BASIS, TW, TH, false, 2>), g, b, 0, h->stream,
+                            nullptr, h->fused_done, 0, a);
     else hipLaunchKernelGGL((mpm_fused_kernel<BASIS, TW, TH, false, 2>), g, b, 0, h->stream, a);
   } else {
     const dim3 b(fused_threads<TW, TH, 1>());
     if (prime) hipLaunchKernelGGL((mpm_fused_kernel<BASIS, TW, TH, true, 1>), g, b, 0, h->stream, a);
+    else if (h->fused_done)
+      hipExtLaunchKernelGGL((mpm_fused_kernel<BASIS, TW, TH, false, 1>), g, b, 0, h->stream,
+                            nullptr, h->fused_done, 0, a);
     else hipLaunchKernelGGL((mpm_fused_kernel<BASIS, TW, TH, false, 1>), g, b, 0, h->stream, a);
   }
 }
@@ -2025,8 +2031,10 @@ hg_status hg_mpm_launch_coupled_grid(hg_mpm *h, double dt, const GrainNodes *g, 
   return HG_OK;
 }
 
-void hg_mpm_launch_fused_g2p(hg_mpm *h) {
+void hg_mpm_launch_fused_g2p(hg_mpm *h, hipEvent_t done) {
+  h->fused_done = done;
   launch_fused(h, false);
+  h->fused_done = nullptr;
   h->cur = 1 - h->cur;
   h->acc_ready = true;
   h->grid_ready = false;

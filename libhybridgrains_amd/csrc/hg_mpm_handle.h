@@ -45,6 +45,7 @@ struct hg_mpm {
   unsigned long long *acc = nullptr;
   bool acc_ready = false, grid_ready = false;
   bool pinv_ok = false;  // pinv holds the inverse of perm
+  hipEvent_t fused_done = nullptr;  // recorded by the next fused G2P's completion
   uint32_t grid_store = 0;
   int32_t *err = nullptr;
   bool keep_all = false;
@@ -85,7 +86,7 @@ hg_status hg_mpm_prepare_fused(hg_mpm *h, double dt);
 hg_status hg_mpm_launch_coupled_grid(hg_mpm *h, double dt, const hgmpm::GrainNodes *g,
                                      bool keep_mass, hipEvent_t done = nullptr);
 // the fused G2P and rasterisation alone, then the sort of the moved points
-void hg_mpm_launch_fused_g2p(hg_mpm *h);
+void hg_mpm_launch_fused_g2p(hg_mpm *h, hipEvent_t done = nullptr);
 void hg_mpm_launch_fused_sort(hg_mpm *h);
 // the node mass sums (whole quanta) and the quantum of scene 0
 const unsigned long long *hg_mpm_mass_sums(const hg_mpm *h);
