@@ -625,12 +625,13 @@ static void launch_hybrid_steps(hg_hybrid *h, double dt, uint32_t n, bool concur
     const hgmpm::GrainNodes gn = {h->g.dmass, h->g.dmom, h->g.vafter, h->g.aafter,
                                   listless ? h->g.dmass : nullptr,
                                   listless ? h->g.dmom : nullptr};
-    auto grain_mass_on = [&](hipStream_t st) {
+    auto grain_mass_on = [&](hipStream_t st, hipEvent_t done = nullptr) {
       HybArgs a = make_args(h);
       a.acc_mass = hg_mpm_mass_sums(h->mpm);
       a.mass_quantum = hg_mpm_mass_quantum(h->mpm);
       ProfScope ps(prof, "hyb_grain_mass", st);
-      hipLaunchKernelGGL((hyb_grain_mass<T, BASIS>), gb, tb, 0, st, a);
+      if (done) hipExtLaunchKernelGGL((hyb_grain_mass<T, BASIS>), gb, tb, 0, st, nullptr, done, 0, a);
+      else hipLaunchKernelGGL((hyb_grain_mass<T, BASIS>), gb, tb, 0, st, a);
     };
     order(h->fork_ev, s, side);
     dem_step();
@@ -655,10 +656,13 @@ static void launch_hybrid_steps(hg_hybrid *h, double dt, uint32_t n, bool concur
       h->par = 1 - h->par;
       hg_mpm_launch_fused_sort(h->mpm);
       if (gm_side) {
+        // the join is the grain mass's own completion
         hipStreamWaitEvent(side, h->g2p_ev, 0);
-        grain_mass_on(side);
+        grain_mass_on(side, h->join_ev);
+        hipStreamWaitEvent(s, h->join_ev, 0);
+      } else {
+        order(h->join_ev, side, s);
       }
-      order(h->join_ev, side, s);
       if (k + 1 < n && !gm_side) grain_mass_on(s);
     }
     return;

@@ -245,18 +245,19 @@ struct P2GRaw {
 template <typename T>
 __device__ __forceinline__ P2GRaw<T> p2g_load(const PtsRaw &pts, uint32_t p, const SceneDev &sc) {
   P2GRaw<T> r;
+  r.id = pts.id[p];
   r.x = pts.x[p];
   r.y = pts.y[p];
-  r.m = pts.m ? ld<T>(pts.m, p) : T(sc.m_uniform);
+  // mass, volume and half width are indexed by point id (never moved)
+  r.m = pts.m ? ld<T>(pts.m, r.id) : T(sc.m_uniform);
   r.v0 = ld<T>(pts.vx, p);
   r.v1 = ld<T>(pts.vy, p);
-  r.V = pts.V ? ld<T>(pts.V, p) : T(sc.V_uniform);
+  r.V = pts.V ? ld<T>(pts.V, r.id) : T(sc.V_uniform);
   r.J = ld<T>(pts.J, p);
   r.b00 = ld<T>(pts.b00, p);
   r.b01 = ld<T>(pts.b01, p);
   r.b11 = ld<T>(pts.b11, p);
-  r.hl = pts.hl ? ld<T>(pts.hl, p) : T(sc.hl_uniform);
-  r.id = pts.id[p];
+  r.hl = pts.hl ? ld<T>(pts.hl, r.id) : T(sc.hl_uniform);
   return r;
 }
 
@@ -959,7 +960,8 @@ __global__ __launch_bounds__(kG2PThreads) void mpm_g2p_kernel(G2PArgs a) {
   double c00 = strain_load<T>(ld<T>(a.src.b00, p));
   double c01 = double(ld<T>(a.src.b01, p));
   double c11 = strain_load<T>(ld<T>(a.src.b11, p));
-  const T hl = a.src.hl ? ld<T>(a.src.hl, p) : T(sc.hl_uniform);
+  const uint32_t id = a.src.id[p];
+  const T hl = a.src.hl ? ld<T>(a.src.hl, id) : T(sc.hl_uniform);
   const bool scripted = a.src.flags ? (a.src.flags[p] != 0) : false;
 
   int bx, by, cx, cy;
@@ -980,12 +982,7 @@ __global__ __launch_bounds__(kG2PThreads) void mpm_g2p_kernel(G2PArgs a) {
   st<T>(a.dst.b01, o, T(c01));
   st<T>(a.dst.b11, o, strain_store<T>(c11));
   st<T>(a.dst.J, o, strain_store<T>(J1));
-  if (a.src.m) {
-    st<T>(a.dst.m, o, ld<T>(a.src.m, p));
-    st<T>(a.dst.V, o, ld<T>(a.src.V, p));
-  }
-  a.dst.id[o] = a.src.id[p];
-  if (a.src.hl) st<T>(a.dst.hl, o, hl);
+  a.dst.id[o] = id;  // (mass, volume and half width stay where the id puts them)
   if (a.src.flags) a.dst.flags[o] = a.src.flags[p];
 
   int nbx, nby, ncx, ncy;
@@ -1015,7 +1012,7 @@ __global__ __launch_bounds__(256) void mpm_bin_kernel(BinArgs a) {
   if (i >= a.npoints) return;
   const uint32_t sid = scene_of_point(a.p_off, a.nscenes, i);
   const SceneDev &sc = a.scenes[sid];
-  const T hl = a.pts.hl ? ld<T>(a.pts.hl, i) : T(sc.hl_uniform);
+  const T hl = a.pts.hl ? ld<T>(a.pts.hl, a.pts.id[i]) : T(sc.hl_uniform);
   int bx, by, cx, cy;
   const bool ok = point_key<BASIS>(sc, a.pts.x[i], a.pts.y[i], double(hl), bx, by, cx, cy);
   if (!ok) atomicOr(&a.err[sid], 1);
@@ -1237,7 +1234,7 @@ __device__ __forceinline__ void fused_strain(FusedPoint &q) {
 
 // The stored fields only (the strain as stored), so that a load can be issued
 // a pass ahead of its use without holding the FP64 strain in registers.
-__device__ __forceinline__ FusedPoint fused_load_raw(const PtsRaw &p, uint32_t i,
+__device__ __forceinline__ FusedPoint fused_load_raw(const PtsRaw &p, uint32_t i, uint32_t id,
                                                      const SceneDev &sc) {
   FusedPoint q;
   q.x = p.x[i];
@@ -1248,9 +1245,10 @@ __device__ __forceinline__ FusedPoint fused_load_raw(const PtsRaw &p, uint32_t i
   q.s00 = ld<float>(p.b00, i);
   q.s01 = ld<float>(p.b01, i);
   q.s11 = ld<float>(p.b11, i);
-  q.m = p.m ? ld<float>(p.m, i) : float(sc.m_uniform);
-  q.V = p.V ? ld<float>(p.V, i) : float(sc.V_uniform);
-  q.hl = p.hl ? ld<float>(p.hl, i) : float(sc.hl_uniform);
+  // mass, volume and half width are indexed by point id (never moved)
+  q.m = p.m ? ld<float>(p.m, id) : float(sc.m_uniform);
+  q.V = p.V ? ld<float>(p.V, id) : float(sc.V_uniform);
+  q.hl = p.hl ? ld<float>(p.hl, id) : float(sc.hl_uniform);
   return q;
 }
 
@@ -1381,11 +1379,9 @@ __global__ __launch_bounds__((fused_threads<TW, TH, MULT>())) __attribute__((amd
       p_n = a.perm[sp_n];
     }
     if (uint32_t(t) < total) {
-      q = fused_load_raw(a.src, p, sc);
-      if constexpr (!PRIME) {
-        id = a.src.id[p];
-        scripted = a.src.flags ? (a.src.flags[p] != 0) : false;
-      }
+      id = a.src.id[p];
+      q = fused_load_raw(a.src, p, id, sc);
+      if constexpr (!PRIME) scripted = a.src.flags ? (a.src.flags[p] != 0) : false;
     }
     for (uint32_t k0 = 0; k0 < total; k0 += uint32_t(NT)) {
       const uint32_t k = k0 + uint32_t(t);
@@ -1430,12 +1426,7 @@ __global__ __launch_bounds__((fused_threads<TW, TH, MULT>())) __attribute__((amd
           st<float>(a.dst.b01, sp, q.s01);
           st<float>(a.dst.b11, sp, q.s11);
           st<float>(a.dst.J, sp, q.sJ);
-          if (a.src.m) {
-            st<float>(a.dst.m, sp, q.m);
-            st<float>(a.dst.V, sp, q.V);
-          }
           a.dst.id[sp] = id;
-          if (a.src.hl) st<float>(a.dst.hl, sp, q.hl);
           if (a.src.flags) a.dst.flags[sp] = scripted ? 1 : 0;
           const bool ok = point_key<BASIS>(sc, q.x, q.y, double(q.hl), bx, by, cx, cy);
           const bool finite = isfinite(q.x) && isfinite(q.y) && isfinite(double(q.v0)) &&
@@ -1476,11 +1467,9 @@ __global__ __launch_bounds__((fused_threads<TW, TH, MULT>())) __attribute__((amd
         sp = sp_n;
         p = p_n;
         if (k1 < total) {
-          q = fused_load_raw(a.src, p, sc);
-          if constexpr (!PRIME) {
-            id = a.src.id[p];
-            scripted = a.src.flags ? (a.src.flags[p] != 0) : false;
-          }
+          id = a.src.id[p];
+          q = fused_load_raw(a.src, p, id, sc);
+          if constexpr (!PRIME) scripted = a.src.flags ? (a.src.flags[p] != 0) : false;
         }
         if (k2 < total) {
           sp_n = order_pos(k2);
@@ -1541,7 +1530,8 @@ __global__ __launch_bounds__((fused_threads<TW, TH, MULT>())) __attribute__((amd
     const uint32_t nfar = min(s_nfar, uint32_t(kFarCap));
     for (uint32_t f = uint32_t(t); f < nfar; f += uint32_t(NT)) {
       const uint32_t i = s_far[f];
-      const FusedPoint q = fused_load_raw(PRIME ? a.src : a.dst, i, sc);
+      const PtsRaw &fp = PRIME ? a.src : a.dst;
+      const FusedPoint q = fused_load_raw(fp, i, fp.id[i], sc);
       int bx, by, cx, cy;
       point_key<BASIS>(sc, q.x, q.y, double(q.hl), bx, by, cx, cy);
       P2GTerms<BASIS> o;
@@ -2157,6 +2147,13 @@ static hg_status upload_points(hg_mpm *h, const hg_mpm_desc *descs) {
     for (int k = 0; k < 9; ++k) {
       if ((k == 8 && !h->has_hl) || ((k == 6 || k == 7) && !h->has_mv)) {
         *fs[k] = nullptr;
+        continue;
+      }
+      // mass, volume and half width are indexed by point id: one copy that
+      // both buffers share
+      if (k >= 6 && b == 1) {
+        void *const shared[3] = {h->buf[0].m, h->buf[0].V, h->buf[0].hl};
+        *fs[k] = shared[k - 6];
         continue;
       }
       HG_TRY(dalloc(h, fs[k], N * sizeof(T)));

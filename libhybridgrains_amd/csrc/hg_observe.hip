@@ -50,7 +50,8 @@ __host__ __device__ inline double obs_value(uint64_t c) {
 struct ObsArgs {
   const double *x, *y;
   const void *vx, *vy, *m;
-  double m_uniform;  // every item's mass where m is null
+  const uint32_t *id;  // where not null, m is indexed by id[i]
+  double m_uniform;    // every item's mass where m is null
   uint32_t n;
   double x0, inv_w;
   uint32_t nbins;
@@ -72,7 +73,8 @@ __global__ __launch_bounds__(kObsThreads) void obs_points_kernel(ObsArgs a) {
     const double x = a.x[i], y = a.y[i];
     const double vx = double(static_cast<const T *>(a.vx)[i]);
     const double vy = double(static_cast<const T *>(a.vy)[i]);
-    const double m = a.m ? double(static_cast<const T *>(a.m)[i]) : a.m_uniform;
+    const double m =
+        a.m ? double(static_cast<const T *>(a.m)[a.id ? a.id[i] : i]) : a.m_uniform;
     ke += 0.5 * m * (vx * vx + vy * vy);
     mx = fmax(mx, x);
     my = fmax(my, y);
@@ -189,7 +191,8 @@ ObsArgs mpm_args(hg_mpm *h, uint32_t scene, double x0, double bin_width, uint32_
   a.y = p.y + sc.p_off;
   a.vx = static_cast<const char *>(p.vx) + T * sc.p_off;
   a.vy = static_cast<const char *>(p.vy) + T * sc.p_off;
-  a.m = p.m ? static_cast<const char *>(p.m) + T * sc.p_off : nullptr;
+  a.m = p.m;  // indexed by point id
+  a.id = p.id + sc.p_off;
   a.m_uniform = double(h->precision == HG_FP64 ? sc.m_uniform : double(float(sc.m_uniform)));
   a.n = sc.p_cnt;
   a.x0 = x0;
@@ -255,6 +258,7 @@ extern "C" hg_status hg_dem_observables(hg_dem *h, uint32_t scene, double x0, do
   a.vx = static_cast<const char *>(h->b.vx) + T * sc.b_off;
   a.vy = static_cast<const char *>(h->b.vy) + T * sc.b_off;
   a.m = static_cast<const char *>(h->b.m) + T * sc.b_off;
+  a.id = nullptr;
   a.m_uniform = 0.0;
   a.n = sc.b_cnt;
   a.x0 = x0;
