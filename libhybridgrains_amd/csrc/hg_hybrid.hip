@@ -318,9 +318,7 @@ __global__ __launch_bounds__(kThreads) void hyb_homog_factor(HybArgs a) {
 
 // HybridCoupling2D::transferToGrain (Coupling.cpp:468-527), FLIP with alpha 1.
 template <typename T, int BASIS>
-__global__ __launch_bounds__(kThreads) void hyb_transfer(HybArgs a) {
-  const uint32_t b = blockIdx.x * kThreads + threadIdx.x;
-  if (b >= a.nbodies || !a.coupled[b]) return;
+__device__ __forceinline__ void transfer_grain(const HybArgs &a, uint32_t b) {
   constexpr int S = GrainStencil<BASIS>::S;
   const hgmpm::SceneDev &sc = *a.msc;
   GrainStencil<BASIS> g;
@@ -341,28 +339,49 @@ __global__ __launch_bounds__(kThreads) void hyb_transfer(HybArgs a) {
   static_cast<T *>(a.b.vy)[b] = T(vp1 * (1.0 - 1.0) + vf1 * 1.0);
 }
 
+template <typename T, int BASIS>
+__global__ __launch_bounds__(kThreads) void hyb_transfer(HybArgs a) {
+  const uint32_t b = blockIdx.x * kThreads + threadIdx.x;
+  if (b >= a.nbodies || !a.coupled[b]) return;
+  transfer_grain<T, BASIS>(a, b);
+}
+
 // End of the step: HybridCoupling2D::updatePositions_node_node
 // (Coupling.cpp:581-591) for the coupled grains, every grain filed in the
 // bucket table of the next DEM step at its final position, its position for
 // the next step's coupling, and the clean-up of the touched nodes and of the
 // next step's list counter.
 template <typename T>
+__device__ __forceinline__ void finish_grain(const HybArgs &a, uint32_t b, double dt) {
+  double x = a.b.x[b], y = a.b.y[b];
+  if (a.coupled[b]) {
+    x = a.prex[b] + dt * double(ld<T>(a.b.vx, b));
+    y = a.prey[b] + dt * double(ld<T>(a.b.vy, b));
+    a.b.x[b] = x;
+    a.b.y[b] = y;
+  }
+  a.prex[b] = x;
+  a.prey[b] = y;
+  hgdem::table_insert(a.dtab, a.kslots, hgdem::dem_bucket(*a.dsc, x, y), b);
+}
+
+// Transfer and finish of each grain in one launch, where no node list is kept
+// (listless): nothing then clears node values a transfer of another grain
+// still reads.
+template <typename T, int BASIS>
+__global__ __launch_bounds__(kThreads) void hyb_transfer_finish(HybArgs a, double dt) {
+  const uint32_t b = blockIdx.x * kThreads + threadIdx.x;
+  if (b == 0) *a.next_count = 0u;
+  if (b >= a.nbodies) return;
+  if (a.coupled[b]) transfer_grain<T, BASIS>(a, b);
+  finish_grain<T>(a, b, dt);
+}
+
+template <typename T>
 __global__ __launch_bounds__(kThreads) void hyb_finish(HybArgs a, double dt) {
   const uint32_t tid = blockIdx.x * kThreads + threadIdx.x;
   if (tid == 0) *a.next_count = 0u;
-  if (tid < a.nbodies) {
-    const uint32_t b = tid;
-    double x = a.b.x[b], y = a.b.y[b];
-    if (a.coupled[b]) {
-      x = a.prex[b] + dt * double(ld<T>(a.b.vx, b));
-      y = a.prey[b] + dt * double(ld<T>(a.b.vy, b));
-      a.b.x[b] = x;
-      a.b.y[b] = y;
-    }
-    a.prex[b] = x;
-    a.prey[b] = y;
-    hgdem::table_insert(a.dtab, a.kslots, hgdem::dem_bucket(*a.dsc, x, y), b);
-  }
+  if (tid < a.nbodies) finish_grain<T>(a, tid, dt);
   const uint32_t cnt = min(*a.count, a.g.max_list);
   for (uint32_t k = tid; k < cnt; k += gridDim.x * kThreads) {
     const uint32_t n = a.g.list[k];
@@ -601,13 +620,18 @@ static void launch_hybrid_steps(hg_hybrid *h, double dt, uint32_t n, bool concur
     hipLaunchKernelGGL(hyb_homog_factor, dim3(per_row, uint32_t(msc.nny)), tb, 0, s, a);
   };
   auto grain_end = [&](const HybArgs &a) {
-    {
-      ProfScope ps(prof, "hyb_transfer", side);
-      hipLaunchKernelGGL((hyb_transfer<T, BASIS>), gb, tb, 0, side, a);
-    }
-    {
-      ProfScope ps(prof, "hyb_finish", side);
-      hipLaunchKernelGGL(hyb_finish<T>, gb, tb, 0, side, a, dt);
+    if (a.listless) {
+      ProfScope ps(prof, "hyb_transfer_finish", side);
+      hipLaunchKernelGGL((hyb_transfer_finish<T, BASIS>), gb, tb, 0, side, a, dt);
+    } else {
+      {
+        ProfScope ps(prof, "hyb_transfer", side);
+        hipLaunchKernelGGL((hyb_transfer<T, BASIS>), gb, tb, 0, side, a);
+      }
+      {
+        ProfScope ps(prof, "hyb_finish", side);
+        hipLaunchKernelGGL(hyb_finish<T>, gb, tb, 0, side, a, dt);
+      }
     }
     homog(a);
   };
