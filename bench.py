@@ -79,7 +79,8 @@ def log(msg: str) -> None:
 
 def _time_hybrid(m, d, w, precision, steps, warmup, dt):
     """Eager steps (the grain branch on its own stream, overlapping the
-    continuum) and hipGraph steps (one stream); returns both walls and the
+    continuum) and hipGraph steps (six steps captured on both streams);
+    returns both walls and the
     per-kernel event times."""
     from libhybridgrains_amd import GpuHybrid
 

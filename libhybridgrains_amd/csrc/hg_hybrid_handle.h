@@ -44,6 +44,7 @@ struct hg_hybrid {
   hipStream_t side = nullptr;
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   hipEvent_t g2p_ev = nullptr;  // the fused G2P's completion, for the next grain mass
+  bool capturing = false;       // steps are being captured into a graph
   hghyb::GrainGrid g = {};
   double *prex = nullptr, *prey = nullptr;
   uint8_t *coupled = nullptr;
