@@ -172,9 +172,10 @@ __global__ __launch_bounds__(kTeamThreads) void dem_step_kernel(ContactArgs a) {
   __shared__ uint32_t s_in[kTeamBodies][CAP];
   __shared__ int s_nown[kTeamBodies], s_nin[kTeamBodies];
   // the terms the integration sums: own and incoming pairs by slot, planes
-  // by index (with their activity)
-  __shared__ double s_of[kTeamBodies][CAP][3], s_if[kTeamBodies][CAP][3];
-  __shared__ double s_pf[kTeamBodies][kMaxPlanes][3];
+  // by index (with their activity), each rounded to T (the FP32 mode's
+  // terms then take half the LDS, so that more workgroups fit on a CU)
+  __shared__ T s_of[kTeamBodies][CAP][3], s_if[kTeamBodies][CAP][3];
+  __shared__ T s_pf[kTeamBodies][kMaxPlanes][3];
   __shared__ uint8_t s_pon[kTeamBodies][kMaxPlanes];
   const int tb = int(threadIdx.x) / kTeam, lane = int(threadIdx.x) % kTeam;
   const uint32_t blk = hg_xcd_remap(blockIdx.x, a.nblocks_padded);
@@ -334,9 +335,9 @@ __global__ __launch_bounds__(kTeamThreads) void dem_step_kernel(ContactArgs a) {
       st<T>(a.new_s.ods, 2 * sl + 1, T(ds1));
       st<T>(a.f.oF, 2 * sl, T(F0));
       st<T>(a.f.oF, 2 * sl + 1, T(F1));
-      s_of[tb][rank][0] = double(T(F0));
-      s_of[tb][rank][1] = double(T(F1));
-      s_of[tb][rank][2] = double(T(r00 * F1 - r01 * F0));
+      s_of[tb][rank][0] = T(F0);
+      s_of[tb][rank][1] = T(F1);
+      s_of[tb][rank][2] = T(r00 * F1 - r01 * F0);
     }
     // The contacts owned by a smaller partner j: the owner's computation above
     // (j as the body, this body as its partner: geometry, spring history from
@@ -393,9 +394,9 @@ __global__ __launch_bounds__(kTeamThreads) void dem_step_kernel(ContactArgs a) {
       double ds0, ds1, F0, F1;
       spring(had, o0, o1, dt, vr0, vr1, n0, n1, ds0, ds1);
       penalty_force(sc, w, depth, n0, n1, vr0, vr1, ds0, ds1, F0, F1);
-      s_if[tb][rank][0] = double(T(F0));
-      s_if[tb][rank][1] = double(T(F1));
-      s_if[tb][rank][2] = double(T(r10 * F1 - r11 * F0));
+      s_if[tb][rank][0] = T(F0);
+      s_if[tb][rank][1] = T(F1);
+      s_if[tb][rank][2] = T(r10 * F1 - r11 * F0);
     }
     if (lane == 0) a.new_s.ocnt[i] = uint8_t(nown);
 
@@ -425,9 +426,9 @@ __global__ __launch_bounds__(kTeamThreads) void dem_step_kernel(ContactArgs a) {
       st<T>(a.new_s.pds, 2 * ps, T(ds0));
       st<T>(a.new_s.pds, 2 * ps + 1, T(ds1));
       s_pon[tb][k] = 1;
-      s_pf[tb][k][0] = double(T(F0));
-      s_pf[tb][k][1] = double(T(F1));
-      s_pf[tb][k][2] = double(T(r00 * F1 - r01 * F0));
+      s_pf[tb][k][0] = T(F0);
+      s_pf[tb][k][1] = T(F1);
+      s_pf[tb][k][2] = T(r00 * F1 - r01 * F0);
     }
   }
 
@@ -445,19 +446,19 @@ __global__ __launch_bounds__(kTeamThreads) void dem_step_kernel(ContactArgs a) {
   F1 += m * sc.g1;
   for (int k = 0; k < sc.nplanes; ++k) {
     if (!s_pon[tb][k]) continue;
-    F0 += s_pf[tb][k][0];
-    F1 += s_pf[tb][k][1];
-    F2 += s_pf[tb][k][2];
+    F0 += double(s_pf[tb][k][0]);
+    F1 += double(s_pf[tb][k][1]);
+    F2 += double(s_pf[tb][k][2]);
   }
   for (int k = 0; k < nin; ++k) {
-    F0 -= s_if[tb][k][0];
-    F1 -= s_if[tb][k][1];
-    F2 -= s_if[tb][k][2];
+    F0 -= double(s_if[tb][k][0]);
+    F1 -= double(s_if[tb][k][1]);
+    F2 -= double(s_if[tb][k][2]);
   }
   for (int q = 0; q < nown; ++q) {
-    F0 += s_of[tb][q][0];
-    F1 += s_of[tb][q][1];
-    F2 += s_of[tb][q][2];
+    F0 += double(s_of[tb][q][0]);
+    F1 += double(s_of[tb][q][1]);
+    F2 += double(s_of[tb][q][2]);
   }
   if (fi) F0 = F1 = F2 = 0.0;
   if (a.F_out) {
